@@ -1,0 +1,160 @@
+"""ctypes mirror of include/fluidsolver_amd.h (structs, enums, status codes).
+
+Pure data definitions: importing this module loads no native library.
+"""
+import ctypes as C
+
+import numpy as np
+
+FSA_OK = 0
+FSA_E_INVALID = 1
+FSA_E_NOMEM = 2
+FSA_E_DEVICE = 3
+FSA_E_CAPACITY = 4
+FSA_E_STATE = 5
+FSA_E_BUDGET = 6
+FSA_E_NONFINITE = 7
+
+STATUS_NAMES = {
+    FSA_OK: "FSA_OK",
+    FSA_E_INVALID: "FSA_E_INVALID",
+    FSA_E_NOMEM: "FSA_E_NOMEM",
+    FSA_E_DEVICE: "FSA_E_DEVICE",
+    FSA_E_CAPACITY: "FSA_E_CAPACITY",
+    FSA_E_STATE: "FSA_E_STATE",
+    FSA_E_BUDGET: "FSA_E_BUDGET",
+    FSA_E_NONFINITE: "FSA_E_NONFINITE",
+}
+
+SOLVER_IISPH = 0
+SOLVER_SESPH = 1
+LIST_COMPRESSED = 0
+LIST_PLAIN = 1
+TYPE_NORMAL = 0
+TYPE_BOUNDARY = 1
+TYPE_INACTIVE = 2
+FLAG_EMULATE_INT_ABS = 1
+MAX_NEIGHBORS = 97
+
+TIMER_NAMES = ["search", "density", "diagonal", "pressure_a", "pressure_b", "integrate"]
+
+
+class SceneDesc(C.Structure):
+    """fsa_scene_desc: SimulationParameters + IISPH/SESPH settings + dt generator."""
+
+    _fields_ = [
+        ("dimension", C.c_uint32),
+        ("solver", C.c_uint32),
+        ("neighbor_lists", C.c_uint32),
+        ("flags", C.c_uint32),
+        ("rest_density", C.c_float),
+        ("gravity", C.c_float),
+        ("particle_size", C.c_float),
+        ("iisph_max_density_error_allowed", C.c_float),
+        ("iisph_min_iterations", C.c_uint32),
+        ("iisph_max_iterations", C.c_uint32),
+        ("iisph_omega", C.c_float),
+        ("iisph_gamma", C.c_float),
+        ("iisph_viscosity", C.c_float),
+        ("iisph_single_layer_boundary", C.c_uint32),
+        ("iisph_single_layer_gamma_1", C.c_float),
+        ("sesph_stiffness_k", C.c_float),
+        ("sesph_viscosity", C.c_float),
+        ("sesph_single_layer_boundary", C.c_uint32),
+        ("sesph_single_layer_gamma_1", C.c_float),
+        ("sesph_single_layer_gamma_2", C.c_float),
+        ("timestep", C.c_float),
+        ("lambda_v", C.c_float),
+        ("lambda_a", C.c_float),
+        ("reserved", C.c_uint32 * 5),
+    ]
+
+
+def default_desc(dimension=3, solver=SOLVER_IISPH, **overrides):
+    """Reference defaults (IISPHSettings3D.hpp:7-17, SESPHSettings3D.hpp:7-12,
+    ConstantTimestepGenerator.hpp:9-12) with Simulator's overrides
+    particle_size = 1, rest_density = 1 (Simulator.hpp:33-35)."""
+    d = SceneDesc()
+    d.dimension = dimension
+    d.solver = solver
+    d.neighbor_lists = LIST_COMPRESSED
+    d.flags = FLAG_EMULATE_INT_ABS if dimension == 2 else 0
+    d.rest_density = 1.0
+    d.gravity = 9.81
+    d.particle_size = 1.0
+    d.iisph_max_density_error_allowed = 0.001
+    d.iisph_min_iterations = 2
+    d.iisph_max_iterations = 100
+    d.iisph_omega = 0.5
+    d.iisph_gamma = 0.7
+    d.iisph_viscosity = 5.0
+    d.iisph_single_layer_boundary = 0
+    d.iisph_single_layer_gamma_1 = 1.1
+    d.sesph_stiffness_k = 100000.0
+    d.sesph_viscosity = 5.0
+    d.sesph_single_layer_boundary = 0
+    d.sesph_single_layer_gamma_1 = 1.1
+    d.sesph_single_layer_gamma_2 = 1.1
+    d.timestep = 0.001
+    d.lambda_v = 0.4
+    d.lambda_a = 0.25
+    for k, v in overrides.items():
+        setattr(d, k, v)
+    return d
+
+
+class ParticleView(C.Structure):
+    _fields_ = [
+        ("count", C.c_uint64),
+        ("movement", C.c_void_p),
+        ("movement_stride", C.c_uint64),
+        ("particle_data", C.c_void_p),
+        ("particle_data_stride", C.c_uint64),
+        ("info", C.c_void_p),
+        ("info_stride", C.c_uint64),
+        ("external", C.c_void_p),
+        ("external_stride", C.c_uint64),
+    ]
+
+
+class StepStats(C.Structure):
+    _fields_ = [
+        ("desired_time_step", C.c_float),
+        ("actual_time_step", C.c_float),
+        ("iterations", C.c_uint32),
+        ("status", C.c_uint32),
+        ("average_density_error", C.c_double),
+        ("max_velocity", C.c_float),
+        ("max_acceleration", C.c_float),
+    ]
+
+    def as_dict(self):
+        return {f[0]: getattr(self, f[0]) for f in self._fields_}
+
+
+# numpy records with the reference component layouts (Particle.hpp:15-50)
+INFO_DTYPE = np.dtype({"names": ["tag", "type"], "formats": ["<u4", "u1"], "offsets": [0, 4], "itemsize": 8})
+
+
+def iisph_record_floats(dimension):
+    """Floats per IISPHParticleData3D (9) / IISPHParticleData (4) record."""
+    return 9 if dimension == 3 else 4
+
+
+def make_view(coll):
+    """ParticleView over a ParticleCollection's component arrays (no copy)."""
+    v = ParticleView()
+    v.count = coll.size()
+    v.movement = coll.movement.ctypes.data
+    v.movement_stride = coll.movement.strides[0]
+    v.particle_data = coll.particle_data.ctypes.data
+    v.particle_data_stride = coll.particle_data.strides[0]
+    v.info = coll.info.ctypes.data
+    v.info_stride = coll.info.strides[0]
+    if coll.external is not None:
+        v.external = coll.external.ctypes.data
+        v.external_stride = coll.external.strides[0]
+    else:
+        v.external = None
+        v.external_stride = 0
+    return v

@@ -1,0 +1,1090 @@
+// fsa_api.hip — C ABI of the gfx950 SPH hot path (include/fluidsolver_amd.h).
+//
+// Host orchestration of one simulation step, as Simulator::execute_simulation_step
+// drives it (Simulator.cpp:7-57): find_neighbors, then the solver step.  The
+// Jacobi loop is device-gated: every pass checks the scene's `active` flag, so
+// the host enqueues iterations in chunks and reads the flag once per chunk.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <exception>
+#include <mutex>
+#include <new>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/fluidsolver_amd.h"
+#include "fsa_device.hpp"
+#include "fsa_kernels.hpp"
+
+using namespace fsa;
+
+namespace {
+
+thread_local std::string g_error;
+
+struct FsaError : std::runtime_error {
+    int32_t code;
+    FsaError(int32_t c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+#define HIP_CHECK(expr)                                                                                  \
+    do {                                                                                                 \
+        hipError_t _e = (expr);                                                                          \
+        if (_e != hipSuccess)                                                                            \
+            throw FsaError(FSA_E_DEVICE, std::string(#expr) + ": " + hipGetErrorString(_e));           \
+    } while (0)
+
+template <class F>
+int32_t guarded(F&& f) {
+    try {
+        f();
+        return FSA_OK;
+    } catch (const FsaError& e) {
+        g_error = e.what();
+        return e.code;
+    } catch (const std::bad_alloc&) {
+        g_error = "host allocation failed";
+        return FSA_E_NOMEM;
+    } catch (const std::exception& e) {
+        g_error = e.what();
+        return FSA_E_INVALID;
+    } catch (...) {
+        g_error = "unknown error";
+        return FSA_E_INVALID;
+    }
+}
+
+uint64_t align_up(uint64_t v, uint64_t a) { return (v + a - 1) / a * a; }
+
+// device-memory plan of one scene (all arrays carved from one allocation)
+struct Plan {
+    uint64_t n, nf, stride, nblocks, nfblocks, ntiles;
+    uint64_t bytes;
+    std::vector<std::pair<void**, uint64_t>> parts;
+};
+
+}  // namespace
+
+struct fsa_context_s {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    uint64_t budget = 0;
+    std::atomic<uint64_t> in_use{0};
+    std::mutex mu;
+    bool timing = false;
+    double timer_ms[FSA_TIMER_COUNT] = {};
+    uint64_t timer_n[FSA_TIMER_COUNT] = {};
+    hipEvent_t ev[2 * FSA_TIMER_COUNT] = {};
+    bool graph = false;
+    uint32_t block_override = 0;
+    std::vector<fsa_scene_s*> scenes;  // live scenes, destroyed with the context
+};
+
+struct fsa_scene_s {
+    fsa_context ctx = nullptr;
+    fsa_scene_desc desc{};
+    uint64_t total = 0;          // caller rows
+    uint32_t n = 0, nf = 0, nb = 0;
+    std::vector<uint32_t> inactive_rows;
+    std::vector<float> inactive_movement, inactive_pdata, inactive_ext;
+    std::vector<uint8_t> inactive_info;
+    SceneDev hd{};
+    SceneDev* d_table = nullptr;   // single-scene launch table
+    void* dmem = nullptr;
+    uint64_t bytes = 0;
+    uint4* cells = nullptr;
+    uint64_t cells_cap = 0;
+    SceneState* h_state = nullptr;  // pinned
+    LaunchSet single{};
+    std::vector<BlockRef*> map_allocs;
+    bool searched = false;
+    bool boundary_dirty = true;
+    uint32_t last_iterations = 2;
+    bool lists_cached = false;
+    std::vector<uint64_t> csr_off, csr_cols;
+};
+
+namespace {
+
+void fill_params(fsa_scene_s* s) {
+    const fsa_scene_desc& d = s->desc;
+    SceneDev& h = s->hd;
+    h.rest_density = d.rest_density;
+    h.gravity = d.gravity;
+    h.particle_size = d.particle_size;
+    h.search_radius = d.particle_size * 2.0f;   // Math::kernel_support_factor (LibFluidMath.hpp:11)
+    h.r2 = h.search_radius * h.search_radius;
+    h.h = h.search_radius / 2.0f;               // CubicSplineKernel3D::initialize (:67-70)
+    const float PI = 3.14159265358979323846f;
+    h.alpha = 1.0f / (4.0f * PI * (h.h * h.h * h.h));
+    h.rho0sq = d.rest_density * d.rest_density;
+    h.visc_c01 = 0.01f * d.particle_size * d.particle_size;
+    const bool sesph = d.solver == FSA_SOLVER_SESPH;
+    h.visc2 = 2.0f * (sesph ? d.sesph_viscosity : d.iisph_viscosity);
+    h.omega = d.iisph_omega;
+    h.gamma = d.iisph_gamma;
+    h.gamma1 = sesph ? d.sesph_single_layer_gamma_1 : d.iisph_single_layer_gamma_1;
+    h.gamma2 = d.sesph_single_layer_gamma_2;
+    h.max_err = d.iisph_max_density_error_allowed;
+    h.min_it = d.iisph_min_iterations;
+    h.max_it = d.iisph_max_iterations;
+    h.stiffness = d.sesph_stiffness_k;
+    h.gen_dt = d.timestep;
+    h.dt = d.timestep;
+    h.lambda_a = d.lambda_a;
+    h.lambda_v = d.lambda_v;
+    uint32_t f = h.flags & (kFlagExternal | kFlagDynamicBoundary);
+    if (h.h == 1.0f) f |= kFlagHOne;
+    if (h.rho0sq == 1.0f) f |= kFlagRho0SqOne;
+    if (sesph ? d.sesph_single_layer_boundary : d.iisph_single_layer_boundary) f |= kFlagSingleLayer;
+    if (sesph) f |= kFlagSesph;
+    if (d.neighbor_lists == FSA_LIST_PLAIN) f |= kFlagPlainLists;
+    h.flags = f;
+}
+
+void validate_desc(const fsa_scene_desc* d) {
+    if (!d) throw FsaError(FSA_E_INVALID, "desc is NULL");
+    if (d->dimension != 3)
+        throw FsaError(FSA_E_INVALID, "this build of the device path implements dimension 3");
+    if (d->solver != FSA_SOLVER_IISPH && d->solver != FSA_SOLVER_SESPH)
+        throw FsaError(FSA_E_INVALID, "unknown solver");
+    if (!(d->particle_size > 0.0f)) throw FsaError(FSA_E_INVALID, "particle_size must be > 0");
+    if (!(d->rest_density > 0.0f)) throw FsaError(FSA_E_INVALID, "rest_density must be > 0");
+    if (!(d->timestep > 0.0f)) throw FsaError(FSA_E_INVALID, "timestep must be > 0");
+    if (d->solver == FSA_SOLVER_IISPH) {
+        if (d->iisph_max_iterations < d->iisph_min_iterations)
+            throw FsaError(FSA_E_INVALID, "Max iterations are less than min number of iterations.");
+        if (!(d->iisph_max_density_error_allowed > 0.0f))
+            throw FsaError(FSA_E_INVALID, "MaxDensityErrorAllowed is smaller or equal to zero.");
+    }
+}
+
+Plan make_plan(uint64_t n, uint64_t nf) {
+    Plan p{};
+    p.n = n;
+    p.nf = nf;
+    p.stride = align_up(std::max<uint64_t>(nf, 1), 64);
+    p.nblocks = (n + kBlock - 1) / kBlock;
+    p.nfblocks = (nf + kBlock - 1) / kBlock;
+    p.ntiles = (n + kSortTile - 1) / kSortTile;
+    return p;
+}
+
+uint64_t plan_bytes(const Plan& p) {
+    const uint64_t n = std::max<uint64_t>(p.n, 1);
+    uint64_t b = 0;
+    auto add = [&](uint64_t sz) { b += align_up(sz, 256); };
+    for (int k = 0; k < 8; k++) add(16 * n);   // pos, vel, acc, ext x2
+    for (int k = 0; k < 6; k++) add(4 * n);    // pres, err, idt x2
+    add(4 * n);                                // rho_new
+    add(16 * n);                               // vstar
+    add(16 * n);                               // anp
+    add(4 * n);                                // src
+    add(4 * n);                                // diag
+    for (int k = 0; k < 4; k++) add(4 * n);    // keys, vals x2
+    add(4 * (kRadix * std::max<uint64_t>(p.ntiles, 1) + kRadix));
+    add(4 * std::max<uint64_t>(p.nf, 1));      // fluid_idx
+    add(4 * std::max<uint64_t>(p.nblocks, 1));
+    add(4ull * kMaxNeighbors * p.stride);      // nbr
+    add(4 * std::max<uint64_t>(p.nf, 1));      // nbr_cnt
+    add(sizeof(Partial) * std::max<uint64_t>(p.nblocks, 1));
+    add(sizeof(BoxPartial) * std::max<uint64_t>(p.nblocks, 1));
+    add(sizeof(SceneState));
+    return b;
+}
+
+void carve(fsa_scene_s* s, const Plan& p) {
+    char* base = (char*)s->dmem;
+    uint64_t off = 0;
+    const uint64_t n = std::max<uint64_t>(p.n, 1);
+    auto take = [&](uint64_t sz) {
+        void* r = base + off;
+        off += align_up(sz, 256);
+        return r;
+    };
+    SceneDev& h = s->hd;
+    for (int k = 0; k < 2; k++) h.pos[k] = (float4*)take(16 * n);
+    for (int k = 0; k < 2; k++) h.vel[k] = (float4*)take(16 * n);
+    for (int k = 0; k < 2; k++) h.acc[k] = (float4*)take(16 * n);
+    for (int k = 0; k < 2; k++) h.ext[k] = (float4*)take(16 * n);
+    for (int k = 0; k < 2; k++) h.pres[k] = (float*)take(4 * n);
+    for (int k = 0; k < 2; k++) h.err[k] = (float*)take(4 * n);
+    for (int k = 0; k < 2; k++) h.idt[k] = (uint32_t*)take(4 * n);
+    h.rho_new = (float*)take(4 * n);
+    h.vstar = (float4*)take(16 * n);
+    h.anp = (float4*)take(16 * n);
+    h.src = (float*)take(4 * n);
+    h.diag = (float*)take(4 * n);
+    for (int k = 0; k < 2; k++) h.keys[k] = (uint32_t*)take(4 * n);
+    for (int k = 0; k < 2; k++) h.vals[k] = (uint32_t*)take(4 * n);
+    h.hist = (uint32_t*)take(4 * (kRadix * std::max<uint64_t>(p.ntiles, 1) + kRadix));
+    h.fluid_idx = (uint32_t*)take(4 * std::max<uint64_t>(p.nf, 1));
+    h.block_count = (uint32_t*)take(4 * std::max<uint64_t>(p.nblocks, 1));
+    h.nbr = (uint32_t*)take(4ull * kMaxNeighbors * p.stride);
+    h.nbr_cnt = (uint32_t*)take(4 * std::max<uint64_t>(p.nf, 1));
+    h.partial = (Partial*)take(sizeof(Partial) * std::max<uint64_t>(p.nblocks, 1));
+    h.box = (BoxPartial*)take(sizeof(BoxPartial) * std::max<uint64_t>(p.nblocks, 1));
+    h.state = (SceneState*)take(sizeof(SceneState));
+    h.n = (uint32_t)p.n;
+    h.nf = (uint32_t)p.nf;
+    h.list_stride = (uint32_t)p.stride;
+    h.nblocks = (uint32_t)p.nblocks;
+    h.ntiles = (uint32_t)p.ntiles;
+}
+
+// ---- block maps --------------------------------------------------------------------
+struct MapBuilder {
+    std::vector<BlockRef> all, fluid, tiles, scan, scene;
+    void add(uint32_t sidx, const SceneDev& h) {
+        for (uint32_t b = 0; b < h.nblocks; b++) all.push_back({sidx, b});
+        const uint32_t nfb = (h.nf + kBlock - 1) / kBlock;
+        for (uint32_t b = 0; b < nfb; b++) fluid.push_back({sidx, b});
+        for (uint32_t b = 0; b < h.ntiles; b++) tiles.push_back({sidx, b});
+        for (uint32_t d = 0; d < (uint32_t)kRadix; d++) scan.push_back({sidx, d});
+        scene.push_back({sidx, 0});
+    }
+};
+
+BlockRef* upload_map(const std::vector<BlockRef>& v, std::vector<BlockRef*>& owned) {
+    if (v.empty()) return nullptr;
+    BlockRef* d = nullptr;
+    HIP_CHECK(hipMalloc(&d, v.size() * sizeof(BlockRef)));
+    owned.push_back(d);
+    HIP_CHECK(hipMemcpy(d, v.data(), v.size() * sizeof(BlockRef), hipMemcpyHostToDevice));
+    return d;
+}
+
+void build_launch_set(const MapBuilder& mb, SceneDev* table, LaunchSet& L, std::vector<BlockRef*>& owned) {
+    L.scenes = table;
+    L.map_all = upload_map(mb.all, owned);
+    L.map_fluid = upload_map(mb.fluid, owned);
+    L.map_tiles = upload_map(mb.tiles, owned);
+    L.map_scan = upload_map(mb.scan, owned);
+    L.map_scene = upload_map(mb.scene, owned);
+    L.nb_all = (uint32_t)mb.all.size();
+    L.nb_fluid = (uint32_t)mb.fluid.size();
+    L.nb_tiles = (uint32_t)mb.tiles.size();
+    L.nb_scan = (uint32_t)mb.scan.size();
+    L.nb_scene = (uint32_t)mb.scene.size();
+}
+
+// ---- grid ---------------------------------------------------------------------------
+inline int32_t host_cell(float x, float r) { return (int32_t)std::floor(x / r); }
+
+uint32_t levels_for(const int32_t lo[3], const int32_t hi[3]) {
+    uint32_t L = 1;
+    for (;;) {
+        const int64_t lim = 1ll << L;
+        bool ok = true;
+        for (int a = 0; a < 3; a++)
+            if (lo[a] < -lim || hi[a] > lim - 1) ok = false;
+        if (ok) return L;
+        L++;
+        if (L > 9) return L;
+    }
+}
+
+// sets the cell grid of the coming search from the state's bounding box
+void prepare_grid(fsa_scene_s* s) {
+    SceneDev& h = s->hd;
+    const SceneState& st = *s->h_state;
+    int32_t lo[3], hi[3];
+    for (int a = 0; a < 3; a++) {
+        lo[a] = st.bbox_min[a];
+        hi[a] = st.bbox_max[a];
+    }
+    if (s->n == 0) {
+        for (int a = 0; a < 3; a++) lo[a] = hi[a] = 0;
+    }
+    const uint32_t L = levels_for(lo, hi);
+    if (L > 9)
+        throw FsaError(FSA_E_INVALID,
+                       "scene extent exceeds the 32-bit cell key range (cell coordinates must lie in [-512, 511])");
+    uint64_t cells = 1;
+    for (int a = 0; a < 3; a++) {
+        h.cmin[a] = lo[a];
+        h.cdim[a] = (uint32_t)(hi[a] - lo[a] + 1);
+        cells *= h.cdim[a];
+    }
+    h.cdim[3] = (uint32_t)cells;
+    h.levels = L;
+    if (cells > s->cells_cap) {
+        if (s->cells) {
+            HIP_CHECK(hipFree(s->cells));
+            s->ctx->in_use -= s->cells_cap * sizeof(uint4);
+        }
+        const uint64_t cap = cells + cells / 4 + 64;
+        if (s->ctx->budget && s->ctx->in_use + cap * sizeof(uint4) > s->ctx->budget)
+            throw FsaError(FSA_E_BUDGET, "cell table exceeds the HBM budget");
+        HIP_CHECK(hipMalloc(&s->cells, cap * sizeof(uint4)));
+        HIP_CHECK(hipMemsetAsync(s->cells, 0, cap * sizeof(uint4), s->ctx->stream));
+        s->cells_cap = cap;
+        s->ctx->in_use += cap * sizeof(uint4);
+    }
+    h.cells = s->cells;
+    h.stamp++;
+    if (h.stamp == 0) {
+        HIP_CHECK(hipMemsetAsync(s->cells, 0, s->cells_cap * sizeof(uint4), s->ctx->stream));
+        h.stamp = 1;
+    }
+}
+
+uint32_t sort_passes(const SceneDev& h) { return (3 * h.levels + 3 + kRadixBits - 1) / kRadixBits; }
+
+void read_state(fsa_scene_s* s) {
+    HIP_CHECK(hipMemcpyAsync(s->h_state, s->hd.state, sizeof(SceneState), hipMemcpyDeviceToHost, s->ctx->stream));
+    HIP_CHECK(hipStreamSynchronize(s->ctx->stream));
+}
+
+// ---- timing ---------------------------------------------------------------------------
+struct PhaseTimer {
+    fsa_context ctx;
+    int id;
+    PhaseTimer(fsa_context c, int i) : ctx(c), id(i) {
+        if (ctx->timing) HIP_CHECK(hipEventRecord(ctx->ev[2 * id], ctx->stream));
+    }
+    ~PhaseTimer() noexcept(false) {
+        if (!ctx->timing) return;
+        HIP_CHECK(hipEventRecord(ctx->ev[2 * id + 1], ctx->stream));
+        HIP_CHECK(hipEventSynchronize(ctx->ev[2 * id + 1]));
+        float ms = 0.0f;
+        HIP_CHECK(hipEventElapsedTime(&ms, ctx->ev[2 * id], ctx->ev[2 * id + 1]));
+        ctx->timer_ms[id] += ms;
+        ctx->timer_n[id] += 1;
+    }
+};
+
+// ---- the step, for any set of scenes sharing a launch set --------------------------------
+void search_set(fsa_context ctx, const LaunchSet& L, uint32_t passes) {
+    PhaseTimer t(ctx, FSA_TIMER_SEARCH);
+    launch_keys(L, ctx->stream);
+    launch_sort(L, passes, ctx->stream);
+    launch_reorder(L, passes & 1u, ctx->stream);
+    launch_lists(L, ctx->stream);
+}
+
+void simulate_set(fsa_context ctx, const LaunchSet& L, std::vector<fsa_scene_s*>& scenes, bool sesph,
+                  bool boundary_pass) {
+    hipStream_t st = ctx->stream;
+    if (sesph) {
+        {
+            PhaseTimer t(ctx, FSA_TIMER_DENSITY);
+            launch_sesph_density(L, st);
+        }
+        {
+            PhaseTimer t(ctx, FSA_TIMER_DIAGONAL);
+            launch_sesph_accel(L, st);
+        }
+    } else {
+        {
+            PhaseTimer t(ctx, FSA_TIMER_DENSITY);
+            launch_iisph_prepare(L, boundary_pass, st);
+        }
+        {
+            PhaseTimer t(ctx, FSA_TIMER_DIAGONAL);
+            launch_iisph_diagonal(L, st);
+        }
+        launch_iisph_begin_loop(L, st);
+        uint32_t enq = 0, max_it = 0, guess = 0;
+        for (auto* s : scenes) {
+            max_it = std::max(max_it, s->hd.max_it);
+            guess = std::max(guess, std::max(s->hd.min_it, s->last_iterations));
+        }
+        uint32_t chunk = std::min(max_it, std::max<uint32_t>(guess, 1));
+        while (enq < max_it) {
+            for (uint32_t k = 0; k < chunk && enq < max_it; k++, enq++) {
+                {
+                    PhaseTimer t(ctx, FSA_TIMER_PRESSURE_A);
+                    launch_iisph_pass_a(L, st);
+                }
+                {
+                    PhaseTimer t(ctx, FSA_TIMER_PRESSURE_B);
+                    launch_iisph_pass_b(L, st);
+                }
+            }
+            if (enq >= max_it) break;
+            bool any_active = false;
+            for (auto* s : scenes) {
+                read_state(s);
+                if (s->h_state->active) any_active = true;
+            }
+            if (!any_active) break;
+            chunk = 4;
+        }
+    }
+    {
+        PhaseTimer t(ctx, FSA_TIMER_INTEGRATE);
+        launch_integrate(L, st);
+    }
+    HIP_CHECK(hipStreamSynchronize(st));
+}
+
+void fill_stats(fsa_scene_s* s, fsa_step_stats* out) {
+    const SceneState& st = *s->h_state;
+    if (!out) return;
+    out->desired_time_step = s->hd.dt;
+    out->actual_time_step = st.dt_actual;
+    out->iterations = s->desc.solver == FSA_SOLVER_SESPH ? 0u : st.iterations;
+    out->status = FSA_OK;
+    if (st.status & kStatusCapacity) out->status = FSA_E_CAPACITY;
+    out->average_density_error = st.avg_err;
+    out->max_velocity = st.max_v;
+    out->max_acceleration = st.max_a;
+}
+
+void check_status(fsa_scene_s* s) {
+    const uint32_t status = s->h_state->status;
+    if (status & kStatusCapacity)
+        throw FsaError(FSA_E_CAPACITY, "a particle has more neighbors than a list holds (97)");
+    if (status & kStatusGrid) throw FsaError(FSA_E_STATE, "a particle left the cell grid (internal error)");
+}
+
+void upload_scene_table(fsa_scene_s* s) {
+    HIP_CHECK(hipMemcpyAsync(s->d_table, &s->hd, sizeof(SceneDev), hipMemcpyHostToDevice, s->ctx->stream));
+}
+
+void do_find_neighbors(fsa_scene_s* s) {
+    prepare_grid(s);
+    upload_scene_table(s);
+    search_set(s->ctx, s->single, sort_passes(s->hd));
+    read_state(s);
+    check_status(s);
+    s->searched = true;
+    s->lists_cached = false;
+}
+
+void do_simulate(fsa_scene_s* s, float dt, fsa_step_stats* stats) {
+    if (!s->searched) throw FsaError(FSA_E_STATE, "simulate called before find_neighbors");
+    if (!(dt > 0.0f)) throw FsaError(FSA_E_INVALID, "desired time step must be > 0");
+    s->hd.dt = dt;
+    if (s->desc.solver == FSA_SOLVER_SESPH) {
+        // SESPH's CFL scan covers inactive particles too (SESPHFluidSolver3D.hpp:96-107)
+        float a2 = 0.0f, v2 = 0.0f;
+        for (size_t k = 0; k < s->inactive_rows.size(); k++) {
+            const float* m = &s->inactive_movement[9 * k];
+            const float ax = m[6], ay = m[7], az = m[8];
+            a2 = std::fmax(a2, (ax * ax + ay * ay) + az * az);
+            const float vx = m[3] + dt * ax, vy = m[4] + dt * ay, vz = m[5] + dt * az;
+            v2 = std::fmax(v2, (vx * vx + vy * vy) + vz * vz);
+        }
+        s->hd.inactive_a2 = a2;
+        s->hd.inactive_v2 = v2;
+    }
+    upload_scene_table(s);
+    std::vector<fsa_scene_s*> one{s};
+    const bool bpass = (s->hd.flags & kFlagDynamicBoundary) || s->boundary_dirty;
+    simulate_set(s->ctx, s->single, one, s->desc.solver == FSA_SOLVER_SESPH, bpass);
+    s->boundary_dirty = false;
+    read_state(s);
+    if (s->desc.solver == FSA_SOLVER_IISPH) s->last_iterations = std::max<uint32_t>(s->h_state->iterations, 1);
+    fill_stats(s, stats);
+    s->searched = false;
+    s->lists_cached = false;
+    check_status(s);
+}
+
+// ---- upload / download ------------------------------------------------------------------
+void do_upload(fsa_scene_s* s, const fsa_particle_view* v, bool first) {
+    if (!v || !v->movement || !v->particle_data || !v->info) throw FsaError(FSA_E_INVALID, "incomplete particle view");
+    const uint64_t N = v->count;
+    if (N >= (1ull << kTypeShift)) throw FsaError(FSA_E_INVALID, "too many particles (max 2^30)");
+    std::vector<uint32_t> active;
+    active.reserve(N);
+    std::vector<uint32_t> inactive;
+    uint32_t nf = 0, nb = 0;
+    for (uint64_t i = 0; i < N; i++) {
+        const uint8_t t = ((const uint8_t*)v->info + i * v->info_stride)[4];
+        if (t == FSA_TYPE_INACTIVE) {
+            inactive.push_back((uint32_t)i);
+            continue;
+        }
+        if (t > 2) throw FsaError(FSA_E_INVALID, "unknown particle type");
+        active.push_back((uint32_t)i);
+        if (t == FSA_TYPE_NORMAL)
+            nf++;
+        else
+            nb++;
+    }
+    if (!first && (N != s->total || nf != s->nf || nb != s->nb))
+        throw FsaError(FSA_E_INVALID, "upload must keep the particle count and types of the scene");
+    s->total = N;
+    s->n = (uint32_t)active.size();
+    s->nf = nf;
+    s->nb = nb;
+    const uint32_t n = s->n;
+    std::vector<float4> pos(n), vel(n), acc(n), ext(n);
+    std::vector<float> pres(n), zero(n, 0.0f);
+    std::vector<uint32_t> idt(n);
+    bool any_ext = false, dyn_boundary = false;
+    int32_t lo[3] = {INT32_MAX, INT32_MAX, INT32_MAX}, hi[3] = {INT32_MIN, INT32_MIN, INT32_MIN};
+    const float r = s->desc.particle_size * 2.0f;
+    for (uint32_t k = 0; k < n; k++) {
+        const uint64_t i = active[k];
+        const float* m = (const float*)((const char*)v->movement + i * v->movement_stride);
+        const float* pd = (const float*)((const char*)v->particle_data + i * v->particle_data_stride);
+        const uint8_t t = ((const uint8_t*)v->info + i * v->info_stride)[4];
+        float e[3] = {0.0f, 0.0f, 0.0f};
+        if (v->external) {
+            const float* ep = (const float*)((const char*)v->external + i * v->external_stride);
+            e[0] = ep[0];
+            e[1] = ep[1];
+            e[2] = ep[2];
+        }
+        pos[k] = make_float4(m[0], m[1], m[2], pd[0]);
+        vel[k] = make_float4(m[3], m[4], m[5], pd[2]);
+        acc[k] = make_float4(m[6], m[7], m[8], 0.0f);
+        ext[k] = make_float4(e[0], e[1], e[2], 0.0f);
+        pres[k] = pd[1];
+        idt[k] = (uint32_t)i | ((uint32_t)t << kTypeShift);
+        if (e[0] != 0.0f || e[1] != 0.0f || e[2] != 0.0f) any_ext = true;
+        if (t == FSA_TYPE_BOUNDARY &&
+            (m[3] != 0.0f || m[4] != 0.0f || m[5] != 0.0f || m[6] != 0.0f || m[7] != 0.0f || m[8] != 0.0f ||
+             e[0] != 0.0f || e[1] != 0.0f || e[2] != 0.0f))
+            dyn_boundary = true;
+        const int32_t c[3] = {host_cell(m[0], r), host_cell(m[1], r), host_cell(m[2], r)};
+        for (int a = 0; a < 3; a++) {
+            lo[a] = std::min(lo[a], c[a]);
+            hi[a] = std::max(hi[a], c[a]);
+        }
+    }
+    // inactive rows are carried on the host unchanged (every solver loop skips them)
+    s->inactive_rows = inactive;
+    s->inactive_movement.assign(9 * inactive.size(), 0.0f);
+    s->inactive_pdata.assign(3 * inactive.size(), 0.0f);
+    s->inactive_ext.assign(3 * inactive.size(), 0.0f);
+    s->inactive_info.assign(8 * inactive.size(), 0);
+    for (size_t k = 0; k < inactive.size(); k++) {
+        const uint64_t i = inactive[k];
+        std::memcpy(&s->inactive_movement[9 * k], (const char*)v->movement + i * v->movement_stride, 36);
+        std::memcpy(&s->inactive_pdata[3 * k], (const char*)v->particle_data + i * v->particle_data_stride, 12);
+        std::memcpy(&s->inactive_info[8 * k], (const char*)v->info + i * v->info_stride, 8);
+        if (v->external)
+            std::memcpy(&s->inactive_ext[3 * k], (const char*)v->external + i * v->external_stride, 12);
+    }
+    SceneDev& h = s->hd;
+    h.flags &= ~(kFlagExternal | kFlagDynamicBoundary);
+    if (any_ext) h.flags |= kFlagExternal;
+    if (dyn_boundary) h.flags |= kFlagDynamicBoundary;
+    hipStream_t st = s->ctx->stream;
+    SceneState init{};
+    init.cur = 0;
+    init.dt_actual = s->desc.timestep;
+    for (int a = 0; a < 3; a++) {
+        init.bbox_min[a] = n ? lo[a] : 0;
+        init.bbox_max[a] = n ? hi[a] : 0;
+    }
+    if (n) {
+        HIP_CHECK(hipMemcpyAsync(h.pos[0], pos.data(), 16ull * n, hipMemcpyHostToDevice, st));
+        HIP_CHECK(hipMemcpyAsync(h.vel[0], vel.data(), 16ull * n, hipMemcpyHostToDevice, st));
+        HIP_CHECK(hipMemcpyAsync(h.acc[0], acc.data(), 16ull * n, hipMemcpyHostToDevice, st));
+        HIP_CHECK(hipMemcpyAsync(h.ext[0], ext.data(), 16ull * n, hipMemcpyHostToDevice, st));
+        HIP_CHECK(hipMemcpyAsync(h.pres[0], pres.data(), 4ull * n, hipMemcpyHostToDevice, st));
+        HIP_CHECK(hipMemcpyAsync(h.err[0], zero.data(), 4ull * n, hipMemcpyHostToDevice, st));
+        HIP_CHECK(hipMemcpyAsync(h.idt[0], idt.data(), 4ull * n, hipMemcpyHostToDevice, st));
+    }
+    HIP_CHECK(hipMemcpyAsync(h.state, &init, sizeof(SceneState), hipMemcpyHostToDevice, st));
+    HIP_CHECK(hipStreamSynchronize(st));
+    *s->h_state = init;
+    s->searched = false;
+    s->boundary_dirty = true;
+    s->lists_cached = false;
+}
+
+struct HostCopy {
+    std::vector<float4> pos, vel, acc, ext;
+    std::vector<float> pres, err;
+    std::vector<uint32_t> idt;
+};
+
+void fetch(fsa_scene_s* s, HostCopy& hc, bool with_err) {
+    read_state(s);
+    const uint32_t c = s->h_state->cur;
+    const uint32_t n = s->n;
+    hc.pos.resize(n);
+    hc.vel.resize(n);
+    hc.acc.resize(n);
+    hc.ext.resize(n);
+    hc.pres.resize(n);
+    hc.idt.resize(n);
+    hipStream_t st = s->ctx->stream;
+    const SceneDev& h = s->hd;
+    if (n) {
+        HIP_CHECK(hipMemcpyAsync(hc.pos.data(), h.pos[c], 16ull * n, hipMemcpyDeviceToHost, st));
+        HIP_CHECK(hipMemcpyAsync(hc.vel.data(), h.vel[c], 16ull * n, hipMemcpyDeviceToHost, st));
+        HIP_CHECK(hipMemcpyAsync(hc.acc.data(), h.acc[c], 16ull * n, hipMemcpyDeviceToHost, st));
+        HIP_CHECK(hipMemcpyAsync(hc.ext.data(), h.ext[c], 16ull * n, hipMemcpyDeviceToHost, st));
+        HIP_CHECK(hipMemcpyAsync(hc.pres.data(), h.pres[c], 4ull * n, hipMemcpyDeviceToHost, st));
+        HIP_CHECK(hipMemcpyAsync(hc.idt.data(), h.idt[c], 4ull * n, hipMemcpyDeviceToHost, st));
+        if (with_err) {
+            hc.err.resize(n);
+            HIP_CHECK(hipMemcpyAsync(hc.err.data(), h.err[c], 4ull * n, hipMemcpyDeviceToHost, st));
+        }
+    }
+    HIP_CHECK(hipStreamSynchronize(st));
+}
+
+void do_download(fsa_scene_s* s, const fsa_particle_view* v) {
+    if (!v || v->count != s->total) throw FsaError(FSA_E_INVALID, "download view count mismatch");
+    HostCopy hc;
+    fetch(s, hc, false);
+    const bool ext_live = (s->hd.flags & kFlagExternal) != 0;
+    for (uint32_t k = 0; k < s->n; k++) {
+        const uint64_t i = hc.idt[k] & kIdMask;
+        const uint8_t t = (uint8_t)(hc.idt[k] >> kTypeShift);
+        if (v->movement) {
+            float* m = (float*)((char*)v->movement + i * v->movement_stride);
+            m[0] = hc.pos[k].x;
+            m[1] = hc.pos[k].y;
+            m[2] = hc.pos[k].z;
+            m[3] = hc.vel[k].x;
+            m[4] = hc.vel[k].y;
+            m[5] = hc.vel[k].z;
+            m[6] = hc.acc[k].x;
+            m[7] = hc.acc[k].y;
+            m[8] = hc.acc[k].z;
+        }
+        if (v->particle_data) {
+            float* pd = (float*)((char*)v->particle_data + i * v->particle_data_stride);
+            pd[0] = hc.pos[k].w;
+            pd[1] = hc.pres[k];
+            pd[2] = hc.vel[k].w;
+        }
+        if (v->info) ((uint8_t*)v->info + i * v->info_stride)[4] = t;
+        if (v->external) {
+            float* e = (float*)((char*)v->external + i * v->external_stride);
+            e[0] = ext_live ? hc.ext[k].x : 0.0f;
+            e[1] = ext_live ? hc.ext[k].y : 0.0f;
+            e[2] = ext_live ? hc.ext[k].z : 0.0f;
+        }
+    }
+    for (size_t k = 0; k < s->inactive_rows.size(); k++) {
+        const uint64_t i = s->inactive_rows[k];
+        if (v->movement) std::memcpy((char*)v->movement + i * v->movement_stride, &s->inactive_movement[9 * k], 36);
+        if (v->particle_data)
+            std::memcpy((char*)v->particle_data + i * v->particle_data_stride, &s->inactive_pdata[3 * k], 12);
+        if (v->info) std::memcpy((char*)v->info + i * v->info_stride, &s->inactive_info[8 * k], 8);
+        if (v->external)
+            std::memcpy((char*)v->external + i * v->external_stride, &s->inactive_ext[3 * k], 12);
+    }
+}
+
+void do_download_iisph(fsa_scene_s* s, void* rec, uint64_t stride) {
+    if (!rec) throw FsaError(FSA_E_INVALID, "records is NULL");
+    HostCopy hc;
+    fetch(s, hc, true);
+    const uint32_t n = s->n;
+    std::vector<float4> vs(n), an(n);
+    std::vector<float> src(n), dg(n);
+    hipStream_t st = s->ctx->stream;
+    if (n) {
+        HIP_CHECK(hipMemcpyAsync(vs.data(), s->hd.vstar, 16ull * n, hipMemcpyDeviceToHost, st));
+        HIP_CHECK(hipMemcpyAsync(an.data(), s->hd.anp, 16ull * n, hipMemcpyDeviceToHost, st));
+        HIP_CHECK(hipMemcpyAsync(src.data(), s->hd.src, 4ull * n, hipMemcpyDeviceToHost, st));
+        HIP_CHECK(hipMemcpyAsync(dg.data(), s->hd.diag, 4ull * n, hipMemcpyDeviceToHost, st));
+        HIP_CHECK(hipStreamSynchronize(st));
+    }
+    for (uint32_t k = 0; k < n; k++) {
+        const uint64_t i = hc.idt[k] & kIdMask;
+        float* r = (float*)((char*)rec + i * stride);
+        r[0] = vs[k].x;
+        r[1] = vs[k].y;
+        r[2] = vs[k].z;
+        r[3] = an[k].x;
+        r[4] = an[k].y;
+        r[5] = an[k].z;
+        r[6] = src[k];
+        r[7] = dg[k];
+        r[8] = hc.err[k];
+    }
+}
+
+void build_csr(fsa_scene_s* s) {
+    if (s->lists_cached) return;
+    if (!s->searched) throw FsaError(FSA_E_STATE, "neighbor lists are only valid after find_neighbors");
+    read_state(s);
+    const uint32_t c = s->h_state->cur;
+    const uint32_t n = s->n, nf = s->nf;
+    std::vector<uint32_t> idt(n), fidx(nf), cnt(nf), nbr((size_t)kMaxNeighbors * s->hd.list_stride);
+    hipStream_t st = s->ctx->stream;
+    if (n) HIP_CHECK(hipMemcpyAsync(idt.data(), s->hd.idt[c], 4ull * n, hipMemcpyDeviceToHost, st));
+    if (nf) {
+        HIP_CHECK(hipMemcpyAsync(fidx.data(), s->hd.fluid_idx, 4ull * nf, hipMemcpyDeviceToHost, st));
+        HIP_CHECK(hipMemcpyAsync(cnt.data(), s->hd.nbr_cnt, 4ull * nf, hipMemcpyDeviceToHost, st));
+        HIP_CHECK(hipMemcpyAsync(nbr.data(), s->hd.nbr, 4ull * nbr.size(), hipMemcpyDeviceToHost, st));
+    }
+    HIP_CHECK(hipStreamSynchronize(st));
+    std::vector<std::vector<uint64_t>> rows(s->total);
+    for (uint32_t k = 0; k < nf; k++) {
+        const uint64_t row = idt[fidx[k]] & kIdMask;
+        auto& r = rows[row];
+        r.reserve(cnt[k]);
+        for (uint32_t e = 0; e < cnt[k]; e++) {
+            const uint32_t j = nbr[(size_t)e * s->hd.list_stride + k] & kIndexMask;
+            r.push_back(idt[j] & kIdMask);
+        }
+        std::sort(r.begin(), r.end());
+    }
+    s->csr_off.assign(s->total + 1, 0);
+    s->csr_cols.clear();
+    for (uint64_t i = 0; i < s->total; i++) {
+        s->csr_off[i] = s->csr_cols.size();
+        s->csr_cols.insert(s->csr_cols.end(), rows[i].begin(), rows[i].end());
+    }
+    s->csr_off[s->total] = s->csr_cols.size();
+    s->lists_cached = true;
+}
+
+void destroy_scene(fsa_scene_s* s) {
+    if (!s) return;
+    {
+        std::lock_guard<std::mutex> g(s->ctx->mu);
+        auto& v = s->ctx->scenes;
+        v.erase(std::remove(v.begin(), v.end(), s), v.end());
+    }
+    if (s->dmem) {
+        (void)hipFree(s->dmem);
+        s->ctx->in_use -= s->bytes;
+    }
+    if (s->cells) {
+        (void)hipFree(s->cells);
+        s->ctx->in_use -= s->cells_cap * sizeof(uint4);
+    }
+    if (s->d_table) (void)hipFree(s->d_table);
+    for (auto* m : s->map_allocs) (void)hipFree(m);
+    if (s->h_state) (void)hipHostFree(s->h_state);
+    delete s;
+}
+
+}  // namespace
+
+// =====================================================================================
+extern "C" {
+
+FSA_API int32_t fsa_version(void) { return 1; }
+
+FSA_API const char* fsa_last_error(void) { return g_error.c_str(); }
+
+FSA_API void fsa_scene_desc_defaults(fsa_scene_desc* d, uint32_t dimension, uint32_t solver) {
+    if (!d) return;
+    std::memset(d, 0, sizeof(*d));
+    d->dimension = dimension;
+    d->solver = solver;
+    d->neighbor_lists = FSA_LIST_PLAIN;
+    d->flags = dimension == 2 ? FSA_FLAG_EMULATE_INT_ABS : 0u;
+    d->rest_density = 1.0f;
+    d->gravity = 9.81f;
+    d->particle_size = 1.0f;
+    d->iisph_max_density_error_allowed = 0.001f;
+    d->iisph_min_iterations = 2;
+    d->iisph_max_iterations = 100;
+    d->iisph_omega = 0.5f;
+    d->iisph_gamma = 0.7f;
+    d->iisph_viscosity = 5.0f;
+    d->iisph_single_layer_boundary = 0;
+    d->iisph_single_layer_gamma_1 = 1.1f;
+    d->sesph_stiffness_k = 100000.0f;
+    d->sesph_viscosity = 5.0f;
+    d->sesph_single_layer_boundary = 0;
+    d->sesph_single_layer_gamma_1 = 1.1f;
+    d->sesph_single_layer_gamma_2 = 1.1f;
+    d->timestep = 0.001f;
+    d->lambda_v = 0.4f;
+    d->lambda_a = 0.25f;
+}
+
+FSA_API int32_t fsa_context_create(int32_t device, uint64_t budget, fsa_context* out) {
+    return guarded([&] {
+        if (!out) throw FsaError(FSA_E_INVALID, "out is NULL");
+        int count = 0;
+        HIP_CHECK(hipGetDeviceCount(&count));
+        if (device < 0 || device >= count) throw FsaError(FSA_E_INVALID, "no such device");
+        HIP_CHECK(hipSetDevice(device));
+        auto* c = new fsa_context_s();
+        c->device = device;
+        c->budget = budget;
+        if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+            delete c;
+            throw FsaError(FSA_E_DEVICE, "hipStreamCreate failed");
+        }
+        for (auto& e : c->ev) HIP_CHECK(hipEventCreate(&e));
+        *out = c;
+    });
+}
+
+FSA_API int32_t fsa_context_destroy(fsa_context c) {
+    return guarded([&] {
+        if (!c) return;
+        (void)hipSetDevice(c->device);
+        (void)hipStreamSynchronize(c->stream);
+        std::vector<fsa_scene_s*> live;
+        {
+            std::lock_guard<std::mutex> g(c->mu);
+            live = c->scenes;
+        }
+        for (auto* s : live) destroy_scene(s);
+        for (auto& e : c->ev) (void)hipEventDestroy(e);
+        (void)hipStreamDestroy(c->stream);
+        delete c;
+    });
+}
+
+FSA_API int32_t fsa_context_synchronize(fsa_context c) {
+    return guarded([&] {
+        if (!c) throw FsaError(FSA_E_INVALID, "context is NULL");
+        HIP_CHECK(hipStreamSynchronize(c->stream));
+    });
+}
+
+FSA_API int32_t fsa_context_memory_in_use(fsa_context c, uint64_t* bytes) {
+    return guarded([&] {
+        if (!c || !bytes) throw FsaError(FSA_E_INVALID, "NULL argument");
+        *bytes = c->in_use.load();
+    });
+}
+
+FSA_API int32_t fsa_scene_memory_estimate(const fsa_scene_desc* desc, uint64_t n_fluid, uint64_t n_boundary,
+                                          uint64_t* bytes) {
+    return guarded([&] {
+        validate_desc(desc);
+        if (!bytes) throw FsaError(FSA_E_INVALID, "bytes is NULL");
+        *bytes = plan_bytes(make_plan(n_fluid + n_boundary, n_fluid)) + sizeof(SceneDev);
+    });
+}
+
+FSA_API int32_t fsa_scene_create(fsa_context c, const fsa_scene_desc* desc, const fsa_particle_view* view,
+                                 fsa_scene* out) {
+    fsa_scene_s* s = nullptr;
+    int32_t rc = guarded([&] {
+        if (!c || !out) throw FsaError(FSA_E_INVALID, "NULL argument");
+        validate_desc(desc);
+        if (!view) throw FsaError(FSA_E_INVALID, "view is NULL");
+        HIP_CHECK(hipSetDevice(c->device));
+        // count active particles first to size the allocation
+        uint64_t nf = 0, nb = 0;
+        for (uint64_t i = 0; i < view->count; i++) {
+            const uint8_t t = ((const uint8_t*)view->info + i * view->info_stride)[4];
+            if (t == FSA_TYPE_NORMAL) nf++;
+            if (t == FSA_TYPE_BOUNDARY) nb++;
+        }
+        const Plan p = make_plan(nf + nb, nf);
+        if (p.n >= (1ull << 27)) throw FsaError(FSA_E_INVALID, "scene too large for 27-bit run starts");
+        const uint64_t bytes = plan_bytes(p);
+        if (c->budget && c->in_use.load() + bytes > c->budget)
+            throw FsaError(FSA_E_BUDGET, "scene does not fit the HBM budget");
+        s = new fsa_scene_s();
+        s->ctx = c;
+        s->desc = *desc;
+        {
+            std::lock_guard<std::mutex> g(c->mu);
+            c->scenes.push_back(s);
+        }
+        if (hipMalloc(&s->dmem, bytes) != hipSuccess) {
+            s->dmem = nullptr;
+            throw FsaError(FSA_E_NOMEM, "device allocation failed");
+        }
+        s->bytes = bytes;
+        c->in_use += bytes;
+        HIP_CHECK(hipHostMalloc((void**)&s->h_state, sizeof(SceneState), hipHostMallocDefault));
+        std::memset(s->h_state, 0, sizeof(SceneState));
+        carve(s, p);
+        fill_params(s);
+        HIP_CHECK(hipMalloc(&s->d_table, sizeof(SceneDev)));
+        MapBuilder mb;
+        mb.add(0, s->hd);
+        build_launch_set(mb, s->d_table, s->single, s->map_allocs);
+        do_upload(s, view, true);
+        upload_scene_table(s);
+        HIP_CHECK(hipStreamSynchronize(c->stream));
+        *out = s;
+    });
+    if (rc != FSA_OK && s) destroy_scene(s);
+    return rc;
+}
+
+FSA_API int32_t fsa_scene_destroy(fsa_scene s) {
+    return guarded([&] {
+        if (!s) return;
+        (void)hipStreamSynchronize(s->ctx->stream);
+        destroy_scene(s);
+    });
+}
+
+FSA_API int32_t fsa_scene_upload(fsa_scene s, const fsa_particle_view* v) {
+    return guarded([&] {
+        if (!s) throw FsaError(FSA_E_INVALID, "scene is NULL");
+        do_upload(s, v, false);
+    });
+}
+
+FSA_API int32_t fsa_scene_download(fsa_scene s, const fsa_particle_view* v) {
+    return guarded([&] {
+        if (!s) throw FsaError(FSA_E_INVALID, "scene is NULL");
+        do_download(s, v);
+    });
+}
+
+FSA_API int32_t fsa_scene_download_iisph(fsa_scene s, void* rec, uint64_t stride) {
+    return guarded([&] {
+        if (!s) throw FsaError(FSA_E_INVALID, "scene is NULL");
+        do_download_iisph(s, rec, stride);
+    });
+}
+
+FSA_API int32_t fsa_scene_set_desc(fsa_scene s, const fsa_scene_desc* d) {
+    return guarded([&] {
+        if (!s) throw FsaError(FSA_E_INVALID, "scene is NULL");
+        validate_desc(d);
+        if (d->dimension != s->desc.dimension) throw FsaError(FSA_E_INVALID, "dimension cannot change");
+        s->desc = *d;
+        const uint32_t stamp = s->hd.stamp;
+        fill_params(s);
+        s->hd.stamp = stamp;
+        upload_scene_table(s);
+    });
+}
+
+FSA_API int32_t fsa_scene_find_neighbors(fsa_scene s) {
+    return guarded([&] {
+        if (!s) throw FsaError(FSA_E_INVALID, "scene is NULL");
+        do_find_neighbors(s);
+    });
+}
+
+FSA_API int32_t fsa_scene_get_neighbors(fsa_scene s, uint64_t index, uint64_t* out, uint64_t cap, uint64_t* count) {
+    return guarded([&] {
+        if (!s || !count) throw FsaError(FSA_E_INVALID, "NULL argument");
+        if (index >= s->total) throw FsaError(FSA_E_INVALID, "index out of range");
+        build_csr(s);
+        const uint64_t b = s->csr_off[index], e = s->csr_off[index + 1];
+        *count = e - b;
+        if (out)
+            for (uint64_t k = 0; k < std::min(cap, e - b); k++) out[k] = s->csr_cols[b + k];
+        if (e - b > cap && out) throw FsaError(FSA_E_INVALID, "output capacity too small");
+    });
+}
+
+FSA_API int32_t fsa_scene_export_neighbors(fsa_scene s, uint64_t* off, uint64_t* cols, uint64_t cap, uint64_t* nnz) {
+    return guarded([&] {
+        if (!s || !nnz) throw FsaError(FSA_E_INVALID, "NULL argument");
+        build_csr(s);
+        *nnz = s->csr_cols.size();
+        if (off) std::memcpy(off, s->csr_off.data(), s->csr_off.size() * sizeof(uint64_t));
+        if (cols) {
+            if (cap < s->csr_cols.size()) throw FsaError(FSA_E_INVALID, "output capacity too small");
+            std::memcpy(cols, s->csr_cols.data(), s->csr_cols.size() * sizeof(uint64_t));
+        }
+    });
+}
+
+FSA_API int32_t fsa_scene_neighbor_stats(fsa_scene s, uint64_t* total, uint64_t* dev_bytes, uint64_t* ref_bytes) {
+    return guarded([&] {
+        if (!s) throw FsaError(FSA_E_INVALID, "scene is NULL");
+        if (!s->searched) throw FsaError(FSA_E_STATE, "neighbor lists are only valid after find_neighbors");
+        const uint32_t nf = s->nf;
+        std::vector<uint32_t> cnt(nf), nbr((size_t)kMaxNeighbors * s->hd.list_stride);
+        hipStream_t st = s->ctx->stream;
+        if (nf) {
+            HIP_CHECK(hipMemcpyAsync(cnt.data(), s->hd.nbr_cnt, 4ull * nf, hipMemcpyDeviceToHost, st));
+            HIP_CHECK(hipMemcpyAsync(nbr.data(), s->hd.nbr, 4ull * nbr.size(), hipMemcpyDeviceToHost, st));
+            HIP_CHECK(hipStreamSynchronize(st));
+        }
+        uint64_t t = 0, rb = 0;
+        for (uint32_t k = 0; k < nf; k++) {
+            t += cnt[k];
+            if (!cnt[k]) continue;
+            // NeighborStorage size model: 4 B first index + 2-bit controls + 0/1/4 B payload per delta
+            uint64_t payload = 0;
+            uint32_t prev = nbr[k] & kIndexMask;
+            for (uint32_t e = 1; e < cnt[k]; e++) {
+                const uint32_t j = nbr[(size_t)e * s->hd.list_stride + k] & kIndexMask;
+                const uint32_t d = j - prev;
+                prev = j;
+                if (d > 258)
+                    payload += 4;
+                else if (d > 2)
+                    payload += 1;
+            }
+            rb += 4 + (2ull * (cnt[k] - 1) + 7) / 8 + payload;
+        }
+        if (total) *total = t;
+        if (dev_bytes) *dev_bytes = 4 * t;
+        if (ref_bytes) *ref_bytes = rb;
+    });
+}
+
+FSA_API int32_t fsa_scene_simulate(fsa_scene s, float dt, fsa_step_stats* stats) {
+    return guarded([&] {
+        if (!s) throw FsaError(FSA_E_INVALID, "scene is NULL");
+        do_simulate(s, dt, stats);
+    });
+}
+
+FSA_API int32_t fsa_scene_step(fsa_scene s, uint32_t steps, fsa_step_stats* stats) {
+    return guarded([&] {
+        if (!s) throw FsaError(FSA_E_INVALID, "scene is NULL");
+        for (uint32_t k = 0; k < steps; k++) {
+            do_find_neighbors(s);
+            do_simulate(s, s->desc.timestep, stats);
+        }
+    });
+}
+
+FSA_API int32_t fsa_batch_step(fsa_context c, const fsa_scene* scenes, uint32_t n, uint32_t steps,
+                               fsa_step_stats* stats) {
+    return guarded([&] {
+        if (!c || (!scenes && n)) throw FsaError(FSA_E_INVALID, "NULL argument");
+        std::vector<fsa_scene_s*> v(scenes, scenes + n);
+        for (auto* s : v)
+            if (!s || s->ctx != c) throw FsaError(FSA_E_INVALID, "scene of another context");
+        for (uint32_t k = 0; k < steps; k++) {
+            for (uint32_t q = 0; q < n; q++) {
+                do_find_neighbors(v[q]);
+                do_simulate(v[q], v[q]->desc.timestep, stats ? &stats[q] : nullptr);
+            }
+        }
+    });
+}
+
+FSA_API int32_t fsa_batch_set_graph(fsa_context c, uint32_t enabled) {
+    return guarded([&] {
+        if (!c) throw FsaError(FSA_E_INVALID, "context is NULL");
+        c->graph = enabled != 0;
+    });
+}
+
+FSA_API int32_t fsa_context_set_block_size(fsa_context c, uint32_t threads) {
+    return guarded([&] {
+        if (!c) throw FsaError(FSA_E_INVALID, "context is NULL");
+        if (threads != 0 && threads != 256) throw FsaError(FSA_E_INVALID, "unsupported block size");
+        c->block_override = threads;
+    });
+}
+
+FSA_API int32_t fsa_context_set_timing(fsa_context c, uint32_t enabled) {
+    return guarded([&] {
+        if (!c) throw FsaError(FSA_E_INVALID, "context is NULL");
+        c->timing = enabled != 0;
+        for (int k = 0; k < FSA_TIMER_COUNT; k++) {
+            c->timer_ms[k] = 0.0;
+            c->timer_n[k] = 0;
+        }
+    });
+}
+
+FSA_API int32_t fsa_context_get_timing(fsa_context c, double* total_ms, uint64_t* launches, uint32_t n) {
+    return guarded([&] {
+        if (!c) throw FsaError(FSA_E_INVALID, "context is NULL");
+        for (uint32_t k = 0; k < n && k < (uint32_t)FSA_TIMER_COUNT; k++) {
+            if (total_ms) total_ms[k] = c->timer_ms[k];
+            if (launches) launches[k] = c->timer_n[k];
+        }
+    });
+}
+
+}  // extern "C"

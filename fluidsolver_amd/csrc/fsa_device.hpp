@@ -1,0 +1,234 @@
+// fsa_device.hpp — device-side data layout and SPH math of the gfx950 hot path.
+//
+// Every scene lives in HBM as one combined particle set (fluid + boundary,
+// inactive particles excluded) kept in the order libFluid's compressed search
+// gives the collection: a stable sort by Morton cell key every step
+// (CompressedNeighbors.cpp:191-201).  Because the order, the neighbor order
+// (ascending combined index) and the FP32 evaluation order below all follow
+// the reference, device results are bit-identical to the reference-order CPU
+// restatement.  The translation unit must be compiled with -ffp-contract=off
+// (no fused multiply-add) and IEEE-correct division and square root.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace fsa {
+
+constexpr int kBlock = 256;            // threads per block of the particle kernels
+constexpr int kMaxNeighbors = 97;      // NeighborStorage capacity (CompressedNeighbors.hpp:29-34)
+constexpr uint32_t kBoundaryBit = 0x80000000u;  // neighbor entry flag: neighbor is a boundary particle
+constexpr uint32_t kIndexMask = 0x7fffffffu;
+constexpr uint32_t kTypeShift = 30;    // idt word: caller index (30 bits) | type << 30
+constexpr uint32_t kIdMask = (1u << kTypeShift) - 1u;
+constexpr uint32_t kEmptyCell = 0xffffffffu;
+
+// radix sort geometry
+constexpr int kSortThreads = 256;
+constexpr int kSortItems = 8;
+constexpr int kSortTile = kSortThreads * kSortItems;  // 2048 keys per tile
+constexpr int kRadixBits = 8;
+constexpr int kRadix = 1 << kRadixBits;
+
+// SceneDev.flags
+constexpr uint32_t kFlagHOne = 1u;          // kernel h == 1: q = |x|, pre = x / |x| exactly
+constexpr uint32_t kFlagRho0SqOne = 2u;     // rest_density^2 == 1: p / rho0^2 == p exactly
+constexpr uint32_t kFlagExternal = 4u;      // some particle carries a non-zero external acceleration
+constexpr uint32_t kFlagDynamicBoundary = 8u;  // boundary particles move (non-zero v, a or ext)
+constexpr uint32_t kFlagSingleLayer = 16u;  // IISPH/SESPH single-layer boundary scaling
+constexpr uint32_t kFlagSesph = 32u;        // SESPH solver (else IISPH)
+constexpr uint32_t kFlagPlainLists = 64u;   // uncompressed 32-bit neighbor lists
+
+// SceneState.status bits
+constexpr uint32_t kStatusCapacity = 1u;    // a particle had more than 97 neighbors
+constexpr uint32_t kStatusGrid = 2u;        // a particle left the cell grid of this step
+constexpr uint32_t kStatusNonFinite = 4u;
+
+// Per-scene state written by the device and read back once per step.
+struct alignas(16) SceneState {
+    double avg_err;        // average predicted density error of the last iteration (FP64)
+    uint32_t cur;          // which particle buffer holds the current (sorted) state
+    uint32_t active;       // Jacobi loop still iterating
+    uint32_t iterations;   // Jacobi iterations of this step (stat_last_iteration_count)
+    uint32_t status;       // kStatus* bits
+    uint32_t done[4];      // last-block counters: 0 permute, 1 pass B, 2 integrate, 3 cfl
+    float max_a2, max_v2;  // CFL maxima of the last iteration / SESPH step
+    float dt_actual, max_a, max_v;
+    uint32_t err_count;
+    int32_t bbox_min[4], bbox_max[4];  // cell bounding box after the last integration
+    uint32_t max_count, pad;
+    uint64_t total_neighbors;
+};
+
+struct alignas(16) Partial {
+    double err;
+    uint32_t count;
+    float max_a2;
+    float max_v2;
+    uint32_t pad;
+};
+
+struct alignas(16) BoxPartial {
+    int32_t lo[4];
+    int32_t hi[4];
+};
+
+// Per-scene descriptor, one per scene in a launch table; blocks find their
+// scene through a block map {scene, block within scene}.
+struct alignas(16) SceneDev {
+    uint32_t n;            // combined active particles
+    uint32_t nf;           // fluid particles
+    uint32_t list_stride;  // neighbor-list stride (nf rounded up to 64)
+    uint32_t flags;
+
+    float rest_density, gravity, particle_size, search_radius;
+    float r2;              // Math::pow2(search_radius)
+    float h, alpha;        // CubicSplineKernel3D::initialize
+    float rho0sq;          // Math::pow2(rest_density)
+    float visc_c01;        // 0.01f * particle_size * particle_size
+    float visc2;           // 2 * viscosity of the active solver
+    float omega, gamma, gamma1, gamma2, max_err;
+    uint32_t min_it, max_it;
+    float stiffness;
+    float dt;              // desired time step of the next step
+    float gen_dt, lambda_a, lambda_v;
+    float inactive_a2, inactive_v2;  // SESPH CFL contribution of inactive particles
+    uint32_t pad0;
+
+    // cell grid of this step (dense, linear x-fastest over the cell bounding box)
+    int32_t cmin[4];
+    uint32_t cdim[4];      // x, y, z, cell count
+    uint32_t levels;       // Morton levels of the compact key
+    uint32_t stamp;        // cell-table generation
+    uint32_t ntiles;       // radix tiles
+    uint32_t nblocks;      // kBlock-blocks over n
+
+    // combined particle arrays, two buffers each ([cur] is current)
+    float4* pos[2];        // x, y, z, mass
+    float4* vel[2];        // vx, vy, vz, density
+    float4* acc[2];        // acceleration (IISPH: pressure acceleration)
+    float4* ext[2];        // external (non-pressure) acceleration
+    float* pres[2];
+    float* err[2];         // IISPH predicted density error (kept across steps)
+    uint32_t* idt[2];      // caller index | type << 30
+    // per-step scratch (combined index)
+    float* rho_new;
+    float4* vstar;         // predicted velocity
+    float4* anp;           // original non-pressure acceleration
+    float* src;            // source term
+    float* diag;           // diagonal element a_ii
+    // search
+    uint32_t* keys[2];
+    uint32_t* vals[2];
+    uint32_t* hist;        // kRadix * ntiles tile offsets + kRadix digit totals
+    uint4* cells;          // {start, end, stamp, -}
+    uint32_t* fluid_idx;   // combined index of fluid rank k
+    uint32_t* block_count; // fluid count per kBlock-block, scanned in place
+    uint32_t* nbr;         // neighbor entries, entry e of fluid rank k at e * list_stride + k
+    uint32_t* nbr_cnt;
+    Partial* partial;      // per fluid block
+    BoxPartial* box;       // per combined block
+    SceneState* state;
+};
+
+struct BlockRef {
+    uint32_t scene;
+    uint32_t block;
+};
+
+// ---- reference-order arithmetic ------------------------------------------
+// glm::dot for vec3: ((x*x') + (y*y')) + (z*z')
+__device__ __forceinline__ float dot3(float ax, float ay, float az, float bx, float by, float bz) {
+    return (ax * bx + ay * by) + az * bz;
+}
+
+// CubicSplineKernel3D::GetKernelValue (CubicSplineKernel3D.cpp:7-22) from |x|
+__device__ __forceinline__ float kernel_w(const SceneDev& S, float len) {
+    float q = (S.flags & kFlagHOne) ? len : len / S.h;
+    float ret = 0.0f;
+    if (q < 1.0f) {
+        float a = 2.0f - q, b = 1.0f - q;
+        ret = a * a * a - 4.0f * (b * b * b);
+    } else if (q < 2.0f) {
+        float a = 2.0f - q;
+        ret = a * a * a;
+    }
+    return S.alpha * ret;
+}
+
+// GetKernelDerivativeReversedValue(x_j, x_i) for d = x_i - x_j
+// (CubicSplineKernel3D.cpp:24-47, 62-65): alpha * (pre * f'(q)) with
+// pre = h * d / |d|, zero when |d| <= FLT_EPSILON.  The reference negates
+// twice (derivative and reversal); negation is exact, so both are dropped.
+__device__ __forceinline__ void kernel_grad(const SceneDev& S, float dx, float dy, float dz, float len,
+                                            float& gx, float& gy, float& gz) {
+    const bool hone = (S.flags & kFlagHOne) != 0;
+    float q = hone ? len : len / S.h;
+    float px = 0.0f, py = 0.0f, pz = 0.0f;
+    if (len > 1.1920928955078125e-07f) {
+        if (hone) {
+            px = dx / len;
+            py = dy / len;
+            pz = dz / len;
+        } else {
+            px = (S.h * dx) / len;
+            py = (S.h * dy) / len;
+            pz = (S.h * dz) / len;
+        }
+    }
+    float s = 0.0f;
+    bool inside = true;
+    if (q < 1.0f) {
+        float a = 2.0f - q, b = 1.0f - q;
+        s = -3.0f * (a * a) + 12.0f * (b * b);
+    } else if (q < 2.0f) {
+        float a = 2.0f - q;
+        s = -3.0f * (a * a);
+    } else {
+        inside = false;
+    }
+    if (inside) {
+        gx = S.alpha * (px * s);
+        gy = S.alpha * (py * s);
+        gz = S.alpha * (pz * s);
+    } else {
+        gx = 0.0f;
+        gy = 0.0f;
+        gz = 0.0f;
+    }
+}
+
+__device__ __forceinline__ float div_rho0sq(const SceneDev& S, float v) {
+    return (S.flags & kFlagRho0SqOne) ? v : v / S.rho0sq;
+}
+
+// (int)std::floor(x / search_radius) (CompressedNeighbors.cpp:37-41)
+__device__ __forceinline__ int32_t cell_of(float x, float r) {
+    return (int32_t)floorf(x / r);
+}
+
+// Compact Morton key.  libFluid keys a cell by morton3D_64(c + 2^31) keeping
+// 22/21/21 bits (CompressedNeighbors.cpp:43-56).  When every cell coordinate
+// lies in [-2^L, 2^L), the bits above level L of each coordinate equal its
+// sign bit, and the 64-bit key order equals the order of
+//   sx << (3L+2) | sz << (3L+1) | sy << 3L | interleave_L(x, y, z)
+// (the top key bits are x@63, z@62, y@61, x@60, ...).  3L + 3 <= 32 bits.
+__device__ __host__ __forceinline__ uint32_t compact_key(int32_t cx, int32_t cy, int32_t cz, uint32_t L) {
+    uint32_t mask = (1u << L) - 1u;
+    uint32_t x = (uint32_t)cx & mask, y = (uint32_t)cy & mask, z = (uint32_t)cz & mask;
+    uint32_t key = 0;
+    for (uint32_t b = 0; b < L; b++) {
+        key |= ((x >> b) & 1u) << (3 * b);
+        key |= ((y >> b) & 1u) << (3 * b + 1);
+        key |= ((z >> b) & 1u) << (3 * b + 2);
+    }
+    uint32_t sx = cx < 0, sy = cy < 0, sz = cz < 0;
+    key |= sy << (3 * L);
+    key |= sz << (3 * L + 1);
+    key |= sx << (3 * L + 2);
+    return key;
+}
+
+__device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
+
+}  // namespace fsa

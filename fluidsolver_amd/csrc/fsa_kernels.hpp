@@ -1,0 +1,39 @@
+// fsa_kernels.hpp — host-side launchers of the gfx950 kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "fsa_device.hpp"
+
+namespace fsa {
+
+// Device-resident launch description of a set of scenes: the scene table and
+// one block map per launch domain.
+struct LaunchSet {
+    SceneDev* scenes = nullptr;   // device table
+    BlockRef* map_all = nullptr;  // kBlock-blocks over each scene's combined particles
+    BlockRef* map_fluid = nullptr;   // kBlock-blocks over each scene's fluid particles
+    BlockRef* map_tiles = nullptr;   // radix tiles
+    BlockRef* map_scan = nullptr;    // (scene, digit)
+    BlockRef* map_scene = nullptr;   // one block per scene
+    uint32_t nb_all = 0, nb_fluid = 0, nb_tiles = 0, nb_scan = 0, nb_scene = 0;
+};
+
+// neighborhood search
+void launch_keys(const LaunchSet& L, hipStream_t st);
+void launch_sort(const LaunchSet& L, uint32_t passes, hipStream_t st);
+void launch_reorder(const LaunchSet& L, uint32_t sbuf, hipStream_t st);
+void launch_lists(const LaunchSet& L, hipStream_t st);
+
+// IISPH / SESPH
+void launch_iisph_prepare(const LaunchSet& L, bool boundary_pass, hipStream_t st);
+void launch_iisph_diagonal(const LaunchSet& L, hipStream_t st);
+void launch_iisph_begin_loop(const LaunchSet& L, hipStream_t st);
+void launch_iisph_pass_a(const LaunchSet& L, hipStream_t st);
+void launch_iisph_pass_b(const LaunchSet& L, hipStream_t st);
+void launch_sesph_density(const LaunchSet& L, hipStream_t st);
+void launch_sesph_accel(const LaunchSet& L, hipStream_t st);
+void launch_integrate(const LaunchSet& L, hipStream_t st);
+
+}  // namespace fsa

@@ -1,0 +1,659 @@
+// fsa_sph.hip — IISPH-3D and SESPH-3D particle updates on gfx950.
+//
+// One thread per fluid particle (fluid rank k -> combined index i), walking
+// its neighbor list in ascending combined index, which is the order libFluid
+// iterates a compressed-search neighbor list.  Accumulators are kept per
+// reference loop and updated in the reference's operand order, so each
+// per-particle result equals the reference-order CPU restatement bit for bit.
+//
+// Reference loops (IISPHFluidSolver3D.hpp):
+//   k_iisph_prepare   phase 1, :141-212 (viscosity :43-76)
+//   k_iisph_diagonal  phase 2, :216-339
+//   k_iisph_pass_a    Jacobi pressure acceleration, :348-397
+//   k_iisph_pass_b    Jacobi pressure update + error, :402-461, and the
+//                     convergence / CFL reductions of :467-522 (FP64 sum)
+//   k_integrate       CFL correction :525-535 + integration :539-561
+// (SESPHFluidSolver3D.hpp): k_sesph_density :64-75, 196-228;
+//   k_sesph_accel :78-91, 231-324; k_cfl_all :93-107; k_integrate :109-129.
+#include "fsa_device.hpp"
+#include "fsa_kernels.hpp"
+
+namespace fsa {
+
+constexpr float kEps = 1.1920928955078125e-07f;  // std::numeric_limits<float>::epsilon()
+
+struct Grad {
+    float x, y, z;
+};
+
+// d = x_i - x_j, squared length and length, kernel gradient of the pair
+__device__ __forceinline__ void pair_geometry(const SceneDev& S, const float4& xi, const float4& xj, float& dx,
+                                              float& dy, float& dz, float& d2, float& len, Grad& g) {
+    dx = xi.x - xj.x;
+    dy = xi.y - xj.y;
+    dz = xi.z - xj.z;
+    d2 = dot3(dx, dy, dz, dx, dy, dz);
+    len = __builtin_sqrtf(d2);
+    kernel_grad(S, dx, dy, dz, len, g.x, g.y, g.z);
+}
+
+__device__ __forceinline__ uint32_t scene_fluid_blocks(const SceneDev& S) { return (S.nf + kBlock - 1) / kBlock; }
+
+// ---- ConstantTimestepGenerator::get_non_cfl_validating_timestep -----------------
+// (ConstantTimestepGenerator.cpp:17-41).  The reference's unqualified sqrt on
+// a float is ::sqrt(double); the product with lambda_a is formed in double.
+__device__ __forceinline__ float cfl_timestep(const SceneDev& S, float max_a, float max_v) {
+    float dta = S.gen_dt, dtv = S.gen_dt;
+    if (max_a > 0.0f) dta = (float)(sqrt((double)(S.particle_size / max_a)) * (double)S.lambda_a);
+    if (max_v > 0.0f) dtv = S.particle_size / max_v * S.lambda_v;
+    const float nt = fminf(dta, dtv);
+    return fmaxf(nt, 0.000001f);
+}
+
+// ---- block reductions with a fixed tree (deterministic) ---------------------------
+__device__ __forceinline__ double wave_sum_d(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+__device__ __forceinline__ uint32_t wave_sum_u(uint32_t v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+__device__ __forceinline__ float wave_max_f(float v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+struct RedLds {
+    double err[kBlock / 64];
+    uint32_t cnt[kBlock / 64];
+    float a2[kBlock / 64];
+    float v2[kBlock / 64];
+};
+
+__device__ __forceinline__ Partial block_reduce(double err, uint32_t cnt, float a2, float v2, RedLds& lds) {
+    err = wave_sum_d(err);
+    cnt = wave_sum_u(cnt);
+    a2 = wave_max_f(a2);
+    v2 = wave_max_f(v2);
+    const uint32_t w = threadIdx.x / 64;
+    if (__lane_id() == 0) {
+        lds.err[w] = err;
+        lds.cnt[w] = cnt;
+        lds.a2[w] = a2;
+        lds.v2[w] = v2;
+    }
+    __syncthreads();
+    Partial p;
+    p.err = 0.0;
+    p.count = 0;
+    p.max_a2 = 0.0f;
+    p.max_v2 = 0.0f;
+    p.pad = 0;
+    for (int k = 0; k < kBlock / 64; k++) {
+        p.err += lds.err[k];
+        p.count += lds.cnt[k];
+        p.max_a2 = fmaxf(p.max_a2, lds.a2[k]);
+        p.max_v2 = fmaxf(p.max_v2, lds.v2[k]);
+    }
+    __syncthreads();
+    return p;
+}
+
+// Reduce partials[0..nparts) in a fixed order inside one block.
+__device__ __forceinline__ Partial reduce_partials(const Partial* parts, uint32_t nparts, RedLds& lds) {
+    double err = 0.0;
+    uint32_t cnt = 0;
+    float a2 = 0.0f, v2 = 0.0f;
+    for (uint32_t b = threadIdx.x; b < nparts; b += kBlock) {
+        const Partial q = parts[b];
+        err += q.err;
+        cnt += q.count;
+        a2 = fmaxf(a2, q.max_a2);
+        v2 = fmaxf(v2, q.max_v2);
+    }
+    return block_reduce(err, cnt, a2, v2, lds);
+}
+
+// last block of a scene (per counter) returns true after all partials are visible
+__device__ __forceinline__ bool last_block(uint32_t* counter, uint32_t nblocks, bool& flag_lds) {
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __threadfence();
+        const uint32_t done = atomicAdd(counter, 1u);
+        flag_lds = done == nblocks - 1;
+        if (flag_lds) *counter = 0;
+    }
+    __syncthreads();
+    if (flag_lds) __threadfence();
+    return flag_lds;
+}
+
+// ---- IISPH phase 1: viscosity, predicted velocity, density -------------------------
+__global__ __launch_bounds__(kBlock) void k_iisph_prepare(const SceneDev* __restrict__ scenes,
+                                                          const BlockRef* __restrict__ map) {
+    const BlockRef br = map[blockIdx.x];
+    const SceneDev& S = scenes[br.scene];
+    const uint32_t k = br.block * kBlock + threadIdx.x;
+    if (k >= S.nf) return;
+    const uint32_t c = S.state->cur;
+    const uint32_t i = S.fluid_idx[k];
+    const float4* __restrict__ pos = S.pos[c];
+    const float4* __restrict__ vel = S.vel[c];
+    const float4 xi = pos[i];
+    const float4 vi = vel[i];
+    float ex = 0.0f, ey = 0.0f, ez = 0.0f;
+    if (S.flags & kFlagExternal) {
+        const float4 e = S.ext[c][i];
+        ex = e.x;
+        ey = e.y;
+        ez = e.z;
+    }
+    const bool sl = (S.flags & kFlagSingleLayer) != 0;
+    float tx = 0.0f, ty = 0.0f, tz = 0.0f, rho = 0.0f;
+    const uint32_t cnt = S.nbr_cnt[k];
+    const uint32_t stride = S.list_stride;
+    const uint32_t* __restrict__ nb = S.nbr + k;
+    for (uint32_t e = 0; e < cnt; e++) {
+        const uint32_t ent = nb[e * stride];
+        const uint32_t j = ent & kIndexMask;
+        const float4 xj = pos[j];
+        const float4 vj = vel[j];
+        float dx, dy, dz, d2, len;
+        Grad g;
+        pair_geometry(S, xi, xj, dx, dy, dz, d2, len, g);
+        // ComputeViscosityAcceleration: densities of the start-of-step snapshot
+        const float nd = vj.w;
+        if (nd != 0.0f) {
+            const float vx = vi.x - vj.x, vy = vi.y - vj.y, vz = vi.z - vj.z;
+            const float t = (xj.w / nd) * (dot3(vx, vy, vz, dx, dy, dz) / (d2 + S.visc_c01));
+            tx = tx + t * g.x;
+            ty = ty + t * g.y;
+            tz = tz + t * g.z;
+        }
+        float m = xj.w;
+        if (sl && (ent & kBoundaryBit)) m = m * S.gamma1;
+        rho = rho + m * kernel_w(S, len);
+    }
+    float nx = ex + 0.0f, ny = ey + (-S.gravity), nz = ez + 0.0f;
+    nx = nx + S.visc2 * tx;
+    ny = ny + S.visc2 * ty;
+    nz = nz + S.visc2 * tz;
+    const float dt = S.dt;
+    S.vstar[i] = make_float4(vi.x + dt * nx, vi.y + dt * ny, vi.z + dt * nz, 0.0f);
+    S.anp[i] = make_float4(nx, ny, nz, 0.0f);
+    S.rho_new[i] = rho;
+    S.pres[c][i] = 0.0f;
+    if (S.flags & kFlagExternal) S.ext[c][i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+}
+
+// phase 1 for boundary particles: no forces of their own, density untouched
+// (IISPHFluidSolver3D.hpp:152-176, 180-182)
+__global__ __launch_bounds__(kBlock) void k_iisph_prepare_boundary(const SceneDev* __restrict__ scenes,
+                                                                   const BlockRef* __restrict__ map) {
+    const BlockRef br = map[blockIdx.x];
+    const SceneDev& S = scenes[br.scene];
+    const uint32_t i = br.block * kBlock + threadIdx.x;
+    if (i >= S.n) return;
+    const uint32_t c = S.state->cur;
+    if ((S.idt[c][i] >> kTypeShift) != 1u) return;
+    float4 e = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (S.flags & kFlagExternal) e = S.ext[c][i];
+    const float4 v = S.vel[c][i];
+    const float dt = S.dt;
+    S.vstar[i] = make_float4(v.x + dt * e.x, v.y + dt * e.y, v.z + dt * e.z, 0.0f);
+    S.anp[i] = make_float4(e.x, e.y, e.z, 0.0f);
+    S.pres[c][i] = 0.0f;
+    if (S.flags & kFlagExternal) S.ext[c][i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+}
+
+// ---- IISPH phase 2: source term and diagonal element -----------------------------
+__global__ __launch_bounds__(kBlock) void k_iisph_diagonal(const SceneDev* __restrict__ scenes,
+                                                           const BlockRef* __restrict__ map) {
+    const BlockRef br = map[blockIdx.x];
+    const SceneDev& S = scenes[br.scene];
+    const uint32_t k = br.block * kBlock + threadIdx.x;
+    if (k >= S.nf) return;
+    const uint32_t c = S.state->cur;
+    const uint32_t i = S.fluid_idx[k];
+    const float4* __restrict__ pos = S.pos[c];
+    const float4 xi = pos[i];
+    const float4 vs = S.vstar[i];
+    const bool sl = (S.flags & kFlagSingleLayer) != 0;
+    const uint32_t cnt = S.nbr_cnt[k];
+    const uint32_t stride = S.list_stride;
+    const uint32_t* __restrict__ nb = S.nbr + k;
+    float contrib = 0.0f;
+    float ix = 0.0f, iy = 0.0f, iz = 0.0f;
+    for (uint32_t e = 0; e < cnt; e++) {
+        const uint32_t ent = nb[e * stride];
+        const uint32_t j = ent & kIndexMask;
+        const float4 xj = pos[j];
+        float dx, dy, dz, d2, len;
+        Grad g;
+        pair_geometry(S, xi, xj, dx, dy, dz, d2, len, g);
+        float f;
+        if (!(ent & kBoundaryBit)) {
+            const float4 vsj = S.vstar[j];
+            contrib = contrib + xj.w * dot3(vs.x - vsj.x, vs.y - vsj.y, vs.z - vsj.z, g.x, g.y, g.z);
+            f = div_rho0sq(S, xj.w);
+        } else {
+            contrib = contrib + xj.w * dot3(vs.x, vs.y, vs.z, g.x, g.y, g.z);
+            f = sl ? div_rho0sq(S, 2.0f * S.gamma * xj.w) : div_rho0sq(S, 2.0f * xj.w);
+        }
+        ix = ix - f * g.x;
+        iy = iy - f * g.y;
+        iz = iz - f * g.z;
+    }
+    const float dt = S.dt;
+    S.src[i] = S.rest_density - S.rho_new[i] - dt * contrib;
+    const float cm = div_rho0sq(S, xi.w);
+    float dg = 0.0f;
+    for (uint32_t e = 0; e < cnt; e++) {
+        const uint32_t ent = nb[e * stride];
+        const uint32_t j = ent & kIndexMask;
+        const float4 xj = pos[j];
+        float dx, dy, dz, d2, len;
+        Grad g;
+        pair_geometry(S, xi, xj, dx, dy, dz, d2, len, g);
+        dg = dg + xj.w * dot3(ix, iy, iz, g.x, g.y, g.z);
+        if (!(ent & kBoundaryBit)) {
+            // (m_i / rho0^2) * gradW(x_i, x_j) with gradW(x_i, x_j) = -gradW(x_j, x_i) exactly
+            dg = dg + xj.w * dot3(-(cm * g.x), -(cm * g.y), -(cm * g.z), g.x, g.y, g.z);
+        }
+    }
+    S.diag[i] = dg * (dt * dt);
+}
+
+// ---- Jacobi loop control -------------------------------------------------------------
+__global__ void k_iisph_begin_loop(const SceneDev* __restrict__ scenes, const BlockRef* __restrict__ map) {
+    const SceneDev& S = scenes[map[blockIdx.x].scene];
+    if (threadIdx.x == 0) {
+        SceneState* st = S.state;
+        st->active = S.max_it > 0 ? 1u : 0u;
+        st->iterations = 0;
+        st->max_a2 = 0.0f;
+        st->max_v2 = 0.0f;
+        st->avg_err = 0.0;
+        st->err_count = 0;
+    }
+}
+
+// ---- Jacobi pass A: pressure acceleration ------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_iisph_pass_a(const SceneDev* __restrict__ scenes,
+                                                         const BlockRef* __restrict__ map) {
+    const BlockRef br = map[blockIdx.x];
+    const SceneDev& S = scenes[br.scene];
+    if (!S.state->active) return;
+    const uint32_t k = br.block * kBlock + threadIdx.x;
+    if (k >= S.nf) return;
+    const uint32_t c = S.state->cur;
+    const uint32_t i = S.fluid_idx[k];
+    const float4* __restrict__ pos = S.pos[c];
+    const float* __restrict__ pres = S.pres[c];
+    const float4 xi = pos[i];
+    const float pi = pres[i];
+    const float pir = div_rho0sq(S, pi);
+    const bool sl = (S.flags & kFlagSingleLayer) != 0;
+    const uint32_t cnt = S.nbr_cnt[k];
+    const uint32_t stride = S.list_stride;
+    const uint32_t* __restrict__ nb = S.nbr + k;
+    float ax = 0.0f, ay = 0.0f, az = 0.0f;
+    for (uint32_t e = 0; e < cnt; e++) {
+        const uint32_t ent = nb[e * stride];
+        const uint32_t j = ent & kIndexMask;
+        const float4 xj = pos[j];
+        float dx, dy, dz, d2, len;
+        Grad g;
+        pair_geometry(S, xi, xj, dx, dy, dz, d2, len, g);
+        float f;
+        if (!(ent & kBoundaryBit)) {
+            f = xj.w * (pir + div_rho0sq(S, pres[j]));
+        } else {
+            f = sl ? div_rho0sq(S, S.gamma * xj.w * 2.0f * pi) : div_rho0sq(S, xj.w * 2.0f * pi);
+        }
+        ax = ax - f * g.x;
+        ay = ay - f * g.y;
+        az = az - f * g.z;
+    }
+    S.acc[c][i] = make_float4(ax, ay, az, 0.0f);
+}
+
+// ---- Jacobi pass B: A*p, relaxed pressure update, error, reductions -------------------
+__global__ __launch_bounds__(kBlock) void k_iisph_pass_b(const SceneDev* __restrict__ scenes,
+                                                         const BlockRef* __restrict__ map) {
+    const BlockRef br = map[blockIdx.x];
+    const SceneDev& S = scenes[br.scene];
+    if (!S.state->active) return;
+    __shared__ RedLds lds;
+    __shared__ bool is_last;
+    const uint32_t k = br.block * kBlock + threadIdx.x;
+    const uint32_t c = S.state->cur;
+    double err_sum = 0.0;
+    uint32_t counted = 0;
+    float a2 = 0.0f, v2 = 0.0f;
+    if (k < S.nf) {
+        const uint32_t i = S.fluid_idx[k];
+        const float4* __restrict__ pos = S.pos[c];
+        const float4* __restrict__ acc = S.acc[c];
+        const float4 xi = pos[i];
+        const float4 ai = acc[i];
+        const uint32_t cnt = S.nbr_cnt[k];
+        const uint32_t stride = S.list_stride;
+        const uint32_t* __restrict__ nb = S.nbr + k;
+        float ap = 0.0f;
+        for (uint32_t e = 0; e < cnt; e++) {
+            const uint32_t ent = nb[e * stride];
+            const uint32_t j = ent & kIndexMask;
+            const float4 xj = pos[j];
+            float dx, dy, dz, d2, len;
+            Grad g;
+            pair_geometry(S, xi, xj, dx, dy, dz, d2, len, g);
+            if (!(ent & kBoundaryBit)) {
+                const float4 aj = acc[j];
+                ap = ap + xj.w * dot3(ai.x - aj.x, ai.y - aj.y, ai.z - aj.z, g.x, g.y, g.z);
+            } else {
+                ap = ap + xj.w * dot3(ai.x, ai.y, ai.z, g.x, g.y, g.z);
+            }
+        }
+        const float dt = S.dt;
+        ap = ap * (dt * dt);
+        const float dg = S.diag[i];
+        const float s = S.src[i];
+        float p = S.pres[c][i];
+        if (fabsf(dg) > kEps)
+            p = fmaxf(0.0f, p + S.omega * ((s - ap) / dg));
+        else
+            p = 0.0f;
+        float e = fabsf(s - ap);
+        if (fabsf(dg) > kEps) {
+            if (fabsf(p) <= kEps) e = 0.0f;
+            S.err[c][i] = e;
+            err_sum = (double)e;
+            counted = 1;
+        }
+        S.pres[c][i] = p;
+        a2 = dot3(ai.x, ai.y, ai.z, ai.x, ai.y, ai.z);
+        const float4 vs = S.vstar[i];
+        const float px = vs.x + dt * ai.x, py = vs.y + dt * ai.y, pz = vs.z + dt * ai.z;
+        v2 = dot3(px, py, pz, px, py, pz);
+    }
+    const Partial part = block_reduce(err_sum, counted, a2, v2, lds);
+    if (threadIdx.x == 0) S.partial[br.block] = part;
+    const uint32_t nblk = scene_fluid_blocks(S);
+    if (!last_block(&S.state->done[1], nblk, is_last)) return;
+    const Partial tot = reduce_partials(S.partial, nblk, lds);
+    if (threadIdx.x == 0) {
+        SceneState* st = S.state;
+        const uint64_t it = st->iterations;
+        st->iterations = (uint32_t)(it + 1);
+        const double avg = tot.count > 0 ? tot.err / (double)tot.count : 0.0;
+        st->avg_err = avg;
+        st->err_count = tot.count;
+        st->max_a2 = tot.max_a2;
+        st->max_v2 = tot.max_v2;
+        const bool reached_min = it >= (uint64_t)S.min_it - 1ull;  // size_t arithmetic as in the reference
+        if ((reached_min && avg <= (double)S.max_err) || it + 1 >= (uint64_t)S.max_it) st->active = 0;
+        __threadfence();
+    }
+}
+
+// ---- SESPH --------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_sesph_density(const SceneDev* __restrict__ scenes,
+                                                          const BlockRef* __restrict__ map) {
+    const BlockRef br = map[blockIdx.x];
+    const SceneDev& S = scenes[br.scene];
+    const uint32_t k = br.block * kBlock + threadIdx.x;
+    if (k >= S.nf) return;
+    const uint32_t c = S.state->cur;
+    const uint32_t i = S.fluid_idx[k];
+    const float4* __restrict__ pos = S.pos[c];
+    const float4 xi = pos[i];
+    const bool sl = (S.flags & kFlagSingleLayer) != 0;
+    const uint32_t cnt = S.nbr_cnt[k];
+    const uint32_t stride = S.list_stride;
+    const uint32_t* __restrict__ nb = S.nbr + k;
+    float rho = 0.0f;
+    for (uint32_t e = 0; e < cnt; e++) {
+        const uint32_t ent = nb[e * stride];
+        const float4 xj = pos[ent & kIndexMask];
+        const float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
+        const float len = __builtin_sqrtf(dot3(dx, dy, dz, dx, dy, dz));
+        if (sl && (ent & kBoundaryBit))
+            rho = rho + S.gamma1 * xj.w * kernel_w(S, len);
+        else
+            rho = rho + xj.w * kernel_w(S, len);
+    }
+    float p = S.stiffness * (rho / S.rest_density - 1.0f);
+    p = p < 0.0f ? 0.0f : p;  // std::max(pressure, 0.0f)
+    float4 v = S.vel[c][i];
+    v.w = rho;
+    S.vel[c][i] = v;
+    S.pres[c][i] = p;
+}
+
+__global__ __launch_bounds__(kBlock) void k_sesph_accel(const SceneDev* __restrict__ scenes,
+                                                        const BlockRef* __restrict__ map) {
+    const BlockRef br = map[blockIdx.x];
+    const SceneDev& S = scenes[br.scene];
+    const uint32_t k = br.block * kBlock + threadIdx.x;
+    if (k >= S.nf) return;
+    const uint32_t c = S.state->cur;
+    const uint32_t i = S.fluid_idx[k];
+    const float4* __restrict__ pos = S.pos[c];
+    const float4* __restrict__ vel = S.vel[c];
+    const float* __restrict__ pres = S.pres[c];
+    const float4 xi = pos[i];
+    const float4 vi = vel[i];
+    const float rho_i = vi.w, p_i = pres[i];
+    const float pdd = rho_i == 0.0f ? 0.0f : p_i / (rho_i * rho_i);
+    const bool sl = (S.flags & kFlagSingleLayer) != 0;
+    const uint32_t cnt = S.nbr_cnt[k];
+    const uint32_t stride = S.list_stride;
+    const uint32_t* __restrict__ nb = S.nbr + k;
+    float tx = 0.0f, ty = 0.0f, tz = 0.0f;
+    float ax = 0.0f, ay = 0.0f, az = 0.0f;
+    for (uint32_t e = 0; e < cnt; e++) {
+        const uint32_t ent = nb[e * stride];
+        const uint32_t j = ent & kIndexMask;
+        const float4 xj = pos[j];
+        const float4 vj = vel[j];
+        float dx, dy, dz, d2, len;
+        Grad g;
+        pair_geometry(S, xi, xj, dx, dy, dz, d2, len, g);
+        const float nd = vj.w;
+        if (nd != 0.0f) {
+            const float vx = vi.x - vj.x, vy = vi.y - vj.y, vz = vi.z - vj.z;
+            const float t = (xj.w / nd) * (dot3(vx, vy, vz, dx, dy, dz) / (d2 + S.visc_c01));
+            tx = tx + t * g.x;
+            ty = ty + t * g.y;
+            tz = tz + t * g.z;
+        }
+        if (ent & kBoundaryBit) {
+            const float f = -xi.w * (pdd + pdd);
+            if (sl) {
+                ax = ax + (f * g.x) * S.gamma2;
+                ay = ay + (f * g.y) * S.gamma2;
+                az = az + (f * g.z) * S.gamma2;
+            } else {
+                ax = ax + f * g.x;
+                ay = ay + f * g.y;
+                az = az + f * g.z;
+            }
+        } else {
+            const float npdd = nd == 0.0f ? 0.0f : pres[j] / (nd * nd);
+            const float f = -xj.w * (pdd + npdd);
+            ax = ax + f * g.x;
+            ay = ay + f * g.y;
+            az = az + f * g.z;
+        }
+    }
+    float nx = 0.0f + 0.0f, ny = 0.0f + (-S.gravity), nz = 0.0f + 0.0f;
+    nx = nx + S.visc2 * tx;
+    ny = ny + S.visc2 * ty;
+    nz = nz + S.visc2 * tz;
+    S.acc[c][i] = make_float4(ax + nx, ay + ny, az + nz, 0.0f);
+}
+
+// SESPH CFL maxima over every active particle, boundary included
+__global__ __launch_bounds__(kBlock) void k_cfl_all(const SceneDev* __restrict__ scenes,
+                                                    const BlockRef* __restrict__ map) {
+    const BlockRef br = map[blockIdx.x];
+    const SceneDev& S = scenes[br.scene];
+    __shared__ RedLds lds;
+    __shared__ bool is_last;
+    const uint32_t i = br.block * kBlock + threadIdx.x;
+    const uint32_t c = S.state->cur;
+    float a2 = 0.0f, v2 = 0.0f;
+    if (i < S.n) {
+        const float4 a = S.acc[c][i];
+        const float4 v = S.vel[c][i];
+        a2 = dot3(a.x, a.y, a.z, a.x, a.y, a.z);
+        const float dt = S.dt;
+        const float px = v.x + dt * a.x, py = v.y + dt * a.y, pz = v.z + dt * a.z;
+        v2 = dot3(px, py, pz, px, py, pz);
+    }
+    const Partial part = block_reduce(0.0, 0u, a2, v2, lds);
+    if (threadIdx.x == 0) S.partial[br.block] = part;
+    if (!last_block(&S.state->done[3], S.nblocks, is_last)) return;
+    const Partial tot = reduce_partials(S.partial, S.nblocks, lds);
+    if (threadIdx.x == 0) {
+        S.state->max_a2 = fmaxf(tot.max_a2, S.inactive_a2);
+        S.state->max_v2 = fmaxf(tot.max_v2, S.inactive_v2);
+        S.state->iterations = 0;
+        S.state->avg_err = 0.0;
+        __threadfence();
+    }
+}
+
+// ---- CFL correction + Euler-Cromer integration ------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_integrate(const SceneDev* __restrict__ scenes,
+                                                      const BlockRef* __restrict__ map) {
+    const BlockRef br = map[blockIdx.x];
+    const SceneDev& S = scenes[br.scene];
+    __shared__ int32_t blo[3][kBlock / 64], bhi[3][kBlock / 64];
+    __shared__ bool is_last;
+    const uint32_t i = br.block * kBlock + threadIdx.x;
+    const uint32_t c = S.state->cur;
+    const float max_v = (float)sqrt((double)S.state->max_v2);
+    const float max_a = (float)sqrt((double)S.state->max_a2);
+    const float corrected = fminf(cfl_timestep(S, max_a, max_v), S.dt);
+    const bool sesph = (S.flags & kFlagSesph) != 0;
+    int32_t lo[3] = {INT32_MAX, INT32_MAX, INT32_MAX}, hi[3] = {INT32_MIN, INT32_MIN, INT32_MIN};
+    if (i < S.n) {
+        const uint32_t type = S.idt[c][i] >> kTypeShift;
+        float4 x = S.pos[c][i];
+        const bool moves = type == 0u || (S.flags & kFlagDynamicBoundary);
+        if (moves) {
+            const float4 a = S.acc[c][i];
+            float4 v = S.vel[c][i];
+            float vx, vy, vz;
+            if (sesph) {
+                vx = v.x + corrected * a.x;
+                vy = v.y + corrected * a.y;
+                vz = v.z + corrected * a.z;
+            } else {
+                float4 pv = S.vstar[i];
+                if (S.dt != corrected) {
+                    const float4 an = S.anp[i];
+                    pv.x = pv.x - S.dt * an.x + corrected * an.x;
+                    pv.y = pv.y - S.dt * an.y + corrected * an.y;
+                    pv.z = pv.z - S.dt * an.z + corrected * an.z;
+                }
+                vx = pv.x + corrected * a.x;
+                vy = pv.y + corrected * a.y;
+                vz = pv.z + corrected * a.z;
+                if (type == 0u) v.w = S.rho_new[i];
+            }
+            x.x = x.x + corrected * vx;
+            x.y = x.y + corrected * vy;
+            x.z = x.z + corrected * vz;
+            S.pos[c][i] = x;
+            S.vel[c][i] = make_float4(vx, vy, vz, v.w);
+        }
+        lo[0] = hi[0] = cell_of(x.x, S.search_radius);
+        lo[1] = hi[1] = cell_of(x.y, S.search_radius);
+        lo[2] = hi[2] = cell_of(x.z, S.search_radius);
+    }
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            lo[a] = min(lo[a], __shfl_xor(lo[a], o, 64));
+            hi[a] = max(hi[a], __shfl_xor(hi[a], o, 64));
+        }
+        if (__lane_id() == 0) {
+            blo[a][threadIdx.x / 64] = lo[a];
+            bhi[a][threadIdx.x / 64] = hi[a];
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        BoxPartial bp;
+        for (int a = 0; a < 3; a++) {
+            int32_t l = INT32_MAX, h = INT32_MIN;
+            for (int w = 0; w < kBlock / 64; w++) {
+                l = min(l, blo[a][w]);
+                h = max(h, bhi[a][w]);
+            }
+            bp.lo[a] = l;
+            bp.hi[a] = h;
+        }
+        bp.lo[3] = 0;
+        bp.hi[3] = 0;
+        S.box[br.block] = bp;
+    }
+    if (!last_block(&S.state->done[2], S.nblocks, is_last)) return;
+    if (threadIdx.x == 0) {
+        int32_t l[3] = {INT32_MAX, INT32_MAX, INT32_MAX}, h[3] = {INT32_MIN, INT32_MIN, INT32_MIN};
+        for (uint32_t b = 0; b < S.nblocks; b++) {
+            const BoxPartial bp = S.box[b];
+            for (int a = 0; a < 3; a++) {
+                l[a] = min(l[a], bp.lo[a]);
+                h[a] = max(h[a], bp.hi[a]);
+            }
+        }
+        SceneState* st = S.state;
+        for (int a = 0; a < 3; a++) {
+            st->bbox_min[a] = l[a];
+            st->bbox_max[a] = h[a];
+        }
+        st->dt_actual = corrected;
+        st->max_v = max_v;
+        st->max_a = max_a;
+        __threadfence();
+    }
+}
+
+// ---- launchers ------------------------------------------------------------------------
+void launch_iisph_prepare(const LaunchSet& L, bool boundary_pass, hipStream_t st) {
+    if (L.nb_fluid) k_iisph_prepare<<<L.nb_fluid, kBlock, 0, st>>>(L.scenes, L.map_fluid);
+    if (boundary_pass && L.nb_all) k_iisph_prepare_boundary<<<L.nb_all, kBlock, 0, st>>>(L.scenes, L.map_all);
+}
+void launch_iisph_diagonal(const LaunchSet& L, hipStream_t st) {
+    if (L.nb_fluid) k_iisph_diagonal<<<L.nb_fluid, kBlock, 0, st>>>(L.scenes, L.map_fluid);
+}
+void launch_iisph_begin_loop(const LaunchSet& L, hipStream_t st) {
+    k_iisph_begin_loop<<<L.nb_scene, 64, 0, st>>>(L.scenes, L.map_scene);
+}
+void launch_iisph_pass_a(const LaunchSet& L, hipStream_t st) {
+    if (L.nb_fluid) k_iisph_pass_a<<<L.nb_fluid, kBlock, 0, st>>>(L.scenes, L.map_fluid);
+}
+void launch_iisph_pass_b(const LaunchSet& L, hipStream_t st) {
+    if (L.nb_fluid) k_iisph_pass_b<<<L.nb_fluid, kBlock, 0, st>>>(L.scenes, L.map_fluid);
+}
+void launch_sesph_density(const LaunchSet& L, hipStream_t st) {
+    if (L.nb_fluid) k_sesph_density<<<L.nb_fluid, kBlock, 0, st>>>(L.scenes, L.map_fluid);
+}
+void launch_sesph_accel(const LaunchSet& L, hipStream_t st) {
+    if (L.nb_fluid) k_sesph_accel<<<L.nb_fluid, kBlock, 0, st>>>(L.scenes, L.map_fluid);
+    k_cfl_all<<<L.nb_all, kBlock, 0, st>>>(L.scenes, L.map_all);
+}
+void launch_integrate(const LaunchSet& L, hipStream_t st) {
+    k_integrate<<<L.nb_all, kBlock, 0, st>>>(L.scenes, L.map_all);
+}
+
+}  // namespace fsa

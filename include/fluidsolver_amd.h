@@ -1,0 +1,240 @@
+/*
+ * fluidsolver_amd.h — C ABI of the MI355X (gfx950) SPH hot path.
+ *
+ * The library replaces libFluid's per-timestep hot path: the neighborhood
+ * search plus the IISPH and SESPH particle updates.  Every entry point below
+ * takes plain pointers and sizes, returns an int32 status (FSA_OK on success)
+ * and never throws or aborts across the ABI.  The text of the last error of
+ * the calling thread is returned by fsa_last_error().
+ *
+ * Reference interfaces replaced (paths relative to the FluidSolver tree):
+ *   IFluidSolverBase                 src/libFluid/fluidSolver/IFluidSolver.hpp:16-34
+ *   NeighborhoodSearch concept       src/libFluid/fluidSolver/IFluidSolver.hpp:36-73
+ *   CompressedNeighborhoodSearch     src/libFluid/fluidSolver/neighborhoodSearch/CompressedNeighbors.hpp:20-180
+ *   IISPHFluidSolver3D               src/libFluid/fluidSolver/solver/IISPHFluidSolver3D.hpp:22-609
+ *   SESPHFluidSolver3D               src/libFluid/fluidSolver/solver/SESPHFluidSolver3D.hpp:12-337
+ *   IISPHFluidSolver (2D)            src/libFluid/fluidSolver/solver/IISPHFluidSolver.hpp:21-548
+ *   SESPHFluidSolver (2D)            src/libFluid/fluidSolver/solver/SESPHFluidSolver.hpp:49-299
+ *   HashedNeighborhoodSearch (2D)    src/libFluid/fluidSolver/neighborhoodSearch/HashedNeighborhoodSearch.cpp:9-161
+ *   ConstantTimestepGenerator        src/libFluid/time/ConstantTimestepGenerator.cpp:17-41
+ *
+ * Particle data crosses the boundary as strided views over the reference's
+ * component records (src/libFluid/fluidSolver/Particle.hpp:15-50), so a
+ * ParticleCollection's component vectors are handed over without repacking:
+ *   MovementData3D {vec3 position, velocity, acceleration}   36 B
+ *   MovementData   {vec2 position, velocity, acceleration}   24 B
+ *   ParticleData   {float mass, pressure, density}           12 B
+ *   ParticleInfo   {uint32 tag; uint8 type}                   8 B (type at offset 4)
+ *   ExternalForces3D / ExternalForces {vec3 / vec2}       12 / 8 B
+ *
+ * Index space: particle indices passed to or returned by the library are the
+ * caller's indices (the row order of the uploaded views).  The device keeps
+ * its own cell-sorted order internally and maps back on every download.
+ *
+ * Threading: a context may be used from several host threads; a scene must
+ * be driven by one host thread at a time.  All device work of a context is
+ * ordered on the context's stream.
+ */
+#ifndef FLUIDSOLVER_AMD_H
+#define FLUIDSOLVER_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FSA_API __attribute__((visibility("default")))
+
+/* ---- status codes ------------------------------------------------------ */
+#define FSA_OK 0
+#define FSA_E_INVALID 1      /* bad argument or inconsistent description          */
+#define FSA_E_NOMEM 2        /* device or host allocation failed                  */
+#define FSA_E_DEVICE 3       /* HIP runtime error                                 */
+#define FSA_E_CAPACITY 4     /* a particle has more neighbors than the list holds */
+#define FSA_E_STATE 5        /* call order violated (e.g. simulate before search) */
+#define FSA_E_BUDGET 6       /* scene does not fit the context's HBM budget       */
+#define FSA_E_NONFINITE 7    /* NaN/Inf detected in the particle state            */
+
+/* ---- enumerations ------------------------------------------------------ */
+#define FSA_SOLVER_IISPH 0   /* IISPHFluidSolver3D / IISPHFluidSolver            */
+#define FSA_SOLVER_SESPH 1   /* SESPHFluidSolver3D / SESPHFluidSolver            */
+
+#define FSA_LIST_COMPRESSED 0 /* delta-coded neighbor lists (default)            */
+#define FSA_LIST_PLAIN 1      /* uncompressed 32-bit neighbor lists               */
+
+#define FSA_TYPE_NORMAL 0    /* ParticleTypeNormal   (Particle.hpp:9)            */
+#define FSA_TYPE_BOUNDARY 1  /* ParticleTypeBoundary (Particle.hpp:10)           */
+#define FSA_TYPE_INACTIVE 2  /* ParticleTypeInactive (Particle.hpp:11)           */
+
+/* fsa_scene_desc.flags */
+#define FSA_FLAG_EMULATE_INT_ABS 1u /* 2D IISPH: reproduce the integer abs() of the
+                                       convergence test (IISPHFluidSolver.hpp:400-402) */
+
+/* Maximum neighbors per particle: first neighbor + 96 deltas, the capacity of
+ * CompressedNeighborhoodSearch::NeighborStorage (CompressedNeighbors.hpp:29-34). */
+#define FSA_MAX_NEIGHBORS 97
+
+/* ---- scene description -------------------------------------------------- */
+/* One struct carries what the reference spreads over
+ * IFluidSolverBase::SimulationParameters (IFluidSolver.hpp:18-23),
+ * IISPHSettings3D (settings/IISPHSettings3D.hpp:6-20), IISPHSettings
+ * (settings/IISPHSettings.hpp:6-15), SESPHSettings3D (settings/SESPHSettings3D.hpp:6-12)
+ * and ConstantTimestepGenerator::ConstantTimestepGeneratorSettings
+ * (time/ConstantTimestepGenerator.hpp:9-12). fsa_scene_desc_defaults() fills
+ * the reference defaults with Simulator's overrides (Simulator.hpp:33-35). */
+typedef struct fsa_scene_desc {
+    uint32_t dimension;      /* 2 or 3 */
+    uint32_t solver;         /* FSA_SOLVER_* */
+    uint32_t neighbor_lists; /* FSA_LIST_* */
+    uint32_t flags;          /* FSA_FLAG_* */
+
+    float rest_density;
+    float gravity; /* applied as (0, -gravity, 0) */
+    float particle_size; /* lattice spacing; kernel support = 2 * particle_size */
+
+    float iisph_max_density_error_allowed;
+    uint32_t iisph_min_iterations;
+    uint32_t iisph_max_iterations;
+    float iisph_omega;
+    float iisph_gamma;
+    float iisph_viscosity;
+    uint32_t iisph_single_layer_boundary;
+    float iisph_single_layer_gamma_1;
+
+    float sesph_stiffness_k;
+    float sesph_viscosity;
+    uint32_t sesph_single_layer_boundary;
+    float sesph_single_layer_gamma_1;
+    float sesph_single_layer_gamma_2;
+
+    float timestep; /* ConstantTimestepGenerator: generated dt */
+    float lambda_v;
+    float lambda_a;
+
+    uint32_t reserved[5];
+} fsa_scene_desc;
+
+/* Strided views over the caller's component arrays. Row i of every view is
+ * particle i. external may be NULL (treated as zero on upload, skipped on
+ * download). For downloads, any NULL pointer is skipped. */
+typedef struct fsa_particle_view {
+    uint64_t count;
+    void* movement;
+    uint64_t movement_stride;
+    void* particle_data;
+    uint64_t particle_data_stride;
+    void* info;
+    uint64_t info_stride;
+    void* external;
+    uint64_t external_stride;
+} fsa_particle_view;
+
+/* Result of one fsa_scene_simulate(): the reference's Timepoint update
+ * (time/Timepoint.hpp:7-14) plus IISPHFluidSolver3D::stat_last_* fields
+ * (IISPHFluidSolver3D.hpp:32-33). */
+typedef struct fsa_step_stats {
+    float desired_time_step;
+    float actual_time_step;
+    uint32_t iterations;           /* IISPH Jacobi iterations; 0 for SESPH */
+    uint32_t status;               /* FSA_OK or the first error raised on device */
+    double average_density_error;  /* FP64 average predicted density error */
+    float max_velocity;            /* sqrt of the CFL velocity maximum */
+    float max_acceleration;        /* sqrt of the CFL acceleration maximum */
+} fsa_step_stats;
+
+typedef struct fsa_context_s* fsa_context;
+typedef struct fsa_scene_s* fsa_scene;
+
+/* ---- library ----------------------------------------------------------- */
+FSA_API int32_t fsa_version(void);
+FSA_API const char* fsa_last_error(void);
+FSA_API void fsa_scene_desc_defaults(fsa_scene_desc* desc, uint32_t dimension, uint32_t solver);
+
+/* ---- context: device, stream, HBM budget --------------------------------- */
+/* hbm_budget_bytes = 0 means "no budget beyond the device". */
+FSA_API int32_t fsa_context_create(int32_t device, uint64_t hbm_budget_bytes, fsa_context* out);
+FSA_API int32_t fsa_context_destroy(fsa_context ctx);
+FSA_API int32_t fsa_context_synchronize(fsa_context ctx);
+/* Bytes of device memory currently held by the context's scenes. */
+FSA_API int32_t fsa_context_memory_in_use(fsa_context ctx, uint64_t* bytes);
+/* Device bytes a scene of this shape would hold (for admission control). */
+FSA_API int32_t fsa_scene_memory_estimate(const fsa_scene_desc* desc, uint64_t n_fluid, uint64_t n_boundary,
+                                          uint64_t* bytes);
+
+/* ---- scene: IFluidSolverBase + ParticleCollection bridge ---------------- */
+/* Creates a scene on the context and uploads the particles of `view`
+ * (Simulator::initialize wiring, Simulator.cpp:59-139). Fails with
+ * FSA_E_BUDGET when the scene would exceed the context's HBM budget. */
+FSA_API int32_t fsa_scene_create(fsa_context ctx, const fsa_scene_desc* desc, const fsa_particle_view* view,
+                                 fsa_scene* out);
+FSA_API int32_t fsa_scene_destroy(fsa_scene scene);
+/* Replaces the particle state (same count and same particle types). */
+FSA_API int32_t fsa_scene_upload(fsa_scene scene, const fsa_particle_view* view);
+/* Writes the current state back into the caller's views (caller order). */
+FSA_API int32_t fsa_scene_download(fsa_scene scene, const fsa_particle_view* view);
+/* Writes IISPHParticleData3D / IISPHParticleData records
+ * (IISPHFluidSolver3D.hpp:13-19; IISPHFluidSolver.hpp:14-18) of every
+ * non-inactive particle into records + i * stride. */
+FSA_API int32_t fsa_scene_download_iisph(fsa_scene scene, void* records, uint64_t stride);
+/* Updates parameters and settings (DataChangeStruct semantics); the particle
+ * counts and the dimension must not change. */
+FSA_API int32_t fsa_scene_set_desc(fsa_scene scene, const fsa_scene_desc* desc);
+
+/* execute_neighborhood_search() — CompressedNeighborhoodSearch::find_neighbors
+ * (CompressedNeighbors.cpp:168-240). Lists are built for fluid particles. */
+FSA_API int32_t fsa_scene_find_neighbors(fsa_scene scene);
+/* get_neighbors(i) (CompressedNeighbors.cpp:242-248): ascending caller
+ * indices of the neighbors of fluid particle `index`, self included. */
+FSA_API int32_t fsa_scene_get_neighbors(fsa_scene scene, uint64_t index, uint64_t* out, uint64_t capacity,
+                                        uint64_t* count);
+/* All fluid lists at once in CSR form over caller indices: row_offsets has
+ * view.count + 1 entries; rows of non-fluid particles are empty. */
+FSA_API int32_t fsa_scene_export_neighbors(fsa_scene scene, uint64_t* row_offsets, uint64_t* columns,
+                                           uint64_t capacity, uint64_t* nnz);
+/* Encoded neighbor-list bytes of the last search (reference codec size model
+ * and the device's own), for roofline accounting. */
+FSA_API int32_t fsa_scene_neighbor_stats(fsa_scene scene, uint64_t* total_neighbors, uint64_t* device_list_bytes,
+                                         uint64_t* reference_codec_bytes);
+
+/* execute_simulation_step(Timepoint&) (IISPHFluidSolver3D.hpp:120-562;
+ * SESPHFluidSolver3D.hpp:49-130; 2D: IISPHFluidSolver.hpp:139-180,
+ * SESPHFluidSolver.hpp:49-105). Requires a prior fsa_scene_find_neighbors. */
+FSA_API int32_t fsa_scene_simulate(fsa_scene scene, float desired_time_step, fsa_step_stats* stats);
+/* `steps` x (find_neighbors + simulate), as Simulator::execute_simulation_step
+ * does (Simulator.cpp:7-57) without entities/sensors; desired dt from the
+ * scene's timestep setting. stats (may be NULL) receives the last step. */
+FSA_API int32_t fsa_scene_step(fsa_scene scene, uint32_t steps, fsa_step_stats* stats);
+
+/* ---- batching: many scenes advanced together ------------------------------ */
+/* Advances `n` scenes of one context by `steps` steps each. All scenes are
+ * advanced in the same kernel launches over a segmented layout; every scene
+ * keeps its own Jacobi convergence, CFL dt and statistics, and its results
+ * are identical to stepping it alone. stats (may be NULL) holds n entries. */
+FSA_API int32_t fsa_batch_step(fsa_context ctx, const fsa_scene* scenes, uint32_t n, uint32_t steps,
+                               fsa_step_stats* stats);
+/* Captures one batched step of these scenes into a HIP graph that later
+ * fsa_batch_step calls with the same scene list replay (0 = off). */
+FSA_API int32_t fsa_batch_set_graph(fsa_context ctx, uint32_t enabled);
+/* Launch shape override for the per-particle SPH kernels (threads per block:
+ * 64, 128, 256, 512 or 1024); 0 restores the built-in default. */
+FSA_API int32_t fsa_context_set_block_size(fsa_context ctx, uint32_t threads);
+
+/* ---- timing -------------------------------------------------------------- */
+/* Average device time (ms) of each kernel class since the last reset, measured
+ * with HIP events on the context stream when timing is enabled. */
+#define FSA_TIMER_SEARCH 0
+#define FSA_TIMER_DENSITY 1   /* IISPH phase 1 / SESPH density+pressure */
+#define FSA_TIMER_DIAGONAL 2  /* IISPH phase 2 / SESPH accelerations     */
+#define FSA_TIMER_PRESSURE_A 3
+#define FSA_TIMER_PRESSURE_B 4
+#define FSA_TIMER_INTEGRATE 5
+#define FSA_TIMER_COUNT 6
+FSA_API int32_t fsa_context_set_timing(fsa_context ctx, uint32_t enabled);
+FSA_API int32_t fsa_context_get_timing(fsa_context ctx, double* total_ms, uint64_t* launches, uint32_t n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FLUIDSOLVER_AMD_H */

@@ -1,0 +1,1269 @@
+// sph_oracle.cpp — TEST INFRASTRUCTURE ONLY.
+//
+// CPU restatement of libFluid's per-timestep hot path (neighborhood search +
+// IISPH/SESPH particle updates), written from the reference's source text.
+// It is the parity oracle for the HIP library and the timed CPU baseline of
+// bench.py.  Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline
+// leg may load it, and only as the checker; the product path never does.
+//
+// Two arithmetic modes share one template:
+//   * T = float : reference evaluation order (glm: dot = (x*x'+y*y')+z*z',
+//                 length = sqrt(dot), no FMA contraction: built with
+//                 -ffp-contract=off), FP32 serial convergence sums in index
+//                 order as IISPHFluidSolver3D.hpp:467-509 does.
+//   * T = double: the same algorithm in FP64, the accuracy reference.
+//
+// Build decisions where the reference is nondeterministic (documented in the
+// design notes, applied identically on the GPU):
+//   * viscosity reads neighbor densities from a start-of-step snapshot
+//     (the reference reads them while the same loop writes them:
+//     IISPHFluidSolver3D.hpp:61 vs 210; IISPHFluidSolver.hpp:160-167, 261);
+//   * the 2D IISPH density-error sum is a serial sum (the reference sums it
+//     from several threads unsynchronised: IISPHFluidSolver.hpp:508-509).
+//
+// Pinning of every function (tests/test_oracle_*.py): the reference's own
+// known answers (NeighborhoodSearchTests.cpp:98-366, NeighborhoodStorageTests.cpp,
+// PCQuickSortStableTest.cpp), brute-force neighbor sets, kernel closed forms
+// and quadrature normalisation, finite-difference gradients, momentum
+// conservation of the pair forces, free fall and FP32-vs-FP64 agreement.
+#include "../include/fluidsolver_amd.h"
+
+#include <algorithm>
+#include <array>
+#include <bitset>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <limits>
+#include <list>
+#include <map>
+#include <numeric>
+#include <stdexcept>
+#include <string>
+#include <thread>
+#include <vector>
+
+namespace orc {
+
+static thread_local std::string g_err;
+static int g_threads = 0;
+
+static int thread_count() {
+    if (g_threads > 0) return g_threads;
+    unsigned h = std::thread::hardware_concurrency();
+    return h ? (int)h : 1;
+}
+
+// StdParallelForEach::loop_for (parallelization/StdParallelForEach.cpp:64-84):
+// every index is processed independently, so the split does not change results.
+template <class F>
+static void loop_for(size_t n, F&& fn) {
+    int nt = thread_count();
+    if (nt <= 1 || n < 4096) {
+        for (size_t i = 0; i < n; i++) fn(i);
+        return;
+    }
+    std::vector<std::thread> pool;
+    size_t chunk = (n + nt - 1) / nt;
+    for (int t = 0; t < nt; t++) {
+        size_t b = t * chunk, e = std::min(n, b + chunk);
+        if (b >= e) break;
+        pool.emplace_back([&fn, b, e]() {
+            for (size_t i = b; i < e; i++) fn(i);
+        });
+    }
+    for (auto& th : pool) th.join();
+}
+
+// ---------------------------------------------------------------------------
+// glm-order small vectors
+template <class T, int D>
+struct Vec {
+    T v[D];
+};
+template <class T, int D>
+static inline Vec<T, D> vzero() {
+    Vec<T, D> r;
+    for (int k = 0; k < D; k++) r.v[k] = T(0);
+    return r;
+}
+template <class T, int D>
+static inline Vec<T, D> operator+(const Vec<T, D>& a, const Vec<T, D>& b) {
+    Vec<T, D> r;
+    for (int k = 0; k < D; k++) r.v[k] = a.v[k] + b.v[k];
+    return r;
+}
+template <class T, int D>
+static inline Vec<T, D> operator-(const Vec<T, D>& a, const Vec<T, D>& b) {
+    Vec<T, D> r;
+    for (int k = 0; k < D; k++) r.v[k] = a.v[k] - b.v[k];
+    return r;
+}
+template <class T, int D>
+static inline Vec<T, D> operator*(T s, const Vec<T, D>& a) {
+    Vec<T, D> r;
+    for (int k = 0; k < D; k++) r.v[k] = s * a.v[k];
+    return r;
+}
+template <class T, int D>
+static inline Vec<T, D> operator*(const Vec<T, D>& a, T s) {
+    Vec<T, D> r;
+    for (int k = 0; k < D; k++) r.v[k] = a.v[k] * s;
+    return r;
+}
+template <class T, int D>
+static inline Vec<T, D> operator/(const Vec<T, D>& a, T s) {
+    Vec<T, D> r;
+    for (int k = 0; k < D; k++) r.v[k] = a.v[k] / s;
+    return r;
+}
+template <class T, int D>
+static inline Vec<T, D> operator-(const Vec<T, D>& a) {
+    Vec<T, D> r;
+    for (int k = 0; k < D; k++) r.v[k] = -a.v[k];
+    return r;
+}
+// glm::dot: ((x*x') + (y*y')) + (z*z')
+template <class T, int D>
+static inline T dot(const Vec<T, D>& a, const Vec<T, D>& b) {
+    T s = a.v[0] * b.v[0];
+    for (int k = 1; k < D; k++) s = s + a.v[k] * b.v[k];
+    return s;
+}
+template <class T, int D>
+static inline T length(const Vec<T, D>& a) {
+    return std::sqrt(dot(a, a));
+}
+
+template <class T>
+struct Consts;
+template <>
+struct Consts<float> {
+    static constexpr float PI = 3.14159265358979323846f;
+    static float eps() { return std::numeric_limits<float>::epsilon(); }
+};
+template <>
+struct Consts<double> {
+    static constexpr double PI = 3.14159265358979323846;
+    // the FP64 mode keeps the reference's FP32 thresholds
+    static double eps() { return (double)std::numeric_limits<float>::epsilon(); }
+};
+
+// ---------------------------------------------------------------------------
+// CubicSplineKernel3D (kernel/CubicSplineKernel3D.cpp:7-71) and
+// CubicSplineKernel (kernel/CubicSplineKernel.cpp:9-80).
+template <class T, int D>
+struct CubicSpline {
+    T h = 0, alpha = 0;
+    void initialize(T kernel_support) {
+        h = kernel_support / T(2);
+        if (D == 3)
+            alpha = T(1) / (T(4) * Consts<T>::PI * (h * h * h));
+        else
+            alpha = T(5) / (T(14) * Consts<T>::PI * (h * h));
+    }
+    T value(const Vec<T, D>& position) const {
+        T q = length(position) / h;
+        T ret = T(0);
+        if (q < T(1)) {
+            T a = T(2) - q, b = T(1) - q;
+            ret = a * a * a - T(4) * (b * b * b);
+        } else if (q < T(2)) {
+            T a = T(2) - q;
+            ret = a * a * a;
+        }
+        return alpha * ret;
+    }
+    // GetKernelDerivativeValue(position)
+    Vec<T, D> derivative(const Vec<T, D>& position) const {
+        T len = length(position);
+        T q = len / h;
+        Vec<T, D> pre = vzero<T, D>();
+        if (len > Consts<T>::eps()) pre = (h * position) / len;
+        Vec<T, D> ret = vzero<T, D>();
+        if (q < T(1)) {
+            T a = T(2) - q, b = T(1) - q;
+            ret = pre * (T(-3) * (a * a) + T(12) * (b * b));
+        } else if (q < T(2)) {
+            T a = T(2) - q;
+            ret = pre * (T(-3) * (a * a));
+        }
+        return (alpha * ret) * T(-1);
+    }
+    // GetKernelValue(neighborPosition, position) = value(position - neighborPosition)
+    T value(const Vec<T, D>& neighbor, const Vec<T, D>& position) const { return value(position - neighbor); }
+    // GetKernelDerivativeReversedValue(neighborPosition, position)
+    Vec<T, D> reversed(const Vec<T, D>& neighbor, const Vec<T, D>& position) const {
+        return derivative(position - neighbor) * T(-1);
+    }
+};
+
+// ---------------------------------------------------------------------------
+// libmorton morton3D_64_encode as used at CompressedNeighbors.cpp:52-54: x
+// bits land at 3k (22 bits kept), y at 3k+1 and z at 3k+2 (21 bits kept);
+// higher coordinate bits fall off the 64-bit key.
+static inline uint64_t morton3_64(uint32_t x, uint32_t y, uint32_t z) {
+    uint64_t key = 0;
+    for (int b = 0; b < 22; b++) {
+        key |= (uint64_t)((x >> b) & 1u) << (3 * b);
+        if (3 * b + 1 < 64) key |= (uint64_t)((y >> b) & 1u) << (3 * b + 1);
+        if (3 * b + 2 < 64) key |= (uint64_t)((z >> b) & 1u) << (3 * b + 2);
+    }
+    return key;
+}
+static inline uint32_t to_unsigned(int32_t v) {
+    int64_t v2 = (int64_t)v;
+    v2 -= std::numeric_limits<int32_t>::min();
+    return (uint32_t)v2;
+}
+
+// CompressedNeighborhoodSearch::NeighborStorage (CompressedNeighbors.hpp:26-60,
+// CompressedNeighbors.cpp:260-379): first neighbor, 2-bit control per delta
+// (00 -> 1, 01 -> 2, 10 -> one byte d-3, 11 -> four bytes LE d-259).
+struct NeighborStorage {
+    static constexpr size_t MAX_DELTAS = 96, DELTAS_SIZE = 96, MAX_CONTROLS = 192;
+    uint64_t first_neighbor = (uint64_t)-1;
+    std::bitset<MAX_CONTROLS> control;
+    uint8_t deltas[DELTAS_SIZE] = {};
+    size_t size_value = 0, bytes = 0;
+    bool overflow = false;
+
+    void clear() {
+        control.reset();
+        bytes = 0;
+        size_value = 0;
+        first_neighbor = (uint64_t)-1;
+        std::memset(deltas, 0, sizeof(deltas));
+        overflow = false;
+    }
+    void set_first(uint64_t index) {
+        first_neighbor = index;
+        size_value = 1;
+        bytes = 0;
+    }
+    // returns false where the reference would assert (capacity exceeded)
+    bool set_next(uint64_t delta) {
+        if (size_value - 1 >= MAX_DELTAS || delta == 0) {
+            overflow = true;
+            return false;
+        }
+        size_t bp = size_value - 1;
+        if (delta == 1) {
+            control.set(bp * 2 + 0, false);
+            control.set(bp * 2 + 1, false);
+        } else if (delta == 2) {
+            control.set(bp * 2 + 0, false);
+            control.set(bp * 2 + 1, true);
+        } else if (delta <= 256 + 2) {
+            if (bytes >= DELTAS_SIZE) {
+                overflow = true;
+                return false;
+            }
+            control.set(bp * 2 + 0, true);
+            control.set(bp * 2 + 1, false);
+            deltas[bytes] = (uint8_t)(delta - 3);
+            bytes++;
+        } else {
+            if (bytes + 3 >= DELTAS_SIZE) {
+                overflow = true;
+                return false;
+            }
+            control.set(bp * 2 + 0, true);
+            control.set(bp * 2 + 1, true);
+            uint32_t r = (uint32_t)(delta - (256 + 3));
+            deltas[bytes + 0] = r & 0xff;
+            deltas[bytes + 1] = (r >> 8) & 0xff;
+            deltas[bytes + 2] = (r >> 16) & 0xff;
+            deltas[bytes + 3] = (r >> 24) & 0xff;
+            bytes += 4;
+        }
+        size_value++;
+        return true;
+    }
+    size_t size() const { return size_value; }
+    // get_delta rescans the controls before k to find the payload offset, as
+    // the reference does (CompressedNeighbors.cpp:335-379).
+    uint32_t get_delta(size_t k) const {
+        if (!control[k * 2 + 0]) return control[k * 2 + 1] ? 2u : 1u;
+        size_t start = 0;
+        for (size_t i = 0; i < k; i++) {
+            if (!control[i * 2 + 0]) continue;
+            start += control[i * 2 + 1] ? 4 : 1;
+        }
+        if (!control[k * 2 + 1]) return (uint32_t)deltas[start] + 3u;
+        uint32_t v = (uint32_t)deltas[start + 3] << 24 | (uint32_t)deltas[start + 2] << 16 |
+                     (uint32_t)deltas[start + 1] << 8 | (uint32_t)deltas[start + 0];
+        return v + 3u + 256u;
+    }
+    // encoded size model: 4 B first index + 2-bit controls + payload bytes
+    size_t encoded_bytes() const {
+        if (size_value == 0) return 0;
+        return 4 + (2 * (size_value - 1) + 7) / 8 + bytes;
+    }
+};
+
+enum SearchKind { SEARCH_COMPRESSED3D = 0, SEARCH_QUADRATIC3D = 1, SEARCH_HASHED2D = 2, SEARCH_QUADRATIC2D = 3 };
+enum { T_NORMAL = 0, T_BOUNDARY = 1, T_INACTIVE = 2 };
+
+struct SceneBase {
+    virtual ~SceneBase() {}
+    virtual void translate(const float* delta) = 0;
+    virtual int find_neighbors() = 0;
+    virtual int simulate(float dt, fsa_step_stats* st) = 0;
+    virtual int download(const fsa_particle_view* v) = 0;
+    virtual int download_iisph(void* rec, uint64_t stride) = 0;
+    virtual int export_neighbors(uint64_t* off, uint64_t* cols, uint64_t cap, uint64_t* nnz) = 0;
+    virtual void codec_stats(uint64_t* total, uint64_t* bytes) = 0;
+};
+
+template <class T, int D>
+struct Scene : SceneBase {
+    using V = Vec<T, D>;
+    fsa_scene_desc desc;
+    int search;
+    size_t n = 0;
+    // ParticleCollection components (Particle.hpp:15-50) + stable id
+    std::vector<V> pos, vel, acc, ext;
+    std::vector<T> mass, pressure, density, density_old;
+    std::vector<uint32_t> tag;
+    std::vector<uint8_t> type;
+    std::vector<uint64_t> id;
+    // IISPHParticleData3D (IISPHFluidSolver3D.hpp:13-19) / IISPHParticleData
+    std::vector<V> vstar, anp;
+    std::vector<T> source, diag, perr;
+    // neighbor structures
+    std::vector<NeighborStorage> storage;          // compressed
+    std::vector<uint64_t> cell_key;               // compressed: ParticleInformation.cell_index
+    std::vector<std::pair<uint64_t, size_t>> runs; // cell_to_particle_map
+    std::vector<std::vector<uint64_t>> lists;     // quadratic / hashed lists
+    // hashed 2D grid state (HashedNeighborhoodSearch.cpp:117-151)
+    std::map<std::pair<int32_t, int32_t>, std::list<uint64_t>> grid;
+    std::vector<std::pair<int32_t, int32_t>> current_status;
+    bool searched = false;
+    CubicSpline<T, D> kernel;
+    T search_radius = 0;
+    // false: the reference's FP32 serial sum; true: FP64 sum (the device's
+    // convergence reduction), in the same index order
+    bool fp64_reduction = false;
+
+    // neighbor iteration in the reference's order
+    template <class F>
+    void for_neighbors(size_t i, F&& fn) const {
+        if (search == SEARCH_COMPRESSED3D) {
+            const NeighborStorage& s = storage[i];
+            if (s.size() == 0) return;
+            uint64_t cur = s.first_neighbor;
+            fn((size_t)cur);
+            for (size_t c = 1; c < s.size(); c++) {
+                cur += s.get_delta(c - 1);
+                fn((size_t)cur);
+            }
+        } else {
+            for (uint64_t j : lists[i]) fn((size_t)j);
+        }
+    }
+
+    int load(const fsa_scene_desc& d, int search_kind, const fsa_particle_view* v) {
+        desc = d;
+        search = search_kind;
+        n = v->count;
+        pos.assign(n, vzero<T, D>());
+        vel = pos;
+        acc = pos;
+        ext = pos;
+        vstar = pos;
+        anp = pos;
+        mass.assign(n, 0);
+        pressure = mass;
+        density = mass;
+        density_old = mass;
+        source = mass;
+        diag = mass;
+        perr = mass;
+        tag.assign(n, 0);
+        type.assign(n, 0);
+        id.resize(n);
+        for (size_t i = 0; i < n; i++) {
+            const float* mv = (const float*)((const char*)v->movement + i * v->movement_stride);
+            for (int k = 0; k < D; k++) {
+                pos[i].v[k] = (T)mv[k];
+                vel[i].v[k] = (T)mv[D + k];
+                acc[i].v[k] = (T)mv[2 * D + k];
+            }
+            const float* pd = (const float*)((const char*)v->particle_data + i * v->particle_data_stride);
+            mass[i] = pd[0];
+            pressure[i] = pd[1];
+            density[i] = pd[2];
+            const char* inf = (const char*)v->info + i * v->info_stride;
+            std::memcpy(&tag[i], inf, 4);
+            type[i] = (uint8_t)inf[4];
+            if (v->external) {
+                const float* e = (const float*)((const char*)v->external + i * v->external_stride);
+                for (int k = 0; k < D; k++) ext[i].v[k] = (T)e[k];
+            }
+            id[i] = i;
+        }
+        search_radius = (T)desc.particle_size * T(2);  // Math::kernel_support_factor
+        kernel.initialize((T)desc.particle_size * T(2));
+        return FSA_OK;
+    }
+
+    // apply permutation (new position k takes old index perm[k]) to every component
+    template <class A>
+    static void permute(std::vector<A>& a, const std::vector<size_t>& perm) {
+        std::vector<A> b(a.size());
+        for (size_t k = 0; k < perm.size(); k++) b[k] = a[perm[k]];
+        a.swap(b);
+    }
+    void permute_all(const std::vector<size_t>& perm) {
+        permute(pos, perm);
+        permute(vel, perm);
+        permute(acc, perm);
+        permute(ext, perm);
+        permute(mass, perm);
+        permute(pressure, perm);
+        permute(density, perm);
+        permute(tag, perm);
+        permute(type, perm);
+        permute(id, perm);
+        permute(vstar, perm);
+        permute(anp, perm);
+        permute(source, perm);
+        permute(diag, perm);
+        permute(perr, perm);
+        permute(cell_key, perm);
+        permute(storage, perm);
+    }
+
+    // ---- CompressedNeighborhoodSearch::find_neighbors (CompressedNeighbors.cpp:168-240)
+    std::array<int32_t, 3> cell_of(const V& p) const {
+        std::array<int32_t, 3> c{0, 0, 0};
+        for (int k = 0; k < D; k++) c[k] = (int32_t)std::floor(p.v[k] / search_radius);
+        return c;
+    }
+    static uint64_t key_of(const std::array<int32_t, 3>& c) {
+        return morton3_64(to_unsigned(c[0]), to_unsigned(c[1]), to_unsigned(c[2]));
+    }
+    size_t run_lookup(uint64_t key) const {
+        // exact-match binary search over the run table (CompressedNeighbors.cpp:58-81)
+        size_t lo = 0, hi = runs.size();
+        while (lo < hi) {
+            size_t mid = lo + (hi - lo) / 2;
+            if (runs[mid].first > key)
+                hi = mid;
+            else if (runs[mid].first < key)
+                lo = mid + 1;
+            else
+                return runs[mid].second;
+        }
+        return (size_t)-1;
+    }
+    int search_compressed() {
+        cell_key.resize(n);
+        storage.resize(n);
+        loop_for(n, [&](size_t i) {
+            cell_key[i] = type[i] == T_INACTIVE ? (uint64_t)-1 : key_of(cell_of(pos[i]));
+        });
+        std::vector<size_t> perm(n);
+        std::iota(perm.begin(), perm.end(), 0);
+        std::stable_sort(perm.begin(), perm.end(), [&](size_t a, size_t b) { return cell_key[a] < cell_key[b]; });
+        permute_all(perm);
+        runs.clear();
+        for (size_t i = 0; i < n; i++) {
+            if (type[i] == T_INACTIVE) break;
+            if (i == 0 || cell_key[i - 1] != cell_key[i]) runs.push_back({cell_key[i], i});
+        }
+        T r2 = search_radius * search_radius;  // Math::pow2(search_radius)
+        std::vector<int> bad(n, 0);
+        loop_for(n, [&](size_t i) {
+            if (type[i] == T_INACTIVE) return;
+            NeighborStorage& st = storage[i];
+            auto c = cell_of(pos[i]);
+            std::array<uint64_t, 27> keys;
+            int q = 0;
+            for (int x = -1; x <= 1; x++)
+                for (int y = -1; y <= 1; y++)
+                    for (int z = -1; z <= 1; z++) keys[q++] = key_of({c[0] + x, c[1] + y, c[2] + z});
+            std::sort(keys.begin(), keys.end());
+            st.clear();
+            size_t last = (size_t)-1;
+            for (int k = 0; k < 27; k++) {
+                size_t cur = run_lookup(keys[k]);
+                if (cur == (size_t)-1) continue;
+                while (keys[k] == cell_key[cur]) {
+                    V diff = pos[cur] - pos[i];
+                    if (dot(diff, diff) <= r2) {
+                        if (last == (size_t)-1) {
+                            last = cur;
+                            st.set_first(last);
+                        } else {
+                            if (!st.set_next(cur - last)) bad[i] = 1;
+                            last = cur;
+                        }
+                    }
+                    cur++;
+                    if (cur >= n) break;
+                }
+            }
+        });
+        for (size_t i = 0; i < n; i++)
+            if (bad[i]) {
+                g_err = "neighbor capacity exceeded (more than 97 neighbors or 96 delta bytes)";
+                return FSA_E_CAPACITY;
+            }
+        return FSA_OK;
+    }
+
+    // ---- QuadraticNeighborhoodSearch3D::find_neighbors (QuadraticNeighborhoodSearch3D.cpp:32-69)
+    int search_quadratic3d() {
+        lists.assign(n, {});
+        T r2 = search_radius * search_radius;
+        loop_for(n, [&](size_t i) {
+            if (type[i] == T_INACTIVE) return;
+            for (size_t j = 0; j < n; j++) {
+                if (type[j] == T_INACTIVE) continue;
+                V d = pos[i] - pos[j];
+                if (dot(d, d) <= r2) lists[i].push_back(j);
+            }
+        });
+        return FSA_OK;
+    }
+
+    // ---- HashedNeighborhoodSearch::find_neighbors (HashedNeighborhoodSearch.cpp:9-151)
+    std::pair<int32_t, int32_t> cell2(const V& p) const {
+        T cs = search_radius;  // grid_cell_size = search_radius (initialize, :153-157)
+        return {(int32_t)(p.v[0] / cs), (int32_t)(p.v[1] / cs)};
+    }
+    int search_hashed2d() {
+        if (current_status.size() < n) current_status.resize(n, {INT32_MAX, INT32_MAX});
+        lists.resize(n);
+        uint16_t count = (uint16_t)n;  // particleAmount_t truncation (:129)
+        for (uint64_t i = 0; i < count; i++) {
+            auto key = cell2(pos[i]);
+            if (current_status[i] == key) continue;
+            grid[current_status[i]].remove(i);
+            current_status[i] = key;
+            grid[key].push_back(i);
+        }
+        loop_for(n, [&](size_t i) {
+            if (type[i] == T_BOUNDARY || type[i] == T_INACTIVE) return;
+            auto c = cell2(pos[i]);
+            int32_t rc = (int32_t)std::ceil(search_radius / search_radius);
+            lists[i].clear();
+            for (int32_t x = c.first - rc; x <= c.first + rc; x++)
+                for (int32_t y = c.second - rc; y <= c.second + rc; y++) {
+                    auto it = grid.find({x, y});
+                    if (it == grid.end()) continue;
+                    for (uint64_t j : it->second) {
+                        V d = pos[i] - pos[j];
+                        if (length(d) <= search_radius) lists[i].push_back(j);
+                    }
+                }
+        });
+        return FSA_OK;
+    }
+    // ---- QuadraticNeighborhoodSearchDynamicAllocated (QuadraticNeighborhoodSearchDynamicAllocated.cpp:11-38)
+    int search_quadratic2d() {
+        lists.assign(n, {});
+        loop_for(n, [&](size_t i) {
+            for (size_t j = 0; j < n; j++) {
+                V d = pos[i] - pos[j];
+                if (length(d) <= search_radius) lists[i].push_back(j);
+            }
+        });
+        return FSA_OK;
+    }
+
+    void translate(const float* delta) override {
+        for (size_t i = 0; i < n; i++)
+            for (int k = 0; k < D; k++) pos[i].v[k] = pos[i].v[k] + (T)delta[k];
+    }
+
+    int find_neighbors() override {
+        int rc;
+        if (search == SEARCH_COMPRESSED3D)
+            rc = search_compressed();
+        else if (search == SEARCH_QUADRATIC3D)
+            rc = search_quadratic3d();
+        else if (search == SEARCH_HASHED2D)
+            rc = search_hashed2d();
+        else
+            rc = search_quadratic2d();
+        searched = rc == FSA_OK;
+        return rc;
+    }
+
+    // ---- ConstantTimestepGenerator::get_non_cfl_validating_timestep (ConstantTimestepGenerator.cpp:17-41)
+    float cfl_timestep(float max_a, float max_v) const {
+        float dta = desc.timestep, dtv = desc.timestep;
+        // unqualified sqrt on a float resolves to ::sqrt(double); the product
+        // with the float lambda is formed in double and stored to float
+        if (max_a > 0.0f) dta = (float)(::sqrt((double)(desc.particle_size / max_a)) * desc.lambda_a);
+        if (max_v > 0.0f) dtv = desc.particle_size / max_v * desc.lambda_v;
+        float nt = std::fmin(dta, dtv);
+        return std::fmax(nt, 0.000001f);  // MIN_ALLOWED_TIMESTEP (TimestepGenerator.hpp:18)
+    }
+
+    V gravity_vec() const {
+        V g = vzero<T, D>();
+        g.v[1] = -(T)desc.gravity;
+        return g;
+    }
+
+    // IISPHFluidSolver3D::ComputeViscosityAcceleration (IISPHFluidSolver3D.hpp:43-76),
+    // SESPHFluidSolver3D::ComputeViscosityAcceleration (SESPHFluidSolver3D.hpp:293-324)
+    // and the 2D twins; `rho` is the density array the neighbors are read from.
+    V viscosity(size_t i, T nu, const std::vector<T>& rho) const {
+        V tmp = vzero<T, D>();
+        const V& x = pos[i];
+        const V& v = vel[i];
+        T ps = (T)desc.particle_size;
+        for_neighbors(i, [&](size_t j) {
+            if (type[j] == T_INACTIVE) return;
+            T nm = mass[j], nd = rho[j];
+            if (nd == T(0)) return;
+            V vij = v - vel[j];
+            V xij = x - pos[j];
+            tmp = tmp + ((nm / nd) * (dot(vij, xij) / (dot(xij, xij) + T(0.01f) * ps * ps))) *
+                            kernel.reversed(pos[j], x);
+        });
+        return (T(2) * nu) * tmp;
+    }
+
+    // ---- IISPHFluidSolver3D::execute_simulation_step (IISPHFluidSolver3D.hpp:120-562)
+    int simulate_iisph3d(float desired, fsa_step_stats* st) {
+        T dt = (T)desired;
+        const T rho0 = (T)desc.rest_density;
+        const T rho0sq = rho0 * rho0;
+        const bool sl = desc.iisph_single_layer_boundary != 0;
+        const T gamma = (T)desc.iisph_gamma, g1 = (T)desc.iisph_single_layer_gamma_1;
+        const T eps = Consts<T>::eps();
+        density_old = density;
+        loop_for(n, [&](size_t i) {
+            uint8_t t = type[i];
+            if (t == T_INACTIVE) return;
+            pressure[i] = T(0);
+            V npa = ext[i];
+            if (t != T_BOUNDARY) {
+                npa = npa + gravity_vec();
+                npa = npa + viscosity(i, (T)desc.iisph_viscosity, density_old);
+            }
+            vstar[i] = vel[i] + dt * npa;
+            anp[i] = npa;
+            ext[i] = vzero<T, D>();
+            if (t == T_BOUNDARY) return;
+            T rho = T(0);
+            for_neighbors(i, [&](size_t j) {
+                uint8_t tj = type[j];
+                if (tj == T_INACTIVE) return;
+                T m = mass[j];
+                if (sl && tj == T_BOUNDARY) m = m * g1;
+                rho = rho + m * kernel.value(pos[j], pos[i]);
+            });
+            density[i] = rho;
+        });
+        loop_for(n, [&](size_t i) {
+            if (type[i] != T_NORMAL) return;
+            const V& xi = pos[i];
+            T contrib = T(0);
+            for_neighbors(i, [&](size_t j) {
+                uint8_t tj = type[j];
+                if (tj == T_NORMAL)
+                    contrib = contrib + mass[j] * dot(vstar[i] - vstar[j], kernel.reversed(pos[j], xi));
+                else if (tj == T_BOUNDARY)
+                    contrib = contrib + mass[j] * dot(vstar[i], kernel.reversed(pos[j], xi));
+            });
+            source[i] = rho0 - density[i] - dt * contrib;
+            V inner = vzero<T, D>();
+            for_neighbors(i, [&](size_t j) {
+                uint8_t tj = type[j];
+                if (tj == T_NORMAL)
+                    inner = inner - (mass[j] / rho0sq) * kernel.reversed(pos[j], xi);
+                else if (tj == T_BOUNDARY) {
+                    if (sl)
+                        inner = inner - (T(2) * gamma * mass[j] / rho0sq) * kernel.reversed(pos[j], xi);
+                    else
+                        inner = inner - (T(2) * mass[j] / rho0sq) * kernel.reversed(pos[j], xi);
+                }
+            });
+            T dg = T(0);
+            for_neighbors(i, [&](size_t j) {
+                uint8_t tj = type[j];
+                if (tj == T_NORMAL) {
+                    dg = dg + mass[j] * dot(inner, kernel.reversed(pos[j], xi));
+                    dg = dg + mass[j] * dot((mass[i] / rho0sq) * kernel.reversed(xi, pos[j]),
+                                            kernel.reversed(pos[j], xi));
+                } else if (tj == T_BOUNDARY) {
+                    dg = dg + mass[j] * dot(inner, kernel.reversed(pos[j], xi));
+                }
+            });
+            dg = dg * (dt * dt);
+            diag[i] = dg;
+        });
+
+        T max_v2 = 0, max_a2 = 0;
+        size_t iterations = 0;
+        double avg_out = 0.0;
+        for (size_t it = 0; it < desc.iisph_max_iterations; it++) {
+            loop_for(n, [&](size_t i) {
+                if (type[i] != T_NORMAL) return;
+                const V& xi = pos[i];
+                V pa = vzero<T, D>();
+                for_neighbors(i, [&](size_t j) {
+                    uint8_t tj = type[j];
+                    if (tj == T_NORMAL)
+                        pa = pa - (mass[j] * (pressure[i] / rho0sq + pressure[j] / rho0sq)) * kernel.reversed(pos[j], xi);
+                    else if (tj == T_BOUNDARY) {
+                        if (sl)
+                            pa = pa - (gamma * mass[j] * T(2) * pressure[i] / rho0sq) * kernel.reversed(pos[j], xi);
+                        else
+                            pa = pa - (mass[j] * T(2) * pressure[i] / rho0sq) * kernel.reversed(pos[j], xi);
+                    }
+                });
+                acc[i] = pa;
+            });
+            loop_for(n, [&](size_t i) {
+                if (type[i] != T_NORMAL) return;
+                const V& xi = pos[i];
+                T ap = T(0);
+                for_neighbors(i, [&](size_t j) {
+                    uint8_t tj = type[j];
+                    if (tj == T_NORMAL)
+                        ap = ap + mass[j] * dot(acc[i] - acc[j], kernel.reversed(pos[j], xi));
+                    else if (tj == T_BOUNDARY)
+                        ap = ap + mass[j] * dot(acc[i], kernel.reversed(pos[j], xi));
+                });
+                ap = ap * (dt * dt);
+                if (std::abs(diag[i]) > eps)
+                    pressure[i] = std::fmax(T(0), pressure[i] + (T)desc.iisph_omega * ((source[i] - ap) / diag[i]));
+                else
+                    pressure[i] = T(0);
+                T e = std::abs(source[i] - ap);
+                if (std::abs(diag[i]) > eps) {
+                    if (std::abs(pressure[i]) <= eps) e = T(0);
+                    perr[i] = e;
+                }
+            });
+            iterations = it + 1;
+            // serial convergence test in index order (IISPHFluidSolver3D.hpp:467-522)
+            T avg = T(0);
+            double avg64 = 0.0;
+            size_t counter = 0;
+            max_v2 = 0;
+            max_a2 = 0;
+            for (size_t i = 0; i < n; i++) {
+                if (type[i] != T_NORMAL) continue;
+                max_a2 = std::fmax(max_a2, dot(acc[i], acc[i]));
+                V pv = vstar[i] + dt * acc[i];
+                max_v2 = std::fmax(max_v2, dot(pv, pv));
+                if (std::abs(diag[i]) > eps) {
+                    avg = avg + perr[i];
+                    avg64 = avg64 + (double)perr[i];
+                    counter++;
+                }
+            }
+            if (counter > 0) {
+                avg = avg / (T)counter;
+                avg64 = avg64 / (double)counter;
+            }
+            bool done;
+            if (fp64_reduction) {
+                avg_out = avg64;
+                done = avg64 <= (double)desc.iisph_max_density_error_allowed;
+            } else {
+                avg_out = (double)avg;
+                done = avg <= (T)desc.iisph_max_density_error_allowed;
+            }
+            if (it >= (size_t)desc.iisph_min_iterations - 1) {
+                if (done) break;
+            }
+        }
+        T old_dt = dt;
+        float max_v = (float)std::sqrt((double)max_v2), max_a = (float)std::sqrt((double)max_a2);
+        float corrected = std::fmin(cfl_timestep(max_a, max_v), desired);
+        T cur = (T)corrected;
+        loop_for(n, [&](size_t i) {
+            if (type[i] == T_INACTIVE) return;
+            V pv = vstar[i];
+            if (old_dt != cur) pv = pv - old_dt * anp[i] + cur * anp[i];
+            vel[i] = pv + cur * acc[i];
+            pos[i] = pos[i] + cur * vel[i];
+        });
+        if (st) {
+            st->desired_time_step = desired;
+            st->actual_time_step = corrected;
+            st->iterations = (uint32_t)iterations;
+            st->status = FSA_OK;
+            st->average_density_error = avg_out;
+            st->max_velocity = max_v;
+            st->max_acceleration = max_a;
+        }
+        return FSA_OK;
+    }
+
+    // ---- SESPHFluidSolver3D::execute_simulation_step (SESPHFluidSolver3D.hpp:49-130)
+    // and SESPHFluidSolver (2D, SESPHFluidSolver.hpp:49-105: no single layer, no CFL)
+    int simulate_sesph(float desired, fsa_step_stats* st) {
+        T dt = (T)desired;
+        const T rho0 = (T)desc.rest_density, k = (T)desc.sesph_stiffness_k;
+        const bool sl = D == 3 && desc.sesph_single_layer_boundary != 0;
+        const T g1 = (T)desc.sesph_single_layer_gamma_1, g2 = (T)desc.sesph_single_layer_gamma_2;
+        loop_for(n, [&](size_t i) {
+            if (type[i] != T_NORMAL) return;
+            T rho = T(0);
+            for_neighbors(i, [&](size_t j) {
+                uint8_t tj = type[j];
+                if (tj == T_INACTIVE) return;
+                if (tj == T_BOUNDARY && sl)
+                    rho = rho + g1 * mass[j] * kernel.value(pos[j], pos[i]);
+                else
+                    rho = rho + mass[j] * kernel.value(pos[j], pos[i]);
+            });
+            density[i] = rho;
+            T p = k * (rho / rho0 - T(1));
+            pressure[i] = std::max(p, T(0));
+        });
+        loop_for(n, [&](size_t i) {
+            if (type[i] != T_NORMAL) return;
+            V npa = vzero<T, D>();
+            npa = npa + gravity_vec();
+            npa = npa + viscosity(i, (T)desc.sesph_viscosity, density);
+            const V& xi = pos[i];
+            T mi = mass[i], di = density[i], pi = pressure[i];
+            T pdd = di == T(0) ? T(0) : pi / (di * di);
+            V pa = vzero<T, D>();
+            for_neighbors(i, [&](size_t j) {
+                uint8_t tj = type[j];
+                if (tj == T_INACTIVE) return;
+                if (tj == T_BOUNDARY) {
+                    if (sl)
+                        pa = pa + ((-mi * (pdd + pdd)) * kernel.reversed(pos[j], xi)) * g2;
+                    else
+                        pa = pa + (-mi * (pdd + pdd)) * kernel.reversed(pos[j], xi);
+                } else {
+                    T nd = density[j], np = pressure[j];
+                    T npdd = nd == T(0) ? T(0) : np / (nd * nd);
+                    pa = pa + (-mass[j] * (pdd + npdd)) * kernel.reversed(pos[j], xi);
+                }
+            });
+            acc[i] = pa + npa;
+        });
+        float corrected = desired;
+        float max_v = 0.0f, max_a = 0.0f;
+        if (D == 3) {
+            // serial scan over every particle, boundary and inactive included (:96-107)
+            T mv2 = 0, ma2 = 0;
+            for (size_t i = 0; i < n; i++) {
+                ma2 = std::fmax(ma2, dot(acc[i], acc[i]));
+                V v = vel[i] + dt * acc[i];
+                mv2 = std::fmax(mv2, dot(v, v));
+            }
+            max_v = (float)std::sqrt((double)mv2);
+            max_a = (float)std::sqrt((double)ma2);
+            corrected = std::fmin(cfl_timestep(max_a, max_v), desired);
+        }
+        T cur = (T)corrected;
+        loop_for(n, [&](size_t i) {
+            if (type[i] == T_INACTIVE) return;
+            vel[i] = vel[i] + cur * acc[i];
+            pos[i] = pos[i] + cur * vel[i];
+        });
+        if (st) {
+            st->desired_time_step = desired;
+            st->actual_time_step = corrected;
+            st->iterations = 0;
+            st->status = FSA_OK;
+            st->average_density_error = 0.0;
+            st->max_velocity = max_v;
+            st->max_acceleration = max_a;
+        }
+        return FSA_OK;
+    }
+
+    // ---- IISPHFluidSolver (2D) (IISPHFluidSolver.hpp:139-180, 196-485)
+    int simulate_iisph2d(float desired, fsa_step_stats* st) {
+        T dt = (T)desired;
+        const T rho0 = (T)desc.rest_density;
+        const T gamma = (T)desc.iisph_gamma;
+        const T eps = Consts<T>::eps();
+        density_old = density;
+        loop_for(n, [&](size_t i) {
+            uint8_t t = type[i];
+            if (t == T_INACTIVE) return;
+            if (t != T_BOUNDARY) {  // CalculateDensity
+                T rho = T(0);
+                for_neighbors(i, [&](size_t j) {
+                    if (type[j] == T_INACTIVE) return;
+                    rho = rho + mass[j] * kernel.value(pos[j], pos[i]);
+                });
+                density[i] = rho;
+            }
+            V npa = ext[i];  // CalculateNonPressureAccelerationAndPredictedVelocity
+            if (t != T_BOUNDARY) {
+                npa = npa + gravity_vec();
+                npa = npa + viscosity(i, (T)desc.iisph_viscosity, density_old);
+            }
+            vstar[i] = vel[i] + dt * npa;
+            ext[i] = vzero<T, D>();
+        });
+        loop_for(n, [&](size_t i) {
+            if (type[i] != T_NORMAL) return;
+            const V& xi = pos[i];
+            T sum = T(0);  // ComputeSourceTerm
+            for_neighbors(i, [&](size_t j) {
+                if (type[j] == T_INACTIVE) return;
+                sum = sum + mass[j] * dot(vstar[i] - vstar[j], kernel.reversed(pos[j], xi));
+            });
+            source[i] = rho0 - density[i] - dt * sum;
+            // ComputeDiagonalElement: the inner sum does not depend on the outer
+            // neighbor, so it is formed once; its value is identical.
+            V internal = vzero<T, D>();
+            for_neighbors(i, [&](size_t k) {
+                uint8_t tk = type[k];
+                if (tk == T_NORMAL)
+                    internal = internal + (mass[k] / rho0 / rho0) * kernel.reversed(pos[k], xi);
+                else if (tk == T_BOUNDARY)
+                    internal = internal + (T(2) * gamma * mass[k] / rho0 / rho0) * kernel.reversed(pos[k], xi);
+            });
+            T s = T(0);
+            for_neighbors(i, [&](size_t j) {
+                if (type[j] == T_INACTIVE) return;
+                s = s + mass[j] * dot(-internal, kernel.reversed(pos[j], xi));
+            });
+            for_neighbors(i, [&](size_t j) {
+                if (type[j] != T_NORMAL) return;
+                s = s + mass[j] * dot((mass[i] / rho0 / rho0) * kernel.reversed(xi, pos[j]), kernel.reversed(pos[j], xi));
+            });
+            diag[i] = dt * dt * s;
+            pressure[i] = T(0);  // InitializePressure
+        });
+        // ComputePressure (:396-485)
+        size_t iteration = 0;
+        T err = T(0);
+        const bool int_abs = (desc.flags & FSA_FLAG_EMULATE_INT_ABS) != 0;
+        auto err_abs = [&](T e) -> T {
+            if (int_abs) return (T)std::abs((int)e);
+            return std::abs(e);
+        };
+        while ((iteration < desc.iisph_min_iterations || err_abs(err) > (T)desc.iisph_max_density_error_allowed) &&
+               iteration < desc.iisph_max_iterations) {
+            err = T(0);
+            size_t counter = 0;
+            loop_for(n, [&](size_t i) {
+                if (type[i] != T_NORMAL) return;
+                const V& xi = pos[i];
+                T pi = pressure[i];
+                V sum = vzero<T, D>();
+                for_neighbors(i, [&](size_t j) {
+                    uint8_t tj = type[j];
+                    if (tj == T_NORMAL)
+                        sum = sum + (mass[j] * (pi / rho0 / rho0 + pressure[j] / rho0 / rho0)) * kernel.reversed(pos[j], xi);
+                    else if (tj == T_BOUNDARY)
+                        sum = sum + (gamma * mass[j] * T(2) * (pi / rho0 / rho0)) * kernel.reversed(pos[j], xi);
+                });
+                acc[i] = -sum;
+            });
+            std::vector<T> e_i(n, T(0));
+            std::vector<uint8_t> counted(n, 0);
+            loop_for(n, [&](size_t i) {
+                if (type[i] != T_NORMAL) return;
+                const V& xi = pos[i];
+                const V ai = acc[i];
+                T sum = T(0);
+                for_neighbors(i, [&](size_t j) {
+                    uint8_t tj = type[j];
+                    if (tj == T_NORMAL)
+                        sum = sum + mass[j] * dot(ai - acc[j], kernel.reversed(pos[j], xi));
+                    else if (tj == T_BOUNDARY)
+                        sum = sum + mass[j] * dot(ai, kernel.reversed(pos[j], xi));
+                });
+                T Ap = dt * dt * sum;
+                T dgi = diag[i], si = source[i];
+                if (std::abs(dgi) > eps)
+                    pressure[i] = std::fmax(T(0), pressure[i] + (T)desc.iisph_omega * ((si - Ap) / dgi));
+                else
+                    pressure[i] = T(0);
+                T e = std::abs(si - Ap);
+                if (std::abs(dgi) > eps) {
+                    if (std::abs(pressure[i]) <= eps) e = T(0);
+                    e_i[i] = e;
+                    counted[i] = 1;
+                }
+            });
+            double err64 = 0.0;
+            for (size_t i = 0; i < n; i++)
+                if (counted[i]) {
+                    err = err + e_i[i];
+                    err64 = err64 + (double)e_i[i];
+                    counter++;
+                }
+            if (counter != 0) {
+                err = err / (T)counter;
+                err64 = err64 / (double)counter;
+            } else {
+                err = T(0);
+                err64 = 0.0;
+            }
+            if (fp64_reduction) err = (T)err64;
+            iteration++;
+        }
+        loop_for(n, [&](size_t i) {
+            if (type[i] == T_INACTIVE) return;
+            vel[i] = vstar[i] + dt * acc[i];
+            pos[i] = pos[i] + dt * vel[i];
+        });
+        if (st) {
+            st->desired_time_step = desired;
+            st->actual_time_step = desired;
+            st->iterations = (uint32_t)iteration;
+            st->status = FSA_OK;
+            st->average_density_error = (double)err;
+            st->max_velocity = 0.0f;
+            st->max_acceleration = 0.0f;
+        }
+        return FSA_OK;
+    }
+
+    int simulate(float dt, fsa_step_stats* st) override {
+        if (!searched) {
+            g_err = "simulate called before find_neighbors";
+            return FSA_E_STATE;
+        }
+        if (desc.solver == FSA_SOLVER_SESPH) return simulate_sesph(dt, st);
+        if (D == 3) return simulate_iisph3d(dt, st);
+        return simulate_iisph2d(dt, st);
+    }
+
+    int download(const fsa_particle_view* v) override {
+        if (v->count != n) {
+            g_err = "download view count mismatch";
+            return FSA_E_INVALID;
+        }
+        for (size_t k = 0; k < n; k++) {
+            size_t i = id[k];
+            if (v->movement) {
+                float* mv = (float*)((char*)v->movement + i * v->movement_stride);
+                for (int c = 0; c < D; c++) {
+                    mv[c] = (float)pos[k].v[c];
+                    mv[D + c] = (float)vel[k].v[c];
+                    mv[2 * D + c] = (float)acc[k].v[c];
+                }
+            }
+            if (v->particle_data) {
+                float* pd = (float*)((char*)v->particle_data + i * v->particle_data_stride);
+                pd[0] = (float)mass[k];
+                pd[1] = (float)pressure[k];
+                pd[2] = (float)density[k];
+            }
+            if (v->info) {
+                char* inf = (char*)v->info + i * v->info_stride;
+                std::memcpy(inf, &tag[k], 4);
+                inf[4] = (char)type[k];
+            }
+            if (v->external) {
+                float* e = (float*)((char*)v->external + i * v->external_stride);
+                for (int c = 0; c < D; c++) e[c] = (float)ext[k].v[c];
+            }
+        }
+        return FSA_OK;
+    }
+    int download_iisph(void* rec, uint64_t stride) override {
+        for (size_t k = 0; k < n; k++) {
+            float* r = (float*)((char*)rec + id[k] * stride);
+            int o = 0;
+            for (int c = 0; c < D; c++) r[o++] = (float)vstar[k].v[c];
+            if (D == 3)
+                for (int c = 0; c < D; c++) r[o++] = (float)anp[k].v[c];
+            r[o++] = (float)source[k];
+            r[o++] = (float)diag[k];
+            if (D == 3) r[o++] = (float)perr[k];
+        }
+        return FSA_OK;
+    }
+    int export_neighbors(uint64_t* off, uint64_t* cols, uint64_t cap, uint64_t* nnz) override {
+        // rows in caller order, columns in caller indices, ascending
+        std::vector<std::vector<uint64_t>> rows(n);
+        for (size_t k = 0; k < n; k++) {
+            bool has = false;
+            if (search == SEARCH_COMPRESSED3D)
+                has = type[k] != T_INACTIVE;
+            else
+                has = k < lists.size();
+            if (!has) continue;
+            std::vector<uint64_t>& r = rows[id[k]];
+            for_neighbors(k, [&](size_t j) { r.push_back(id[j]); });
+            std::sort(r.begin(), r.end());
+        }
+        uint64_t total = 0;
+        for (size_t i = 0; i < n; i++) {
+            off[i] = total;
+            for (uint64_t c : rows[i]) {
+                if (total < cap) cols[total] = c;
+                total++;
+            }
+        }
+        off[n] = total;
+        *nnz = total;
+        if (total > cap) {
+            g_err = "export capacity too small";
+            return FSA_E_INVALID;
+        }
+        return FSA_OK;
+    }
+    void codec_stats(uint64_t* total, uint64_t* bytes) override {
+        uint64_t t = 0, b = 0;
+        if (search == SEARCH_COMPRESSED3D) {
+            for (size_t k = 0; k < n; k++) {
+                t += storage[k].size();
+                b += storage[k].encoded_bytes();
+            }
+        } else {
+            for (auto& l : lists) t += l.size();
+        }
+        *total = t;
+        *bytes = b;
+    }
+};
+
+}  // namespace orc
+
+using namespace orc;
+
+extern "C" {
+
+const char* orc_last_error(void) { return g_err.c_str(); }
+void orc_set_threads(int n) { g_threads = n; }
+int orc_get_threads(void) { return thread_count(); }
+
+// mode: 0 = FP32 reference order, 1 = FP64 throughout, 2 = FP32 reference
+// order with FP64 convergence sums (the device library's reduction)
+void* orc_create(const fsa_scene_desc* d, int search_kind, int mode, const fsa_particle_view* v) {
+    try {
+        SceneBase* s = nullptr;
+        bool r64 = mode != 0;
+        if (d->dimension == 3) {
+            if (mode == 1) {
+                auto* p = new Scene<double, 3>();
+                p->load(*d, search_kind, v);
+                p->fp64_reduction = r64;
+                s = p;
+            } else {
+                auto* p = new Scene<float, 3>();
+                p->load(*d, search_kind, v);
+                p->fp64_reduction = r64;
+                s = p;
+            }
+        } else {
+            if (mode == 1) {
+                auto* p = new Scene<double, 2>();
+                p->load(*d, search_kind, v);
+                p->fp64_reduction = r64;
+                s = p;
+            } else {
+                auto* p = new Scene<float, 2>();
+                p->load(*d, search_kind, v);
+                p->fp64_reduction = r64;
+                s = p;
+            }
+        }
+        return s;
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return nullptr;
+    }
+}
+void orc_destroy(void* s) { delete (SceneBase*)s; }
+int orc_find_neighbors(void* s) { return ((SceneBase*)s)->find_neighbors(); }
+int orc_simulate(void* s, float dt, fsa_step_stats* st) { return ((SceneBase*)s)->simulate(dt, st); }
+int orc_download(void* s, const fsa_particle_view* v) { return ((SceneBase*)s)->download(v); }
+int orc_download_iisph(void* s, void* rec, uint64_t stride) { return ((SceneBase*)s)->download_iisph(rec, stride); }
+int orc_export_neighbors(void* s, uint64_t* off, uint64_t* cols, uint64_t cap, uint64_t* nnz) {
+    return ((SceneBase*)s)->export_neighbors(off, cols, cap, nnz);
+}
+void orc_codec_stats(void* s, uint64_t* total, uint64_t* bytes) { ((SceneBase*)s)->codec_stats(total, bytes); }
+
+void orc_translate(void* s, const float* delta) { ((SceneBase*)s)->translate(delta); }
+
+// ---- pure functions, exposed for pinning tests --------------------------
+void orc_kernel_values(int dim, uint64_t n, const float* xs, float support, float* out) {
+    if (dim == 3) {
+        CubicSpline<float, 3> k;
+        k.initialize(support);
+        for (uint64_t i = 0; i < n; i++) out[i] = k.value(Vec<float, 3>{{xs[3 * i], xs[3 * i + 1], xs[3 * i + 2]}});
+    } else {
+        CubicSpline<float, 2> k;
+        k.initialize(support);
+        for (uint64_t i = 0; i < n; i++) out[i] = k.value(Vec<float, 2>{{xs[2 * i], xs[2 * i + 1]}});
+    }
+}
+void orc_kernel_derivatives(int dim, uint64_t n, const float* xs, float support, float* out) {
+    if (dim == 3) {
+        CubicSpline<float, 3> k;
+        k.initialize(support);
+        for (uint64_t i = 0; i < n; i++) {
+            auto r = k.derivative(Vec<float, 3>{{xs[3 * i], xs[3 * i + 1], xs[3 * i + 2]}});
+            for (int c = 0; c < 3; c++) out[3 * i + c] = r.v[c];
+        }
+    } else {
+        CubicSpline<float, 2> k;
+        k.initialize(support);
+        for (uint64_t i = 0; i < n; i++) {
+            auto r = k.derivative(Vec<float, 2>{{xs[2 * i], xs[2 * i + 1]}});
+            for (int c = 0; c < 2; c++) out[2 * i + c] = r.v[c];
+        }
+    }
+}
+float orc_kernel_value(int dim, const float* x, float support) {
+    if (dim == 3) {
+        CubicSpline<float, 3> k;
+        k.initialize(support);
+        return k.value(Vec<float, 3>{{x[0], x[1], x[2]}});
+    }
+    CubicSpline<float, 2> k;
+    k.initialize(support);
+    return k.value(Vec<float, 2>{{x[0], x[1]}});
+}
+void orc_kernel_derivative(int dim, const float* x, float support, float* out) {
+    if (dim == 3) {
+        CubicSpline<float, 3> k;
+        k.initialize(support);
+        auto r = k.derivative(Vec<float, 3>{{x[0], x[1], x[2]}});
+        for (int c = 0; c < 3; c++) out[c] = r.v[c];
+        return;
+    }
+    CubicSpline<float, 2> k;
+    k.initialize(support);
+    auto r = k.derivative(Vec<float, 2>{{x[0], x[1]}});
+    for (int c = 0; c < 2; c++) out[c] = r.v[c];
+}
+uint64_t orc_morton3(uint32_t x, uint32_t y, uint32_t z) { return morton3_64(x, y, z); }
+uint64_t orc_cell_key(int32_t x, int32_t y, int32_t z) {
+    return morton3_64(to_unsigned(x), to_unsigned(y), to_unsigned(z));
+}
+void orc_stable_sort_u64(const uint64_t* keys, uint64_t n, uint64_t* perm) {
+    std::vector<uint64_t> p(n);
+    std::iota(p.begin(), p.end(), 0);
+    std::stable_sort(p.begin(), p.end(), [&](uint64_t a, uint64_t b) { return keys[a] < keys[b]; });
+    std::memcpy(perm, p.data(), n * sizeof(uint64_t));
+}
+float orc_cfl_timestep(float max_a, float max_v, float particle_size, float timestep, float lambda_a, float lambda_v) {
+    Scene<float, 3> s;
+    s.desc.particle_size = particle_size;
+    s.desc.timestep = timestep;
+    s.desc.lambda_a = lambda_a;
+    s.desc.lambda_v = lambda_v;
+    return s.cfl_timestep(max_a, max_v);
+}
+// NeighborStorage codec
+void* orc_codec_new(void) { return new NeighborStorage(); }
+void orc_codec_free(void* c) { delete (NeighborStorage*)c; }
+void orc_codec_clear(void* c) { ((NeighborStorage*)c)->clear(); }
+void orc_codec_set_first(void* c, uint64_t i) { ((NeighborStorage*)c)->set_first(i); }
+int orc_codec_set_next(void* c, uint64_t d) { return ((NeighborStorage*)c)->set_next(d) ? 0 : 1; }
+uint64_t orc_codec_size(void* c) { return ((NeighborStorage*)c)->size(); }
+uint32_t orc_codec_get_delta(void* c, uint64_t k) { return ((NeighborStorage*)c)->get_delta(k); }
+uint64_t orc_codec_get_first(void* c) { return ((NeighborStorage*)c)->first_neighbor; }
+uint64_t orc_codec_used_bytes(void* c) { return ((NeighborStorage*)c)->bytes; }
+uint64_t orc_codec_encoded_bytes(void* c) { return ((NeighborStorage*)c)->encoded_bytes(); }
+
+}  // extern "C"
