@@ -80,7 +80,9 @@ struct fsa_context_s {
     bool timing = false;
     double timer_ms[FSA_TIMER_COUNT] = {};
     uint64_t timer_n[FSA_TIMER_COUNT] = {};
-    hipEvent_t ev[2 * FSA_TIMER_COUNT] = {};
+    std::vector<hipEvent_t> ev_pool;   // start/stop pairs of the phases in flight
+    std::vector<int> ev_phase;
+    size_t ev_used = 0;
     bool graph = false;
     uint32_t block_override = 0;
     std::vector<fsa_scene_s*> scenes;  // live scenes, destroyed with the context
@@ -335,6 +337,13 @@ void prepare_grid(fsa_scene_s* s) {
     }
 }
 
+uint32_t spec_of(const SceneDev& h) {
+    uint32_t sp = 0;
+    if (h.flags & kFlagHOne) sp |= kSpecHOne;
+    if (h.flags & kFlagRho0SqOne) sp |= kSpecRho0SqOne;
+    return sp;
+}
+
 uint32_t sort_passes(const SceneDev& h) { return (3 * h.levels + 3 + kRadixBits - 1) / kRadixBits; }
 
 void read_state(fsa_scene_s* s) {
@@ -343,20 +352,33 @@ void read_state(fsa_scene_s* s) {
 }
 
 // ---- timing ---------------------------------------------------------------------------
+// Phase timing records HIP events on the context stream without waiting; the
+// elapsed times are collected when the pool fills or when they are read.
+void collect_timing(fsa_context ctx) {
+    if (ctx->ev_used == 0) return;
+    HIP_CHECK(hipEventSynchronize(ctx->ev_pool[ctx->ev_used - 1]));
+    for (size_t k = 0; k + 1 < ctx->ev_used; k += 2) {
+        float ms = 0.0f;
+        HIP_CHECK(hipEventElapsedTime(&ms, ctx->ev_pool[k], ctx->ev_pool[k + 1]));
+        ctx->timer_ms[ctx->ev_phase[k / 2]] += ms;
+        ctx->timer_n[ctx->ev_phase[k / 2]] += 1;
+    }
+    ctx->ev_used = 0;
+}
+
 struct PhaseTimer {
     fsa_context ctx;
     int id;
     PhaseTimer(fsa_context c, int i) : ctx(c), id(i) {
-        if (ctx->timing) HIP_CHECK(hipEventRecord(ctx->ev[2 * id], ctx->stream));
+        if (!ctx->timing) return;
+        if (ctx->ev_used + 2 > ctx->ev_pool.size()) collect_timing(ctx);
+        HIP_CHECK(hipEventRecord(ctx->ev_pool[ctx->ev_used], ctx->stream));
     }
     ~PhaseTimer() noexcept(false) {
         if (!ctx->timing) return;
-        HIP_CHECK(hipEventRecord(ctx->ev[2 * id + 1], ctx->stream));
-        HIP_CHECK(hipEventSynchronize(ctx->ev[2 * id + 1]));
-        float ms = 0.0f;
-        HIP_CHECK(hipEventElapsedTime(&ms, ctx->ev[2 * id], ctx->ev[2 * id + 1]));
-        ctx->timer_ms[id] += ms;
-        ctx->timer_n[id] += 1;
+        HIP_CHECK(hipEventRecord(ctx->ev_pool[ctx->ev_used + 1], ctx->stream));
+        ctx->ev_phase[ctx->ev_used / 2] = id;
+        ctx->ev_used += 2;
     }
 };
 
@@ -446,6 +468,7 @@ void check_status(fsa_scene_s* s) {
 }
 
 void upload_scene_table(fsa_scene_s* s) {
+    s->single.spec = spec_of(s->hd);
     HIP_CHECK(hipMemcpyAsync(s->d_table, &s->hd, sizeof(SceneDev), hipMemcpyHostToDevice, s->ctx->stream));
 }
 
@@ -812,7 +835,9 @@ FSA_API int32_t fsa_context_create(int32_t device, uint64_t budget, fsa_context*
             delete c;
             throw FsaError(FSA_E_DEVICE, "hipStreamCreate failed");
         }
-        for (auto& e : c->ev) HIP_CHECK(hipEventCreate(&e));
+        c->ev_pool.resize(4096);
+        c->ev_phase.resize(2048);
+        for (auto& e : c->ev_pool) HIP_CHECK(hipEventCreate(&e));
         *out = c;
     });
 }
@@ -828,7 +853,7 @@ FSA_API int32_t fsa_context_destroy(fsa_context c) {
             live = c->scenes;
         }
         for (auto* s : live) destroy_scene(s);
-        for (auto& e : c->ev) (void)hipEventDestroy(e);
+        for (auto& e : c->ev_pool) (void)hipEventDestroy(e);
         (void)hipStreamDestroy(c->stream);
         delete c;
     });
@@ -1069,6 +1094,7 @@ FSA_API int32_t fsa_context_set_block_size(fsa_context c, uint32_t threads) {
 FSA_API int32_t fsa_context_set_timing(fsa_context c, uint32_t enabled) {
     return guarded([&] {
         if (!c) throw FsaError(FSA_E_INVALID, "context is NULL");
+        collect_timing(c);
         c->timing = enabled != 0;
         for (int k = 0; k < FSA_TIMER_COUNT; k++) {
             c->timer_ms[k] = 0.0;
@@ -1080,6 +1106,7 @@ FSA_API int32_t fsa_context_set_timing(fsa_context c, uint32_t enabled) {
 FSA_API int32_t fsa_context_get_timing(fsa_context c, double* total_ms, uint64_t* launches, uint32_t n) {
     return guarded([&] {
         if (!c) throw FsaError(FSA_E_INVALID, "context is NULL");
+        collect_timing(c);
         for (uint32_t k = 0; k < n && k < (uint32_t)FSA_TIMER_COUNT; k++) {
             if (total_ms) total_ms[k] = c->timer_ms[k];
             if (launches) launches[k] = c->timer_n[k];

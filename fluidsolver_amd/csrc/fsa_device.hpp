@@ -142,17 +142,31 @@ __device__ __forceinline__ float dot3(float ax, float ay, float az, float bx, fl
     return (ax * bx + ay * by) + az * bz;
 }
 
-// CubicSplineKernel3D::GetKernelValue (CubicSplineKernel3D.cpp:7-22) from |x|
+// Compile-time specialisation of the two parameters that make operations
+// exact no-ops: Spec bit 0 = every scene of the launch has h == 1, bit 1 =
+// every scene has rest_density^2 == 1.  Without the bit the scene flag decides.
+constexpr uint32_t kSpecHOne = 1u;
+constexpr uint32_t kSpecRho0SqOne = 2u;
+
+template <uint32_t Spec>
+__device__ __forceinline__ bool spec_h_one(const SceneDev& S) {
+    return (Spec & kSpecHOne) ? true : (S.flags & kFlagHOne) != 0;
+}
+template <uint32_t Spec>
+__device__ __forceinline__ bool spec_rho_one(const SceneDev& S) {
+    return (Spec & kSpecRho0SqOne) ? true : (S.flags & kFlagRho0SqOne) != 0;
+}
+
+// CubicSplineKernel3D::GetKernelValue (CubicSplineKernel3D.cpp:7-22) from |x|.
+// Both polynomial branches are formed and selected; the selected value is the
+// one the reference's if/else computes.
+template <uint32_t Spec>
 __device__ __forceinline__ float kernel_w(const SceneDev& S, float len) {
-    float q = (S.flags & kFlagHOne) ? len : len / S.h;
-    float ret = 0.0f;
-    if (q < 1.0f) {
-        float a = 2.0f - q, b = 1.0f - q;
-        ret = a * a * a - 4.0f * (b * b * b);
-    } else if (q < 2.0f) {
-        float a = 2.0f - q;
-        ret = a * a * a;
-    }
+    const float q = spec_h_one<Spec>(S) ? len : len / S.h;
+    const float a = 2.0f - q, b = 1.0f - q;
+    const float a3 = a * a * a;
+    const float inner = a3 - 4.0f * (b * b * b);
+    const float ret = q < 1.0f ? inner : (q < 2.0f ? a3 : 0.0f);
     return S.alpha * ret;
 }
 
@@ -160,46 +174,37 @@ __device__ __forceinline__ float kernel_w(const SceneDev& S, float len) {
 // (CubicSplineKernel3D.cpp:24-47, 62-65): alpha * (pre * f'(q)) with
 // pre = h * d / |d|, zero when |d| <= FLT_EPSILON.  The reference negates
 // twice (derivative and reversal); negation is exact, so both are dropped.
+template <uint32_t Spec>
 __device__ __forceinline__ void kernel_grad(const SceneDev& S, float dx, float dy, float dz, float len,
                                             float& gx, float& gy, float& gz) {
-    const bool hone = (S.flags & kFlagHOne) != 0;
-    float q = hone ? len : len / S.h;
-    float px = 0.0f, py = 0.0f, pz = 0.0f;
-    if (len > 1.1920928955078125e-07f) {
-        if (hone) {
-            px = dx / len;
-            py = dy / len;
-            pz = dz / len;
-        } else {
-            px = (S.h * dx) / len;
-            py = (S.h * dy) / len;
-            pz = (S.h * dz) / len;
-        }
-    }
-    float s = 0.0f;
-    bool inside = true;
-    if (q < 1.0f) {
-        float a = 2.0f - q, b = 1.0f - q;
-        s = -3.0f * (a * a) + 12.0f * (b * b);
-    } else if (q < 2.0f) {
-        float a = 2.0f - q;
-        s = -3.0f * (a * a);
+    const bool hone = spec_h_one<Spec>(S);
+    const float q = hone ? len : len / S.h;
+    const bool nz = len > 1.1920928955078125e-07f;
+    float px, py, pz;
+    if (hone) {
+        px = dx / len;
+        py = dy / len;
+        pz = dz / len;
     } else {
-        inside = false;
+        px = (S.h * dx) / len;
+        py = (S.h * dy) / len;
+        pz = (S.h * dz) / len;
     }
-    if (inside) {
-        gx = S.alpha * (px * s);
-        gy = S.alpha * (py * s);
-        gz = S.alpha * (pz * s);
-    } else {
-        gx = 0.0f;
-        gy = 0.0f;
-        gz = 0.0f;
-    }
+    px = nz ? px : 0.0f;
+    py = nz ? py : 0.0f;
+    pz = nz ? pz : 0.0f;
+    const float a = 2.0f - q, b = 1.0f - q;
+    const float m3a2 = -3.0f * (a * a);
+    const float s = q < 1.0f ? m3a2 + 12.0f * (b * b) : m3a2;
+    const bool inside = q < 2.0f;
+    gx = inside ? S.alpha * (px * s) : 0.0f;
+    gy = inside ? S.alpha * (py * s) : 0.0f;
+    gz = inside ? S.alpha * (pz * s) : 0.0f;
 }
 
+template <uint32_t Spec>
 __device__ __forceinline__ float div_rho0sq(const SceneDev& S, float v) {
-    return (S.flags & kFlagRho0SqOne) ? v : v / S.rho0sq;
+    return spec_rho_one<Spec>(S) ? v : v / S.rho0sq;
 }
 
 // (int)std::floor(x / search_radius) (CompressedNeighbors.cpp:37-41)

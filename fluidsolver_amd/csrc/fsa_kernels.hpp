@@ -18,6 +18,7 @@ struct LaunchSet {
     BlockRef* map_scan = nullptr;    // (scene, digit)
     BlockRef* map_scene = nullptr;   // one block per scene
     uint32_t nb_all = 0, nb_fluid = 0, nb_tiles = 0, nb_scan = 0, nb_scene = 0;
+    uint32_t spec = 0;  // kSpec* bits shared by every scene of the set
 };
 
 // neighborhood search

@@ -218,17 +218,7 @@ __global__ __launch_bounds__(kBlock) void k_permute(const SceneDev* __restrict__
         fluid = (idt >> kTypeShift) == 0u;
     }
     const int cnt = __syncthreads_count(fluid);
-    if (threadIdx.x == 0) {
-        S.block_count[br.block] = (uint32_t)cnt;
-        __threadfence();
-        const uint32_t done = atomicAdd(&S.state->done[0], 1u);
-        if (done == S.nblocks - 1) {
-            // every other block of this scene has finished reading [cur]
-            S.state->done[0] = 0;
-            S.state->cur = o;
-            __threadfence();
-        }
-    }
+    if (threadIdx.x == 0) S.block_count[br.block] = (uint32_t)cnt;
 }
 
 // one block per scene: exclusive scan of the per-block fluid counts
@@ -245,6 +235,8 @@ __global__ __launch_bounds__(kBlock) void k_fluid_scan(const SceneDev* __restric
         if (idx < S.nblocks) S.block_count[idx] = carry + ex;
         carry += tot;
     }
+    // the permuted buffer becomes current (k_permute has finished reading [cur])
+    if (threadIdx.x == 0) S.state->cur ^= 1u;
 }
 
 __global__ __launch_bounds__(kBlock) void k_compact(const SceneDev* __restrict__ scenes,
@@ -316,58 +308,77 @@ __device__ __forceinline__ void bitonic_sort(uint32_t (&a)[N]) {
     }
 }
 
+// Distance from x to the face of its own cell in direction d (d = -1, 0, +1).
+__device__ __forceinline__ float face_gap(float x, int32_t c, int d, float r) {
+    const float lo = (float)c * r;
+    return d < 0 ? x - lo : (d > 0 ? (lo + r) - x : 0.0f);
+}
+
 __global__ __launch_bounds__(kBlock) void k_lists(const SceneDev* __restrict__ scenes,
                                                   const BlockRef* __restrict__ map) {
     const BlockRef br = map[blockIdx.x];
     const SceneDev& S = scenes[br.scene];
-    const uint32_t k = br.block * kBlock + threadIdx.x;
+    __shared__ uint32_t s_run[27][kBlock];  // sorted runs of this thread, column per thread
+    const uint32_t t = threadIdx.x;
+    const uint32_t k = br.block * kBlock + t;
     if (k >= S.nf) return;
     const uint32_t c = S.state->cur;
     const uint32_t i = S.fluid_idx[k];
     const float4 p = S.pos[c][i];
-    const int32_t cx = cell_of(p.x, S.search_radius);
-    const int32_t cy = cell_of(p.y, S.search_radius);
-    const int32_t cz = cell_of(p.z, S.search_radius);
-    // runs of the 27 neighbor cells, ordered by run start (= by cell key)
+    const float r = S.search_radius;
+    const int32_t cx = cell_of(p.x, r);
+    const int32_t cy = cell_of(p.y, r);
+    const int32_t cz = cell_of(p.z, r);
+    // cells whose box lies farther than r (with a rounding margin) hold no neighbor
+    const float cull2 = S.r2 * 1.0001f;
+    float gx[3], gy[3], gz[3];
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        const float ax = face_gap(p.x, cx, d - 1, r), ay = face_gap(p.y, cy, d - 1, r), az = face_gap(p.z, cz, d - 1, r);
+        gx[d] = ax * ax;
+        gy[d] = ay * ay;
+        gz[d] = az * az;
+    }
     uint32_t run[32];
 #pragma unroll
-    for (int s = 0; s < 32; s++) run[s] = kEmptyCell;
+    for (int s = 27; s < 32; s++) run[s] = kEmptyCell;
 #pragma unroll
     for (int s = 0; s < 27; s++) {
-        const int dx = s / 9 - 1, dy = (s / 3) % 3 - 1, dz = s % 3 - 1;
+        const int dx = s / 9, dy = (s / 3) % 3, dz = s % 3;
+        run[s] = kEmptyCell;
         uint32_t lin;
-        if (cell_linear(S, cx + dx, cy + dy, cz + dz, lin)) {
+        if (gx[dx] + gy[dy] + gz[dz] <= cull2 && cell_linear(S, cx + dx - 1, cy + dy - 1, cz + dz - 1, lin)) {
             const uint4 e = S.cells[lin];
             if (e.z == S.stamp) run[s] = (e.x << 5) | (uint32_t)s;
         }
     }
     bitonic_sort<32>(run);
+#pragma unroll
+    for (int q = 0; q < 27; q++) s_run[q][t] = run[q];
     const float r2 = S.r2;
     const uint32_t stride = S.list_stride;
     uint32_t cnt = 0;
     bool overflow = false;
     const float4* __restrict__ pos = S.pos[c];
     const uint32_t* __restrict__ idt = S.idt[c];
-#pragma unroll
+    uint32_t* __restrict__ out = S.nbr + k;
     for (int q = 0; q < 27; q++) {
-        const uint32_t rv = run[q];
-        if (rv != kEmptyCell) {
-            const uint32_t s = rv & 31u;
-            const int dx = (int)(s / 9) - 1, dy = (int)((s / 3) % 3) - 1, dz = (int)(s % 3) - 1;
-            uint32_t lin;
-            cell_linear(S, cx + dx, cy + dy, cz + dz, lin);
-            const uint32_t end = S.cells[lin].y;
-            for (uint32_t j = rv >> 5; j < end; j++) {
-                const float4 pj = pos[j];
-                const float ddx = pj.x - p.x, ddy = pj.y - p.y, ddz = pj.z - p.z;
-                if (dot3(ddx, ddy, ddz, ddx, ddy, ddz) <= r2) {
-                    if (cnt < (uint32_t)kMaxNeighbors) {
-                        const uint32_t bnd = (idt[j] >> kTypeShift) == 1u ? kBoundaryBit : 0u;
-                        S.nbr[cnt * stride + k] = j | bnd;
-                        cnt++;
-                    } else {
-                        overflow = true;
-                    }
+        const uint32_t rv = s_run[q][t];
+        if (rv == kEmptyCell) break;
+        const uint32_t s = rv & 31u;
+        uint32_t lin;
+        cell_linear(S, cx + (int)(s / 9) - 1, cy + (int)((s / 3) % 3) - 1, cz + (int)(s % 3) - 1, lin);
+        const uint32_t end = S.cells[lin].y;
+        for (uint32_t j = rv >> 5; j < end; j++) {
+            const float4 pj = pos[j];
+            const float ddx = pj.x - p.x, ddy = pj.y - p.y, ddz = pj.z - p.z;
+            if (dot3(ddx, ddy, ddz, ddx, ddy, ddz) <= r2) {
+                if (cnt < (uint32_t)kMaxNeighbors) {
+                    const uint32_t bnd = (idt[j] >> kTypeShift) == 1u ? kBoundaryBit : 0u;
+                    out[cnt * stride] = j | bnd;
+                    cnt++;
+                } else {
+                    overflow = true;
                 }
             }
         }

@@ -27,6 +27,7 @@ struct Grad {
 };
 
 // d = x_i - x_j, squared length and length, kernel gradient of the pair
+template <uint32_t Spec>
 __device__ __forceinline__ void pair_geometry(const SceneDev& S, const float4& xi, const float4& xj, float& dx,
                                               float& dy, float& dz, float& d2, float& len, Grad& g) {
     dx = xi.x - xj.x;
@@ -34,7 +35,7 @@ __device__ __forceinline__ void pair_geometry(const SceneDev& S, const float4& x
     dz = xi.z - xj.z;
     d2 = dot3(dx, dy, dz, dx, dy, dz);
     len = __builtin_sqrtf(d2);
-    kernel_grad(S, dx, dy, dz, len, g.x, g.y, g.z);
+    kernel_grad<Spec>(S, dx, dy, dz, len, g.x, g.y, g.z);
 }
 
 __device__ __forceinline__ uint32_t scene_fluid_blocks(const SceneDev& S) { return (S.nf + kBlock - 1) / kBlock; }
@@ -118,21 +119,8 @@ __device__ __forceinline__ Partial reduce_partials(const Partial* parts, uint32_
     return block_reduce(err, cnt, a2, v2, lds);
 }
 
-// last block of a scene (per counter) returns true after all partials are visible
-__device__ __forceinline__ bool last_block(uint32_t* counter, uint32_t nblocks, bool& flag_lds) {
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        __threadfence();
-        const uint32_t done = atomicAdd(counter, 1u);
-        flag_lds = done == nblocks - 1;
-        if (flag_lds) *counter = 0;
-    }
-    __syncthreads();
-    if (flag_lds) __threadfence();
-    return flag_lds;
-}
-
 // ---- IISPH phase 1: viscosity, predicted velocity, density -------------------------
+template <uint32_t Spec>
 __global__ __launch_bounds__(kBlock) void k_iisph_prepare(const SceneDev* __restrict__ scenes,
                                                           const BlockRef* __restrict__ map) {
     const BlockRef br = map[blockIdx.x];
@@ -164,7 +152,7 @@ __global__ __launch_bounds__(kBlock) void k_iisph_prepare(const SceneDev* __rest
         const float4 vj = vel[j];
         float dx, dy, dz, d2, len;
         Grad g;
-        pair_geometry(S, xi, xj, dx, dy, dz, d2, len, g);
+        pair_geometry<Spec>(S, xi, xj, dx, dy, dz, d2, len, g);
         // ComputeViscosityAcceleration: densities of the start-of-step snapshot
         const float nd = vj.w;
         if (nd != 0.0f) {
@@ -176,7 +164,7 @@ __global__ __launch_bounds__(kBlock) void k_iisph_prepare(const SceneDev* __rest
         }
         float m = xj.w;
         if (sl && (ent & kBoundaryBit)) m = m * S.gamma1;
-        rho = rho + m * kernel_w(S, len);
+        rho = rho + m * kernel_w<Spec>(S, len);
     }
     float nx = ex + 0.0f, ny = ey + (-S.gravity), nz = ez + 0.0f;
     nx = nx + S.visc2 * tx;
@@ -211,6 +199,7 @@ __global__ __launch_bounds__(kBlock) void k_iisph_prepare_boundary(const SceneDe
 }
 
 // ---- IISPH phase 2: source term and diagonal element -----------------------------
+template <uint32_t Spec>
 __global__ __launch_bounds__(kBlock) void k_iisph_diagonal(const SceneDev* __restrict__ scenes,
                                                            const BlockRef* __restrict__ map) {
     const BlockRef br = map[blockIdx.x];
@@ -234,15 +223,15 @@ __global__ __launch_bounds__(kBlock) void k_iisph_diagonal(const SceneDev* __res
         const float4 xj = pos[j];
         float dx, dy, dz, d2, len;
         Grad g;
-        pair_geometry(S, xi, xj, dx, dy, dz, d2, len, g);
+        pair_geometry<Spec>(S, xi, xj, dx, dy, dz, d2, len, g);
         float f;
         if (!(ent & kBoundaryBit)) {
             const float4 vsj = S.vstar[j];
             contrib = contrib + xj.w * dot3(vs.x - vsj.x, vs.y - vsj.y, vs.z - vsj.z, g.x, g.y, g.z);
-            f = div_rho0sq(S, xj.w);
+            f = div_rho0sq<Spec>(S, xj.w);
         } else {
             contrib = contrib + xj.w * dot3(vs.x, vs.y, vs.z, g.x, g.y, g.z);
-            f = sl ? div_rho0sq(S, 2.0f * S.gamma * xj.w) : div_rho0sq(S, 2.0f * xj.w);
+            f = sl ? div_rho0sq<Spec>(S, 2.0f * S.gamma * xj.w) : div_rho0sq<Spec>(S, 2.0f * xj.w);
         }
         ix = ix - f * g.x;
         iy = iy - f * g.y;
@@ -250,7 +239,7 @@ __global__ __launch_bounds__(kBlock) void k_iisph_diagonal(const SceneDev* __res
     }
     const float dt = S.dt;
     S.src[i] = S.rest_density - S.rho_new[i] - dt * contrib;
-    const float cm = div_rho0sq(S, xi.w);
+    const float cm = div_rho0sq<Spec>(S, xi.w);
     float dg = 0.0f;
     for (uint32_t e = 0; e < cnt; e++) {
         const uint32_t ent = nb[e * stride];
@@ -258,7 +247,7 @@ __global__ __launch_bounds__(kBlock) void k_iisph_diagonal(const SceneDev* __res
         const float4 xj = pos[j];
         float dx, dy, dz, d2, len;
         Grad g;
-        pair_geometry(S, xi, xj, dx, dy, dz, d2, len, g);
+        pair_geometry<Spec>(S, xi, xj, dx, dy, dz, d2, len, g);
         dg = dg + xj.w * dot3(ix, iy, iz, g.x, g.y, g.z);
         if (!(ent & kBoundaryBit)) {
             // (m_i / rho0^2) * gradW(x_i, x_j) with gradW(x_i, x_j) = -gradW(x_j, x_i) exactly
@@ -283,6 +272,7 @@ __global__ void k_iisph_begin_loop(const SceneDev* __restrict__ scenes, const Bl
 }
 
 // ---- Jacobi pass A: pressure acceleration ------------------------------------------
+template <uint32_t Spec>
 __global__ __launch_bounds__(kBlock) void k_iisph_pass_a(const SceneDev* __restrict__ scenes,
                                                          const BlockRef* __restrict__ map) {
     const BlockRef br = map[blockIdx.x];
@@ -296,7 +286,7 @@ __global__ __launch_bounds__(kBlock) void k_iisph_pass_a(const SceneDev* __restr
     const float* __restrict__ pres = S.pres[c];
     const float4 xi = pos[i];
     const float pi = pres[i];
-    const float pir = div_rho0sq(S, pi);
+    const float pir = div_rho0sq<Spec>(S, pi);
     const bool sl = (S.flags & kFlagSingleLayer) != 0;
     const uint32_t cnt = S.nbr_cnt[k];
     const uint32_t stride = S.list_stride;
@@ -308,12 +298,12 @@ __global__ __launch_bounds__(kBlock) void k_iisph_pass_a(const SceneDev* __restr
         const float4 xj = pos[j];
         float dx, dy, dz, d2, len;
         Grad g;
-        pair_geometry(S, xi, xj, dx, dy, dz, d2, len, g);
+        pair_geometry<Spec>(S, xi, xj, dx, dy, dz, d2, len, g);
         float f;
         if (!(ent & kBoundaryBit)) {
-            f = xj.w * (pir + div_rho0sq(S, pres[j]));
+            f = xj.w * (pir + div_rho0sq<Spec>(S, pres[j]));
         } else {
-            f = sl ? div_rho0sq(S, S.gamma * xj.w * 2.0f * pi) : div_rho0sq(S, xj.w * 2.0f * pi);
+            f = sl ? div_rho0sq<Spec>(S, S.gamma * xj.w * 2.0f * pi) : div_rho0sq<Spec>(S, xj.w * 2.0f * pi);
         }
         ax = ax - f * g.x;
         ay = ay - f * g.y;
@@ -323,13 +313,13 @@ __global__ __launch_bounds__(kBlock) void k_iisph_pass_a(const SceneDev* __restr
 }
 
 // ---- Jacobi pass B: A*p, relaxed pressure update, error, reductions -------------------
+template <uint32_t Spec>
 __global__ __launch_bounds__(kBlock) void k_iisph_pass_b(const SceneDev* __restrict__ scenes,
                                                          const BlockRef* __restrict__ map) {
     const BlockRef br = map[blockIdx.x];
     const SceneDev& S = scenes[br.scene];
     if (!S.state->active) return;
     __shared__ RedLds lds;
-    __shared__ bool is_last;
     const uint32_t k = br.block * kBlock + threadIdx.x;
     const uint32_t c = S.state->cur;
     double err_sum = 0.0;
@@ -351,7 +341,7 @@ __global__ __launch_bounds__(kBlock) void k_iisph_pass_b(const SceneDev* __restr
             const float4 xj = pos[j];
             float dx, dy, dz, d2, len;
             Grad g;
-            pair_geometry(S, xi, xj, dx, dy, dz, d2, len, g);
+            pair_geometry<Spec>(S, xi, xj, dx, dy, dz, d2, len, g);
             if (!(ent & kBoundaryBit)) {
                 const float4 aj = acc[j];
                 ap = ap + xj.w * dot3(ai.x - aj.x, ai.y - aj.y, ai.z - aj.z, g.x, g.y, g.z);
@@ -383,9 +373,16 @@ __global__ __launch_bounds__(kBlock) void k_iisph_pass_b(const SceneDev* __restr
     }
     const Partial part = block_reduce(err_sum, counted, a2, v2, lds);
     if (threadIdx.x == 0) S.partial[br.block] = part;
-    const uint32_t nblk = scene_fluid_blocks(S);
-    if (!last_block(&S.state->done[1], nblk, is_last)) return;
-    const Partial tot = reduce_partials(S.partial, nblk, lds);
+}
+
+// convergence decision of one Jacobi iteration (IISPHFluidSolver3D.hpp:463-522),
+// one block per scene over the pass-B partials in block order
+__global__ __launch_bounds__(kBlock) void k_iisph_converge(const SceneDev* __restrict__ scenes,
+                                                           const BlockRef* __restrict__ map) {
+    const SceneDev& S = scenes[map[blockIdx.x].scene];
+    if (!S.state->active) return;
+    __shared__ RedLds lds;
+    const Partial tot = reduce_partials(S.partial, scene_fluid_blocks(S), lds);
     if (threadIdx.x == 0) {
         SceneState* st = S.state;
         const uint64_t it = st->iterations;
@@ -397,11 +394,11 @@ __global__ __launch_bounds__(kBlock) void k_iisph_pass_b(const SceneDev* __restr
         st->max_v2 = tot.max_v2;
         const bool reached_min = it >= (uint64_t)S.min_it - 1ull;  // size_t arithmetic as in the reference
         if ((reached_min && avg <= (double)S.max_err) || it + 1 >= (uint64_t)S.max_it) st->active = 0;
-        __threadfence();
     }
 }
 
 // ---- SESPH --------------------------------------------------------------------------
+template <uint32_t Spec>
 __global__ __launch_bounds__(kBlock) void k_sesph_density(const SceneDev* __restrict__ scenes,
                                                           const BlockRef* __restrict__ map) {
     const BlockRef br = map[blockIdx.x];
@@ -423,9 +420,9 @@ __global__ __launch_bounds__(kBlock) void k_sesph_density(const SceneDev* __rest
         const float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
         const float len = __builtin_sqrtf(dot3(dx, dy, dz, dx, dy, dz));
         if (sl && (ent & kBoundaryBit))
-            rho = rho + S.gamma1 * xj.w * kernel_w(S, len);
+            rho = rho + S.gamma1 * xj.w * kernel_w<Spec>(S, len);
         else
-            rho = rho + xj.w * kernel_w(S, len);
+            rho = rho + xj.w * kernel_w<Spec>(S, len);
     }
     float p = S.stiffness * (rho / S.rest_density - 1.0f);
     p = p < 0.0f ? 0.0f : p;  // std::max(pressure, 0.0f)
@@ -435,6 +432,7 @@ __global__ __launch_bounds__(kBlock) void k_sesph_density(const SceneDev* __rest
     S.pres[c][i] = p;
 }
 
+template <uint32_t Spec>
 __global__ __launch_bounds__(kBlock) void k_sesph_accel(const SceneDev* __restrict__ scenes,
                                                         const BlockRef* __restrict__ map) {
     const BlockRef br = map[blockIdx.x];
@@ -463,7 +461,7 @@ __global__ __launch_bounds__(kBlock) void k_sesph_accel(const SceneDev* __restri
         const float4 vj = vel[j];
         float dx, dy, dz, d2, len;
         Grad g;
-        pair_geometry(S, xi, xj, dx, dy, dz, d2, len, g);
+        pair_geometry<Spec>(S, xi, xj, dx, dy, dz, d2, len, g);
         const float nd = vj.w;
         if (nd != 0.0f) {
             const float vx = vi.x - vj.x, vy = vi.y - vj.y, vz = vi.z - vj.z;
@@ -504,7 +502,6 @@ __global__ __launch_bounds__(kBlock) void k_cfl_all(const SceneDev* __restrict__
     const BlockRef br = map[blockIdx.x];
     const SceneDev& S = scenes[br.scene];
     __shared__ RedLds lds;
-    __shared__ bool is_last;
     const uint32_t i = br.block * kBlock + threadIdx.x;
     const uint32_t c = S.state->cur;
     float a2 = 0.0f, v2 = 0.0f;
@@ -518,14 +515,18 @@ __global__ __launch_bounds__(kBlock) void k_cfl_all(const SceneDev* __restrict__
     }
     const Partial part = block_reduce(0.0, 0u, a2, v2, lds);
     if (threadIdx.x == 0) S.partial[br.block] = part;
-    if (!last_block(&S.state->done[3], S.nblocks, is_last)) return;
+}
+
+__global__ __launch_bounds__(kBlock) void k_cfl_finalize(const SceneDev* __restrict__ scenes,
+                                                         const BlockRef* __restrict__ map) {
+    const SceneDev& S = scenes[map[blockIdx.x].scene];
+    __shared__ RedLds lds;
     const Partial tot = reduce_partials(S.partial, S.nblocks, lds);
     if (threadIdx.x == 0) {
         S.state->max_a2 = fmaxf(tot.max_a2, S.inactive_a2);
         S.state->max_v2 = fmaxf(tot.max_v2, S.inactive_v2);
         S.state->iterations = 0;
         S.state->avg_err = 0.0;
-        __threadfence();
     }
 }
 
@@ -535,7 +536,6 @@ __global__ __launch_bounds__(kBlock) void k_integrate(const SceneDev* __restrict
     const BlockRef br = map[blockIdx.x];
     const SceneDev& S = scenes[br.scene];
     __shared__ int32_t blo[3][kBlock / 64], bhi[3][kBlock / 64];
-    __shared__ bool is_last;
     const uint32_t i = br.block * kBlock + threadIdx.x;
     const uint32_t c = S.state->cur;
     const float max_v = (float)sqrt((double)S.state->max_v2);
@@ -606,54 +606,91 @@ __global__ __launch_bounds__(kBlock) void k_integrate(const SceneDev* __restrict
         bp.hi[3] = 0;
         S.box[br.block] = bp;
     }
-    if (!last_block(&S.state->done[2], S.nblocks, is_last)) return;
-    if (threadIdx.x == 0) {
-        int32_t l[3] = {INT32_MAX, INT32_MAX, INT32_MAX}, h[3] = {INT32_MIN, INT32_MIN, INT32_MIN};
-        for (uint32_t b = 0; b < S.nblocks; b++) {
-            const BoxPartial bp = S.box[b];
-            for (int a = 0; a < 3; a++) {
-                l[a] = min(l[a], bp.lo[a]);
-                h[a] = max(h[a], bp.hi[a]);
-            }
+}
+
+// bounding box of the cells of the next search and the step's dt statistics
+__global__ __launch_bounds__(kBlock) void k_integrate_finalize(const SceneDev* __restrict__ scenes,
+                                                               const BlockRef* __restrict__ map) {
+    const SceneDev& S = scenes[map[blockIdx.x].scene];
+    __shared__ int32_t blo[3][kBlock / 64], bhi[3][kBlock / 64];
+    int32_t l[3] = {INT32_MAX, INT32_MAX, INT32_MAX}, h[3] = {INT32_MIN, INT32_MIN, INT32_MIN};
+    for (uint32_t b = threadIdx.x; b < S.nblocks; b += kBlock) {
+        const BoxPartial bp = S.box[b];
+        for (int a = 0; a < 3; a++) {
+            l[a] = min(l[a], bp.lo[a]);
+            h[a] = max(h[a], bp.hi[a]);
         }
+    }
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            l[a] = min(l[a], __shfl_xor(l[a], o, 64));
+            h[a] = max(h[a], __shfl_xor(h[a], o, 64));
+        }
+        if (__lane_id() == 0) {
+            blo[a][threadIdx.x / 64] = l[a];
+            bhi[a][threadIdx.x / 64] = h[a];
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
         SceneState* st = S.state;
         for (int a = 0; a < 3; a++) {
-            st->bbox_min[a] = l[a];
-            st->bbox_max[a] = h[a];
+            int32_t lo2 = INT32_MAX, hi2 = INT32_MIN;
+            for (int w = 0; w < kBlock / 64; w++) {
+                lo2 = min(lo2, blo[a][w]);
+                hi2 = max(hi2, bhi[a][w]);
+            }
+            st->bbox_min[a] = lo2;
+            st->bbox_max[a] = hi2;
         }
-        st->dt_actual = corrected;
+        const float max_v = (float)sqrt((double)st->max_v2);
+        const float max_a = (float)sqrt((double)st->max_a2);
+        st->dt_actual = fminf(cfl_timestep(S, max_a, max_v), S.dt);
         st->max_v = max_v;
         st->max_a = max_a;
-        __threadfence();
     }
 }
 
 // ---- launchers ------------------------------------------------------------------------
+#define FSA_SPEC_DISPATCH(kernel, grid, map)                                                        \
+    do {                                                                                            \
+        switch (L.spec) {                                                                           \
+            case 3: kernel<3><<<grid, kBlock, 0, st>>>(L.scenes, map); break;                       \
+            case 1: kernel<1><<<grid, kBlock, 0, st>>>(L.scenes, map); break;                       \
+            default: kernel<0><<<grid, kBlock, 0, st>>>(L.scenes, map); break;                      \
+        }                                                                                           \
+    } while (0)
+
 void launch_iisph_prepare(const LaunchSet& L, bool boundary_pass, hipStream_t st) {
-    if (L.nb_fluid) k_iisph_prepare<<<L.nb_fluid, kBlock, 0, st>>>(L.scenes, L.map_fluid);
+    if (L.nb_fluid) FSA_SPEC_DISPATCH(k_iisph_prepare, L.nb_fluid, L.map_fluid);
     if (boundary_pass && L.nb_all) k_iisph_prepare_boundary<<<L.nb_all, kBlock, 0, st>>>(L.scenes, L.map_all);
 }
 void launch_iisph_diagonal(const LaunchSet& L, hipStream_t st) {
-    if (L.nb_fluid) k_iisph_diagonal<<<L.nb_fluid, kBlock, 0, st>>>(L.scenes, L.map_fluid);
+    if (L.nb_fluid) FSA_SPEC_DISPATCH(k_iisph_diagonal, L.nb_fluid, L.map_fluid);
 }
 void launch_iisph_begin_loop(const LaunchSet& L, hipStream_t st) {
     k_iisph_begin_loop<<<L.nb_scene, 64, 0, st>>>(L.scenes, L.map_scene);
 }
 void launch_iisph_pass_a(const LaunchSet& L, hipStream_t st) {
-    if (L.nb_fluid) k_iisph_pass_a<<<L.nb_fluid, kBlock, 0, st>>>(L.scenes, L.map_fluid);
+    if (L.nb_fluid) FSA_SPEC_DISPATCH(k_iisph_pass_a, L.nb_fluid, L.map_fluid);
 }
 void launch_iisph_pass_b(const LaunchSet& L, hipStream_t st) {
-    if (L.nb_fluid) k_iisph_pass_b<<<L.nb_fluid, kBlock, 0, st>>>(L.scenes, L.map_fluid);
+    if (L.nb_fluid) FSA_SPEC_DISPATCH(k_iisph_pass_b, L.nb_fluid, L.map_fluid);
+    k_iisph_converge<<<L.nb_scene, kBlock, 0, st>>>(L.scenes, L.map_scene);
 }
 void launch_sesph_density(const LaunchSet& L, hipStream_t st) {
-    if (L.nb_fluid) k_sesph_density<<<L.nb_fluid, kBlock, 0, st>>>(L.scenes, L.map_fluid);
+    if (L.nb_fluid) FSA_SPEC_DISPATCH(k_sesph_density, L.nb_fluid, L.map_fluid);
 }
 void launch_sesph_accel(const LaunchSet& L, hipStream_t st) {
-    if (L.nb_fluid) k_sesph_accel<<<L.nb_fluid, kBlock, 0, st>>>(L.scenes, L.map_fluid);
+    if (L.nb_fluid) FSA_SPEC_DISPATCH(k_sesph_accel, L.nb_fluid, L.map_fluid);
     k_cfl_all<<<L.nb_all, kBlock, 0, st>>>(L.scenes, L.map_all);
+    k_cfl_finalize<<<L.nb_scene, kBlock, 0, st>>>(L.scenes, L.map_scene);
 }
 void launch_integrate(const LaunchSet& L, hipStream_t st) {
     k_integrate<<<L.nb_all, kBlock, 0, st>>>(L.scenes, L.map_all);
+    k_integrate_finalize<<<L.nb_scene, kBlock, 0, st>>>(L.scenes, L.map_scene);
 }
 
 }  // namespace fsa
