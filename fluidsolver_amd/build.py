@@ -53,7 +53,25 @@ def build(verbose=False):
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, LIB)
+    build_dropin()
     return LIB
+
+
+DROPIN_SRC = os.path.join(os.path.dirname(HERE), "tests", "cpp", "dropin_dam.cpp")
+DROPIN_BIN = os.path.join(LIBDIR, "dropin_dam")
+
+
+def build_dropin():
+    """The C++ drop-in example, a plain g++ program linked against the library."""
+    if not os.path.exists(DROPIN_SRC):
+        return None
+    inc = os.path.join(os.path.dirname(HERE), "include")
+    cmd = ["g++", "-O2", "-std=c++17", "-I", inc, DROPIN_SRC, "-o", DROPIN_BIN, "-L", LIBDIR,
+           "-lfluidsolver_amd", "-Wl,-rpath,$ORIGIN"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"g++ failed for the drop-in example:\n{r.stdout}\n{r.stderr}")
+    return DROPIN_BIN
 
 
 if __name__ == "__main__":

@@ -12,6 +12,7 @@
 #include <cstdio>
 #include <cstring>
 #include <exception>
+#include <map>
 #include <mutex>
 #include <new>
 #include <stdexcept>
@@ -86,6 +87,21 @@ struct fsa_context_s {
     bool graph = false;
     uint32_t block_override = 0;
     std::vector<fsa_scene_s*> scenes;  // live scenes, destroyed with the context
+    // per-scene device state lives in one pool so a batch reads it back in one copy
+    static constexpr uint32_t kMaxScenes = 4096;
+    SceneState* d_states = nullptr;
+    SceneState* h_states = nullptr;  // pinned mirror
+    std::vector<uint32_t> free_slots;
+    std::map<std::vector<fsa_scene_s*>, struct BatchSet*> batches;
+};
+
+// Launch set over several scenes (one table entry and block-map range each).
+struct BatchSet {
+    std::vector<fsa_scene_s*> scenes;
+    SceneDev* d_table = nullptr;
+    std::vector<SceneDev> h_table;
+    LaunchSet ls{};
+    std::vector<BlockRef*> owned;
 };
 
 struct fsa_scene_s {
@@ -102,13 +118,15 @@ struct fsa_scene_s {
     uint64_t bytes = 0;
     uint4* cells = nullptr;
     uint64_t cells_cap = 0;
-    SceneState* h_state = nullptr;  // pinned
+    SceneState* h_state = nullptr;  // pinned mirror of this scene's slot in the context pool
+    uint32_t slot = 0;
     LaunchSet single{};
     std::vector<BlockRef*> map_allocs;
     bool searched = false;
     bool boundary_dirty = true;
     uint32_t last_iterations = 2;
     bool lists_cached = false;
+    bool lists_valid = false;  // lists of the last search (valid until the next search or upload)
     std::vector<uint64_t> csr_off, csr_cols;
 };
 
@@ -192,12 +210,12 @@ uint64_t plan_bytes(const Plan& p) {
     for (int k = 0; k < 4; k++) add(4 * n);    // keys, vals x2
     add(4 * (kRadix * std::max<uint64_t>(p.ntiles, 1) + kRadix));
     add(4 * std::max<uint64_t>(p.nf, 1));      // fluid_idx
+    add(4 * n);                                // rank_of
     add(4 * std::max<uint64_t>(p.nblocks, 1));
     add(4ull * kMaxNeighbors * p.stride);      // nbr
     add(4 * std::max<uint64_t>(p.nf, 1));      // nbr_cnt
     add(sizeof(Partial) * std::max<uint64_t>(p.nblocks, 1));
     add(sizeof(BoxPartial) * std::max<uint64_t>(p.nblocks, 1));
-    add(sizeof(SceneState));
     return b;
 }
 
@@ -227,12 +245,12 @@ void carve(fsa_scene_s* s, const Plan& p) {
     for (int k = 0; k < 2; k++) h.vals[k] = (uint32_t*)take(4 * n);
     h.hist = (uint32_t*)take(4 * (kRadix * std::max<uint64_t>(p.ntiles, 1) + kRadix));
     h.fluid_idx = (uint32_t*)take(4 * std::max<uint64_t>(p.nf, 1));
+    h.rank_of = (uint32_t*)take(4 * n);
     h.block_count = (uint32_t*)take(4 * std::max<uint64_t>(p.nblocks, 1));
     h.nbr = (uint32_t*)take(4ull * kMaxNeighbors * p.stride);
     h.nbr_cnt = (uint32_t*)take(4 * std::max<uint64_t>(p.nf, 1));
     h.partial = (Partial*)take(sizeof(Partial) * std::max<uint64_t>(p.nblocks, 1));
     h.box = (BoxPartial*)take(sizeof(BoxPartial) * std::max<uint64_t>(p.nblocks, 1));
-    h.state = (SceneState*)take(sizeof(SceneState));
     h.n = (uint32_t)p.n;
     h.nf = (uint32_t)p.nf;
     h.list_stride = (uint32_t)p.stride;
@@ -351,6 +369,19 @@ void read_state(fsa_scene_s* s) {
     HIP_CHECK(hipStreamSynchronize(s->ctx->stream));
 }
 
+// one copy of the pool range covering every scene of the list
+void read_states(fsa_context ctx, const std::vector<fsa_scene_s*>& v) {
+    if (v.empty()) return;
+    uint32_t lo = UINT32_MAX, hi = 0;
+    for (auto* s : v) {
+        lo = std::min(lo, s->slot);
+        hi = std::max(hi, s->slot);
+    }
+    HIP_CHECK(hipMemcpyAsync(ctx->h_states + lo, ctx->d_states + lo, sizeof(SceneState) * (hi - lo + 1),
+                             hipMemcpyDeviceToHost, ctx->stream));
+    HIP_CHECK(hipStreamSynchronize(ctx->stream));
+}
+
 // ---- timing ---------------------------------------------------------------------------
 // Phase timing records HIP events on the context stream without waiting; the
 // elapsed times are collected when the pool fills or when they are read.
@@ -388,7 +419,7 @@ void search_set(fsa_context ctx, const LaunchSet& L, uint32_t passes) {
     launch_keys(L, ctx->stream);
     launch_sort(L, passes, ctx->stream);
     launch_reorder(L, passes & 1u, ctx->stream);
-    launch_lists(L, ctx->stream);
+    launch_lists(L, passes & 1u, ctx->stream);
 }
 
 void simulate_set(fsa_context ctx, const LaunchSet& L, std::vector<fsa_scene_s*>& scenes, bool sesph,
@@ -432,10 +463,9 @@ void simulate_set(fsa_context ctx, const LaunchSet& L, std::vector<fsa_scene_s*>
             }
             if (enq >= max_it) break;
             bool any_active = false;
-            for (auto* s : scenes) {
-                read_state(s);
+            read_states(ctx, scenes);
+            for (auto* s : scenes)
                 if (s->h_state->active) any_active = true;
-            }
             if (!any_active) break;
             chunk = 4;
         }
@@ -479,10 +509,11 @@ void do_find_neighbors(fsa_scene_s* s) {
     read_state(s);
     check_status(s);
     s->searched = true;
+    s->lists_valid = true;
     s->lists_cached = false;
 }
 
-void do_simulate(fsa_scene_s* s, float dt, fsa_step_stats* stats) {
+void pre_simulate(fsa_scene_s* s, float dt) {
     if (!s->searched) throw FsaError(FSA_E_STATE, "simulate called before find_neighbors");
     if (!(dt > 0.0f)) throw FsaError(FSA_E_INVALID, "desired time step must be > 0");
     s->hd.dt = dt;
@@ -499,17 +530,95 @@ void do_simulate(fsa_scene_s* s, float dt, fsa_step_stats* stats) {
         s->hd.inactive_a2 = a2;
         s->hd.inactive_v2 = v2;
     }
+}
+
+// after the step's final state read
+void post_simulate(fsa_scene_s* s, fsa_step_stats* stats) {
+    s->boundary_dirty = false;
+    if (s->desc.solver == FSA_SOLVER_IISPH) s->last_iterations = std::max<uint32_t>(s->h_state->iterations, 1);
+    fill_stats(s, stats);
+    s->searched = false;
+    check_status(s);
+}
+
+void do_simulate(fsa_scene_s* s, float dt, fsa_step_stats* stats) {
+    pre_simulate(s, dt);
     upload_scene_table(s);
     std::vector<fsa_scene_s*> one{s};
     const bool bpass = (s->hd.flags & kFlagDynamicBoundary) || s->boundary_dirty;
     simulate_set(s->ctx, s->single, one, s->desc.solver == FSA_SOLVER_SESPH, bpass);
-    s->boundary_dirty = false;
     read_state(s);
-    if (s->desc.solver == FSA_SOLVER_IISPH) s->last_iterations = std::max<uint32_t>(s->h_state->iterations, 1);
-    fill_stats(s, stats);
-    s->searched = false;
-    s->lists_cached = false;
-    check_status(s);
+    post_simulate(s, stats);
+}
+
+// ---- batching -------------------------------------------------------------------------
+BatchSet* batch_for(fsa_context ctx, const std::vector<fsa_scene_s*>& v) {
+    auto it = ctx->batches.find(v);
+    if (it != ctx->batches.end()) return it->second;
+    auto* b = new BatchSet();
+    b->scenes = v;
+    b->h_table.resize(v.size());
+    HIP_CHECK(hipMalloc(&b->d_table, sizeof(SceneDev) * v.size()));
+    MapBuilder mb;
+    for (uint32_t q = 0; q < v.size(); q++) mb.add(q, v[q]->hd);
+    build_launch_set(mb, b->d_table, b->ls, b->owned);
+    ctx->batches[v] = b;
+    return b;
+}
+
+void free_batch(BatchSet* b) {
+    (void)hipFree(b->d_table);
+    for (auto* m : b->owned) (void)hipFree(m);
+    delete b;
+}
+
+void drop_batches_of(fsa_context ctx, fsa_scene_s* s) {
+    for (auto it = ctx->batches.begin(); it != ctx->batches.end();) {
+        if (std::find(it->first.begin(), it->first.end(), s) != it->first.end()) {
+            free_batch(it->second);
+            it = ctx->batches.erase(it);
+        } else {
+            ++it;
+        }
+    }
+}
+
+void upload_batch_table(fsa_context ctx, BatchSet* b) {
+    uint32_t spec = kSpecHOne | kSpecRho0SqOne;
+    for (uint32_t q = 0; q < b->scenes.size(); q++) {
+        b->h_table[q] = b->scenes[q]->hd;
+        spec &= spec_of(b->scenes[q]->hd);
+    }
+    b->ls.spec = spec;
+    HIP_CHECK(hipMemcpyAsync(b->d_table, b->h_table.data(), sizeof(SceneDev) * b->h_table.size(),
+                             hipMemcpyHostToDevice, ctx->stream));
+}
+
+// One step of every scene in v (same solver), in shared launches.
+void batch_step_group(fsa_context ctx, const std::vector<fsa_scene_s*>& v, std::vector<fsa_step_stats*>& out) {
+    BatchSet* b = batch_for(ctx, v);
+    uint32_t passes = 1;
+    for (auto* s : v) {
+        prepare_grid(s);
+        passes = std::max(passes, sort_passes(s->hd));
+    }
+    upload_batch_table(ctx, b);
+    search_set(ctx, b->ls, passes);
+    read_states(ctx, v);
+    bool bpass = false;
+    for (auto* s : v) {
+        check_status(s);
+        s->searched = true;
+        s->lists_valid = true;
+        s->lists_cached = false;
+        pre_simulate(s, s->desc.timestep);
+        bpass = bpass || (s->hd.flags & kFlagDynamicBoundary) || s->boundary_dirty;
+    }
+    upload_batch_table(ctx, b);
+    std::vector<fsa_scene_s*> vv = v;
+    simulate_set(ctx, b->ls, vv, v[0]->desc.solver == FSA_SOLVER_SESPH, bpass);
+    read_states(ctx, v);
+    for (uint32_t q = 0; q < v.size(); q++) post_simulate(v[q], out[q]);
 }
 
 // ---- upload / download ------------------------------------------------------------------
@@ -617,6 +726,7 @@ void do_upload(fsa_scene_s* s, const fsa_particle_view* v, bool first) {
     s->searched = false;
     s->boundary_dirty = true;
     s->lists_cached = false;
+    s->lists_valid = false;
 }
 
 struct HostCopy {
@@ -729,7 +839,7 @@ void do_download_iisph(fsa_scene_s* s, void* rec, uint64_t stride) {
 
 void build_csr(fsa_scene_s* s) {
     if (s->lists_cached) return;
-    if (!s->searched) throw FsaError(FSA_E_STATE, "neighbor lists are only valid after find_neighbors");
+    if (!s->lists_valid) throw FsaError(FSA_E_STATE, "no neighbor lists: call find_neighbors first");
     read_state(s);
     const uint32_t c = s->h_state->cur;
     const uint32_t n = s->n, nf = s->nf;
@@ -780,7 +890,11 @@ void destroy_scene(fsa_scene_s* s) {
     }
     if (s->d_table) (void)hipFree(s->d_table);
     for (auto* m : s->map_allocs) (void)hipFree(m);
-    if (s->h_state) (void)hipHostFree(s->h_state);
+    {
+        std::lock_guard<std::mutex> g(s->ctx->mu);
+        if (s->h_state) s->ctx->free_slots.push_back(s->slot);
+        drop_batches_of(s->ctx, s);
+    }
     delete s;
 }
 
@@ -835,6 +949,11 @@ FSA_API int32_t fsa_context_create(int32_t device, uint64_t budget, fsa_context*
             delete c;
             throw FsaError(FSA_E_DEVICE, "hipStreamCreate failed");
         }
+        HIP_CHECK(hipMalloc(&c->d_states, sizeof(SceneState) * fsa_context_s::kMaxScenes));
+        HIP_CHECK(hipMemset(c->d_states, 0, sizeof(SceneState) * fsa_context_s::kMaxScenes));
+        HIP_CHECK(hipHostMalloc((void**)&c->h_states, sizeof(SceneState) * fsa_context_s::kMaxScenes,
+                                hipHostMallocDefault));
+        for (uint32_t k = fsa_context_s::kMaxScenes; k > 0; k--) c->free_slots.push_back(k - 1);
         c->ev_pool.resize(4096);
         c->ev_phase.resize(2048);
         for (auto& e : c->ev_pool) HIP_CHECK(hipEventCreate(&e));
@@ -853,6 +972,10 @@ FSA_API int32_t fsa_context_destroy(fsa_context c) {
             live = c->scenes;
         }
         for (auto* s : live) destroy_scene(s);
+        for (auto& kv : c->batches) free_batch(kv.second);
+        c->batches.clear();
+        if (c->d_states) (void)hipFree(c->d_states);
+        if (c->h_states) (void)hipHostFree(c->h_states);
         for (auto& e : c->ev_pool) (void)hipEventDestroy(e);
         (void)hipStreamDestroy(c->stream);
         delete c;
@@ -915,9 +1038,16 @@ FSA_API int32_t fsa_scene_create(fsa_context c, const fsa_scene_desc* desc, cons
         }
         s->bytes = bytes;
         c->in_use += bytes;
-        HIP_CHECK(hipHostMalloc((void**)&s->h_state, sizeof(SceneState), hipHostMallocDefault));
+        {
+            std::lock_guard<std::mutex> g(c->mu);
+            if (c->free_slots.empty()) throw FsaError(FSA_E_BUDGET, "too many scenes on this context");
+            s->slot = c->free_slots.back();
+            c->free_slots.pop_back();
+        }
+        s->h_state = &c->h_states[s->slot];
         std::memset(s->h_state, 0, sizeof(SceneState));
         carve(s, p);
+        s->hd.state = &c->d_states[s->slot];
         fill_params(s);
         HIP_CHECK(hipMalloc(&s->d_table, sizeof(SceneDev)));
         MapBuilder mb;
@@ -1010,7 +1140,7 @@ FSA_API int32_t fsa_scene_export_neighbors(fsa_scene s, uint64_t* off, uint64_t*
 FSA_API int32_t fsa_scene_neighbor_stats(fsa_scene s, uint64_t* total, uint64_t* dev_bytes, uint64_t* ref_bytes) {
     return guarded([&] {
         if (!s) throw FsaError(FSA_E_INVALID, "scene is NULL");
-        if (!s->searched) throw FsaError(FSA_E_STATE, "neighbor lists are only valid after find_neighbors");
+        if (!s->lists_valid) throw FsaError(FSA_E_STATE, "no neighbor lists: call find_neighbors first");
         const uint32_t nf = s->nf;
         std::vector<uint32_t> cnt(nf), nbr((size_t)kMaxNeighbors * s->hd.list_stride);
         hipStream_t st = s->ctx->stream;
@@ -1067,12 +1197,19 @@ FSA_API int32_t fsa_batch_step(fsa_context c, const fsa_scene* scenes, uint32_t 
         std::vector<fsa_scene_s*> v(scenes, scenes + n);
         for (auto* s : v)
             if (!s || s->ctx != c) throw FsaError(FSA_E_INVALID, "scene of another context");
-        for (uint32_t k = 0; k < steps; k++) {
-            for (uint32_t q = 0; q < n; q++) {
-                do_find_neighbors(v[q]);
-                do_simulate(v[q], v[q]->desc.timestep, stats ? &stats[q] : nullptr);
-            }
+        std::vector<fsa_scene_s*> groups[2];
+        std::vector<fsa_step_stats*> gstats[2];
+        for (uint32_t q = 0; q < n; q++) {
+            const int g = v[q]->desc.solver == FSA_SOLVER_SESPH ? 1 : 0;
+            if (std::find(groups[g].begin(), groups[g].end(), v[q]) != groups[g].end())
+                throw FsaError(FSA_E_INVALID, "a scene appears twice in the batch");
+            groups[g].push_back(v[q]);
+            gstats[g].push_back(stats ? &stats[q] : nullptr);
         }
+        HIP_CHECK(hipSetDevice(c->device));
+        for (uint32_t k = 0; k < steps; k++)
+            for (int g = 0; g < 2; g++)
+                if (!groups[g].empty()) batch_step_group(c, groups[g], gstats[g]);
     });
 }
 

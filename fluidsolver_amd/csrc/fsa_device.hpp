@@ -56,7 +56,7 @@ struct alignas(16) SceneState {
     float dt_actual, max_a, max_v;
     uint32_t err_count;
     int32_t bbox_min[4], bbox_max[4];  // cell bounding box after the last integration
-    uint32_t max_count, pad;
+    uint32_t pad0, pad1;
     uint64_t total_neighbors;
 };
 
@@ -123,6 +123,7 @@ struct alignas(16) SceneDev {
     uint32_t* hist;        // kRadix * ntiles tile offsets + kRadix digit totals
     uint4* cells;          // {start, end, stamp, -}
     uint32_t* fluid_idx;   // combined index of fluid rank k
+    uint32_t* rank_of;     // fluid rank of combined index i (fluid particles only)
     uint32_t* block_count; // fluid count per kBlock-block, scanned in place
     uint32_t* nbr;         // neighbor entries, entry e of fluid rank k at e * list_stride + k
     uint32_t* nbr_cnt;

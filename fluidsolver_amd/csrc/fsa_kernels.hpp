@@ -25,7 +25,7 @@ struct LaunchSet {
 void launch_keys(const LaunchSet& L, hipStream_t st);
 void launch_sort(const LaunchSet& L, uint32_t passes, hipStream_t st);
 void launch_reorder(const LaunchSet& L, uint32_t sbuf, hipStream_t st);
-void launch_lists(const LaunchSet& L, hipStream_t st);
+void launch_lists(const LaunchSet& L, uint32_t sbuf, hipStream_t st);
 
 // IISPH / SESPH
 void launch_iisph_prepare(const LaunchSet& L, bool boundary_pass, hipStream_t st);

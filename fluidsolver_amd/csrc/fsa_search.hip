@@ -249,7 +249,11 @@ __global__ __launch_bounds__(kBlock) void k_compact(const SceneDev* __restrict__
     __shared__ uint32_t lw[kBlock / 64];
     uint32_t tot;
     const uint32_t ex = block_exclusive_scan(fluid ? 1u : 0u, lw, tot);
-    if (fluid) S.fluid_idx[S.block_count[br.block] + ex] = i;
+    if (fluid) {
+        const uint32_t k = S.block_count[br.block] + ex;
+        S.fluid_idx[k] = i;
+        S.rank_of[i] = k;
+    }
 }
 
 // ---- 5. cell table -----------------------------------------------------------
@@ -288,103 +292,171 @@ __global__ __launch_bounds__(kBlock) void k_cell_table(const SceneDev* __restric
 }
 
 // ---- 6. neighbor lists ---------------------------------------------------------
-template <int N>
-__device__ __forceinline__ void bitonic_sort(uint32_t (&a)[N]) {
+// One wave per 64 consecutive sorted particles processes every cell whose run
+// starts among them.  For a cell, the wave sorts the runs of the 27 neighbor
+// cells by start (= by cell key, the reference's std::sort of the 27 keys),
+// stages their particles in LDS in ascending combined index, and filters them
+// for each fluid particle of the cell with a ballot; the ballot prefix keeps
+// every list in ascending order.
+constexpr int kWaves = kBlock / 64;
+constexpr int kListChunk = 256;
+
+__device__ __forceinline__ uint32_t wave_sort_u32(uint32_t v) {
+    // bitonic sort across the 64 lanes, ascending by lane
+    const uint32_t lane = __lane_id();
 #pragma unroll
-    for (int k = 2; k <= N; k <<= 1) {
+    for (int k = 2; k <= 64; k <<= 1) {
 #pragma unroll
         for (int j = k >> 1; j > 0; j >>= 1) {
-#pragma unroll
-            for (int i = 0; i < N; i++) {
-                const int l = i ^ j;
-                if (l > i) {
-                    const uint32_t x = a[i], y = a[l];
-                    const bool up = (i & k) == 0;
-                    a[i] = up ? min(x, y) : max(x, y);
-                    a[l] = up ? max(x, y) : min(x, y);
-                }
-            }
+            const uint32_t o = __shfl_xor(v, j, 64);
+            const bool up = (lane & k) == 0;
+            const bool lower = (lane & j) == 0;
+            v = (up == lower) ? min(v, o) : max(v, o);
         }
     }
+    return v;
 }
 
-// Distance from x to the face of its own cell in direction d (d = -1, 0, +1).
-__device__ __forceinline__ float face_gap(float x, int32_t c, int d, float r) {
-    const float lo = (float)c * r;
-    return d < 0 ? x - lo : (d > 0 ? (lo + r) - x : 0.0f);
+__device__ __forceinline__ uint32_t wave_exclusive_scan(uint32_t v, uint32_t& total) {
+    const uint32_t lane = __lane_id();
+    uint32_t x = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t y = __shfl_up(x, o, 64);
+        if (lane >= (uint32_t)o) x += y;
+    }
+    total = __shfl(x, 63, 64);
+    return x - v;
+}
+
+__device__ __forceinline__ void wave_sync_lds() {
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+    __builtin_amdgcn_wave_barrier();
 }
 
 __global__ __launch_bounds__(kBlock) void k_lists(const SceneDev* __restrict__ scenes,
-                                                  const BlockRef* __restrict__ map) {
+                                                  const BlockRef* __restrict__ map, uint32_t sbuf) {
     const BlockRef br = map[blockIdx.x];
     const SceneDev& S = scenes[br.scene];
-    __shared__ uint32_t s_run[27][kBlock];  // sorted runs of this thread, column per thread
-    const uint32_t t = threadIdx.x;
-    const uint32_t k = br.block * kBlock + t;
-    if (k >= S.nf) return;
+    __shared__ float4 s_pos[kWaves][kListChunk];
+    __shared__ uint32_t s_j[kWaves][kListChunk];
+    __shared__ uint32_t s_cnt[kWaves][64];
+    const uint32_t wave = threadIdx.x / 64;
+    const uint32_t lane = __lane_id();
+    const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+    const uint32_t base = br.block * kBlock + wave * 64;
+    if (base >= S.n) return;
     const uint32_t c = S.state->cur;
-    const uint32_t i = S.fluid_idx[k];
-    const float4 p = S.pos[c][i];
-    const float r = S.search_radius;
-    const int32_t cx = cell_of(p.x, r);
-    const int32_t cy = cell_of(p.y, r);
-    const int32_t cz = cell_of(p.z, r);
-    // cells whose box lies farther than r (with a rounding margin) hold no neighbor
-    const float cull2 = S.r2 * 1.0001f;
-    float gx[3], gy[3], gz[3];
-#pragma unroll
-    for (int d = 0; d < 3; d++) {
-        const float ax = face_gap(p.x, cx, d - 1, r), ay = face_gap(p.y, cy, d - 1, r), az = face_gap(p.z, cz, d - 1, r);
-        gx[d] = ax * ax;
-        gy[d] = ay * ay;
-        gz[d] = az * az;
-    }
-    uint32_t run[32];
-#pragma unroll
-    for (int s = 27; s < 32; s++) run[s] = kEmptyCell;
-#pragma unroll
-    for (int s = 0; s < 27; s++) {
-        const int dx = s / 9, dy = (s / 3) % 3, dz = s % 3;
-        run[s] = kEmptyCell;
-        uint32_t lin;
-        if (gx[dx] + gy[dy] + gz[dz] <= cull2 && cell_linear(S, cx + dx - 1, cy + dy - 1, cz + dz - 1, lin)) {
-            const uint4 e = S.cells[lin];
-            if (e.z == S.stamp) run[s] = (e.x << 5) | (uint32_t)s;
-        }
-    }
-    bitonic_sort<32>(run);
-#pragma unroll
-    for (int q = 0; q < 27; q++) s_run[q][t] = run[q];
-    const float r2 = S.r2;
-    const uint32_t stride = S.list_stride;
-    uint32_t cnt = 0;
-    bool overflow = false;
     const float4* __restrict__ pos = S.pos[c];
     const uint32_t* __restrict__ idt = S.idt[c];
-    uint32_t* __restrict__ out = S.nbr + k;
-    for (int q = 0; q < 27; q++) {
-        const uint32_t rv = s_run[q][t];
-        if (rv == kEmptyCell) break;
-        const uint32_t s = rv & 31u;
-        uint32_t lin;
-        cell_linear(S, cx + (int)(s / 9) - 1, cy + (int)((s / 3) % 3) - 1, cz + (int)(s % 3) - 1, lin);
-        const uint32_t end = S.cells[lin].y;
-        for (uint32_t j = rv >> 5; j < end; j++) {
-            const float4 pj = pos[j];
-            const float ddx = pj.x - p.x, ddy = pj.y - p.y, ddz = pj.z - p.z;
-            if (dot3(ddx, ddy, ddz, ddx, ddy, ddz) <= r2) {
-                if (cnt < (uint32_t)kMaxNeighbors) {
-                    const uint32_t bnd = (idt[j] >> kTypeShift) == 1u ? kBoundaryBit : 0u;
-                    out[cnt * stride] = j | bnd;
-                    cnt++;
-                } else {
-                    overflow = true;
+    const uint32_t* __restrict__ keys = S.keys[sbuf];
+    const float r = S.search_radius;
+    const float r2 = S.r2;
+    const uint32_t stride = S.list_stride;
+    // run starts among this wave's particles
+    const uint32_t mi = base + lane;
+    bool start = false;
+    if (mi < S.n) start = mi == 0 || keys[mi - 1] != keys[mi];
+    uint64_t starts = __ballot(start);
+    bool overflow = false;
+    while (starts) {
+        const uint32_t cs = base + (uint32_t)__ffsll((unsigned long long)starts) - 1u;
+        starts &= starts - 1;
+        const float4 p0 = pos[cs];
+        const int32_t cx = cell_of(p0.x, r), cy = cell_of(p0.y, r), cz = cell_of(p0.z, r);
+        uint32_t lin0;
+        cell_linear(S, cx, cy, cz, lin0);
+        const uint32_t ce = S.cells[lin0].y;
+        bool anyf = false;
+        for (uint32_t t = cs + lane; t < ce; t += 64) anyf |= (idt[t] >> kTypeShift) == 0u;
+        if (!__any(anyf)) continue;
+        // runs of the 27 neighbor cells, sorted by start
+        uint32_t key = kEmptyCell;
+        if (lane < 27) {
+            uint32_t lin;
+            if (cell_linear(S, cx + (int)(lane / 9) - 1, cy + (int)((lane / 3) % 3) - 1, cz + (int)(lane % 3) - 1,
+                            lin)) {
+                const uint4 e = S.cells[lin];
+                if (e.z == S.stamp) key = (e.x << 5) | lane;
+            }
+        }
+        key = wave_sort_u32(key);
+        const uint32_t nvalid = (uint32_t)__popcll(__ballot(key != kEmptyCell));
+        uint32_t rs = 0, rl = 0;
+        if (key != kEmptyCell) {
+            const uint32_t sl = key & 31u;
+            uint32_t lin;
+            cell_linear(S, cx + (int)(sl / 9) - 1, cy + (int)((sl / 3) % 3) - 1, cz + (int)(sl % 3) - 1, lin);
+            rs = key >> 5;
+            rl = S.cells[lin].y - rs;
+        }
+        uint32_t total;
+        const uint32_t roff = wave_exclusive_scan(rl, total);
+        for (uint32_t pbase = cs; pbase < ce; pbase += 64) {
+            const uint32_t pi = pbase + lane;
+            const bool isf = pi < ce && (idt[pi] >> kTypeShift) == 0u;
+            const uint64_t fmask = __ballot(isf);
+            if (fmask == 0) continue;
+            for (uint32_t c0 = 0; c0 < total; c0 += kListChunk) {
+                const uint32_t nch = min((uint32_t)kListChunk, total - c0);
+                wave_sync_lds();
+                // every lane stages its candidates: run q of candidate g by binary search over the offsets
+                // uniform loop: every lane is active at each shuffle (a shuffle
+                // from an inactive source lane does not return its register)
+                for (uint32_t tb = 0; tb < nch; tb += 64) {
+                    const uint32_t t = tb + lane;
+                    const uint32_t g = c0 + t;
+                    uint32_t q = 0;
+#pragma unroll
+                    for (uint32_t step = 16; step > 0; step >>= 1) {
+                        const uint32_t probe = q + step;
+                        const uint32_t off = __shfl(roff, (int)min(probe, 63u), 64);
+                        if (probe < nvalid && off <= g) q = probe;
+                    }
+                    const uint32_t qs = __shfl(rs, (int)q, 64);
+                    const uint32_t qo = __shfl(roff, (int)q, 64);
+                    if (t < nch) {
+                        const uint32_t j = qs + (g - qo);
+                        s_pos[wave][t] = pos[j];
+                        s_j[wave][t] = j | ((idt[j] >> kTypeShift) == 1u ? kBoundaryBit : 0u);
+                    }
+                }
+                wave_sync_lds();
+                uint64_t fm = fmask;
+                while (fm) {
+                    const uint32_t b = (uint32_t)__ffsll((unsigned long long)fm) - 1u;
+                    fm &= fm - 1;
+                    const uint32_t p = pbase + b;
+                    const float4 xi = pos[p];
+                    const uint32_t k = S.rank_of[p];
+                    uint32_t cnt = c0 == 0 ? 0u : s_cnt[wave][b];
+                    uint32_t* __restrict__ out = S.nbr + k;
+                    for (uint32_t tb = 0; tb < nch; tb += 64) {
+                        const uint32_t t = tb + lane;
+                        bool pass = false;
+                        if (t < nch) {
+                            const float4 pj = s_pos[wave][t];
+                            const float dx = pj.x - xi.x, dy = pj.y - xi.y, dz = pj.z - xi.z;
+                            pass = dot3(dx, dy, dz, dx, dy, dz) <= r2;
+                        }
+                        const uint64_t m = __ballot(pass);
+                        if (pass) {
+                            const uint32_t slot = cnt + (uint32_t)__popcll(m & lt);
+                            if (slot < (uint32_t)kMaxNeighbors) out[slot * stride] = s_j[wave][t];
+                        }
+                        cnt += (uint32_t)__popcll(m);
+                    }
+                    if (c0 + kListChunk < total) {
+                        if (lane == 0) s_cnt[wave][b] = cnt;
+                    } else {
+                        if (cnt > (uint32_t)kMaxNeighbors) overflow = true;
+                        if (lane == 0) S.nbr_cnt[k] = min(cnt, (uint32_t)kMaxNeighbors);
+                    }
                 }
             }
         }
     }
-    S.nbr_cnt[k] = cnt;
-    if (overflow) atomicOr(&S.state->status, kStatusCapacity);
+    if (overflow && lane == 0) atomicOr(&S.state->status, kStatusCapacity);
 }
 
 // ---- launchers -------------------------------------------------------------------
@@ -409,8 +481,8 @@ void launch_reorder(const LaunchSet& L, uint32_t sbuf, hipStream_t st) {
     k_cell_table<<<L.nb_all, kBlock, 0, st>>>(L.scenes, L.map_all, sbuf);
 }
 
-void launch_lists(const LaunchSet& L, hipStream_t st) {
-    if (L.nb_fluid) k_lists<<<L.nb_fluid, kBlock, 0, st>>>(L.scenes, L.map_fluid);
+void launch_lists(const LaunchSet& L, uint32_t sbuf, hipStream_t st) {
+    if (L.nb_all) k_lists<<<L.nb_all, kBlock, 0, st>>>(L.scenes, L.map_all, sbuf);
 }
 
 }  // namespace fsa

@@ -73,3 +73,194 @@ def test_steps_bitwise_equal_oracle(gpu_context, oracle_lib, solver):
         assert st.average_density_error == ost.average_density_error, step
         np.testing.assert_array_equal(g.movement, r.movement, err_msg=f"movement step {step}")
         np.testing.assert_array_equal(g.particle_data, r.particle_data, err_msg=f"particle data step {step}")
+
+
+def _compare_steps(sc, orc, coll, steps, label, check_iisph=True):
+    fluid = coll.type == abi.TYPE_NORMAL
+    for step in range(steps):
+        st = sc.step(1)
+        ost = orc.step(1)
+        g = sc.download(coll.copy())
+        r = orc.download()
+        assert st.iterations == ost.iterations, (label, step)
+        assert st.actual_time_step == ost.actual_time_step, (label, step)
+        assert st.average_density_error == pytest.approx(ost.average_density_error, rel=1e-12, abs=1e-300), (label, step)
+        if check_iisph:
+            np.testing.assert_array_equal(sc.download_iisph()[fluid], orc.download_iisph()[fluid], err_msg=f"{label} {step}")
+        np.testing.assert_array_equal(g.movement, r.movement, err_msg=f"{label} movement step {step}")
+        np.testing.assert_array_equal(g.particle_data, r.particle_data, err_msg=f"{label} pdata step {step}")
+        np.testing.assert_array_equal(g.external, r.external, err_msg=f"{label} external step {step}")
+
+
+def test_dense_lattice_lists_and_steps(gpu_context, oracle_lib):
+    """Spacing 0.75: about 80 neighbors per particle and neighbor cells holding up
+    to 27 particles, so the 27-cell candidate sets exceed one LDS chunk."""
+    O = oracle_lib
+    rng = np.random.default_rng(21)
+    f = (scenes.lattice((16, 16, 16)) * np.float32(0.75) + rng.uniform(-0.01, 0.01, (4096, 3))).astype(np.float32)
+    b = (scenes.box_shell((16, 16, 16), 2) * np.float32(0.75)).astype(np.float32)
+    from fluidsolver_amd.collection import ParticleCollection
+
+    coll = ParticleCollection.from_arrays(
+        3, np.concatenate([f, b]), np.concatenate([np.zeros(len(f), np.uint8), np.ones(len(b), np.uint8)])
+    )
+    desc = abi.default_desc(3, abi.SOLVER_IISPH)
+    print(f"case dense lattice n={coll.size()}", flush=True)
+    sc = Scene(gpu_context, desc, coll.copy())
+    sc.find_neighbors()
+    goff, gcols = sc.export_neighbors()
+    orc = O.OracleScene(desc, coll.copy(), search=O.SEARCH_QUADRATIC3D)
+    orc.find_neighbors()
+    ooff, ocols = orc.export_neighbors()
+    rows = _fluid_rows(coll)
+    assert _csr_rows(goff, gcols, rows) == _csr_rows(ooff, ocols, rows)
+    assert max(len(v) for v in _csr_rows(goff, gcols, rows).values()) > 70
+    # stepping a lattice this dense compresses it past the 97-neighbor capacity;
+    # step a lattice of spacing 0.85 (about 55 neighbors) instead
+    f2 = (scenes.lattice((14, 14, 14)) * np.float32(0.85) + rng.uniform(-0.01, 0.01, (2744, 3))).astype(np.float32)
+    b2 = (scenes.box_shell((14, 14, 14), 2) * np.float32(0.85)).astype(np.float32)
+    coll2 = ParticleCollection.from_arrays(
+        3, np.concatenate([f2, b2]), np.concatenate([np.zeros(len(f2), np.uint8), np.ones(len(b2), np.uint8)])
+    )
+    sc2 = Scene(gpu_context, desc, coll2.copy())
+    orc2 = O.OracleScene(desc, coll2.copy(), mode=O.MODE_FP32_R64)
+    _compare_steps(sc2, orc2, coll2, 3, "dense")
+
+
+def test_capacity_overflow_is_reported(gpu_context):
+    """More than 97 neighbors (NeighborStorage capacity) raises FSA_E_CAPACITY."""
+    from fluidsolver_amd.collection import ParticleCollection
+    from fluidsolver_amd.solver import FsaError
+
+    pos = (scenes.lattice((10, 10, 10)) * np.float32(0.45)).astype(np.float32)
+    coll = ParticleCollection.from_arrays(3, pos, np.zeros(len(pos), np.uint8))
+    sc = Scene(gpu_context, abi.default_desc(3, abi.SOLVER_IISPH), coll)
+    with pytest.raises(FsaError) as e:
+        sc.find_neighbors()
+    assert e.value.code == abi.FSA_E_CAPACITY
+
+
+@pytest.mark.parametrize("name", ["small_breaking_dam_3d", "filled_container_single_layer_3d",
+                                  "boundary_test_multi_layer_3d"])
+def test_reference_fixtures_bitwise(gpu_context, oracle_lib, name):
+    """The reference's own scenarios with their JSON settings (SESPH, single-layer
+    IISPH), GPU against the oracle, step by step."""
+    from fixtures import load
+
+    O = oracle_lib
+    coll, desc, _ = load(name)
+    print(f"case fixture {name} n={coll.size()}", flush=True)
+    sc = Scene(gpu_context, desc, coll.copy())
+    orc = O.OracleScene(desc, coll.copy(), mode=O.MODE_FP32_R64)
+    _compare_steps(sc, orc, coll, 4, name, check_iisph=desc.solver == abi.SOLVER_IISPH)
+
+
+@pytest.mark.parametrize("solver", [abi.SOLVER_IISPH, abi.SOLVER_SESPH])
+def test_inactive_dynamic_boundary_and_external(gpu_context, oracle_lib, solver):
+    """Inactive particles (skipped by every loop, kept as uploaded), a moving
+    boundary (non-zero velocity) and external accelerations (IISPH adds and then
+    resets them)."""
+    O = oracle_lib
+    coll = scenes.dam_scene((8, 8, 8), (24, 12, 12), 3, seed=9)
+    # keep every cell coordinate positive: with inactive particles present the
+    # reference misses cell (-1,-1,-1) (test_inactive_sentinel_collides_with_cell_minus_one)
+    coll.position[:] += np.float32(10.0)
+    rng = np.random.default_rng(9)
+    n = coll.size()
+    t = coll.type.copy()
+    far = np.nonzero(coll.position[:, 1] > 16.0)[0]
+    t[far[:20]] = abi.TYPE_INACTIVE  # inactive fluid away from cell (-1,-1,-1)
+    coll.info["type"] = t
+    bnd = np.nonzero(t == abi.TYPE_BOUNDARY)[0]
+    coll.velocity[bnd[:50]] = np.float32([0.0, 0.0, 0.5])
+    fl = np.nonzero(t == abi.TYPE_NORMAL)[0]
+    coll.external[fl[:100]] = rng.uniform(-1, 1, (100, 3)).astype(np.float32)
+    desc = abi.default_desc(3, solver)
+    if solver == abi.SOLVER_SESPH:
+        desc.sesph_stiffness_k = 1000.0
+    print(f"case inactive/dynamic solver={solver} n={n}", flush=True)
+    sc = Scene(gpu_context, desc, coll.copy())
+    orc = O.OracleScene(desc, coll.copy(), mode=O.MODE_FP32_R64)
+    _compare_steps(sc, orc, coll, 4, "mixed", check_iisph=False)
+    g = sc.download(coll.copy())
+    ina = t == abi.TYPE_INACTIVE
+    np.testing.assert_array_equal(g.movement[ina], coll.movement[ina])
+    np.testing.assert_array_equal(g.particle_data[ina], coll.particle_data[ina])
+
+
+def test_simulator_mirror_runs(gpu_context, oracle_lib):
+    """The reference-shaped host API (Simulator + IISPHFluidSolver3D) drives the
+    device and keeps data.collection in sync after each step."""
+    from fluidsolver_amd.solver import ConstantTimestepGenerator, IISPHFluidSolver3D, Simulator
+
+    O = oracle_lib
+    coll = scenes.dam_scene((6, 6, 6), (18, 9, 9), 3, seed=12)
+    solver = IISPHFluidSolver3D(gpu_context)
+    sim = Simulator(solver, ConstantTimestepGenerator(), coll)
+    assert solver.create_compatibility_report() == []
+    for _ in range(3):
+        sim.execute_simulation_step()
+    assert solver.stat_last_iteration_count >= 2
+    assert sim.timepoint.timestep_number == 3
+    orc = O.OracleScene(abi.default_desc(3, abi.SOLVER_IISPH), scenes.dam_scene((6, 6, 6), (18, 9, 9), 3, seed=12),
+                        mode=O.MODE_FP32_R64)
+    orc.step(3)
+    r = orc.download()
+    np.testing.assert_array_equal(coll.movement, r.movement)
+    nb = solver.neighborhood_search.get_neighbors(0)
+    assert 0 in nb
+
+
+def test_batch_equals_alone(gpu_context):
+    """Scenes advanced together in shared launches give bit-identical results to
+    each scene stepped alone (uneven sizes, different viscosities, both solvers)."""
+    descs, colls = [], []
+    for q, (n, nu, solver) in enumerate([(5, 5.0, 0), (9, 0.01, 0), (7, 0.05, 0), (6, 5.0, 1), (4, 1.0, 1)]):
+        c = scenes.dam_scene((n, n + 1, n), (3 * n, 2 * n, 2 * n), 3, seed=30 + q)
+        d = abi.default_desc(3, solver)
+        if solver == abi.SOLVER_IISPH:
+            d.iisph_viscosity = nu
+        else:
+            d.sesph_viscosity = nu
+            d.sesph_stiffness_k = 1000.0
+        descs.append(d)
+        colls.append(c)
+    alone = []
+    for d, c in zip(descs, colls):
+        sc = Scene(gpu_context, d, c.copy())
+        stats = [sc.step(1).as_dict() for _ in range(6)]
+        alone.append((stats, sc.download(c.copy())))
+        sc.close()
+    batch = [Scene(gpu_context, d, c.copy()) for d, c in zip(descs, colls)]
+    bstats = [[] for _ in batch]
+    for _ in range(6):
+        for q, st in enumerate(gpu_context.batch_step(batch, 1)):
+            bstats[q].append(st.as_dict())
+    for q, sc in enumerate(batch):
+        assert bstats[q] == alone[q][0], q
+        g = sc.download(colls[q].copy())
+        np.testing.assert_array_equal(g.movement, alone[q][1].movement)
+        np.testing.assert_array_equal(g.particle_data, alone[q][1].particle_data)
+
+
+def test_cpp_dropin_program(oracle_lib, tmp_path):
+    """A C++ program using include/fluidsolver_amd.hpp (ParticleCollection +
+    IISPHFluidSolver3D + Simulator) reproduces the oracle bit for bit."""
+    import os
+    import subprocess
+
+    from fluidsolver_amd import build as fbuild
+
+    O = oracle_lib
+    if not os.path.exists(fbuild.DROPIN_BIN):
+        pytest.skip("drop-in example not built")
+    out = tmp_path / "movement.bin"
+    print("case cpp dropin 6x6x6", flush=True)
+    r = subprocess.run([fbuild.DROPIN_BIN, "6", "6", "6", "3", str(out)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    assert "iterations" in r.stdout
+    mv = np.fromfile(out, dtype=np.float32).reshape(-1, 9)
+    coll = scenes.dam_scene((6, 6, 6), (18, 12, 12), 3, seed=0, amplitude=0.0)
+    orc = O.OracleScene(abi.default_desc(3, abi.SOLVER_IISPH), coll, mode=O.MODE_FP32_R64)
+    orc.step(3)
+    np.testing.assert_array_equal(mv, orc.download().movement)
