@@ -8,6 +8,8 @@
 
 #include <algorithm>
 #include <atomic>
+#include <chrono>
+#include <deque>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -898,6 +900,113 @@ void destroy_scene(fsa_scene_s* s) {
     delete s;
 }
 
+
+// ---- serving ----------------------------------------------------------------------------
+struct Job {
+    uint64_t id = 0;
+    fsa_scene_desc desc{};
+    uint32_t steps_total = 0, steps_done = 0;
+    uint32_t state = FSA_JOB_QUEUED, status = FSA_OK;
+    uint64_t fluid = 0;
+    double submit_s = 0, admit_s = 0, finish_s = 0;
+    fsa_step_stats last{};
+    // host copy of the particles (upload source, then the final state)
+    uint64_t count = 0;
+    std::vector<float> movement, pdata, external;
+    std::vector<uint8_t> info;
+    fsa_scene_s* scene = nullptr;
+    fsa_particle_view view() {
+        fsa_particle_view v{};
+        v.count = count;
+        v.movement = movement.data();
+        v.movement_stride = 36;
+        v.particle_data = pdata.data();
+        v.particle_data_stride = 12;
+        v.info = info.data();
+        v.info_stride = 8;
+        v.external = external.data();
+        v.external_stride = 12;
+        return v;
+    }
+};
+
+}  // namespace
+
+struct fsa_server_s {
+    fsa_context ctx = nullptr;
+    fsa_server_config cfg{};
+    uint64_t next_id = 1;
+    std::map<uint64_t, Job*> jobs;
+    std::deque<uint64_t> queue;
+    std::vector<uint64_t> running;
+    std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
+    uint64_t ticks = 0, scene_steps = 0, particle_updates = 0, submitted = 0, done = 0, failed = 0;
+    double busy_s = 0.0;
+    std::vector<std::pair<double, uint32_t>> step_ms;  // tick latency, scenes stepped
+    double now() const {
+        return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    }
+};
+
+namespace {
+
+uint64_t server_budget(fsa_server srv) {
+    if (srv->cfg.hbm_budget_bytes) return srv->cfg.hbm_budget_bytes;
+    return srv->ctx->budget;
+}
+
+void admit(fsa_server srv) {
+    while (!srv->queue.empty()) {
+        if (srv->cfg.max_running && srv->running.size() >= srv->cfg.max_running) return;
+        Job* j = srv->jobs[srv->queue.front()];
+        uint64_t nb = 0;
+        for (uint64_t i = 0; i < j->count; i++)
+            if (j->info[8 * i + 4] == FSA_TYPE_BOUNDARY) nb++;
+        // scene plan + cell table allowance (the grid of a dam scene is about the particle count)
+        const uint64_t need = plan_bytes(make_plan(j->fluid + nb, j->fluid)) + sizeof(uint4) * (j->fluid + nb);
+        const uint64_t budget = server_budget(srv);
+        if (budget && srv->ctx->in_use.load() + need > budget) return;  // FIFO: wait for memory
+        srv->queue.pop_front();
+        fsa_particle_view v = j->view();
+        fsa_scene sc = nullptr;
+        const int32_t rc = fsa_scene_create(srv->ctx, &j->desc, &v, &sc);
+        if (rc != FSA_OK) {
+            j->state = FSA_JOB_FAILED;
+            j->status = (uint32_t)rc;
+            j->finish_s = srv->now();
+            srv->failed++;
+            continue;
+        }
+        j->scene = sc;
+        j->state = FSA_JOB_RUNNING;
+        j->admit_s = srv->now();
+        srv->running.push_back(j->id);
+    }
+}
+
+void retire(fsa_server srv, Job* j, uint32_t state, uint32_t status) {
+    if (j->scene) {
+        if (state == FSA_JOB_DONE) {
+            fsa_particle_view v = j->view();
+            do_download(j->scene, &v);
+        }
+        destroy_scene(j->scene);
+        j->scene = nullptr;
+    }
+    j->state = state;
+    j->status = status;
+    j->finish_s = srv->now();
+    srv->running.erase(std::remove(srv->running.begin(), srv->running.end(), j->id), srv->running.end());
+    if (state == FSA_JOB_DONE) srv->done++;
+    if (state == FSA_JOB_FAILED) srv->failed++;
+    if (state != FSA_JOB_DONE) {
+        j->movement.clear();
+        j->pdata.clear();
+        j->external.clear();
+        j->info.clear();
+    }
+}
+
 }  // namespace
 
 // =====================================================================================
@@ -1247,6 +1356,196 @@ FSA_API int32_t fsa_context_get_timing(fsa_context c, double* total_ms, uint64_t
         for (uint32_t k = 0; k < n && k < (uint32_t)FSA_TIMER_COUNT; k++) {
             if (total_ms) total_ms[k] = c->timer_ms[k];
             if (launches) launches[k] = c->timer_n[k];
+        }
+    });
+}
+
+FSA_API int32_t fsa_server_create(fsa_context c, const fsa_server_config* cfg, fsa_server* out) {
+    return guarded([&] {
+        if (!c || !out) throw FsaError(FSA_E_INVALID, "NULL argument");
+        auto* s = new fsa_server_s();
+        s->ctx = c;
+        if (cfg) s->cfg = *cfg;
+        *out = s;
+    });
+}
+
+FSA_API int32_t fsa_server_destroy(fsa_server srv) {
+    return guarded([&] {
+        if (!srv) return;
+        for (auto& kv : srv->jobs) {
+            if (kv.second->scene) destroy_scene(kv.second->scene);
+            delete kv.second;
+        }
+        delete srv;
+    });
+}
+
+FSA_API int32_t fsa_server_submit(fsa_server srv, const fsa_scene_desc* desc, const fsa_particle_view* v,
+                                  uint32_t steps, uint64_t* job_id) {
+    return guarded([&] {
+        if (!srv || !desc || !v || !job_id) throw FsaError(FSA_E_INVALID, "NULL argument");
+        validate_desc(desc);
+        if (!v->movement || !v->particle_data || !v->info) throw FsaError(FSA_E_INVALID, "incomplete particle view");
+        auto* j = new Job();
+        j->id = srv->next_id++;
+        j->desc = *desc;
+        j->steps_total = steps;
+        j->count = v->count;
+        j->movement.resize(9 * v->count);
+        j->pdata.resize(3 * v->count);
+        j->external.assign(3 * v->count, 0.0f);
+        j->info.resize(8 * v->count);
+        for (uint64_t i = 0; i < v->count; i++) {
+            std::memcpy(&j->movement[9 * i], (const char*)v->movement + i * v->movement_stride, 36);
+            std::memcpy(&j->pdata[3 * i], (const char*)v->particle_data + i * v->particle_data_stride, 12);
+            std::memcpy(&j->info[8 * i], (const char*)v->info + i * v->info_stride, 8);
+            if (v->external)
+                std::memcpy(&j->external[3 * i], (const char*)v->external + i * v->external_stride, 12);
+            if (j->info[8 * i + 4] == FSA_TYPE_NORMAL) j->fluid++;
+        }
+        j->submit_s = srv->now();
+        srv->jobs[j->id] = j;
+        srv->queue.push_back(j->id);
+        srv->submitted++;
+        *job_id = j->id;
+    });
+}
+
+FSA_API int32_t fsa_server_cancel(fsa_server srv, uint64_t id) {
+    return guarded([&] {
+        if (!srv) throw FsaError(FSA_E_INVALID, "server is NULL");
+        auto it = srv->jobs.find(id);
+        if (it == srv->jobs.end()) throw FsaError(FSA_E_INVALID, "unknown job");
+        Job* j = it->second;
+        if (j->state == FSA_JOB_QUEUED) {
+            srv->queue.erase(std::remove(srv->queue.begin(), srv->queue.end(), id), srv->queue.end());
+            j->state = FSA_JOB_CANCELLED;
+            j->finish_s = srv->now();
+        } else if (j->state == FSA_JOB_RUNNING) {
+            retire(srv, j, FSA_JOB_CANCELLED, FSA_OK);
+        }
+    });
+}
+
+FSA_API int32_t fsa_server_tick(fsa_server srv, uint32_t* running) {
+    return guarded([&] {
+        if (!srv) throw FsaError(FSA_E_INVALID, "server is NULL");
+        const double t0 = srv->now();
+        admit(srv);
+        // jobs with no steps left (steps == 0) finish without a device step
+        for (uint64_t id : std::vector<uint64_t>(srv->running)) {
+            Job* j = srv->jobs[id];
+            if (j->steps_done >= j->steps_total) retire(srv, j, FSA_JOB_DONE, FSA_OK);
+        }
+        std::vector<fsa_scene> scenes;
+        std::vector<Job*> js;
+        for (uint64_t id : srv->running) {
+            js.push_back(srv->jobs[id]);
+            scenes.push_back(srv->jobs[id]->scene);
+        }
+        if (running) *running = (uint32_t)scenes.size();
+        if (scenes.empty()) return;
+        std::vector<fsa_step_stats> st(scenes.size());
+        const int32_t rc = fsa_batch_step(srv->ctx, scenes.data(), (uint32_t)scenes.size(), 1, st.data());
+        const double t1 = srv->now();
+        srv->ticks++;
+        srv->busy_s += t1 - t0;
+        srv->step_ms.push_back({1000.0 * (t1 - t0), (uint32_t)scenes.size()});
+        srv->scene_steps += scenes.size();
+        for (size_t q = 0; q < js.size(); q++) {
+            Job* j = js[q];
+            if (rc != FSA_OK) {
+                retire(srv, j, FSA_JOB_FAILED, (uint32_t)rc);
+                continue;
+            }
+            j->last = st[q];
+            j->steps_done++;
+            srv->particle_updates += j->fluid;
+            if (j->steps_done >= j->steps_total) retire(srv, j, FSA_JOB_DONE, FSA_OK);
+        }
+    });
+}
+
+FSA_API int32_t fsa_server_job(fsa_server srv, uint64_t id, fsa_job_info* info) {
+    return guarded([&] {
+        if (!srv || !info) throw FsaError(FSA_E_INVALID, "NULL argument");
+        auto it = srv->jobs.find(id);
+        if (it == srv->jobs.end()) throw FsaError(FSA_E_INVALID, "unknown job");
+        const Job* j = it->second;
+        info->job_id = j->id;
+        info->state = j->state;
+        info->status = j->status;
+        info->steps_done = j->steps_done;
+        info->steps_total = j->steps_total;
+        info->fluid_particles = j->fluid;
+        info->submit_s = j->submit_s;
+        info->admit_s = j->admit_s;
+        info->finish_s = j->finish_s;
+        info->last = j->last;
+    });
+}
+
+FSA_API int32_t fsa_server_collect(fsa_server srv, uint64_t id, const fsa_particle_view* v) {
+    return guarded([&] {
+        if (!srv) throw FsaError(FSA_E_INVALID, "server is NULL");
+        auto it = srv->jobs.find(id);
+        if (it == srv->jobs.end()) throw FsaError(FSA_E_INVALID, "unknown job");
+        Job* j = it->second;
+        if (j->state != FSA_JOB_DONE) throw FsaError(FSA_E_STATE, "job has not finished");
+        if (v) {
+            if (v->count != j->count) throw FsaError(FSA_E_INVALID, "view count mismatch");
+            for (uint64_t i = 0; i < j->count; i++) {
+                if (v->movement) std::memcpy((char*)v->movement + i * v->movement_stride, &j->movement[9 * i], 36);
+                if (v->particle_data)
+                    std::memcpy((char*)v->particle_data + i * v->particle_data_stride, &j->pdata[3 * i], 12);
+                if (v->info) std::memcpy((char*)v->info + i * v->info_stride, &j->info[8 * i], 8);
+                if (v->external)
+                    std::memcpy((char*)v->external + i * v->external_stride, &j->external[3 * i], 12);
+            }
+        }
+        delete j;
+        srv->jobs.erase(it);
+    });
+}
+
+FSA_API int32_t fsa_server_get_stats(fsa_server srv, fsa_server_stats* out) {
+    return guarded([&] {
+        if (!srv || !out) throw FsaError(FSA_E_INVALID, "NULL argument");
+        std::memset(out, 0, sizeof(*out));
+        out->ticks = srv->ticks;
+        out->scene_steps = srv->scene_steps;
+        out->particle_updates = srv->particle_updates;
+        out->jobs_submitted = srv->submitted;
+        out->jobs_done = srv->done;
+        out->jobs_failed = srv->failed;
+        out->jobs_queued = srv->queue.size();
+        out->jobs_running = srv->running.size();
+        out->hbm_in_use = srv->ctx->in_use.load();
+        out->busy_s = srv->busy_s;
+        // percentiles over scene-steps: a tick's latency counts once per scene it stepped
+        std::vector<std::pair<double, uint32_t>> v = srv->step_ms;
+        std::sort(v.begin(), v.end());
+        uint64_t total = 0;
+        double sum = 0.0;
+        for (auto& e : v) {
+            total += e.second;
+            sum += e.first * e.second;
+        }
+        if (total) {
+            auto pct = [&](double q) {
+                const uint64_t target = (uint64_t)std::ceil(q * (double)total);
+                uint64_t acc = 0;
+                for (auto& e : v) {
+                    acc += e.second;
+                    if (acc >= std::max<uint64_t>(target, 1)) return e.first;
+                }
+                return v.back().first;
+            };
+            out->p50_step_ms = pct(0.50);
+            out->p99_step_ms = pct(0.99);
+            out->mean_step_ms = sum / (double)total;
+            out->max_step_ms = v.back().first;
         }
     });
 }

@@ -237,4 +237,53 @@ __device__ __host__ __forceinline__ uint32_t compact_key(int32_t cx, int32_t cy,
 
 __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
 
+// Global-address-space views of device pointers: loads through them are
+// global_load (not flat_load, which also waits on the LDS counter).
+#define FSA_GLOBAL(T) __attribute__((address_space(1))) T
+typedef float fsa_f4 __attribute__((ext_vector_type(4)));
+
+struct GView4 {
+    const FSA_GLOBAL(fsa_f4) * p;
+    __device__ __forceinline__ float4 operator[](uint32_t i) const {
+        const fsa_f4 v = p[i];
+        return make_float4(v.x, v.y, v.z, v.w);
+    }
+};
+struct GViewF {
+    const FSA_GLOBAL(float) * p;
+    __device__ __forceinline__ float operator[](uint32_t i) const { return p[i]; }
+};
+struct GViewU {
+    const FSA_GLOBAL(uint32_t) * p;
+    __device__ __forceinline__ uint32_t operator[](uint32_t i) const { return p[i]; }
+};
+__device__ __forceinline__ GView4 gload_ptr(const float4* p) { return GView4{(const FSA_GLOBAL(fsa_f4)*)p}; }
+__device__ __forceinline__ GViewF gload_ptr(const float* p) { return GViewF{(const FSA_GLOBAL(float)*)p}; }
+__device__ __forceinline__ GViewU gload_ptr(const uint32_t* p) { return GViewU{(const FSA_GLOBAL(uint32_t)*)p}; }
+
+// Walks a neighbor list in order, U entries at a time: the list entries and
+// the per-neighbor gathers of a group are issued before any of them is used,
+// then `body` runs for each entry in list order (the accumulation order of
+// the reference loop is unchanged).
+template <int U, class Load, class Body>
+__device__ __forceinline__ void walk_neighbors(GViewU nb, uint32_t stride, uint32_t cnt, Load load, Body body) {
+    using T = decltype(load(0u));
+    uint32_t e = 0;
+    for (; e + U <= cnt; e += U) {
+        uint32_t ent[U];
+        T d[U];
+#pragma unroll
+        for (int q = 0; q < U; q++) ent[q] = nb[(e + q) * stride];
+#pragma unroll
+        for (int q = 0; q < U; q++) d[q] = load(ent[q] & kIndexMask);
+#pragma unroll
+        for (int q = 0; q < U; q++) body(ent[q], d[q]);
+    }
+    for (; e < cnt; e++) {
+        const uint32_t en = nb[e * stride];
+        const T dd = load(en & kIndexMask);
+        body(en, dd);
+    }
+}
+
 }  // namespace fsa

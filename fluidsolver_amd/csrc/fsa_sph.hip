@@ -120,6 +120,19 @@ __device__ __forceinline__ Partial reduce_partials(const Partial* parts, uint32_
 }
 
 // ---- IISPH phase 1: viscosity, predicted velocity, density -------------------------
+constexpr int kUnroll = 4;
+
+struct PosVel {
+    float4 x, v;
+};
+struct PosAcc {
+    float4 x, a;
+};
+struct PosP {
+    float4 x;
+    float p;
+};
+
 template <uint32_t Spec>
 __global__ __launch_bounds__(kBlock) void k_iisph_prepare(const SceneDev* __restrict__ scenes,
                                                           const BlockRef* __restrict__ map) {
@@ -129,8 +142,8 @@ __global__ __launch_bounds__(kBlock) void k_iisph_prepare(const SceneDev* __rest
     if (k >= S.nf) return;
     const uint32_t c = S.state->cur;
     const uint32_t i = S.fluid_idx[k];
-    const float4* __restrict__ pos = S.pos[c];
-    const float4* __restrict__ vel = S.vel[c];
+    const auto pos = gload_ptr(S.pos[c]);
+    const auto vel = gload_ptr(S.vel[c]);
     const float4 xi = pos[i];
     const float4 vi = vel[i];
     float ex = 0.0f, ey = 0.0f, ez = 0.0f;
@@ -142,30 +155,27 @@ __global__ __launch_bounds__(kBlock) void k_iisph_prepare(const SceneDev* __rest
     }
     const bool sl = (S.flags & kFlagSingleLayer) != 0;
     float tx = 0.0f, ty = 0.0f, tz = 0.0f, rho = 0.0f;
-    const uint32_t cnt = S.nbr_cnt[k];
-    const uint32_t stride = S.list_stride;
-    const uint32_t* __restrict__ nb = S.nbr + k;
-    for (uint32_t e = 0; e < cnt; e++) {
-        const uint32_t ent = nb[e * stride];
-        const uint32_t j = ent & kIndexMask;
-        const float4 xj = pos[j];
-        const float4 vj = vel[j];
-        float dx, dy, dz, d2, len;
-        Grad g;
-        pair_geometry<Spec>(S, xi, xj, dx, dy, dz, d2, len, g);
-        // ComputeViscosityAcceleration: densities of the start-of-step snapshot
-        const float nd = vj.w;
-        if (nd != 0.0f) {
-            const float vx = vi.x - vj.x, vy = vi.y - vj.y, vz = vi.z - vj.z;
-            const float t = (xj.w / nd) * (dot3(vx, vy, vz, dx, dy, dz) / (d2 + S.visc_c01));
-            tx = tx + t * g.x;
-            ty = ty + t * g.y;
-            tz = tz + t * g.z;
-        }
-        float m = xj.w;
-        if (sl && (ent & kBoundaryBit)) m = m * S.gamma1;
-        rho = rho + m * kernel_w<Spec>(S, len);
-    }
+    walk_neighbors<kUnroll>(
+        gload_ptr(S.nbr + k), S.list_stride, S.nbr_cnt[k],
+        [&](uint32_t j) { return PosVel{pos[j], vel[j]}; },
+        [&](uint32_t ent, const PosVel& d) {
+            const float4 xj = d.x, vj = d.v;
+            float dx, dy, dz, d2, len;
+            Grad g;
+            pair_geometry<Spec>(S, xi, xj, dx, dy, dz, d2, len, g);
+            // ComputeViscosityAcceleration: densities of the start-of-step snapshot
+            const float nd = vj.w;
+            if (nd != 0.0f) {
+                const float vx = vi.x - vj.x, vy = vi.y - vj.y, vz = vi.z - vj.z;
+                const float t = (xj.w / nd) * (dot3(vx, vy, vz, dx, dy, dz) / (d2 + S.visc_c01));
+                tx = tx + t * g.x;
+                ty = ty + t * g.y;
+                tz = tz + t * g.z;
+            }
+            float m = xj.w;
+            if (sl && (ent & kBoundaryBit)) m = m * S.gamma1;
+            rho = rho + m * kernel_w<Spec>(S, len);
+        });
     float nx = ex + 0.0f, ny = ey + (-S.gravity), nz = ez + 0.0f;
     nx = nx + S.visc2 * tx;
     ny = ny + S.visc2 * ty;
@@ -208,52 +218,51 @@ __global__ __launch_bounds__(kBlock) void k_iisph_diagonal(const SceneDev* __res
     if (k >= S.nf) return;
     const uint32_t c = S.state->cur;
     const uint32_t i = S.fluid_idx[k];
-    const float4* __restrict__ pos = S.pos[c];
+    const auto pos = gload_ptr(S.pos[c]);
+    const auto vstar = gload_ptr(S.vstar);
     const float4 xi = pos[i];
-    const float4 vs = S.vstar[i];
+    const float4 vs = vstar[i];
     const bool sl = (S.flags & kFlagSingleLayer) != 0;
     const uint32_t cnt = S.nbr_cnt[k];
-    const uint32_t stride = S.list_stride;
-    const uint32_t* __restrict__ nb = S.nbr + k;
+    const auto nb = gload_ptr(S.nbr + k);
     float contrib = 0.0f;
     float ix = 0.0f, iy = 0.0f, iz = 0.0f;
-    for (uint32_t e = 0; e < cnt; e++) {
-        const uint32_t ent = nb[e * stride];
-        const uint32_t j = ent & kIndexMask;
-        const float4 xj = pos[j];
-        float dx, dy, dz, d2, len;
-        Grad g;
-        pair_geometry<Spec>(S, xi, xj, dx, dy, dz, d2, len, g);
-        float f;
-        if (!(ent & kBoundaryBit)) {
-            const float4 vsj = S.vstar[j];
-            contrib = contrib + xj.w * dot3(vs.x - vsj.x, vs.y - vsj.y, vs.z - vsj.z, g.x, g.y, g.z);
-            f = div_rho0sq<Spec>(S, xj.w);
-        } else {
-            contrib = contrib + xj.w * dot3(vs.x, vs.y, vs.z, g.x, g.y, g.z);
-            f = sl ? div_rho0sq<Spec>(S, 2.0f * S.gamma * xj.w) : div_rho0sq<Spec>(S, 2.0f * xj.w);
-        }
-        ix = ix - f * g.x;
-        iy = iy - f * g.y;
-        iz = iz - f * g.z;
-    }
+    walk_neighbors<kUnroll>(
+        nb, S.list_stride, cnt, [&](uint32_t j) { return PosVel{pos[j], vstar[j]}; },
+        [&](uint32_t ent, const PosVel& d) {
+            const float4 xj = d.x;
+            float dx, dy, dz, d2, len;
+            Grad g;
+            pair_geometry<Spec>(S, xi, xj, dx, dy, dz, d2, len, g);
+            float f;
+            if (!(ent & kBoundaryBit)) {
+                const float4 vsj = d.v;
+                contrib = contrib + xj.w * dot3(vs.x - vsj.x, vs.y - vsj.y, vs.z - vsj.z, g.x, g.y, g.z);
+                f = div_rho0sq<Spec>(S, xj.w);
+            } else {
+                contrib = contrib + xj.w * dot3(vs.x, vs.y, vs.z, g.x, g.y, g.z);
+                f = sl ? div_rho0sq<Spec>(S, 2.0f * S.gamma * xj.w) : div_rho0sq<Spec>(S, 2.0f * xj.w);
+            }
+            ix = ix - f * g.x;
+            iy = iy - f * g.y;
+            iz = iz - f * g.z;
+        });
     const float dt = S.dt;
     S.src[i] = S.rest_density - S.rho_new[i] - dt * contrib;
     const float cm = div_rho0sq<Spec>(S, xi.w);
     float dg = 0.0f;
-    for (uint32_t e = 0; e < cnt; e++) {
-        const uint32_t ent = nb[e * stride];
-        const uint32_t j = ent & kIndexMask;
-        const float4 xj = pos[j];
-        float dx, dy, dz, d2, len;
-        Grad g;
-        pair_geometry<Spec>(S, xi, xj, dx, dy, dz, d2, len, g);
-        dg = dg + xj.w * dot3(ix, iy, iz, g.x, g.y, g.z);
-        if (!(ent & kBoundaryBit)) {
-            // (m_i / rho0^2) * gradW(x_i, x_j) with gradW(x_i, x_j) = -gradW(x_j, x_i) exactly
-            dg = dg + xj.w * dot3(-(cm * g.x), -(cm * g.y), -(cm * g.z), g.x, g.y, g.z);
-        }
-    }
+    walk_neighbors<kUnroll>(
+        nb, S.list_stride, cnt, [&](uint32_t j) { return pos[j]; },
+        [&](uint32_t ent, const float4& xj) {
+            float dx, dy, dz, d2, len;
+            Grad g;
+            pair_geometry<Spec>(S, xi, xj, dx, dy, dz, d2, len, g);
+            dg = dg + xj.w * dot3(ix, iy, iz, g.x, g.y, g.z);
+            if (!(ent & kBoundaryBit)) {
+                // (m_i / rho0^2) * gradW(x_i, x_j) with gradW(x_i, x_j) = -gradW(x_j, x_i) exactly
+                dg = dg + xj.w * dot3(-(cm * g.x), -(cm * g.y), -(cm * g.z), g.x, g.y, g.z);
+            }
+        });
     S.diag[i] = dg * (dt * dt);
 }
 
@@ -282,33 +291,30 @@ __global__ __launch_bounds__(kBlock) void k_iisph_pass_a(const SceneDev* __restr
     if (k >= S.nf) return;
     const uint32_t c = S.state->cur;
     const uint32_t i = S.fluid_idx[k];
-    const float4* __restrict__ pos = S.pos[c];
-    const float* __restrict__ pres = S.pres[c];
+    const auto pos = gload_ptr(S.pos[c]);
+    const auto pres = gload_ptr(S.pres[c]);
     const float4 xi = pos[i];
     const float pi = pres[i];
     const float pir = div_rho0sq<Spec>(S, pi);
     const bool sl = (S.flags & kFlagSingleLayer) != 0;
-    const uint32_t cnt = S.nbr_cnt[k];
-    const uint32_t stride = S.list_stride;
-    const uint32_t* __restrict__ nb = S.nbr + k;
     float ax = 0.0f, ay = 0.0f, az = 0.0f;
-    for (uint32_t e = 0; e < cnt; e++) {
-        const uint32_t ent = nb[e * stride];
-        const uint32_t j = ent & kIndexMask;
-        const float4 xj = pos[j];
-        float dx, dy, dz, d2, len;
-        Grad g;
-        pair_geometry<Spec>(S, xi, xj, dx, dy, dz, d2, len, g);
-        float f;
-        if (!(ent & kBoundaryBit)) {
-            f = xj.w * (pir + div_rho0sq<Spec>(S, pres[j]));
-        } else {
-            f = sl ? div_rho0sq<Spec>(S, S.gamma * xj.w * 2.0f * pi) : div_rho0sq<Spec>(S, xj.w * 2.0f * pi);
-        }
-        ax = ax - f * g.x;
-        ay = ay - f * g.y;
-        az = az - f * g.z;
-    }
+    walk_neighbors<kUnroll>(
+        gload_ptr(S.nbr + k), S.list_stride, S.nbr_cnt[k], [&](uint32_t j) { return PosP{pos[j], pres[j]}; },
+        [&](uint32_t ent, const PosP& d) {
+            const float4 xj = d.x;
+            float dx, dy, dz, d2, len;
+            Grad g;
+            pair_geometry<Spec>(S, xi, xj, dx, dy, dz, d2, len, g);
+            float f;
+            if (!(ent & kBoundaryBit)) {
+                f = xj.w * (pir + div_rho0sq<Spec>(S, d.p));
+            } else {
+                f = sl ? div_rho0sq<Spec>(S, S.gamma * xj.w * 2.0f * pi) : div_rho0sq<Spec>(S, xj.w * 2.0f * pi);
+            }
+            ax = ax - f * g.x;
+            ay = ay - f * g.y;
+            az = az - f * g.z;
+        });
     S.acc[c][i] = make_float4(ax, ay, az, 0.0f);
 }
 
@@ -327,28 +333,25 @@ __global__ __launch_bounds__(kBlock) void k_iisph_pass_b(const SceneDev* __restr
     float a2 = 0.0f, v2 = 0.0f;
     if (k < S.nf) {
         const uint32_t i = S.fluid_idx[k];
-        const float4* __restrict__ pos = S.pos[c];
-        const float4* __restrict__ acc = S.acc[c];
+        const auto pos = gload_ptr(S.pos[c]);
+        const auto acc = gload_ptr(S.acc[c]);
         const float4 xi = pos[i];
         const float4 ai = acc[i];
-        const uint32_t cnt = S.nbr_cnt[k];
-        const uint32_t stride = S.list_stride;
-        const uint32_t* __restrict__ nb = S.nbr + k;
         float ap = 0.0f;
-        for (uint32_t e = 0; e < cnt; e++) {
-            const uint32_t ent = nb[e * stride];
-            const uint32_t j = ent & kIndexMask;
-            const float4 xj = pos[j];
-            float dx, dy, dz, d2, len;
-            Grad g;
-            pair_geometry<Spec>(S, xi, xj, dx, dy, dz, d2, len, g);
-            if (!(ent & kBoundaryBit)) {
-                const float4 aj = acc[j];
-                ap = ap + xj.w * dot3(ai.x - aj.x, ai.y - aj.y, ai.z - aj.z, g.x, g.y, g.z);
-            } else {
-                ap = ap + xj.w * dot3(ai.x, ai.y, ai.z, g.x, g.y, g.z);
-            }
-        }
+        walk_neighbors<kUnroll>(
+            gload_ptr(S.nbr + k), S.list_stride, S.nbr_cnt[k], [&](uint32_t j) { return PosAcc{pos[j], acc[j]}; },
+            [&](uint32_t ent, const PosAcc& d) {
+                const float4 xj = d.x;
+                float dx, dy, dz, d2, len;
+                Grad g;
+                pair_geometry<Spec>(S, xi, xj, dx, dy, dz, d2, len, g);
+                if (!(ent & kBoundaryBit)) {
+                    const float4 aj = d.a;
+                    ap = ap + xj.w * dot3(ai.x - aj.x, ai.y - aj.y, ai.z - aj.z, g.x, g.y, g.z);
+                } else {
+                    ap = ap + xj.w * dot3(ai.x, ai.y, ai.z, g.x, g.y, g.z);
+                }
+            });
         const float dt = S.dt;
         ap = ap * (dt * dt);
         const float dg = S.diag[i];
@@ -407,23 +410,20 @@ __global__ __launch_bounds__(kBlock) void k_sesph_density(const SceneDev* __rest
     if (k >= S.nf) return;
     const uint32_t c = S.state->cur;
     const uint32_t i = S.fluid_idx[k];
-    const float4* __restrict__ pos = S.pos[c];
+    const auto pos = gload_ptr(S.pos[c]);
     const float4 xi = pos[i];
     const bool sl = (S.flags & kFlagSingleLayer) != 0;
-    const uint32_t cnt = S.nbr_cnt[k];
-    const uint32_t stride = S.list_stride;
-    const uint32_t* __restrict__ nb = S.nbr + k;
     float rho = 0.0f;
-    for (uint32_t e = 0; e < cnt; e++) {
-        const uint32_t ent = nb[e * stride];
-        const float4 xj = pos[ent & kIndexMask];
-        const float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
-        const float len = __builtin_sqrtf(dot3(dx, dy, dz, dx, dy, dz));
-        if (sl && (ent & kBoundaryBit))
-            rho = rho + S.gamma1 * xj.w * kernel_w<Spec>(S, len);
-        else
-            rho = rho + xj.w * kernel_w<Spec>(S, len);
-    }
+    walk_neighbors<kUnroll>(
+        gload_ptr(S.nbr + k), S.list_stride, S.nbr_cnt[k], [&](uint32_t j) { return pos[j]; },
+        [&](uint32_t ent, const float4& xj) {
+            const float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
+            const float len = __builtin_sqrtf(dot3(dx, dy, dz, dx, dy, dz));
+            if (sl && (ent & kBoundaryBit))
+                rho = rho + S.gamma1 * xj.w * kernel_w<Spec>(S, len);
+            else
+                rho = rho + xj.w * kernel_w<Spec>(S, len);
+        });
     float p = S.stiffness * (rho / S.rest_density - 1.0f);
     p = p < 0.0f ? 0.0f : p;  // std::max(pressure, 0.0f)
     float4 v = S.vel[c][i];
@@ -431,6 +431,11 @@ __global__ __launch_bounds__(kBlock) void k_sesph_density(const SceneDev* __rest
     S.vel[c][i] = v;
     S.pres[c][i] = p;
 }
+
+struct PosVelP {
+    float4 x, v;
+    float p;
+};
 
 template <uint32_t Spec>
 __global__ __launch_bounds__(kBlock) void k_sesph_accel(const SceneDev* __restrict__ scenes,
@@ -441,54 +446,51 @@ __global__ __launch_bounds__(kBlock) void k_sesph_accel(const SceneDev* __restri
     if (k >= S.nf) return;
     const uint32_t c = S.state->cur;
     const uint32_t i = S.fluid_idx[k];
-    const float4* __restrict__ pos = S.pos[c];
-    const float4* __restrict__ vel = S.vel[c];
-    const float* __restrict__ pres = S.pres[c];
+    const auto pos = gload_ptr(S.pos[c]);
+    const auto vel = gload_ptr(S.vel[c]);
+    const auto pres = gload_ptr(S.pres[c]);
     const float4 xi = pos[i];
     const float4 vi = vel[i];
     const float rho_i = vi.w, p_i = pres[i];
     const float pdd = rho_i == 0.0f ? 0.0f : p_i / (rho_i * rho_i);
     const bool sl = (S.flags & kFlagSingleLayer) != 0;
-    const uint32_t cnt = S.nbr_cnt[k];
-    const uint32_t stride = S.list_stride;
-    const uint32_t* __restrict__ nb = S.nbr + k;
     float tx = 0.0f, ty = 0.0f, tz = 0.0f;
     float ax = 0.0f, ay = 0.0f, az = 0.0f;
-    for (uint32_t e = 0; e < cnt; e++) {
-        const uint32_t ent = nb[e * stride];
-        const uint32_t j = ent & kIndexMask;
-        const float4 xj = pos[j];
-        const float4 vj = vel[j];
-        float dx, dy, dz, d2, len;
-        Grad g;
-        pair_geometry<Spec>(S, xi, xj, dx, dy, dz, d2, len, g);
-        const float nd = vj.w;
-        if (nd != 0.0f) {
-            const float vx = vi.x - vj.x, vy = vi.y - vj.y, vz = vi.z - vj.z;
-            const float t = (xj.w / nd) * (dot3(vx, vy, vz, dx, dy, dz) / (d2 + S.visc_c01));
-            tx = tx + t * g.x;
-            ty = ty + t * g.y;
-            tz = tz + t * g.z;
-        }
-        if (ent & kBoundaryBit) {
-            const float f = -xi.w * (pdd + pdd);
-            if (sl) {
-                ax = ax + (f * g.x) * S.gamma2;
-                ay = ay + (f * g.y) * S.gamma2;
-                az = az + (f * g.z) * S.gamma2;
+    walk_neighbors<kUnroll>(
+        gload_ptr(S.nbr + k), S.list_stride, S.nbr_cnt[k],
+        [&](uint32_t j) { return PosVelP{pos[j], vel[j], pres[j]}; },
+        [&](uint32_t ent, const PosVelP& d) {
+            const float4 xj = d.x, vj = d.v;
+            float dx, dy, dz, d2, len;
+            Grad g;
+            pair_geometry<Spec>(S, xi, xj, dx, dy, dz, d2, len, g);
+            const float nd = vj.w;
+            if (nd != 0.0f) {
+                const float vx = vi.x - vj.x, vy = vi.y - vj.y, vz = vi.z - vj.z;
+                const float t = (xj.w / nd) * (dot3(vx, vy, vz, dx, dy, dz) / (d2 + S.visc_c01));
+                tx = tx + t * g.x;
+                ty = ty + t * g.y;
+                tz = tz + t * g.z;
+            }
+            if (ent & kBoundaryBit) {
+                const float f = -xi.w * (pdd + pdd);
+                if (sl) {
+                    ax = ax + (f * g.x) * S.gamma2;
+                    ay = ay + (f * g.y) * S.gamma2;
+                    az = az + (f * g.z) * S.gamma2;
+                } else {
+                    ax = ax + f * g.x;
+                    ay = ay + f * g.y;
+                    az = az + f * g.z;
+                }
             } else {
+                const float npdd = nd == 0.0f ? 0.0f : d.p / (nd * nd);
+                const float f = -xj.w * (pdd + npdd);
                 ax = ax + f * g.x;
                 ay = ay + f * g.y;
                 az = az + f * g.z;
             }
-        } else {
-            const float npdd = nd == 0.0f ? 0.0f : pres[j] / (nd * nd);
-            const float f = -xj.w * (pdd + npdd);
-            ax = ax + f * g.x;
-            ay = ay + f * g.y;
-            az = az + f * g.z;
-        }
-    }
+        });
     float nx = 0.0f + 0.0f, ny = 0.0f + (-S.gravity), nz = 0.0f + 0.0f;
     nx = nx + S.visc2 * tx;
     ny = ny + S.visc2 * ty;

@@ -221,6 +221,64 @@ FSA_API int32_t fsa_batch_set_graph(fsa_context ctx, uint32_t enabled);
  * 64, 128, 256, 512 or 1024); 0 restores the built-in default. */
 FSA_API int32_t fsa_context_set_block_size(fsa_context ctx, uint32_t threads);
 
+/* ---- serving: continuous batching of scene jobs ---------------------------- */
+/* A server owns a FIFO of submitted scene jobs.  Each fsa_server_tick admits
+ * queued jobs while the HBM budget and the slot limit allow (admission is by
+ * fsa_scene_memory_estimate), advances every running scene by one step in
+ * shared batched launches, and retires finished jobs: their final state moves
+ * to host memory and their device memory is released, so jobs that arrive
+ * while others run are admitted as memory frees up.  Results of a job are
+ * identical to running it alone. */
+#define FSA_JOB_QUEUED 0
+#define FSA_JOB_RUNNING 1
+#define FSA_JOB_DONE 2
+#define FSA_JOB_FAILED 3
+#define FSA_JOB_CANCELLED 4
+
+typedef struct fsa_server_config {
+    uint64_t hbm_budget_bytes;   /* admission budget; 0 = the context's budget */
+    uint32_t max_running;        /* slot limit; 0 = no limit */
+    uint32_t reserved[5];
+} fsa_server_config;
+
+typedef struct fsa_job_info {
+    uint64_t job_id;
+    uint32_t state;              /* FSA_JOB_* */
+    uint32_t status;             /* FSA_OK or the error that failed the job */
+    uint32_t steps_done;
+    uint32_t steps_total;
+    uint64_t fluid_particles;
+    double submit_s, admit_s, finish_s;  /* server clock, seconds */
+    fsa_step_stats last;
+} fsa_job_info;
+
+typedef struct fsa_server_stats {
+    uint64_t ticks;
+    uint64_t scene_steps;        /* sum over ticks of running scenes */
+    uint64_t particle_updates;   /* fluid particles x steps */
+    uint64_t jobs_submitted, jobs_done, jobs_failed, jobs_queued, jobs_running;
+    uint64_t hbm_in_use;
+    double busy_s;               /* time spent inside ticks */
+    double p50_step_ms, p99_step_ms, mean_step_ms, max_step_ms;  /* per scene-step latency */
+} fsa_server_stats;
+
+typedef struct fsa_server_s* fsa_server;
+
+FSA_API int32_t fsa_server_create(fsa_context ctx, const fsa_server_config* cfg, fsa_server* out);
+FSA_API int32_t fsa_server_destroy(fsa_server srv);
+/* Copies the particles of the job; the caller's view may be released after the call. */
+FSA_API int32_t fsa_server_submit(fsa_server srv, const fsa_scene_desc* desc, const fsa_particle_view* view,
+                                  uint32_t steps, uint64_t* job_id);
+FSA_API int32_t fsa_server_cancel(fsa_server srv, uint64_t job_id);
+/* Admit, step every running scene once, retire; *running receives the number
+ * of scenes stepped (0 when nothing could run). */
+FSA_API int32_t fsa_server_tick(fsa_server srv, uint32_t* running);
+FSA_API int32_t fsa_server_job(fsa_server srv, uint64_t job_id, fsa_job_info* info);
+/* Writes the final state of a finished job into the caller's view (caller
+ * order) and forgets the job. */
+FSA_API int32_t fsa_server_collect(fsa_server srv, uint64_t job_id, const fsa_particle_view* view);
+FSA_API int32_t fsa_server_get_stats(fsa_server srv, fsa_server_stats* stats);
+
 /* ---- timing -------------------------------------------------------------- */
 /* Average device time (ms) of each kernel class since the last reset, measured
  * with HIP events on the context stream when timing is enabled. */
