@@ -61,6 +61,126 @@ def cpu_baseline(desc, coll, steps, threads):
     return nf * steps / dt, O.get_threads(), dt
 
 
+def sweep_descs(sweep):
+    descs = []
+    for sc in sweep:
+        descs.append(abi.default_desc(3, abi.SOLVER_IISPH, iisph_gamma=sc["gamma"], iisph_viscosity=sc["viscosity"]))
+    return descs
+
+
+def bench_batched(ctx, n_scenes, steps, alone_steps):
+    """Parameter sweep of independent dam scenes (fluid counts log-uniform in
+    [5k, 200k], gamma U[1,7], viscosity U[0.001, 0.05]): all scenes advanced
+    together in shared launches, against the same scenes stepped one at a
+    time.  Both sides run search + full IISPH step per scene-step."""
+    sweep = scenes.config4_sweep(seed=4, n_scenes=n_scenes)
+    descs = sweep_descs(sweep)
+    nfs = [int(np.count_nonzero(s["collection"].type == abi.TYPE_NORMAL)) for s in sweep]
+    nbs = [int(np.count_nonzero(s["collection"].type == abi.TYPE_BOUNDARY)) for s in sweep]
+    total_f = sum(nfs)
+    log(f"batched sweep: {n_scenes} scenes, {total_f} fluid + {sum(nbs)} boundary")
+
+    # one scene at a time: each scene's steps back to back, scenes in sequence
+    alone = [Scene(ctx, d, s["collection"].copy()) for d, s in zip(descs, sweep)]
+    for sc in alone:
+        sc.step(1)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for sc in alone:
+        sc.step(alone_steps)
+    ctx.synchronize()
+    t_alone = time.perf_counter() - t0
+    for sc in alone:
+        sc.close()
+    alone_pus = total_f * alone_steps / t_alone
+    log(f"  one at a time: {alone_steps} steps each in {t_alone:.2f}s -> {alone_pus:.3e} particle-updates/s")
+
+    batch = [Scene(ctx, d, s["collection"].copy()) for d, s in zip(descs, sweep)]
+    ctx.batch_step(batch, 1)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    st = ctx.batch_step(batch, steps)
+    ctx.synchronize()
+    t_batch = time.perf_counter() - t0
+    for sc in batch:
+        sc.close()
+    pus = total_f * steps / t_batch
+    log(f"  batched: {steps} steps in {t_batch:.2f}s -> {pus:.3e} particle-updates/s "
+        f"(last-step iterations {sorted(set(x.iterations for x in st))})")
+    return {
+        "workload": f"{n_scenes} independent 3D IISPH dam scenes, fluid log-uniform in [5k, 200k] "
+        f"({total_f} fluid, {sum(nbs)} boundary), gamma U[1,7], viscosity U[0.001,0.05]",
+        "steps": steps,
+        "scenes_per_s": n_scenes / t_batch,
+        "scene_steps_per_s": n_scenes * steps / t_batch,
+        "particle_updates_per_s": pus,
+        "ms_per_step": 1000.0 * t_batch / steps,
+        "one_at_a_time_particle_updates_per_s": alone_pus,
+        "one_at_a_time_steps": alone_steps,
+        "speedup_vs_one_at_a_time": pus / alone_pus,
+    }
+
+
+def _templates(lo, hi, buckets, seed):
+    """Dam scenes at log-spaced fluid counts; arrivals reuse the nearest one."""
+    sizes = np.exp(np.linspace(np.log(lo), np.log(hi), buckets))
+    return [(int(n), scenes.scaled_dam3d(int(n), seed * 100 + k)) for k, n in enumerate(sizes)]
+
+
+def bench_served(ctx, window_s, rate, max_running, budget_gb, buckets=12):
+    """Served scene stream: Poisson arrivals (rate/s) of 3D IISPH dam jobs with
+    fluid counts log-uniform in [10k, 500k] and 100-2000 steps.  Jobs are
+    submitted when their arrival time has passed, the server admits them
+    against the HBM budget and slot limit, and every tick steps all running
+    jobs once.  The run stops at the end of the window; unfinished jobs count
+    only through their completed steps."""
+    from fluidsolver_amd.serving import Server
+
+    arrivals = scenes.config5_arrivals(seed=5, duration_s=window_s, rate=rate)
+    tmpl = _templates(10000, 500000, buckets, 5)
+    tsizes = np.array([np.log(n) for n, _ in tmpl])
+    desc = abi.default_desc(3, abi.SOLVER_IISPH)
+    srv = Server(ctx, hbm_budget_bytes=int(budget_gb * 1e9), max_running=max_running)
+    k = 0
+    t0 = time.perf_counter()
+    next_log = t0 + 5.0
+    while True:
+        now = time.perf_counter() - t0
+        if now >= window_s:
+            break
+        while k < len(arrivals) and arrivals[k][0] <= now:
+            _, nfl, steps = arrivals[k]
+            _, coll = tmpl[int(np.argmin(np.abs(tsizes - np.log(nfl))))]
+            srv.submit(desc, coll, steps)
+            k += 1
+        if srv.tick() == 0:
+            time.sleep(0.001)
+        if time.perf_counter() > next_log:
+            st = srv.stats()
+            log(f"  served t={now:.0f}s: done {st['jobs_done']} running {st['jobs_running']} "
+                f"queued {st['jobs_queued']} p99 {st['p99_step_ms']:.2f} ms")
+            next_log += 5.0
+    elapsed = time.perf_counter() - t0
+    st = srv.stats()
+    srv.close()
+    return {
+        "workload": f"Poisson arrivals {rate}/s of 3D IISPH dam jobs, fluid log-uniform in [10k, 500k] "
+        f"({buckets} size templates), 100-2000 steps each",
+        "window_s": window_s,
+        "max_running": max_running,
+        "hbm_budget_gb": budget_gb,
+        "jobs_arrived": k,
+        "jobs_done": st["jobs_done"],
+        "scenes_per_s": st["jobs_done"] / elapsed,
+        "scene_steps_per_s": st["scene_steps"] / elapsed,
+        "particle_updates_per_s": st["particle_updates"] / elapsed,
+        "p50_step_ms": st["p50_step_ms"],
+        "p99_step_ms": st["p99_step_ms"],
+        "mean_step_ms": st["mean_step_ms"],
+        "busy_fraction": st["busy_s"] / elapsed,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -71,6 +191,15 @@ def main():
     ap.add_argument("--cpu-side", type=int, default=100)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--profile-steps", type=int, default=3)
+    ap.add_argument("--batch-scenes", type=int, default=64)
+    ap.add_argument("--batch-steps", type=int, default=500)
+    ap.add_argument("--alone-steps", type=int, default=50)
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--serve-seconds", type=float, default=30.0)
+    ap.add_argument("--serve-rate", type=float, default=20.0)
+    ap.add_argument("--serve-slots", type=int, default=32)
+    ap.add_argument("--serve-budget-gb", type=float, default=200.0)
+    ap.add_argument("--no-serve", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus above 1 is not supported"}))
@@ -140,6 +269,15 @@ def main():
         "avg_launch_ms": phases[dom]["avg_ms"],
     }
 
+    sc.close()
+    batched = None if args.no_batch else bench_batched(ctx, args.batch_scenes, args.batch_steps, args.alone_steps)
+    if batched:
+        log(json.dumps(batched))
+    served = None
+    if not args.no_serve:
+        served = bench_served(ctx, args.serve_seconds, args.serve_rate, args.serve_slots, args.serve_budget_gb)
+        log(json.dumps(served))
+
     cpu = None
     if not args.no_cpu:
         cside = args.cpu_side
@@ -156,7 +294,7 @@ def main():
         log(json.dumps(cpu))
 
     out = {
-        "metric": "IISPH particle-updates/s @1M 3D",
+        "metric": "IISPH particle-updates/s @1M 3D; batched scenes/s + p99 step latency",
         "value": value,
         "unit": "particle-updates/s",
         "n_gpus": 1,
@@ -177,9 +315,10 @@ def main():
         },
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "batched": batched,
+        "served": served,
     }
     print(json.dumps(out), flush=True)
-    sc.close()
     ctx.close()
 
 

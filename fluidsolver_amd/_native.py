@@ -7,7 +7,7 @@ import ctypes as C
 import os
 import re
 
-from .abi import ParticleView, SceneDesc, StepStats
+from .abi import JobInfo, ParticleView, SceneDesc, ServerConfig, ServerStats, StepStats
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "lib", "libfluidsolver_amd.so")
@@ -49,6 +49,14 @@ def _declare(L):
         "fsa_context_set_block_size": (i32, [vp, u32]),
         "fsa_context_set_timing": (i32, [vp, u32]),
         "fsa_context_get_timing": (i32, [vp, P(C.c_double), P(u64), u32]),
+        "fsa_server_create": (i32, [vp, P(ServerConfig), P(vp)]),
+        "fsa_server_destroy": (i32, [vp]),
+        "fsa_server_submit": (i32, [vp, P(SceneDesc), P(ParticleView), u32, P(u64)]),
+        "fsa_server_cancel": (i32, [vp, u64]),
+        "fsa_server_tick": (i32, [vp, P(u32)]),
+        "fsa_server_job": (i32, [vp, u64, P(JobInfo)]),
+        "fsa_server_collect": (i32, [vp, u64, P(ParticleView)]),
+        "fsa_server_get_stats": (i32, [vp, P(ServerStats)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)

@@ -36,6 +36,14 @@ TYPE_INACTIVE = 2
 FLAG_EMULATE_INT_ABS = 1
 MAX_NEIGHBORS = 97
 
+JOB_QUEUED = 0
+JOB_RUNNING = 1
+JOB_DONE = 2
+JOB_FAILED = 3
+JOB_CANCELLED = 4
+JOB_STATE_NAMES = {JOB_QUEUED: "queued", JOB_RUNNING: "running", JOB_DONE: "done", JOB_FAILED: "failed",
+                   JOB_CANCELLED: "cancelled"}
+
 TIMER_NAMES = ["search", "density", "diagonal", "pressure_a", "pressure_b", "integrate"]
 
 
@@ -126,6 +134,51 @@ class StepStats(C.Structure):
         ("average_density_error", C.c_double),
         ("max_velocity", C.c_float),
         ("max_acceleration", C.c_float),
+    ]
+
+    def as_dict(self):
+        return {f[0]: getattr(self, f[0]) for f in self._fields_}
+
+
+class ServerConfig(C.Structure):
+    _fields_ = [
+        ("hbm_budget_bytes", C.c_uint64),
+        ("max_running", C.c_uint32),
+        ("reserved", C.c_uint32 * 5),
+    ]
+
+
+class JobInfo(C.Structure):
+    _fields_ = [
+        ("job_id", C.c_uint64),
+        ("state", C.c_uint32),
+        ("status", C.c_uint32),
+        ("steps_done", C.c_uint32),
+        ("steps_total", C.c_uint32),
+        ("fluid_particles", C.c_uint64),
+        ("submit_s", C.c_double),
+        ("admit_s", C.c_double),
+        ("finish_s", C.c_double),
+        ("last", StepStats),
+    ]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [
+        ("ticks", C.c_uint64),
+        ("scene_steps", C.c_uint64),
+        ("particle_updates", C.c_uint64),
+        ("jobs_submitted", C.c_uint64),
+        ("jobs_done", C.c_uint64),
+        ("jobs_failed", C.c_uint64),
+        ("jobs_queued", C.c_uint64),
+        ("jobs_running", C.c_uint64),
+        ("hbm_in_use", C.c_uint64),
+        ("busy_s", C.c_double),
+        ("p50_step_ms", C.c_double),
+        ("p99_step_ms", C.c_double),
+        ("mean_step_ms", C.c_double),
+        ("max_step_ms", C.c_double),
     ]
 
     def as_dict(self):

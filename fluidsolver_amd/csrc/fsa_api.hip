@@ -95,6 +95,8 @@ struct fsa_context_s {
     SceneState* h_states = nullptr;  // pinned mirror
     std::vector<uint32_t> free_slots;
     std::map<std::vector<fsa_scene_s*>, struct BatchSet*> batches;
+    uint64_t batch_clock = 0;
+    std::vector<struct fsa_server_s*> servers;  // destroyed with the context
 };
 
 // Launch set over several scenes (one table entry and block-map range each).
@@ -104,6 +106,7 @@ struct BatchSet {
     std::vector<SceneDev> h_table;
     LaunchSet ls{};
     std::vector<BlockRef*> owned;
+    uint64_t last_use = 0;
 };
 
 struct fsa_scene_s {
@@ -554,10 +557,27 @@ void do_simulate(fsa_scene_s* s, float dt, fsa_step_stats* stats) {
 }
 
 // ---- batching -------------------------------------------------------------------------
+void free_batch(BatchSet* b);
+
+// Launch tables are cached per scene set; a serving loop changes the set as
+// jobs come and go, so the cache keeps the most recently used sets only.
+constexpr size_t kBatchCacheSize = 16;
+
 BatchSet* batch_for(fsa_context ctx, const std::vector<fsa_scene_s*>& v) {
     auto it = ctx->batches.find(v);
-    if (it != ctx->batches.end()) return it->second;
+    if (it != ctx->batches.end()) {
+        it->second->last_use = ++ctx->batch_clock;
+        return it->second;
+    }
+    while (ctx->batches.size() >= kBatchCacheSize) {
+        auto lru = ctx->batches.begin();
+        for (auto jt = ctx->batches.begin(); jt != ctx->batches.end(); ++jt)
+            if (jt->second->last_use < lru->second->last_use) lru = jt;
+        free_batch(lru->second);
+        ctx->batches.erase(lru);
+    }
     auto* b = new BatchSet();
+    b->last_use = ++ctx->batch_clock;
     b->scenes = v;
     b->h_table.resize(v.size());
     HIP_CHECK(hipMalloc(&b->d_table, sizeof(SceneDev) * v.size()));
@@ -596,8 +616,10 @@ void upload_batch_table(fsa_context ctx, BatchSet* b) {
                              hipMemcpyHostToDevice, ctx->stream));
 }
 
-// One step of every scene in v (same solver), in shared launches.
-void batch_step_group(fsa_context ctx, const std::vector<fsa_scene_s*>& v, std::vector<fsa_step_stats*>& out) {
+// One step of every scene in v (same solver), in shared launches.  A scene
+// whose search fails (neighbor capacity) is not simulated: its stats carry the
+// error, the other scenes complete the step, and the first error is returned.
+int32_t batch_step_group(fsa_context ctx, const std::vector<fsa_scene_s*>& v, std::vector<fsa_step_stats*>& out) {
     BatchSet* b = batch_for(ctx, v);
     uint32_t passes = 1;
     for (auto* s : v) {
@@ -608,19 +630,41 @@ void batch_step_group(fsa_context ctx, const std::vector<fsa_scene_s*>& v, std::
     search_set(ctx, b->ls, passes);
     read_states(ctx, v);
     bool bpass = false;
-    for (auto* s : v) {
-        check_status(s);
+    int32_t first_error = FSA_OK;
+    std::vector<fsa_scene_s*> ok;
+    std::vector<fsa_step_stats*> ok_out;
+    for (uint32_t q = 0; q < v.size(); q++) {
+        fsa_scene_s* s = v[q];
+        try {
+            check_status(s);
+        } catch (const FsaError& e) {
+            s->searched = false;
+            s->lists_valid = false;
+            if (out[q]) {
+                fill_stats(s, out[q]);
+                out[q]->status = (uint32_t)e.code;
+            }
+            if (first_error == FSA_OK) {
+                first_error = e.code;
+                g_error = e.what();
+            }
+            continue;
+        }
         s->searched = true;
         s->lists_valid = true;
         s->lists_cached = false;
         pre_simulate(s, s->desc.timestep);
         bpass = bpass || (s->hd.flags & kFlagDynamicBoundary) || s->boundary_dirty;
+        ok.push_back(s);
+        ok_out.push_back(out[q]);
     }
+    if (ok.empty()) return first_error;
+    if (ok.size() != v.size()) b = batch_for(ctx, ok);
     upload_batch_table(ctx, b);
-    std::vector<fsa_scene_s*> vv = v;
-    simulate_set(ctx, b->ls, vv, v[0]->desc.solver == FSA_SOLVER_SESPH, bpass);
-    read_states(ctx, v);
-    for (uint32_t q = 0; q < v.size(); q++) post_simulate(v[q], out[q]);
+    simulate_set(ctx, b->ls, ok, ok[0]->desc.solver == FSA_SOLVER_SESPH, bpass);
+    read_states(ctx, ok);
+    for (uint32_t q = 0; q < ok.size(); q++) post_simulate(ok[q], ok_out[q]);
+    return first_error;
 }
 
 // ---- upload / download ------------------------------------------------------------------
@@ -965,11 +1009,24 @@ void admit(fsa_server srv) {
         // scene plan + cell table allowance (the grid of a dam scene is about the particle count)
         const uint64_t need = plan_bytes(make_plan(j->fluid + nb, j->fluid)) + sizeof(uint4) * (j->fluid + nb);
         const uint64_t budget = server_budget(srv);
-        if (budget && srv->ctx->in_use.load() + need > budget) return;  // FIFO: wait for memory
+        if (budget && srv->ctx->in_use.load() + need > budget) {
+            if (!srv->running.empty()) return;  // FIFO: wait until running jobs free memory
+            // nothing will free memory: the job can never be admitted
+            srv->queue.pop_front();
+            j->state = FSA_JOB_FAILED;
+            j->status = FSA_E_BUDGET;
+            j->finish_s = srv->now();
+            srv->failed++;
+            continue;
+        }
         srv->queue.pop_front();
         fsa_particle_view v = j->view();
         fsa_scene sc = nullptr;
         const int32_t rc = fsa_scene_create(srv->ctx, &j->desc, &v, &sc);
+        if (rc == FSA_E_BUDGET && !srv->running.empty()) {
+            srv->queue.push_front(j->id);  // the estimate was short: wait for running jobs to free memory
+            return;
+        }
         if (rc != FSA_OK) {
             j->state = FSA_JOB_FAILED;
             j->status = (uint32_t)rc;
@@ -982,6 +1039,16 @@ void admit(fsa_server srv) {
         j->admit_s = srv->now();
         srv->running.push_back(j->id);
     }
+}
+
+void destroy_server(fsa_server srv) {
+    for (auto& kv : srv->jobs) {
+        if (kv.second->scene) destroy_scene(kv.second->scene);
+        delete kv.second;
+    }
+    auto& v = srv->ctx->servers;
+    v.erase(std::remove(v.begin(), v.end(), srv), v.end());
+    delete srv;
 }
 
 void retire(fsa_server srv, Job* j, uint32_t state, uint32_t status) {
@@ -1075,6 +1142,7 @@ FSA_API int32_t fsa_context_destroy(fsa_context c) {
         if (!c) return;
         (void)hipSetDevice(c->device);
         (void)hipStreamSynchronize(c->stream);
+        for (auto* srv : std::vector<fsa_server_s*>(c->servers)) destroy_server(srv);
         std::vector<fsa_scene_s*> live;
         {
             std::lock_guard<std::mutex> g(c->mu);
@@ -1316,9 +1384,14 @@ FSA_API int32_t fsa_batch_step(fsa_context c, const fsa_scene* scenes, uint32_t 
             gstats[g].push_back(stats ? &stats[q] : nullptr);
         }
         HIP_CHECK(hipSetDevice(c->device));
-        for (uint32_t k = 0; k < steps; k++)
+        int32_t first_error = FSA_OK;
+        for (uint32_t k = 0; k < steps && first_error == FSA_OK; k++)
             for (int g = 0; g < 2; g++)
-                if (!groups[g].empty()) batch_step_group(c, groups[g], gstats[g]);
+                if (!groups[g].empty()) {
+                    const int32_t rc = batch_step_group(c, groups[g], gstats[g]);
+                    if (first_error == FSA_OK) first_error = rc;
+                }
+        if (first_error != FSA_OK) throw FsaError(first_error, g_error);
     });
 }
 
@@ -1366,18 +1439,14 @@ FSA_API int32_t fsa_server_create(fsa_context c, const fsa_server_config* cfg, f
         auto* s = new fsa_server_s();
         s->ctx = c;
         if (cfg) s->cfg = *cfg;
+        c->servers.push_back(s);
         *out = s;
     });
 }
 
 FSA_API int32_t fsa_server_destroy(fsa_server srv) {
     return guarded([&] {
-        if (!srv) return;
-        for (auto& kv : srv->jobs) {
-            if (kv.second->scene) destroy_scene(kv.second->scene);
-            delete kv.second;
-        }
-        delete srv;
+        if (srv) destroy_server(srv);
     });
 }
 
@@ -1447,6 +1516,8 @@ FSA_API int32_t fsa_server_tick(fsa_server srv, uint32_t* running) {
         if (running) *running = (uint32_t)scenes.size();
         if (scenes.empty()) return;
         std::vector<fsa_step_stats> st(scenes.size());
+        constexpr uint32_t kNotStepped = 0xffffffffu;
+        for (auto& e : st) e.status = kNotStepped;
         const int32_t rc = fsa_batch_step(srv->ctx, scenes.data(), (uint32_t)scenes.size(), 1, st.data());
         const double t1 = srv->now();
         srv->ticks++;
@@ -1455,8 +1526,10 @@ FSA_API int32_t fsa_server_tick(fsa_server srv, uint32_t* running) {
         srv->scene_steps += scenes.size();
         for (size_t q = 0; q < js.size(); q++) {
             Job* j = js[q];
-            if (rc != FSA_OK) {
-                retire(srv, j, FSA_JOB_FAILED, (uint32_t)rc);
+            // a failed batch reports per scene: scenes whose status is OK completed the step
+            const uint32_t status = rc == FSA_OK ? FSA_OK : (st[q].status == kNotStepped ? (uint32_t)rc : st[q].status);
+            if (status != FSA_OK) {
+                retire(srv, j, FSA_JOB_FAILED, status);
                 continue;
             }
             j->last = st[q];

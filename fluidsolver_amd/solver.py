@@ -42,6 +42,7 @@ class Context:
         check(lib().fsa_context_create(int(device), int(hbm_budget_bytes), C.byref(h)))
         self._h = h
         self._scenes = weakref.WeakSet()
+        self._servers = weakref.WeakSet()
 
     @property
     def handle(self):
@@ -49,6 +50,8 @@ class Context:
 
     def close(self):
         if getattr(self, "_h", None) is not None and self._h.value:
+            for srv in list(self._servers):
+                srv.close()
             for sc in list(self._scenes):
                 sc.close()
             lib().fsa_context_destroy(self._h)

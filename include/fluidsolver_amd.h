@@ -264,6 +264,7 @@ typedef struct fsa_server_stats {
 
 typedef struct fsa_server_s* fsa_server;
 
+/* A server is destroyed with its context if not destroyed before. */
 FSA_API int32_t fsa_server_create(fsa_context ctx, const fsa_server_config* cfg, fsa_server* out);
 FSA_API int32_t fsa_server_destroy(fsa_server srv);
 /* Copies the particles of the job; the caller's view may be released after the call. */

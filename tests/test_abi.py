@@ -53,6 +53,10 @@ def test_null_arguments_return_status():
     assert L.fsa_scene_find_neighbors(None) == abi.FSA_E_INVALID
     assert L.fsa_scene_simulate(None, 0.001, None) == abi.FSA_E_INVALID
     assert L.fsa_context_create(0, 0, None) == abi.FSA_E_INVALID
+    assert L.fsa_server_create(None, None, None) == abi.FSA_E_INVALID
+    assert L.fsa_server_tick(None, None) == abi.FSA_E_INVALID
+    assert L.fsa_server_submit(None, None, None, 1, None) == abi.FSA_E_INVALID
+    assert L.fsa_server_get_stats(None, None) == abi.FSA_E_INVALID
 
 
 LAYOUT_C = r"""
@@ -63,6 +67,9 @@ int main(void) {
   printf("%zu %zu %zu\n", sizeof(fsa_scene_desc), sizeof(fsa_particle_view), sizeof(fsa_step_stats));
   printf("%zu %zu %zu\n", offsetof(fsa_scene_desc, timestep), offsetof(fsa_particle_view, external_stride),
          offsetof(fsa_step_stats, average_density_error));
+  printf("%zu %zu %zu\n", sizeof(fsa_server_config), sizeof(fsa_job_info), sizeof(fsa_server_stats));
+  printf("%zu %zu %zu\n", offsetof(fsa_job_info, last), offsetof(fsa_server_stats, busy_s),
+         offsetof(fsa_server_stats, max_step_ms));
   return 0;
 }
 """
@@ -81,6 +88,10 @@ def test_struct_layouts_match_header(tmp_path):
     assert sizes[3] == abi.SceneDesc.timestep.offset
     assert sizes[4] == abi.ParticleView.external_stride.offset
     assert sizes[5] == abi.StepStats.average_density_error.offset
+    assert sizes[6:9] == [C.sizeof(abi.ServerConfig), C.sizeof(abi.JobInfo), C.sizeof(abi.ServerStats)]
+    assert sizes[9] == abi.JobInfo.last.offset
+    assert sizes[10] == abi.ServerStats.busy_s.offset
+    assert sizes[11] == abi.ServerStats.max_step_ms.offset
 
 
 def test_info_record_layout():
