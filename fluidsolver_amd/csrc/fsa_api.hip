@@ -672,6 +672,13 @@ void do_upload(fsa_scene_s* s, const fsa_particle_view* v, bool first) {
     if (!v || !v->movement || !v->particle_data || !v->info) throw FsaError(FSA_E_INVALID, "incomplete particle view");
     const uint64_t N = v->count;
     if (N >= (1ull << kTypeShift)) throw FsaError(FSA_E_INVALID, "too many particles (max 2^30)");
+    {
+        // neighbor-list entries are addressed with 32-bit indices (row * stride + rank)
+        uint64_t nf_all = 0;
+        for (uint64_t i = 0; i < N; i++) nf_all += ((const uint8_t*)v->info + i * v->info_stride)[4] == FSA_TYPE_NORMAL;
+        if ((uint64_t)kMaxNeighbors * make_plan(N, nf_all).stride > 0xffffffffull)
+            throw FsaError(FSA_E_INVALID, "too many fluid particles in one scene (max 44,000,000)");
+    }
     std::vector<uint32_t> active;
     active.reserve(N);
     std::vector<uint32_t> inactive;

@@ -294,12 +294,13 @@ __global__ __launch_bounds__(kBlock) void k_cell_table(const SceneDev* __restric
 // ---- 6. neighbor lists ---------------------------------------------------------
 // One wave per 64 consecutive sorted particles processes every cell whose run
 // starts among them.  For a cell, the wave sorts the runs of the 27 neighbor
-// cells by start (= by cell key, the reference's std::sort of the 27 keys),
-// stages their particles in LDS in ascending combined index, and filters them
-// for each fluid particle of the cell with a ballot; the ballot prefix keeps
-// every list in ascending order.
+// cells by start (= by cell key, the reference's std::sort of the 27 keys), so
+// candidate g of the concatenated runs has ascending combined index.  Chunks of
+// 64 candidates are held one per lane; the cell's fluid particles (one per lane
+// as well) are broadcast in turn and tested against the whole chunk with one
+// ballot, whose prefix gives each accepted candidate its slot, so every list is
+// written in ascending order without staging.
 constexpr int kWaves = kBlock / 64;
-constexpr int kListChunk = 256;
 
 __device__ __forceinline__ uint32_t wave_sort_u32(uint32_t v) {
     // bitonic sort across the 64 lanes, ascending by lane
@@ -329,27 +330,27 @@ __device__ __forceinline__ uint32_t wave_exclusive_scan(uint32_t v, uint32_t& to
     return x - v;
 }
 
-__device__ __forceinline__ void wave_sync_lds() {
-    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
-    __builtin_amdgcn_wave_barrier();
+__device__ __forceinline__ float readlane_f(float v, uint32_t lane) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), (int)lane));
+}
+__device__ __forceinline__ uint32_t readlane_u(uint32_t v, uint32_t lane) {
+    return (uint32_t)__builtin_amdgcn_readlane((int)v, (int)lane);
 }
 
 __global__ __launch_bounds__(kBlock) void k_lists(const SceneDev* __restrict__ scenes,
                                                   const BlockRef* __restrict__ map, uint32_t sbuf) {
     const BlockRef br = map[blockIdx.x];
     const SceneDev& S = scenes[br.scene];
-    __shared__ float4 s_pos[kWaves][kListChunk];
-    __shared__ uint32_t s_j[kWaves][kListChunk];
-    __shared__ uint32_t s_cnt[kWaves][64];
     const uint32_t wave = threadIdx.x / 64;
     const uint32_t lane = __lane_id();
-    const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
     const uint32_t base = br.block * kBlock + wave * 64;
     if (base >= S.n) return;
     const uint32_t c = S.state->cur;
-    const float4* __restrict__ pos = S.pos[c];
-    const uint32_t* __restrict__ idt = S.idt[c];
-    const uint32_t* __restrict__ keys = S.keys[sbuf];
+    const auto pos = gload_ptr(S.pos[c]);
+    const auto idt = gload_ptr(S.idt[c]);
+    const auto keys = gload_ptr(S.keys[sbuf]);
+    const auto rank_of = gload_ptr(S.rank_of);
+    FSA_GLOBAL(uint32_t)* __restrict__ nbr = (FSA_GLOBAL(uint32_t)*)S.nbr;
     const float r = S.search_radius;
     const float r2 = S.r2;
     const uint32_t stride = S.list_stride;
@@ -393,70 +394,65 @@ __global__ __launch_bounds__(kBlock) void k_lists(const SceneDev* __restrict__ s
         uint32_t total;
         const uint32_t roff = wave_exclusive_scan(rl, total);
         for (uint32_t pbase = cs; pbase < ce; pbase += 64) {
+            // the cell's particles, one per lane
             const uint32_t pi = pbase + lane;
             const bool isf = pi < ce && (idt[pi] >> kTypeShift) == 0u;
             const uint64_t fmask = __ballot(isf);
             if (fmask == 0) continue;
-            for (uint32_t c0 = 0; c0 < total; c0 += kListChunk) {
-                const uint32_t nch = min((uint32_t)kListChunk, total - c0);
-                wave_sync_lds();
-                // every lane stages its candidates: run q of candidate g by binary search over the offsets
-                // uniform loop: every lane is active at each shuffle (a shuffle
-                // from an inactive source lane does not return its register)
-                for (uint32_t tb = 0; tb < nch; tb += 64) {
-                    const uint32_t t = tb + lane;
-                    const uint32_t g = c0 + t;
-                    uint32_t q = 0;
+            float4 xl = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            uint32_t kl = 0;
+            if (isf) {
+                xl = pos[pi];
+                kl = rank_of[pi];
+            }
+            uint32_t cntl = 0;
+            for (uint32_t c0 = 0; c0 < total; c0 += 64) {
+                // candidate g of the concatenated runs: run q by binary search over the offsets
+                // (uniform: every lane takes part in each shuffle)
+                const uint32_t g = c0 + lane;
+                uint32_t q = 0;
 #pragma unroll
-                    for (uint32_t step = 16; step > 0; step >>= 1) {
-                        const uint32_t probe = q + step;
-                        const uint32_t off = __shfl(roff, (int)min(probe, 63u), 64);
-                        if (probe < nvalid && off <= g) q = probe;
-                    }
-                    const uint32_t qs = __shfl(rs, (int)q, 64);
-                    const uint32_t qo = __shfl(roff, (int)q, 64);
-                    if (t < nch) {
-                        const uint32_t j = qs + (g - qo);
-                        s_pos[wave][t] = pos[j];
-                        s_j[wave][t] = j | ((idt[j] >> kTypeShift) == 1u ? kBoundaryBit : 0u);
-                    }
+                for (uint32_t step = 16; step > 0; step >>= 1) {
+                    const uint32_t probe = q + step;
+                    const uint32_t off = __shfl(roff, (int)min(probe, 63u), 64);
+                    if (probe < nvalid && off <= g) q = probe;
                 }
-                wave_sync_lds();
+                const uint32_t qs = __shfl(rs, (int)q, 64);
+                const uint32_t qo = __shfl(roff, (int)q, 64);
+                const bool valid = g < total;
+                // a lane past the last candidate sits at infinity and fails every test
+                float4 pj = make_float4(__builtin_inff(), __builtin_inff(), __builtin_inff(), 0.0f);
+                uint32_t ej = 0;
+                if (valid) {
+                    const uint32_t j = qs + (g - qo);
+                    pj = pos[j];
+                    ej = j | ((idt[j] >> kTypeShift) == 1u ? kBoundaryBit : 0u);
+                }
                 uint64_t fm = fmask;
                 while (fm) {
                     const uint32_t b = (uint32_t)__ffsll((unsigned long long)fm) - 1u;
                     fm &= fm - 1;
-                    const uint32_t p = pbase + b;
-                    const float4 xi = pos[p];
-                    const uint32_t k = S.rank_of[p];
-                    uint32_t cnt = c0 == 0 ? 0u : s_cnt[wave][b];
-                    uint32_t* __restrict__ out = S.nbr + k;
-                    for (uint32_t tb = 0; tb < nch; tb += 64) {
-                        const uint32_t t = tb + lane;
-                        bool pass = false;
-                        if (t < nch) {
-                            const float4 pj = s_pos[wave][t];
-                            const float dx = pj.x - xi.x, dy = pj.y - xi.y, dz = pj.z - xi.z;
-                            pass = dot3(dx, dy, dz, dx, dy, dz) <= r2;
-                        }
-                        const uint64_t m = __ballot(pass);
-                        if (pass) {
-                            const uint32_t slot = cnt + (uint32_t)__popcll(m & lt);
-                            if (slot < (uint32_t)kMaxNeighbors) out[slot * stride] = s_j[wave][t];
-                        }
-                        cnt += (uint32_t)__popcll(m);
+                    const float xs = readlane_f(xl.x, b), ys = readlane_f(xl.y, b), zs = readlane_f(xl.z, b);
+                    const float dx = pj.x - xs, dy = pj.y - ys, dz = pj.z - zs;
+                    const bool pass = dot3(dx, dy, dz, dx, dy, dz) <= r2;
+                    const uint64_t m = __ballot(pass);
+                    const uint32_t cb = readlane_u(cntl, b);
+                    if (pass) {
+                        // past the capacity the last row is overwritten; the count reports the overflow
+                        const uint32_t slot = cb + __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
+                                                                             __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+                        nbr[min(slot, (uint32_t)kMaxNeighbors - 1u) * stride + readlane_u(kl, b)] = ej;
                     }
-                    if (c0 + kListChunk < total) {
-                        if (lane == 0) s_cnt[wave][b] = cnt;
-                    } else {
-                        if (cnt > (uint32_t)kMaxNeighbors) overflow = true;
-                        if (lane == 0) S.nbr_cnt[k] = min(cnt, (uint32_t)kMaxNeighbors);
-                    }
+                    if (lane == b) cntl = cb + (uint32_t)__popcll(m);
                 }
+            }
+            if (isf) {
+                if (cntl > (uint32_t)kMaxNeighbors) overflow = true;
+                S.nbr_cnt[kl] = min(cntl, (uint32_t)kMaxNeighbors);
             }
         }
     }
-    if (overflow && lane == 0) atomicOr(&S.state->status, kStatusCapacity);
+    if (__any(overflow) && lane == 0) atomicOr(&S.state->status, kStatusCapacity);
 }
 
 // ---- launchers -------------------------------------------------------------------
