@@ -30,6 +30,10 @@ SOLVER_IISPH = 0
 SOLVER_SESPH = 1
 LIST_COMPRESSED = 0
 LIST_PLAIN = 1
+SEARCH_DEFAULT = 0
+SEARCH_COMPRESSED_3D = 1
+SEARCH_HASHED_2D = 4
+SEARCH_QUADRATIC_2D = 5
 TYPE_NORMAL = 0
 TYPE_BOUNDARY = 1
 TYPE_INACTIVE = 2
@@ -74,7 +78,8 @@ class SceneDesc(C.Structure):
         ("timestep", C.c_float),
         ("lambda_v", C.c_float),
         ("lambda_a", C.c_float),
-        ("reserved", C.c_uint32 * 5),
+        ("search", C.c_uint32),
+        ("reserved", C.c_uint32 * 4),
     ]
 
 
@@ -106,6 +111,7 @@ def default_desc(dimension=3, solver=SOLVER_IISPH, **overrides):
     d.timestep = 0.001
     d.lambda_v = 0.4
     d.lambda_a = 0.25
+    d.search = SEARCH_HASHED_2D if dimension == 2 else SEARCH_COMPRESSED_3D
     for k, v in overrides.items():
         setattr(d, k, v)
     return d

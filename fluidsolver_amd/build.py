@@ -15,7 +15,7 @@ LIBDIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIBDIR, "libfluidsolver_amd.so")
 OBJDIR = os.path.join(LIBDIR, "obj")
 ARCH = os.environ.get("FSA_OFFLOAD_ARCH", "gfx950")
-SOURCES = ["fsa_search.hip", "fsa_sph.hip", "fsa_api.hip"]
+SOURCES = ["fsa_search.hip", "fsa_sph.hip", "fsa_sph2d.hip", "fsa_api.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [

@@ -67,7 +67,7 @@ uint64_t align_up(uint64_t v, uint64_t a) { return (v + a - 1) / a * a; }
 
 // device-memory plan of one scene (all arrays carved from one allocation)
 struct Plan {
-    uint64_t n, nf, stride, nblocks, nfblocks, ntiles;
+    uint64_t n, nf, stride, nblocks, nfblocks, ntiles, dim;
     uint64_t bytes;
     std::vector<std::pair<void**, uint64_t>> parts;
 };
@@ -137,6 +137,12 @@ struct fsa_scene_s {
 
 namespace {
 
+// the search a description names, with FSA_SEARCH_DEFAULT resolved per dimension
+uint32_t search_of(const fsa_scene_desc* d) {
+    if (d->search != FSA_SEARCH_DEFAULT) return d->search;
+    return d->dimension == 2 ? FSA_SEARCH_HASHED_2D : FSA_SEARCH_COMPRESSED_3D;
+}
+
 void fill_params(fsa_scene_s* s) {
     const fsa_scene_desc& d = s->desc;
     SceneDev& h = s->hd;
@@ -147,7 +153,11 @@ void fill_params(fsa_scene_s* s) {
     h.r2 = h.search_radius * h.search_radius;
     h.h = h.search_radius / 2.0f;               // CubicSplineKernel3D::initialize (:67-70)
     const float PI = 3.14159265358979323846f;
-    h.alpha = 1.0f / (4.0f * PI * (h.h * h.h * h.h));
+    h.dim = d.dimension;
+    if (d.dimension == 3)
+        h.alpha = 1.0f / (4.0f * PI * (h.h * h.h * h.h));
+    else
+        h.alpha = 5.0f / (14.0f * PI * (h.h * h.h));  // CubicSplineKernel::initialize (CubicSplineKernel.cpp:70-80)
     h.rho0sq = d.rest_density * d.rest_density;
     h.visc_c01 = 0.01f * d.particle_size * d.particle_size;
     const bool sesph = d.solver == FSA_SOLVER_SESPH;
@@ -170,13 +180,19 @@ void fill_params(fsa_scene_s* s) {
     if (sesph ? d.sesph_single_layer_boundary : d.iisph_single_layer_boundary) f |= kFlagSingleLayer;
     if (sesph) f |= kFlagSesph;
     if (d.neighbor_lists == FSA_LIST_PLAIN) f |= kFlagPlainLists;
+    if (d.dimension == 2 && !sesph && (d.flags & FSA_FLAG_EMULATE_INT_ABS)) f |= kFlagIntAbs;
+    if (search_of(&d) == FSA_SEARCH_QUADRATIC_2D) f |= kFlagQuadratic;
     h.flags = f;
 }
 
 void validate_desc(const fsa_scene_desc* d) {
     if (!d) throw FsaError(FSA_E_INVALID, "desc is NULL");
-    if (d->dimension != 3)
-        throw FsaError(FSA_E_INVALID, "this build of the device path implements dimension 3");
+    if (d->dimension != 2 && d->dimension != 3) throw FsaError(FSA_E_INVALID, "dimension must be 2 or 3");
+    const uint32_t search = search_of(d);
+    if (d->dimension == 3 && search != FSA_SEARCH_COMPRESSED_3D)
+        throw FsaError(FSA_E_INVALID, "3D scenes use the compressed-3d neighbor order");
+    if (d->dimension == 2 && search != FSA_SEARCH_HASHED_2D && search != FSA_SEARCH_QUADRATIC_2D)
+        throw FsaError(FSA_E_INVALID, "2D scenes use the hashed or quadratic-dynamic-allocated neighbor order");
     if (d->solver != FSA_SOLVER_IISPH && d->solver != FSA_SOLVER_SESPH)
         throw FsaError(FSA_E_INVALID, "unknown solver");
     if (!(d->particle_size > 0.0f)) throw FsaError(FSA_E_INVALID, "particle_size must be > 0");
@@ -190,8 +206,9 @@ void validate_desc(const fsa_scene_desc* d) {
     }
 }
 
-Plan make_plan(uint64_t n, uint64_t nf) {
+Plan make_plan(uint64_t n, uint64_t nf, uint32_t dim = 3) {
     Plan p{};
+    p.dim = dim;
     p.n = n;
     p.nf = nf;
     p.stride = align_up(std::max<uint64_t>(nf, 1), 64);
@@ -221,6 +238,10 @@ uint64_t plan_bytes(const Plan& p) {
     add(4 * std::max<uint64_t>(p.nf, 1));      // nbr_cnt
     add(sizeof(Partial) * std::max<uint64_t>(p.nblocks, 1));
     add(sizeof(BoxPartial) * std::max<uint64_t>(p.nblocks, 1));
+    if (p.dim == 2) {
+        add(8 * n);  // cell_prev
+        add(4 * n);  // entry
+    }
     return b;
 }
 
@@ -256,6 +277,10 @@ void carve(fsa_scene_s* s, const Plan& p) {
     h.nbr_cnt = (uint32_t*)take(4 * std::max<uint64_t>(p.nf, 1));
     h.partial = (Partial*)take(sizeof(Partial) * std::max<uint64_t>(p.nblocks, 1));
     h.box = (BoxPartial*)take(sizeof(BoxPartial) * std::max<uint64_t>(p.nblocks, 1));
+    if (p.dim == 2) {
+        h.cell_prev = (int2*)take(8 * n);
+        h.entry = (uint32_t*)take(4 * n);
+    }
     h.n = (uint32_t)p.n;
     h.nf = (uint32_t)p.nf;
     h.list_stride = (uint32_t)p.stride;
@@ -327,10 +352,17 @@ void prepare_grid(fsa_scene_s* s) {
     if (s->n == 0) {
         for (int a = 0; a < 3; a++) lo[a] = hi[a] = 0;
     }
-    const uint32_t L = levels_for(lo, hi);
-    if (L > 9)
-        throw FsaError(FSA_E_INVALID,
-                       "scene extent exceeds the 32-bit cell key range (cell coordinates must lie in [-512, 511])");
+    uint32_t L = 0;
+    if (h.dim == 3) {
+        L = levels_for(lo, hi);
+        if (L > 9)
+            throw FsaError(FSA_E_INVALID,
+                           "scene extent exceeds the 32-bit cell key range (cell coordinates must lie in [-512, 511])");
+    } else {
+        lo[2] = hi[2] = 0;
+        if ((uint64_t)(hi[0] - (int64_t)lo[0] + 1) * (uint64_t)(hi[1] - (int64_t)lo[1] + 1) > (1ull << 28))
+            throw FsaError(FSA_E_INVALID, "2D scene extent exceeds 2^28 grid cells");
+    }
     uint64_t cells = 1;
     for (int a = 0; a < 3; a++) {
         h.cmin[a] = lo[a];
@@ -430,25 +462,40 @@ void search_set(fsa_context ctx, const LaunchSet& L, uint32_t passes) {
 void simulate_set(fsa_context ctx, const LaunchSet& L, std::vector<fsa_scene_s*>& scenes, bool sesph,
                   bool boundary_pass) {
     hipStream_t st = ctx->stream;
+    const bool d2 = !scenes.empty() && scenes[0]->hd.dim == 2;
     if (sesph) {
-        {
+        if (d2) {
             PhaseTimer t(ctx, FSA_TIMER_DENSITY);
-            launch_sesph_density(L, st);
-        }
-        {
-            PhaseTimer t(ctx, FSA_TIMER_DIAGONAL);
-            launch_sesph_accel(L, st);
+            launch2_sesph(L, st);
+        } else {
+            {
+                PhaseTimer t(ctx, FSA_TIMER_DENSITY);
+                launch_sesph_density(L, st);
+            }
+            {
+                PhaseTimer t(ctx, FSA_TIMER_DIAGONAL);
+                launch_sesph_accel(L, st);
+            }
         }
     } else {
         {
             PhaseTimer t(ctx, FSA_TIMER_DENSITY);
-            launch_iisph_prepare(L, boundary_pass, st);
+            if (d2)
+                launch2_iisph_prepare(L, st);
+            else
+                launch_iisph_prepare(L, boundary_pass, st);
         }
         {
             PhaseTimer t(ctx, FSA_TIMER_DIAGONAL);
-            launch_iisph_diagonal(L, st);
+            if (d2)
+                launch2_iisph_diagonal(L, st);
+            else
+                launch_iisph_diagonal(L, st);
         }
-        launch_iisph_begin_loop(L, st);
+        if (d2)
+            launch2_iisph_begin_loop(L, st);
+        else
+            launch_iisph_begin_loop(L, st);
         uint32_t enq = 0, max_it = 0, guess = 0;
         for (auto* s : scenes) {
             max_it = std::max(max_it, s->hd.max_it);
@@ -459,11 +506,17 @@ void simulate_set(fsa_context ctx, const LaunchSet& L, std::vector<fsa_scene_s*>
             for (uint32_t k = 0; k < chunk && enq < max_it; k++, enq++) {
                 {
                     PhaseTimer t(ctx, FSA_TIMER_PRESSURE_A);
-                    launch_iisph_pass_a(L, st);
+                    if (d2)
+                        launch2_iisph_pass_a(L, st);
+                    else
+                        launch_iisph_pass_a(L, st);
                 }
                 {
                     PhaseTimer t(ctx, FSA_TIMER_PRESSURE_B);
-                    launch_iisph_pass_b(L, st);
+                    if (d2)
+                        launch2_iisph_pass_b(L, st);
+                    else
+                        launch_iisph_pass_b(L, st);
                 }
             }
             if (enq >= max_it) break;
@@ -477,7 +530,10 @@ void simulate_set(fsa_context ctx, const LaunchSet& L, std::vector<fsa_scene_s*>
     }
     {
         PhaseTimer t(ctx, FSA_TIMER_INTEGRATE);
-        launch_integrate(L, st);
+        if (d2)
+            launch2_integrate(L, st);
+        else
+            launch_integrate(L, st);
     }
     HIP_CHECK(hipStreamSynchronize(st));
 }
@@ -507,10 +563,38 @@ void upload_scene_table(fsa_scene_s* s) {
     HIP_CHECK(hipMemcpyAsync(s->d_table, &s->hd, sizeof(SceneDev), hipMemcpyHostToDevice, s->ctx->stream));
 }
 
+uint32_t bits_of(uint64_t v) {
+    uint32_t b = 0;
+    while (v) {
+        b++;
+        v >>= 1;
+    }
+    return b;
+}
+
+// 2D: radix passes of the entry-count sort and of the cell sort
+uint32_t entry_passes(const SceneDev& h) {
+    if (h.flags & kFlagQuadratic) return 0;
+    return (bits_of(h.search_count) + kRadixBits - 1) / kRadixBits;
+}
+uint32_t cell_passes(const SceneDev& h) {
+    return std::max<uint32_t>(1, (bits_of(h.cdim[3]) + kRadixBits - 1) / kRadixBits);
+}
+
+void search2d_set(fsa_context ctx, const LaunchSet& L, uint32_t epasses, uint32_t cpasses) {
+    PhaseTimer t(ctx, FSA_TIMER_SEARCH);
+    launch2_search(L, epasses, cpasses, ctx->stream);
+}
+
 void do_find_neighbors(fsa_scene_s* s) {
     prepare_grid(s);
     upload_scene_table(s);
-    search_set(s->ctx, s->single, sort_passes(s->hd));
+    if (s->hd.dim == 2) {
+        search2d_set(s->ctx, s->single, entry_passes(s->hd), cell_passes(s->hd));
+        s->hd.search_count++;
+    } else {
+        search_set(s->ctx, s->single, sort_passes(s->hd));
+    }
     read_state(s);
     check_status(s);
     s->searched = true;
@@ -621,13 +705,24 @@ void upload_batch_table(fsa_context ctx, BatchSet* b) {
 // error, the other scenes complete the step, and the first error is returned.
 int32_t batch_step_group(fsa_context ctx, const std::vector<fsa_scene_s*>& v, std::vector<fsa_step_stats*>& out) {
     BatchSet* b = batch_for(ctx, v);
-    uint32_t passes = 1;
+    const bool d2 = v[0]->hd.dim == 2;
+    uint32_t passes = 1, epasses = 0, cpasses = 1;
     for (auto* s : v) {
         prepare_grid(s);
-        passes = std::max(passes, sort_passes(s->hd));
+        if (d2) {
+            epasses = std::max(epasses, entry_passes(s->hd));
+            cpasses = std::max(cpasses, cell_passes(s->hd));
+        } else {
+            passes = std::max(passes, sort_passes(s->hd));
+        }
     }
     upload_batch_table(ctx, b);
-    search_set(ctx, b->ls, passes);
+    if (d2) {
+        search2d_set(ctx, b->ls, epasses, cpasses);
+        for (auto* s : v) s->hd.search_count++;
+    } else {
+        search_set(ctx, b->ls, passes);
+    }
     read_states(ctx, v);
     bool bpass = false;
     int32_t first_error = FSA_OK;
@@ -668,8 +763,111 @@ int32_t batch_step_group(fsa_context ctx, const std::vector<fsa_scene_s*>& v, st
 }
 
 // ---- upload / download ------------------------------------------------------------------
+// 2D: every particle on the device in the caller's order, z = 0
+// (MovementData {vec2 position, velocity, acceleration}, ExternalForces {vec2})
+void do_upload2d(fsa_scene_s* s, const fsa_particle_view* v, bool first) {
+    const uint64_t N = v->count;
+    if (N >= (1ull << kTypeShift)) throw FsaError(FSA_E_INVALID, "too many particles (max 2^30)");
+    std::vector<uint32_t> fidx;
+    uint32_t nb = 0;
+    for (uint64_t i = 0; i < N; i++) {
+        const uint8_t t = ((const uint8_t*)v->info + i * v->info_stride)[4];
+        if (t > 2) throw FsaError(FSA_E_INVALID, "unknown particle type");
+        if (t == FSA_TYPE_NORMAL) fidx.push_back((uint32_t)i);
+        if (t == FSA_TYPE_BOUNDARY) nb++;
+    }
+    const uint32_t nf = (uint32_t)fidx.size();
+    if (!first && (N != s->total || nf != s->nf || nb != s->nb))
+        throw FsaError(FSA_E_INVALID, "upload must keep the particle count and types of the scene");
+    s->total = N;
+    s->n = (uint32_t)N;
+    s->nf = nf;
+    s->nb = nb;
+    const uint32_t n = s->n;
+    std::vector<float4> pos(n), vel(n), acc(n), ext(n);
+    std::vector<float> pres(n), zero(n, 0.0f);
+    std::vector<uint32_t> idt(n), rank(n, 0);
+    int32_t lo[2] = {INT32_MAX, INT32_MAX}, hi[2] = {INT32_MIN, INT32_MIN};
+    const float r = s->desc.particle_size * 2.0f;
+    bool any_ext = false;
+    for (uint32_t i = 0; i < n; i++) {
+        const float* m = (const float*)((const char*)v->movement + (uint64_t)i * v->movement_stride);
+        const float* pd = (const float*)((const char*)v->particle_data + (uint64_t)i * v->particle_data_stride);
+        const uint8_t t = ((const uint8_t*)v->info + (uint64_t)i * v->info_stride)[4];
+        float e[2] = {0.0f, 0.0f};
+        if (v->external) {
+            const float* ep = (const float*)((const char*)v->external + (uint64_t)i * v->external_stride);
+            e[0] = ep[0];
+            e[1] = ep[1];
+        }
+        pos[i] = make_float4(m[0], m[1], 0.0f, pd[0]);
+        vel[i] = make_float4(m[2], m[3], 0.0f, pd[2]);
+        acc[i] = make_float4(m[4], m[5], 0.0f, 0.0f);
+        ext[i] = make_float4(e[0], e[1], 0.0f, 0.0f);
+        pres[i] = pd[1];
+        idt[i] = i | ((uint32_t)t << kTypeShift);
+        if (e[0] != 0.0f || e[1] != 0.0f) any_ext = true;
+        // GetGridCellByPosition: truncation toward zero
+        const int32_t c[2] = {(int32_t)(m[0] / r), (int32_t)(m[1] / r)};
+        for (int a = 0; a < 2; a++) {
+            lo[a] = std::min(lo[a], c[a]);
+            hi[a] = std::max(hi[a], c[a]);
+        }
+    }
+    for (uint32_t k = 0; k < nf; k++) rank[fidx[k]] = k;
+    s->inactive_rows.clear();
+    SceneDev& h = s->hd;
+    h.flags &= ~(kFlagExternal | kFlagDynamicBoundary);
+    if (any_ext) h.flags |= kFlagExternal;
+    h.grid_count = (h.flags & kFlagQuadratic) ? n : (uint32_t)(uint16_t)n;  // particleAmount_t (uint16) count
+    hipStream_t st = s->ctx->stream;
+    SceneState init{};
+    init.dt_actual = s->desc.timestep;
+    for (int a = 0; a < 2; a++) {
+        init.bbox_min[a] = n ? lo[a] : 0;
+        init.bbox_max[a] = n ? hi[a] : 0;
+    }
+    if (n) {
+        HIP_CHECK(hipMemcpyAsync(h.pos[0], pos.data(), 16ull * n, hipMemcpyHostToDevice, st));
+        HIP_CHECK(hipMemcpyAsync(h.vel[0], vel.data(), 16ull * n, hipMemcpyHostToDevice, st));
+        HIP_CHECK(hipMemcpyAsync(h.acc[0], acc.data(), 16ull * n, hipMemcpyHostToDevice, st));
+        HIP_CHECK(hipMemcpyAsync(h.ext[0], ext.data(), 16ull * n, hipMemcpyHostToDevice, st));
+        HIP_CHECK(hipMemcpyAsync(h.pres[0], pres.data(), 4ull * n, hipMemcpyHostToDevice, st));
+        HIP_CHECK(hipMemcpyAsync(h.idt[0], idt.data(), 4ull * n, hipMemcpyHostToDevice, st));
+        HIP_CHECK(hipMemcpyAsync(h.rank_of, rank.data(), 4ull * n, hipMemcpyHostToDevice, st));
+        // rho_new holds the densities of the step; start from the uploaded ones
+        std::vector<float> dens(n);
+        for (uint32_t i = 0; i < n; i++) dens[i] = vel[i].w;
+        HIP_CHECK(hipMemcpyAsync(h.rho_new, dens.data(), 4ull * n, hipMemcpyHostToDevice, st));
+        HIP_CHECK(hipMemcpyAsync(h.src, zero.data(), 4ull * n, hipMemcpyHostToDevice, st));
+        HIP_CHECK(hipMemcpyAsync(h.diag, zero.data(), 4ull * n, hipMemcpyHostToDevice, st));
+        if (first) {
+            // the grid starts empty: every particle's previous cell is the reference's
+            // {INT32_MAX, INT32_MAX} sentinel (HashedNeighborhoodSearch.cpp:122-126)
+            std::vector<int2> prev(n, make_int2(INT32_MAX, INT32_MAX));
+            std::vector<uint32_t> ent(n, 0u);
+            HIP_CHECK(hipMemcpyAsync(h.cell_prev, prev.data(), 8ull * n, hipMemcpyHostToDevice, st));
+            HIP_CHECK(hipMemcpyAsync(h.entry, ent.data(), 4ull * n, hipMemcpyHostToDevice, st));
+            h.search_count = 0;
+            HIP_CHECK(hipStreamSynchronize(st));
+        }
+    }
+    if (nf) HIP_CHECK(hipMemcpyAsync(h.fluid_idx, fidx.data(), 4ull * nf, hipMemcpyHostToDevice, st));
+    HIP_CHECK(hipMemcpyAsync(h.state, &init, sizeof(SceneState), hipMemcpyHostToDevice, st));
+    HIP_CHECK(hipStreamSynchronize(st));
+    *s->h_state = init;
+    s->searched = false;
+    s->boundary_dirty = true;
+    s->lists_cached = false;
+    s->lists_valid = false;
+}
+
 void do_upload(fsa_scene_s* s, const fsa_particle_view* v, bool first) {
     if (!v || !v->movement || !v->particle_data || !v->info) throw FsaError(FSA_E_INVALID, "incomplete particle view");
+    if (s->desc.dimension == 2) {
+        do_upload2d(s, v, first);
+        return;
+    }
     const uint64_t N = v->count;
     if (N >= (1ull << kTypeShift)) throw FsaError(FSA_E_INVALID, "too many particles (max 2^30)");
     {
@@ -819,6 +1017,32 @@ void do_download(fsa_scene_s* s, const fsa_particle_view* v) {
     if (!v || v->count != s->total) throw FsaError(FSA_E_INVALID, "download view count mismatch");
     HostCopy hc;
     fetch(s, hc, false);
+    if (s->desc.dimension == 2) {
+        for (uint32_t i = 0; i < s->n; i++) {
+            if (v->movement) {
+                float* m = (float*)((char*)v->movement + (uint64_t)i * v->movement_stride);
+                m[0] = hc.pos[i].x;
+                m[1] = hc.pos[i].y;
+                m[2] = hc.vel[i].x;
+                m[3] = hc.vel[i].y;
+                m[4] = hc.acc[i].x;
+                m[5] = hc.acc[i].y;
+            }
+            if (v->particle_data) {
+                float* pd = (float*)((char*)v->particle_data + (uint64_t)i * v->particle_data_stride);
+                pd[0] = hc.pos[i].w;
+                pd[1] = hc.pres[i];
+                pd[2] = hc.vel[i].w;
+            }
+            if (v->info) ((uint8_t*)v->info + (uint64_t)i * v->info_stride)[4] = (uint8_t)(hc.idt[i] >> kTypeShift);
+            if (v->external) {
+                float* e = (float*)((char*)v->external + (uint64_t)i * v->external_stride);
+                e[0] = hc.ext[i].x;
+                e[1] = hc.ext[i].y;
+            }
+        }
+        return;
+    }
     const bool ext_live = (s->hd.flags & kFlagExternal) != 0;
     for (uint32_t k = 0; k < s->n; k++) {
         const uint64_t i = hc.idt[k] & kIdMask;
@@ -874,6 +1098,17 @@ void do_download_iisph(fsa_scene_s* s, void* rec, uint64_t stride) {
         HIP_CHECK(hipMemcpyAsync(src.data(), s->hd.src, 4ull * n, hipMemcpyDeviceToHost, st));
         HIP_CHECK(hipMemcpyAsync(dg.data(), s->hd.diag, 4ull * n, hipMemcpyDeviceToHost, st));
         HIP_CHECK(hipStreamSynchronize(st));
+    }
+    if (s->desc.dimension == 2) {
+        // IISPHParticleData (IISPHFluidSolver.hpp:14-18): predicted velocity, source term, diagonal
+        for (uint32_t i = 0; i < n; i++) {
+            float* r = (float*)((char*)rec + (uint64_t)i * stride);
+            r[0] = vs[i].x;
+            r[1] = vs[i].y;
+            r[2] = src[i];
+            r[3] = dg[i];
+        }
+        return;
     }
     for (uint32_t k = 0; k < n; k++) {
         const uint64_t i = hc.idt[k] & kIdMask;
@@ -1116,6 +1351,7 @@ FSA_API void fsa_scene_desc_defaults(fsa_scene_desc* d, uint32_t dimension, uint
     d->timestep = 0.001f;
     d->lambda_v = 0.4f;
     d->lambda_a = 0.25f;
+    d->search = dimension == 2 ? FSA_SEARCH_HASHED_2D : FSA_SEARCH_COMPRESSED_3D;
 }
 
 FSA_API int32_t fsa_context_create(int32_t device, uint64_t budget, fsa_context* out) {
@@ -1185,7 +1421,7 @@ FSA_API int32_t fsa_scene_memory_estimate(const fsa_scene_desc* desc, uint64_t n
     return guarded([&] {
         validate_desc(desc);
         if (!bytes) throw FsaError(FSA_E_INVALID, "bytes is NULL");
-        *bytes = plan_bytes(make_plan(n_fluid + n_boundary, n_fluid)) + sizeof(SceneDev);
+        *bytes = plan_bytes(make_plan(n_fluid + n_boundary, n_fluid, desc->dimension)) + sizeof(SceneDev);
     });
 }
 
@@ -1197,14 +1433,16 @@ FSA_API int32_t fsa_scene_create(fsa_context c, const fsa_scene_desc* desc, cons
         validate_desc(desc);
         if (!view) throw FsaError(FSA_E_INVALID, "view is NULL");
         HIP_CHECK(hipSetDevice(c->device));
-        // count active particles first to size the allocation
+        // count the particles the device holds first to size the allocation
+        // (3D: active ones; 2D: all, the 2D searches see inactive particles)
         uint64_t nf = 0, nb = 0;
         for (uint64_t i = 0; i < view->count; i++) {
             const uint8_t t = ((const uint8_t*)view->info + i * view->info_stride)[4];
             if (t == FSA_TYPE_NORMAL) nf++;
             if (t == FSA_TYPE_BOUNDARY) nb++;
         }
-        const Plan p = make_plan(nf + nb, nf);
+        const uint64_t ndev = desc->dimension == 2 ? view->count : nf + nb;
+        const Plan p = make_plan(ndev, nf, desc->dimension);
         if (p.n >= (1ull << 27)) throw FsaError(FSA_E_INVALID, "scene too large for 27-bit run starts");
         const uint64_t bytes = plan_bytes(p);
         if (c->budget && c->in_use.load() + bytes > c->budget)
@@ -1381,10 +1619,11 @@ FSA_API int32_t fsa_batch_step(fsa_context c, const fsa_scene* scenes, uint32_t 
         std::vector<fsa_scene_s*> v(scenes, scenes + n);
         for (auto* s : v)
             if (!s || s->ctx != c) throw FsaError(FSA_E_INVALID, "scene of another context");
-        std::vector<fsa_scene_s*> groups[2];
-        std::vector<fsa_step_stats*> gstats[2];
+        // scenes of one solver and dimension share launches
+        std::vector<fsa_scene_s*> groups[4];
+        std::vector<fsa_step_stats*> gstats[4];
         for (uint32_t q = 0; q < n; q++) {
-            const int g = v[q]->desc.solver == FSA_SOLVER_SESPH ? 1 : 0;
+            const int g = (v[q]->desc.solver == FSA_SOLVER_SESPH ? 1 : 0) + (v[q]->desc.dimension == 2 ? 2 : 0);
             if (std::find(groups[g].begin(), groups[g].end(), v[q]) != groups[g].end())
                 throw FsaError(FSA_E_INVALID, "a scene appears twice in the batch");
             groups[g].push_back(v[q]);
@@ -1393,7 +1632,7 @@ FSA_API int32_t fsa_batch_step(fsa_context c, const fsa_scene* scenes, uint32_t 
         HIP_CHECK(hipSetDevice(c->device));
         int32_t first_error = FSA_OK;
         for (uint32_t k = 0; k < steps && first_error == FSA_OK; k++)
-            for (int g = 0; g < 2; g++)
+            for (int g = 0; g < 4; g++)
                 if (!groups[g].empty()) {
                     const int32_t rc = batch_step_group(c, groups[g], gstats[g]);
                     if (first_error == FSA_OK) first_error = rc;

@@ -18,7 +18,8 @@ namespace fsa {
 constexpr int kBlock = 256;            // threads per block of the particle kernels
 constexpr int kMaxNeighbors = 97;      // NeighborStorage capacity (CompressedNeighbors.hpp:29-34)
 constexpr uint32_t kBoundaryBit = 0x80000000u;  // neighbor entry flag: neighbor is a boundary particle
-constexpr uint32_t kIndexMask = 0x7fffffffu;
+constexpr uint32_t kInactiveBit = 0x40000000u;  // neighbor entry flag (2D searches): neighbor is inactive
+constexpr uint32_t kIndexMask = 0x3fffffffu;    // neighbor entry: particle index
 constexpr uint32_t kTypeShift = 30;    // idt word: caller index (30 bits) | type << 30
 constexpr uint32_t kIdMask = (1u << kTypeShift) - 1u;
 constexpr uint32_t kEmptyCell = 0xffffffffu;
@@ -38,6 +39,14 @@ constexpr uint32_t kFlagDynamicBoundary = 8u;  // boundary particles move (non-z
 constexpr uint32_t kFlagSingleLayer = 16u;  // IISPH/SESPH single-layer boundary scaling
 constexpr uint32_t kFlagSesph = 32u;        // SESPH solver (else IISPH)
 constexpr uint32_t kFlagPlainLists = 64u;   // uncompressed 32-bit neighbor lists
+constexpr uint32_t kFlagIntAbs = 128u;      // 2D IISPH: integer abs() in the loop condition
+constexpr uint32_t kFlagQuadratic = 256u;   // 2D: QuadraticNeighborhoodSearchDynamicAllocated order
+
+// 2D scenes (IISPHFluidSolver / SESPHFluidSolver with HashedNeighborhoodSearch or
+// QuadraticNeighborhoodSearchDynamicAllocated) keep every particle, inactive
+// ones included, in the caller's order (the 2D searches never sort the
+// collection); positions carry z = 0, so the 3-component arithmetic below
+// gives the 2D values: glm's vec2 dot (x*x' + y*y') plus an exact + 0.
 
 // SceneState.status bits
 constexpr uint32_t kStatusCapacity = 1u;    // a particle had more than 97 neighbors
@@ -93,7 +102,15 @@ struct alignas(16) SceneDev {
     float dt;              // desired time step of the next step
     float gen_dt, lambda_a, lambda_v;
     float inactive_a2, inactive_v2;  // SESPH CFL contribution of inactive particles
-    uint32_t pad0;
+    uint32_t dim;          // 2 or 3
+    // 2D hashed grid (HashedNeighborhoodSearch.cpp:117-151): members are the
+    // first grid_count particles; a particle that changes cell is appended to
+    // its new cell, so a cell lists its members by (entry, index)
+    uint32_t search_count; // searches done before this one
+    uint32_t grid_count;   // particles the grid update visits (uint16 count)
+    uint32_t pad1;
+    int2* cell_prev;       // 2D: cell of the last grid update
+    uint32_t* entry;       // 2D: search count at which the particle entered its cell
 
     // cell grid of this step (dense, linear x-fastest over the cell bounding box)
     int32_t cmin[4];

@@ -26,6 +26,8 @@ void launch_keys(const LaunchSet& L, hipStream_t st);
 void launch_sort(const LaunchSet& L, uint32_t passes, hipStream_t st);
 void launch_reorder(const LaunchSet& L, uint32_t sbuf, hipStream_t st);
 void launch_lists(const LaunchSet& L, uint32_t sbuf, hipStream_t st);
+// stable LSD radix passes over (keys, vals)[first_src], digits from bit shift0 upward
+void launch_sort_range(const LaunchSet& L, uint32_t first_src, uint32_t shift0, uint32_t passes, hipStream_t st);
 
 // IISPH / SESPH
 void launch_iisph_prepare(const LaunchSet& L, bool boundary_pass, hipStream_t st);
@@ -36,5 +38,15 @@ void launch_iisph_pass_b(const LaunchSet& L, hipStream_t st);
 void launch_sesph_density(const LaunchSet& L, hipStream_t st);
 void launch_sesph_accel(const LaunchSet& L, hipStream_t st);
 void launch_integrate(const LaunchSet& L, hipStream_t st);
+
+// 2D (hashed / quadratic-dynamic-allocated searches, IISPH and SESPH)
+void launch2_search(const LaunchSet& L, uint32_t entry_passes, uint32_t cell_passes, hipStream_t st);
+void launch2_iisph_prepare(const LaunchSet& L, hipStream_t st);
+void launch2_iisph_diagonal(const LaunchSet& L, hipStream_t st);
+void launch2_iisph_begin_loop(const LaunchSet& L, hipStream_t st);
+void launch2_iisph_pass_a(const LaunchSet& L, hipStream_t st);
+void launch2_iisph_pass_b(const LaunchSet& L, hipStream_t st);
+void launch2_sesph(const LaunchSet& L, hipStream_t st);
+void launch2_integrate(const LaunchSet& L, hipStream_t st);
 
 }  // namespace fsa

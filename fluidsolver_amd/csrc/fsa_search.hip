@@ -300,7 +300,6 @@ __global__ __launch_bounds__(kBlock) void k_cell_table(const SceneDev* __restric
 // as well) are broadcast in turn and tested against the whole chunk with one
 // ballot, whose prefix gives each accepted candidate its slot, so every list is
 // written in ascending order without staging.
-constexpr int kWaves = kBlock / 64;
 
 __device__ __forceinline__ uint32_t wave_sort_u32(uint32_t v) {
     // bitonic sort across the 64 lanes, ascending by lane
@@ -464,6 +463,16 @@ void launch_sort(const LaunchSet& L, uint32_t passes, hipStream_t st) {
     for (uint32_t p = 0; p < passes; p++) {
         const uint32_t src = p & 1u;
         const uint32_t shift = p * kRadixBits;
+        k_radix_hist<<<L.nb_tiles, kSortThreads, 0, st>>>(L.scenes, L.map_tiles, shift, src);
+        k_radix_scan<<<L.nb_scan, kBlock, 0, st>>>(L.scenes, L.map_scan);
+        k_radix_scatter<<<L.nb_tiles, kSortThreads, 0, st>>>(L.scenes, L.map_tiles, shift, src);
+    }
+}
+
+void launch_sort_range(const LaunchSet& L, uint32_t first_src, uint32_t shift0, uint32_t passes, hipStream_t st) {
+    for (uint32_t p = 0; p < passes; p++) {
+        const uint32_t src = first_src ^ (p & 1u);
+        const uint32_t shift = shift0 + p * kRadixBits;
         k_radix_hist<<<L.nb_tiles, kSortThreads, 0, st>>>(L.scenes, L.map_tiles, shift, src);
         k_radix_scan<<<L.nb_scan, kBlock, 0, st>>>(L.scenes, L.map_scan);
         k_radix_scatter<<<L.nb_tiles, kSortThreads, 0, st>>>(L.scenes, L.map_tiles, shift, src);

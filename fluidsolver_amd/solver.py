@@ -133,7 +133,7 @@ class Scene:
         return c
 
     def download_iisph(self):
-        rec = np.zeros((self.n, 9), dtype=np.float32)
+        rec = np.zeros((self.n, abi.iisph_record_floats(self.desc.dimension)), dtype=np.float32)
         check(lib().fsa_scene_download_iisph(self._h, rec.ctypes.data, rec.strides[0]))
         return rec
 

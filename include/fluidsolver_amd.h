@@ -64,6 +64,13 @@ extern "C" {
 #define FSA_LIST_COMPRESSED 0 /* delta-coded neighbor lists (default)            */
 #define FSA_LIST_PLAIN 1      /* uncompressed 32-bit neighbor lists               */
 
+/* fsa_scene_desc.search: the reference search whose neighbor order the device
+ * reproduces (registration strings of SolverSerializer.cpp:137-219). */
+#define FSA_SEARCH_DEFAULT 0       /* compressed-3d in 3D, hashed in 2D               */
+#define FSA_SEARCH_COMPRESSED_3D 1 /* "compressed-3d": CompressedNeighborhoodSearch  */
+#define FSA_SEARCH_HASHED_2D 4     /* "hashed": HashedNeighborhoodSearch (2D)         */
+#define FSA_SEARCH_QUADRATIC_2D 5  /* "quadratic-dynamic-allocated" (2D)             */
+
 #define FSA_TYPE_NORMAL 0    /* ParticleTypeNormal   (Particle.hpp:9)            */
 #define FSA_TYPE_BOUNDARY 1  /* ParticleTypeBoundary (Particle.hpp:10)           */
 #define FSA_TYPE_INACTIVE 2  /* ParticleTypeInactive (Particle.hpp:11)           */
@@ -113,7 +120,8 @@ typedef struct fsa_scene_desc {
     float lambda_v;
     float lambda_a;
 
-    uint32_t reserved[5];
+    uint32_t search;         /* FSA_SEARCH_* */
+    uint32_t reserved[4];
 } fsa_scene_desc;
 
 /* Strided views over the caller's component arrays. Row i of every view is
