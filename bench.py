@@ -200,6 +200,7 @@ def main():
     ap.add_argument("--serve-slots", type=int, default=32)
     ap.add_argument("--serve-budget-gb", type=float, default=200.0)
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--graph", type=int, default=1, help="replay step heads from hipGraphs (1) or launch eagerly (0)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus above 1 is not supported"}))
@@ -214,6 +215,7 @@ def main():
     log(f"scene: {nf} fluid, {nb} boundary")
 
     ctx = Context(0)
+    ctx.set_graph(bool(args.graph))
     sc = Scene(ctx, desc, coll)
     t0 = time.perf_counter()
     for _ in range(args.warmup):
@@ -278,6 +280,10 @@ def main():
         served = bench_served(ctx, args.serve_seconds, args.serve_rate, args.serve_slots, args.serve_budget_gb)
         log(json.dumps(served))
 
+    tuning = ctx.tuning()
+    if tuning:
+        log("autotuned neighbor-walk unroll:\n" + tuning.rstrip())
+
     cpu = None
     if not args.no_cpu:
         cside = args.cpu_side
@@ -317,6 +323,8 @@ def main():
         "cpu_baseline": cpu,
         "batched": batched,
         "served": served,
+        "graph": bool(args.graph),
+        "autotune": [ln for ln in tuning.splitlines() if ln],
     }
     print(json.dumps(out), flush=True)
     ctx.close()

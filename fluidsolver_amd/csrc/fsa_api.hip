@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <atomic>
 #include <chrono>
 #include <deque>
@@ -97,6 +98,14 @@ struct fsa_context_s {
     std::map<std::vector<fsa_scene_s*>, struct BatchSet*> batches;
     uint64_t batch_clock = 0;
     std::vector<struct fsa_server_s*> servers;  // destroyed with the context
+    // autotuned neighbor-walk unroll per size class, with the timings behind each choice
+    bool autotune = true;
+    struct Tuned {
+        std::array<uint8_t, 4> unroll;
+        std::array<std::array<float, 3>, 4> us;  // per family, per candidate (2, 4, 8)
+        uint64_t fluid;
+    };
+    std::map<uint64_t, Tuned> tuned;
 };
 
 // Launch set over several scenes (one table entry and block-map range each).
@@ -107,6 +116,10 @@ struct BatchSet {
     LaunchSet ls{};
     std::vector<BlockRef*> owned;
     uint64_t last_use = 0;
+    // instantiated step heads, keyed by what the captured launches depend on:
+    // sort passes (2D: entry and cell passes), Jacobi passes, spec, boundary pass
+    std::map<std::array<uint32_t, 5>, hipGraphExec_t> graphs;
+    bool tuned = false;
 };
 
 struct fsa_scene_s {
@@ -602,9 +615,16 @@ void do_find_neighbors(fsa_scene_s* s) {
     s->lists_cached = false;
 }
 
+void set_step_dt(fsa_scene_s* s, float dt);
+
 void pre_simulate(fsa_scene_s* s, float dt) {
     if (!s->searched) throw FsaError(FSA_E_STATE, "simulate called before find_neighbors");
     if (!(dt > 0.0f)) throw FsaError(FSA_E_INVALID, "desired time step must be > 0");
+    set_step_dt(s, dt);
+}
+
+// time step of the coming solver step, and the SESPH CFL share of inactive particles
+void set_step_dt(fsa_scene_s* s, float dt) {
     s->hd.dt = dt;
     if (s->desc.solver == FSA_SOLVER_SESPH) {
         // SESPH's CFL scan covers inactive particles too (SESPHFluidSolver3D.hpp:96-107)
@@ -673,6 +693,7 @@ BatchSet* batch_for(fsa_context ctx, const std::vector<fsa_scene_s*>& v) {
 }
 
 void free_batch(BatchSet* b) {
+    for (auto& kv : b->graphs) (void)hipGraphExecDestroy(kv.second);
     (void)hipFree(b->d_table);
     for (auto* m : b->owned) (void)hipFree(m);
     delete b;
@@ -700,13 +721,201 @@ void upload_batch_table(fsa_context ctx, BatchSet* b) {
                              hipMemcpyHostToDevice, ctx->stream));
 }
 
-// One step of every scene in v (same solver), in shared launches.  A scene
-// whose search fails (neighbor capacity) is not simulated: its stats carry the
-// error, the other scenes complete the step, and the first error is returned.
+// One step of every scene in v (same solver and dimension) in shared launches.
+// The head of the step (search, solver phases, the predicted number of gated
+// Jacobi iterations) is one launch sequence, replayed from a hipGraph when
+// graphs are enabled; the host then checks the scenes once, runs further
+// iterations for scenes still converging, integrates and reads the states.
+// A scene whose search fails (neighbor capacity) is skipped by the solver
+// kernels: its stats carry the error, the other scenes complete the step, and
+// the first error is returned.
+// ---- autotuning ----------------------------------------------------------------------------
+// The neighbor-walk unroll of the four IISPH kernel families is chosen per size
+// class (kernel spec, total fluid particles rounded to a power of two) by
+// timing every candidate on the live neighbor lists of the first step of that
+// class.  The timed launches read the real particle arrays and lists but write
+// to a shadow scene table whose written arrays and state are scratch copies,
+// so the simulation state is untouched.
+uint64_t tune_class(const BatchSet* b, const std::vector<fsa_scene_s*>& v) {
+    uint64_t nf = 0;
+    for (auto* s : v) nf += s->nf;
+    return (uint64_t)b->ls.spec << 32 | bits_of(nf);
+}
+
+void autotune(fsa_context ctx, BatchSet* b, const std::vector<fsa_scene_s*>& v) {
+    const uint64_t cls = tune_class(b, v);
+    auto hit = ctx->tuned.find(cls);
+    if (hit != ctx->tuned.end()) {
+        for (int f = 0; f < kTuneFamilies; f++) b->ls.unroll[f] = hit->second.unroll[f];
+        return;
+    }
+    hipStream_t st = ctx->stream;
+    // scratch per scene: acc, vstar, anp, ext (16 B) and pres, err, rho_new, src, diag (4 B)
+    // per particle, partials per block, one state
+    std::vector<uint64_t> offs(v.size());
+    uint64_t total = 0;
+    for (size_t q = 0; q < v.size(); q++) {
+        offs[q] = total;
+        const uint64_t n = std::max<uint32_t>(v[q]->n, 1);
+        total += align_up(4 * 16 * n, 256) + align_up(5 * 4 * n, 256) +
+                 align_up(sizeof(Partial) * std::max<uint32_t>(v[q]->hd.nblocks, 1), 256) +
+                 align_up(sizeof(SceneState), 256);
+    }
+    if (ctx->budget && ctx->in_use.load() + total > ctx->budget) return;  // no room: keep the defaults
+    char* scratch = nullptr;
+    if (hipMalloc(&scratch, total) != hipSuccess) return;
+    SceneDev* table = nullptr;
+    if (hipMalloc(&table, sizeof(SceneDev) * v.size()) != hipSuccess) {
+        (void)hipFree(scratch);
+        return;
+    }
+    std::vector<SceneDev> sh(v.size());
+    for (size_t q = 0; q < v.size(); q++) {
+        const SceneDev& h = v[q]->hd;
+        const uint32_t c = v[q]->h_state->cur;
+        const uint64_t n = std::max<uint32_t>(v[q]->n, 1);
+        char* p = scratch + offs[q];
+        auto take = [&](uint64_t sz) {
+            char* r = p;
+            p += align_up(sz, 256);
+            return r;
+        };
+        SceneDev& d = sh[q];
+        d = h;
+        float4* acc = (float4*)take(4 * 16 * n);
+        float4* vstar = acc + n;
+        float4* anp = acc + 2 * n;
+        float4* ext = acc + 3 * n;
+        float* f4 = (float*)take(5 * 4 * n);
+        d.acc[0] = d.acc[1] = acc;
+        d.vstar = vstar;
+        d.anp = anp;
+        d.ext[0] = d.ext[1] = ext;
+        d.pres[0] = d.pres[1] = f4;
+        d.err[0] = d.err[1] = f4 + n;
+        d.rho_new = f4 + 2 * n;
+        d.src = f4 + 3 * n;
+        d.diag = f4 + 4 * n;
+        d.partial = (Partial*)take(sizeof(Partial) * std::max<uint32_t>(h.nblocks, 1));
+        d.state = (SceneState*)take(sizeof(SceneState));
+        // realistic inputs: copies of the step's arrays, a state that keeps the loop active
+        const uint64_t nn = v[q]->n;
+        if (nn) {
+            HIP_CHECK(hipMemcpyAsync(acc, h.acc[c], 16 * nn, hipMemcpyDeviceToDevice, st));
+            HIP_CHECK(hipMemcpyAsync(vstar, h.vstar, 16 * nn, hipMemcpyDeviceToDevice, st));
+            HIP_CHECK(hipMemcpyAsync(anp, h.anp, 16 * nn, hipMemcpyDeviceToDevice, st));
+            HIP_CHECK(hipMemcpyAsync(ext, h.ext[c], 16 * nn, hipMemcpyDeviceToDevice, st));
+            HIP_CHECK(hipMemcpyAsync(d.pres[0], h.pres[c], 4 * nn, hipMemcpyDeviceToDevice, st));
+            HIP_CHECK(hipMemcpyAsync(d.err[0], h.err[c], 4 * nn, hipMemcpyDeviceToDevice, st));
+            HIP_CHECK(hipMemcpyAsync(d.rho_new, h.rho_new, 4 * nn, hipMemcpyDeviceToDevice, st));
+            HIP_CHECK(hipMemcpyAsync(d.src, h.src, 4 * nn, hipMemcpyDeviceToDevice, st));
+            HIP_CHECK(hipMemcpyAsync(d.diag, h.diag, 4 * nn, hipMemcpyDeviceToDevice, st));
+        }
+        SceneState ss = *v[q]->h_state;
+        ss.active = 1;
+        ss.status = 0;
+        HIP_CHECK(hipMemcpyAsync(d.state, &ss, sizeof(SceneState), hipMemcpyHostToDevice, st));
+        HIP_CHECK(hipStreamSynchronize(st));  // ss is a stack copy
+    }
+    HIP_CHECK(hipMemcpyAsync(table, sh.data(), sizeof(SceneDev) * v.size(), hipMemcpyHostToDevice, st));
+    LaunchSet L = b->ls;
+    L.scenes = table;
+    hipEvent_t e0, e1;
+    HIP_CHECK(hipEventCreate(&e0));
+    HIP_CHECK(hipEventCreate(&e1));
+    fsa_context_s::Tuned t{};
+    t.fluid = 0;
+    for (auto* s : v) t.fluid += s->nf;
+    const uint8_t cand[3] = {2, 4, 8};
+    for (int f = 0; f < kTuneFamilies; f++) {
+        float best = 0.0f;
+        uint8_t pick = 4;
+        for (int c = 0; c < 3; c++) {
+            L.unroll[f] = cand[c];
+            launch_tuned_kernel(L, f, st);  // warm-up
+            HIP_CHECK(hipEventRecord(e0, st));
+            for (int r = 0; r < 3; r++) launch_tuned_kernel(L, f, st);
+            HIP_CHECK(hipEventRecord(e1, st));
+            HIP_CHECK(hipEventSynchronize(e1));
+            float ms = 0.0f;
+            HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+            t.us[f][c] = 1000.0f * ms / 3.0f;
+            if (c == 0 || t.us[f][c] < best) {
+                best = t.us[f][c];
+                pick = cand[c];
+            }
+        }
+        L.unroll[f] = pick;
+        t.unroll[f] = pick;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipFree(table);
+    (void)hipFree(scratch);
+    for (int f = 0; f < kTuneFamilies; f++) b->ls.unroll[f] = t.unroll[f];
+    ctx->tuned[cls] = t;
+}
+
+void step_head(fsa_context ctx, const LaunchSet& L, bool d2, uint32_t passes, uint32_t ep, uint32_t cp, bool sesph,
+               bool bpass, uint32_t chunk) {
+    hipStream_t st = ctx->stream;
+    if (d2)
+        search2d_set(ctx, L, ep, cp);
+    else
+        search_set(ctx, L, passes);
+    if (sesph) {
+        PhaseTimer t(ctx, FSA_TIMER_DENSITY);
+        if (d2) {
+            launch2_sesph(L, st);
+        } else {
+            launch_sesph_density(L, st);
+            launch_sesph_accel(L, st);
+        }
+        return;
+    }
+    {
+        PhaseTimer t(ctx, FSA_TIMER_DENSITY);
+        if (d2)
+            launch2_iisph_prepare(L, st);
+        else
+            launch_iisph_prepare(L, bpass, st);
+    }
+    {
+        PhaseTimer t(ctx, FSA_TIMER_DIAGONAL);
+        if (d2)
+            launch2_iisph_diagonal(L, st);
+        else
+            launch_iisph_diagonal(L, st);
+    }
+    if (d2)
+        launch2_iisph_begin_loop(L, st);
+    else
+        launch_iisph_begin_loop(L, st);
+    for (uint32_t k = 0; k < chunk; k++) {
+        {
+            PhaseTimer t(ctx, FSA_TIMER_PRESSURE_A);
+            if (d2)
+                launch2_iisph_pass_a(L, st);
+            else
+                launch_iisph_pass_a(L, st);
+        }
+        {
+            PhaseTimer t(ctx, FSA_TIMER_PRESSURE_B);
+            if (d2)
+                launch2_iisph_pass_b(L, st);
+            else
+                launch_iisph_pass_b(L, st);
+        }
+    }
+}
+
 int32_t batch_step_group(fsa_context ctx, const std::vector<fsa_scene_s*>& v, std::vector<fsa_step_stats*>& out) {
     BatchSet* b = batch_for(ctx, v);
+    hipStream_t st = ctx->stream;
     const bool d2 = v[0]->hd.dim == 2;
-    uint32_t passes = 1, epasses = 0, cpasses = 1;
+    const bool sesph = v[0]->desc.solver == FSA_SOLVER_SESPH;
+    uint32_t passes = 1, epasses = 0, cpasses = 1, max_it = 0, guess = 0;
+    bool bpass = false;
     for (auto* s : v) {
         prepare_grid(s);
         if (d2) {
@@ -715,19 +924,77 @@ int32_t batch_step_group(fsa_context ctx, const std::vector<fsa_scene_s*>& v, st
         } else {
             passes = std::max(passes, sort_passes(s->hd));
         }
+        set_step_dt(s, s->desc.timestep);
+        bpass = bpass || (s->hd.flags & kFlagDynamicBoundary) || s->boundary_dirty;
+        max_it = std::max(max_it, s->hd.max_it);
+        guess = std::max(guess, std::max(s->hd.min_it, s->last_iterations));
     }
+    const uint32_t chunk = sesph ? 0u : std::min(max_it, std::max<uint32_t>(guess, 1));
     upload_batch_table(ctx, b);
-    if (d2) {
-        search2d_set(ctx, b->ls, epasses, cpasses);
-        for (auto* s : v) s->hd.search_count++;
+    if (ctx->graph && !ctx->timing) {
+        const std::array<uint32_t, 5> key = {
+            d2 ? (epasses << 8 | cpasses) : passes, chunk, b->ls.spec, (bpass ? 1u : 0u) | (sesph ? 2u : 0u),
+            (uint32_t)b->ls.unroll[0] | (uint32_t)b->ls.unroll[1] << 8 | (uint32_t)b->ls.unroll[2] << 16 |
+                (uint32_t)b->ls.unroll[3] << 24};
+        auto it = b->graphs.find(key);
+        if (it == b->graphs.end()) {
+            hipGraph_t g = nullptr;
+            hipGraphExec_t exec = nullptr;
+            HIP_CHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+            step_head(ctx, b->ls, d2, passes, epasses, cpasses, sesph, bpass, chunk);
+            HIP_CHECK(hipStreamEndCapture(st, &g));
+            const hipError_t e = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            if (e != hipSuccess) throw FsaError(FSA_E_DEVICE, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+            it = b->graphs.emplace(key, exec).first;
+        }
+        HIP_CHECK(hipGraphLaunch(it->second, st));
     } else {
-        search_set(ctx, b->ls, passes);
+        step_head(ctx, b->ls, d2, passes, epasses, cpasses, sesph, bpass, chunk);
+    }
+    if (d2)
+        for (auto* s : v) s->hd.search_count++;
+    read_states(ctx, v);
+    if (!b->tuned) {
+        if (!d2 && !sesph && ctx->autotune) autotune(ctx, b, v);
+        b->tuned = true;
+    }
+    if (!sesph) {
+        // scenes still converging: further iterations, four between checks
+        uint32_t enq = chunk;
+        while (enq < max_it) {
+            bool any_active = false;
+            for (auto* s : v)
+                if (s->h_state->active) any_active = true;
+            if (!any_active) break;
+            for (uint32_t k = 0; k < 4 && enq < max_it; k++, enq++) {
+                {
+                    PhaseTimer t(ctx, FSA_TIMER_PRESSURE_A);
+                    if (d2)
+                        launch2_iisph_pass_a(b->ls, st);
+                    else
+                        launch_iisph_pass_a(b->ls, st);
+                }
+                {
+                    PhaseTimer t(ctx, FSA_TIMER_PRESSURE_B);
+                    if (d2)
+                        launch2_iisph_pass_b(b->ls, st);
+                    else
+                        launch_iisph_pass_b(b->ls, st);
+                }
+            }
+            read_states(ctx, v);
+        }
+    }
+    {
+        PhaseTimer t(ctx, FSA_TIMER_INTEGRATE);
+        if (d2)
+            launch2_integrate(b->ls, st);
+        else
+            launch_integrate(b->ls, st);
     }
     read_states(ctx, v);
-    bool bpass = false;
     int32_t first_error = FSA_OK;
-    std::vector<fsa_scene_s*> ok;
-    std::vector<fsa_step_stats*> ok_out;
     for (uint32_t q = 0; q < v.size(); q++) {
         fsa_scene_s* s = v[q];
         try {
@@ -748,17 +1015,8 @@ int32_t batch_step_group(fsa_context ctx, const std::vector<fsa_scene_s*>& v, st
         s->searched = true;
         s->lists_valid = true;
         s->lists_cached = false;
-        pre_simulate(s, s->desc.timestep);
-        bpass = bpass || (s->hd.flags & kFlagDynamicBoundary) || s->boundary_dirty;
-        ok.push_back(s);
-        ok_out.push_back(out[q]);
+        post_simulate(s, out[q]);
     }
-    if (ok.empty()) return first_error;
-    if (ok.size() != v.size()) b = batch_for(ctx, ok);
-    upload_batch_table(ctx, b);
-    simulate_set(ctx, b->ls, ok, ok[0]->desc.solver == FSA_SOLVER_SESPH, bpass);
-    read_states(ctx, ok);
-    for (uint32_t q = 0; q < ok.size(); q++) post_simulate(ok[q], ok_out[q]);
     return first_error;
 }
 
@@ -1605,9 +1863,13 @@ FSA_API int32_t fsa_scene_simulate(fsa_scene s, float dt, fsa_step_stats* stats)
 FSA_API int32_t fsa_scene_step(fsa_scene s, uint32_t steps, fsa_step_stats* stats) {
     return guarded([&] {
         if (!s) throw FsaError(FSA_E_INVALID, "scene is NULL");
+        HIP_CHECK(hipSetDevice(s->ctx->device));
+        // a step of one scene is a batch of one: search and solve without a host check in between
+        std::vector<fsa_scene_s*> one{s};
+        std::vector<fsa_step_stats*> out{stats};
         for (uint32_t k = 0; k < steps; k++) {
-            do_find_neighbors(s);
-            do_simulate(s, s->desc.timestep, stats);
+            const int32_t rc = batch_step_group(s->ctx, one, out);
+            if (rc != FSA_OK) throw FsaError(rc, g_error);
         }
     });
 }
@@ -1645,6 +1907,40 @@ FSA_API int32_t fsa_batch_set_graph(fsa_context c, uint32_t enabled) {
     return guarded([&] {
         if (!c) throw FsaError(FSA_E_INVALID, "context is NULL");
         c->graph = enabled != 0;
+    });
+}
+
+FSA_API int32_t fsa_context_set_autotune(fsa_context c, uint32_t enabled) {
+    return guarded([&] {
+        if (!c) throw FsaError(FSA_E_INVALID, "context is NULL");
+        c->autotune = enabled != 0;
+    });
+}
+
+FSA_API int32_t fsa_context_get_tuning(fsa_context c, char* buf, uint64_t cap, uint64_t* len) {
+    return guarded([&] {
+        if (!c || !len) throw FsaError(FSA_E_INVALID, "NULL argument");
+        static const char* names[kTuneFamilies] = {"prepare", "diagonal", "pass_a", "pass_b"};
+        std::string out;
+        char line[256];
+        for (auto& kv : c->tuned) {
+            snprintf(line, sizeof(line), "class spec=%u fluid<2^%u (first seen with %llu fluid):",
+                     (unsigned)(kv.first >> 32), (unsigned)(kv.first & 0xffffffffu),
+                     (unsigned long long)kv.second.fluid);
+            out += line;
+            for (int f = 0; f < kTuneFamilies; f++) {
+                snprintf(line, sizeof(line), " %s=U%u [U2 %.1f us, U4 %.1f us, U8 %.1f us]", names[f],
+                         (unsigned)kv.second.unroll[f], kv.second.us[f][0], kv.second.us[f][1], kv.second.us[f][2]);
+                out += line;
+            }
+            out += "\n";
+        }
+        *len = out.size();
+        if (buf && cap) {
+            const uint64_t m = std::min<uint64_t>(cap - 1, out.size());
+            std::memcpy(buf, out.data(), m);
+            buf[m] = 0;
+        }
     });
 }
 

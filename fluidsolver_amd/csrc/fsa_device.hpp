@@ -154,6 +154,13 @@ struct BlockRef {
     uint32_t block;
 };
 
+// A scene whose search reported an error (neighbor capacity, cell grid) is
+// left untouched by the solver kernels of that step: its particle state stays
+// as the search left it, and the host reports the error.
+__device__ __forceinline__ bool scene_failed(const SceneDev& S) {
+    return (S.state->status & (kStatusCapacity | kStatusGrid)) != 0u;
+}
+
 // ---- reference-order arithmetic ------------------------------------------
 // glm::dot for vec3: ((x*x') + (y*y')) + (z*z')
 __device__ __forceinline__ float dot3(float ax, float ay, float az, float bx, float by, float bz) {

@@ -19,7 +19,16 @@ struct LaunchSet {
     BlockRef* map_scene = nullptr;   // one block per scene
     uint32_t nb_all = 0, nb_fluid = 0, nb_tiles = 0, nb_scan = 0, nb_scene = 0;
     uint32_t spec = 0;  // kSpec* bits shared by every scene of the set
+    // neighbor-walk unroll per IISPH kernel family (kTune*), chosen by the autotuner
+    uint8_t unroll[4] = {4, 4, 4, 4};
 };
+
+// kernel families the autotuner times
+constexpr int kTunePrepare = 0;
+constexpr int kTuneDiagonal = 1;
+constexpr int kTunePassA = 2;
+constexpr int kTunePassB = 3;
+constexpr int kTuneFamilies = 4;
 
 // neighborhood search
 void launch_keys(const LaunchSet& L, hipStream_t st);
@@ -38,6 +47,8 @@ void launch_iisph_pass_b(const LaunchSet& L, hipStream_t st);
 void launch_sesph_density(const LaunchSet& L, hipStream_t st);
 void launch_sesph_accel(const LaunchSet& L, hipStream_t st);
 void launch_integrate(const LaunchSet& L, hipStream_t st);
+// one kernel of a tuned family with the set's current unroll choice
+void launch_tuned_kernel(const LaunchSet& L, int family, hipStream_t st);
 
 // 2D (hashed / quadratic-dynamic-allocated searches, IISPH and SESPH)
 void launch2_search(const LaunchSet& L, uint32_t entry_passes, uint32_t cell_passes, hipStream_t st);

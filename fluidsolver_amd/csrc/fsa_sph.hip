@@ -133,11 +133,12 @@ struct PosP {
     float p;
 };
 
-template <uint32_t Spec>
+template <uint32_t Spec, int U>
 __global__ __launch_bounds__(kBlock) void k_iisph_prepare(const SceneDev* __restrict__ scenes,
                                                           const BlockRef* __restrict__ map) {
     const BlockRef br = map[blockIdx.x];
     const SceneDev& S = scenes[br.scene];
+    if (scene_failed(S)) return;
     const uint32_t k = br.block * kBlock + threadIdx.x;
     if (k >= S.nf) return;
     const uint32_t c = S.state->cur;
@@ -155,7 +156,7 @@ __global__ __launch_bounds__(kBlock) void k_iisph_prepare(const SceneDev* __rest
     }
     const bool sl = (S.flags & kFlagSingleLayer) != 0;
     float tx = 0.0f, ty = 0.0f, tz = 0.0f, rho = 0.0f;
-    walk_neighbors<kUnroll>(
+    walk_neighbors<U>(
         gload_ptr(S.nbr + k), S.list_stride, S.nbr_cnt[k],
         [&](uint32_t j) { return PosVel{pos[j], vel[j]}; },
         [&](uint32_t ent, const PosVel& d) {
@@ -194,6 +195,7 @@ __global__ __launch_bounds__(kBlock) void k_iisph_prepare_boundary(const SceneDe
                                                                    const BlockRef* __restrict__ map) {
     const BlockRef br = map[blockIdx.x];
     const SceneDev& S = scenes[br.scene];
+    if (scene_failed(S)) return;
     const uint32_t i = br.block * kBlock + threadIdx.x;
     if (i >= S.n) return;
     const uint32_t c = S.state->cur;
@@ -209,11 +211,12 @@ __global__ __launch_bounds__(kBlock) void k_iisph_prepare_boundary(const SceneDe
 }
 
 // ---- IISPH phase 2: source term and diagonal element -----------------------------
-template <uint32_t Spec>
+template <uint32_t Spec, int U>
 __global__ __launch_bounds__(kBlock) void k_iisph_diagonal(const SceneDev* __restrict__ scenes,
                                                            const BlockRef* __restrict__ map) {
     const BlockRef br = map[blockIdx.x];
     const SceneDev& S = scenes[br.scene];
+    if (scene_failed(S)) return;
     const uint32_t k = br.block * kBlock + threadIdx.x;
     if (k >= S.nf) return;
     const uint32_t c = S.state->cur;
@@ -227,7 +230,7 @@ __global__ __launch_bounds__(kBlock) void k_iisph_diagonal(const SceneDev* __res
     const auto nb = gload_ptr(S.nbr + k);
     float contrib = 0.0f;
     float ix = 0.0f, iy = 0.0f, iz = 0.0f;
-    walk_neighbors<kUnroll>(
+    walk_neighbors<U>(
         nb, S.list_stride, cnt, [&](uint32_t j) { return PosVel{pos[j], vstar[j]}; },
         [&](uint32_t ent, const PosVel& d) {
             const float4 xj = d.x;
@@ -251,7 +254,7 @@ __global__ __launch_bounds__(kBlock) void k_iisph_diagonal(const SceneDev* __res
     S.src[i] = S.rest_density - S.rho_new[i] - dt * contrib;
     const float cm = div_rho0sq<Spec>(S, xi.w);
     float dg = 0.0f;
-    walk_neighbors<kUnroll>(
+    walk_neighbors<U>(
         nb, S.list_stride, cnt, [&](uint32_t j) { return pos[j]; },
         [&](uint32_t ent, const float4& xj) {
             float dx, dy, dz, d2, len;
@@ -269,6 +272,7 @@ __global__ __launch_bounds__(kBlock) void k_iisph_diagonal(const SceneDev* __res
 // ---- Jacobi loop control -------------------------------------------------------------
 __global__ void k_iisph_begin_loop(const SceneDev* __restrict__ scenes, const BlockRef* __restrict__ map) {
     const SceneDev& S = scenes[map[blockIdx.x].scene];
+    if (scene_failed(S)) return;
     if (threadIdx.x == 0) {
         SceneState* st = S.state;
         st->active = S.max_it > 0 ? 1u : 0u;
@@ -281,11 +285,12 @@ __global__ void k_iisph_begin_loop(const SceneDev* __restrict__ scenes, const Bl
 }
 
 // ---- Jacobi pass A: pressure acceleration ------------------------------------------
-template <uint32_t Spec>
+template <uint32_t Spec, int U>
 __global__ __launch_bounds__(kBlock) void k_iisph_pass_a(const SceneDev* __restrict__ scenes,
                                                          const BlockRef* __restrict__ map) {
     const BlockRef br = map[blockIdx.x];
     const SceneDev& S = scenes[br.scene];
+    if (scene_failed(S)) return;
     if (!S.state->active) return;
     const uint32_t k = br.block * kBlock + threadIdx.x;
     if (k >= S.nf) return;
@@ -298,7 +303,7 @@ __global__ __launch_bounds__(kBlock) void k_iisph_pass_a(const SceneDev* __restr
     const float pir = div_rho0sq<Spec>(S, pi);
     const bool sl = (S.flags & kFlagSingleLayer) != 0;
     float ax = 0.0f, ay = 0.0f, az = 0.0f;
-    walk_neighbors<kUnroll>(
+    walk_neighbors<U>(
         gload_ptr(S.nbr + k), S.list_stride, S.nbr_cnt[k], [&](uint32_t j) { return PosP{pos[j], pres[j]}; },
         [&](uint32_t ent, const PosP& d) {
             const float4 xj = d.x;
@@ -319,11 +324,12 @@ __global__ __launch_bounds__(kBlock) void k_iisph_pass_a(const SceneDev* __restr
 }
 
 // ---- Jacobi pass B: A*p, relaxed pressure update, error, reductions -------------------
-template <uint32_t Spec>
+template <uint32_t Spec, int U>
 __global__ __launch_bounds__(kBlock) void k_iisph_pass_b(const SceneDev* __restrict__ scenes,
                                                          const BlockRef* __restrict__ map) {
     const BlockRef br = map[blockIdx.x];
     const SceneDev& S = scenes[br.scene];
+    if (scene_failed(S)) return;
     if (!S.state->active) return;
     __shared__ RedLds lds;
     const uint32_t k = br.block * kBlock + threadIdx.x;
@@ -338,7 +344,7 @@ __global__ __launch_bounds__(kBlock) void k_iisph_pass_b(const SceneDev* __restr
         const float4 xi = pos[i];
         const float4 ai = acc[i];
         float ap = 0.0f;
-        walk_neighbors<kUnroll>(
+        walk_neighbors<U>(
             gload_ptr(S.nbr + k), S.list_stride, S.nbr_cnt[k], [&](uint32_t j) { return PosAcc{pos[j], acc[j]}; },
             [&](uint32_t ent, const PosAcc& d) {
                 const float4 xj = d.x;
@@ -383,6 +389,7 @@ __global__ __launch_bounds__(kBlock) void k_iisph_pass_b(const SceneDev* __restr
 __global__ __launch_bounds__(kBlock) void k_iisph_converge(const SceneDev* __restrict__ scenes,
                                                            const BlockRef* __restrict__ map) {
     const SceneDev& S = scenes[map[blockIdx.x].scene];
+    if (scene_failed(S)) return;
     if (!S.state->active) return;
     __shared__ RedLds lds;
     const Partial tot = reduce_partials(S.partial, scene_fluid_blocks(S), lds);
@@ -406,6 +413,7 @@ __global__ __launch_bounds__(kBlock) void k_sesph_density(const SceneDev* __rest
                                                           const BlockRef* __restrict__ map) {
     const BlockRef br = map[blockIdx.x];
     const SceneDev& S = scenes[br.scene];
+    if (scene_failed(S)) return;
     const uint32_t k = br.block * kBlock + threadIdx.x;
     if (k >= S.nf) return;
     const uint32_t c = S.state->cur;
@@ -442,6 +450,7 @@ __global__ __launch_bounds__(kBlock) void k_sesph_accel(const SceneDev* __restri
                                                         const BlockRef* __restrict__ map) {
     const BlockRef br = map[blockIdx.x];
     const SceneDev& S = scenes[br.scene];
+    if (scene_failed(S)) return;
     const uint32_t k = br.block * kBlock + threadIdx.x;
     if (k >= S.nf) return;
     const uint32_t c = S.state->cur;
@@ -503,6 +512,7 @@ __global__ __launch_bounds__(kBlock) void k_cfl_all(const SceneDev* __restrict__
                                                     const BlockRef* __restrict__ map) {
     const BlockRef br = map[blockIdx.x];
     const SceneDev& S = scenes[br.scene];
+    if (scene_failed(S)) return;
     __shared__ RedLds lds;
     const uint32_t i = br.block * kBlock + threadIdx.x;
     const uint32_t c = S.state->cur;
@@ -522,6 +532,7 @@ __global__ __launch_bounds__(kBlock) void k_cfl_all(const SceneDev* __restrict__
 __global__ __launch_bounds__(kBlock) void k_cfl_finalize(const SceneDev* __restrict__ scenes,
                                                          const BlockRef* __restrict__ map) {
     const SceneDev& S = scenes[map[blockIdx.x].scene];
+    if (scene_failed(S)) return;
     __shared__ RedLds lds;
     const Partial tot = reduce_partials(S.partial, S.nblocks, lds);
     if (threadIdx.x == 0) {
@@ -537,6 +548,7 @@ __global__ __launch_bounds__(kBlock) void k_integrate(const SceneDev* __restrict
                                                       const BlockRef* __restrict__ map) {
     const BlockRef br = map[blockIdx.x];
     const SceneDev& S = scenes[br.scene];
+    if (scene_failed(S)) return;
     __shared__ int32_t blo[3][kBlock / 64], bhi[3][kBlock / 64];
     const uint32_t i = br.block * kBlock + threadIdx.x;
     const uint32_t c = S.state->cur;
@@ -614,6 +626,7 @@ __global__ __launch_bounds__(kBlock) void k_integrate(const SceneDev* __restrict
 __global__ __launch_bounds__(kBlock) void k_integrate_finalize(const SceneDev* __restrict__ scenes,
                                                                const BlockRef* __restrict__ map) {
     const SceneDev& S = scenes[map[blockIdx.x].scene];
+    if (scene_failed(S)) return;
     __shared__ int32_t blo[3][kBlock / 64], bhi[3][kBlock / 64];
     int32_t l[3] = {INT32_MAX, INT32_MAX, INT32_MAX}, h[3] = {INT32_MIN, INT32_MIN, INT32_MIN};
     for (uint32_t b = threadIdx.x; b < S.nblocks; b += kBlock) {
@@ -665,22 +678,49 @@ __global__ __launch_bounds__(kBlock) void k_integrate_finalize(const SceneDev* _
         }                                                                                           \
     } while (0)
 
+// spec x neighbor-walk unroll (the unroll chosen by the autotuner per kernel family)
+#define FSA_SPEC_U(kernel, U, grid, map)                                                            \
+    do {                                                                                            \
+        switch (L.spec) {                                                                           \
+            case 3: kernel<3, U><<<grid, kBlock, 0, st>>>(L.scenes, map); break;                    \
+            case 1: kernel<1, U><<<grid, kBlock, 0, st>>>(L.scenes, map); break;                    \
+            default: kernel<0, U><<<grid, kBlock, 0, st>>>(L.scenes, map); break;                   \
+        }                                                                                           \
+    } while (0)
+#define FSA_UNROLL_DISPATCH(kernel, family, grid, map)                                              \
+    do {                                                                                            \
+        switch (L.unroll[family]) {                                                                 \
+            case 2: FSA_SPEC_U(kernel, 2, grid, map); break;                                        \
+            case 8: FSA_SPEC_U(kernel, 8, grid, map); break;                                        \
+            default: FSA_SPEC_U(kernel, 4, grid, map); break;                                       \
+        }                                                                                           \
+    } while (0)
+
 void launch_iisph_prepare(const LaunchSet& L, bool boundary_pass, hipStream_t st) {
-    if (L.nb_fluid) FSA_SPEC_DISPATCH(k_iisph_prepare, L.nb_fluid, L.map_fluid);
+    if (L.nb_fluid) FSA_UNROLL_DISPATCH(k_iisph_prepare, kTunePrepare, L.nb_fluid, L.map_fluid);
     if (boundary_pass && L.nb_all) k_iisph_prepare_boundary<<<L.nb_all, kBlock, 0, st>>>(L.scenes, L.map_all);
 }
 void launch_iisph_diagonal(const LaunchSet& L, hipStream_t st) {
-    if (L.nb_fluid) FSA_SPEC_DISPATCH(k_iisph_diagonal, L.nb_fluid, L.map_fluid);
+    if (L.nb_fluid) FSA_UNROLL_DISPATCH(k_iisph_diagonal, kTuneDiagonal, L.nb_fluid, L.map_fluid);
 }
 void launch_iisph_begin_loop(const LaunchSet& L, hipStream_t st) {
     k_iisph_begin_loop<<<L.nb_scene, 64, 0, st>>>(L.scenes, L.map_scene);
 }
 void launch_iisph_pass_a(const LaunchSet& L, hipStream_t st) {
-    if (L.nb_fluid) FSA_SPEC_DISPATCH(k_iisph_pass_a, L.nb_fluid, L.map_fluid);
+    if (L.nb_fluid) FSA_UNROLL_DISPATCH(k_iisph_pass_a, kTunePassA, L.nb_fluid, L.map_fluid);
 }
 void launch_iisph_pass_b(const LaunchSet& L, hipStream_t st) {
-    if (L.nb_fluid) FSA_SPEC_DISPATCH(k_iisph_pass_b, L.nb_fluid, L.map_fluid);
+    if (L.nb_fluid) FSA_UNROLL_DISPATCH(k_iisph_pass_b, kTunePassB, L.nb_fluid, L.map_fluid);
     k_iisph_converge<<<L.nb_scene, kBlock, 0, st>>>(L.scenes, L.map_scene);
+}
+void launch_tuned_kernel(const LaunchSet& L, int family, hipStream_t st) {
+    if (!L.nb_fluid) return;
+    switch (family) {
+        case kTunePrepare: FSA_UNROLL_DISPATCH(k_iisph_prepare, kTunePrepare, L.nb_fluid, L.map_fluid); break;
+        case kTuneDiagonal: FSA_UNROLL_DISPATCH(k_iisph_diagonal, kTuneDiagonal, L.nb_fluid, L.map_fluid); break;
+        case kTunePassA: FSA_UNROLL_DISPATCH(k_iisph_pass_a, kTunePassA, L.nb_fluid, L.map_fluid); break;
+        default: FSA_UNROLL_DISPATCH(k_iisph_pass_b, kTunePassB, L.nb_fluid, L.map_fluid); break;
+    }
 }
 void launch_sesph_density(const LaunchSet& L, hipStream_t st) {
     if (L.nb_fluid) FSA_SPEC_DISPATCH(k_sesph_density, L.nb_fluid, L.map_fluid);

@@ -182,6 +182,7 @@ __global__ __launch_bounds__(kBlock) void k2_iisph_prepare(const SceneDev* __res
                                                            const BlockRef* __restrict__ map) {
     const BlockRef br = map[blockIdx.x];
     const SceneDev& S = scenes[br.scene];
+    if (scene_failed(S)) return;
     const uint32_t k = br.block * kBlock + threadIdx.x;
     if (k >= S.nf) return;
     const uint32_t i = S.fluid_idx[k];
@@ -230,6 +231,7 @@ __global__ __launch_bounds__(kBlock) void k2_prepare_boundary(const SceneDev* __
                                                               const BlockRef* __restrict__ map) {
     const BlockRef br = map[blockIdx.x];
     const SceneDev& S = scenes[br.scene];
+    if (scene_failed(S)) return;
     const uint32_t i = br.block * kBlock + threadIdx.x;
     if (i >= S.n || type_of(S, i) != 1u) return;
     const float4 e = S.ext[0][i];
@@ -246,6 +248,7 @@ __global__ __launch_bounds__(kBlock) void k2_iisph_diagonal(const SceneDev* __re
                                                             const BlockRef* __restrict__ map) {
     const BlockRef br = map[blockIdx.x];
     const SceneDev& S = scenes[br.scene];
+    if (scene_failed(S)) return;
     const uint32_t k = br.block * kBlock + threadIdx.x;
     if (k >= S.nf) return;
     const uint32_t i = S.fluid_idx[k];
@@ -301,6 +304,7 @@ __global__ __launch_bounds__(kBlock) void k2_iisph_diagonal(const SceneDev* __re
 // loop entry: while ((it < min || |err| > max_err) && it < max) with err = 0
 __global__ void k2_iisph_begin_loop(const SceneDev* __restrict__ scenes, const BlockRef* __restrict__ map) {
     const SceneDev& S = scenes[map[blockIdx.x].scene];
+    if (scene_failed(S)) return;
     if (threadIdx.x == 0) {
         SceneState* st = S.state;
         st->iterations = 0;
@@ -317,6 +321,7 @@ __global__ __launch_bounds__(kBlock) void k2_iisph_pass_a(const SceneDev* __rest
                                                           const BlockRef* __restrict__ map) {
     const BlockRef br = map[blockIdx.x];
     const SceneDev& S = scenes[br.scene];
+    if (scene_failed(S)) return;
     if (!S.state->active) return;
     const uint32_t k = br.block * kBlock + threadIdx.x;
     if (k >= S.nf) return;
@@ -349,6 +354,7 @@ __global__ __launch_bounds__(kBlock) void k2_iisph_pass_b(const SceneDev* __rest
                                                           const BlockRef* __restrict__ map) {
     const BlockRef br = map[blockIdx.x];
     const SceneDev& S = scenes[br.scene];
+    if (scene_failed(S)) return;
     if (!S.state->active) return;
     __shared__ double werr[kBlock / 64];
     __shared__ uint32_t wcnt[kBlock / 64];
@@ -420,6 +426,7 @@ __global__ __launch_bounds__(kBlock) void k2_iisph_pass_b(const SceneDev* __rest
 __global__ __launch_bounds__(kBlock) void k2_iisph_converge(const SceneDev* __restrict__ scenes,
                                                             const BlockRef* __restrict__ map) {
     const SceneDev& S = scenes[map[blockIdx.x].scene];
+    if (scene_failed(S)) return;
     if (!S.state->active) return;
     __shared__ double werr[kBlock / 64];
     __shared__ uint32_t wcnt[kBlock / 64];
@@ -464,6 +471,7 @@ __global__ __launch_bounds__(kBlock) void k2_sesph_density(const SceneDev* __res
                                                            const BlockRef* __restrict__ map) {
     const BlockRef br = map[blockIdx.x];
     const SceneDev& S = scenes[br.scene];
+    if (scene_failed(S)) return;
     const uint32_t k = br.block * kBlock + threadIdx.x;
     if (k >= S.nf) return;
     const uint32_t i = S.fluid_idx[k];
@@ -489,6 +497,7 @@ __global__ __launch_bounds__(kBlock) void k2_sesph_commit(const SceneDev* __rest
                                                           const BlockRef* __restrict__ map) {
     const BlockRef br = map[blockIdx.x];
     const SceneDev& S = scenes[br.scene];
+    if (scene_failed(S)) return;
     const uint32_t k = br.block * kBlock + threadIdx.x;
     if (k >= S.nf) return;
     const uint32_t i = S.fluid_idx[k];
@@ -500,6 +509,7 @@ __global__ __launch_bounds__(kBlock) void k2_sesph_accel(const SceneDev* __restr
                                                          const BlockRef* __restrict__ map) {
     const BlockRef br = map[blockIdx.x];
     const SceneDev& S = scenes[br.scene];
+    if (scene_failed(S)) return;
     const uint32_t k = br.block * kBlock + threadIdx.x;
     if (k >= S.nf) return;
     const uint32_t i = S.fluid_idx[k];
@@ -560,6 +570,7 @@ __global__ __launch_bounds__(kBlock) void k2_integrate(const SceneDev* __restric
                                                        const BlockRef* __restrict__ map) {
     const BlockRef br = map[blockIdx.x];
     const SceneDev& S = scenes[br.scene];
+    if (scene_failed(S)) return;
     __shared__ int32_t blo[2][kBlock / 64], bhi[2][kBlock / 64];
     const uint32_t i = br.block * kBlock + threadIdx.x;
     int32_t lo[2] = {INT32_MAX, INT32_MAX}, hi[2] = {INT32_MIN, INT32_MIN};
@@ -623,6 +634,7 @@ __global__ __launch_bounds__(kBlock) void k2_integrate(const SceneDev* __restric
 __global__ __launch_bounds__(kBlock) void k2_integrate_finalize(const SceneDev* __restrict__ scenes,
                                                                 const BlockRef* __restrict__ map) {
     const SceneDev& S = scenes[map[blockIdx.x].scene];
+    if (scene_failed(S)) return;
     __shared__ int32_t blo[2][kBlock / 64], bhi[2][kBlock / 64];
     int32_t l[2] = {INT32_MAX, INT32_MAX}, h[2] = {INT32_MIN, INT32_MIN};
     for (uint32_t b = threadIdx.x; b < S.nblocks; b += kBlock) {

@@ -81,6 +81,21 @@ class Context:
         check(lib().fsa_context_get_timing(self._h, ms, cnt, n))
         return {abi.TIMER_NAMES[k]: (ms[k], cnt[k]) for k in range(n)}
 
+    def set_graph(self, enabled):
+        """Replay the head of batched steps from captured hipGraphs."""
+        check(lib().fsa_batch_set_graph(self._h, 1 if enabled else 0))
+
+    def set_autotune(self, enabled):
+        check(lib().fsa_context_set_autotune(self._h, 1 if enabled else 0))
+
+    def tuning(self):
+        """Text report of the autotuned unroll choices and their timings."""
+        n = C.c_uint64(0)
+        check(lib().fsa_context_get_tuning(self._h, None, 0, C.byref(n)))
+        buf = C.create_string_buffer(n.value + 1)
+        check(lib().fsa_context_get_tuning(self._h, buf, n.value + 1, C.byref(n)))
+        return buf.value.decode()
+
     def batch_step(self, scenes, steps=1):
         arr = (C.c_void_p * len(scenes))(*[s.handle.value for s in scenes])
         st = (abi.StepStats * len(scenes))()

@@ -222,11 +222,21 @@ FSA_API int32_t fsa_scene_step(fsa_scene scene, uint32_t steps, fsa_step_stats* 
  * are identical to stepping it alone. stats (may be NULL) holds n entries. */
 FSA_API int32_t fsa_batch_step(fsa_context ctx, const fsa_scene* scenes, uint32_t n, uint32_t steps,
                                fsa_step_stats* stats);
-/* Captures one batched step of these scenes into a HIP graph that later
- * fsa_batch_step calls with the same scene list replay (0 = off). */
+/* With graphs on (0 = off, the default), the head of every batched step and of
+ * fsa_scene_step (search, solver phases and the predicted number of Jacobi
+ * iterations) is captured once per scene list and launch configuration into
+ * a HIP graph and replayed; further iterations and the integration are
+ * launched directly.  Results are identical either way. */
 FSA_API int32_t fsa_batch_set_graph(fsa_context ctx, uint32_t enabled);
-/* Launch shape override for the per-particle SPH kernels (threads per block:
- * 64, 128, 256, 512 or 1024); 0 restores the built-in default. */
+/* Autotuning (no counterpart in the reference): on the first batched step of
+ * a size class (kernel spec, total fluid rounded to a power of two) the
+ * neighbor-walk unroll of each IISPH kernel family is chosen by timing the
+ * candidates on the live neighbor lists, writing only to scratch copies.
+ * Enabled by default; the choices and their timings are reported as text. */
+FSA_API int32_t fsa_context_set_autotune(fsa_context ctx, uint32_t enabled);
+FSA_API int32_t fsa_context_get_tuning(fsa_context ctx, char* buf, uint64_t cap, uint64_t* len);
+/* Threads per block of the per-particle kernels: 0 (the default) or 256, the
+ * one shape this build's block maps are laid out for. */
 FSA_API int32_t fsa_context_set_block_size(fsa_context ctx, uint32_t threads);
 
 /* ---- serving: continuous batching of scene jobs ---------------------------- */
