@@ -32,6 +32,8 @@ LIST_COMPRESSED = 0
 LIST_PLAIN = 1
 SEARCH_DEFAULT = 0
 SEARCH_COMPRESSED_3D = 1
+SEARCH_HASHED_3D = 2
+SEARCH_QUADRATIC_3D = 3
 SEARCH_HASHED_2D = 4
 SEARCH_QUADRATIC_2D = 5
 TYPE_NORMAL = 0

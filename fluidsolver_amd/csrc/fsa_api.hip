@@ -16,6 +16,7 @@
 #include <cstring>
 #include <exception>
 #include <map>
+#include <numeric>
 #include <mutex>
 #include <new>
 #include <stdexcept>
@@ -69,6 +70,7 @@ uint64_t align_up(uint64_t v, uint64_t a) { return (v + a - 1) / a * a; }
 // device-memory plan of one scene (all arrays carved from one allocation)
 struct Plan {
     uint64_t n, nf, stride, nblocks, nfblocks, ntiles, dim;
+    bool unsorted;  // 3D hashed / quadratic: per-particle cell of the last grid update
     uint64_t bytes;
     std::vector<std::pair<void**, uint64_t>> parts;
 };
@@ -143,6 +145,10 @@ struct fsa_scene_s {
     bool searched = false;
     bool boundary_dirty = true;
     uint32_t last_iterations = 2;
+    // HashedNeighborhoodSearch3D: calls since the last re-sort and the collection
+    // order the reference would hold (caller rows, inactive included)
+    uint32_t hashed_calls = 0;
+    std::vector<uint32_t> ref_order;
     bool lists_cached = false;
     bool lists_valid = false;  // lists of the last search (valid until the next search or upload)
     std::vector<uint64_t> csr_off, csr_cols;
@@ -194,7 +200,10 @@ void fill_params(fsa_scene_s* s) {
     if (sesph) f |= kFlagSesph;
     if (d.neighbor_lists == FSA_LIST_PLAIN) f |= kFlagPlainLists;
     if (d.dimension == 2 && !sesph && (d.flags & FSA_FLAG_EMULATE_INT_ABS)) f |= kFlagIntAbs;
-    if (search_of(&d) == FSA_SEARCH_QUADRATIC_2D) f |= kFlagQuadratic;
+    const uint32_t search = search_of(&d);
+    if (search == FSA_SEARCH_QUADRATIC_2D || search == FSA_SEARCH_QUADRATIC_3D) f |= kFlagQuadratic;
+    if (search == FSA_SEARCH_HASHED_3D || search == FSA_SEARCH_QUADRATIC_3D) f |= kFlagUnsorted;
+    if (search == FSA_SEARCH_COMPRESSED_3D && d.neighbor_lists == FSA_LIST_COMPRESSED) f |= kFlagCodec;
     h.flags = f;
 }
 
@@ -202,8 +211,9 @@ void validate_desc(const fsa_scene_desc* d) {
     if (!d) throw FsaError(FSA_E_INVALID, "desc is NULL");
     if (d->dimension != 2 && d->dimension != 3) throw FsaError(FSA_E_INVALID, "dimension must be 2 or 3");
     const uint32_t search = search_of(d);
-    if (d->dimension == 3 && search != FSA_SEARCH_COMPRESSED_3D)
-        throw FsaError(FSA_E_INVALID, "3D scenes use the compressed-3d neighbor order");
+    if (d->dimension == 3 && search != FSA_SEARCH_COMPRESSED_3D && search != FSA_SEARCH_HASHED_3D &&
+        search != FSA_SEARCH_QUADRATIC_3D)
+        throw FsaError(FSA_E_INVALID, "3D scenes use the compressed-3d, hashed-3d or quadratic-3d neighbor order");
     if (d->dimension == 2 && search != FSA_SEARCH_HASHED_2D && search != FSA_SEARCH_QUADRATIC_2D)
         throw FsaError(FSA_E_INVALID, "2D scenes use the hashed or quadratic-dynamic-allocated neighbor order");
     if (d->solver != FSA_SOLVER_IISPH && d->solver != FSA_SOLVER_SESPH)
@@ -219,9 +229,10 @@ void validate_desc(const fsa_scene_desc* d) {
     }
 }
 
-Plan make_plan(uint64_t n, uint64_t nf, uint32_t dim = 3) {
+Plan make_plan(uint64_t n, uint64_t nf, uint32_t dim = 3, bool unsorted = false) {
     Plan p{};
     p.dim = dim;
+    p.unsorted = unsorted;
     p.n = n;
     p.nf = nf;
     p.stride = align_up(std::max<uint64_t>(nf, 1), 64);
@@ -255,6 +266,7 @@ uint64_t plan_bytes(const Plan& p) {
         add(8 * n);  // cell_prev
         add(4 * n);  // entry
     }
+    if (p.unsorted) add(16 * n);  // cell_last
     return b;
 }
 
@@ -294,6 +306,7 @@ void carve(fsa_scene_s* s, const Plan& p) {
         h.cell_prev = (int2*)take(8 * n);
         h.entry = (uint32_t*)take(4 * n);
     }
+    if (p.unsorted) h.cell_last = (int4*)take(16 * n);
     h.n = (uint32_t)p.n;
     h.nf = (uint32_t)p.nf;
     h.list_stride = (uint32_t)p.stride;
@@ -366,7 +379,11 @@ void prepare_grid(fsa_scene_s* s) {
         for (int a = 0; a < 3; a++) lo[a] = hi[a] = 0;
     }
     uint32_t L = 0;
-    if (h.dim == 3) {
+    if (h.dim == 3 && (h.flags & kFlagUnsorted)) {
+        uint64_t cells = 1;
+        for (int a = 0; a < 3; a++) cells *= (uint64_t)(hi[a] - (int64_t)lo[a] + 1);
+        if (cells > (1ull << 28)) throw FsaError(FSA_E_INVALID, "3D scene extent exceeds 2^28 grid cells");
+    } else if (h.dim == 3) {
         L = levels_for(lo, hi);
         if (L > 9)
             throw FsaError(FSA_E_INVALID,
@@ -573,6 +590,7 @@ void check_status(fsa_scene_s* s) {
 
 void upload_scene_table(fsa_scene_s* s) {
     s->single.spec = spec_of(s->hd);
+    s->single.codec = (s->hd.flags & kFlagCodec) != 0;
     HIP_CHECK(hipMemcpyAsync(s->d_table, &s->hd, sizeof(SceneDev), hipMemcpyHostToDevice, s->ctx->stream));
 }
 
@@ -594,17 +612,123 @@ uint32_t cell_passes(const SceneDev& h) {
     return std::max<uint32_t>(1, (bits_of(h.cdim[3]) + kRadixBits - 1) / kRadixBits);
 }
 
+// 3D hashed / quadratic: radix passes of the cell sort
+uint32_t unsorted_passes(const SceneDev& h) {
+    return std::max<uint32_t>(1, (bits_of(h.cdim[3]) + kRadixBits - 1) / kRadixBits);
+}
+
+void search3u_set(fsa_context ctx, const LaunchSet& L, uint32_t passes) {
+    PhaseTimer t(ctx, FSA_TIMER_SEARCH);
+    launch3u_search(L, passes, ctx->stream);
+}
+
+// libmorton morton3D_64_encode of the cell key (CompressedNeighbors.cpp:43-56,
+// HashedNeighborhoodSearch3D.cpp:236-244): coordinate + 2^31, x bits at 3k
+// (22 kept), y at 3k + 1 and z at 3k + 2 (21 kept)
+uint64_t reference_cell_key(int32_t cx, int32_t cy, int32_t cz) {
+    const uint32_t u[3] = {(uint32_t)((int64_t)cx - INT32_MIN), (uint32_t)((int64_t)cy - INT32_MIN),
+                           (uint32_t)((int64_t)cz - INT32_MIN)};
+    uint64_t key = 0;
+    for (int b = 0; b < 22; b++)
+        for (int a = 0; a < 3; a++) {
+            const int bit = 3 * b + a;
+            if (bit < 64) key |= (uint64_t)((u[a] >> b) & 1u) << bit;
+        }
+    return key;
+}
+
+// Sort::quick_sort (ParticleCollectionAlgorithm.hpp:28-127): Hoare partition
+// around the middle element's key; the permutation (position k takes old
+// position perm[k]) is what swapping every component would produce.
+std::vector<uint32_t> quick_sort_order(std::vector<uint64_t> key) {
+    const size_t n = key.size();
+    std::vector<uint32_t> perm(n);
+    std::iota(perm.begin(), perm.end(), 0u);
+    std::vector<std::pair<int64_t, int64_t>> todo;
+    if (n > 1) todo.push_back({0, (int64_t)n - 1});
+    while (!todo.empty()) {
+        const auto [low, high] = todo.back();
+        todo.pop_back();
+        if (low >= high) continue;
+        const uint64_t pivot = key[low + (high - low) / 2];
+        int64_t i = low - 1, j = high + 1;
+        for (;;) {
+            do {
+                i++;
+            } while (key[i] < pivot);
+            do {
+                j--;
+            } while (key[j] > pivot);
+            if (i >= j) break;
+            std::swap(key[i], key[j]);
+            std::swap(perm[i], perm[j]);
+        }
+        todo.push_back({low, j});
+        todo.push_back({j + 1, high});
+    }
+    return perm;
+}
+
+// HashedNeighborhoodSearch3D::improve_cache_efficiency (HashedNeighborhoodSearch3D.cpp:219-247):
+// on every 101st call the collection, inactive particles included, is re-sorted
+// by the Morton key of each particle's cell from the previous grid update
+// (GridCellState.current; INT_MIN, key 0, for particles never updated).  The
+// device applies the active particles' share of that permutation.
+void hashed_resort_if_due(fsa_scene_s* s) {
+    if (search_of(&s->desc) != FSA_SEARCH_HASHED_3D) return;
+    s->hashed_calls++;
+    if (s->hashed_calls <= 100) return;
+    s->hashed_calls = 0;
+    const uint32_t n = s->n;
+    const uint64_t N = s->total;
+    hipStream_t st = s->ctx->stream;
+    read_state(s);
+    std::vector<uint32_t> idt(n);
+    std::vector<int4> cells(n);
+    if (n) {
+        HIP_CHECK(hipMemcpyAsync(idt.data(), s->hd.idt[s->h_state->cur], 4ull * n, hipMemcpyDeviceToHost, st));
+        HIP_CHECK(hipMemcpyAsync(cells.data(), s->hd.cell_last, 16ull * n, hipMemcpyDeviceToHost, st));
+        HIP_CHECK(hipStreamSynchronize(st));
+    }
+    const uint64_t never = reference_cell_key(INT32_MIN, INT32_MIN, INT32_MIN);
+    std::vector<uint64_t> key_of_row(N, never);
+    std::vector<uint32_t> pos_of_row(N, UINT32_MAX);
+    for (uint32_t k = 0; k < n; k++) {
+        const uint32_t row = idt[k] & kIdMask;
+        key_of_row[row] = reference_cell_key(cells[k].x, cells[k].y, cells[k].z);
+        pos_of_row[row] = k;
+    }
+    std::vector<uint64_t> keys(N);
+    for (uint64_t p = 0; p < N; p++) keys[p] = key_of_row[s->ref_order[p]];
+    const std::vector<uint32_t> perm = quick_sort_order(keys);
+    std::vector<uint32_t> next(N), src;
+    src.reserve(n);
+    for (uint64_t p = 0; p < N; p++) {
+        next[p] = s->ref_order[perm[p]];
+        if (pos_of_row[next[p]] != UINT32_MAX) src.push_back(pos_of_row[next[p]]);
+    }
+    s->ref_order.swap(next);
+    if (!n) return;
+    HIP_CHECK(hipMemcpyAsync(s->hd.vals[0], src.data(), 4ull * n, hipMemcpyHostToDevice, st));
+    upload_scene_table(s);
+    launch_permute_only(s->single, 0u, st);
+    HIP_CHECK(hipStreamSynchronize(st));
+}
+
 void search2d_set(fsa_context ctx, const LaunchSet& L, uint32_t epasses, uint32_t cpasses) {
     PhaseTimer t(ctx, FSA_TIMER_SEARCH);
     launch2_search(L, epasses, cpasses, ctx->stream);
 }
 
 void do_find_neighbors(fsa_scene_s* s) {
+    hashed_resort_if_due(s);
     prepare_grid(s);
     upload_scene_table(s);
     if (s->hd.dim == 2) {
         search2d_set(s->ctx, s->single, entry_passes(s->hd), cell_passes(s->hd));
         s->hd.search_count++;
+    } else if (s->hd.flags & kFlagUnsorted) {
+        search3u_set(s->ctx, s->single, unsorted_passes(s->hd));
     } else {
         search_set(s->ctx, s->single, sort_passes(s->hd));
     }
@@ -712,11 +836,14 @@ void drop_batches_of(fsa_context ctx, fsa_scene_s* s) {
 
 void upload_batch_table(fsa_context ctx, BatchSet* b) {
     uint32_t spec = kSpecHOne | kSpecRho0SqOne;
+    bool codec = false;
     for (uint32_t q = 0; q < b->scenes.size(); q++) {
         b->h_table[q] = b->scenes[q]->hd;
         spec &= spec_of(b->scenes[q]->hd);
+        codec = codec || (b->scenes[q]->hd.flags & kFlagCodec);
     }
     b->ls.spec = spec;
+    b->ls.codec = codec;
     HIP_CHECK(hipMemcpyAsync(b->d_table, b->h_table.data(), sizeof(SceneDev) * b->h_table.size(),
                              hipMemcpyHostToDevice, ctx->stream));
 }
@@ -856,11 +983,13 @@ void autotune(fsa_context ctx, BatchSet* b, const std::vector<fsa_scene_s*>& v) 
     ctx->tuned[cls] = t;
 }
 
-void step_head(fsa_context ctx, const LaunchSet& L, bool d2, uint32_t passes, uint32_t ep, uint32_t cp, bool sesph,
-               bool bpass, uint32_t chunk) {
+void step_head(fsa_context ctx, const LaunchSet& L, bool d2, bool unsorted, uint32_t passes, uint32_t ep,
+               uint32_t cp, bool sesph, bool bpass, uint32_t chunk) {
     hipStream_t st = ctx->stream;
     if (d2)
         search2d_set(ctx, L, ep, cp);
+    else if (unsorted)
+        search3u_set(ctx, L, passes);
     else
         search_set(ctx, L, passes);
     if (sesph) {
@@ -913,14 +1042,18 @@ int32_t batch_step_group(fsa_context ctx, const std::vector<fsa_scene_s*>& v, st
     BatchSet* b = batch_for(ctx, v);
     hipStream_t st = ctx->stream;
     const bool d2 = v[0]->hd.dim == 2;
+    const bool unsorted = (v[0]->hd.flags & kFlagUnsorted) != 0;
     const bool sesph = v[0]->desc.solver == FSA_SOLVER_SESPH;
     uint32_t passes = 1, epasses = 0, cpasses = 1, max_it = 0, guess = 0;
     bool bpass = false;
     for (auto* s : v) {
+        hashed_resort_if_due(s);
         prepare_grid(s);
         if (d2) {
             epasses = std::max(epasses, entry_passes(s->hd));
             cpasses = std::max(cpasses, cell_passes(s->hd));
+        } else if (unsorted) {
+            passes = std::max(passes, unsorted_passes(s->hd));
         } else {
             passes = std::max(passes, sort_passes(s->hd));
         }
@@ -933,7 +1066,7 @@ int32_t batch_step_group(fsa_context ctx, const std::vector<fsa_scene_s*>& v, st
     upload_batch_table(ctx, b);
     if (ctx->graph && !ctx->timing) {
         const std::array<uint32_t, 5> key = {
-            d2 ? (epasses << 8 | cpasses) : passes, chunk, b->ls.spec, (bpass ? 1u : 0u) | (sesph ? 2u : 0u),
+            d2 ? (epasses << 8 | cpasses) : passes, chunk, b->ls.spec, (bpass ? 1u : 0u) | (sesph ? 2u : 0u) | (unsorted ? 4u : 0u),
             (uint32_t)b->ls.unroll[0] | (uint32_t)b->ls.unroll[1] << 8 | (uint32_t)b->ls.unroll[2] << 16 |
                 (uint32_t)b->ls.unroll[3] << 24};
         auto it = b->graphs.find(key);
@@ -941,7 +1074,7 @@ int32_t batch_step_group(fsa_context ctx, const std::vector<fsa_scene_s*>& v, st
             hipGraph_t g = nullptr;
             hipGraphExec_t exec = nullptr;
             HIP_CHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-            step_head(ctx, b->ls, d2, passes, epasses, cpasses, sesph, bpass, chunk);
+            step_head(ctx, b->ls, d2, unsorted, passes, epasses, cpasses, sesph, bpass, chunk);
             HIP_CHECK(hipStreamEndCapture(st, &g));
             const hipError_t e = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
             (void)hipGraphDestroy(g);
@@ -950,7 +1083,7 @@ int32_t batch_step_group(fsa_context ctx, const std::vector<fsa_scene_s*>& v, st
         }
         HIP_CHECK(hipGraphLaunch(it->second, st));
     } else {
-        step_head(ctx, b->ls, d2, passes, epasses, cpasses, sesph, bpass, chunk);
+        step_head(ctx, b->ls, d2, unsorted, passes, epasses, cpasses, sesph, bpass, chunk);
     }
     if (d2)
         for (auto* s : v) s->hd.search_count++;
@@ -1154,6 +1287,34 @@ void do_upload(fsa_scene_s* s, const fsa_particle_view* v, bool first) {
     }
     if (!first && (N != s->total || nf != s->nf || nb != s->nb))
         throw FsaError(FSA_E_INVALID, "upload must keep the particle count and types of the scene");
+    // Device order of the active particles: the caller's order on the first upload;
+    // afterwards the order the reference collection holds by now (its searches
+    // re-sort the collection in place), so later sorts start from the same order.
+    const bool hashed = search_of(&s->desc) == FSA_SEARCH_HASHED_3D;
+    if (first && hashed) {
+        s->ref_order.resize(N);
+        std::iota(s->ref_order.begin(), s->ref_order.end(), 0u);
+        s->hashed_calls = 0;
+    }
+    if (!first) {
+        std::vector<uint32_t> order;
+        if (hashed && s->ref_order.size() == N) {
+            for (uint32_t r : s->ref_order)
+                if (((const uint8_t*)v->info + (uint64_t)r * v->info_stride)[4] != FSA_TYPE_INACTIVE) order.push_back(r);
+        } else if (!(s->hd.flags & kFlagUnsorted) && s->n) {
+            read_state(s);
+            std::vector<uint32_t> idt(s->n);
+            HIP_CHECK(hipMemcpyAsync(idt.data(), s->hd.idt[s->h_state->cur], 4ull * s->n, hipMemcpyDeviceToHost,
+                                     s->ctx->stream));
+            HIP_CHECK(hipStreamSynchronize(s->ctx->stream));
+            for (uint32_t w : idt) order.push_back(w & kIdMask);
+        }
+        if (order.size() == active.size()) {
+            std::vector<uint32_t> a = active, b = order;
+            std::sort(b.begin(), b.end());
+            if (a == b) active.swap(order);
+        }
+    }
     s->total = N;
     s->n = (uint32_t)active.size();
     s->nf = nf;
@@ -1213,6 +1374,19 @@ void do_upload(fsa_scene_s* s, const fsa_particle_view* v, bool first) {
     if (any_ext) h.flags |= kFlagExternal;
     if (dyn_boundary) h.flags |= kFlagDynamicBoundary;
     hipStream_t st = s->ctx->stream;
+    if ((h.flags & kFlagUnsorted) && n) {
+        // the collection keeps its order between searches: fluid ranks are fixed here
+        std::vector<uint32_t> fidx, rank(n, 0);
+        for (uint32_t k = 0; k < n; k++)
+            if ((idt[k] >> kTypeShift) == 0u) {
+                rank[k] = (uint32_t)fidx.size();
+                fidx.push_back(k);
+            }
+        if (!fidx.empty())
+            HIP_CHECK(hipMemcpyAsync(h.fluid_idx, fidx.data(), 4ull * fidx.size(), hipMemcpyHostToDevice, st));
+        HIP_CHECK(hipMemcpyAsync(h.rank_of, rank.data(), 4ull * n, hipMemcpyHostToDevice, st));
+        HIP_CHECK(hipStreamSynchronize(st));
+    }
     SceneState init{};
     init.cur = 0;
     init.dt_actual = s->desc.timestep;
@@ -1588,7 +1762,7 @@ FSA_API void fsa_scene_desc_defaults(fsa_scene_desc* d, uint32_t dimension, uint
     std::memset(d, 0, sizeof(*d));
     d->dimension = dimension;
     d->solver = solver;
-    d->neighbor_lists = FSA_LIST_PLAIN;
+    d->neighbor_lists = FSA_LIST_COMPRESSED;
     d->flags = dimension == 2 ? FSA_FLAG_EMULATE_INT_ABS : 0u;
     d->rest_density = 1.0f;
     d->gravity = 9.81f;
@@ -1679,7 +1853,10 @@ FSA_API int32_t fsa_scene_memory_estimate(const fsa_scene_desc* desc, uint64_t n
     return guarded([&] {
         validate_desc(desc);
         if (!bytes) throw FsaError(FSA_E_INVALID, "bytes is NULL");
-        *bytes = plan_bytes(make_plan(n_fluid + n_boundary, n_fluid, desc->dimension)) + sizeof(SceneDev);
+        const uint32_t search = search_of(desc);
+        *bytes = plan_bytes(make_plan(n_fluid + n_boundary, n_fluid, desc->dimension,
+                                      search == FSA_SEARCH_HASHED_3D || search == FSA_SEARCH_QUADRATIC_3D)) +
+                 sizeof(SceneDev);
     });
 }
 
@@ -1700,7 +1877,9 @@ FSA_API int32_t fsa_scene_create(fsa_context c, const fsa_scene_desc* desc, cons
             if (t == FSA_TYPE_BOUNDARY) nb++;
         }
         const uint64_t ndev = desc->dimension == 2 ? view->count : nf + nb;
-        const Plan p = make_plan(ndev, nf, desc->dimension);
+        const uint32_t search = search_of(desc);
+        const Plan p = make_plan(ndev, nf, desc->dimension,
+                                 search == FSA_SEARCH_HASHED_3D || search == FSA_SEARCH_QUADRATIC_3D);
         if (p.n >= (1ull << 27)) throw FsaError(FSA_E_INVALID, "scene too large for 27-bit run starts");
         const uint64_t bytes = plan_bytes(p);
         if (c->budget && c->in_use.load() + bytes > c->budget)
@@ -1821,35 +2000,14 @@ FSA_API int32_t fsa_scene_neighbor_stats(fsa_scene s, uint64_t* total, uint64_t*
     return guarded([&] {
         if (!s) throw FsaError(FSA_E_INVALID, "scene is NULL");
         if (!s->lists_valid) throw FsaError(FSA_E_STATE, "no neighbor lists: call find_neighbors first");
-        const uint32_t nf = s->nf;
-        std::vector<uint32_t> cnt(nf), nbr((size_t)kMaxNeighbors * s->hd.list_stride);
-        hipStream_t st = s->ctx->stream;
-        if (nf) {
-            HIP_CHECK(hipMemcpyAsync(cnt.data(), s->hd.nbr_cnt, 4ull * nf, hipMemcpyDeviceToHost, st));
-            HIP_CHECK(hipMemcpyAsync(nbr.data(), s->hd.nbr, 4ull * nbr.size(), hipMemcpyDeviceToHost, st));
-            HIP_CHECK(hipStreamSynchronize(st));
-        }
-        uint64_t t = 0, rb = 0;
-        for (uint32_t k = 0; k < nf; k++) {
-            t += cnt[k];
-            if (!cnt[k]) continue;
-            // NeighborStorage size model: 4 B first index + 2-bit controls + 0/1/4 B payload per delta
-            uint64_t payload = 0;
-            uint32_t prev = nbr[k] & kIndexMask;
-            for (uint32_t e = 1; e < cnt[k]; e++) {
-                const uint32_t j = nbr[(size_t)e * s->hd.list_stride + k] & kIndexMask;
-                const uint32_t d = j - prev;
-                prev = j;
-                if (d > 258)
-                    payload += 4;
-                else if (d > 2)
-                    payload += 1;
-            }
-            rb += 4 + (2ull * (cnt[k] - 1) + 7) / 8 + payload;
-        }
-        if (total) *total = t;
-        if (dev_bytes) *dev_bytes = 4 * t;
-        if (ref_bytes) *ref_bytes = rb;
+        // the NeighborStorage sizes of the current lists, counted on the device
+        upload_scene_table(s);
+        LaunchSet L = s->single;
+        launch_codec(L, s->ctx->stream);
+        read_state(s);
+        if (total) *total = s->h_state->total_neighbors;
+        if (dev_bytes) *dev_bytes = 4 * s->h_state->total_neighbors;
+        if (ref_bytes) *ref_bytes = s->h_state->codec_bytes;
     });
 }
 
@@ -1881,11 +2039,12 @@ FSA_API int32_t fsa_batch_step(fsa_context c, const fsa_scene* scenes, uint32_t 
         std::vector<fsa_scene_s*> v(scenes, scenes + n);
         for (auto* s : v)
             if (!s || s->ctx != c) throw FsaError(FSA_E_INVALID, "scene of another context");
-        // scenes of one solver and dimension share launches
-        std::vector<fsa_scene_s*> groups[4];
-        std::vector<fsa_step_stats*> gstats[4];
+        // scenes of one solver, dimension and search family share launches
+        std::vector<fsa_scene_s*> groups[8];
+        std::vector<fsa_step_stats*> gstats[8];
         for (uint32_t q = 0; q < n; q++) {
-            const int g = (v[q]->desc.solver == FSA_SOLVER_SESPH ? 1 : 0) + (v[q]->desc.dimension == 2 ? 2 : 0);
+            const int g = (v[q]->desc.solver == FSA_SOLVER_SESPH ? 1 : 0) + (v[q]->desc.dimension == 2 ? 2 : 0) +
+                          ((v[q]->hd.flags & kFlagUnsorted) ? 4 : 0);
             if (std::find(groups[g].begin(), groups[g].end(), v[q]) != groups[g].end())
                 throw FsaError(FSA_E_INVALID, "a scene appears twice in the batch");
             groups[g].push_back(v[q]);
@@ -1894,7 +2053,7 @@ FSA_API int32_t fsa_batch_step(fsa_context c, const fsa_scene* scenes, uint32_t 
         HIP_CHECK(hipSetDevice(c->device));
         int32_t first_error = FSA_OK;
         for (uint32_t k = 0; k < steps && first_error == FSA_OK; k++)
-            for (int g = 0; g < 4; g++)
+            for (int g = 0; g < 8; g++)
                 if (!groups[g].empty()) {
                     const int32_t rc = batch_step_group(c, groups[g], gstats[g]);
                     if (first_error == FSA_OK) first_error = rc;

@@ -41,6 +41,8 @@ constexpr uint32_t kFlagSesph = 32u;        // SESPH solver (else IISPH)
 constexpr uint32_t kFlagPlainLists = 64u;   // uncompressed 32-bit neighbor lists
 constexpr uint32_t kFlagIntAbs = 128u;      // 2D IISPH: integer abs() in the loop condition
 constexpr uint32_t kFlagQuadratic = 256u;   // 2D: QuadraticNeighborhoodSearchDynamicAllocated order
+constexpr uint32_t kFlagCodec = 512u;       // enforce the NeighborStorage codec capacity each search
+constexpr uint32_t kFlagUnsorted = 1024u;   // 3D hashed / quadratic searches: the collection keeps its order
 
 // 2D scenes (IISPHFluidSolver / SESPHFluidSolver with HashedNeighborhoodSearch or
 // QuadraticNeighborhoodSearchDynamicAllocated) keep every particle, inactive
@@ -67,6 +69,7 @@ struct alignas(16) SceneState {
     int32_t bbox_min[4], bbox_max[4];  // cell bounding box after the last integration
     uint32_t pad0, pad1;
     uint64_t total_neighbors;
+    uint64_t codec_bytes;  // NeighborStorage encoded bytes of all lists of the last search
 };
 
 struct alignas(16) Partial {
@@ -111,6 +114,7 @@ struct alignas(16) SceneDev {
     uint32_t pad1;
     int2* cell_prev;       // 2D: cell of the last grid update
     uint32_t* entry;       // 2D: search count at which the particle entered its cell
+    int4* cell_last;       // 3D hashed / quadratic: cell of the last grid update (GridCellState.current)
 
     // cell grid of this step (dense, linear x-fastest over the cell bounding box)
     int32_t cmin[4];

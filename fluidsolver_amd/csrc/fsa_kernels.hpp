@@ -21,6 +21,7 @@ struct LaunchSet {
     uint32_t spec = 0;  // kSpec* bits shared by every scene of the set
     // neighbor-walk unroll per IISPH kernel family (kTune*), chosen by the autotuner
     uint8_t unroll[4] = {4, 4, 4, 4};
+    bool codec = false;  // some scene of the set checks the NeighborStorage codec each search
 };
 
 // kernel families the autotuner times
@@ -35,6 +36,13 @@ void launch_keys(const LaunchSet& L, hipStream_t st);
 void launch_sort(const LaunchSet& L, uint32_t passes, hipStream_t st);
 void launch_reorder(const LaunchSet& L, uint32_t sbuf, hipStream_t st);
 void launch_lists(const LaunchSet& L, uint32_t sbuf, hipStream_t st);
+// hashed / quadratic 3D searches over the kept collection order
+void launch3u_search(const LaunchSet& L, uint32_t passes, hipStream_t st);
+// gather every particle array into the order vals[sbuf] gives, flip the buffers,
+// rebuild the fluid rank maps
+void launch_permute_only(const LaunchSet& L, uint32_t sbuf, hipStream_t st);
+// NeighborStorage sizes and capacity of the lists just built
+void launch_codec(const LaunchSet& L, hipStream_t st);
 // stable LSD radix passes over (keys, vals)[first_src], digits from bit shift0 upward
 void launch_sort_range(const LaunchSet& L, uint32_t first_src, uint32_t shift0, uint32_t passes, hipStream_t st);
 

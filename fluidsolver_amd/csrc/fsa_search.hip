@@ -454,6 +454,161 @@ __global__ __launch_bounds__(kBlock) void k_lists(const SceneDev* __restrict__ s
     if (__any(overflow) && lane == 0) atomicOr(&S.state->status, kStatusCapacity);
 }
 
+// ---- 8. hashed / quadratic 3D searches (collection order kept) -------------------
+// HashedNeighborhoodSearch3D (HashedNeighborhoodSearch3D.cpp:69-217) and
+// QuadraticNeighborhoodSearch3D (QuadraticNeighborhoodSearch3D.cpp:32-69) do
+// not sort the collection each step.  Particles are bucketed by a stable sort
+// on their dense cell index, so each cell lists its members ascending (the
+// hashed grid's std::set); a list walks the 27 cells dx-outer, dz-inner
+// (hashed), or is kept ascending overall by insertion (quadratic).  Neighbor
+// test dot(x_i - x_j, x_i - x_j) <= r^2.
+__global__ __launch_bounds__(kBlock) void k3u_keys(const SceneDev* __restrict__ scenes,
+                                                   const BlockRef* __restrict__ map) {
+    const BlockRef br = map[blockIdx.x];
+    const SceneDev& S = scenes[br.scene];
+    const uint32_t i = br.block * kBlock + threadIdx.x;
+    if (i >= S.n) return;
+    const uint32_t c = S.state->cur;
+    const float4 p = S.pos[c][i];
+    const int32_t cx = cell_of(p.x, S.search_radius), cy = cell_of(p.y, S.search_radius),
+                  cz = cell_of(p.z, S.search_radius);
+    S.cell_last[i] = make_int4(cx, cy, cz, 0);
+    uint32_t lin = S.cdim[3];
+    if (!cell_linear(S, cx, cy, cz, lin)) atomicOr(&S.state->status, kStatusGrid);
+    S.keys[0][i] = lin;
+    S.vals[0][i] = i;
+}
+
+__global__ __launch_bounds__(kBlock) void k3u_cell_table(const SceneDev* __restrict__ scenes,
+                                                         const BlockRef* __restrict__ map, uint32_t buf) {
+    const BlockRef br = map[blockIdx.x];
+    const SceneDev& S = scenes[br.scene];
+    const uint32_t k = br.block * kBlock + threadIdx.x;
+    if (k >= S.n) return;
+    const uint32_t* __restrict__ keys = S.keys[buf];
+    const uint32_t key = keys[k];
+    if (key >= S.cdim[3]) return;
+    const bool first = k == 0 || keys[k - 1] != key;
+    const bool last = k + 1 == S.n || keys[k + 1] != key;
+    if (first) {
+        S.cells[key].x = k;
+        S.cells[key].z = S.stamp;
+    }
+    if (last) S.cells[key].y = k + 1;
+}
+
+__global__ __launch_bounds__(kBlock) void k3u_lists(const SceneDev* __restrict__ scenes,
+                                                    const BlockRef* __restrict__ map, uint32_t buf) {
+    const BlockRef br = map[blockIdx.x];
+    const SceneDev& S = scenes[br.scene];
+    const uint32_t k = br.block * kBlock + threadIdx.x;
+    if (k >= S.nf) return;
+    const uint32_t c = S.state->cur;
+    const auto pos = gload_ptr(S.pos[c]);
+    const auto idt = gload_ptr(S.idt[c]);
+    const uint32_t i = S.fluid_idx[k];
+    const float4 xi = pos[i];
+    const int4 ci = S.cell_last[i];
+    const uint32_t* __restrict__ vals = S.vals[buf];
+    uint32_t* __restrict__ out = S.nbr + k;
+    const uint32_t stride = S.list_stride;
+    const bool quad = (S.flags & kFlagQuadratic) != 0;
+    const float r2 = S.r2;
+    uint32_t cnt = 0;
+    for (int dx = -1; dx <= 1; dx++)
+        for (int dy = -1; dy <= 1; dy++)
+            for (int dz = -1; dz <= 1; dz++) {
+                uint32_t lin;
+                if (!cell_linear(S, ci.x + dx, ci.y + dy, ci.z + dz, lin)) continue;
+                const uint4 e = S.cells[lin];
+                if (e.z != S.stamp) continue;
+                for (uint32_t m = e.x; m < e.y; m++) {
+                    const uint32_t j = vals[m];
+                    const float4 xj = pos[j];
+                    const float ddx = xi.x - xj.x, ddy = xi.y - xj.y, ddz = xi.z - xj.z;
+                    if (!(dot3(ddx, ddy, ddz, ddx, ddy, ddz) <= r2)) continue;
+                    const uint32_t ent = j | ((idt[j] >> kTypeShift) == 1u ? kBoundaryBit : 0u);
+                    if (quad) {
+                        // ascending index: insert before the first larger entry
+                        uint32_t q = min(cnt, (uint32_t)kMaxNeighbors);
+                        while (q > 0) {
+                            const uint32_t prev = out[(q - 1) * stride];
+                            if ((prev & kIndexMask) < j) break;
+                            if (q < (uint32_t)kMaxNeighbors) out[q * stride] = prev;
+                            q--;
+                        }
+                        if (q < (uint32_t)kMaxNeighbors) out[q * stride] = ent;
+                    } else if (cnt < (uint32_t)kMaxNeighbors) {
+                        out[cnt * stride] = ent;
+                    }
+                    cnt++;
+                }
+            }
+    if (cnt > (uint32_t)kMaxNeighbors) atomicOr(&S.state->status, kStatusCapacity);
+    S.nbr_cnt[k] = min(cnt, (uint32_t)kMaxNeighbors);
+}
+
+// ---- 7. NeighborStorage codec (CompressedNeighbors.cpp:260-330) -----------------
+// Size of each list in the reference's delta code: the first index, then per
+// delta a 2-bit control and 0 (delta 1, 2), 1 (delta <= 258) or 4 payload
+// bytes.  A list needs more than 96 payload bytes or more than 96 deltas
+// where the reference's fixed NeighborStorage overflows; the scene then
+// reports the capacity error.  Block totals of neighbors and encoded bytes go
+// to the scene state.
+__global__ __launch_bounds__(kBlock) void k_codec(const SceneDev* __restrict__ scenes,
+                                                  const BlockRef* __restrict__ map) {
+    const BlockRef br = map[blockIdx.x];
+    const SceneDev& S = scenes[br.scene];
+    __shared__ uint64_t wsum[2][kBlock / 64];
+    const uint32_t k = br.block * kBlock + threadIdx.x;
+    uint64_t cnt = 0, bytes = 0;
+    bool over = false;
+    if (k < S.nf) {
+        const uint32_t n = S.nbr_cnt[k];
+        cnt = n;
+        if (n) {
+            const auto nb = gload_ptr(S.nbr + k);
+            uint32_t prev = nb[0] & kIndexMask, payload = 0;
+            for (uint32_t e = 1; e < n; e++) {
+                const uint32_t j = nb[e * S.list_stride] & kIndexMask;
+                const uint32_t d = j - prev;
+                prev = j;
+                payload += d <= 2u ? 0u : (d <= 258u ? 1u : 4u);
+            }
+            over = payload > 96u || n - 1u > 96u;
+            bytes = 4u + (2u * (n - 1u) + 7u) / 8u + payload;
+        }
+    }
+    if ((S.flags & kFlagCodec) && __any(over) && __lane_id() == 0) atomicOr(&S.state->status, kStatusCapacity);
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        cnt += __shfl_xor(cnt, o, 64);
+        bytes += __shfl_xor(bytes, o, 64);
+    }
+    if (__lane_id() == 0) {
+        wsum[0][threadIdx.x / 64] = cnt;
+        wsum[1][threadIdx.x / 64] = bytes;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint64_t c = 0, b = 0;
+        for (int w = 0; w < kBlock / 64; w++) {
+            c += wsum[0][w];
+            b += wsum[1][w];
+        }
+        atomicAdd((unsigned long long*)&S.state->total_neighbors, (unsigned long long)c);
+        atomicAdd((unsigned long long*)&S.state->codec_bytes, (unsigned long long)b);
+    }
+}
+
+__global__ void k_codec_reset(const SceneDev* __restrict__ scenes, const BlockRef* __restrict__ map) {
+    const SceneDev& S = scenes[map[blockIdx.x].scene];
+    if (threadIdx.x == 0) {
+        S.state->total_neighbors = 0;
+        S.state->codec_bytes = 0;
+    }
+}
+
 // ---- launchers -------------------------------------------------------------------
 void launch_keys(const LaunchSet& L, hipStream_t st) {
     if (L.nb_all) k_keys<<<L.nb_all, kBlock, 0, st>>>(L.scenes, L.map_all);
@@ -488,6 +643,26 @@ void launch_reorder(const LaunchSet& L, uint32_t sbuf, hipStream_t st) {
 
 void launch_lists(const LaunchSet& L, uint32_t sbuf, hipStream_t st) {
     if (L.nb_all) k_lists<<<L.nb_all, kBlock, 0, st>>>(L.scenes, L.map_all, sbuf);
+    if (L.codec) launch_codec(L, st);
+}
+
+void launch3u_search(const LaunchSet& L, uint32_t passes, hipStream_t st) {
+    k3u_keys<<<L.nb_all, kBlock, 0, st>>>(L.scenes, L.map_all);
+    launch_sort_range(L, 0u, 0u, passes, st);
+    const uint32_t buf = passes & 1u;
+    k3u_cell_table<<<L.nb_all, kBlock, 0, st>>>(L.scenes, L.map_all, buf);
+    if (L.nb_fluid) k3u_lists<<<L.nb_fluid, kBlock, 0, st>>>(L.scenes, L.map_fluid, buf);
+}
+
+void launch_permute_only(const LaunchSet& L, uint32_t sbuf, hipStream_t st) {
+    k_permute<<<L.nb_all, kBlock, 0, st>>>(L.scenes, L.map_all, sbuf);
+    k_fluid_scan<<<L.nb_scene, kBlock, 0, st>>>(L.scenes, L.map_scene);
+    k_compact<<<L.nb_all, kBlock, 0, st>>>(L.scenes, L.map_all);
+}
+
+void launch_codec(const LaunchSet& L, hipStream_t st) {
+    k_codec_reset<<<L.nb_scene, 64, 0, st>>>(L.scenes, L.map_scene);
+    if (L.nb_fluid) k_codec<<<L.nb_fluid, kBlock, 0, st>>>(L.scenes, L.map_fluid);
 }
 
 }  // namespace fsa

@@ -61,13 +61,20 @@ extern "C" {
 #define FSA_SOLVER_IISPH 0   /* IISPHFluidSolver3D / IISPHFluidSolver            */
 #define FSA_SOLVER_SESPH 1   /* SESPHFluidSolver3D / SESPHFluidSolver            */
 
-#define FSA_LIST_COMPRESSED 0 /* delta-coded neighbor lists (default)            */
-#define FSA_LIST_PLAIN 1      /* uncompressed 32-bit neighbor lists               */
+/* fsa_scene_desc.neighbor_lists (3D).  The device walks 32-bit neighbor lists
+ * in either case; COMPRESSED also sizes every list in the compressed search's
+ * NeighborStorage delta code each search (CompressedNeighbors.cpp:260-330) and
+ * fails the step with FSA_E_CAPACITY where that fixed-size storage would
+ * overflow (more than 96 deltas or 96 payload bytes), as the reference does. */
+#define FSA_LIST_COMPRESSED 0 /* NeighborStorage capacity and sizes (default)    */
+#define FSA_LIST_PLAIN 1      /* 97-neighbor capacity only                       */
 
 /* fsa_scene_desc.search: the reference search whose neighbor order the device
  * reproduces (registration strings of SolverSerializer.cpp:137-219). */
 #define FSA_SEARCH_DEFAULT 0       /* compressed-3d in 3D, hashed in 2D               */
 #define FSA_SEARCH_COMPRESSED_3D 1 /* "compressed-3d": CompressedNeighborhoodSearch  */
+#define FSA_SEARCH_HASHED_3D 2     /* "hashed-3d": HashedNeighborhoodSearch3D         */
+#define FSA_SEARCH_QUADRATIC_3D 3  /* "quadratic-dynamic-allocated-3d" (brute force)  */
 #define FSA_SEARCH_HASHED_2D 4     /* "hashed": HashedNeighborhoodSearch (2D)         */
 #define FSA_SEARCH_QUADRATIC_2D 5  /* "quadratic-dynamic-allocated" (2D)             */
 

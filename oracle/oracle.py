@@ -18,6 +18,7 @@ SEARCH_COMPRESSED3D = 0
 SEARCH_QUADRATIC3D = 1
 SEARCH_HASHED2D = 2
 SEARCH_QUADRATIC2D = 3
+SEARCH_HASHED3D = 4
 
 MODE_FP32 = 0
 MODE_FP64 = 1
@@ -50,6 +51,11 @@ def lib():
         "orc_download_iisph": (i32, [vp, vp, u64]),
         "orc_export_neighbors": (i32, [vp, vp, vp, u64, C.POINTER(u64)]),
         "orc_codec_stats": (None, [vp, C.POINTER(u64), C.POINTER(u64)]),
+        "orc_codec_sizes": (None, [vp, vp, vp]),
+        "orc_quick_sort_u64": (None, [vp, u64, vp]),
+        "orc_internal_ids": (None, [vp, vp]),
+        "orc_upload": (C.c_int, [vp, C.POINTER(ParticleView)]),
+        "orc_export_raw": (C.c_int, [vp, vp, vp, u64, C.POINTER(u64)]),
         "orc_kernel_value": (f32, [i32, C.POINTER(f32), f32]),
         "orc_kernel_values": (None, [i32, u64, vp, f32, vp]),
         "orc_kernel_derivatives": (None, [i32, u64, vp, f32, vp]),
@@ -98,7 +104,7 @@ def _check(rc):
 class OracleScene:
     """One oracle scene: ParticleCollection + solver + neighborhood search.
 
-    search: SEARCH_* (compressed-3d / quadratic-3d / hashed-2d / quadratic-dynamic-allocated-2d)
+    search: SEARCH_* (compressed-3d / quadratic-3d / hashed-2d / quadratic-dynamic-allocated-2d / hashed-3d)
     mode:   MODE_FP32 = reference FP32 order and FP32 serial convergence sums,
             MODE_FP64 = FP64 accuracy reference,
             MODE_FP32_R64 = FP32 reference order with FP64 convergence sums
@@ -165,6 +171,33 @@ class OracleScene:
         d = (C.c_float * 3)(*[float(v) for v in list(delta) + [0.0] * (3 - len(delta))])
         lib().orc_translate(self._h, d)
 
+    def upload(self, collection):
+        """Rewrite every particle's data in place (the collection keeps its current order)."""
+        v = make_view(collection)
+        _check(lib().orc_upload(self._h, C.byref(v)))
+
+    def export_raw(self):
+        """Lists as stored: rows and columns are positions of the current order, entries in list order."""
+        off = np.zeros(self.n + 1, dtype=np.uint64)
+        nnz = C.c_uint64(0)
+        lib().orc_export_raw(self._h, off.ctypes.data, None, 0, C.byref(nnz))
+        cols = np.zeros(max(1, nnz.value), dtype=np.uint64)
+        _check(lib().orc_export_raw(self._h, off.ctypes.data, cols.ctypes.data, nnz.value, C.byref(nnz)))
+        return off, cols[: nnz.value]
+
+    def internal_ids(self):
+        """Caller row at each position of the collection's current (sorted) order."""
+        out = np.zeros(self.n, dtype=np.uint64)
+        lib().orc_internal_ids(self._h, out.ctypes.data)
+        return out
+
+    def codec_sizes(self):
+        """(neighbor count, NeighborStorage encoded bytes) of every caller row."""
+        cnt = np.zeros(self.n, dtype=np.uint64)
+        byt = np.zeros(self.n, dtype=np.uint64)
+        lib().orc_codec_sizes(self._h, cnt.ctypes.data, byt.ctypes.data)
+        return cnt, byt
+
     def codec_stats(self):
         t = C.c_uint64(0)
         b = C.c_uint64(0)
@@ -205,6 +238,14 @@ def morton3(x, y, z):
 
 def cell_key(x, y, z):
     return int(lib().orc_cell_key(x, y, z))
+
+
+def quick_sort(keys):
+    """Permutation of Sort::quick_sort (unstable Hoare, middle pivot): new position k holds old index perm[k]."""
+    k = np.ascontiguousarray(keys, dtype=np.uint64)
+    perm = np.zeros(len(k), dtype=np.uint64)
+    lib().orc_quick_sort_u64(k.ctypes.data, len(k), perm.ctypes.data)
+    return perm
 
 
 def stable_sort(keys):

@@ -302,7 +302,48 @@ struct NeighborStorage {
     }
 };
 
-enum SearchKind { SEARCH_COMPRESSED3D = 0, SEARCH_QUADRATIC3D = 1, SEARCH_HASHED2D = 2, SEARCH_QUADRATIC2D = 3 };
+enum SearchKind {
+    SEARCH_COMPRESSED3D = 0,
+    SEARCH_QUADRATIC3D = 1,
+    SEARCH_HASHED2D = 2,
+    SEARCH_QUADRATIC2D = 3,
+    SEARCH_HASHED3D = 4
+};
+
+// ---- Sort::quick_sort (ParticleCollectionAlgorithm.hpp:28-127) ----------------
+// Hoare partition around the key of the middle element, recursing on [low, j]
+// and [j + 1, high]; the two halves are disjoint, so processing them in any
+// order (the reference splits large ones onto threads) gives one permutation.
+// perm[k] = old index now at position k.
+static std::vector<size_t> quick_sort_perm(const std::vector<uint64_t>& keys_in) {
+    const size_t n = keys_in.size();
+    std::vector<uint64_t> key = keys_in;
+    std::vector<size_t> perm(n);
+    std::iota(perm.begin(), perm.end(), (size_t)0);
+    if (n < 2) return perm;
+    std::vector<std::pair<int64_t, int64_t>> todo{{0, (int64_t)n - 1}};
+    while (!todo.empty()) {
+        const int64_t low = todo.back().first, high = todo.back().second;
+        todo.pop_back();
+        if (low >= high) continue;
+        const uint64_t pivot = key[low + (high - low) / 2];
+        int64_t i = low - 1, j = high + 1;
+        for (;;) {
+            do {
+                i++;
+            } while (key[i] < pivot);
+            do {
+                j--;
+            } while (key[j] > pivot);
+            if (i >= j) break;
+            std::swap(key[i], key[j]);
+            std::swap(perm[i], perm[j]);
+        }
+        todo.push_back({low, j});
+        todo.push_back({j + 1, high});
+    }
+    return perm;
+}
 enum { T_NORMAL = 0, T_BOUNDARY = 1, T_INACTIVE = 2 };
 
 struct SceneBase {
@@ -314,6 +355,10 @@ struct SceneBase {
     virtual int download_iisph(void* rec, uint64_t stride) = 0;
     virtual int export_neighbors(uint64_t* off, uint64_t* cols, uint64_t cap, uint64_t* nnz) = 0;
     virtual void codec_stats(uint64_t* total, uint64_t* bytes) = 0;
+    virtual void codec_sizes(uint64_t* count_per_row, uint64_t* bytes_per_row) = 0;
+    virtual void internal_ids(uint64_t* out) = 0;
+    virtual int export_raw(uint64_t* off, uint64_t* cols, uint64_t cap, uint64_t* nnz) = 0;
+    virtual int upload(const fsa_particle_view* v) = 0;
 };
 
 template <class T, int D>
@@ -336,6 +381,11 @@ struct Scene : SceneBase {
     std::vector<uint64_t> cell_key;               // compressed: ParticleInformation.cell_index
     std::vector<std::pair<uint64_t, size_t>> runs; // cell_to_particle_map
     std::vector<std::vector<uint64_t>> lists;     // quadratic / hashed lists
+    // hashed 3D grid state (HashedNeighborhoodSearch3D.cpp:69-127): GridCellState.current
+    // of every particle (INT_MIN until its first grid update) and the call counter of
+    // improve_cache_efficiency
+    std::vector<std::array<int32_t, 3>> grid_current;
+    size_t calls_since_sort = 0;
     // hashed 2D grid state (HashedNeighborhoodSearch.cpp:117-151)
     std::map<std::pair<int32_t, int32_t>, std::list<uint64_t>> grid;
     std::vector<std::pair<int32_t, int32_t>> current_status;
@@ -411,6 +461,7 @@ struct Scene : SceneBase {
     // apply permutation (new position k takes old index perm[k]) to every component
     template <class A>
     static void permute(std::vector<A>& a, const std::vector<size_t>& perm) {
+        if (a.size() != perm.size()) return;  // component not in use by this search
         std::vector<A> b(a.size());
         for (size_t k = 0; k < perm.size(); k++) b[k] = a[perm[k]];
         a.swap(b);
@@ -433,6 +484,7 @@ struct Scene : SceneBase {
         permute(perr, perm);
         permute(cell_key, perm);
         permute(storage, perm);
+        permute(grid_current, perm);
     }
 
     // ---- CompressedNeighborhoodSearch::find_neighbors (CompressedNeighbors.cpp:168-240)
@@ -514,6 +566,49 @@ struct Scene : SceneBase {
         return FSA_OK;
     }
 
+    // ---- HashedNeighborhoodSearch3D::find_neighbors (HashedNeighborhoodSearch3D.cpp:69-217)
+    // Every 101st call the collection is re-sorted with the unstable quick sort
+    // by the Morton key of each particle's cell of the last grid update
+    // (improve_cache_efficiency, :219-247).  The grid then maps each cell to the
+    // ascending set of its non-inactive particles, and lists walk the 27 cells
+    // dx-outer, dz-inner.
+    int search_hashed3d() {
+        if (grid_current.size() != n) {
+            const int32_t lo = std::numeric_limits<int32_t>::min();
+            grid_current.assign(n, {lo, lo, lo});
+        }
+        calls_since_sort++;
+        if (calls_since_sort > 100) {
+            std::vector<uint64_t> keys(n);
+            for (size_t i = 0; i < n; i++) keys[i] = key_of(grid_current[i]);
+            permute_all(quick_sort_perm(keys));
+            calls_since_sort = 0;
+        }
+        std::map<std::array<int32_t, 3>, std::vector<uint64_t>> grid;
+        for (size_t i = 0; i < n; i++) {
+            if (type[i] == T_INACTIVE) continue;
+            grid_current[i] = cell_of(pos[i]);
+            grid[grid_current[i]].push_back(i);  // ascending, as std::set
+        }
+        lists.assign(n, {});
+        const T r2 = search_radius * search_radius;
+        loop_for(n, [&](size_t i) {
+            if (type[i] == T_INACTIVE) return;
+            const auto c = grid_current[i];
+            for (int dx = -1; dx <= 1; dx++)
+                for (int dy = -1; dy <= 1; dy++)
+                    for (int dz = -1; dz <= 1; dz++) {
+                        auto it = grid.find({c[0] + dx, c[1] + dy, c[2] + dz});
+                        if (it == grid.end()) continue;
+                        for (uint64_t j : it->second) {
+                            V d = pos[i] - pos[j];
+                            if (dot(d, d) <= r2) lists[i].push_back(j);
+                        }
+                    }
+        });
+        return FSA_OK;
+    }
+
     // ---- QuadraticNeighborhoodSearch3D::find_neighbors (QuadraticNeighborhoodSearch3D.cpp:32-69)
     int search_quadratic3d() {
         lists.assign(n, {});
@@ -585,6 +680,8 @@ struct Scene : SceneBase {
             rc = search_compressed();
         else if (search == SEARCH_QUADRATIC3D)
             rc = search_quadratic3d();
+        else if (search == SEARCH_HASHED3D)
+            rc = search_hashed3d();
         else if (search == SEARCH_HASHED2D)
             rc = search_hashed2d();
         else
@@ -1110,6 +1207,71 @@ struct Scene : SceneBase {
         }
         return FSA_OK;
     }
+    // NeighborStorage size and encoded bytes of each particle's list, by caller row
+    // (compressed search only; other searches report sizes and zero bytes)
+    void codec_sizes(uint64_t* count_per_row, uint64_t* bytes_per_row) override {
+        for (size_t k = 0; k < n; k++) {
+            const size_t row = id[k];
+            if (search == SEARCH_COMPRESSED3D) {
+                count_per_row[row] = storage[k].size();
+                bytes_per_row[row] = storage[k].encoded_bytes();
+            } else {
+                count_per_row[row] = k < lists.size() ? lists[k].size() : 0;
+                bytes_per_row[row] = 0;
+            }
+        }
+    }
+    // the caller rewrites particle data in place: the collection keeps the order
+    // its searches gave it (ParticleCollection rows are edited where they are)
+    int upload(const fsa_particle_view* v) override {
+        if (v->count != n) {
+            g_err = "upload view count mismatch";
+            return FSA_E_INVALID;
+        }
+        for (size_t k = 0; k < n; k++) {
+            const size_t i = id[k];
+            const float* mv = (const float*)((const char*)v->movement + i * v->movement_stride);
+            for (int c = 0; c < D; c++) {
+                pos[k].v[c] = (T)mv[c];
+                vel[k].v[c] = (T)mv[D + c];
+                acc[k].v[c] = (T)mv[2 * D + c];
+            }
+            const float* pd = (const float*)((const char*)v->particle_data + i * v->particle_data_stride);
+            mass[k] = pd[0];
+            pressure[k] = pd[1];
+            density[k] = pd[2];
+            const char* inf = (const char*)v->info + i * v->info_stride;
+            std::memcpy(&tag[k], inf, 4);
+            type[k] = (uint8_t)inf[4];
+            if (v->external) {
+                const float* e = (const float*)((const char*)v->external + i * v->external_stride);
+                for (int c = 0; c < D; c++) ext[k].v[c] = (T)e[c];
+            }
+        }
+        searched = false;
+        return FSA_OK;
+    }
+    // lists as the search stores them: rows and columns in the collection's
+    // current order, entries in list order
+    int export_raw(uint64_t* off, uint64_t* cols, uint64_t cap, uint64_t* nnz) override {
+        uint64_t total = 0;
+        for (size_t k = 0; k < n; k++) {
+            off[k] = total;
+            const bool has = search == SEARCH_COMPRESSED3D ? type[k] != T_INACTIVE : k < lists.size();
+            if (!has) continue;
+            for_neighbors(k, [&](size_t j) {
+                if (total < cap) cols[total] = j;
+                total++;
+            });
+        }
+        off[n] = total;
+        *nnz = total;
+        return total > cap ? FSA_E_INVALID : FSA_OK;
+    }
+    // caller row of each position of the collection's current order
+    void internal_ids(uint64_t* out) override {
+        for (size_t k = 0; k < n; k++) out[k] = id[k];
+    }
     void codec_stats(uint64_t* total, uint64_t* bytes) override {
         uint64_t t = 0, b = 0;
         if (search == SEARCH_COMPRESSED3D) {
@@ -1184,6 +1346,16 @@ void orc_codec_stats(void* s, uint64_t* total, uint64_t* bytes) { ((SceneBase*)s
 
 void orc_translate(void* s, const float* delta) { ((SceneBase*)s)->translate(delta); }
 
+void orc_internal_ids(void* s, uint64_t* out) { ((SceneBase*)s)->internal_ids(out); }
+int orc_upload(void* s, const fsa_particle_view* v) { return ((SceneBase*)s)->upload(v); }
+int orc_export_raw(void* s, uint64_t* off, uint64_t* cols, uint64_t cap, uint64_t* nnz) {
+    return ((SceneBase*)s)->export_raw(off, cols, cap, nnz);
+}
+
+void orc_codec_sizes(void* s, uint64_t* count_per_row, uint64_t* bytes_per_row) {
+    ((SceneBase*)s)->codec_sizes(count_per_row, bytes_per_row);
+}
+
 // ---- pure functions, exposed for pinning tests --------------------------
 void orc_kernel_values(int dim, uint64_t n, const float* xs, float support, float* out) {
     if (dim == 3) {
@@ -1240,6 +1412,12 @@ uint64_t orc_morton3(uint32_t x, uint32_t y, uint32_t z) { return morton3_64(x, 
 uint64_t orc_cell_key(int32_t x, int32_t y, int32_t z) {
     return morton3_64(to_unsigned(x), to_unsigned(y), to_unsigned(z));
 }
+void orc_quick_sort_u64(const uint64_t* keys, uint64_t n, uint64_t* perm) {
+    std::vector<uint64_t> k(keys, keys + n);
+    const std::vector<size_t> p = quick_sort_perm(k);
+    for (uint64_t i = 0; i < n; i++) perm[i] = p[i];
+}
+
 void orc_stable_sort_u64(const uint64_t* keys, uint64_t n, uint64_t* perm) {
     std::vector<uint64_t> p(n);
     std::iota(p.begin(), p.end(), 0);

@@ -264,3 +264,70 @@ def test_cpp_dropin_program(oracle_lib, tmp_path):
     orc = O.OracleScene(abi.default_desc(3, abi.SOLVER_IISPH), coll, mode=O.MODE_FP32_R64)
     orc.step(3)
     np.testing.assert_array_equal(mv, orc.download().movement)
+
+
+def test_codec_sizes_match_oracle(gpu_context, oracle_lib):
+    """NeighborStorage neighbor counts and encoded bytes of the fluid lists,
+    counted on the device, equal the oracle's compressed search."""
+    O = oracle_lib
+    coll = scenes.dam_scene((10, 9, 8), (30, 14, 12), 3, seed=41)
+    desc = abi.default_desc(3, abi.SOLVER_IISPH)
+    print("case codec sizes", coll.size(), flush=True)
+    sc = Scene(gpu_context, desc, coll.copy())
+    orc = O.OracleScene(desc, coll.copy())
+    for step in range(3):
+        sc.find_neighbors()
+        orc.find_neighbors()
+        st = sc.neighbor_stats()
+        cnt, byt = orc.codec_sizes()
+        fl = coll.type == abi.TYPE_NORMAL
+        assert st["total_neighbors"] == int(cnt[fl].sum()), step
+        assert st["reference_codec_bytes"] == int(byt[fl].sum()), step
+        sc.simulate()
+        orc.simulate()
+
+
+def _codec_probe(m):
+    """One fluid particle P whose m neighbors in the next cell are separated,
+    in the collection order that the stable sort keeps within a cell, by 260
+    boundary particles outside its radius: its list has m - 1 four-byte deltas."""
+    from fluidsolver_amd.collection import ParticleCollection
+
+    rows = [np.array([[1.0, 1.0, 1.0]], np.float32)]
+    far = scenes.lattice((13, 20, 25)).astype(np.float32) * np.float32([0.038, 0.1, 0.08]) + np.float32([3.5, 0.0, 0.0])
+    for q in range(m):
+        rows.append(np.array([[2.0 + 0.02 * q, 1.0, 1.0]], np.float32))
+        if q + 1 < m:
+            rows.append(far[260 * q : 260 * (q + 1)])
+    pos = np.concatenate(rows).astype(np.float32)
+    types = np.ones(len(pos), np.uint8)
+    types[0] = abi.TYPE_NORMAL
+    return ParticleCollection.from_arrays(3, pos, types)
+
+
+def test_codec_capacity_boundary(gpu_context, oracle_lib):
+    """The compressed search's fixed NeighborStorage holds 96 payload bytes:
+    24 four-byte deltas fit, 25 overflow (FSA_E_CAPACITY); plain lists take both."""
+    from fluidsolver_amd.solver import FsaError
+
+    O = oracle_lib
+    for m, fits in [(25, True), (26, False)]:
+        coll = _codec_probe(m)
+        print(f"case codec capacity m={m} n={coll.size()}", flush=True)
+        ref = O.Codec()
+        ref.set_first_neighbor(0)
+        ok = [ref.set_next_neighbor(1)] + [ref.set_next_neighbor(261) for _ in range(m - 1)]
+        assert all(ok) == fits
+        sc = Scene(gpu_context, abi.default_desc(3, abi.SOLVER_IISPH), coll.copy())
+        if fits:
+            sc.find_neighbors()
+            assert sc.neighbor_stats()["reference_codec_bytes"] == ref.encoded_bytes()
+        else:
+            with pytest.raises(FsaError) as e:
+                sc.find_neighbors()
+            assert e.value.code == abi.FSA_E_CAPACITY
+        sc.close()
+        plain = Scene(gpu_context, abi.default_desc(3, abi.SOLVER_IISPH, neighbor_lists=abi.LIST_PLAIN), coll.copy())
+        plain.find_neighbors()
+        assert plain.get_neighbors(0) == sorted(plain.get_neighbors(0)) and len(plain.get_neighbors(0)) == m + 1
+        plain.close()

@@ -258,3 +258,31 @@ def test_cfl_timestep(oracle_lib):
     assert O.cfl_timestep(0.0, v) == np.float32(np.float32(1.0) / np.float32(v) * np.float32(0.4))
     assert O.cfl_timestep(1e30, 1e30) == np.float32(0.000001)  # MIN_ALLOWED_TIMESTEP
     assert O.cfl_timestep(1.0, 1.0) == np.float32(0.25)  # min(0.25, 0.4); the solver clamps to the desired dt
+
+
+# ---- unstable quick sort, PCQuickSortTest.cpp --------------------------------------------
+
+
+@pytest.mark.parametrize("n", [16000])
+def test_quick_sort_orders(oracle_lib, n):
+    """PCQuickSortTest QuickSort1-3: ascending, descending and mt19937-shuffled tags sort."""
+    O = oracle_lib
+    asc = np.arange(n, dtype=np.uint64)
+    assert np.array_equal(asc[O.quick_sort(asc)], asc)
+    desc = asc[::-1].copy()
+    assert np.array_equal(desc[O.quick_sort(desc)], asc)
+    sh = np.random.default_rng(1).permutation(n).astype(np.uint64)
+    assert np.array_equal(sh[O.quick_sort(sh)], asc)
+
+
+def test_quick_sort_equal_keys_by_hand(oracle_lib):
+    """Hoare partition with the middle element as pivot, traced by hand:
+    [2,1,2,1] -> swap(0,3) split at 1; [0,1] swaps, [2,3] swaps -> perm [1,3,0,2];
+    [1,1,1] -> swap(0,2) split at 1; [0,1] swaps -> perm [1,2,0] (not stable)."""
+    O = oracle_lib
+    assert O.quick_sort(np.array([2, 1, 2, 1], np.uint64)).tolist() == [1, 3, 0, 2]
+    assert O.quick_sort(np.array([1, 1, 1], np.uint64)).tolist() == [1, 2, 0]
+    keys = np.random.default_rng(3).integers(0, 7, 5000).astype(np.uint64)
+    perm = O.quick_sort(keys).astype(np.int64)
+    assert sorted(perm.tolist()) == list(range(5000))
+    assert np.all(np.diff(keys[perm].astype(np.int64)) >= 0)

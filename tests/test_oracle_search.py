@@ -196,3 +196,58 @@ def test_inactive_sentinel_collides_with_cell_minus_one(oracle_lib):
     assert neighbors(off, cols, 1) == [2]  # particle 1 misses itself
     assert neighbors(off, cols, 2) == [2]  # and particle 2 misses particle 1
     assert brute_force_sets(pos, types, 2.0)[1] == [1, 2]
+
+
+# ---- HashedNeighborhoodSearch3D (HashedNeighborhoodSearch3D.cpp:69-247) ----------------------
+
+
+@pytest.mark.parametrize("seed", [4, 5])
+def test_hashed3d_sets_and_stencil_order(oracle_lib, seed):
+    """Same sets as brute force; each list walks the 27 cells dx-outer, dz-inner
+    with each cell's members ascending (std::set), inactive particles excluded."""
+    O = oracle_lib
+    rng = np.random.default_rng(seed)
+    n = 1200
+    pos = rng.uniform(-5.0, 5.0, size=(n, 3)).astype(np.float32)
+    pos[:40] = np.round(pos[:40])
+    types = rng.choice([0, 0, 0, 1, 2], size=n).astype(np.uint8)
+    s, _ = scene3d(O, pos, types, O.SEARCH_HASHED3D)
+    s.find_neighbors()
+    off, cols = s.export_neighbors()
+    bf = brute_force_sets(pos, types, 2.0)
+    for i in range(n):
+        assert neighbors(off, cols, i) == (bf[i] if types[i] != abi.TYPE_INACTIVE else []), i
+    roff, rcols = s.export_raw()
+    ids = s.internal_ids().astype(np.int64)
+    assert np.array_equal(ids, np.arange(n))  # no re-sort before the 101st call
+    cell = np.floor(pos / np.float32(2.0)).astype(np.int64)
+    for i in range(n):
+        lst = rcols[roff[i] : roff[i + 1]].astype(np.int64)
+        rel = cell[lst] - cell[i] + 1  # offsets in {0,1,2}
+        order = rel[:, 0] * 9 + rel[:, 1] * 3 + rel[:, 2]
+        key = order * n + lst
+        assert np.all(np.diff(key) > 0), i
+
+
+def test_hashed3d_resorts_on_call_101(oracle_lib):
+    """improve_cache_efficiency: calls 1-100 keep the order; call 101 applies the
+    unstable quick sort by the Morton key of each particle's cell from the grid
+    update of call 100 (inactive particles keep the INT_MIN default cell)."""
+    O = oracle_lib
+    rng = np.random.default_rng(7)
+    n = 600
+    pos = rng.uniform(0.0, 12.0, size=(n, 3)).astype(np.float32)
+    types = rng.choice([0, 0, 1, 2], size=n).astype(np.uint8)
+    s, c = scene3d(O, pos, types, O.SEARCH_HASHED3D)
+    for call in range(1, 101):
+        s.find_neighbors()
+        assert np.array_equal(s.internal_ids(), np.arange(n)), call
+        if call == 100:
+            p100 = s.download().position.copy()  # positions of the grid update of call 100
+        s.translate((0.013, -0.007, 0.011))
+    cells = np.floor(p100 / np.float32(2.0)).astype(np.int64)
+    imin = np.int64(-(2 ** 31))
+    cells[types == abi.TYPE_INACTIVE] = imin
+    keys = np.array([O.morton3(*[int(v - imin) for v in cells[i]]) for i in range(n)], np.uint64)
+    s.find_neighbors()
+    np.testing.assert_array_equal(s.internal_ids(), O.quick_sort(keys))
