@@ -181,6 +181,35 @@ def bench_served(ctx, window_s, rate, max_running, budget_gb, buckets=12):
     }
 
 
+def pmc_workload(args):
+    """Run under rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE: one plain device copy of
+    COPY_BYTES (read and written once each) calibrates the counters; then the
+    dam scene steps, so each kernel's per-dispatch counters can be scaled."""
+    import torch
+
+    side = args.side
+    coll = scenes.config2_dam3d(seed=2, side=side, interior=(3 * side, 3 * side // 2, 3 * side // 2))
+    ctx = Context(0)
+    sc = Scene(ctx, abi.default_desc(3, abi.SOLVER_IISPH), coll)
+    sc.step(2)
+    ctx.synchronize()
+    log("pmc workload: calibration copy")
+    a = torch.empty(COPY_BYTES // 4, dtype=torch.float32, device="cuda").fill_(1.0)
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    b.copy_(a)
+    torch.cuda.synchronize()
+    log("pmc workload: 3 steps")
+    sc.step(3)
+    ctx.synchronize()
+    log("pmc workload: done")
+    sc.close()
+    ctx.close()
+
+
+COPY_BYTES = 1 << 30
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -201,10 +230,16 @@ def main():
     ap.add_argument("--serve-budget-gb", type=float, default=200.0)
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--graph", type=int, default=1, help="replay step heads from hipGraphs (1) or launch eagerly (0)")
+    ap.add_argument("--pmc-workload", action="store_true",
+                    help="workload for rocprofv3 --pmc: a device copy of 1 GiB (counter calibration), then 3 steps")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus above 1 is not supported"}))
         sys.exit(2)
+
+    if args.pmc_workload:
+        pmc_workload(args)
+        return
 
     side = args.side
     interior = (3 * side, 3 * side // 2, 3 * side // 2)

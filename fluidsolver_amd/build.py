@@ -53,8 +53,23 @@ def build(verbose=False):
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, LIB)
+    build_mathcheck()
     build_dropin()
     return LIB
+
+
+MATHCHECK_LIB = os.path.join(LIBDIR, "libfsa_mathcheck.so")
+
+
+def build_mathcheck():
+    """Device checks of the pair-math helpers (loaded by tests/test_gpu_math.py only)."""
+    tmp = MATHCHECK_LIB + ".tmp"
+    cmd = [HIPCC] + FLAGS + ["-shared", os.path.join(CSRC, "fsa_mathcheck.hip"), "-o", tmp]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"hipcc failed for fsa_mathcheck.hip:\n{r.stdout}\n{r.stderr}")
+    os.replace(tmp, MATHCHECK_LIB)
+    return MATHCHECK_LIB
 
 
 DROPIN_SRC = os.path.join(os.path.dirname(HERE), "tests", "cpp", "dropin_dam.cpp")

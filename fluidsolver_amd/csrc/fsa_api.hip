@@ -327,6 +327,42 @@ struct MapBuilder {
     }
 };
 
+// Workgroups are dispatched to the 8 XCDs round-robin (workgroup i runs on XCD
+// i mod 8), and each XCD has its own L2.  Reordering a block map so that the
+// workgroups of one XCD take one contiguous range of blocks keeps the neighbor
+// gathers of a block, which hit the blocks next to it in the sorted order, in
+// the L2 that already holds them.  Every per-block result is indexed by the
+// logical block, so the order of dispatch does not change any result.
+constexpr uint32_t kXcds = 8;
+
+// chunk = 0: each XCD takes one contiguous eighth of the map (the fluid maps,
+// whose blocks all carry the same kind of work).  chunk = C: runs of C blocks
+// go to the XCDs in turn, which keeps neighbors together and still spreads
+// regions of uneven work (boundary-only cells) over all XCDs.
+std::vector<BlockRef> xcd_order(const std::vector<BlockRef>& v, size_t chunk) {
+    const size_t n = v.size();
+    if (n <= kXcds) return v;
+    if (chunk == 0) chunk = (n + kXcds - 1) / kXcds;
+    // the logical blocks each XCD should run, in order
+    std::vector<std::vector<size_t>> want(kXcds);
+    for (size_t c = 0; c * chunk < n; c++)
+        for (size_t b = c * chunk; b < std::min(n, (c + 1) * chunk); b++) want[c % kXcds].push_back(b);
+    // workgroup i runs on XCD i mod 8; an XCD whose list runs dry takes from the longest other list
+    std::vector<size_t> next(kXcds, 0);
+    std::vector<BlockRef> out(n);
+    for (size_t i = 0; i < n; i++) {
+        size_t k = i % kXcds;
+        if (next[k] == want[k].size()) {
+            size_t best = k;
+            for (size_t q = 0; q < kXcds; q++)
+                if (want[q].size() - next[q] > want[best].size() - next[best]) best = q;
+            k = best;
+        }
+        out[i] = v[want[k][next[k]++]];
+    }
+    return out;
+}
+
 BlockRef* upload_map(const std::vector<BlockRef>& v, std::vector<BlockRef*>& owned) {
     if (v.empty()) return nullptr;
     BlockRef* d = nullptr;
@@ -338,8 +374,8 @@ BlockRef* upload_map(const std::vector<BlockRef>& v, std::vector<BlockRef*>& own
 
 void build_launch_set(const MapBuilder& mb, SceneDev* table, LaunchSet& L, std::vector<BlockRef*>& owned) {
     L.scenes = table;
-    L.map_all = upload_map(mb.all, owned);
-    L.map_fluid = upload_map(mb.fluid, owned);
+    L.map_all = upload_map(xcd_order(mb.all, 16), owned);
+    L.map_fluid = upload_map(xcd_order(mb.fluid, 0), owned);
     L.map_tiles = upload_map(mb.tiles, owned);
     L.map_scan = upload_map(mb.scan, owned);
     L.map_scene = upload_map(mb.scene, owned);

@@ -171,6 +171,45 @@ __device__ __forceinline__ float dot3(float ax, float ay, float az, float bx, fl
     return (ax * bx + ay * by) + az * bz;
 }
 
+// ---- correctly rounded sqrt and division on the restricted ranges of the pair math ----
+// sqrt_rn_pair(x), x >= 0: the hardware square root (within 1 ulp) corrected by
+// comparing x with the products of its neighbors (s - 1 ulp, s + 1 ulp): the
+// residual x - t*s of a candidate t is exact in an FMA and its sign tells on
+// which side of the rounding midpoint sqrt(x) lies.  For x >= 2^-96 this is
+// the IEEE result bit for bit (checked exhaustively on the device); below, it
+// returns some value in [0, 2^-40], which gives the same kernel value and a
+// zero gradient as the exact root, since both then see q < 2^-25 and a length
+// under FLT_EPSILON.
+__device__ __forceinline__ float sqrt_rn_pair(float x) {
+    const float s = __builtin_amdgcn_sqrtf(x);
+    const float lo = __int_as_float(__float_as_int(s) - 1);
+    const float hi = __int_as_float(__float_as_int(s) + 1);
+    float r = s;
+    if (__builtin_fmaf(-lo, s, x) <= 0.0f) r = lo;
+    if (__builtin_fmaf(-hi, s, x) > 0.0f) r = hi;
+    return r;
+}
+
+// Reciprocal of b refined by one Newton step, for quotients sharing b.
+__device__ __forceinline__ float recip_pair(float b) {
+    const float y0 = __builtin_amdgcn_rcpf(b);
+    const float e = __builtin_fmaf(-b, y0, 1.0f);
+    return __builtin_fmaf(e, y0, y0);
+}
+
+// a / b from the refined reciprocal y of b: two FMA corrections, the same
+// operations the IEEE division expansion performs when its operand scaling is
+// the identity, which holds for 2^-23 < b <= 2^22 and a = 0 or 2^-59 <= |a| <= b.
+// The sign of a zero quotient is restored from a.
+__device__ __forceinline__ float div_pair(float a, float b, float y) {
+    const float q0 = a * y;
+    const float r0 = __builtin_fmaf(-b, q0, a);
+    const float q1 = __builtin_fmaf(r0, y, q0);
+    const float r1 = __builtin_fmaf(-b, q1, a);
+    const float q = __builtin_fmaf(r1, y, q1);
+    return __builtin_copysignf(q, a);
+}
+
 // Compile-time specialisation of the two parameters that make operations
 // exact no-ops: Spec bit 0 = every scene of the launch has h == 1, bit 1 =
 // every scene has rest_density^2 == 1.  Without the bit the scene flag decides.

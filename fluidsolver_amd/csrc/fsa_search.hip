@@ -568,9 +568,23 @@ __global__ __launch_bounds__(kBlock) void k_codec(const SceneDev* __restrict__ s
         cnt = n;
         if (n) {
             const auto nb = gload_ptr(S.nbr + k);
+            const uint32_t stride = S.list_stride;
             uint32_t prev = nb[0] & kIndexMask, payload = 0;
-            for (uint32_t e = 1; e < n; e++) {
-                const uint32_t j = nb[e * S.list_stride] & kIndexMask;
+            uint32_t e = 1;
+            // eight independent loads in flight, then the deltas in list order
+            for (; e + 8 <= n; e += 8) {
+                uint32_t j[8];
+#pragma unroll
+                for (int q = 0; q < 8; q++) j[q] = nb[(e + q) * stride] & kIndexMask;
+#pragma unroll
+                for (int q = 0; q < 8; q++) {
+                    const uint32_t d = j[q] - prev;
+                    prev = j[q];
+                    payload += d <= 2u ? 0u : (d <= 258u ? 1u : 4u);
+                }
+            }
+            for (; e < n; e++) {
+                const uint32_t j = nb[e * stride] & kIndexMask;
                 const uint32_t d = j - prev;
                 prev = j;
                 payload += d <= 2u ? 0u : (d <= 258u ? 1u : 4u);
@@ -596,16 +610,44 @@ __global__ __launch_bounds__(kBlock) void k_codec(const SceneDev* __restrict__ s
             c += wsum[0][w];
             b += wsum[1][w];
         }
-        atomicAdd((unsigned long long*)&S.state->total_neighbors, (unsigned long long)c);
-        atomicAdd((unsigned long long*)&S.state->codec_bytes, (unsigned long long)b);
+        // block totals as a partial: neighbors in count, encoded bytes in err
+        Partial p{};
+        p.count = (uint32_t)c;
+        p.err = (double)b;
+        S.partial[br.block] = p;
     }
 }
 
-__global__ void k_codec_reset(const SceneDev* __restrict__ scenes, const BlockRef* __restrict__ map) {
+// one block per scene: the block partials of k_codec, in block order
+__global__ __launch_bounds__(kBlock) void k_codec_finalize(const SceneDev* __restrict__ scenes,
+                                                           const BlockRef* __restrict__ map) {
     const SceneDev& S = scenes[map[blockIdx.x].scene];
+    __shared__ uint64_t wsum[2][kBlock / 64];
+    const uint32_t nparts = (S.nf + kBlock - 1) / kBlock;
+    uint64_t c = 0, b = 0;
+    for (uint32_t q = threadIdx.x; q < nparts; q += kBlock) {
+        const Partial p = S.partial[q];
+        c += p.count;
+        b += (uint64_t)p.err;
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        c += __shfl_xor(c, o, 64);
+        b += __shfl_xor(b, o, 64);
+    }
+    if (__lane_id() == 0) {
+        wsum[0][threadIdx.x / 64] = c;
+        wsum[1][threadIdx.x / 64] = b;
+    }
+    __syncthreads();
     if (threadIdx.x == 0) {
-        S.state->total_neighbors = 0;
-        S.state->codec_bytes = 0;
+        uint64_t tc = 0, tb = 0;
+        for (int w = 0; w < kBlock / 64; w++) {
+            tc += wsum[0][w];
+            tb += wsum[1][w];
+        }
+        S.state->total_neighbors = tc;
+        S.state->codec_bytes = tb;
     }
 }
 
@@ -661,8 +703,8 @@ void launch_permute_only(const LaunchSet& L, uint32_t sbuf, hipStream_t st) {
 }
 
 void launch_codec(const LaunchSet& L, hipStream_t st) {
-    k_codec_reset<<<L.nb_scene, 64, 0, st>>>(L.scenes, L.map_scene);
     if (L.nb_fluid) k_codec<<<L.nb_fluid, kBlock, 0, st>>>(L.scenes, L.map_fluid);
+    k_codec_finalize<<<L.nb_scene, kBlock, 0, st>>>(L.scenes, L.map_scene);
 }
 
 }  // namespace fsa

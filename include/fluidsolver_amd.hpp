@@ -259,6 +259,58 @@ class SESPHFluidSolver3D : public DeviceSolverBase {
     }
 };
 
+// Serving loop over the C ABI (fsa_server_*): jobs are copied at submit, admitted
+// against the HBM budget and slot limit, stepped together, and collected in the
+// caller's row order.
+class Server {
+  public:
+    explicit Server(Context& ctx, uint64_t hbm_budget_bytes = 0, uint32_t max_running = 0) {
+        fsa_server_config cfg{};
+        cfg.hbm_budget_bytes = hbm_budget_bytes;
+        cfg.max_running = max_running;
+        check(fsa_server_create(ctx.handle(), &cfg, &h_));
+    }
+    ~Server() {
+        if (h_) fsa_server_destroy(h_);
+    }
+    Server(const Server&) = delete;
+    Server& operator=(const Server&) = delete;
+
+    uint64_t submit(const fsa_scene_desc& desc, ParticleCollection& c, uint32_t steps) {
+        uint64_t id = 0;
+        fsa_particle_view v = c.view();
+        check(fsa_server_submit(h_, &desc, &v, steps, &id));
+        return id;
+    }
+    uint32_t tick() {
+        uint32_t n = 0;
+        check(fsa_server_tick(h_, &n));
+        return n;
+    }
+    fsa_server_stats stats() const {
+        fsa_server_stats st{};
+        check(fsa_server_get_stats(h_, &st));
+        return st;
+    }
+    bool idle() const {
+        const fsa_server_stats st = stats();
+        return st.jobs_queued == 0 && st.jobs_running == 0;
+    }
+    fsa_job_info job(uint64_t id) const {
+        fsa_job_info info{};
+        check(fsa_server_job(h_, id, &info));
+        return info;
+    }
+    void collect(uint64_t id, ParticleCollection& c) {
+        fsa_particle_view v = c.view();
+        check(fsa_server_collect(h_, id, &v));
+    }
+    void cancel(uint64_t id) { check(fsa_server_cancel(h_, id)); }
+
+  private:
+    fsa_server h_ = nullptr;
+};
+
 // Simulator::execute_simulation_step without entities and sensors (Simulator.cpp:7-57).
 template <class Solver>
 class Simulator {

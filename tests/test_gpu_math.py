@@ -1,0 +1,44 @@
+"""The pair-math helpers against the IEEE operations, on the device.
+
+sqrt_rn_pair: every non-negative finite float; bitwise equal to the correctly
+rounded square root from 2^-96 up, and within [0, 2^-40] below (where a length
+feeds only q < 2^-25 and a zero gradient).
+div_pair: every divisor mantissa in [1, 2) at every exponent of 2^-23 .. 2^21
+(divisors up to 2^22), against 16 numerators each: signed zeros, +-b, values
+near 2^-59 and random magnitudes in [2^-59, b]; bitwise equal to IEEE a / b."""
+import ctypes as C
+import os
+
+import pytest
+
+from fluidsolver_amd import build as fbuild
+
+pytestmark = pytest.mark.gpu
+
+
+def _lib():
+    if not os.path.exists(fbuild.MATHCHECK_LIB):
+        pytest.skip("math check library not built")
+    L = C.CDLL(fbuild.MATHCHECK_LIB)
+    L.fsa_mc_sqrt.restype = C.c_int
+    L.fsa_mc_sqrt.argtypes = [C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]
+    L.fsa_mc_div.restype = C.c_int
+    L.fsa_mc_div.argtypes = [C.c_int, C.c_int, C.c_uint32, C.c_uint64, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]
+    return L
+
+
+def test_sqrt_pair_exhaustive():
+    L = _lib()
+    m, b = C.c_uint64(0), C.c_uint64(0)
+    print("case sqrt exhaustive", flush=True)
+    assert L.fsa_mc_sqrt(C.byref(m), C.byref(b)) == 0
+    assert m.value == 0 and b.value == 0
+
+
+def test_div_pair_all_divisor_mantissas():
+    L = _lib()
+    m, t = C.c_uint64(0), C.c_uint64(0)
+    print("case div mantissas", flush=True)
+    assert L.fsa_mc_div(-23, 21, 16, 12345, C.byref(m), C.byref(t)) == 0
+    assert t.value == (1 << 23) * 45 * 16
+    assert m.value == 0
