@@ -193,7 +193,7 @@ void fill_params(fsa_scene_s* s) {
     h.dt = d.timestep;
     h.lambda_a = d.lambda_a;
     h.lambda_v = d.lambda_v;
-    uint32_t f = h.flags & (kFlagExternal | kFlagDynamicBoundary);
+    uint32_t f = h.flags & (kFlagExternal | kFlagDynamicBoundary | kFlagTinyCoords);
     if (h.h == 1.0f) f |= kFlagHOne;
     if (h.rho0sq == 1.0f) f |= kFlagRho0SqOne;
     if (sesph ? d.sesph_single_layer_boundary : d.iisph_single_layer_boundary) f |= kFlagSingleLayer;
@@ -462,7 +462,15 @@ uint32_t spec_of(const SceneDev& h) {
     uint32_t sp = 0;
     if (h.flags & kFlagHOne) sp |= kSpecHOne;
     if (h.flags & kFlagRho0SqOne) sp |= kSpecRho0SqOne;
+    // the shared-reciprocal division and the short sqrt correction are exact for
+    // h == 1 as long as no coordinate is tiny (2D runs its own kernels)
+    if (h.dim == 3 && (h.flags & kFlagHOne) && !(h.flags & kFlagTinyCoords)) sp |= kSpecFastMath;
     return sp;
+}
+
+bool tiny_coordinate(float v) {
+    const float a = std::fabs(v);
+    return a > 0.0f && a < kTinyCoord;
 }
 
 uint32_t sort_passes(const SceneDev& h) { return (3 * h.levels + 3 + kRadixBits - 1) / kRadixBits; }
@@ -804,6 +812,7 @@ void set_step_dt(fsa_scene_s* s, float dt) {
 // after the step's final state read
 void post_simulate(fsa_scene_s* s, fsa_step_stats* stats) {
     s->boundary_dirty = false;
+    if (s->h_state->status & kStatusTinyCoord) s->hd.flags |= kFlagTinyCoords;
     if (s->desc.solver == FSA_SOLVER_IISPH) s->last_iterations = std::max<uint32_t>(s->h_state->iterations, 1);
     fill_stats(s, stats);
     s->searched = false;
@@ -871,7 +880,7 @@ void drop_batches_of(fsa_context ctx, fsa_scene_s* s) {
 }
 
 void upload_batch_table(fsa_context ctx, BatchSet* b) {
-    uint32_t spec = kSpecHOne | kSpecRho0SqOne;
+    uint32_t spec = kSpecHOne | kSpecRho0SqOne | kSpecFastMath;  // the specialisations every scene allows
     bool codec = false;
     for (uint32_t q = 0; q < b->scenes.size(); q++) {
         b->h_table[q] = b->scenes[q]->hd;
@@ -1359,7 +1368,7 @@ void do_upload(fsa_scene_s* s, const fsa_particle_view* v, bool first) {
     std::vector<float4> pos(n), vel(n), acc(n), ext(n);
     std::vector<float> pres(n), zero(n, 0.0f);
     std::vector<uint32_t> idt(n);
-    bool any_ext = false, dyn_boundary = false;
+    bool any_ext = false, dyn_boundary = false, any_tiny = false;
     int32_t lo[3] = {INT32_MAX, INT32_MAX, INT32_MAX}, hi[3] = {INT32_MIN, INT32_MIN, INT32_MIN};
     const float r = s->desc.particle_size * 2.0f;
     for (uint32_t k = 0; k < n; k++) {
@@ -1381,6 +1390,7 @@ void do_upload(fsa_scene_s* s, const fsa_particle_view* v, bool first) {
         pres[k] = pd[1];
         idt[k] = (uint32_t)i | ((uint32_t)t << kTypeShift);
         if (e[0] != 0.0f || e[1] != 0.0f || e[2] != 0.0f) any_ext = true;
+        if (tiny_coordinate(m[0]) || tiny_coordinate(m[1]) || tiny_coordinate(m[2])) any_tiny = true;
         if (t == FSA_TYPE_BOUNDARY &&
             (m[3] != 0.0f || m[4] != 0.0f || m[5] != 0.0f || m[6] != 0.0f || m[7] != 0.0f || m[8] != 0.0f ||
              e[0] != 0.0f || e[1] != 0.0f || e[2] != 0.0f))
@@ -1406,9 +1416,10 @@ void do_upload(fsa_scene_s* s, const fsa_particle_view* v, bool first) {
             std::memcpy(&s->inactive_ext[3 * k], (const char*)v->external + i * v->external_stride, 12);
     }
     SceneDev& h = s->hd;
-    h.flags &= ~(kFlagExternal | kFlagDynamicBoundary);
+    h.flags &= ~(kFlagExternal | kFlagDynamicBoundary | kFlagTinyCoords);
     if (any_ext) h.flags |= kFlagExternal;
     if (dyn_boundary) h.flags |= kFlagDynamicBoundary;
+    if (any_tiny) h.flags |= kFlagTinyCoords;
     hipStream_t st = s->ctx->stream;
     if ((h.flags & kFlagUnsorted) && n) {
         // the collection keeps its order between searches: fluid ranks are fixed here

@@ -43,6 +43,7 @@ constexpr uint32_t kFlagIntAbs = 128u;      // 2D IISPH: integer abs() in the lo
 constexpr uint32_t kFlagQuadratic = 256u;   // 2D: QuadraticNeighborhoodSearchDynamicAllocated order
 constexpr uint32_t kFlagCodec = 512u;       // enforce the NeighborStorage codec capacity each search
 constexpr uint32_t kFlagUnsorted = 1024u;   // 3D hashed / quadratic searches: the collection keeps its order
+constexpr uint32_t kFlagTinyCoords = 2048u; // some coordinate lies in (0, 2^-36): pair math takes the IEEE path
 
 // 2D scenes (IISPHFluidSolver / SESPHFluidSolver with HashedNeighborhoodSearch or
 // QuadraticNeighborhoodSearchDynamicAllocated) keep every particle, inactive
@@ -54,6 +55,7 @@ constexpr uint32_t kFlagUnsorted = 1024u;   // 3D hashed / quadratic searches: t
 constexpr uint32_t kStatusCapacity = 1u;    // a particle had more than 97 neighbors
 constexpr uint32_t kStatusGrid = 2u;        // a particle left the cell grid of this step
 constexpr uint32_t kStatusNonFinite = 4u;
+constexpr uint32_t kStatusTinyCoord = 8u;   // a coordinate lies in (0, 2^-36) in magnitude (not an error)
 
 // Per-scene state written by the device and read back once per step.
 struct alignas(16) SceneState {
@@ -215,6 +217,13 @@ __device__ __forceinline__ float div_pair(float a, float b, float y) {
 // every scene has rest_density^2 == 1.  Without the bit the scene flag decides.
 constexpr uint32_t kSpecHOne = 1u;
 constexpr uint32_t kSpecRho0SqOne = 2u;
+// Bit 2 (with bit 0 only): every scene of the launch has h == 1 and no coordinate
+// in (0, 2^-36).  Then every pair difference is 0 or at least 2^-59 in magnitude
+// and lengths with a nonzero gradient lie in (2^-23, 2], the ranges on which
+// sqrt_rn_pair and div_pair equal the IEEE operations.
+constexpr uint32_t kSpecFastMath = 4u;
+// smallest magnitude a nonzero coordinate may have for the fast pair math
+constexpr float kTinyCoord = 1.4551915228366852e-11f;  // 2^-36
 
 template <uint32_t Spec>
 __device__ __forceinline__ bool spec_h_one(const SceneDev& S) {
@@ -249,7 +258,13 @@ __device__ __forceinline__ void kernel_grad(const SceneDev& S, float dx, float d
     const float q = hone ? len : len / S.h;
     const bool nz = len > 1.1920928955078125e-07f;
     float px, py, pz;
-    if (hone) {
+    if (Spec & kSpecFastMath) {
+        // h == 1: d / |d| with one shared reciprocal (exact on this range)
+        const float y = recip_pair(len);
+        px = div_pair(dx, len, y);
+        py = div_pair(dy, len, y);
+        pz = div_pair(dz, len, y);
+    } else if (hone) {
         px = dx / len;
         py = dy / len;
         pz = dz / len;
@@ -268,6 +283,12 @@ __device__ __forceinline__ void kernel_grad(const SceneDev& S, float dx, float d
     gx = inside ? S.alpha * (px * s) : 0.0f;
     gy = inside ? S.alpha * (py * s) : 0.0f;
     gz = inside ? S.alpha * (pz * s) : 0.0f;
+}
+
+// length(x_i - x_j) from the squared length, correctly rounded where it matters
+template <uint32_t Spec>
+__device__ __forceinline__ float pair_length(float d2) {
+    return (Spec & kSpecFastMath) ? sqrt_rn_pair(d2) : __builtin_sqrtf(d2);
 }
 
 template <uint32_t Spec>
@@ -300,6 +321,26 @@ __device__ __host__ __forceinline__ uint32_t compact_key(int32_t cx, int32_t cy,
     key |= sz << (3 * L + 1);
     key |= sx << (3 * L + 2);
     return key;
+}
+
+// Cell coordinates of a compact key (the inverse of compact_key): the
+// interleaved bits of each axis gathered with shift-and-mask steps, then the
+// axis sign bit extends the L-bit value.
+__device__ __host__ __forceinline__ uint32_t compact_axis(uint32_t v) {
+    v &= 0x09249249u;
+    v = (v ^ (v >> 2)) & 0x030c30c3u;
+    v = (v ^ (v >> 4)) & 0x0300f00fu;
+    v = (v ^ (v >> 8)) & 0xff0000ffu;
+    v = (v ^ (v >> 16)) & 0x000003ffu;
+    return v;
+}
+__device__ __host__ __forceinline__ void cell_of_key(uint32_t key, uint32_t L, int32_t& cx, int32_t& cy, int32_t& cz) {
+    const uint32_t body = key & ((1u << (3 * L)) - 1u);
+    const uint32_t ext = ~((1u << L) - 1u);
+    const uint32_t x = compact_axis(body), y = compact_axis(body >> 1), z = compact_axis(body >> 2);
+    cy = (int32_t)(((key >> (3 * L)) & 1u) ? (y | ext) : y);
+    cz = (int32_t)(((key >> (3 * L + 1)) & 1u) ? (z | ext) : z);
+    cx = (int32_t)(((key >> (3 * L + 2)) & 1u) ? (x | ext) : x);
 }
 
 __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }

@@ -350,45 +350,52 @@ __global__ __launch_bounds__(kBlock) void k_lists(const SceneDev* __restrict__ s
     const auto keys = gload_ptr(S.keys[sbuf]);
     const auto rank_of = gload_ptr(S.rank_of);
     FSA_GLOBAL(uint32_t)* __restrict__ nbr = (FSA_GLOBAL(uint32_t)*)S.nbr;
-    const float r = S.search_radius;
     const float r2 = S.r2;
     const uint32_t stride = S.list_stride;
     // run starts among this wave's particles
     const uint32_t mi = base + lane;
     bool start = false;
-    if (mi < S.n) start = mi == 0 || keys[mi - 1] != keys[mi];
+    uint32_t mykey = 0;
+    if (mi < S.n) {
+        mykey = keys[mi];
+        start = mi == 0 || keys[mi - 1] != mykey;
+    }
     uint64_t starts = __ballot(start);
     bool overflow = false;
     while (starts) {
-        const uint32_t cs = base + (uint32_t)__ffsll((unsigned long long)starts) - 1u;
+        const uint32_t sl0 = (uint32_t)__ffsll((unsigned long long)starts) - 1u;
+        const uint32_t cs = base + sl0;
         starts &= starts - 1;
-        const float4 p0 = pos[cs];
-        const int32_t cx = cell_of(p0.x, r), cy = cell_of(p0.y, r), cz = cell_of(p0.z, r);
+        // the cell of the run, from its key (no position load, no division)
+        int32_t cx, cy, cz;
+        cell_of_key(readlane_u(mykey, sl0), S.levels, cx, cy, cz);
         uint32_t lin0;
         cell_linear(S, cx, cy, cz, lin0);
         const uint32_t ce = S.cells[lin0].y;
         bool anyf = false;
         for (uint32_t t = cs + lane; t < ce; t += 64) anyf |= (idt[t] >> kTypeShift) == 0u;
         if (!__any(anyf)) continue;
-        // runs of the 27 neighbor cells, sorted by start
-        uint32_t key = kEmptyCell;
+        // runs of the 27 neighbor cells, sorted by start; each run's end rides
+        // along by lane and is fetched back with a shuffle after the sort
+        uint32_t key = kEmptyCell, cend = 0;
         if (lane < 27) {
             uint32_t lin;
             if (cell_linear(S, cx + (int)(lane / 9) - 1, cy + (int)((lane / 3) % 3) - 1, cz + (int)(lane % 3) - 1,
                             lin)) {
                 const uint4 e = S.cells[lin];
-                if (e.z == S.stamp) key = (e.x << 5) | lane;
+                if (e.z == S.stamp) {
+                    key = (e.x << 5) | lane;
+                    cend = e.y;
+                }
             }
         }
         key = wave_sort_u32(key);
         const uint32_t nvalid = (uint32_t)__popcll(__ballot(key != kEmptyCell));
+        const uint32_t end_of_key = __shfl(cend, (int)(key & 31u), 64);
         uint32_t rs = 0, rl = 0;
         if (key != kEmptyCell) {
-            const uint32_t sl = key & 31u;
-            uint32_t lin;
-            cell_linear(S, cx + (int)(sl / 9) - 1, cy + (int)((sl / 3) % 3) - 1, cz + (int)(sl % 3) - 1, lin);
             rs = key >> 5;
-            rl = S.cells[lin].y - rs;
+            rl = end_of_key - rs;
         }
         uint32_t total;
         const uint32_t roff = wave_exclusive_scan(rl, total);

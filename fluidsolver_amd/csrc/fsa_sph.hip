@@ -34,7 +34,7 @@ __device__ __forceinline__ void pair_geometry(const SceneDev& S, const float4& x
     dy = xi.y - xj.y;
     dz = xi.z - xj.z;
     d2 = dot3(dx, dy, dz, dx, dy, dz);
-    len = __builtin_sqrtf(d2);
+    len = pair_length<Spec>(d2);
     kernel_grad<Spec>(S, dx, dy, dz, len, g.x, g.y, g.z);
 }
 
@@ -426,7 +426,7 @@ __global__ __launch_bounds__(kBlock) void k_sesph_density(const SceneDev* __rest
         gload_ptr(S.nbr + k), S.list_stride, S.nbr_cnt[k], [&](uint32_t j) { return pos[j]; },
         [&](uint32_t ent, const float4& xj) {
             const float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
-            const float len = __builtin_sqrtf(dot3(dx, dy, dz, dx, dy, dz));
+            const float len = pair_length<Spec>(dot3(dx, dy, dz, dx, dy, dz));
             if (sl && (ent & kBoundaryBit))
                 rho = rho + S.gamma1 * xj.w * kernel_w<Spec>(S, len);
             else
@@ -586,6 +586,9 @@ __global__ __launch_bounds__(kBlock) void k_integrate(const SceneDev* __restrict
             x.y = x.y + corrected * vy;
             x.z = x.z + corrected * vz;
             S.pos[c][i] = x;
+            const float ax_ = fabsf(x.x), ay_ = fabsf(x.y), az_ = fabsf(x.z);
+            if ((ax_ > 0.0f && ax_ < kTinyCoord) || (ay_ > 0.0f && ay_ < kTinyCoord) || (az_ > 0.0f && az_ < kTinyCoord))
+                atomicOr(&S.state->status, kStatusTinyCoord);
             S.vel[c][i] = make_float4(vx, vy, vz, v.w);
         }
         lo[0] = hi[0] = cell_of(x.x, S.search_radius);
@@ -672,6 +675,8 @@ __global__ __launch_bounds__(kBlock) void k_integrate_finalize(const SceneDev* _
 #define FSA_SPEC_DISPATCH(kernel, grid, map)                                                        \
     do {                                                                                            \
         switch (L.spec) {                                                                           \
+            case 7: kernel<7><<<grid, kBlock, 0, st>>>(L.scenes, map); break;                       \
+            case 5: kernel<5><<<grid, kBlock, 0, st>>>(L.scenes, map); break;                       \
             case 3: kernel<3><<<grid, kBlock, 0, st>>>(L.scenes, map); break;                       \
             case 1: kernel<1><<<grid, kBlock, 0, st>>>(L.scenes, map); break;                       \
             default: kernel<0><<<grid, kBlock, 0, st>>>(L.scenes, map); break;                      \
@@ -682,6 +687,8 @@ __global__ __launch_bounds__(kBlock) void k_integrate_finalize(const SceneDev* _
 #define FSA_SPEC_U(kernel, U, grid, map)                                                            \
     do {                                                                                            \
         switch (L.spec) {                                                                           \
+            case 7: kernel<7, U><<<grid, kBlock, 0, st>>>(L.scenes, map); break;                    \
+            case 5: kernel<5, U><<<grid, kBlock, 0, st>>>(L.scenes, map); break;                    \
             case 3: kernel<3, U><<<grid, kBlock, 0, st>>>(L.scenes, map); break;                    \
             case 1: kernel<1, U><<<grid, kBlock, 0, st>>>(L.scenes, map); break;                    \
             default: kernel<0, U><<<grid, kBlock, 0, st>>>(L.scenes, map); break;                   \

@@ -81,4 +81,5 @@ def test_autotune_keeps_results_and_reports():
     for x, y in zip(am, bm):
         np.testing.assert_array_equal(x, y)
     assert "pass_a=U" in rep and "diagonal=U" in rep and rep0 == ""
+    assert "spec=7" in rep  # h = 1, rest density 1, no tiny coordinates: the exact fast pair math
     print(rep, flush=True)

@@ -331,3 +331,18 @@ def test_codec_capacity_boundary(gpu_context, oracle_lib):
         plain.find_neighbors()
         assert plain.get_neighbors(0) == sorted(plain.get_neighbors(0)) and len(plain.get_neighbors(0)) == m + 1
         plain.close()
+
+
+def test_tiny_coordinates_take_the_ieee_path(gpu_context, oracle_lib):
+    """A coordinate in (0, 2^-36) can make a pair difference smaller than the
+    shared-reciprocal division covers; such scenes run the IEEE pair math and
+    stay bit-identical (the fast path is checked by every other test)."""
+    O = oracle_lib
+    coll = scenes.dam_scene((6, 6, 6), (18, 10, 10), 2, seed=71)
+    fl = np.nonzero(coll.type == abi.TYPE_NORMAL)[0]
+    coll.position[fl[0], 0] = np.float32(3e-20)  # next to particles at x = 0 (jittered lattice)
+    desc = abi.default_desc(3, abi.SOLVER_IISPH)
+    print("case tiny coordinates", flush=True)
+    sc = Scene(gpu_context, desc, coll.copy())
+    orc = O.OracleScene(desc, coll.copy(), mode=O.MODE_FP32_R64)
+    _compare_steps(sc, orc, coll, 4, "tiny")
