@@ -632,9 +632,13 @@ void check_status(fsa_scene_s* s) {
     if (status & kStatusGrid) throw FsaError(FSA_E_STATE, "a particle left the cell grid (internal error)");
 }
 
+// neighbor-list byte offsets need more than 32 bits
+bool wide_lists(const SceneDev& h) { return 4ull * kMaxNeighbors * h.list_stride > 0xffffffffull; }
+
 void upload_scene_table(fsa_scene_s* s) {
     s->single.spec = spec_of(s->hd);
     s->single.codec = (s->hd.flags & kFlagCodec) != 0;
+    s->single.wide_lists = wide_lists(s->hd);
     HIP_CHECK(hipMemcpyAsync(s->d_table, &s->hd, sizeof(SceneDev), hipMemcpyHostToDevice, s->ctx->stream));
 }
 
@@ -881,14 +885,16 @@ void drop_batches_of(fsa_context ctx, fsa_scene_s* s) {
 
 void upload_batch_table(fsa_context ctx, BatchSet* b) {
     uint32_t spec = kSpecHOne | kSpecRho0SqOne | kSpecFastMath;  // the specialisations every scene allows
-    bool codec = false;
+    bool codec = false, wide = false;
     for (uint32_t q = 0; q < b->scenes.size(); q++) {
         b->h_table[q] = b->scenes[q]->hd;
         spec &= spec_of(b->scenes[q]->hd);
         codec = codec || (b->scenes[q]->hd.flags & kFlagCodec);
+        wide = wide || wide_lists(b->scenes[q]->hd);
     }
     b->ls.spec = spec;
     b->ls.codec = codec;
+    b->ls.wide_lists = wide;
     HIP_CHECK(hipMemcpyAsync(b->d_table, b->h_table.data(), sizeof(SceneDev) * b->h_table.size(),
                              hipMemcpyHostToDevice, ctx->stream));
 }

@@ -22,6 +22,7 @@ struct LaunchSet {
     // neighbor-walk unroll per IISPH kernel family (kTune*), chosen by the autotuner
     uint8_t unroll[4] = {4, 4, 4, 4};
     bool codec = false;  // some scene of the set checks the NeighborStorage codec each search
+    bool wide_lists = false;  // some scene's neighbor-list array spans 4 GiB or more
 };
 
 // kernel families the autotuner times

@@ -295,37 +295,57 @@ __global__ __launch_bounds__(kBlock) void k_cell_table(const SceneDev* __restric
 // One wave per 64 consecutive sorted particles processes every cell whose run
 // starts among them.  For a cell, the wave sorts the runs of the 27 neighbor
 // cells by start (= by cell key, the reference's std::sort of the 27 keys), so
-// candidate g of the concatenated runs has ascending combined index.  Chunks of
-// 64 candidates are held one per lane; the cell's fluid particles (one per lane
-// as well) are broadcast in turn and tested against the whole chunk with one
-// ballot, whose prefix gives each accepted candidate its slot, so every list is
-// written in ascending order without staging.
+// candidate g of the concatenated runs has ascending combined index.  Up to
+// kListChunks chunks of 64 candidates are held one per lane; the cell's fluid
+// particles (one per lane as well) are taken in turn, each tested against the
+// held chunks with one ballot per chunk, whose prefix (on top of the particle's
+// running count, a scalar) gives each accepted candidate its slot, so every list
+// is written in ascending order without staging.
 
-__device__ __forceinline__ uint32_t wave_sort_u32(uint32_t v) {
-    // bitonic sort across the 64 lanes, ascending by lane
+constexpr uint32_t kListChunks = 4;
+
+// one compare-exchange stage of a bitonic sort within 32-lane groups; the
+// partner comes from ds_swizzle's bit mode (and 0x1f, xor J), which needs no
+// address registers
+template <uint32_t K, uint32_t J>
+__device__ __forceinline__ uint32_t bitonic32_stage(uint32_t v, uint32_t lane) {
+    const uint32_t o = (uint32_t)__builtin_amdgcn_ds_swizzle((int)v, (int)(0x1fu | (J << 10)));
+    const bool up = (lane & K) == 0;
+    const bool lower = (lane & J) == 0;
+    return (up == lower) ? min(v, o) : max(v, o);
+}
+
+// lanes 0..31 ascending by lane (lanes 32..63 are sorted descending among themselves)
+__device__ __forceinline__ uint32_t sort32_u32(uint32_t v) {
     const uint32_t lane = __lane_id();
-#pragma unroll
-    for (int k = 2; k <= 64; k <<= 1) {
-#pragma unroll
-        for (int j = k >> 1; j > 0; j >>= 1) {
-            const uint32_t o = __shfl_xor(v, j, 64);
-            const bool up = (lane & k) == 0;
-            const bool lower = (lane & j) == 0;
-            v = (up == lower) ? min(v, o) : max(v, o);
-        }
-    }
+    v = bitonic32_stage<2, 1>(v, lane);
+    v = bitonic32_stage<4, 2>(v, lane);
+    v = bitonic32_stage<4, 1>(v, lane);
+    v = bitonic32_stage<8, 4>(v, lane);
+    v = bitonic32_stage<8, 2>(v, lane);
+    v = bitonic32_stage<8, 1>(v, lane);
+    v = bitonic32_stage<16, 8>(v, lane);
+    v = bitonic32_stage<16, 4>(v, lane);
+    v = bitonic32_stage<16, 2>(v, lane);
+    v = bitonic32_stage<16, 1>(v, lane);
+    v = bitonic32_stage<32, 16>(v, lane);
+    v = bitonic32_stage<32, 8>(v, lane);
+    v = bitonic32_stage<32, 4>(v, lane);
+    v = bitonic32_stage<32, 2>(v, lane);
+    v = bitonic32_stage<32, 1>(v, lane);
     return v;
 }
 
-__device__ __forceinline__ uint32_t wave_exclusive_scan(uint32_t v, uint32_t& total) {
-    const uint32_t lane = __lane_id();
+// exclusive prefix sum over lanes 0..31 (each 32-lane half on its own); total = sum of lanes 0..31
+__device__ __forceinline__ uint32_t exclusive_scan32(uint32_t v, uint32_t& total) {
+    const uint32_t lane = __lane_id() & 31u;
     uint32_t x = v;
 #pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        const uint32_t y = __shfl_up(x, o, 64);
+    for (int o = 1; o < 32; o <<= 1) {
+        const uint32_t y = __shfl_up(x, o, 32);
         if (lane >= (uint32_t)o) x += y;
     }
-    total = __shfl(x, 63, 64);
+    total = (uint32_t)__builtin_amdgcn_readlane((int)x, 31);  // uniform, in a scalar register
     return x - v;
 }
 
@@ -336,6 +356,19 @@ __device__ __forceinline__ uint32_t readlane_u(uint32_t v, uint32_t lane) {
     return (uint32_t)__builtin_amdgcn_readlane((int)v, (int)lane);
 }
 
+// slot * stride for operands below 2^24: the full-rate 24-bit multiply (the
+// compiler would otherwise fold the byte shift into the multiplier and issue
+// the quarter-rate 32-bit multiply)
+__device__ __forceinline__ uint32_t mul_u24(uint32_t v, uint32_t s) {
+    uint32_t r;
+    asm("v_mul_u32_u24 %0, %1, %2" : "=v"(r) : "s"(s), "v"(v));
+    return r;
+}
+
+// Wide: the list array spans 4 GiB or more (element indices still fit 32 bits),
+// so slots are addressed in 64 bits; otherwise a slot is one 24-bit multiply and
+// a 32-bit byte offset from the particle's list base.
+template <bool Wide>
 __global__ __launch_bounds__(kBlock) void k_lists(const SceneDev* __restrict__ scenes,
                                                   const BlockRef* __restrict__ map, uint32_t sbuf) {
     const BlockRef br = map[blockIdx.x];
@@ -375,8 +408,8 @@ __global__ __launch_bounds__(kBlock) void k_lists(const SceneDev* __restrict__ s
         bool anyf = false;
         for (uint32_t t = cs + lane; t < ce; t += 64) anyf |= (idt[t] >> kTypeShift) == 0u;
         if (!__any(anyf)) continue;
-        // runs of the 27 neighbor cells, sorted by start; each run's end rides
-        // along by lane and is fetched back with a shuffle after the sort
+        // runs of the 27 neighbor cells in lanes 0..26, sorted by start; each
+        // run's end rides along by lane and is fetched back after the sort
         uint32_t key = kEmptyCell, cend = 0;
         if (lane < 27) {
             uint32_t lin;
@@ -389,16 +422,16 @@ __global__ __launch_bounds__(kBlock) void k_lists(const SceneDev* __restrict__ s
                 }
             }
         }
-        key = wave_sort_u32(key);
-        const uint32_t nvalid = (uint32_t)__popcll(__ballot(key != kEmptyCell));
+        key = sort32_u32(key);
+        const uint32_t nvalid = (uint32_t)__popcll(__ballot(lane < 32 && key != kEmptyCell));
         const uint32_t end_of_key = __shfl(cend, (int)(key & 31u), 64);
         uint32_t rs = 0, rl = 0;
-        if (key != kEmptyCell) {
+        if (lane < 32 && key != kEmptyCell) {
             rs = key >> 5;
             rl = end_of_key - rs;
         }
         uint32_t total;
-        const uint32_t roff = wave_exclusive_scan(rl, total);
+        const uint32_t roff = exclusive_scan32(rl, total);
         for (uint32_t pbase = cs; pbase < ce; pbase += 64) {
             // the cell's particles, one per lane
             const uint32_t pi = pbase + lane;
@@ -412,44 +445,67 @@ __global__ __launch_bounds__(kBlock) void k_lists(const SceneDev* __restrict__ s
                 kl = rank_of[pi];
             }
             uint32_t cntl = 0;
-            for (uint32_t c0 = 0; c0 < total; c0 += 64) {
-                // candidate g of the concatenated runs: run q by binary search over the offsets
-                // (uniform: every lane takes part in each shuffle)
-                const uint32_t g = c0 + lane;
-                uint32_t q = 0;
+            for (uint32_t c0 = 0; c0 < total; c0 += kListChunks * 64) {
+                // candidates c0 + 64k + lane; a lane past the last candidate sits
+                // at infinity and fails every test
+                float4 pj[kListChunks];
+                uint32_t ej[kListChunks];
 #pragma unroll
-                for (uint32_t step = 16; step > 0; step >>= 1) {
-                    const uint32_t probe = q + step;
-                    const uint32_t off = __shfl(roff, (int)min(probe, 63u), 64);
-                    if (probe < nvalid && off <= g) q = probe;
+                for (uint32_t k = 0; k < kListChunks; k++) {
+                    pj[k] = make_float4(__builtin_inff(), __builtin_inff(), __builtin_inff(), 0.0f);
+                    ej[k] = 0;
+                    if (c0 + k * 64 < total) {
+                        // run q of candidate g by binary search over the offsets
+                        // (uniform: every lane takes part in each shuffle)
+                        const uint32_t g = c0 + k * 64 + lane;
+                        uint32_t q = 0;
+#pragma unroll
+                        for (uint32_t step = 16; step > 0; step >>= 1) {
+                            const uint32_t probe = q + step;
+                            const uint32_t off = __shfl(roff, (int)min(probe, 31u), 64);
+                            if (probe < nvalid && off <= g) q = probe;
+                        }
+                        const uint32_t qs = __shfl(rs, (int)q, 64);
+                        const uint32_t qo = __shfl(roff, (int)q, 64);
+                        if (g < total) {
+                            const uint32_t j = qs + (g - qo);
+                            pj[k] = pos[j];
+                            ej[k] = j | ((idt[j] >> kTypeShift) == 1u ? kBoundaryBit : 0u);
+                        }
+                    }
                 }
-                const uint32_t qs = __shfl(rs, (int)q, 64);
-                const uint32_t qo = __shfl(roff, (int)q, 64);
-                const bool valid = g < total;
-                // a lane past the last candidate sits at infinity and fails every test
-                float4 pj = make_float4(__builtin_inff(), __builtin_inff(), __builtin_inff(), 0.0f);
-                uint32_t ej = 0;
-                if (valid) {
-                    const uint32_t j = qs + (g - qo);
-                    pj = pos[j];
-                    ej = j | ((idt[j] >> kTypeShift) == 1u ? kBoundaryBit : 0u);
-                }
+                const uint32_t left = total - c0;
                 uint64_t fm = fmask;
                 while (fm) {
                     const uint32_t b = (uint32_t)__ffsll((unsigned long long)fm) - 1u;
                     fm &= fm - 1;
                     const float xs = readlane_f(xl.x, b), ys = readlane_f(xl.y, b), zs = readlane_f(xl.z, b);
-                    const float dx = pj.x - xs, dy = pj.y - ys, dz = pj.z - zs;
-                    const bool pass = dot3(dx, dy, dz, dx, dy, dz) <= r2;
-                    const uint64_t m = __ballot(pass);
-                    const uint32_t cb = readlane_u(cntl, b);
-                    if (pass) {
-                        // past the capacity the last row is overwritten; the count reports the overflow
-                        const uint32_t slot = cb + __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
-                                                                             __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
-                        nbr[min(slot, (uint32_t)kMaxNeighbors - 1u) * stride + readlane_u(kl, b)] = ej;
+                    uint32_t cnt = readlane_u(cntl, b);  // the particle's running count (uniform)
+                    const uint32_t kb = readlane_u(kl, b);
+                    FSA_GLOBAL(uint32_t)* const row = nbr + kb;
+#pragma unroll
+                    for (uint32_t k = 0; k < kListChunks; k++) {
+                        if (k * 64 < left) {
+                            const float dx = pj[k].x - xs, dy = pj[k].y - ys, dz = pj[k].z - zs;
+                            const bool pass = dot3(dx, dy, dz, dx, dy, dz) <= r2;
+                            const uint64_t m = __ballot(pass);
+                            if (pass) {
+                                // past the capacity the last row is overwritten; the count reports the overflow
+                                const uint32_t slot =
+                                    min(__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
+                                                                  __builtin_amdgcn_mbcnt_lo((uint32_t)m, cnt)),
+                                        (uint32_t)kMaxNeighbors - 1u);
+                                if (Wide) {
+                                    nbr[slot * stride + kb] = ej[k];
+                                } else {
+                                    const uint32_t off = mul_u24(slot, stride) << 2;
+                                    *(FSA_GLOBAL(uint32_t)*)((FSA_GLOBAL(char)*)row + off) = ej[k];
+                                }
+                            }
+                            cnt += (uint32_t)__popcll(m);
+                        }
                     }
-                    if (lane == b) cntl = cb + (uint32_t)__popcll(m);
+                    if (lane == b) cntl = cnt;
                 }
             }
             if (isf) {
@@ -691,7 +747,12 @@ void launch_reorder(const LaunchSet& L, uint32_t sbuf, hipStream_t st) {
 }
 
 void launch_lists(const LaunchSet& L, uint32_t sbuf, hipStream_t st) {
-    if (L.nb_all) k_lists<<<L.nb_all, kBlock, 0, st>>>(L.scenes, L.map_all, sbuf);
+    if (L.nb_all) {
+        if (L.wide_lists)
+            k_lists<true><<<L.nb_all, kBlock, 0, st>>>(L.scenes, L.map_all, sbuf);
+        else
+            k_lists<false><<<L.nb_all, kBlock, 0, st>>>(L.scenes, L.map_all, sbuf);
+    }
     if (L.codec) launch_codec(L, st);
 }
 
