@@ -47,6 +47,24 @@ def algorithmic_bytes(nf, nba, L):
     }
 
 
+TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "iisph_dam3d_1M_pmc_traffic.md")
+
+
+def measured_traffic(phase, side):
+    """HBM bytes per launch of a phase at the 1M configuration, from the
+    committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary (scaled by the counters'
+    reading of a 1 GiB copy); None for other sizes or without the summary."""
+    if side != 100 or not os.path.exists(TRAFFIC_SUMMARY):
+        return None
+    with open(TRAFFIC_SUMMARY) as fh:
+        for line in fh:
+            line = line.strip()
+            if line.startswith("traffic-per-launch "):
+                fields = dict(kv.split("=", 1) for kv in line.split()[1:])
+                return int(fields[phase]) if phase in fields else None
+    return None
+
+
 def cpu_baseline(desc, coll, steps, threads):
     from oracle import oracle as O
 
@@ -301,7 +319,8 @@ def main():
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBS, 4),
-        "traffic": None,
+        "traffic": measured_traffic(dom, side),
+        "traffic_source": "profiles/iisph_dam3d_1M_pmc_traffic.md (rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0)",
         "algorithmic_bytes_per_launch": int(algo[dom]),
         "avg_launch_ms": phases[dom]["avg_ms"],
     }

@@ -192,24 +192,26 @@ __device__ __forceinline__ float sqrt_rn_pair(float x) {
     return r;
 }
 
-// Reciprocal of b refined by one Newton step, for quotients sharing b.
+// Reciprocal of b refined by one Newton step, for quotients sharing b.  For b
+// in [2^-23, 2^2) this is the correctly rounded 1 / b (checked on the device for
+// every mantissa at every exponent of that range).
 __device__ __forceinline__ float recip_pair(float b) {
     const float y0 = __builtin_amdgcn_rcpf(b);
     const float e = __builtin_fmaf(-b, y0, 1.0f);
     return __builtin_fmaf(e, y0, y0);
 }
 
-// a / b from the refined reciprocal y of b: two FMA corrections, the same
-// operations the IEEE division expansion performs when its operand scaling is
-// the identity, which holds for 2^-23 < b <= 2^22 and a = 0 or 2^-59 <= |a| <= b.
-// The sign of a zero quotient is restored from a.
+// a / b from y = RN(1 / b) with one FMA correction.  With q0 = RN(a * y) within
+// one ulp of a / b, the residual a - b * q0 is exact in an FMA and RN(q0 + r * y)
+// is the correctly rounded quotient (Markstein's theorem; Muller et al.,
+// Handbook of Floating-Point Arithmetic, on division with a correctly rounded
+// reciprocal).  It needs no underflow in the residual: a = 0 or
+// 2^-59 <= |a| <= b, with 2^-23 < b <= 2.  The sign of a zero quotient is
+// restored from a.
 __device__ __forceinline__ float div_pair(float a, float b, float y) {
     const float q0 = a * y;
     const float r0 = __builtin_fmaf(-b, q0, a);
-    const float q1 = __builtin_fmaf(r0, y, q0);
-    const float r1 = __builtin_fmaf(-b, q1, a);
-    const float q = __builtin_fmaf(r1, y, q1);
-    return __builtin_copysignf(q, a);
+    return __builtin_copysignf(__builtin_fmaf(r0, y, q0), a);
 }
 
 // Compile-time specialisation of the two parameters that make operations
@@ -243,7 +245,9 @@ __device__ __forceinline__ float kernel_w(const SceneDev& S, float len) {
     const float a = 2.0f - q, b = 1.0f - q;
     const float a3 = a * a * a;
     const float inner = a3 - 4.0f * (b * b * b);
-    const float ret = q < 1.0f ? inner : (q < 2.0f ? a3 : 0.0f);
+    // listed pairs have q <= 2 when h == 1 (|d|^2 <= 4), and at q == 2 the
+    // cubic branch is (2 - 2)^3 = +0, the reference's zero
+    const float ret = (Spec & kSpecFastMath) ? (q < 1.0f ? inner : a3) : (q < 1.0f ? inner : (q < 2.0f ? a3 : 0.0f));
     return S.alpha * ret;
 }
 
@@ -279,7 +283,11 @@ __device__ __forceinline__ void kernel_grad(const SceneDev& S, float dx, float d
     const float a = 2.0f - q, b = 1.0f - q;
     const float m3a2 = -3.0f * (a * a);
     const float s = q < 1.0f ? m3a2 + 12.0f * (b * b) : m3a2;
-    const bool inside = q < 2.0f;
+    // With h == 1 a listed pair has q <= 2; at q == 2 the product below is a
+    // signed zero where the reference has +0.  Every consumer adds the gradient
+    // into a sum started at +0 (acc = acc + t or acc - t, denormals kept), which
+    // is never -0, so the sign of a zero term cannot reach a result.
+    const bool inside = (Spec & kSpecFastMath) ? true : q < 2.0f;
     gx = inside ? S.alpha * (px * s) : 0.0f;
     gy = inside ? S.alpha * (py * s) : 0.0f;
     gz = inside ? S.alpha * (pz * s) : 0.0f;

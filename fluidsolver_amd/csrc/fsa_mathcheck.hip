@@ -2,6 +2,7 @@
 // operations (a test library: only tests/test_gpu_math.py loads it).
 //   fsa_mc_sqrt:   every non-negative float below +inf; mismatches in [2^-96, inf)
 //                  and results outside [0, 2^-40] below 2^-96
+//   fsa_mc_recip:  recip_pair against the IEEE 1 / b, every mantissa at the given exponents
 //   fsa_mc_div:    every divisor mantissa in [1, 2) at the given exponents, against
 //                  numerators drawn from a hash (zeros, tiny, extreme and random)
 #include <hip/hip_runtime.h>
@@ -79,7 +80,34 @@ __global__ void k_mc_div(int exp_lo, int exp_hi, uint32_t per_b, uint64_t seed, 
     atomicAdd(tested, t);
 }
 
+// mismatches of recip_pair against the IEEE 1 / b, every mantissa of each exponent
+__global__ void k_mc_recip(int exp_lo, int exp_hi, unsigned long long* mism) {
+    const uint32_t nmant = 1u << 23;
+    const uint64_t total = (uint64_t)nmant * (uint64_t)(exp_hi - exp_lo + 1);
+    unsigned long long m = 0;
+    for (uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; g < total; g += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t mant = (uint32_t)(g % nmant);
+        const int ex = exp_lo + (int)(g / nmant);
+        const float b = __uint_as_float(((uint32_t)(ex + 127) << 23) | mant);
+        if (__float_as_uint(recip_pair(b)) != __float_as_uint(1.0f / b)) m++;
+    }
+    if (m) atomicAdd(mism, m);
+}
+
 }  // namespace
+
+extern "C" __attribute__((visibility("default"))) int fsa_mc_recip(int exp_lo, int exp_hi, uint64_t* mism) {
+    if (exp_lo < -23 || exp_hi > 21 || exp_lo > exp_hi) return 3;
+    unsigned long long* d = nullptr;
+    if (hipMalloc(&d, 8) != hipSuccess) return 1;
+    (void)hipMemset(d, 0, 8);
+    k_mc_recip<<<8192, 256>>>(exp_lo, exp_hi, d);
+    unsigned long long h = 0;
+    const hipError_t e = hipMemcpy(&h, d, 8, hipMemcpyDeviceToHost);
+    (void)hipFree(d);
+    *mism = h;
+    return e == hipSuccess ? 0 : 2;
+}
 
 extern "C" __attribute__((visibility("default"))) int fsa_mc_sqrt(uint64_t* mism, uint64_t* bad_small) {
     unsigned long long* d = nullptr;

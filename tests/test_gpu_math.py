@@ -3,8 +3,11 @@
 sqrt_rn_pair: every non-negative finite float; bitwise equal to the correctly
 rounded square root from 2^-96 up, and within [0, 2^-40] below (where a length
 feeds only q < 2^-25 and a zero gradient).
-div_pair: every divisor mantissa in [1, 2) at every exponent of 2^-23 .. 2^21
-(divisors up to 2^22), against 16 numerators each: signed zeros, +-b, values
+recip_pair: every mantissa at every exponent of 2^-23 .. 2^1; bitwise equal to
+the IEEE 1 / b.  This is the precondition under which div_pair's single FMA
+correction gives the correctly rounded quotient (Markstein's theorem).
+div_pair: every divisor mantissa in [1, 2) at every exponent of 2^-23 .. 2^1
+(divisors in [2^-23, 4)), against 16 numerators each: signed zeros, +-b, values
 near 2^-59 and random magnitudes in [2^-59, b]; bitwise equal to IEEE a / b."""
 import ctypes as C
 import os
@@ -22,6 +25,8 @@ def _lib():
     L = C.CDLL(fbuild.MATHCHECK_LIB)
     L.fsa_mc_sqrt.restype = C.c_int
     L.fsa_mc_sqrt.argtypes = [C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]
+    L.fsa_mc_recip.restype = C.c_int
+    L.fsa_mc_recip.argtypes = [C.c_int, C.c_int, C.POINTER(C.c_uint64)]
     L.fsa_mc_div.restype = C.c_int
     L.fsa_mc_div.argtypes = [C.c_int, C.c_int, C.c_uint32, C.c_uint64, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]
     return L
@@ -39,6 +44,14 @@ def test_div_pair_all_divisor_mantissas():
     L = _lib()
     m, t = C.c_uint64(0), C.c_uint64(0)
     print("case div mantissas", flush=True)
-    assert L.fsa_mc_div(-23, 21, 16, 12345, C.byref(m), C.byref(t)) == 0
-    assert t.value == (1 << 23) * 45 * 16
+    assert L.fsa_mc_div(-23, 1, 16, 12345, C.byref(m), C.byref(t)) == 0
+    assert t.value == (1 << 23) * 25 * 16
+    assert m.value == 0
+
+
+def test_recip_pair_correctly_rounded():
+    L = _lib()
+    m = C.c_uint64(0)
+    print("case recip mantissas", flush=True)
+    assert L.fsa_mc_recip(-23, 1, C.byref(m)) == 0
     assert m.value == 0
