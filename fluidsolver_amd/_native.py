@@ -7,7 +7,7 @@ import ctypes as C
 import os
 import re
 
-from .abi import JobInfo, ParticleView, SceneDesc, ServerConfig, ServerStats, StepStats
+from .abi import DataHeader, JobInfo, ParticleView, Scenario, SceneDesc, ServerConfig, ServerStats, StepStats
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "lib", "libfluidsolver_amd.so")
@@ -44,6 +44,7 @@ def _declare(L):
         "fsa_scene_neighbor_stats": (i32, [vp, P(u64), P(u64), P(u64)]),
         "fsa_scene_simulate": (i32, [vp, f32, P(StepStats)]),
         "fsa_scene_step": (i32, [vp, u32, P(StepStats)]),
+        "fsa_scene_next_timestep": (i32, [vp, P(f32)]),
         "fsa_batch_step": (i32, [vp, P(vp), u32, u32, P(StepStats)]),
         "fsa_batch_set_graph": (i32, [vp, u32]),
         "fsa_context_set_block_size": (i32, [vp, u32]),
@@ -59,6 +60,15 @@ def _declare(L):
         "fsa_server_job": (i32, [vp, u64, P(JobInfo)]),
         "fsa_server_collect": (i32, [vp, u64, P(ParticleView)]),
         "fsa_server_get_stats": (i32, [vp, P(ServerStats)]),
+        "fsa_data_read_header": (i32, [C.c_char_p, P(DataHeader)]),
+        "fsa_data_read": (i32, [C.c_char_p, P(ParticleView)]),
+        "fsa_data_write": (i32, [C.c_char_p, u32, P(ParticleView)]),
+        "fsa_scenario_read": (i32, [C.c_char_p, P(Scenario)]),
+        "fsa_scenario_write": (i32, [C.c_char_p, P(Scenario)]),
+        "fsa_scenario_defaults": (None, [P(Scenario), u32, u32]),
+        "fsa_lz4_frame_decode": (i32, [vp, u64, vp, u64, P(u64)]),
+        "fsa_lz4_frame_encode": (i32, [vp, u64, vp, u64, P(u64)]),
+        "fsa_xxh32": (u32, [vp, u64, u32]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)

@@ -14,6 +14,7 @@ FSA_E_CAPACITY = 4
 FSA_E_STATE = 5
 FSA_E_BUDGET = 6
 FSA_E_NONFINITE = 7
+FSA_E_IO = 8
 
 STATUS_NAMES = {
     FSA_OK: "FSA_OK",
@@ -24,6 +25,7 @@ STATUS_NAMES = {
     FSA_E_STATE: "FSA_E_STATE",
     FSA_E_BUDGET: "FSA_E_BUDGET",
     FSA_E_NONFINITE: "FSA_E_NONFINITE",
+    FSA_E_IO: "FSA_E_IO",
 }
 
 SOLVER_IISPH = 0
@@ -36,6 +38,8 @@ SEARCH_HASHED_3D = 2
 SEARCH_QUADRATIC_3D = 3
 SEARCH_HASHED_2D = 4
 SEARCH_QUADRATIC_2D = 5
+TIMESTEP_CONSTANT = 0
+TIMESTEP_DYNAMIC_CFL = 1
 TYPE_NORMAL = 0
 TYPE_BOUNDARY = 1
 TYPE_INACTIVE = 2
@@ -81,13 +85,16 @@ class SceneDesc(C.Structure):
         ("lambda_v", C.c_float),
         ("lambda_a", C.c_float),
         ("search", C.c_uint32),
-        ("reserved", C.c_uint32 * 4),
+        ("timestep_generator", C.c_uint32),
+        ("max_timestep", C.c_float),
+        ("min_timestep", C.c_float),
+        ("cfl_number", C.c_float),
     ]
 
 
 def default_desc(dimension=3, solver=SOLVER_IISPH, **overrides):
     """Reference defaults (IISPHSettings3D.hpp:7-17, SESPHSettings3D.hpp:7-12,
-    ConstantTimestepGenerator.hpp:9-12) with Simulator's overrides
+    ConstantTimestepGenerator.hpp:9-12, DynamicCflTimestepGenerator.hpp:10-17) with Simulator's overrides
     particle_size = 1, rest_density = 1 (Simulator.hpp:33-35)."""
     d = SceneDesc()
     d.dimension = dimension
@@ -114,6 +121,10 @@ def default_desc(dimension=3, solver=SOLVER_IISPH, **overrides):
     d.lambda_v = 0.4
     d.lambda_a = 0.25
     d.search = SEARCH_HASHED_2D if dimension == 2 else SEARCH_COMPRESSED_3D
+    d.timestep_generator = TIMESTEP_CONSTANT
+    d.max_timestep = 0.02  # DynamicCflTimestepGenerator.hpp:10-17
+    d.min_timestep = 0.0001
+    d.cfl_number = 0.8
     for k, v in overrides.items():
         setattr(d, k, v)
     return d
@@ -191,6 +202,38 @@ class ServerStats(C.Structure):
 
     def as_dict(self):
         return {f[0]: getattr(self, f[0]) for f in self._fields_}
+
+
+# scenario files
+COMPONENT_MOVEMENT_2D = 1
+COMPONENT_MOVEMENT_3D = 2
+COMPONENT_INFO = 3
+COMPONENT_PARTICLE_DATA = 4
+COMPONENT_EXTERNAL_2D = 5
+COMPONENT_EXTERNAL_3D = 6
+
+
+class DataHeader(C.Structure):
+    _fields_ = [
+        ("version", C.c_uint32),
+        ("dimension", C.c_uint32),
+        ("components", C.c_uint32),
+        ("reserved", C.c_uint32),
+        ("count", C.c_uint64),
+    ]
+
+    def component_ids(self):
+        return [k + 1 for k in range(6) if self.components & (1 << k)]
+
+
+class Scenario(C.Structure):
+    _fields_ = [
+        ("desc", SceneDesc),
+        ("entities", C.c_uint32),
+        ("sensors", C.c_uint32),
+        ("particles", C.c_char * 1024),
+        ("particles_path", C.c_char * 4096),
+    ]
 
 
 # numpy records with the reference component layouts (Particle.hpp:15-50)

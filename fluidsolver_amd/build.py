@@ -16,6 +16,8 @@ LIB = os.path.join(LIBDIR, "libfluidsolver_amd.so")
 OBJDIR = os.path.join(LIBDIR, "obj")
 ARCH = os.environ.get("FSA_OFFLOAD_ARCH", "gfx950")
 SOURCES = ["fsa_search.hip", "fsa_sph.hip", "fsa_sph2d.hip", "fsa_api.hip"]
+HOST_SOURCES = ["fsa_io.cpp"]  # host-only C++ (scenario files), built with g++
+HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-fvisibility=hidden", "-Wall", "-Wextra", "-Wno-unused-parameter"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [
@@ -45,8 +47,20 @@ def build(verbose=False):
             print(r.stderr, file=sys.stderr)
         return obj
 
-    with ThreadPoolExecutor(max_workers=len(SOURCES)) as ex:
-        objs = list(ex.map(compile_one, SOURCES))
+    def compile_host(src):
+        obj = os.path.join(OBJDIR, src.replace(".cpp", ".o"))
+        cmd = ["g++"] + HOST_FLAGS + ["-c", os.path.join(CSRC, src), "-o", obj]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"g++ failed for {src}:\n{r.stdout}\n{r.stderr}")
+        if verbose and r.stderr.strip():
+            print(r.stderr, file=sys.stderr)
+        return obj
+
+    with ThreadPoolExecutor(max_workers=len(SOURCES) + len(HOST_SOURCES)) as ex:
+        dev = [ex.submit(compile_one, s) for s in SOURCES]
+        host = [ex.submit(compile_host, s) for s in HOST_SOURCES]
+        objs = [f.result() for f in dev] + [f.result() for f in host]
     tmp = LIB + ".tmp"
     cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp] + objs
     r = subprocess.run(cmd, capture_output=True, text=True)

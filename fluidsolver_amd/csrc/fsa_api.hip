@@ -12,6 +12,7 @@
 #include <chrono>
 #include <deque>
 #include <cmath>
+#include <limits>
 #include <cstdio>
 #include <cstring>
 #include <exception>
@@ -25,18 +26,16 @@
 
 #include "../../include/fluidsolver_amd.h"
 #include "fsa_device.hpp"
+#include "fsa_host.hpp"
 #include "fsa_kernels.hpp"
+
+namespace fsa {
+thread_local std::string g_error;
+}  // namespace fsa
 
 using namespace fsa;
 
 namespace {
-
-thread_local std::string g_error;
-
-struct FsaError : std::runtime_error {
-    int32_t code;
-    FsaError(int32_t c, const std::string& m) : std::runtime_error(m), code(c) {}
-};
 
 #define HIP_CHECK(expr)                                                                                  \
     do {                                                                                                 \
@@ -44,26 +43,6 @@ struct FsaError : std::runtime_error {
         if (_e != hipSuccess)                                                                            \
             throw FsaError(FSA_E_DEVICE, std::string(#expr) + ": " + hipGetErrorString(_e));           \
     } while (0)
-
-template <class F>
-int32_t guarded(F&& f) {
-    try {
-        f();
-        return FSA_OK;
-    } catch (const FsaError& e) {
-        g_error = e.what();
-        return e.code;
-    } catch (const std::bad_alloc&) {
-        g_error = "host allocation failed";
-        return FSA_E_NOMEM;
-    } catch (const std::exception& e) {
-        g_error = e.what();
-        return FSA_E_INVALID;
-    } catch (...) {
-        g_error = "unknown error";
-        return FSA_E_INVALID;
-    }
-}
 
 uint64_t align_up(uint64_t v, uint64_t a) { return (v + a - 1) / a * a; }
 
@@ -152,6 +131,7 @@ struct fsa_scene_s {
     bool lists_cached = false;
     bool lists_valid = false;  // lists of the last search (valid until the next search or upload)
     std::vector<uint64_t> csr_off, csr_cols;
+    float cur_v2 = 0.0f, cur_a2 = 0.0f;  // max |v|^2, |a|^2 of the active particles' current state
 };
 
 namespace {
@@ -189,7 +169,9 @@ void fill_params(fsa_scene_s* s) {
     h.min_it = d.iisph_min_iterations;
     h.max_it = d.iisph_max_iterations;
     h.stiffness = d.sesph_stiffness_k;
-    h.gen_dt = d.timestep;
+    // fallback of get_non_cfl_validating_timestep: the generator's own step
+    // (ConstantTimestepGenerator.cpp:28-29; DynamicCflTimestepGenerator.cpp:96-97)
+    h.gen_dt = d.timestep_generator == FSA_TIMESTEP_DYNAMIC_CFL ? d.max_timestep : d.timestep;
     h.dt = d.timestep;
     h.lambda_a = d.lambda_a;
     h.lambda_v = d.lambda_v;
@@ -221,6 +203,10 @@ void validate_desc(const fsa_scene_desc* d) {
     if (!(d->particle_size > 0.0f)) throw FsaError(FSA_E_INVALID, "particle_size must be > 0");
     if (!(d->rest_density > 0.0f)) throw FsaError(FSA_E_INVALID, "rest_density must be > 0");
     if (!(d->timestep > 0.0f)) throw FsaError(FSA_E_INVALID, "timestep must be > 0");
+    if (d->timestep_generator != FSA_TIMESTEP_CONSTANT && d->timestep_generator != FSA_TIMESTEP_DYNAMIC_CFL)
+        throw FsaError(FSA_E_INVALID, "unknown timestep generator");
+    if (d->timestep_generator == FSA_TIMESTEP_DYNAMIC_CFL && !(d->max_timestep > 0.0f && d->min_timestep > 0.0f))
+        throw FsaError(FSA_E_INVALID, "dynamic CFL time steps must be > 0");
     if (d->solver == FSA_SOLVER_IISPH) {
         if (d->iisph_max_iterations < d->iisph_min_iterations)
             throw FsaError(FSA_E_INVALID, "Max iterations are less than min number of iterations.");
@@ -795,6 +781,27 @@ void pre_simulate(fsa_scene_s* s, float dt) {
     set_step_dt(s, dt);
 }
 
+// The desired time step of the next step (Simulator.cpp:17-24):
+// ConstantTimestepGenerator: desc.timestep.  DynamicCflTimestepGenerator::generate_next_timestep
+// (DynamicCflTimestepGenerator.cpp:46-62): from the largest |v| and |a| of the
+// active particles, with get_non_cfl_validating_timestep (:84-109) falling
+// back to max_timestep; sqrt of a float there is ::sqrt(double).
+float generated_timestep(const fsa_scene_s* s) {
+    const fsa_scene_desc& d = s->desc;
+    if (d.timestep_generator != FSA_TIMESTEP_DYNAMIC_CFL) return d.timestep;
+    const float mv = std::sqrt(s->cur_v2), ma = std::sqrt(s->cur_a2);
+    float ts = d.min_timestep;
+    const float eps = std::numeric_limits<float>::epsilon();
+    if (mv > eps && ma > eps) {
+        float dta = d.max_timestep, dtv = d.max_timestep;
+        if (ma > 0.0f) dta = (float)(std::sqrt((double)(d.particle_size / ma)) * (double)d.lambda_a);
+        if (mv > 0.0f) dtv = d.particle_size / mv * d.lambda_v;
+        const float allowed = std::fmax(std::fmin(dta, dtv), 0.000001f);  // MIN_ALLOWED_TIMESTEP
+        ts = std::max(ts, std::min(d.max_timestep, allowed));
+    }
+    return ts;
+}
+
 // time step of the coming solver step, and the SESPH CFL share of inactive particles
 void set_step_dt(fsa_scene_s* s, float dt) {
     s->hd.dt = dt;
@@ -816,6 +823,8 @@ void set_step_dt(fsa_scene_s* s, float dt) {
 // after the step's final state read
 void post_simulate(fsa_scene_s* s, fsa_step_stats* stats) {
     s->boundary_dirty = false;
+    s->cur_v2 = s->h_state->next_v2;
+    s->cur_a2 = s->h_state->next_a2;
     if (s->h_state->status & kStatusTinyCoord) s->hd.flags |= kFlagTinyCoords;
     if (s->desc.solver == FSA_SOLVER_IISPH) s->last_iterations = std::max<uint32_t>(s->h_state->iterations, 1);
     fill_stats(s, stats);
@@ -1108,7 +1117,7 @@ int32_t batch_step_group(fsa_context ctx, const std::vector<fsa_scene_s*>& v, st
         } else {
             passes = std::max(passes, sort_passes(s->hd));
         }
-        set_step_dt(s, s->desc.timestep);
+        set_step_dt(s, generated_timestep(s));
         bpass = bpass || (s->hd.flags & kFlagDynamicBoundary) || s->boundary_dirty;
         max_it = std::max(max_it, s->hd.max_it);
         guess = std::max(guess, std::max(s->hd.min_it, s->last_iterations));
@@ -1304,8 +1313,28 @@ void do_upload2d(fsa_scene_s* s, const fsa_particle_view* v, bool first) {
     s->lists_valid = false;
 }
 
+// DynamicCflTimestepGenerator::calculate_maximum_velocity_and_acceleration
+// (DynamicCflTimestepGenerator.cpp:6-43) over an uploaded view, as squared
+// lengths: sqrt is correctly rounded and monotone, so the square root of the
+// largest squared length is the largest (float)glm::length.
+void upload_maxima(fsa_scene_s* s, const fsa_particle_view* v) {
+    float v2 = 0.0f, a2 = 0.0f;
+    const bool d3 = s->desc.dimension == 3;
+    for (uint64_t i = 0; i < v->count; i++) {
+        if (((const uint8_t*)v->info + i * v->info_stride)[4] == FSA_TYPE_INACTIVE) continue;
+        const float* m = (const float*)((const char*)v->movement + i * v->movement_stride);
+        const float vv = d3 ? (m[3] * m[3] + m[4] * m[4]) + m[5] * m[5] : m[2] * m[2] + m[3] * m[3];
+        const float aa = d3 ? (m[6] * m[6] + m[7] * m[7]) + m[8] * m[8] : m[4] * m[4] + m[5] * m[5];
+        v2 = std::fmax(v2, vv);
+        a2 = std::fmax(a2, aa);
+    }
+    s->cur_v2 = v2;
+    s->cur_a2 = a2;
+}
+
 void do_upload(fsa_scene_s* s, const fsa_particle_view* v, bool first) {
     if (!v || !v->movement || !v->particle_data || !v->info) throw FsaError(FSA_E_INVALID, "incomplete particle view");
+    upload_maxima(s, v);
     if (s->desc.dimension == 2) {
         do_upload2d(s, v, first);
         return;
@@ -1808,7 +1837,7 @@ extern "C" {
 
 FSA_API int32_t fsa_version(void) { return 1; }
 
-FSA_API const char* fsa_last_error(void) { return g_error.c_str(); }
+FSA_API const char* fsa_last_error(void) { return fsa::g_error.c_str(); }
 
 FSA_API void fsa_scene_desc_defaults(fsa_scene_desc* d, uint32_t dimension, uint32_t solver) {
     if (!d) return;
@@ -1837,6 +1866,11 @@ FSA_API void fsa_scene_desc_defaults(fsa_scene_desc* d, uint32_t dimension, uint
     d->lambda_v = 0.4f;
     d->lambda_a = 0.25f;
     d->search = dimension == 2 ? FSA_SEARCH_HASHED_2D : FSA_SEARCH_COMPRESSED_3D;
+    d->timestep_generator = FSA_TIMESTEP_CONSTANT;
+    // DynamicCflTimestepGenerator::DynamicCflTimestepGeneratorSettings (DynamicCflTimestepGenerator.hpp:10-17)
+    d->max_timestep = 0.02f;
+    d->min_timestep = 0.0001f;
+    d->cfl_number = 0.8f;
 }
 
 FSA_API int32_t fsa_context_create(int32_t device, uint64_t budget, fsa_context* out) {
@@ -2068,6 +2102,13 @@ FSA_API int32_t fsa_scene_simulate(fsa_scene s, float dt, fsa_step_stats* stats)
     return guarded([&] {
         if (!s) throw FsaError(FSA_E_INVALID, "scene is NULL");
         do_simulate(s, dt, stats);
+    });
+}
+
+FSA_API int32_t fsa_scene_next_timestep(fsa_scene s, float* dt) {
+    return guarded([&] {
+        if (!s || !dt) throw FsaError(FSA_E_INVALID, "NULL argument");
+        *dt = generated_timestep(s);
     });
 }
 

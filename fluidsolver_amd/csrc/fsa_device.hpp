@@ -69,10 +69,30 @@ struct alignas(16) SceneState {
     float dt_actual, max_a, max_v;
     uint32_t err_count;
     int32_t bbox_min[4], bbox_max[4];  // cell bounding box after the last integration
-    uint32_t pad0, pad1;
+    float next_v2, next_a2;  // after integration: max |v|^2, |a|^2 over the particles that move
     uint64_t total_neighbors;
     uint64_t codec_bytes;  // NeighborStorage encoded bytes of all lists of the last search
 };
+
+// max of (v2, a2) over a kBlock-thread block, NaN ignored; every thread gets the result
+__device__ __forceinline__ void block_max2(float& v2, float& a2, float* lds /* 2 * kBlock / 64 */) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        v2 = fmaxf(v2, __shfl_xor(v2, o, 64));
+        a2 = fmaxf(a2, __shfl_xor(a2, o, 64));
+    }
+    __syncthreads();
+    if (__lane_id() == 0) {
+        lds[threadIdx.x / 64] = v2;
+        lds[kBlock / 64 + threadIdx.x / 64] = a2;
+    }
+    __syncthreads();
+    v2 = a2 = 0.0f;
+    for (int w = 0; w < kBlock / 64; w++) {
+        v2 = fmaxf(v2, lds[w]);
+        a2 = fmaxf(a2, lds[kBlock / 64 + w]);
+    }
+}
 
 struct alignas(16) Partial {
     double err;

@@ -544,6 +544,26 @@ __global__ __launch_bounds__(kBlock) void k_cfl_finalize(const SceneDev* __restr
 }
 
 // ---- CFL correction + Euler-Cromer integration ------------------------------------------
+// DynamicCflTimestepGenerator::calculate_maximum_velocity_and_acceleration
+// (DynamicCflTimestepGenerator.cpp:6-43) over the state the integration left:
+// the per-block maxima of |v|^2 and |a|^2 (BoxPartial lo[3] / hi[3]) reduced
+// into the scene state for the host's next time step.  sqrt is correctly
+// rounded and monotone, so sqrt of the maximum equals the maximum length.
+__device__ void reduce_next_maxima(const SceneDev& S) {
+    __shared__ float lmax[2 * kBlock / 64];
+    float v2 = 0.0f, a2 = 0.0f;
+    for (uint32_t b = threadIdx.x; b < S.nblocks; b += kBlock) {
+        const BoxPartial bp = S.box[b];
+        v2 = fmaxf(v2, __int_as_float(bp.lo[3]));
+        a2 = fmaxf(a2, __int_as_float(bp.hi[3]));
+    }
+    block_max2(v2, a2, lmax);
+    if (threadIdx.x == 0) {
+        S.state->next_v2 = v2;
+        S.state->next_a2 = a2;
+    }
+}
+
 __global__ __launch_bounds__(kBlock) void k_integrate(const SceneDev* __restrict__ scenes,
                                                       const BlockRef* __restrict__ map) {
     const BlockRef br = map[blockIdx.x];
@@ -557,6 +577,7 @@ __global__ __launch_bounds__(kBlock) void k_integrate(const SceneDev* __restrict
     const float corrected = fminf(cfl_timestep(S, max_a, max_v), S.dt);
     const bool sesph = (S.flags & kFlagSesph) != 0;
     int32_t lo[3] = {INT32_MAX, INT32_MAX, INT32_MAX}, hi[3] = {INT32_MIN, INT32_MIN, INT32_MIN};
+    float nv2 = 0.0f, na2 = 0.0f;  // |v|^2, |a|^2 of the new state (static boundary: 0)
     if (i < S.n) {
         const uint32_t type = S.idt[c][i] >> kTypeShift;
         float4 x = S.pos[c][i];
@@ -590,6 +611,8 @@ __global__ __launch_bounds__(kBlock) void k_integrate(const SceneDev* __restrict
             if ((ax_ > 0.0f && ax_ < kTinyCoord) || (ay_ > 0.0f && ay_ < kTinyCoord) || (az_ > 0.0f && az_ < kTinyCoord))
                 atomicOr(&S.state->status, kStatusTinyCoord);
             S.vel[c][i] = make_float4(vx, vy, vz, v.w);
+            nv2 = dot3(vx, vy, vz, vx, vy, vz);
+            na2 = dot3(a.x, a.y, a.z, a.x, a.y, a.z);
         }
         lo[0] = hi[0] = cell_of(x.x, S.search_radius);
         lo[1] = hi[1] = cell_of(x.y, S.search_radius);
@@ -622,6 +645,14 @@ __global__ __launch_bounds__(kBlock) void k_integrate(const SceneDev* __restrict
         bp.lo[3] = 0;
         bp.hi[3] = 0;
         S.box[br.block] = bp;
+    }
+    {  // max |v|^2, |a|^2 of the new state for the dynamic CFL time step
+        __shared__ float lmax[2 * kBlock / 64];
+        block_max2(nv2, na2, lmax);
+        if (threadIdx.x == 0) {
+            S.box[br.block].lo[3] = __float_as_int(nv2);
+            S.box[br.block].hi[3] = __float_as_int(na2);
+        }
     }
 }
 
@@ -669,6 +700,7 @@ __global__ __launch_bounds__(kBlock) void k_integrate_finalize(const SceneDev* _
         st->max_v = max_v;
         st->max_a = max_a;
     }
+    reduce_next_maxima(S);
 }
 
 // ---- launchers ------------------------------------------------------------------------

@@ -574,6 +574,7 @@ __global__ __launch_bounds__(kBlock) void k2_integrate(const SceneDev* __restric
     __shared__ int32_t blo[2][kBlock / 64], bhi[2][kBlock / 64];
     const uint32_t i = br.block * kBlock + threadIdx.x;
     int32_t lo[2] = {INT32_MAX, INT32_MAX}, hi[2] = {INT32_MIN, INT32_MIN};
+    float nv2 = 0.0f, na2 = 0.0f;  // |v|^2, |a|^2 of the new state (inactive: excluded)
     if (i < S.n) {
         const uint32_t type = type_of(S, i);
         float4 x = S.pos[0][i];
@@ -598,6 +599,8 @@ __global__ __launch_bounds__(kBlock) void k2_integrate(const SceneDev* __restric
             x.z = x.z + dt * vz;
             S.pos[0][i] = x;
             S.vel[0][i] = make_float4(vx, vy, vz, v.w);
+            nv2 = vx * vx + vy * vy;  // glm::length of a vec2
+            na2 = a.x * a.x + a.y * a.y;
         }
         const int2 c = cell2(x.x, x.y, S.search_radius);
         lo[0] = hi[0] = c.x;
@@ -628,6 +631,14 @@ __global__ __launch_bounds__(kBlock) void k2_integrate(const SceneDev* __restric
             bp.hi[a] = h;
         }
         S.box[br.block] = bp;
+    }
+    {  // max |v|^2, |a|^2 of the new state for the dynamic CFL time step
+        __shared__ float lmax[2 * kBlock / 64];
+        block_max2(nv2, na2, lmax);
+        if (threadIdx.x == 0) {
+            S.box[br.block].lo[3] = __float_as_int(nv2);
+            S.box[br.block].hi[3] = __float_as_int(na2);
+        }
     }
 }
 
@@ -675,6 +686,20 @@ __global__ __launch_bounds__(kBlock) void k2_integrate_finalize(const SceneDev* 
         if (S.flags & kFlagSesph) {
             st->iterations = 0;
             st->avg_err = 0.0;
+        }
+    }
+    {  // max |v|^2, |a|^2 of the new state for the dynamic CFL time step
+        __shared__ float lmax[2 * kBlock / 64];
+        float v2 = 0.0f, a2 = 0.0f;
+        for (uint32_t b = threadIdx.x; b < S.nblocks; b += kBlock) {
+            const BoxPartial bp = S.box[b];
+            v2 = fmaxf(v2, __int_as_float(bp.lo[3]));
+            a2 = fmaxf(a2, __int_as_float(bp.hi[3]));
+        }
+        block_max2(v2, a2, lmax);
+        if (threadIdx.x == 0) {
+            S.state->next_v2 = v2;
+            S.state->next_a2 = a2;
         }
     }
 }

@@ -8,6 +8,7 @@ libFluid reads the same:
   SESPHFluidSolver3D          src/libFluid/fluidSolver/solver/SESPHFluidSolver3D.hpp:12-337
   CompressedNeighborhoodSearch src/libFluid/fluidSolver/neighborhoodSearch/CompressedNeighbors.hpp:20-180
   ConstantTimestepGenerator   src/libFluid/time/ConstantTimestepGenerator.cpp
+  DynamicCflTimestepGenerator src/libFluid/time/DynamicCflTimestepGenerator.cpp:6-109
   Timepoint                   src/libFluid/time/Timepoint.hpp:7-14
   Simulator                   src/libFluid/Simulator.cpp:7-57
 Errors: the C ABI returns status codes; here they raise FsaError carrying the
@@ -179,9 +180,15 @@ class Scene:
         check(lib().fsa_scene_neighbor_stats(self._h, C.byref(t), C.byref(d), C.byref(r)))
         return {"total_neighbors": t.value, "device_list_bytes": d.value, "reference_codec_bytes": r.value}
 
+    def next_timestep(self):
+        """The desired step of the scene's timestep generator (constant or dynamic CFL)."""
+        dt = C.c_float(0.0)
+        check(lib().fsa_scene_next_timestep(self._h, C.byref(dt)))
+        return dt.value
+
     def simulate(self, dt=None):
         st = abi.StepStats()
-        check(lib().fsa_scene_simulate(self._h, float(self.desc.timestep if dt is None else dt), C.byref(st)))
+        check(lib().fsa_scene_simulate(self._h, float(self.next_timestep() if dt is None else dt), C.byref(st)))
         return st
 
     def step(self, steps=1):
@@ -240,6 +247,34 @@ class ConstantTimestepGenerator:
 
     def generate_next_timestep(self):
         self._generated = self.settings.timestep
+
+    def get_generated_timestep(self):
+        return self._generated
+
+
+class DynamicCflTimestepGenerator:
+    """time/DynamicCflTimestepGenerator.cpp: the next desired step follows the
+    largest |v| and |a| of the active particles (reduced on the device during
+    the integration), clamped to [min_timestep, max_timestep]; max_timestep is
+    also the fallback of the solver's in-step CFL correction."""
+
+    class Settings:
+        def __init__(self):  # DynamicCflTimestepGenerator.hpp:10-17
+            self.max_timestep = 0.02
+            self.min_timestep = 0.0001
+            self.cfl_number = 0.8
+            self.lambda_v = 0.4
+            self.lambda_a = 0.25
+
+    def __init__(self):
+        self.settings = DynamicCflTimestepGenerator.Settings()
+        self._generated = 0.0
+        self._solver = None  # set when a solver takes this generator
+
+    def generate_next_timestep(self):
+        if self._solver is None:
+            raise RuntimeError("DynamicCflTimestepGenerator has no particle collection (no solver)")
+        self._generated = self._solver._scene_or_create().next_timestep()
 
     def get_generated_timestep(self):
         return self._generated
@@ -307,7 +342,15 @@ class _HipFluidSolverBase:
         d.gravity = self.parameters.gravity
         d.particle_size = self.parameters.particle_size
         tg = self.data.timestep_generator
-        if tg is not None:
+        if isinstance(tg, DynamicCflTimestepGenerator):
+            tg._solver = self
+            d.timestep_generator = abi.TIMESTEP_DYNAMIC_CFL
+            d.max_timestep = tg.settings.max_timestep
+            d.min_timestep = tg.settings.min_timestep
+            d.cfl_number = tg.settings.cfl_number
+            d.lambda_v = tg.settings.lambda_v
+            d.lambda_a = tg.settings.lambda_a
+        elif tg is not None:
             d.timestep = tg.settings.timestep
             d.lambda_v = tg.settings.lambda_v
             d.lambda_a = tg.settings.lambda_a
@@ -419,6 +462,8 @@ class Simulator:
         self.timestep_generator = timestep_generator
         fluid_solver.data.collection = collection
         fluid_solver.data.timestep_generator = timestep_generator
+        if isinstance(timestep_generator, DynamicCflTimestepGenerator):
+            timestep_generator._solver = fluid_solver
 
     def execute_simulation_step(self):
         self.timestep_generator.generate_next_timestep()

@@ -56,6 +56,7 @@ extern "C" {
 #define FSA_E_STATE 5        /* call order violated (e.g. simulate before search) */
 #define FSA_E_BUDGET 6       /* scene does not fit the context's HBM budget       */
 #define FSA_E_NONFINITE 7    /* NaN/Inf detected in the particle state            */
+#define FSA_E_IO 8           /* a file could not be read or written, or is malformed */
 
 /* ---- enumerations ------------------------------------------------------ */
 #define FSA_SOLVER_IISPH 0   /* IISPHFluidSolver3D / IISPHFluidSolver            */
@@ -77,6 +78,13 @@ extern "C" {
 #define FSA_SEARCH_QUADRATIC_3D 3  /* "quadratic-dynamic-allocated-3d" (brute force)  */
 #define FSA_SEARCH_HASHED_2D 4     /* "hashed": HashedNeighborhoodSearch (2D)         */
 #define FSA_SEARCH_QUADRATIC_2D 5  /* "quadratic-dynamic-allocated" (2D)             */
+
+/* fsa_scene_desc.timestep_generator: how fsa_scene_step / fsa_batch_step /
+ * served jobs choose each step's desired time step (Simulator.cpp:17-24). */
+#define FSA_TIMESTEP_CONSTANT 0     /* ConstantTimestepGenerator: desc.timestep               */
+#define FSA_TIMESTEP_DYNAMIC_CFL 1  /* DynamicCflTimestepGenerator (DynamicCflTimestepGenerator.cpp:6-62):
+                                       from the largest |v| and |a| of the active particles,
+                                       clamped to [min_timestep, max_timestep]             */
 
 #define FSA_TYPE_NORMAL 0    /* ParticleTypeNormal   (Particle.hpp:9)            */
 #define FSA_TYPE_BOUNDARY 1  /* ParticleTypeBoundary (Particle.hpp:10)           */
@@ -128,7 +136,10 @@ typedef struct fsa_scene_desc {
     float lambda_a;
 
     uint32_t search;         /* FSA_SEARCH_* */
-    uint32_t reserved[4];
+    uint32_t timestep_generator; /* FSA_TIMESTEP_*                                     */
+    float max_timestep;      /* DynamicCflTimestepGenerator settings (DynamicCflTimestepGenerator.hpp:10-17); */
+    float min_timestep;      /* max_timestep is also the fallback of the in-step CFL correction */
+    float cfl_number;        /* carried for the scenario file; the reference does not use it  */
 } fsa_scene_desc;
 
 /* Strided views over the caller's component arrays. Row i of every view is
@@ -219,8 +230,13 @@ FSA_API int32_t fsa_scene_neighbor_stats(fsa_scene scene, uint64_t* total_neighb
 FSA_API int32_t fsa_scene_simulate(fsa_scene scene, float desired_time_step, fsa_step_stats* stats);
 /* `steps` x (find_neighbors + simulate), as Simulator::execute_simulation_step
  * does (Simulator.cpp:7-57) without entities/sensors; desired dt from the
- * scene's timestep setting. stats (may be NULL) receives the last step. */
+ * scene's timestep generator (desc.timestep_generator). stats (may be NULL)
+ * receives the last step. */
 FSA_API int32_t fsa_scene_step(fsa_scene scene, uint32_t steps, fsa_step_stats* stats);
+/* The desired time step the scene's generator gives for its current state
+ * (TimestepGenerator::generate_next_timestep + get_generated_timestep,
+ * ConstantTimestepGenerator.cpp:4-7, DynamicCflTimestepGenerator.cpp:46-62). */
+FSA_API int32_t fsa_scene_next_timestep(fsa_scene scene, float* dt);
 
 /* ---- batching: many scenes advanced together ------------------------------ */
 /* Advances `n` scenes of one context by `steps` steps each. All scenes are
@@ -317,6 +333,75 @@ FSA_API int32_t fsa_server_get_stats(fsa_server srv, fsa_server_stats* stats);
 #define FSA_TIMER_COUNT 6
 FSA_API int32_t fsa_context_set_timing(fsa_context ctx, uint32_t enabled);
 FSA_API int32_t fsa_context_get_timing(fsa_context ctx, double* total_ms, uint64_t* launches, uint32_t n);
+
+/* ---- scenario files ----------------------------------------------------- */
+/* Particle data files (.data): an LZ4 frame holding u32 version = 1, u32 number
+ * of components, u32 component ids, u64 particle count, then one little-endian
+ * record per particle with the components in id order
+ * (ParticleSerializer::serialize / deserialize, ParticleSerializer.cpp:41-141;
+ * Lz4CompressedStream.cpp:7-187; EndianSafeBinaryStream.hpp:29-30).
+ * Component ids and their record fields: */
+#define FSA_COMPONENT_MOVEMENT_2D 1  /* vec2 position, velocity, acceleration        */
+#define FSA_COMPONENT_MOVEMENT_3D 2  /* vec3 position, velocity, acceleration        */
+#define FSA_COMPONENT_INFO 3         /* u8 type, u32 tag                             */
+#define FSA_COMPONENT_PARTICLE_DATA 4 /* float density, mass, pressure (file order)  */
+#define FSA_COMPONENT_EXTERNAL_2D 5  /* vec2 non-pressure acceleration               */
+#define FSA_COMPONENT_EXTERNAL_3D 6  /* vec3 non-pressure acceleration               */
+
+typedef struct fsa_data_header {
+    uint32_t version;     /* 1 */
+    uint32_t dimension;   /* 2 or 3, from the movement component (0 if none)      */
+    uint32_t components;  /* bit (id - 1) set for each component id in the file   */
+    uint32_t reserved;
+    uint64_t count;       /* particles                                            */
+} fsa_data_header;
+
+/* Reads the header of a particle file. */
+FSA_API int32_t fsa_data_read_header(const char* path, fsa_data_header* out);
+/* Reads a particle file into the caller's view (view->count must equal the
+ * file's count; movement rows are 3 * dimension floats).  Components missing
+ * from the file are zeroed in the view; NULL view pointers skip a component. */
+FSA_API int32_t fsa_data_read(const char* path, const fsa_particle_view* view);
+/* Writes the view as a particle file of the given dimension, with the
+ * movement, info and particle-data components and, when view->external is
+ * not NULL, the external-force component (ParticleSerializer::serialize). */
+FSA_API int32_t fsa_data_write(const char* path, uint32_t dimension, const fsa_particle_view* view);
+
+/* Scenario files: JSON version 2 with the "scenario", "solver" and
+ * "timestep-generator" nodes (MainSerializer.cpp:114-171; ScenarioSerializer.cpp:43-79;
+ * SolverSerializer.cpp:137-259; TimestepGeneratorSerializer.cpp:47-72).
+ * Entities, sensors and the visualizer are not simulated by this library; their
+ * presence is reported. */
+typedef struct fsa_scenario {
+    fsa_scene_desc desc;          /* dimension, solver, search, settings and timestep generator */
+    uint32_t entities;            /* entries in scenario.entities (not simulated here)       */
+    uint32_t sensors;             /* entries in scenario.sensors (not simulated here)        */
+    char particles[1024];         /* scenario.particles as written in the file               */
+    char particles_path[4096];    /* the file the reference opens for it
+                                     (MainSerializer::get_full_particle_data_path, :216-223)  */
+} fsa_scenario;
+
+/* Parses a scenario file.  Missing required keys, unknown solver / search /
+ * kernel combinations and versions other than 2 are errors. */
+FSA_API int32_t fsa_scenario_read(const char* json_path, fsa_scenario* out);
+/* Writes a scenario file (version 2, visualizer "no-visualizer") naming
+ * in->particles as its particle file. */
+FSA_API int32_t fsa_scenario_write(const char* json_path, const fsa_scenario* in);
+/* Scene defaults for the dimension / solver and no particle file. */
+FSA_API void fsa_scenario_defaults(fsa_scenario* out, uint32_t dimension, uint32_t solver);
+
+/* LZ4 frame codec and xxHash32 used by the particle files (the lz4 library's
+ * frame format, which Lz4CompressedStream drives through LZ4F_compressUpdate /
+ * LZ4F_decompress).  Decoding accepts linked and independent blocks, block and
+ * content checksums, a content size and skippable frames; encoding writes
+ * independent 64 KiB blocks with a content checksum.  *out_size receives the
+ * bytes produced; FSA_E_CAPACITY when they exceed capacity (*out_size is then
+ * the size needed). */
+FSA_API int32_t fsa_lz4_frame_decode(const void* src, uint64_t src_size, void* dst, uint64_t capacity,
+                                     uint64_t* out_size);
+FSA_API int32_t fsa_lz4_frame_encode(const void* src, uint64_t src_size, void* dst, uint64_t capacity,
+                                     uint64_t* out_size);
+FSA_API uint32_t fsa_xxh32(const void* data, uint64_t size, uint32_t seed);
 
 #ifdef __cplusplus
 }

@@ -47,6 +47,7 @@ def lib():
         "orc_destroy": (None, [vp]),
         "orc_find_neighbors": (i32, [vp]),
         "orc_simulate": (i32, [vp, f32, C.POINTER(StepStats)]),
+        "orc_next_timestep": (f32, [vp]),
         "orc_download": (i32, [vp, C.POINTER(ParticleView)]),
         "orc_download_iisph": (i32, [vp, vp, u64]),
         "orc_export_neighbors": (i32, [vp, vp, vp, u64, C.POINTER(u64)]),
@@ -132,9 +133,13 @@ class OracleScene:
     def find_neighbors(self):
         _check(lib().orc_find_neighbors(self._h))
 
+    def next_timestep(self):
+        """The scene's timestep generator's next desired step (constant or dynamic CFL)."""
+        return float(lib().orc_next_timestep(self._h))
+
     def simulate(self, dt=None):
         st = StepStats()
-        _check(lib().orc_simulate(self._h, float(self.desc.timestep if dt is None else dt), C.byref(st)))
+        _check(lib().orc_simulate(self._h, float(self.next_timestep() if dt is None else dt), C.byref(st)))
         return st
 
     def step(self, steps=1, dt=None):

@@ -359,6 +359,7 @@ struct SceneBase {
     virtual void internal_ids(uint64_t* out) = 0;
     virtual int export_raw(uint64_t* off, uint64_t* cols, uint64_t cap, uint64_t* nnz) = 0;
     virtual int upload(const fsa_particle_view* v) = 0;
+    virtual float next_timestep() const = 0;
 };
 
 template <class T, int D>
@@ -690,9 +691,12 @@ struct Scene : SceneBase {
         return rc;
     }
 
-    // ---- ConstantTimestepGenerator::get_non_cfl_validating_timestep (ConstantTimestepGenerator.cpp:17-41)
+    // ---- get_non_cfl_validating_timestep of the scene's generator:
+    // ConstantTimestepGenerator.cpp:17-41 (falls back to timestep) or
+    // DynamicCflTimestepGenerator.cpp:84-109 (falls back to max_timestep)
     float cfl_timestep(float max_a, float max_v) const {
-        float dta = desc.timestep, dtv = desc.timestep;
+        const float fallback = desc.timestep_generator == FSA_TIMESTEP_DYNAMIC_CFL ? desc.max_timestep : desc.timestep;
+        float dta = fallback, dtv = fallback;
         // unqualified sqrt on a float resolves to ::sqrt(double); the product
         // with the float lambda is formed in double and stored to float
         if (max_a > 0.0f) dta = (float)(::sqrt((double)(desc.particle_size / max_a)) * desc.lambda_a);
@@ -700,6 +704,25 @@ struct Scene : SceneBase {
         float nt = std::fmin(dta, dtv);
         return std::fmax(nt, 0.000001f);  // MIN_ALLOWED_TIMESTEP (TimestepGenerator.hpp:18)
     }
+
+    // ---- the generator's next desired time step (Simulator.cpp:17-24):
+    // ConstantTimestepGenerator::generate_next_timestep (ConstantTimestepGenerator.cpp:4-7) or
+    // DynamicCflTimestepGenerator::generate_next_timestep (DynamicCflTimestepGenerator.cpp:46-62)
+    // with calculate_maximum_velocity_and_acceleration (:6-43) over the collection.
+    float generated_timestep() const {
+        if (desc.timestep_generator != FSA_TIMESTEP_DYNAMIC_CFL) return desc.timestep;
+        float max_v = 0.0f, max_a = 0.0f;
+        for (size_t i = 0; i < n; i++) {
+            if (type[i] == T_INACTIVE) continue;
+            max_v = std::max(max_v, (float)std::sqrt(dot(vel[i], vel[i])));
+            max_a = std::max(max_a, (float)std::sqrt(dot(acc[i], acc[i])));
+        }
+        float ts = desc.min_timestep;
+        const float eps = std::numeric_limits<float>::epsilon();
+        if (max_v > eps && max_a > eps) ts = std::max(ts, std::min(desc.max_timestep, cfl_timestep(max_a, max_v)));
+        return ts;
+    }
+    float next_timestep() const override { return generated_timestep(); }
 
     V gravity_vec() const {
         V g = vzero<T, D>();
@@ -1337,6 +1360,7 @@ void* orc_create(const fsa_scene_desc* d, int search_kind, int mode, const fsa_p
 void orc_destroy(void* s) { delete (SceneBase*)s; }
 int orc_find_neighbors(void* s) { return ((SceneBase*)s)->find_neighbors(); }
 int orc_simulate(void* s, float dt, fsa_step_stats* st) { return ((SceneBase*)s)->simulate(dt, st); }
+float orc_next_timestep(void* s) { return ((SceneBase*)s)->next_timestep(); }
 int orc_download(void* s, const fsa_particle_view* v) { return ((SceneBase*)s)->download(v); }
 int orc_download_iisph(void* s, void* rec, uint64_t stride) { return ((SceneBase*)s)->download_iisph(rec, stride); }
 int orc_export_neighbors(void* s, uint64_t* off, uint64_t* cols, uint64_t cap, uint64_t* nnz) {
@@ -1426,6 +1450,8 @@ void orc_stable_sort_u64(const uint64_t* keys, uint64_t n, uint64_t* perm) {
 }
 float orc_cfl_timestep(float max_a, float max_v, float particle_size, float timestep, float lambda_a, float lambda_v) {
     Scene<float, 3> s;
+    std::memset(&s.desc, 0, sizeof(s.desc));
+    s.desc.timestep_generator = FSA_TIMESTEP_CONSTANT;
     s.desc.particle_size = particle_size;
     s.desc.timestep = timestep;
     s.desc.lambda_a = lambda_a;
