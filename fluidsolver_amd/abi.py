@@ -204,6 +204,76 @@ class ServerStats(C.Structure):
         return {f[0]: getattr(self, f[0]) for f in self._fields_}
 
 
+# entities and sensors
+ENTITY_BOUNDARY_PREPROCESSOR = 1
+ENTITY_VELOCITY_ALTERATION_BY_TAG = 2
+ENTITY_PARTICLE_REMOVER_3D = 3
+
+
+class Entity(C.Structure):
+    _fields_ = [
+        ("type", C.c_uint32),
+        ("tag", C.c_uint32),
+        ("remove_if_outside", C.c_uint32),
+        ("reserved", C.c_uint32),
+        ("velocity", C.c_float * 3),
+        ("center", C.c_float * 3),
+        ("distance_from_center", C.c_float * 3),
+    ]
+
+
+def boundary_preprocessor():
+    e = Entity()
+    e.type = ENTITY_BOUNDARY_PREPROCESSOR
+    return e
+
+
+def velocity_alteration_by_tag(tag, velocity):
+    e = Entity()
+    e.type = ENTITY_VELOCITY_ALTERATION_BY_TAG
+    e.tag = tag
+    e.velocity[:] = [float(x) for x in velocity]
+    return e
+
+
+def particle_remover_3d(center, distance_from_center, remove_if_outside):
+    e = Entity()
+    e.type = ENTITY_PARTICLE_REMOVER_3D
+    e.center[:] = [float(x) for x in center]
+    e.distance_from_center[:] = [float(x) for x in distance_from_center]
+    e.remove_if_outside = 1 if remove_if_outside else 0
+    return e
+
+
+def entity_array(entities):
+    arr = (Entity * max(1, len(entities)))()
+    for k, e in enumerate(entities):
+        arr[k] = e
+    return arr
+
+
+class SceneStatistics(C.Structure):
+    _fields_ = [
+        ("normal_particles", C.c_uint64),
+        ("boundary_particles", C.c_uint64),
+        ("inactive_particles", C.c_uint64),
+        ("density_max", C.c_float),
+        ("density_min", C.c_float),
+        ("density_avg", C.c_double),
+        ("pressure_max", C.c_float),
+        ("pressure_min", C.c_float),
+        ("pressure_avg", C.c_double),
+        ("velocity_max", C.c_float),
+        ("velocity_min", C.c_float),
+        ("velocity_avg", C.c_double),
+        ("kinetic_energy", C.c_double),
+        ("potential_energy", C.c_double),
+    ]
+
+    def as_dict(self):
+        return {f[0]: getattr(self, f[0]) for f in self._fields_}
+
+
 # scenario files
 COMPONENT_MOVEMENT_2D = 1
 COMPONENT_MOVEMENT_3D = 2
@@ -231,6 +301,9 @@ class Scenario(C.Structure):
         ("desc", SceneDesc),
         ("entities", C.c_uint32),
         ("sensors", C.c_uint32),
+        ("entity_count", C.c_uint32),
+        ("reserved", C.c_uint32),
+        ("entity_list", Entity * 32),
         ("particles", C.c_char * 1024),
         ("particles_path", C.c_char * 4096),
     ]

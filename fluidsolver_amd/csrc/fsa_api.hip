@@ -132,7 +132,21 @@ struct fsa_scene_s {
     bool lists_valid = false;  // lists of the last search (valid until the next search or upload)
     std::vector<uint64_t> csr_off, csr_cols;
     float cur_v2 = 0.0f, cur_a2 = 0.0f;  // max |v|^2, |a|^2 of the active particles' current state
+    // simulation entities (fsa_scene_set_entities)
+    std::vector<fsa_entity> entities;
+    EntityDev* d_entities = nullptr;
+    uint32_t* d_tags = nullptr;          // ParticleInfo.tag per caller row, for velocity alterations
+    uint64_t d_tags_cap = 0;
+    std::vector<uint32_t> row_tags;      // caller rows, as last uploaded
+    std::vector<uint8_t> row_types;
+    bool entity_moves_boundary = false;  // a velocity alteration reaches boundary particles
+    bool removal_pending = false;        // a particle remover found members in the last step
+    std::vector<float> carry_err;        // IISPH predicted density errors by caller row for the next upload
+    void* d_stats = nullptr;             // sensor reduction scratch
+    uint64_t d_stats_bytes = 0;
 };
+
+static_assert(sizeof(EntityDev) == sizeof(fsa_entity), "EntityDev mirrors fsa_entity");
 
 namespace {
 
@@ -523,6 +537,7 @@ void simulate_set(fsa_context ctx, const LaunchSet& L, std::vector<fsa_scene_s*>
                   bool boundary_pass) {
     hipStream_t st = ctx->stream;
     const bool d2 = !scenes.empty() && scenes[0]->hd.dim == 2;
+    if (L.entities_before) launch_entities_before(L, st);
     if (sesph) {
         if (d2) {
             PhaseTimer t(ctx, FSA_TIMER_DENSITY);
@@ -595,6 +610,7 @@ void simulate_set(fsa_context ctx, const LaunchSet& L, std::vector<fsa_scene_s*>
         else
             launch_integrate(L, st);
     }
+    if (L.remover) launch_remover_check(L, st);
     HIP_CHECK(hipStreamSynchronize(st));
 }
 
@@ -625,6 +641,8 @@ void upload_scene_table(fsa_scene_s* s) {
     s->single.spec = spec_of(s->hd);
     s->single.codec = (s->hd.flags & kFlagCodec) != 0;
     s->single.wide_lists = wide_lists(s->hd);
+    s->single.entities_before = (s->hd.entity_bits & (kEntityBitBoundary | kEntityBitVelocity)) != 0;
+    s->single.remover = (s->hd.entity_bits & kEntityBitRemover) != 0;
     HIP_CHECK(hipMemcpyAsync(s->d_table, &s->hd, sizeof(SceneDev), hipMemcpyHostToDevice, s->ctx->stream));
 }
 
@@ -762,6 +780,7 @@ void do_find_neighbors(fsa_scene_s* s) {
         search2d_set(s->ctx, s->single, entry_passes(s->hd), cell_passes(s->hd));
         s->hd.search_count++;
     } else if (s->hd.flags & kFlagUnsorted) {
+        s->hd.vals_buf = unsorted_passes(s->hd) & 1u;
         search3u_set(s->ctx, s->single, unsorted_passes(s->hd));
     } else {
         search_set(s->ctx, s->single, sort_passes(s->hd));
@@ -774,6 +793,7 @@ void do_find_neighbors(fsa_scene_s* s) {
 }
 
 void set_step_dt(fsa_scene_s* s, float dt);
+void apply_removal(fsa_scene_s* s);
 
 void pre_simulate(fsa_scene_s* s, float dt) {
     if (!s->searched) throw FsaError(FSA_E_STATE, "simulate called before find_neighbors");
@@ -823,6 +843,13 @@ void set_step_dt(fsa_scene_s* s, float dt) {
 // after the step's final state read
 void post_simulate(fsa_scene_s* s, fsa_step_stats* stats) {
     s->boundary_dirty = false;
+    if (s->h_state->status & kStatusRemoval) s->removal_pending = true;
+    // velocity alterations reach inactive particles too; they are held on the host
+    for (const auto& e : s->entities)
+        if (e.type == FSA_ENTITY_VELOCITY_ALTERATION_BY_TAG)
+            for (size_t k = 0; k < s->inactive_rows.size(); k++)
+                if (s->row_tags[s->inactive_rows[k]] == e.tag)
+                    for (int a = 0; a < 3; a++) s->inactive_movement[9 * k + 3 + a] = e.velocity[a];
     s->cur_v2 = s->h_state->next_v2;
     s->cur_a2 = s->h_state->next_a2;
     if (s->h_state->status & kStatusTinyCoord) s->hd.flags |= kFlagTinyCoords;
@@ -840,6 +867,7 @@ void do_simulate(fsa_scene_s* s, float dt, fsa_step_stats* stats) {
     simulate_set(s->ctx, s->single, one, s->desc.solver == FSA_SOLVER_SESPH, bpass);
     read_state(s);
     post_simulate(s, stats);
+    if (s->removal_pending) apply_removal(s);
 }
 
 // ---- batching -------------------------------------------------------------------------
@@ -894,13 +922,17 @@ void drop_batches_of(fsa_context ctx, fsa_scene_s* s) {
 
 void upload_batch_table(fsa_context ctx, BatchSet* b) {
     uint32_t spec = kSpecHOne | kSpecRho0SqOne | kSpecFastMath;  // the specialisations every scene allows
-    bool codec = false, wide = false;
+    bool codec = false, wide = false, ent = false, rem = false;
     for (uint32_t q = 0; q < b->scenes.size(); q++) {
         b->h_table[q] = b->scenes[q]->hd;
         spec &= spec_of(b->scenes[q]->hd);
         codec = codec || (b->scenes[q]->hd.flags & kFlagCodec);
         wide = wide || wide_lists(b->scenes[q]->hd);
+        ent = ent || (b->scenes[q]->hd.entity_bits & (kEntityBitBoundary | kEntityBitVelocity));
+        rem = rem || (b->scenes[q]->hd.entity_bits & kEntityBitRemover);
     }
+    b->ls.entities_before = ent;
+    b->ls.remover = rem;
     b->ls.spec = spec;
     b->ls.codec = codec;
     b->ls.wide_lists = wide;
@@ -1052,6 +1084,7 @@ void step_head(fsa_context ctx, const LaunchSet& L, bool d2, bool unsorted, uint
         search3u_set(ctx, L, passes);
     else
         search_set(ctx, L, passes);
+    if (L.entities_before) launch_entities_before(L, ctx->stream);
     if (sesph) {
         PhaseTimer t(ctx, FSA_TIMER_DENSITY);
         if (d2) {
@@ -1123,10 +1156,11 @@ int32_t batch_step_group(fsa_context ctx, const std::vector<fsa_scene_s*>& v, st
         guess = std::max(guess, std::max(s->hd.min_it, s->last_iterations));
     }
     const uint32_t chunk = sesph ? 0u : std::min(max_it, std::max<uint32_t>(guess, 1));
+    for (auto* s : v) s->hd.vals_buf = passes & 1u;  // the unsorted searches' final bucket buffer
     upload_batch_table(ctx, b);
     if (ctx->graph && !ctx->timing) {
         const std::array<uint32_t, 5> key = {
-            d2 ? (epasses << 8 | cpasses) : passes, chunk, b->ls.spec, (bpass ? 1u : 0u) | (sesph ? 2u : 0u) | (unsorted ? 4u : 0u),
+            d2 ? (epasses << 8 | cpasses) : passes, chunk, b->ls.spec, (bpass ? 1u : 0u) | (sesph ? 2u : 0u) | (unsorted ? 4u : 0u) | (b->ls.entities_before ? 8u : 0u),
             (uint32_t)b->ls.unroll[0] | (uint32_t)b->ls.unroll[1] << 8 | (uint32_t)b->ls.unroll[2] << 16 |
                 (uint32_t)b->ls.unroll[3] << 24};
         auto it = b->graphs.find(key);
@@ -1186,6 +1220,7 @@ int32_t batch_step_group(fsa_context ctx, const std::vector<fsa_scene_s*>& v, st
         else
             launch_integrate(b->ls, st);
     }
+    if (b->ls.remover) launch_remover_check(b->ls, st);
     read_states(ctx, v);
     int32_t first_error = FSA_OK;
     for (uint32_t q = 0; q < v.size(); q++) {
@@ -1210,6 +1245,9 @@ int32_t batch_step_group(fsa_context ctx, const std::vector<fsa_scene_s*>& v, st
         s->lists_cached = false;
         post_simulate(s, out[q]);
     }
+    // particle removers act after the step; the launch tables of this group are not used again
+    for (auto* s : v)
+        if (s->removal_pending) apply_removal(s);
     return first_error;
 }
 
@@ -1320,6 +1358,13 @@ void do_upload2d(fsa_scene_s* s, const fsa_particle_view* v, bool first) {
 void upload_maxima(fsa_scene_s* s, const fsa_particle_view* v) {
     float v2 = 0.0f, a2 = 0.0f;
     const bool d3 = s->desc.dimension == 3;
+    s->row_tags.resize(v->count);
+    s->row_types.resize(v->count);
+    for (uint64_t i = 0; i < v->count; i++) {
+        const uint8_t* info = (const uint8_t*)v->info + i * v->info_stride;
+        std::memcpy(&s->row_tags[i], info, 4);
+        s->row_types[i] = info[4];
+    }
     for (uint64_t i = 0; i < v->count; i++) {
         if (((const uint8_t*)v->info + i * v->info_stride)[4] == FSA_TYPE_INACTIVE) continue;
         const float* m = (const float*)((const char*)v->movement + i * v->movement_stride);
@@ -1332,7 +1377,74 @@ void upload_maxima(fsa_scene_s* s, const fsa_particle_view* v) {
     s->cur_a2 = a2;
 }
 
-void do_upload(fsa_scene_s* s, const fsa_particle_view* v, bool first) {
+// CubicSplineKernel3D::GetKernelValue from |x| on the host, with the device's
+// operation order (the scene's h and alpha)
+float host_kernel_w(const SceneDev& h, float len) {
+    const float q = (h.flags & kFlagHOne) ? len : len / h.h;
+    const float a = 2.0f - q, b = 1.0f - q;
+    const float a3 = a * a * a;
+    const float inner = a3 - 4.0f * (b * b * b);
+    const float ret = q < 1.0f ? inner : (q < 2.0f ? a3 : 0.0f);
+    return h.alpha * ret;
+}
+
+// Device copies and bits of the scene's entity list: tags by caller row for
+// velocity alterations, BoundaryPreprocessor::initialize's gamma_1 and mass
+// cap (BoundaryPreprocessor.cpp:50-62, 96-117: a perfectly sampled plane of
+// 5 x 5 particles), and the dynamic-boundary flag when a velocity alteration
+// reaches boundary particles.
+void refresh_entities(fsa_scene_s* s) {
+    SceneDev& h = s->hd;
+    uint32_t bits = 0;
+    for (const auto& e : s->entities) {
+        if (e.type == FSA_ENTITY_BOUNDARY_PREPROCESSOR) bits |= kEntityBitBoundary;
+        if (e.type == FSA_ENTITY_VELOCITY_ALTERATION_BY_TAG) bits |= kEntityBitVelocity;
+        if (e.type == FSA_ENTITY_PARTICLE_REMOVER_3D) bits |= kEntityBitRemover;
+    }
+    h.entity_bits = bits;
+    h.n_entities = (uint32_t)s->entities.size();
+    h.entities = s->d_entities;
+    if (bits & kEntityBitVelocity) {
+        const uint64_t rows = std::max<size_t>(s->row_tags.size(), 1);
+        if (!s->d_tags || s->d_tags_cap < rows) {
+            if (s->d_tags) (void)hipFree(s->d_tags);
+            s->d_tags = nullptr;
+            if (hipMalloc(&s->d_tags, 4 * rows) != hipSuccess) {
+                s->d_tags = nullptr;
+                throw FsaError(FSA_E_NOMEM, "device allocation failed");
+            }
+            s->d_tags_cap = rows;
+        }
+        if (!s->row_tags.empty())
+            HIP_CHECK(hipMemcpyAsync(s->d_tags, s->row_tags.data(), 4 * s->row_tags.size(), hipMemcpyHostToDevice,
+                                     s->ctx->stream));
+    }
+    h.tag_of_row = s->d_tags;
+    if (bits & kEntityBitBoundary) {
+        const float ps = s->desc.particle_size;
+        float ideal = 0.0f;
+        for (int x = 0; x <= 0; x++)
+            for (int y = -2; y <= 2; y++)
+                for (int z = -2; z <= 2; z++) {
+                    const float px = (float)x * ps, py = (float)y * ps, pz = (float)z * ps;
+                    ideal += host_kernel_w(h, std::sqrt((px * px + py * py) + pz * pz));
+                }
+        h.bp_gamma1 = ((ps * ps) * ps) * ideal;  // Math::pow3(particle_size) * ideal_volume_reciprocal
+        h.bp_max_mass = s->desc.rest_density * ((ps * ps) * ps);
+    }
+    bool moves = false;
+    for (const auto& e : s->entities)
+        if (e.type == FSA_ENTITY_VELOCITY_ALTERATION_BY_TAG)
+            for (size_t r = 0; r < s->row_tags.size() && !moves; r++)
+                if (s->row_types[r] == FSA_TYPE_BOUNDARY && s->row_tags[r] == e.tag) moves = true;
+    s->entity_moves_boundary = moves;
+    if (moves) h.flags |= kFlagDynamicBoundary;
+}
+
+// given_order: caller rows in the device order to keep (a relayout after
+// particles turned inactive); the particle counts may then change.
+void do_upload(fsa_scene_s* s, const fsa_particle_view* v, bool first,
+               const std::vector<uint32_t>* given_order = nullptr) {
     if (!v || !v->movement || !v->particle_data || !v->info) throw FsaError(FSA_E_INVALID, "incomplete particle view");
     upload_maxima(s, v);
     if (s->desc.dimension == 2) {
@@ -1365,7 +1477,7 @@ void do_upload(fsa_scene_s* s, const fsa_particle_view* v, bool first) {
         else
             nb++;
     }
-    if (!first && (N != s->total || nf != s->nf || nb != s->nb))
+    if (!first && !given_order && (N != s->total || nf != s->nf || nb != s->nb))
         throw FsaError(FSA_E_INVALID, "upload must keep the particle count and types of the scene");
     // Device order of the active particles: the caller's order on the first upload;
     // afterwards the order the reference collection holds by now (its searches
@@ -1378,7 +1490,11 @@ void do_upload(fsa_scene_s* s, const fsa_particle_view* v, bool first) {
     }
     if (!first) {
         std::vector<uint32_t> order;
-        if (hashed && s->ref_order.size() == N) {
+        if (given_order) {
+            for (uint32_t r : *given_order)
+                if (r < N && ((const uint8_t*)v->info + (uint64_t)r * v->info_stride)[4] != FSA_TYPE_INACTIVE)
+                    order.push_back(r);
+        } else if (hashed && s->ref_order.size() == N) {
             for (uint32_t r : s->ref_order)
                 if (((const uint8_t*)v->info + (uint64_t)r * v->info_stride)[4] != FSA_TYPE_INACTIVE) order.push_back(r);
         } else if (!(s->hd.flags & kFlagUnsorted) && s->n) {
@@ -1482,9 +1598,13 @@ void do_upload(fsa_scene_s* s, const fsa_particle_view* v, bool first) {
         HIP_CHECK(hipMemcpyAsync(h.acc[0], acc.data(), 16ull * n, hipMemcpyHostToDevice, st));
         HIP_CHECK(hipMemcpyAsync(h.ext[0], ext.data(), 16ull * n, hipMemcpyHostToDevice, st));
         HIP_CHECK(hipMemcpyAsync(h.pres[0], pres.data(), 4ull * n, hipMemcpyHostToDevice, st));
-        HIP_CHECK(hipMemcpyAsync(h.err[0], zero.data(), 4ull * n, hipMemcpyHostToDevice, st));
+        std::vector<float> err(n, 0.0f);
+        if (s->carry_err.size() == N)
+            for (uint32_t k = 0; k < n; k++) err[k] = s->carry_err[idt[k] & kIdMask];
+        HIP_CHECK(hipMemcpyAsync(h.err[0], err.data(), 4ull * n, hipMemcpyHostToDevice, st));
         HIP_CHECK(hipMemcpyAsync(h.idt[0], idt.data(), 4ull * n, hipMemcpyHostToDevice, st));
     }
+    s->carry_err.clear();
     HIP_CHECK(hipMemcpyAsync(h.state, &init, sizeof(SceneState), hipMemcpyHostToDevice, st));
     HIP_CHECK(hipStreamSynchronize(st));
     *s->h_state = init;
@@ -1492,6 +1612,7 @@ void do_upload(fsa_scene_s* s, const fsa_particle_view* v, bool first) {
     s->boundary_dirty = true;
     s->lists_cached = false;
     s->lists_valid = false;
+    refresh_entities(s);
 }
 
 struct HostCopy {
@@ -1598,6 +1719,80 @@ void do_download(fsa_scene_s* s, const fsa_particle_view* v) {
     }
 }
 
+// Re-lays the scene out for a collection whose active set changed: a new plan
+// in the same allocation when it fits, new launch maps, and an upload that
+// keeps the given device order of the particles that stay active.
+void relayout(fsa_scene_s* s, const fsa_particle_view* v, const std::vector<uint32_t>& order) {
+    uint64_t nf = 0, nb = 0;
+    for (uint64_t i = 0; i < v->count; i++) {
+        const uint8_t t = ((const uint8_t*)v->info + i * v->info_stride)[4];
+        nf += t == FSA_TYPE_NORMAL;
+        nb += t == FSA_TYPE_BOUNDARY;
+    }
+    const Plan p = make_plan(nf + nb, nf, 3, (s->hd.flags & kFlagUnsorted) != 0);
+    const uint64_t bytes = plan_bytes(p);
+    HIP_CHECK(hipStreamSynchronize(s->ctx->stream));
+    if (bytes > s->bytes) {
+        void* m = nullptr;
+        if (hipMalloc(&m, bytes) != hipSuccess) throw FsaError(FSA_E_NOMEM, "device allocation failed");
+        (void)hipFree(s->dmem);
+        s->ctx->in_use -= s->bytes;
+        s->ctx->in_use += bytes;
+        s->dmem = m;
+        s->bytes = bytes;
+    }
+    carve(s, p);
+    for (auto* m : s->map_allocs) (void)hipFree(m);
+    s->map_allocs.clear();
+    MapBuilder mb;
+    mb.add(0, s->hd);
+    build_launch_set(mb, s->d_table, s->single, s->map_allocs);
+    {
+        std::lock_guard<std::mutex> g(s->ctx->mu);
+        drop_batches_of(s->ctx, s);
+    }
+    do_upload(s, v, false, &order);
+    upload_scene_table(s);
+}
+
+// GroupActions::remove over each ParticleRemover3D's VolumeGroup
+// (ParticleRemover3D.cpp:8-16; VolumeGroup.cpp:43-58) on the host copy of the
+// collection, then a relayout that keeps the device order and the IISPH
+// predicted density errors (IISPHParticleData3D persists in the collection).
+void apply_removal(fsa_scene_s* s) {
+    s->removal_pending = false;
+    const uint64_t N = s->total;
+    std::vector<float> mv(9 * N), pd(3 * N), ex(3 * N);
+    std::vector<uint8_t> info(8 * N, 0);
+    for (uint64_t r = 0; r < N && r < s->row_tags.size(); r++) std::memcpy(&info[8 * r], &s->row_tags[r], 4);
+    fsa_particle_view v{N, mv.data(), 36, pd.data(), 12, info.data(), 8, ex.data(), 12};
+    do_download(s, &v);
+    HostCopy hc;
+    fetch(s, hc, true);
+    std::vector<uint32_t> order(hc.idt.size());
+    s->carry_err.assign(N, 0.0f);
+    for (size_t k = 0; k < hc.idt.size(); k++) {
+        const uint32_t row = hc.idt[k] & kIdMask;
+        order[k] = row;
+        s->carry_err[row] = hc.err[k];
+    }
+    for (const auto& e : s->entities) {
+        if (e.type != FSA_ENTITY_PARTICLE_REMOVER_3D) continue;
+        float lo[3], hi[3];
+        for (int a = 0; a < 3; a++) {
+            hi[a] = e.center[a] + e.distance_from_center[a];
+            lo[a] = e.center[a] - e.distance_from_center[a];
+        }
+        for (uint64_t r = 0; r < N; r++) {
+            const float* m = &mv[9 * r];
+            const bool inside = !(m[0] < lo[0] || m[0] > hi[0] || m[1] > hi[1] || m[1] < lo[1] || m[2] < lo[2] ||
+                                  m[2] > hi[2]);
+            if (inside == (e.remove_if_outside == 0u)) info[8 * r + 4] = FSA_TYPE_INACTIVE;
+        }
+    }
+    relayout(s, &v, order);
+}
+
 void do_download_iisph(fsa_scene_s* s, void* rec, uint64_t stride) {
     if (!rec) throw FsaError(FSA_E_INVALID, "records is NULL");
     HostCopy hc;
@@ -1692,6 +1887,9 @@ void destroy_scene(fsa_scene_s* s) {
     }
     if (s->d_table) (void)hipFree(s->d_table);
     for (auto* m : s->map_allocs) (void)hipFree(m);
+    if (s->d_entities) (void)hipFree(s->d_entities);
+    if (s->d_tags) (void)hipFree(s->d_tags);
+    if (s->d_stats) (void)hipFree(s->d_stats);
     {
         std::lock_guard<std::mutex> g(s->ctx->mu);
         if (s->h_state) s->ctx->free_slots.push_back(s->slot);
@@ -2102,6 +2300,79 @@ FSA_API int32_t fsa_scene_simulate(fsa_scene s, float dt, fsa_step_stats* stats)
     return guarded([&] {
         if (!s) throw FsaError(FSA_E_INVALID, "scene is NULL");
         do_simulate(s, dt, stats);
+    });
+}
+
+FSA_API int32_t fsa_scene_set_entities(fsa_scene s, const fsa_entity* e, uint32_t count) {
+    return guarded([&] {
+        if (!s || (count && !e)) throw FsaError(FSA_E_INVALID, "NULL argument");
+        for (uint32_t k = 0; k < count; k++) {
+            if (e[k].type != FSA_ENTITY_BOUNDARY_PREPROCESSOR && e[k].type != FSA_ENTITY_VELOCITY_ALTERATION_BY_TAG &&
+                e[k].type != FSA_ENTITY_PARTICLE_REMOVER_3D)
+                throw FsaError(FSA_E_INVALID, "unknown entity type");
+        }
+        if (count && s->desc.dimension != 3)
+            throw FsaError(FSA_E_INVALID, "the entities act on 3D collections (MovementData3D)");
+        HIP_CHECK(hipSetDevice(s->ctx->device));
+        HIP_CHECK(hipStreamSynchronize(s->ctx->stream));
+        s->entities.assign(e, e + count);
+        if (s->d_entities) (void)hipFree(s->d_entities);
+        s->d_entities = nullptr;
+        if (count) {
+            if (hipMalloc(&s->d_entities, sizeof(EntityDev) * count) != hipSuccess) {
+                s->d_entities = nullptr;
+                throw FsaError(FSA_E_NOMEM, "device allocation failed");
+            }
+            HIP_CHECK(hipMemcpy(s->d_entities, e, sizeof(EntityDev) * count, hipMemcpyHostToDevice));
+        }
+        refresh_entities(s);
+        s->boundary_dirty = true;
+        upload_scene_table(s);
+        HIP_CHECK(hipStreamSynchronize(s->ctx->stream));
+    });
+}
+
+FSA_API int32_t fsa_scene_get_statistics(fsa_scene s, float zero_height, fsa_scene_statistics* out) {
+    return guarded([&] {
+        if (!s || !out) throw FsaError(FSA_E_INVALID, "NULL argument");
+        HIP_CHECK(hipSetDevice(s->ctx->device));
+        hipStream_t st = s->ctx->stream;
+        const uint64_t need = statistics_scratch_bytes(std::max<uint32_t>(s->hd.nblocks, 1));
+        if (s->d_stats_bytes < need) {
+            if (s->d_stats) (void)hipFree(s->d_stats);
+            s->d_stats = nullptr;
+            if (hipMalloc(&s->d_stats, need) != hipSuccess) {
+                s->d_stats = nullptr;
+                throw FsaError(FSA_E_NOMEM, "device allocation failed");
+            }
+            s->d_stats_bytes = need;
+        }
+        upload_scene_table(s);
+        launch_statistics(s->d_table, s->hd.nblocks, zero_height, s->d_stats, st);
+        const uint64_t rb = statistics_result_bytes();
+        std::vector<uint8_t> res(rb);
+        HIP_CHECK(hipMemcpyAsync(res.data(), (char*)s->d_stats + need - rb, rb, hipMemcpyDeviceToHost, st));
+        HIP_CHECK(hipStreamSynchronize(st));
+        uint64_t cnt[3];
+        float mx[3], mn[3];
+        double sum[5];
+        read_statistics(res.data(), cnt, mx, mn, sum);
+        std::memset(out, 0, sizeof(*out));
+        out->normal_particles = cnt[0];
+        out->boundary_particles = cnt[1];
+        out->inactive_particles = cnt[2] + s->inactive_rows.size();
+        out->density_max = mx[0];
+        out->density_min = mn[0];
+        out->pressure_max = mx[1];
+        out->pressure_min = mn[1];
+        out->velocity_max = mx[2];
+        out->velocity_min = mn[2];
+        const double c = (double)cnt[0];
+        out->density_avg = cnt[0] ? sum[0] / c : 0.0;
+        out->pressure_avg = cnt[0] ? sum[1] / c : 0.0;
+        out->velocity_avg = cnt[0] ? sum[2] / c : 0.0;
+        out->kinetic_energy = sum[3];
+        out->potential_energy = sum[4];
     });
 }
 

@@ -56,6 +56,7 @@ constexpr uint32_t kStatusCapacity = 1u;    // a particle had more than 97 neigh
 constexpr uint32_t kStatusGrid = 2u;        // a particle left the cell grid of this step
 constexpr uint32_t kStatusNonFinite = 4u;
 constexpr uint32_t kStatusTinyCoord = 8u;   // a coordinate lies in (0, 2^-36) in magnitude (not an error)
+constexpr uint32_t kStatusRemoval = 16u;    // a particle remover entity found members among the active particles
 
 // Per-scene state written by the device and read back once per step.
 struct alignas(16) SceneState {
@@ -109,6 +110,21 @@ struct alignas(16) BoxPartial {
 
 // Per-scene descriptor, one per scene in a launch table; blocks find their
 // scene through a block map {scene, block within scene}.
+// a simulation entity of the scene (the C ABI's fsa_entity, same layout)
+struct EntityDev {
+    uint32_t type, tag, remove_if_outside, reserved;
+    float velocity[3];
+    float center[3];
+    float distance_from_center[3];
+};
+constexpr uint32_t kEntityBoundaryPreprocessor = 1u;
+constexpr uint32_t kEntityVelocityAlteration = 2u;
+constexpr uint32_t kEntityParticleRemover = 3u;
+// SceneDev.entity_bits: which kinds the scene's list holds
+constexpr uint32_t kEntityBitBoundary = 1u;
+constexpr uint32_t kEntityBitVelocity = 2u;
+constexpr uint32_t kEntityBitRemover = 4u;
+
 struct alignas(16) SceneDev {
     uint32_t n;            // combined active particles
     uint32_t nf;           // fluid particles
@@ -173,7 +189,23 @@ struct alignas(16) SceneDev {
     Partial* partial;      // per fluid block
     BoxPartial* box;       // per combined block
     SceneState* state;
+    // simulation entities around the solver
+    const EntityDev* entities;
+    const uint32_t* tag_of_row;  // ParticleInfo.tag per caller row (velocity alteration)
+    uint32_t n_entities;
+    uint32_t entity_bits;        // kEntityBit*
+    uint32_t vals_buf;           // hashed / quadratic: the bucket buffer of the last search
+    float bp_gamma1, bp_max_mass;  // BoundaryPreprocessor::initialize (BoundaryPreprocessor.cpp:50-62)
 };
+
+// dense index of cell (cx, cy, cz) in the step's cell grid; false outside it
+__device__ __forceinline__ bool cell_linear(const SceneDev& S, int32_t cx, int32_t cy, int32_t cz, uint32_t& lin) {
+    const int32_t x = cx - S.cmin[0], y = cy - S.cmin[1], z = cz - S.cmin[2];
+    if (x < 0 || y < 0 || z < 0 || x >= (int32_t)S.cdim[0] || y >= (int32_t)S.cdim[1] || z >= (int32_t)S.cdim[2])
+        return false;
+    lin = (uint32_t)x + S.cdim[0] * ((uint32_t)y + S.cdim[1] * (uint32_t)z);
+    return true;
+}
 
 struct BlockRef {
     uint32_t scene;

@@ -667,6 +667,15 @@ const Json& get_object(const Json& node, const char* key, const std::string& whe
     return v;
 }
 
+void get_vec3(const Json& node, const char* key, const std::string& where, float out[3]) {
+    const Json& v = need(node, key, where);
+    if (v.kind != Json::Array || v.items.size() != 3) io_error("scenario: \"" + where + key + "\" is not a 3-vector");
+    for (int k = 0; k < 3; k++) {
+        if (v.items[k].kind != Json::Number) io_error("scenario: \"" + where + key + "\" is not a 3-vector");
+        out[k] = (float)v.items[k].num;
+    }
+}
+
 std::string parent_dir(const std::string& p) {
     const size_t k = p.find_last_of('/');
     return k == std::string::npos ? std::string() : p.substr(0, k);
@@ -721,7 +730,37 @@ void parse_scenario(const char* json_path, fsa_scenario* out) {
     d.gravity = get_float(sc, "gravity", "scenario.");
     d.particle_size = get_float(sc, "particle-size", "scenario.");
     d.rest_density = get_float(sc, "rest-density", "scenario.");
-    if (const Json* e = sc.find("entities")) out->entities = e->kind == Json::Array ? (uint32_t)e->items.size() : 0;
+    if (const Json* e = sc.find("entities")) {
+        if (e->kind != Json::Array) io_error("scenario: \"scenario.entities\" is not an array");
+        out->entities = (uint32_t)e->items.size();
+        for (size_t k = 0; k < e->items.size(); k++) {
+            const Json& en = e->items[k];
+            const std::string where = "scenario.entities[" + std::to_string(k) + "].";
+            if (en.kind != Json::Object) io_error("scenario: an entity is not an object");
+            const std::string t = get_string(en, "type", where);
+            fsa_entity ent{};
+            if (t == "particle-remover-3d") {
+                const Json& vol = get_object(en, "volume", where);
+                get_vec3(vol, "center", where + "volume.", ent.center);
+                get_vec3(vol, "distance-from-center", where + "volume.", ent.distance_from_center);
+                ent.type = FSA_ENTITY_PARTICLE_REMOVER_3D;
+                ent.remove_if_outside = get_bool(en, "remove-if-outside", where) ? 1u : 0u;
+            } else if (t == "boundary-preprocessor") {
+                ent.type = FSA_ENTITY_BOUNDARY_PREPROCESSOR;
+            } else if (t == "velocity-alteration-by-tag") {
+                ent.type = FSA_ENTITY_VELOCITY_ALTERATION_BY_TAG;
+                ent.tag = get_count(en, "tag", where);
+                get_vec3(en, "velocity", where, ent.velocity);
+            } else if (t == "particle-remover" || t == "particle-spawner") {
+                continue;  // 2D entities: reported through the entity count only
+            } else {
+                io_error("Encountered invalid entity type!");
+            }
+            if (out->entity_count >= sizeof(out->entity_list) / sizeof(out->entity_list[0]))
+                io_error("scenario: more than 32 entities");
+            out->entity_list[out->entity_count++] = ent;
+        }
+    }
     if (const Json* s = sc.find("sensors")) out->sensors = s->kind == Json::Array ? (uint32_t)s->items.size() : 0;
 
     if (solver == FSA_SOLVER_SESPH) {
@@ -824,8 +863,31 @@ std::string format_scenario(const fsa_scenario* in) {
     o += I2 + "\"particles\": " + quoted(std::string(in->particles, plen)) + ",\n";
     o += I2 + "\"gravity\": " + num(d.gravity) + ",\n";
     o += I2 + "\"particle-size\": " + num(d.particle_size) + ",\n";
-    o += I2 + "\"rest-density\": " + num(d.rest_density) + "\n";
-    o += I1 + "},\n";
+    o += I2 + "\"rest-density\": " + num(d.rest_density);
+    if (in->entity_count) {
+        if (in->entity_count > sizeof(in->entity_list) / sizeof(in->entity_list[0]))
+            throw FsaError(FSA_E_INVALID, "entity_count exceeds the entity list");
+        auto vec = [&](const float* v) { return "[" + num(v[0]) + ", " + num(v[1]) + ", " + num(v[2]) + "]"; };
+        o += ",\n" + I2 + "\"entities\": [";
+        for (uint32_t k = 0; k < in->entity_count; k++) {
+            const fsa_entity& e = in->entity_list[k];
+            o += k ? ",\n" : "\n";
+            if (e.type == FSA_ENTITY_PARTICLE_REMOVER_3D) {
+                o += I3 + "{\"type\": \"particle-remover-3d\", \"volume\": {\"center\": " + vec(e.center) +
+                     ", \"distance-from-center\": " + vec(e.distance_from_center) + "}, \"remove-if-outside\": " +
+                     (e.remove_if_outside ? "true" : "false") + "}";
+            } else if (e.type == FSA_ENTITY_BOUNDARY_PREPROCESSOR) {
+                o += I3 + "{\"type\": \"boundary-preprocessor\"}";
+            } else if (e.type == FSA_ENTITY_VELOCITY_ALTERATION_BY_TAG) {
+                o += I3 + "{\"type\": \"velocity-alteration-by-tag\", \"tag\": " + std::to_string(e.tag) +
+                     ", \"velocity\": " + vec(e.velocity) + "}";
+            } else {
+                throw FsaError(FSA_E_INVALID, "unknown entity type");
+            }
+        }
+        o += "\n" + I2 + "]";
+    }
+    o += "\n" + I1 + "},\n";
     o += I1 + "\"solver\": {\n";
     o += I2 + "\"type\": \"" + std::string(iisph ? "iisph" : "sesph") + (d.dimension == 3 ? "-3d" : "") + "\",\n";
     o += I2 + "\"neighborhood-search\": {\n" + I3 + "\"type\": \"" + search + "\"\n" + I2 + "},\n";

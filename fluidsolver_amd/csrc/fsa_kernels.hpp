@@ -23,6 +23,8 @@ struct LaunchSet {
     uint8_t unroll[4] = {4, 4, 4, 4};
     bool codec = false;  // some scene of the set checks the NeighborStorage codec each search
     bool wide_lists = false;  // some scene's neighbor-list array spans 4 GiB or more
+    bool entities_before = false;  // some scene has a boundary preprocessor or a velocity alteration
+    bool remover = false;          // some scene has a particle remover
 };
 
 // kernel families the autotuner times
@@ -31,6 +33,14 @@ constexpr int kTuneDiagonal = 1;
 constexpr int kTunePassA = 2;
 constexpr int kTunePassB = 3;
 constexpr int kTuneFamilies = 4;
+
+// simulation entities and sensors
+void launch_entities_before(const LaunchSet& L, hipStream_t st);
+void launch_remover_check(const LaunchSet& L, hipStream_t st);
+uint64_t statistics_scratch_bytes(uint32_t nblocks);
+void launch_statistics(const SceneDev* d_scene, uint32_t nblocks, float zero_height, void* scratch, hipStream_t st);
+void read_statistics(const void* host_copy, uint64_t counts[3], float mx[3], float mn[3], double sums[5]);
+uint64_t statistics_result_bytes();  // one accumulator; the result sits at the end of the scratch
 
 // neighborhood search
 void launch_keys(const LaunchSet& L, hipStream_t st);

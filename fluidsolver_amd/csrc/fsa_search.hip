@@ -257,13 +257,6 @@ __global__ __launch_bounds__(kBlock) void k_compact(const SceneDev* __restrict__
 }
 
 // ---- 5. cell table -----------------------------------------------------------
-__device__ __forceinline__ bool cell_linear(const SceneDev& S, int32_t cx, int32_t cy, int32_t cz, uint32_t& lin) {
-    const int32_t x = cx - S.cmin[0], y = cy - S.cmin[1], z = cz - S.cmin[2];
-    if (x < 0 || y < 0 || z < 0 || x >= (int32_t)S.cdim[0] || y >= (int32_t)S.cdim[1] || z >= (int32_t)S.cdim[2])
-        return false;
-    lin = (uint32_t)x + S.cdim[0] * ((uint32_t)y + S.cdim[1] * (uint32_t)z);
-    return true;
-}
 
 __global__ __launch_bounds__(kBlock) void k_cell_table(const SceneDev* __restrict__ scenes,
                                                        const BlockRef* __restrict__ map, uint32_t sbuf) {

@@ -112,11 +112,36 @@ class Scenario:
 
     @property
     def entities(self):
+        """Entries in the file's entity list."""
         return self.raw.entities
+
+    @property
+    def entity_list(self):
+        """The entities this library runs (abi.Entity), in file order."""
+        return [self.raw.entity_list[k] for k in range(self.raw.entity_count)]
+
+    def set_entity_list(self, entities):
+        if len(entities) > len(self.raw.entity_list):
+            raise ValueError("at most 32 entities")
+        for k, e in enumerate(entities):
+            self.raw.entity_list[k] = e
+        self.raw.entity_count = len(entities)
 
     @property
     def sensors(self):
         return self.raw.sensors
+
+
+def create_scene(context, scenario):
+    """A device scene for a loaded scenario, with its entities."""
+    from .solver import Scene
+
+    if scenario.collection is None:
+        raise ValueError("the scenario's particles are not loaded")
+    sc = Scene(context, scenario.desc, scenario.collection.copy())
+    if scenario.raw.entity_count:
+        sc.set_entities(scenario.entity_list)
+    return sc
 
 
 def default_scenario(dimension=3, solver=abi.SOLVER_IISPH):

@@ -180,6 +180,18 @@ class Scene:
         check(lib().fsa_scene_neighbor_stats(self._h, C.byref(t), C.byref(d), C.byref(r)))
         return {"total_neighbors": t.value, "device_list_bytes": d.value, "reference_codec_bytes": r.value}
 
+    def set_entities(self, entities):
+        """Simulation entities run around each step (abi.Entity records: boundary
+        preprocessor, velocity alteration by tag, particle remover 3D)."""
+        arr = abi.entity_array(list(entities))
+        check(lib().fsa_scene_set_entities(self._h, arr, len(entities)))
+
+    def statistics(self, relative_zero_height=0.0):
+        """The global sensors (density, pressure, |v|, energies, counts) reduced on the device."""
+        out = abi.SceneStatistics()
+        check(lib().fsa_scene_get_statistics(self._h, float(relative_zero_height), C.byref(out)))
+        return out
+
     def next_timestep(self):
         """The desired step of the scene's timestep generator (constant or dynamic CFL)."""
         dt = C.c_float(0.0)

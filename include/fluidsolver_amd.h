@@ -334,6 +334,57 @@ FSA_API int32_t fsa_server_get_stats(fsa_server srv, fsa_server_stats* stats);
 FSA_API int32_t fsa_context_set_timing(fsa_context ctx, uint32_t enabled);
 FSA_API int32_t fsa_context_get_timing(fsa_context ctx, double* total_ms, uint64_t* launches, uint32_t n);
 
+/* ---- entities and sensors around the solver ------------------------------ */
+/* Simulation entities run inside fsa_scene_step / fsa_batch_step / served
+ * jobs in the reference's order (Simulator.cpp:26-45): after the neighborhood
+ * search, the entities whose execution point is before the solver, in list
+ * order; after the solver, the others.  fsa_scene_simulate runs them too. */
+#define FSA_ENTITY_BOUNDARY_PREPROCESSOR 1    /* BoundaryPreprocessor (BoundaryPreprocessor.cpp:7-48, 96-117):
+                                                 before the solver, each boundary particle's mass becomes
+                                                 min(rest_density * particle_size^3,
+                                                     rest_density * gamma_1 / sum of W over its boundary
+                                                     neighbors), gamma_1 from a perfectly sampled plane; 3D */
+#define FSA_ENTITY_VELOCITY_ALTERATION_BY_TAG 2 /* VelocityAlterationByTag (VelocityAlterationByTag.cpp:7-22):
+                                                 before the solver, particles with `tag` get `velocity`; 3D */
+#define FSA_ENTITY_PARTICLE_REMOVER_3D 3      /* ParticleRemover3D (ParticleRemover3D.cpp:8-16, VolumeGroup.cpp:43-58):
+                                                 after the solver, particles inside (or, with
+                                                 remove_if_outside, outside) the box become inactive */
+
+typedef struct fsa_entity {
+    uint32_t type;                 /* FSA_ENTITY_*                                          */
+    uint32_t tag;                  /* velocity alteration: the tag it applies to            */
+    uint32_t remove_if_outside;    /* particle remover: 1 = remove outside the volume        */
+    uint32_t reserved;
+    float velocity[3];             /* velocity alteration                                   */
+    float center[3];               /* particle remover: Volume {center, distance_from_center} */
+    float distance_from_center[3];
+} fsa_entity;
+
+/* Replaces the scene's entity list (count 0 clears it). */
+FSA_API int32_t fsa_scene_set_entities(fsa_scene scene, const fsa_entity* entities, uint32_t count);
+
+/* The global sensors over the scene's current state, reduced on the device
+ * (ParticleStatistics.cpp:17-207): density, pressure and |v| over normal
+ * particles, kinetic and potential energy, particle counts.  Maxima start
+ * at FLT_MIN and minima at FLT_MAX, as the reference's MaxMinAvgSensorData
+ * does (ParticleStatistics.hpp:10-14).  Sums are formed in FP64 (the
+ * reference adds in FP32 in collection order); maxima and minima are exact. */
+typedef struct fsa_scene_statistics {
+    uint64_t normal_particles;
+    uint64_t boundary_particles;
+    uint64_t inactive_particles;
+    float density_max, density_min;
+    double density_avg;
+    float pressure_max, pressure_min;
+    double pressure_avg;
+    float velocity_max, velocity_min;  /* |v| (glm::length) */
+    double velocity_avg;
+    double kinetic_energy;             /* sum of 0.5f * m * |v| * |v|                 */
+    double potential_energy;           /* sum of gravity * m * (y - relative_zero_height) */
+} fsa_scene_statistics;
+
+FSA_API int32_t fsa_scene_get_statistics(fsa_scene scene, float relative_zero_height, fsa_scene_statistics* out);
+
 /* ---- scenario files ----------------------------------------------------- */
 /* Particle data files (.data): an LZ4 frame holding u32 version = 1, u32 number
  * of components, u32 component ids, u64 particle count, then one little-endian
@@ -370,12 +421,17 @@ FSA_API int32_t fsa_data_write(const char* path, uint32_t dimension, const fsa_p
 /* Scenario files: JSON version 2 with the "scenario", "solver" and
  * "timestep-generator" nodes (MainSerializer.cpp:114-171; ScenarioSerializer.cpp:43-79;
  * SolverSerializer.cpp:137-259; TimestepGeneratorSerializer.cpp:47-72).
- * Entities, sensors and the visualizer are not simulated by this library; their
- * presence is reported. */
+ * Sensors and the visualizer are not simulated by this library, nor the 2D
+ * particle remover and spawner entities; their presence is reported. */
 typedef struct fsa_scenario {
     fsa_scene_desc desc;          /* dimension, solver, search, settings and timestep generator */
-    uint32_t entities;            /* entries in scenario.entities (not simulated here)       */
+    uint32_t entities;            /* entries in scenario.entities                            */
     uint32_t sensors;             /* entries in scenario.sensors (not simulated here)        */
+    uint32_t entity_count;        /* the entities this library runs, in file order
+                                     (particle-remover-3d, boundary-preprocessor,
+                                     velocity-alteration-by-tag; EntitySerializer.cpp:66-173) */
+    uint32_t reserved;
+    fsa_entity entity_list[32];
     char particles[1024];         /* scenario.particles as written in the file               */
     char particles_path[4096];    /* the file the reference opens for it
                                      (MainSerializer::get_full_particle_data_path, :216-223)  */
@@ -385,7 +441,7 @@ typedef struct fsa_scenario {
  * kernel combinations and versions other than 2 are errors. */
 FSA_API int32_t fsa_scenario_read(const char* json_path, fsa_scenario* out);
 /* Writes a scenario file (version 2, visualizer "no-visualizer") naming
- * in->particles as its particle file. */
+ * in->particles as its particle file, with in->entity_list as its entities. */
 FSA_API int32_t fsa_scenario_write(const char* json_path, const fsa_scenario* in);
 /* Scene defaults for the dimension / solver and no particle file. */
 FSA_API void fsa_scenario_defaults(fsa_scenario* out, uint32_t dimension, uint32_t solver);

@@ -360,6 +360,8 @@ struct SceneBase {
     virtual int export_raw(uint64_t* off, uint64_t* cols, uint64_t cap, uint64_t* nnz) = 0;
     virtual int upload(const fsa_particle_view* v) = 0;
     virtual float next_timestep() const = 0;
+    virtual void set_entities(const fsa_entity* e, uint32_t count) = 0;
+    virtual void get_statistics(float zero_height, fsa_scene_statistics* out) const = 0;
 };
 
 template <class T, int D>
@@ -723,6 +725,8 @@ struct Scene : SceneBase {
         return ts;
     }
     float next_timestep() const override { return generated_timestep(); }
+    void set_entities(const fsa_entity* e, uint32_t count) override { entities.assign(e, e + count); }
+    void get_statistics(float zero_height, fsa_scene_statistics* out) const override { statistics(zero_height, out); }
 
     V gravity_vec() const {
         V g = vzero<T, D>();
@@ -1144,14 +1148,131 @@ struct Scene : SceneBase {
         return FSA_OK;
     }
 
+    // ---- simulation entities around the solver (Simulator.cpp:26-45)
+    std::vector<fsa_entity> entities;
+
+    // BoundaryPreprocessor::execute_simulation_step and initialize
+    // (BoundaryPreprocessor.cpp:7-62, 96-117): the neighbor lists of the last
+    // search, in their order; float sums and Math::min = std::fmin.
+    void boundary_preprocess() {
+        const float ps = desc.particle_size, rho0 = desc.rest_density;
+        float ideal = 0.0f;
+        for (int x = 0; x <= 0; x++)
+            for (int y = -2; y <= 2; y++)
+                for (int z = -2; z <= 2; z++) {
+                    V p = vzero<T, D>();
+                    p.v[0] = (T)((float)x * ps);
+                    p.v[1] = (T)((float)y * ps);
+                    if (D == 3) p.v[2] = (T)((float)z * ps);
+                    ideal += (float)kernel.value(p);
+                }
+        const float gamma_1 = ((ps * ps) * ps) * ideal;
+        const float max_mass = rho0 * ((ps * ps) * ps);
+        for (size_t i = 0; i < n; i++) {
+            if (type[i] != T_BOUNDARY) continue;
+            float vr = 0.0f;
+            for_neighbors(i, [&](size_t j) {
+                if (type[j] != T_BOUNDARY) return;
+                vr += (float)kernel.value(pos[j], pos[i]);
+            });
+            const float corrected = rho0 * gamma_1 / vr;
+            mass[i] = (T)std::fmin(max_mass, corrected);
+        }
+    }
+
+    // VelocityAlterationByTag::execute_simulation_step (VelocityAlterationByTag.cpp:6-23)
+    void velocity_alteration(const fsa_entity& e) {
+        for (size_t i = 0; i < n; i++) {
+            if (tag[i] != e.tag) continue;
+            for (int k = 0; k < D; k++) vel[i].v[k] = (T)e.velocity[k];
+        }
+    }
+
+    // ParticleRemover3D::execute_simulation_step (ParticleRemover3D.cpp:8-16) with
+    // VolumeGroup::is_inside / is_member (VolumeGroup.cpp:43-58) and GroupActions::remove
+    void particle_remover(const fsa_entity& e) {
+        float lo[3], hi[3];
+        for (int k = 0; k < 3; k++) {
+            hi[k] = e.center[k] + e.distance_from_center[k];
+            lo[k] = e.center[k] - e.distance_from_center[k];
+        }
+        const bool inside_group = !e.remove_if_outside;
+        for (size_t i = 0; i < n; i++) {
+            const float x = (float)pos[i].v[0], y = (float)pos[i].v[1], z = D == 3 ? (float)pos[i].v[2] : 0.0f;
+            const bool inside = !(x < lo[0] || x > hi[0] || y > hi[1] || y < lo[1] || z < lo[2] || z > hi[2]);
+            if (inside == inside_group) type[i] = T_INACTIVE;
+        }
+    }
+
+    void run_entities(bool before) {
+        for (const fsa_entity& e : entities) {
+            if (e.type == FSA_ENTITY_BOUNDARY_PREPROCESSOR && before) boundary_preprocess();
+            else if (e.type == FSA_ENTITY_VELOCITY_ALTERATION_BY_TAG && before) velocity_alteration(e);
+            else if (e.type == FSA_ENTITY_PARTICLE_REMOVER_3D && !before) particle_remover(e);
+        }
+    }
+
+    // the global sensors (ParticleStatistics.cpp:17-207) over the collection in
+    // its order, as the reference computes them: float sums, std::min/std::max
+    // from FLT_MAX / FLT_MIN, the averages divided in float.
+    void statistics(float zero_height, fsa_scene_statistics* out) const {
+        std::memset(out, 0, sizeof(*out));
+        float dmax = std::numeric_limits<float>::min(), dmin = std::numeric_limits<float>::max(), dsum = 0.0f;
+        float pmax = dmax, pmin = dmin, psum = 0.0f, vmax = dmax, vmin = dmin, vsum = 0.0f;
+        float ke = 0.0f, pe = 0.0f;
+        size_t cnt = 0;
+        for (size_t i = 0; i < n; i++) {
+            if (type[i] == T_NORMAL) out->normal_particles++;
+            else if (type[i] == T_BOUNDARY) out->boundary_particles++;
+            else out->inactive_particles++;
+            if (type[i] != T_NORMAL) continue;
+            const float rho = (float)density[i], p = (float)pressure[i], m = (float)mass[i];
+            float v2 = 0.0f;
+            for (int k = 0; k < D; k++) v2 = v2 + (float)vel[i].v[k] * (float)vel[i].v[k];
+            const float v = std::sqrt(v2);
+            dsum += rho;
+            dmin = std::min(dmin, rho);
+            dmax = std::max(dmax, rho);
+            psum += p;
+            pmin = std::min(pmin, p);
+            pmax = std::max(pmax, p);
+            vsum += v;
+            vmin = std::min(vmin, v);
+            vmax = std::max(vmax, v);
+            ke += 0.5f * m * v * v;
+            pe += desc.gravity * m * ((float)pos[i].v[1] - zero_height);
+            cnt++;
+        }
+        if (cnt) {
+            dsum = dsum / cnt;
+            psum = psum / cnt;
+            vsum = vsum / cnt;
+        }
+        out->density_max = dmax;
+        out->density_min = dmin;
+        out->density_avg = dsum;
+        out->pressure_max = pmax;
+        out->pressure_min = pmin;
+        out->pressure_avg = psum;
+        out->velocity_max = vmax;
+        out->velocity_min = vmin;
+        out->velocity_avg = vsum;
+        out->kinetic_energy = ke;
+        out->potential_energy = pe;
+    }
+
     int simulate(float dt, fsa_step_stats* st) override {
         if (!searched) {
             g_err = "simulate called before find_neighbors";
             return FSA_E_STATE;
         }
-        if (desc.solver == FSA_SOLVER_SESPH) return simulate_sesph(dt, st);
-        if (D == 3) return simulate_iisph3d(dt, st);
-        return simulate_iisph2d(dt, st);
+        run_entities(true);
+        int rc;
+        if (desc.solver == FSA_SOLVER_SESPH) rc = simulate_sesph(dt, st);
+        else if (D == 3) rc = simulate_iisph3d(dt, st);
+        else rc = simulate_iisph2d(dt, st);
+        if (rc == FSA_OK) run_entities(false);
+        return rc;
     }
 
     int download(const fsa_particle_view* v) override {
@@ -1361,6 +1482,8 @@ void orc_destroy(void* s) { delete (SceneBase*)s; }
 int orc_find_neighbors(void* s) { return ((SceneBase*)s)->find_neighbors(); }
 int orc_simulate(void* s, float dt, fsa_step_stats* st) { return ((SceneBase*)s)->simulate(dt, st); }
 float orc_next_timestep(void* s) { return ((SceneBase*)s)->next_timestep(); }
+void orc_set_entities(void* s, const fsa_entity* e, uint32_t count) { ((SceneBase*)s)->set_entities(e, count); }
+void orc_statistics(void* s, float zero_height, fsa_scene_statistics* out) { ((SceneBase*)s)->get_statistics(zero_height, out); }
 int orc_download(void* s, const fsa_particle_view* v) { return ((SceneBase*)s)->download(v); }
 int orc_download_iisph(void* s, void* rec, uint64_t stride) { return ((SceneBase*)s)->download_iisph(rec, stride); }
 int orc_export_neighbors(void* s, uint64_t* off, uint64_t* cols, uint64_t cap, uint64_t* nnz) {

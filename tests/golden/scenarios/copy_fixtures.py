@@ -25,6 +25,7 @@ FILES = [
     "3d/single-layer/breaking-dam-1-layer-boundary.json",
     "3d/single-layer/breaking-dam-1-layer-boundary.data",
     "3d/3d-100x100x100-breaking-dam.json",
+    "3d/single-layer/bunny-teapot.json",
     "2d/box.json",
     "2d/box.data",
     "2d/boundary-test.json",

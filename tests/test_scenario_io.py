@@ -359,3 +359,42 @@ def test_scenario_errors(tmp_path):
     j = json.loads(json.dumps(base))
     j["scenario"]["particles"] = "dépôt/\"x\"\\€\U0001F30A.data"
     assert load(j).particles == j["scenario"]["particles"]
+
+
+# ---- entities in scenario files (EntitySerializer.cpp:66-173) ------------------------------------
+def test_scenario_entities_from_reference_file():
+    path = os.path.join(GOLD, "bunny-teapot.json")
+    j = json.load(open(path))
+    sc = S.load_scenario(path, load_particles=False)
+    ents = j["scenario"]["entities"]
+    assert sc.entities == len(ents)
+    runnable = [e for e in ents if e["type"] in ("particle-remover-3d", "boundary-preprocessor",
+                                                 "velocity-alteration-by-tag")]
+    assert len(sc.entity_list) == len(runnable) >= 1
+    for got, e in zip(sc.entity_list, runnable):
+        assert got.type == abi.ENTITY_PARTICLE_REMOVER_3D
+        assert list(got.center) == [f32(x) for x in e["volume"]["center"]]
+        assert list(got.distance_from_center) == [f32(x) for x in e["volume"]["distance-from-center"]]
+        assert got.remove_if_outside == int(e["remove-if-outside"])
+
+
+def test_scenario_entities_round_trip(tmp_path):
+    sc = S.load_scenario(os.path.join(GOLD, "3d-small-breaking-dam.json"), load_particles=False)
+    sc.set_entity_list([abi.boundary_preprocessor(), abi.velocity_alteration_by_tag(7, [0.5, -1.25, 2.0]),
+                        abi.particle_remover_3d([1.0, 2.0, 3.0], [4.0, 5.5, 6.25], True)])
+    out = tmp_path / "ents.json"
+    S.save_scenario(out, sc)
+    j = json.load(open(out))
+    assert [e["type"] for e in j["scenario"]["entities"]] == [
+        "boundary-preprocessor", "velocity-alteration-by-tag", "particle-remover-3d"]
+    back = S.load_scenario(out, load_particles=False)
+    assert [bytes(e) for e in back.entity_list] == [bytes(e) for e in sc.entity_list]
+    # unknown entity types are errors, 2D ones are counted only
+    j["scenario"]["entities"].append({"type": "particle-spawner"})
+    out.write_text(json.dumps(j))
+    back = S.load_scenario(out, load_particles=False)
+    assert back.entities == 4 and len(back.entity_list) == 3
+    j["scenario"]["entities"].append({"type": "teleporter"})
+    out.write_text(json.dumps(j))
+    with pytest.raises(FsaError, match="invalid entity type"):
+        S.load_scenario(out, load_particles=False)
