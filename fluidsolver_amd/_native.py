@@ -7,8 +7,8 @@ import ctypes as C
 import os
 import re
 
-from .abi import (DataHeader, Entity, JobInfo, ParticleView, Scenario, SceneDesc, SceneStatistics, ServerConfig,
-                  ServerStats, StepStats)
+from .abi import (DataHeader, Entity, JobInfo, ParticleView, Scenario, SceneDesc, SceneStatistics, SensorPlane,
+                  ServerConfig, ServerStats, StepStats, VolumeSample)
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "lib", "libfluidsolver_amd.so")
@@ -48,6 +48,10 @@ def _declare(L):
         "fsa_scene_next_timestep": (i32, [vp, P(f32)]),
         "fsa_scene_set_entities": (i32, [vp, P(Entity), u32]),
         "fsa_scene_get_statistics": (i32, [vp, f32, P(SceneStatistics)]),
+        "fsa_scene_query_neighbors": (i32, [vp, vp, u64, vp, vp, u64, P(u64)]),
+        "fsa_sensor_plane_defaults": (None, [P(SensorPlane)]),
+        "fsa_scene_sensor_plane": (i32, [vp, P(SensorPlane), vp]),
+        "fsa_scene_evaluate_volume": (i32, [vp, vp, u64, P(VolumeSample)]),
         "fsa_batch_step": (i32, [vp, P(vp), u32, u32, P(StepStats)]),
         "fsa_batch_set_graph": (i32, [vp, u32]),
         "fsa_context_set_block_size": (i32, [vp, u32]),

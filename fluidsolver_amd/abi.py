@@ -274,6 +274,65 @@ class SceneStatistics(C.Structure):
         return {f[0]: getattr(self, f[0]) for f in self._fields_}
 
 
+# position queries, SensorPlane and the raytracer's volume probe (3D)
+SENSOR_PLANE_DENSITY = 0
+SENSOR_PLANE_VELOCITY = 1
+SENSOR_PLANE_PRESSURE = 2
+
+
+class SensorPlane(C.Structure):
+    """SensorPlane::SensorPlaneSettings (SensorPlane.hpp:17-37)."""
+    _fields_ = [
+        ("origin", C.c_float * 3),
+        ("span_x", C.c_float * 3),
+        ("span_y", C.c_float * 3),
+        ("width", C.c_float),
+        ("height", C.c_float),
+        ("number_of_samples_x", C.c_uint64),
+        ("number_of_samples_y", C.c_uint64),
+        ("max_image_value", C.c_float),
+        ("min_image_value", C.c_float),
+        ("sensor_type", C.c_uint32),
+        ("reserved", C.c_uint32),
+        ("sub_sample_grid_size", C.c_uint64),
+        ("calculate_plane_every_nth_step", C.c_uint64),
+    ]
+
+
+def sensor_plane(origin=(0.0, 0.0, 0.0), span_x=(1.0, 0.0, 0.0), span_y=(0.0, 1.0, 0.0), width=30.0, height=10.0,
+                 samples_x=300, samples_y=100, sensor_type=SENSOR_PLANE_DENSITY, sub_sample_grid_size=2,
+                 max_image_value=1.02, min_image_value=0.98, every_nth_step=1):
+    """A SensorPlane settings record; the defaults are the reference's."""
+    p = SensorPlane()
+    p.origin[:] = [float(v) for v in origin]
+    p.span_x[:] = [float(v) for v in span_x]
+    p.span_y[:] = [float(v) for v in span_y]
+    p.width, p.height = float(width), float(height)
+    p.number_of_samples_x, p.number_of_samples_y = int(samples_x), int(samples_y)
+    p.max_image_value, p.min_image_value = float(max_image_value), float(min_image_value)
+    p.sensor_type = int(sensor_type)
+    p.sub_sample_grid_size = int(sub_sample_grid_size)
+    p.calculate_plane_every_nth_step = int(every_nth_step)
+    return p
+
+
+class VolumeSample(C.Structure):
+    """ParticleIntersectionAccelerator::VolumeEvaluationResult without the position."""
+    _fields_ = [
+        ("density", C.c_float),
+        ("boundary_only_density", C.c_float),
+        ("fluid_only_density", C.c_float),
+        ("fluid_only_normal", C.c_float * 3),
+        ("boundary_only_normal", C.c_float * 3),
+    ]
+
+
+def query_positions(positions):
+    """Query positions as a contiguous (n, 3) float32 array."""
+    q = np.ascontiguousarray(np.asarray(positions, dtype=np.float32).reshape(-1, 3))
+    return q
+
+
 # scenario files
 COMPONENT_MOVEMENT_2D = 1
 COMPONENT_MOVEMENT_3D = 2

@@ -144,6 +144,10 @@ struct fsa_scene_s {
     std::vector<float> carry_err;        // IISPH predicted density errors by caller row for the next upload
     void* d_stats = nullptr;             // sensor reduction scratch
     uint64_t d_stats_bytes = 0;
+    void* d_query = nullptr;             // position queries: positions, counts, offsets
+    uint64_t d_query_bytes = 0;
+    void* d_query_out = nullptr;         // position queries: results
+    uint64_t d_query_out_bytes = 0;
 };
 
 static_assert(sizeof(EntityDev) == sizeof(fsa_entity), "EntityDev mirrors fsa_entity");
@@ -1890,6 +1894,8 @@ void destroy_scene(fsa_scene_s* s) {
     if (s->d_entities) (void)hipFree(s->d_entities);
     if (s->d_tags) (void)hipFree(s->d_tags);
     if (s->d_stats) (void)hipFree(s->d_stats);
+    if (s->d_query) (void)hipFree(s->d_query);
+    if (s->d_query_out) (void)hipFree(s->d_query_out);
     {
         std::lock_guard<std::mutex> g(s->ctx->mu);
         if (s->h_state) s->ctx->free_slots.push_back(s->slot);
@@ -2373,6 +2379,153 @@ FSA_API int32_t fsa_scene_get_statistics(fsa_scene s, float zero_height, fsa_sce
         out->velocity_avg = cnt[0] ? sum[2] / c : 0.0;
         out->kinetic_energy = sum[3];
         out->potential_energy = sum[4];
+    });
+}
+
+// ---- position queries -------------------------------------------------------------------
+// scratch that grows to the largest request and is kept with the scene
+void grow(void*& p, uint64_t& have, uint64_t need) {
+    if (have >= need) return;
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    have = 0;
+    if (hipMalloc(&p, need) != hipSuccess) {
+        p = nullptr;
+        throw FsaError(FSA_E_NOMEM, "device allocation failed");
+    }
+    have = need;
+}
+
+// position queries read the cell structure of the last search over the present layout
+void check_queryable(fsa_scene_s* s) {
+    if (s->hd.dim != 3) throw FsaError(FSA_E_INVALID, "position queries are defined for 3D scenes");
+    if (!s->lists_valid) throw FsaError(FSA_E_STATE, "no search over the present layout: call find_neighbors first");
+}
+
+// positions (count x 3 floats) to float4 scratch at the start of d_query
+float4* upload_queries(fsa_scene_s* s, const float* positions, uint64_t count, uint64_t extra) {
+    if (count >= (1ull << 31)) throw FsaError(FSA_E_INVALID, "too many query positions");
+    if (count && !positions) throw FsaError(FSA_E_INVALID, "positions is NULL");
+    std::vector<float4> q(count);
+    for (uint64_t k = 0; k < count; k++) {
+        const float x = positions[3 * k], y = positions[3 * k + 1], z = positions[3 * k + 2];
+        if (!std::isfinite(x) || !std::isfinite(y) || !std::isfinite(z))
+            throw FsaError(FSA_E_INVALID, "query position is not finite");
+        q[k] = make_float4(x, y, z, 0.0f);
+    }
+    grow(s->d_query, s->d_query_bytes, std::max<uint64_t>(16 * count + extra, 256));
+    if (count)
+        HIP_CHECK(hipMemcpyAsync(s->d_query, q.data(), 16 * count, hipMemcpyHostToDevice, s->ctx->stream));
+    return (float4*)s->d_query;
+}
+
+FSA_API int32_t fsa_scene_query_neighbors(fsa_scene s, const float* positions, uint64_t count, uint64_t* offsets,
+                                          uint64_t* rows, uint64_t capacity, uint64_t* nnz) {
+    return guarded([&] {
+        if (!s || !nnz) throw FsaError(FSA_E_INVALID, "NULL argument");
+        HIP_CHECK(hipSetDevice(s->ctx->device));
+        check_queryable(s);
+        hipStream_t st = s->ctx->stream;
+        const uint64_t cbytes = (4 * count + 15) & ~15ull;
+        float4* q = upload_queries(s, positions, count, cbytes + 8 * count);
+        uint32_t* d_counts = (uint32_t*)((char*)q + 16 * count);
+        uint64_t* d_off = (uint64_t*)((char*)d_counts + cbytes);
+        upload_scene_table(s);
+        launch_query_count(s->d_table, q, (uint32_t)count, d_counts, st);
+        std::vector<uint32_t> cnt(count);
+        if (count) HIP_CHECK(hipMemcpyAsync(cnt.data(), d_counts, 4 * count, hipMemcpyDeviceToHost, st));
+        HIP_CHECK(hipStreamSynchronize(st));
+        std::vector<uint64_t> off(count + 1, 0);
+        for (uint64_t k = 0; k < count; k++) off[k + 1] = off[k] + cnt[k];
+        *nnz = off[count];
+        if (offsets) std::memcpy(offsets, off.data(), 8 * (count + 1));
+        if (!rows) return;
+        if (capacity < off[count]) throw FsaError(FSA_E_INVALID, "output capacity too small");
+        if (!off[count]) return;
+        grow(s->d_query_out, s->d_query_out_bytes, 8 * off[count]);
+        HIP_CHECK(hipMemcpyAsync(d_off, off.data(), 8 * count, hipMemcpyHostToDevice, st));
+        launch_query_fill(s->d_table, q, (uint32_t)count, d_off, (uint64_t*)s->d_query_out, st);
+        HIP_CHECK(hipMemcpyAsync(rows, s->d_query_out, 8 * off[count], hipMemcpyDeviceToHost, st));
+        HIP_CHECK(hipStreamSynchronize(st));
+    });
+}
+
+FSA_API void fsa_sensor_plane_defaults(fsa_sensor_plane* p) {
+    if (!p) return;
+    std::memset(p, 0, sizeof(*p));
+    p->span_x[0] = 1.0f;
+    p->span_y[1] = 1.0f;
+    p->width = 30.0f;
+    p->height = 10.0f;
+    p->number_of_samples_x = 300;
+    p->number_of_samples_y = 100;
+    p->max_image_value = 1.02f;
+    p->min_image_value = 0.98f;
+    p->sensor_type = FSA_SENSOR_PLANE_DENSITY;
+    p->sub_sample_grid_size = 2;
+    p->calculate_plane_every_nth_step = 1;
+}
+
+FSA_API int32_t fsa_scene_sensor_plane(fsa_scene s, const fsa_sensor_plane* p, float* values) {
+    return guarded([&] {
+        if (!s || !p || !values) throw FsaError(FSA_E_INVALID, "NULL argument");
+        HIP_CHECK(hipSetDevice(s->ctx->device));
+        check_queryable(s);
+        if (p->sensor_type > FSA_SENSOR_PLANE_PRESSURE) throw FsaError(FSA_E_INVALID, "unknown sensor plane type");
+        const uint64_t samples = p->number_of_samples_x * p->number_of_samples_y;
+        if (p->number_of_samples_x == 0 || p->number_of_samples_y == 0 || p->number_of_samples_x >= (1ull << 24) ||
+            p->number_of_samples_y >= (1ull << 24) || samples >= (1ull << 31))
+            throw FsaError(FSA_E_INVALID, "sample counts must be positive and below 2^24 each, 2^31 in all");
+        if (p->sub_sample_grid_size >= 4096) throw FsaError(FSA_E_INVALID, "sub_sample_grid_size must be below 4096");
+        for (int k = 0; k < 3; k++)
+            if (!std::isfinite(p->origin[k]) || !std::isfinite(p->span_x[k]) || !std::isfinite(p->span_y[k]))
+                throw FsaError(FSA_E_INVALID, "sensor plane geometry is not finite");
+        if (!std::isfinite(p->width) || !std::isfinite(p->height))
+            throw FsaError(FSA_E_INVALID, "sensor plane geometry is not finite");
+        // glm::normalize(v) = v * (1 / sqrt(dot(v, v))); x_step = width / number_of_samples_x
+        PlaneParams P{};
+        const float x_step = p->width / (float)p->number_of_samples_x;
+        const float y_step = p->height / (float)p->number_of_samples_y;
+        const float* sp[2] = {p->span_x, p->span_y};
+        float* dst[2] = {P.ax, P.ay};
+        const float step[2] = {x_step, y_step};
+        for (int a = 0; a < 2; a++) {
+            const float* v = sp[a];
+            const float d = (v[0] * v[0] + v[1] * v[1]) + v[2] * v[2];
+            const float inv = 1.0f / std::sqrt(d);
+            for (int k = 0; k < 3; k++) dst[a][k] = (v[k] * inv) * step[a];
+        }
+        for (int k = 0; k < 3; k++) P.origin[k] = p->origin[k];
+        P.nx = (uint32_t)p->number_of_samples_x;
+        P.ny = (uint32_t)p->number_of_samples_y;
+        P.sub = (uint32_t)p->sub_sample_grid_size;
+        P.type = p->sensor_type;
+        P.sub_den = (float)p->sub_sample_grid_size + 1.0f;
+        const float g = (float)(p->sub_sample_grid_size + 1);
+        P.div = g * g;
+        hipStream_t st = s->ctx->stream;
+        grow(s->d_query_out, s->d_query_out_bytes, 12 * samples);
+        upload_scene_table(s);
+        launch_sensor_plane(s->d_table, P, (float*)s->d_query_out, st);
+        HIP_CHECK(hipMemcpyAsync(values, s->d_query_out, 12 * samples, hipMemcpyDeviceToHost, st));
+        HIP_CHECK(hipStreamSynchronize(st));
+    });
+}
+
+FSA_API int32_t fsa_scene_evaluate_volume(fsa_scene s, const float* positions, uint64_t count,
+                                          fsa_volume_sample* out) {
+    return guarded([&] {
+        if (!s || (count && !out)) throw FsaError(FSA_E_INVALID, "NULL argument");
+        HIP_CHECK(hipSetDevice(s->ctx->device));
+        check_queryable(s);
+        hipStream_t st = s->ctx->stream;
+        float4* q = upload_queries(s, positions, count, 0);
+        if (!count) return;
+        grow(s->d_query_out, s->d_query_out_bytes, 36 * count);
+        upload_scene_table(s);
+        launch_volume(s->d_table, q, (uint32_t)count, (float*)s->d_query_out, st);
+        HIP_CHECK(hipMemcpyAsync(out, s->d_query_out, 36 * count, hipMemcpyDeviceToHost, st));
+        HIP_CHECK(hipStreamSynchronize(st));
     });
 }
 

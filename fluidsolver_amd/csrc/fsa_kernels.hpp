@@ -37,6 +37,18 @@ constexpr int kTuneFamilies = 4;
 // simulation entities and sensors
 void launch_entities_before(const LaunchSet& L, hipStream_t st);
 void launch_remover_check(const LaunchSet& L, hipStream_t st);
+void launch_query_count(const SceneDev* d_scene, const float4* q, uint32_t nq, uint32_t* counts, hipStream_t st);
+void launch_query_fill(const SceneDev* d_scene, const float4* q, uint32_t nq, const uint64_t* offsets, uint64_t* out,
+                       hipStream_t st);
+// SensorPlane sample geometry: ax = normalize(span_x) * x_step, ay likewise
+struct PlaneParams {
+    float origin[3], ax[3], ay[3];
+    uint32_t nx, ny, sub, type;
+    float sub_den;  // sub_sample_grid_size + 1.0f
+    float div;      // Math::pow2(sub_sample_grid_size + 1)
+};
+void launch_sensor_plane(const SceneDev* d_scene, const PlaneParams& P, float* out, hipStream_t st);
+void launch_volume(const SceneDev* d_scene, const float4* q, uint32_t nq, float* out, hipStream_t st);
 uint64_t statistics_scratch_bytes(uint32_t nblocks);
 void launch_statistics(const SceneDev* d_scene, uint32_t nblocks, float zero_height, void* scratch, hipStream_t st);
 void read_statistics(const void* host_copy, uint64_t counts[3], float mx[3], float mn[3], double sums[5]);

@@ -192,6 +192,34 @@ class Scene:
         check(lib().fsa_scene_get_statistics(self._h, float(relative_zero_height), C.byref(out)))
         return out
 
+    def query_neighbors(self, positions):
+        """NeighborhoodInterface::get_neighbors(position) for each row of
+        positions (n x 3), on the device: CSR (offsets, rows) of the caller
+        rows near each position, in the search's iteration order."""
+        q = abi.query_positions(positions)
+        n = q.shape[0]
+        off = np.zeros(n + 1, dtype=np.uint64)
+        nnz = C.c_uint64(0)
+        check(lib().fsa_scene_query_neighbors(self._h, q.ctypes.data, n, off.ctypes.data, None, 0, C.byref(nnz)))
+        rows = np.zeros(max(nnz.value, 1), dtype=np.uint64)
+        check(lib().fsa_scene_query_neighbors(self._h, q.ctypes.data, n, off.ctypes.data, rows.ctypes.data,
+                                              rows.size, C.byref(nnz)))
+        return off, rows[:nnz.value]
+
+    def sensor_plane(self, plane):
+        """SensorPlane values (abi.SensorPlane settings) over the current state:
+        (samples_x * samples_y, 3) float32 in the reference's sample order."""
+        out = np.zeros((plane.number_of_samples_x * plane.number_of_samples_y, 3), dtype=np.float32)
+        check(lib().fsa_scene_sensor_plane(self._h, C.byref(plane), out.ctypes.data))
+        return out
+
+    def evaluate_volume(self, positions):
+        """The raytracer's volume probe at each position (abi.VolumeSample array)."""
+        q = abi.query_positions(positions)
+        out = (abi.VolumeSample * max(q.shape[0], 1))()
+        check(lib().fsa_scene_evaluate_volume(self._h, q.ctypes.data, q.shape[0], out))
+        return out[:q.shape[0]]
+
     def next_timestep(self):
         """The desired step of the scene's timestep generator (constant or dynamic CFL)."""
         dt = C.c_float(0.0)
@@ -324,8 +352,15 @@ class HipNeighborhoodSearch:
     def find_neighbors(self):
         self._solver._scene_or_create().find_neighbors()
 
-    def get_neighbors(self, index):
-        return self._solver._scene_or_create().get_neighbors(index)
+    def get_neighbors(self, index_or_position):
+        """NeighborhoodInterface::get_neighbors (NeighborhoodInterface.hpp:86-89):
+        by particle index, or by a 3D position (link.get_by_position_3d),
+        answered on the device in the search's order."""
+        scene = self._solver._scene_or_create()
+        if np.ndim(index_or_position) == 0:
+            return scene.get_neighbors(int(index_or_position))
+        off, rows = scene.query_neighbors(np.asarray(index_or_position, dtype=np.float32).reshape(1, 3))
+        return rows
 
 
 class _HipFluidSolverBase:

@@ -385,6 +385,71 @@ typedef struct fsa_scene_statistics {
 
 FSA_API int32_t fsa_scene_get_statistics(fsa_scene scene, float relative_zero_height, fsa_scene_statistics* out);
 
+/* ---- position queries (3D) ------------------------------------------------ */
+/* NeighborhoodInterface::get_neighbors(position) (NeighborhoodInterface.cpp; the
+ * position iterators of CompressedNeighbors.cpp:529-595,
+ * HashedNeighborhoodSearch3D.cpp:272-349, QuadraticNeighborhoodSearch3D.cpp:199-219)
+ * for many positions at once, on the device.  The particles near each
+ * position in the scene search's iteration order, as caller rows:
+ *   compressed  (p - x).(p - x) <= search_radius^2, 27 cells by ascending key;
+ *   hashed      length(x - p) <= search_radius, cells dx-outer, members ascending;
+ *   quadratic   length(x - p) <= search_radius, ascending particle index.
+ * Queries see the cell structure of the last search and the current positions:
+ * the reference's state between a step and the next search.  Inactive particles
+ * are never reported (the reference's quadratic iterator lists them; its
+ * callers either skip them or, SensorPlane, would weigh them at their old
+ * positions).  FSA_E_STATE when the layout changed since the last search (an
+ * upload, or particles a remover took out): call fsa_scene_find_neighbors.
+ * positions: count x 3 floats, finite.  offsets (count + 1 entries, may be
+ * NULL) receives the CSR offsets, rows (may be NULL: count only) the rows;
+ * *nnz = total.  FSA_E_INVALID when capacity < *nnz (offsets and *nnz are
+ * still written). */
+FSA_API int32_t fsa_scene_query_neighbors(fsa_scene scene, const float* positions, uint64_t count, uint64_t* offsets,
+                                          uint64_t* rows, uint64_t capacity, uint64_t* nnz);
+
+/* SensorPlane (SensorPlane.cpp:75-137; settings SensorPlane.hpp:17-37): a grid
+ * of samples on a plane, each the mean over (sub_sample_grid_size + 1)^2
+ * sub-samples of sum over the neighbors (as fsa_scene_query_neighbors) of
+ * (mass, 0, 0), velocity or (0, 0, pressure) times W(|sample - p|), in float in
+ * the search's order.  values: number_of_samples_x * number_of_samples_y x 3
+ * floats, sample i at row y = number_of_samples_y - 1 - i / number_of_samples_x,
+ * column x = i % number_of_samples_x (the reference's output order). */
+#define FSA_SENSOR_PLANE_DENSITY 0
+#define FSA_SENSOR_PLANE_VELOCITY 1
+#define FSA_SENSOR_PLANE_PRESSURE 2
+typedef struct fsa_sensor_plane {
+    float origin[3];
+    float span_x[3];              /* normalized by the call                        */
+    float span_y[3];
+    float width, height;
+    uint64_t number_of_samples_x, number_of_samples_y;
+    float max_image_value, min_image_value;  /* image range (not used by the values) */
+    uint32_t sensor_type;         /* FSA_SENSOR_PLANE_*                             */
+    uint32_t reserved;
+    uint64_t sub_sample_grid_size;
+    uint64_t calculate_plane_every_nth_step; /* not used by the call                */
+} fsa_sensor_plane;
+FSA_API void fsa_sensor_plane_defaults(fsa_sensor_plane* plane);
+FSA_API int32_t fsa_scene_sensor_plane(fsa_scene scene, const fsa_sensor_plane* plane, float* values);
+
+/* ParticleIntersectionAccelerator::evaluate_volume_at_position
+ * (ParticleIntersectionAccelerator.cpp:97-131), the raytracer's volume probe:
+ * over the neighbors of each position, in the search's order, the density
+ * sum of m W(|x - p|), split into fluid-only and boundary-only parts, and the
+ * normals sum of m * GetKernelDerivativeValue(p, x) by type.  The raytracer
+ * builds its own compressed search over the current positions: call
+ * fsa_scene_find_neighbors on a compressed-search scene first for the same
+ * neighbor order. */
+typedef struct fsa_volume_sample {
+    float density;
+    float boundary_only_density;
+    float fluid_only_density;
+    float fluid_only_normal[3];
+    float boundary_only_normal[3];
+} fsa_volume_sample;
+FSA_API int32_t fsa_scene_evaluate_volume(fsa_scene scene, const float* positions, uint64_t count,
+                                          fsa_volume_sample* out);
+
 /* ---- scenario files ----------------------------------------------------- */
 /* Particle data files (.data): an LZ4 frame holding u32 version = 1, u32 number
  * of components, u32 component ids, u64 particle count, then one little-endian

@@ -362,6 +362,10 @@ struct SceneBase {
     virtual float next_timestep() const = 0;
     virtual void set_entities(const fsa_entity* e, uint32_t count) = 0;
     virtual void get_statistics(float zero_height, fsa_scene_statistics* out) const = 0;
+    virtual int query_neighbors(const float* x, uint64_t count, uint64_t* off, uint64_t* rows, uint64_t cap,
+                                uint64_t* nnz) = 0;
+    virtual int sensor_plane(const fsa_sensor_plane* p, float* values) = 0;
+    virtual int evaluate_volume(const float* x, uint64_t count, fsa_volume_sample* out) = 0;
 };
 
 template <class T, int D>
@@ -389,6 +393,11 @@ struct Scene : SceneBase {
     // improve_cache_efficiency
     std::vector<std::array<int32_t, 3>> grid_current;
     size_t calls_since_sort = 0;
+    // the grid of the last hashed 3D search: each cell's non-inactive particles
+    // ascending.  The reference's grid also keeps particles that turned inactive
+    // after their insertion (update_grid skips them, HashedNeighborhoodSearch3D.cpp:111-114);
+    // this grid holds the non-inactive particles of the last search only.
+    std::map<std::array<int32_t, 3>, std::vector<uint64_t>> grid3;
     // hashed 2D grid state (HashedNeighborhoodSearch.cpp:117-151)
     std::map<std::pair<int32_t, int32_t>, std::list<uint64_t>> grid;
     std::vector<std::pair<int32_t, int32_t>> current_status;
@@ -587,7 +596,8 @@ struct Scene : SceneBase {
             permute_all(quick_sort_perm(keys));
             calls_since_sort = 0;
         }
-        std::map<std::array<int32_t, 3>, std::vector<uint64_t>> grid;
+        grid3.clear();
+        auto& grid = grid3;
         for (size_t i = 0; i < n; i++) {
             if (type[i] == T_INACTIVE) continue;
             grid_current[i] = cell_of(pos[i]);
@@ -727,6 +737,157 @@ struct Scene : SceneBase {
     float next_timestep() const override { return generated_timestep(); }
     void set_entities(const fsa_entity* e, uint32_t count) override { entities.assign(e, e + count); }
     void get_statistics(float zero_height, fsa_scene_statistics* out) const override { statistics(zero_height, out); }
+
+    // ---- NeighborhoodInterface::get_neighbors(position), 3D
+    // compressed: find_neighbors_and_save_in_storage(position) (CompressedNeighbors.cpp:536-595):
+    //   the 27 cell keys sorted, each run walked while its key matches the
+    //   particle's stored cell_index, test dot(p - x, p - x) <= pow2(search_radius);
+    // hashed: NeighborsIterator::operator++ (HashedNeighborhoodSearch3D.cpp:272-349):
+    //   cells dx, dy, dz from -1 to 1 (dz fastest), members in set order,
+    //   test length(x - p) <= search_radius;
+    // quadratic: NeighborsIterator::operator++ (QuadraticNeighborhoodSearch3D.cpp:199-219):
+    //   every particle by index, inactive ones included, test length(x - p) <= search_radius.
+    template <class F>
+    void for_position(const V& x, F&& fn) const {
+        if (search == SEARCH_COMPRESSED3D) {
+            const auto c = cell_of(x);
+            std::array<uint64_t, 27> keys;
+            int q = 0;
+            for (int a = -1; a <= 1; a++)
+                for (int b = -1; b <= 1; b++)
+                    for (int z = -1; z <= 1; z++) keys[q++] = key_of({c[0] + a, c[1] + b, c[2] + z});
+            std::sort(keys.begin(), keys.end());
+            const T r2 = search_radius * search_radius;
+            for (int k = 0; k < 27; k++) {
+                size_t cur = run_lookup(keys[k]);
+                if (cur == (size_t)-1) continue;
+                while (keys[k] == cell_key[cur]) {
+                    V diff = pos[cur] - x;
+                    if (dot(diff, diff) <= r2) fn(cur);
+                    cur++;
+                    if (cur >= n) break;
+                }
+            }
+        } else if (search == SEARCH_HASHED3D) {
+            const auto c = cell_of(x);
+            for (int dx = -1; dx <= 1; dx++)
+                for (int dy = -1; dy <= 1; dy++)
+                    for (int dz = -1; dz <= 1; dz++) {
+                        auto it = grid3.find({c[0] + dx, c[1] + dy, c[2] + dz});
+                        if (it == grid3.end()) continue;
+                        for (uint64_t j : it->second)
+                            if (length(x - pos[j]) <= search_radius) fn((size_t)j);
+                    }
+        } else {
+            for (size_t j = 0; j < n; j++)
+                if (length(x - pos[j]) <= search_radius) fn(j);
+        }
+    }
+    bool position_queries_supported() const {
+        if (D == 3 && (search == SEARCH_COMPRESSED3D || search == SEARCH_HASHED3D || search == SEARCH_QUADRATIC3D))
+            return true;
+        g_err = "position queries are defined for the 3D searches";
+        return false;
+    }
+    V point(const float* x) const {
+        V p = vzero<T, D>();
+        for (int k = 0; k < D; k++) p.v[k] = (T)x[k];
+        return p;
+    }
+    int query_neighbors(const float* x, uint64_t count, uint64_t* off, uint64_t* rows, uint64_t cap,
+                        uint64_t* nnz) override {
+        if (!position_queries_supported()) return FSA_E_INVALID;
+        uint64_t total = 0;
+        if (off) off[0] = 0;
+        for (uint64_t k = 0; k < count; k++) {
+            for_position(point(x + 3 * k), [&](size_t j) {
+                if (rows && total < cap) rows[total] = id[j];
+                total++;
+            });
+            if (off) off[k + 1] = total;
+        }
+        *nnz = total;
+        if (rows && total > cap) {
+            g_err = "output capacity too small";
+            return FSA_E_INVALID;
+        }
+        return FSA_OK;
+    }
+    // SensorPlane::calculate_for_timepoint (SensorPlane.cpp:75-137): glm::normalize
+    // = v * (1 / sqrt(dot(v, v))), the sample position
+    // origin + span_x * x_step * (x + sub_x) + span_y * y_step * (y + sub_y),
+    // value += (mass, 0, 0) | velocity | (0, 0, pressure) times
+    // GetKernelValue(p, sample), value /= Math::pow2(float(sub + 1)).
+    int sensor_plane(const fsa_sensor_plane* sp, float* values) override {
+        if (!position_queries_supported()) return FSA_E_INVALID;
+        auto normalize = [](const float* v) {
+            V r = vzero<T, D>();
+            const T d = ((T)v[0] * (T)v[0] + (T)v[1] * (T)v[1]) + (T)v[2] * (T)v[2];
+            const T inv = T(1) / std::sqrt(d);
+            for (int k = 0; k < D; k++) r.v[k] = (T)v[k] * inv;
+            return r;
+        };
+        const V span_x = normalize(sp->span_x), span_y = normalize(sp->span_y);
+        const V origin = point(sp->origin);
+        const T x_step = (T)sp->width / (T)sp->number_of_samples_x;
+        const T y_step = (T)sp->height / (T)sp->number_of_samples_y;
+        const size_t nx = sp->number_of_samples_x, ny = sp->number_of_samples_y, sub = sp->sub_sample_grid_size;
+        for (size_t i = 0; i < nx * ny; i++) {
+            const size_t y = ny - 1 - i / nx, x = i % nx;
+            V value = vzero<T, D>();
+            for (size_t sxc = 0; sxc <= sub; sxc++) {
+                const T sub_x = (T)sxc / ((T)sub + T(1)) - T(0.5);
+                for (size_t syc = 0; syc <= sub; syc++) {
+                    const T sub_y = (T)syc / ((T)sub + T(1)) - T(0.5);
+                    const V sample = origin + span_x * x_step * ((T)x + sub_x) + span_y * y_step * ((T)y + sub_y);
+                    for_position(sample, [&](size_t j) {
+                        const T w = kernel.value(pos[j], sample);
+                        V term = vzero<T, D>();
+                        if (sp->sensor_type == FSA_SENSOR_PLANE_DENSITY) term.v[0] = mass[j];
+                        else if (sp->sensor_type == FSA_SENSOR_PLANE_VELOCITY) term = vel[j];
+                        else term.v[D - 1] = pressure[j];  // z (3D only)
+                        value = value + term * w;
+                    });
+                }
+            }
+            const T g = (T)(sub + 1);
+            value = value / (g * g);
+            for (int k = 0; k < D; k++) values[3 * i + k] = (float)value.v[k];
+        }
+        return FSA_OK;
+    }
+    // ParticleIntersectionAccelerator::evaluate_volume_at_position
+    // (ParticleIntersectionAccelerator.cpp:97-131)
+    int evaluate_volume(const float* xs, uint64_t count, fsa_volume_sample* out) override {
+        if (!position_queries_supported()) return FSA_E_INVALID;
+        for (uint64_t k = 0; k < count; k++) {
+            const V x = point(xs + 3 * k);
+            T rho = 0, rb = 0, rf = 0;
+            V fnrm = vzero<T, D>(), bnrm = vzero<T, D>();
+            for_position(x, [&](size_t j) {
+                if (type[j] == T_INACTIVE) return;
+                const T contribution = mass[j] * kernel.value(pos[j], x);
+                const V normal_contribution = mass[j] * kernel.derivative(x - pos[j]);
+                rho += contribution;
+                if (type[j] == T_NORMAL) {
+                    rf += contribution;
+                    fnrm = fnrm + normal_contribution;
+                } else if (type[j] == T_BOUNDARY) {
+                    rb += contribution;
+                    bnrm = bnrm + normal_contribution;
+                }
+            });
+            fsa_volume_sample& o = out[k];
+            o.density = (float)rho;
+            o.boundary_only_density = (float)rb;
+            o.fluid_only_density = (float)rf;
+            for (int a = 0; a < 3; a++) {
+                o.fluid_only_normal[a] = a < D ? (float)fnrm.v[a] : 0.0f;
+                o.boundary_only_normal[a] = a < D ? (float)bnrm.v[a] : 0.0f;
+            }
+        }
+        return FSA_OK;
+    }
 
     V gravity_vec() const {
         V g = vzero<T, D>();
@@ -1484,6 +1645,14 @@ int orc_simulate(void* s, float dt, fsa_step_stats* st) { return ((SceneBase*)s)
 float orc_next_timestep(void* s) { return ((SceneBase*)s)->next_timestep(); }
 void orc_set_entities(void* s, const fsa_entity* e, uint32_t count) { ((SceneBase*)s)->set_entities(e, count); }
 void orc_statistics(void* s, float zero_height, fsa_scene_statistics* out) { ((SceneBase*)s)->get_statistics(zero_height, out); }
+int orc_query_neighbors(void* s, const float* x, uint64_t count, uint64_t* off, uint64_t* rows, uint64_t cap,
+                        uint64_t* nnz) {
+    return ((SceneBase*)s)->query_neighbors(x, count, off, rows, cap, nnz);
+}
+int orc_sensor_plane(void* s, const fsa_sensor_plane* p, float* values) { return ((SceneBase*)s)->sensor_plane(p, values); }
+int orc_evaluate_volume(void* s, const float* x, uint64_t count, fsa_volume_sample* out) {
+    return ((SceneBase*)s)->evaluate_volume(x, count, out);
+}
 int orc_download(void* s, const fsa_particle_view* v) { return ((SceneBase*)s)->download(v); }
 int orc_download_iisph(void* s, void* rec, uint64_t stride) { return ((SceneBase*)s)->download_iisph(rec, stride); }
 int orc_export_neighbors(void* s, uint64_t* off, uint64_t* cols, uint64_t cap, uint64_t* nnz) {

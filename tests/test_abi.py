@@ -57,6 +57,20 @@ def test_null_arguments_return_status():
     assert L.fsa_server_tick(None, None) == abi.FSA_E_INVALID
     assert L.fsa_server_submit(None, None, None, 1, None) == abi.FSA_E_INVALID
     assert L.fsa_server_get_stats(None, None) == abi.FSA_E_INVALID
+    nnz = C.c_uint64(0)
+    assert L.fsa_scene_query_neighbors(None, None, 0, None, None, 0, C.byref(nnz)) == abi.FSA_E_INVALID
+    assert L.fsa_scene_sensor_plane(None, None, None) == abi.FSA_E_INVALID
+    assert L.fsa_scene_evaluate_volume(None, None, 0, None) == abi.FSA_E_INVALID
+
+
+def test_sensor_plane_defaults_match_reference():
+    """SensorPlaneSettings defaults (SensorPlane.hpp:17-37), from the library and the Python builder."""
+    p = abi.SensorPlane()
+    _native.lib().fsa_sensor_plane_defaults(C.byref(p))
+    assert bytes(p) == bytes(abi.sensor_plane())
+    assert (p.width, p.height, p.number_of_samples_x, p.number_of_samples_y, p.sub_sample_grid_size) == (
+        30.0, 10.0, 300, 100, 2)
+    assert list(p.span_x) == [1.0, 0.0, 0.0] and list(p.span_y) == [0.0, 1.0, 0.0]
 
 
 LAYOUT_C = r"""
@@ -70,6 +84,8 @@ int main(void) {
   printf("%zu %zu %zu\n", sizeof(fsa_server_config), sizeof(fsa_job_info), sizeof(fsa_server_stats));
   printf("%zu %zu %zu\n", offsetof(fsa_job_info, last), offsetof(fsa_server_stats, busy_s),
          offsetof(fsa_server_stats, max_step_ms));
+  printf("%zu %zu %zu\n", sizeof(fsa_sensor_plane), offsetof(fsa_sensor_plane, sub_sample_grid_size),
+         sizeof(fsa_volume_sample));
   return 0;
 }
 """
@@ -92,6 +108,9 @@ def test_struct_layouts_match_header(tmp_path):
     assert sizes[9] == abi.JobInfo.last.offset
     assert sizes[10] == abi.ServerStats.busy_s.offset
     assert sizes[11] == abi.ServerStats.max_step_ms.offset
+    assert sizes[12] == C.sizeof(abi.SensorPlane)
+    assert sizes[13] == abi.SensorPlane.sub_sample_grid_size.offset
+    assert sizes[14] == C.sizeof(abi.VolumeSample)
 
 
 def test_info_record_layout():
