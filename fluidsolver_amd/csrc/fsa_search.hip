@@ -290,12 +290,17 @@ __global__ __launch_bounds__(kBlock) void k_cell_table(const SceneDev* __restric
 // cells by start (= by cell key, the reference's std::sort of the 27 keys), so
 // candidate g of the concatenated runs has ascending combined index.  Up to
 // kListChunks chunks of 64 candidates are held one per lane; the cell's fluid
-// particles (one per lane as well) are taken in turn, each tested against the
-// held chunks with one ballot per chunk, whose prefix (on top of the particle's
-// running count, a scalar) gives each accepted candidate its slot, so every list
-// is written in ascending order without staging.
+// particles, kStageParticles at a time, are taken in turn, each tested against
+// the held chunks with one ballot per chunk, whose prefix (on top of the
+// particle's running count, a scalar) gives each accepted candidate its slot in
+// the particle's row of an LDS stage.  The group's lists then leave the stage
+// row by row: one store covers 64 / kStageParticles list rows of the group's
+// consecutive fluid ranks, instead of one scattered store per particle and chunk.
+// The next cell's table entries and first particle group are loaded while the
+// current cell is processed.
 
 constexpr uint32_t kListChunks = 4;
+constexpr uint32_t kStageParticles = 16;
 
 // one compare-exchange stage of a bitonic sort within 32-lane groups; the
 // partner comes from ds_swizzle's bit mode (and 0x1f, xor J), which needs no
@@ -364,12 +369,14 @@ __device__ __forceinline__ uint32_t mul_u24(uint32_t v, uint32_t s) {
 template <bool Wide>
 __global__ __launch_bounds__(kBlock) void k_lists(const SceneDev* __restrict__ scenes,
                                                   const BlockRef* __restrict__ map, uint32_t sbuf) {
+    __shared__ uint32_t stage_all[kBlock / 64][kStageParticles * kMaxNeighbors];
     const BlockRef br = map[blockIdx.x];
     const SceneDev& S = scenes[br.scene];
     const uint32_t wave = threadIdx.x / 64;
     const uint32_t lane = __lane_id();
     const uint32_t base = br.block * kBlock + wave * 64;
     if (base >= S.n) return;
+    uint32_t* const stage = stage_all[wave];
     const uint32_t c = S.state->cur;
     const auto pos = gload_ptr(S.pos[c]);
     const auto idt = gload_ptr(S.idt[c]);
@@ -387,33 +394,68 @@ __global__ __launch_bounds__(kBlock) void k_lists(const SceneDev* __restrict__ s
         start = mi == 0 || keys[mi - 1] != mykey;
     }
     uint64_t starts = __ballot(start);
+    if (starts == 0) return;
     bool overflow = false;
-    while (starts) {
-        const uint32_t sl0 = (uint32_t)__ffsll((unsigned long long)starts) - 1u;
-        const uint32_t cs = base + sl0;
-        starts &= starts - 1;
-        // the cell of the run, from its key (no position load, no division)
+    // readout lanes: particle column lane % kStageParticles, row offset lane / kStageParticles
+    const uint32_t col = lane % kStageParticles;
+    const uint32_t row_step = 64u / kStageParticles;
+    // The loads a cell starts from: the table entries of its 27 neighbor cells
+    // (lanes 0..26; lane 13 is the cell itself) and the first particle group's
+    // type, position and rank.  They depend on the cell's key only, so the next
+    // cell's are issued while the current one is processed.
+    auto cell_loads = [&](uint32_t sl, uint4& ent, uint32_t& gidt, float4& gpos, uint32_t& grank) {
         int32_t cx, cy, cz;
-        cell_of_key(readlane_u(mykey, sl0), S.levels, cx, cy, cz);
-        uint32_t lin0;
-        cell_linear(S, cx, cy, cz, lin0);
-        const uint32_t ce = S.cells[lin0].y;
-        bool anyf = false;
-        for (uint32_t t = cs + lane; t < ce; t += 64) anyf |= (idt[t] >> kTypeShift) == 0u;
-        if (!__any(anyf)) continue;
-        // runs of the 27 neighbor cells in lanes 0..26, sorted by start; each
-        // run's end rides along by lane and is fetched back after the sort
-        uint32_t key = kEmptyCell, cend = 0;
+        cell_of_key(readlane_u(mykey, sl), S.levels, cx, cy, cz);
+        ent = make_uint4(0u, 0u, 0u, 0u);  // stamp 0 never matches: an absent cell
         if (lane < 27) {
             uint32_t lin;
             if (cell_linear(S, cx + (int)(lane / 9) - 1, cy + (int)((lane / 3) % 3) - 1, cz + (int)(lane % 3) - 1,
-                            lin)) {
-                const uint4 e = S.cells[lin];
-                if (e.z == S.stamp) {
-                    key = (e.x << 5) | lane;
-                    cend = e.y;
-                }
-            }
+                            lin))
+                ent = S.cells[lin];
+        }
+        const uint32_t p = base + sl + lane;
+        gidt = 0xffffffffu;  // not a fluid particle
+        gpos = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        grank = 0;
+        if (lane < kStageParticles && p < S.n) {
+            gidt = idt[p];
+            gpos = pos[p];
+            grank = rank_of[p];
+        }
+    };
+    uint32_t sl0 = (uint32_t)__ffsll((unsigned long long)starts) - 1u;
+    starts &= starts - 1;
+    uint4 ent;
+    uint32_t gidt, grank;
+    float4 gpos;
+    cell_loads(sl0, ent, gidt, gpos, grank);
+    while (true) {
+        const uint32_t cs = base + sl0;
+        const uint4 e = ent;
+        const uint32_t idt0 = gidt, rank0 = grank;
+        const float4 pos0 = gpos;
+        const uint32_t ce = readlane_u(e.y, 13);
+        const bool more = starts != 0;
+        uint32_t nsl = 0;
+        if (more) {
+            nsl = (uint32_t)__ffsll((unsigned long long)starts) - 1u;
+            starts &= starts - 1;
+        }
+        // a cell without fluid particles needs no lists
+        const bool f0 = lane < kStageParticles && cs + lane < ce && (idt0 >> kTypeShift) == 0u;
+        const bool work = __ballot(f0) != 0 || ce - cs > kStageParticles;
+        // runs of the 27 neighbor cells in lanes 0..26, sorted by start; each
+        // run's end rides along by lane and is fetched back after the sort
+        uint32_t key = kEmptyCell, cend = 0;
+        if (lane < 27 && e.z == S.stamp) {
+            key = (e.x << 5) | lane;
+            cend = e.y;
+        }
+        if (more) cell_loads(nsl, ent, gidt, gpos, grank);
+        if (!work) {
+            if (!more) break;
+            sl0 = nsl;
+            continue;
         }
         key = sort32_u32(key);
         const uint32_t nvalid = (uint32_t)__popcll(__ballot(lane < 32 && key != kEmptyCell));
@@ -425,18 +467,22 @@ __global__ __launch_bounds__(kBlock) void k_lists(const SceneDev* __restrict__ s
         }
         uint32_t total;
         const uint32_t roff = exclusive_scan32(rl, total);
-        for (uint32_t pbase = cs; pbase < ce; pbase += 64) {
-            // the cell's particles, one per lane
+        for (uint32_t pbase = cs; pbase < ce; pbase += kStageParticles) {
+            // the group's particles in lanes 0..kStageParticles-1 (the first group's were loaded ahead)
             const uint32_t pi = pbase + lane;
-            const bool isf = pi < ce && (idt[pi] >> kTypeShift) == 0u;
+            uint32_t ti = idt0, kl = rank0;
+            float4 xl = pos0;
+            if (pbase != cs) {
+                ti = 0xffffffffu;
+                if (lane < kStageParticles && pi < ce) {
+                    ti = idt[pi];
+                    xl = pos[pi];
+                    kl = rank_of[pi];
+                }
+            }
+            const bool isf = lane < kStageParticles && pi < ce && (ti >> kTypeShift) == 0u;
             const uint64_t fmask = __ballot(isf);
             if (fmask == 0) continue;
-            float4 xl = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-            uint32_t kl = 0;
-            if (isf) {
-                xl = pos[pi];
-                kl = rank_of[pi];
-            }
             uint32_t cntl = 0;
             for (uint32_t c0 = 0; c0 < total; c0 += kListChunks * 64) {
                 // candidates c0 + 64k + lane; a lane past the last candidate sits
@@ -474,8 +520,7 @@ __global__ __launch_bounds__(kBlock) void k_lists(const SceneDev* __restrict__ s
                     fm &= fm - 1;
                     const float xs = readlane_f(xl.x, b), ys = readlane_f(xl.y, b), zs = readlane_f(xl.z, b);
                     uint32_t cnt = readlane_u(cntl, b);  // the particle's running count (uniform)
-                    const uint32_t kb = readlane_u(kl, b);
-                    FSA_GLOBAL(uint32_t)* const row = nbr + kb;
+                    uint32_t* const srow = stage + b * (uint32_t)kMaxNeighbors;
 #pragma unroll
                     for (uint32_t k = 0; k < kListChunks; k++) {
                         if (k * 64 < left) {
@@ -483,17 +528,12 @@ __global__ __launch_bounds__(kBlock) void k_lists(const SceneDev* __restrict__ s
                             const bool pass = dot3(dx, dy, dz, dx, dy, dz) <= r2;
                             const uint64_t m = __ballot(pass);
                             if (pass) {
-                                // past the capacity the last row is overwritten; the count reports the overflow
+                                // past the capacity the last entry is overwritten; the count reports the overflow
                                 const uint32_t slot =
                                     min(__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
                                                                   __builtin_amdgcn_mbcnt_lo((uint32_t)m, cnt)),
                                         (uint32_t)kMaxNeighbors - 1u);
-                                if (Wide) {
-                                    nbr[slot * stride + kb] = ej[k];
-                                } else {
-                                    const uint32_t off = mul_u24(slot, stride) << 2;
-                                    *(FSA_GLOBAL(uint32_t)*)((FSA_GLOBAL(char)*)row + off) = ej[k];
-                                }
+                                srow[slot] = ej[k];
                             }
                             cnt += (uint32_t)__popcll(m);
                         }
@@ -501,11 +541,40 @@ __global__ __launch_bounds__(kBlock) void k_lists(const SceneDev* __restrict__ s
                     if (lane == b) cntl = cnt;
                 }
             }
+            const uint32_t kept = min(cntl, (uint32_t)kMaxNeighbors);
             if (isf) {
                 if (cntl > (uint32_t)kMaxNeighbors) overflow = true;
-                S.nbr_cnt[kl] = min(cntl, (uint32_t)kMaxNeighbors);
+                S.nbr_cnt[kl] = kept;
             }
+            // longest list of the group (uniform)
+            uint32_t rows = 0;
+            for (uint64_t fm = fmask; fm; fm &= fm - 1)
+                rows = max(rows, readlane_u(kept, (uint32_t)__ffsll((unsigned long long)fm) - 1u));
+            // the stage rows written by other lanes are complete before they are read
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            const uint32_t ccnt = (uint32_t)__shfl(kept, (int)col, 64);
+            const uint32_t crank = (uint32_t)__shfl(kl, (int)col, 64);
+            const bool cfluid = (fmask >> col) & 1u;
+            for (uint32_t e0 = 0; e0 < rows; e0 += row_step) {
+                const uint32_t e = e0 + lane / kStageParticles;
+                if (cfluid && e < ccnt) {
+                    const uint32_t v = stage[col * (uint32_t)kMaxNeighbors + e];
+                    if (Wide) {
+                        nbr[(uint64_t)e * stride + crank] = v;
+                    } else {
+                        nbr[mul_u24(e, stride) + crank] = v;
+                    }
+                }
+            }
+            // every lane's reads of the stage are done before the next group writes it
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         }
+        if (!more) break;
+        sl0 = nsl;
     }
     if (__any(overflow) && lane == 0) atomicOr(&S.state->status, kStatusCapacity);
 }
