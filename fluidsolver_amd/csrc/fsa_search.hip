@@ -221,19 +221,25 @@ __global__ __launch_bounds__(kBlock) void k_permute(const SceneDev* __restrict__
     if (threadIdx.x == 0) S.block_count[br.block] = (uint32_t)cnt;
 }
 
-// one block per scene: exclusive scan of the per-block fluid counts
-__global__ __launch_bounds__(kBlock) void k_fluid_scan(const SceneDev* __restrict__ scenes,
-                                                       const BlockRef* __restrict__ map) {
+// one wide block per scene: exclusive scan of the per-block fluid counts, each
+// thread over a contiguous run of counts
+constexpr uint32_t kScanThreads = 1024;
+
+__global__ __launch_bounds__(kScanThreads) void k_fluid_scan(const SceneDev* __restrict__ scenes,
+                                                             const BlockRef* __restrict__ map) {
     const SceneDev& S = scenes[map[blockIdx.x].scene];
-    __shared__ uint32_t lw[kBlock / 64];
-    uint32_t carry = 0;
-    for (uint32_t base = 0; base < S.nblocks; base += kBlock) {
-        const uint32_t idx = base + threadIdx.x;
-        const uint32_t v = idx < S.nblocks ? S.block_count[idx] : 0u;
-        uint32_t tot;
-        const uint32_t ex = block_exclusive_scan(v, lw, tot);
-        if (idx < S.nblocks) S.block_count[idx] = carry + ex;
-        carry += tot;
+    __shared__ uint32_t lw[kScanThreads / 64];
+    const uint32_t n = S.nblocks;
+    const uint32_t per = (n + kScanThreads - 1) / kScanThreads;
+    const uint32_t b0 = min(n, threadIdx.x * per), b1 = min(n, b0 + per);
+    uint32_t sum = 0;
+    for (uint32_t b = b0; b < b1; b++) sum += S.block_count[b];
+    uint32_t tot;
+    uint32_t run = block_exclusive_scan(sum, lw, tot);
+    for (uint32_t b = b0; b < b1; b++) {
+        const uint32_t v = S.block_count[b];
+        S.block_count[b] = run;
+        run += v;
     }
     // the permuted buffer becomes current (k_permute has finished reading [cur])
     if (threadIdx.x == 0) S.state->cur ^= 1u;
@@ -803,7 +809,7 @@ void launch_sort_range(const LaunchSet& L, uint32_t first_src, uint32_t shift0, 
 
 void launch_reorder(const LaunchSet& L, uint32_t sbuf, hipStream_t st) {
     k_permute<<<L.nb_all, kBlock, 0, st>>>(L.scenes, L.map_all, sbuf);
-    k_fluid_scan<<<L.nb_scene, kBlock, 0, st>>>(L.scenes, L.map_scene);
+    k_fluid_scan<<<L.nb_scene, kScanThreads, 0, st>>>(L.scenes, L.map_scene);
     k_compact<<<L.nb_all, kBlock, 0, st>>>(L.scenes, L.map_all);
     k_cell_table<<<L.nb_all, kBlock, 0, st>>>(L.scenes, L.map_all, sbuf);
 }
@@ -828,7 +834,7 @@ void launch3u_search(const LaunchSet& L, uint32_t passes, hipStream_t st) {
 
 void launch_permute_only(const LaunchSet& L, uint32_t sbuf, hipStream_t st) {
     k_permute<<<L.nb_all, kBlock, 0, st>>>(L.scenes, L.map_all, sbuf);
-    k_fluid_scan<<<L.nb_scene, kBlock, 0, st>>>(L.scenes, L.map_scene);
+    k_fluid_scan<<<L.nb_scene, kScanThreads, 0, st>>>(L.scenes, L.map_scene);
     k_compact<<<L.nb_all, kBlock, 0, st>>>(L.scenes, L.map_all);
 }
 

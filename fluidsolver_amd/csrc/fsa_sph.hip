@@ -109,12 +109,22 @@ __device__ __forceinline__ Partial reduce_partials(const Partial* parts, uint32_
     double err = 0.0;
     uint32_t cnt = 0;
     float a2 = 0.0f, v2 = 0.0f;
-    for (uint32_t b = threadIdx.x; b < nparts; b += kBlock) {
-        const Partial q = parts[b];
-        err += q.err;
-        cnt += q.count;
-        a2 = fmaxf(a2, q.max_a2);
-        v2 = fmaxf(v2, q.max_v2);
+    // each thread adds partials t, t + kBlock, t + 2 kBlock, ... in that order;
+    // four are loaded ahead of their adds
+    for (uint32_t b0 = threadIdx.x; b0 < nparts; b0 += 4 * kBlock) {
+        Partial q[4];
+#pragma unroll
+        for (uint32_t u = 0; u < 4; u++)
+            if (b0 + u * kBlock < nparts) q[u] = parts[b0 + u * kBlock];
+#pragma unroll
+        for (uint32_t u = 0; u < 4; u++) {
+            if (b0 + u * kBlock < nparts) {
+                err += q[u].err;
+                cnt += q[u].count;
+                a2 = fmaxf(a2, q[u].max_a2);
+                v2 = fmaxf(v2, q[u].max_v2);
+            }
+        }
     }
     return block_reduce(err, cnt, a2, v2, lds);
 }
@@ -544,26 +554,6 @@ __global__ __launch_bounds__(kBlock) void k_cfl_finalize(const SceneDev* __restr
 }
 
 // ---- CFL correction + Euler-Cromer integration ------------------------------------------
-// DynamicCflTimestepGenerator::calculate_maximum_velocity_and_acceleration
-// (DynamicCflTimestepGenerator.cpp:6-43) over the state the integration left:
-// the per-block maxima of |v|^2 and |a|^2 (BoxPartial lo[3] / hi[3]) reduced
-// into the scene state for the host's next time step.  sqrt is correctly
-// rounded and monotone, so sqrt of the maximum equals the maximum length.
-__device__ void reduce_next_maxima(const SceneDev& S) {
-    __shared__ float lmax[2 * kBlock / 64];
-    float v2 = 0.0f, a2 = 0.0f;
-    for (uint32_t b = threadIdx.x; b < S.nblocks; b += kBlock) {
-        const BoxPartial bp = S.box[b];
-        v2 = fmaxf(v2, __int_as_float(bp.lo[3]));
-        a2 = fmaxf(a2, __int_as_float(bp.hi[3]));
-    }
-    block_max2(v2, a2, lmax);
-    if (threadIdx.x == 0) {
-        S.state->next_v2 = v2;
-        S.state->next_a2 = a2;
-    }
-}
-
 __global__ __launch_bounds__(kBlock) void k_integrate(const SceneDev* __restrict__ scenes,
                                                       const BlockRef* __restrict__ map) {
     const BlockRef br = map[blockIdx.x];
@@ -656,51 +646,81 @@ __global__ __launch_bounds__(kBlock) void k_integrate(const SceneDev* __restrict
     }
 }
 
-// bounding box of the cells of the next search and the step's dt statistics
-__global__ __launch_bounds__(kBlock) void k_integrate_finalize(const SceneDev* __restrict__ scenes,
-                                                               const BlockRef* __restrict__ map) {
+// bounding box of the cells of the next search, the step's dt statistics, and
+// the maxima of |v|^2 and |a|^2 of the new state over the integrate blocks'
+// partials: DynamicCflTimestepGenerator::calculate_maximum_velocity_and_acceleration
+// (DynamicCflTimestepGenerator.cpp:6-43) for the next desired time step (sqrt
+// is correctly rounded and monotone, so sqrt of the maximum is the maximum length).  One wide block per
+// scene; minima and maxima do not depend on the order.
+constexpr uint32_t kFinalizeThreads = 1024;
+
+__global__ __launch_bounds__(kFinalizeThreads) void k_integrate_finalize(const SceneDev* __restrict__ scenes,
+                                                                         const BlockRef* __restrict__ map) {
     const SceneDev& S = scenes[map[blockIdx.x].scene];
     if (scene_failed(S)) return;
-    __shared__ int32_t blo[3][kBlock / 64], bhi[3][kBlock / 64];
-    int32_t l[3] = {INT32_MAX, INT32_MAX, INT32_MAX}, h[3] = {INT32_MIN, INT32_MIN, INT32_MIN};
-    for (uint32_t b = threadIdx.x; b < S.nblocks; b += kBlock) {
+    constexpr uint32_t kWaves = kFinalizeThreads / 64;
+    __shared__ int32_t red[8][kWaves];
+    int32_t l0 = INT32_MAX, l1 = INT32_MAX, l2 = INT32_MAX, h0 = INT32_MIN, h1 = INT32_MIN, h2 = INT32_MIN;
+    float v2 = 0.0f, a2 = 0.0f;
+#pragma unroll 4
+    for (uint32_t b = threadIdx.x; b < S.nblocks; b += kFinalizeThreads) {
         const BoxPartial bp = S.box[b];
-        for (int a = 0; a < 3; a++) {
-            l[a] = min(l[a], bp.lo[a]);
-            h[a] = max(h[a], bp.hi[a]);
-        }
+        l0 = min(l0, bp.lo[0]);
+        l1 = min(l1, bp.lo[1]);
+        l2 = min(l2, bp.lo[2]);
+        h0 = max(h0, bp.hi[0]);
+        h1 = max(h1, bp.hi[1]);
+        h2 = max(h2, bp.hi[2]);
+        v2 = fmaxf(v2, __int_as_float(bp.lo[3]));
+        a2 = fmaxf(a2, __int_as_float(bp.hi[3]));
     }
 #pragma unroll
-    for (int a = 0; a < 3; a++) {
-#pragma unroll
-        for (int o = 32; o > 0; o >>= 1) {
-            l[a] = min(l[a], __shfl_xor(l[a], o, 64));
-            h[a] = max(h[a], __shfl_xor(h[a], o, 64));
-        }
-        if (__lane_id() == 0) {
-            blo[a][threadIdx.x / 64] = l[a];
-            bhi[a][threadIdx.x / 64] = h[a];
-        }
+    for (int o = 32; o > 0; o >>= 1) {
+        l0 = min(l0, __shfl_xor(l0, o, 64));
+        l1 = min(l1, __shfl_xor(l1, o, 64));
+        l2 = min(l2, __shfl_xor(l2, o, 64));
+        h0 = max(h0, __shfl_xor(h0, o, 64));
+        h1 = max(h1, __shfl_xor(h1, o, 64));
+        h2 = max(h2, __shfl_xor(h2, o, 64));
+        v2 = fmaxf(v2, __shfl_xor(v2, o, 64));
+        a2 = fmaxf(a2, __shfl_xor(a2, o, 64));
+    }
+    const uint32_t w = threadIdx.x / 64;
+    if (__lane_id() == 0) {
+        red[0][w] = l0;
+        red[1][w] = l1;
+        red[2][w] = l2;
+        red[3][w] = h0;
+        red[4][w] = h1;
+        red[5][w] = h2;
+        red[6][w] = __float_as_int(v2);
+        red[7][w] = __float_as_int(a2);
     }
     __syncthreads();
     if (threadIdx.x == 0) {
         SceneState* st = S.state;
         for (int a = 0; a < 3; a++) {
-            int32_t lo2 = INT32_MAX, hi2 = INT32_MIN;
-            for (int w = 0; w < kBlock / 64; w++) {
-                lo2 = min(lo2, blo[a][w]);
-                hi2 = max(hi2, bhi[a][w]);
+            int32_t lo = INT32_MAX, hi = INT32_MIN;
+            for (uint32_t q = 0; q < kWaves; q++) {
+                lo = min(lo, red[a][q]);
+                hi = max(hi, red[3 + a][q]);
             }
-            st->bbox_min[a] = lo2;
-            st->bbox_max[a] = hi2;
+            st->bbox_min[a] = lo;
+            st->bbox_max[a] = hi;
         }
+        float nv2 = 0.0f, na2 = 0.0f;
+        for (uint32_t q = 0; q < kWaves; q++) {
+            nv2 = fmaxf(nv2, __int_as_float(red[6][q]));
+            na2 = fmaxf(na2, __int_as_float(red[7][q]));
+        }
+        st->next_v2 = nv2;
+        st->next_a2 = na2;
         const float max_v = (float)sqrt((double)st->max_v2);
         const float max_a = (float)sqrt((double)st->max_a2);
         st->dt_actual = fminf(cfl_timestep(S, max_a, max_v), S.dt);
         st->max_v = max_v;
         st->max_a = max_a;
     }
-    reduce_next_maxima(S);
 }
 
 // ---- launchers ------------------------------------------------------------------------
@@ -771,7 +791,7 @@ void launch_sesph_accel(const LaunchSet& L, hipStream_t st) {
 }
 void launch_integrate(const LaunchSet& L, hipStream_t st) {
     k_integrate<<<L.nb_all, kBlock, 0, st>>>(L.scenes, L.map_all);
-    k_integrate_finalize<<<L.nb_scene, kBlock, 0, st>>>(L.scenes, L.map_scene);
+    k_integrate_finalize<<<L.nb_scene, kFinalizeThreads, 0, st>>>(L.scenes, L.map_scene);
 }
 
 }  // namespace fsa
