@@ -266,6 +266,17 @@ __device__ __forceinline__ float div_pair(float a, float b, float y) {
     return __builtin_copysignf(__builtin_fmaf(r0, y, q0), a);
 }
 
+// div_pair without restoring the sign of a zero quotient: the same value, but
+// a zero quotient may come out as +0 where IEEE gives -0 (a = -0).  Used for
+// the pair gradient, whose every consumer adds it (or products with it) into
+// a sum started at +0; such a sum is never -0, so the sign of a zero term
+// cannot reach a result.
+__device__ __forceinline__ float div_pair_zs(float a, float b, float y) {
+    const float q0 = a * y;
+    const float r0 = __builtin_fmaf(-b, q0, a);
+    return __builtin_fmaf(r0, y, q0);
+}
+
 // Compile-time specialisation of the two parameters that make operations
 // exact no-ops: Spec bit 0 = every scene of the launch has h == 1, bit 1 =
 // every scene has rest_density^2 == 1.  Without the bit the scene flag decides.
@@ -317,9 +328,9 @@ __device__ __forceinline__ void kernel_grad(const SceneDev& S, float dx, float d
     if (Spec & kSpecFastMath) {
         // h == 1: d / |d| with one shared reciprocal (exact on this range)
         const float y = recip_pair(len);
-        px = div_pair(dx, len, y);
-        py = div_pair(dy, len, y);
-        pz = div_pair(dz, len, y);
+        px = div_pair_zs(dx, len, y);
+        py = div_pair_zs(dy, len, y);
+        pz = div_pair_zs(dz, len, y);
     } else if (hone) {
         px = dx / len;
         py = dy / len;

@@ -4,7 +4,8 @@
 //                  and results outside [0, 2^-40] below 2^-96
 //   fsa_mc_recip:  recip_pair against the IEEE 1 / b, every mantissa at the given exponents
 //   fsa_mc_div:    every divisor mantissa in [1, 2) at the given exponents, against
-//                  numerators drawn from a hash (zeros, tiny, extreme and random)
+//                  numerators drawn from a hash (zeros, tiny, extreme and random);
+//                  div_pair bit for bit, div_pair_zs by value (zeros of either sign)
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -72,7 +73,10 @@ __global__ void k_mc_div(int exp_lo, int exp_hi, uint32_t per_b, uint64_t seed, 
             const float a = numerator(b, k, seed ^ g);
             const float want = a / b;
             const float got = div_pair(a, b, y);
-            if (__float_as_uint(got) != __float_as_uint(want)) m++;
+            // div_pair_zs: the same value, a zero of either sign
+            const float got_zs = div_pair_zs(a, b, y);
+            const bool zs_ok = want == 0.0f ? got_zs == 0.0f : __float_as_uint(got_zs) == __float_as_uint(want);
+            if (__float_as_uint(got) != __float_as_uint(want) || !zs_ok) m++;
             t++;
         }
     }

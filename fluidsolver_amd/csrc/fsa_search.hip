@@ -97,16 +97,19 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_hist(const SceneDev* __r
     S.hist[threadIdx.x * S.ntiles + tile] = h[threadIdx.x];
 }
 
+// wide single-block scans (one block per scene or per (scene, digit))
+constexpr uint32_t kScanThreads = 1024;
+
 // one block per (scene, digit): exclusive scan of the digit's tile counts
-__global__ __launch_bounds__(kBlock) void k_radix_scan(const SceneDev* __restrict__ scenes,
-                                                       const BlockRef* __restrict__ map) {
+__global__ __launch_bounds__(kScanThreads) void k_radix_scan(const SceneDev* __restrict__ scenes,
+                                                             const BlockRef* __restrict__ map) {
     const BlockRef br = map[blockIdx.x];
     const SceneDev& S = scenes[br.scene];
     const uint32_t d = br.block;
     uint32_t* row = S.hist + d * S.ntiles;
-    __shared__ uint32_t lw[kBlock / 64];
+    __shared__ uint32_t lw[kScanThreads / 64];
     uint32_t carry = 0;
-    for (uint32_t base = 0; base < S.ntiles; base += kBlock) {
+    for (uint32_t base = 0; base < S.ntiles; base += kScanThreads) {
         const uint32_t idx = base + threadIdx.x;
         const uint32_t v = idx < S.ntiles ? row[idx] : 0u;
         uint32_t tot;
@@ -223,7 +226,6 @@ __global__ __launch_bounds__(kBlock) void k_permute(const SceneDev* __restrict__
 
 // one wide block per scene: exclusive scan of the per-block fluid counts, each
 // thread over a contiguous run of counts
-constexpr uint32_t kScanThreads = 1024;
 
 __global__ __launch_bounds__(kScanThreads) void k_fluid_scan(const SceneDev* __restrict__ scenes,
                                                              const BlockRef* __restrict__ map) {
@@ -792,7 +794,7 @@ void launch_sort(const LaunchSet& L, uint32_t passes, hipStream_t st) {
         const uint32_t src = p & 1u;
         const uint32_t shift = p * kRadixBits;
         k_radix_hist<<<L.nb_tiles, kSortThreads, 0, st>>>(L.scenes, L.map_tiles, shift, src);
-        k_radix_scan<<<L.nb_scan, kBlock, 0, st>>>(L.scenes, L.map_scan);
+        k_radix_scan<<<L.nb_scan, kScanThreads, 0, st>>>(L.scenes, L.map_scan);
         k_radix_scatter<<<L.nb_tiles, kSortThreads, 0, st>>>(L.scenes, L.map_tiles, shift, src);
     }
 }
@@ -802,7 +804,7 @@ void launch_sort_range(const LaunchSet& L, uint32_t first_src, uint32_t shift0, 
         const uint32_t src = first_src ^ (p & 1u);
         const uint32_t shift = shift0 + p * kRadixBits;
         k_radix_hist<<<L.nb_tiles, kSortThreads, 0, st>>>(L.scenes, L.map_tiles, shift, src);
-        k_radix_scan<<<L.nb_scan, kBlock, 0, st>>>(L.scenes, L.map_scan);
+        k_radix_scan<<<L.nb_scan, kScanThreads, 0, st>>>(L.scenes, L.map_scan);
         k_radix_scatter<<<L.nb_tiles, kSortThreads, 0, st>>>(L.scenes, L.map_tiles, shift, src);
     }
 }

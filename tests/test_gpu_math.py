@@ -8,7 +8,9 @@ the IEEE 1 / b.  This is the precondition under which div_pair's single FMA
 correction gives the correctly rounded quotient (Markstein's theorem).
 div_pair: every divisor mantissa in [1, 2) at every exponent of 2^-23 .. 2^1
 (divisors in [2^-23, 4)), against 16 numerators each: signed zeros, +-b, values
-near 2^-59 and random magnitudes in [2^-59, b]; bitwise equal to IEEE a / b."""
+near 2^-59 and random magnitudes in [2^-59, b]; bitwise equal to IEEE a / b.
+div_pair_zs (the gradient's division): the same quotients by value, a zero
+quotient of either sign (the sums it feeds start at +0 and are never -0)."""
 import ctypes as C
 import os
 
