@@ -50,6 +50,12 @@ def algorithmic_bytes(nf, nba, L):
 TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "iisph_dam3d_1M_pmc_traffic.md")
 
 
+# main-workload searches: the reference search whose neighbor order is reproduced
+SEARCHES = {"compressed": (abi.SEARCH_COMPRESSED_3D, "compressed-order"),
+            "hashed": (abi.SEARCH_HASHED_3D, "hashed-3d-order"),
+            "quadratic": (abi.SEARCH_QUADRATIC_3D, "quadratic-3d-order")}
+
+
 def measured_traffic(phase, side):
     """HBM bytes per launch of a phase at the 1M configuration, from the
     committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary (scaled by the counters'
@@ -234,6 +240,8 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--side", type=int, default=100, help="fluid block side (100 -> 1,000,000 fluid)")
+    ap.add_argument("--search", choices=sorted(SEARCHES), default="compressed",
+                    help="neighbor order of the main workload: the reference search it reproduces")
     ap.add_argument("--cpu-steps", type=int, default=10)
     ap.add_argument("--cpu-side", type=int, default=100)
     ap.add_argument("--no-cpu", action="store_true")
@@ -265,7 +273,8 @@ def main():
     counts = coll.counts()
     nf, nb = counts["fluid"], counts["boundary"]
     desc = abi.default_desc(3, abi.SOLVER_IISPH)
-    log(f"scene: {nf} fluid, {nb} boundary")
+    desc.search = SEARCHES[args.search][0]
+    log(f"scene: {nf} fluid, {nb} boundary, {args.search} search")
 
     ctx = Context(0)
     ctx.set_graph(bool(args.graph))
@@ -319,7 +328,7 @@ def main():
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBS, 4),
-        "traffic": measured_traffic(dom, side),
+        "traffic": measured_traffic(dom, side) if args.search == "compressed" else None,
         "traffic_source": "profiles/iisph_dam3d_1M_pmc_traffic.md (rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0)",
         "algorithmic_bytes_per_launch": int(algo[dom]),
         "avg_launch_ms": phases[dom]["avg_ms"],
@@ -367,7 +376,7 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": {
-            "workload": f"3D IISPH breaking dam, {nf} fluid + {nb} boundary particles, compressed-order search",
+            "workload": f"3D IISPH breaking dam, {nf} fluid + {nb} boundary particles, {SEARCHES[args.search][1]} search",
             "fluid_particles": nf,
             "boundary_particles": nb,
             "mean_jacobi_iterations": float(np.mean(iters)),

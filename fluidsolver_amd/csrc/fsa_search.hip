@@ -593,8 +593,8 @@ __global__ __launch_bounds__(kBlock) void k_lists(const SceneDev* __restrict__ s
 // not sort the collection each step.  Particles are bucketed by a stable sort
 // on their dense cell index, so each cell lists its members ascending (the
 // hashed grid's std::set); a list walks the 27 cells dx-outer, dz-inner
-// (hashed), or is kept ascending overall by insertion (quadratic).  Neighbor
-// test dot(x_i - x_j, x_i - x_j) <= r^2.
+// (hashed), or is ascending overall (quadratic).  Neighbor test
+// dot(x_i - x_j, x_i - x_j) <= r^2.
 __global__ __launch_bounds__(kBlock) void k3u_keys(const SceneDev* __restrict__ scenes,
                                                    const BlockRef* __restrict__ map) {
     const BlockRef br = map[blockIdx.x];
@@ -630,16 +630,13 @@ __global__ __launch_bounds__(kBlock) void k3u_cell_table(const SceneDev* __restr
     if (last) S.cells[key].y = k + 1;
 }
 
-__global__ __launch_bounds__(kBlock) void k3u_lists(const SceneDev* __restrict__ scenes,
-                                                    const BlockRef* __restrict__ map, uint32_t buf) {
-    const BlockRef br = map[blockIdx.x];
-    const SceneDev& S = scenes[br.scene];
-    const uint32_t k = br.block * kBlock + threadIdx.x;
-    if (k >= S.nf) return;
+// One particle's list by a serial walk of the 27 cells (dx-outer, dz-inner),
+// kept ascending by insertion for the quadratic order.  The path for cells
+// whose quadratic candidate set exceeds the held chunks.
+__device__ void k3u_list_serial(const SceneDev& S, uint32_t k, uint32_t i, uint32_t buf) {
     const uint32_t c = S.state->cur;
     const auto pos = gload_ptr(S.pos[c]);
     const auto idt = gload_ptr(S.idt[c]);
-    const uint32_t i = S.fluid_idx[k];
     const float4 xi = pos[i];
     const int4 ci = S.cell_last[i];
     const uint32_t* __restrict__ vals = S.vals[buf];
@@ -679,6 +676,242 @@ __global__ __launch_bounds__(kBlock) void k3u_lists(const SceneDev* __restrict__
             }
     if (cnt > (uint32_t)kMaxNeighbors) atomicOr(&S.state->status, kStatusCapacity);
     S.nbr_cnt[k] = min(cnt, (uint32_t)kMaxNeighbors);
+}
+
+// ascending bitonic sort of R x 64 keys, element r * 64 + lane in v[r]
+template <uint32_t R>
+__device__ __forceinline__ void sort_regs_u32(uint32_t (&v)[R]) {
+    const uint32_t lane = __lane_id();
+#pragma unroll
+    for (uint32_t size = 2; size <= 64 * R; size <<= 1) {
+#pragma unroll
+        for (uint32_t st = size >> 1; st > 0; st >>= 1) {
+            uint32_t o[R];
+#pragma unroll
+            for (uint32_t r = 0; r < R; r++) {
+                const uint32_t p = r * 64 + lane;
+                const bool asc = (p & size) == 0;
+                uint32_t b;
+                bool lower;
+                if (st >= 64) {
+                    b = v[r ^ (st >> 6)];
+                    lower = (r & (st >> 6)) == 0;
+                } else {
+                    b = (uint32_t)__shfl_xor((int)v[r], (int)st, 64);
+                    lower = (lane & st) == 0;
+                }
+                o[r] = (lower == asc) ? min(v[r], b) : max(v[r], b);
+            }
+#pragma unroll
+            for (uint32_t r = 0; r < R; r++) v[r] = o[r];
+        }
+    }
+}
+
+// candidates the quadratic order sorts in registers: two chunk sets
+constexpr uint32_t kQuadSorted = 2 * kListChunks * 64;
+
+// Wave-cooperative lists for the hashed and quadratic orders, in the bucket
+// array (particles stably sorted by cell): one wave takes the cells whose runs
+// start among 64 consecutive bucket positions.  A cell's candidates are the
+// runs of its 27 neighbor cells, concatenated dx-outer, dz-inner (the hashed
+// order: each run lists its members ascending); for the quadratic order the
+// held candidates are sorted by particle index (ascending overall).  The
+// cell's fluid particles, kStageParticles at a time, are tested against the
+// held chunks as in k_lists, staged in LDS, and written row by row.
+__global__ __launch_bounds__(kBlock) void k3u_lists(const SceneDev* __restrict__ scenes,
+                                                    const BlockRef* __restrict__ map, uint32_t buf) {
+    __shared__ uint32_t stage_all[kBlock / 64][kStageParticles * kMaxNeighbors];
+    const BlockRef br = map[blockIdx.x];
+    const SceneDev& S = scenes[br.scene];
+    const uint32_t wave = threadIdx.x / 64;
+    const uint32_t lane = __lane_id();
+    const uint32_t base = br.block * kBlock + wave * 64;
+    if (base >= S.n) return;
+    uint32_t* const stage = stage_all[wave];
+    const uint32_t c = S.state->cur;
+    const auto pos = gload_ptr(S.pos[c]);
+    const auto idt = gload_ptr(S.idt[c]);
+    const auto keys = gload_ptr(S.keys[buf]);
+    const auto vals = gload_ptr(S.vals[buf]);
+    const auto rank_of = gload_ptr(S.rank_of);
+    FSA_GLOBAL(uint32_t)* __restrict__ nbr = (FSA_GLOBAL(uint32_t)*)S.nbr;
+    const float r2 = S.r2;
+    const uint32_t stride = S.list_stride;
+    const bool quad = (S.flags & kFlagQuadratic) != 0;
+    const uint32_t mi = base + lane;
+    bool start = false;
+    uint32_t mykey = 0;
+    if (mi < S.n) {
+        mykey = keys[mi];
+        start = mi == 0 || keys[mi - 1] != mykey;
+    }
+    uint64_t starts = __ballot(start);
+    bool overflow = false;
+    const uint32_t col = lane % kStageParticles;
+    const uint32_t row_step = 64u / kStageParticles;
+    while (starts) {
+        const uint32_t sl0 = (uint32_t)__ffsll((unsigned long long)starts) - 1u;
+        starts &= starts - 1;
+        const uint32_t cs = base + sl0;
+        const uint32_t lin0 = readlane_u(mykey, sl0);
+        if (lin0 >= S.cdim[3]) continue;  // outside the grid: reported by k3u_keys
+        const int32_t cx = (int32_t)(lin0 % S.cdim[0]) + S.cmin[0];
+        const int32_t cy = (int32_t)((lin0 / S.cdim[0]) % S.cdim[1]) + S.cmin[1];
+        const int32_t cz = (int32_t)(lin0 / (S.cdim[0] * S.cdim[1])) + S.cmin[2];
+        uint4 e = make_uint4(0u, 0u, 0u, 0u);
+        if (lane < 27) {
+            uint32_t lin;
+            if (cell_linear(S, cx + (int)(lane / 9) - 1, cy + (int)((lane / 3) % 3) - 1, cz + (int)(lane % 3) - 1,
+                            lin))
+                e = S.cells[lin];
+        }
+        const uint32_t ce = readlane_u(e.y, 13);
+        // runs in lane order (dx-outer, dz-inner); absent cells are empty runs
+        uint32_t rs = 0, rl = 0;
+        if (lane < 27 && e.z == S.stamp) {
+            rs = e.x;
+            rl = e.y - e.x;
+        }
+        uint32_t total;
+        const uint32_t roff = exclusive_scan32(rl, total);
+        const bool serial = quad && total > kQuadSorted;
+        // quadratic: every candidate index, sorted ascending (two chunk sets at most)
+        uint32_t sj[2 * kListChunks];
+        if (quad && !serial) {
+#pragma unroll
+            for (uint32_t k = 0; k < 2 * kListChunks; k++) {
+                sj[k] = 0xffffffffu;
+                const uint32_t g = k * 64 + lane;
+                uint32_t q = 0;
+#pragma unroll
+                for (uint32_t step = 16; step > 0; step >>= 1) {
+                    const uint32_t probe = q + step;
+                    const uint32_t off = __shfl(roff, (int)min(probe, 31u), 64);
+                    if (probe < 27u && off <= g) q = probe;
+                }
+                const uint32_t qs = __shfl(rs, (int)q, 64);
+                const uint32_t qo = __shfl(roff, (int)q, 64);
+                if (g < total) sj[k] = vals[qs + (g - qo)];
+            }
+            if (total <= kListChunks * 64) {
+                uint32_t lo[kListChunks];
+#pragma unroll
+                for (uint32_t k = 0; k < kListChunks; k++) lo[k] = sj[k];
+                sort_regs_u32<kListChunks>(lo);
+#pragma unroll
+                for (uint32_t k = 0; k < kListChunks; k++) sj[k] = lo[k];
+            } else {
+                sort_regs_u32<2 * kListChunks>(sj);
+            }
+        }
+        for (uint32_t pbase = cs; pbase < ce; pbase += kStageParticles) {
+            const uint32_t pm = pbase + lane;
+            uint32_t pi = 0, ti = 0xffffffffu;
+            if (lane < kStageParticles && pm < ce) {
+                pi = vals[pm];
+                ti = idt[pi];
+            }
+            const bool isf = lane < kStageParticles && pm < ce && (ti >> kTypeShift) == 0u;
+            const uint64_t fmask = __ballot(isf);
+            if (fmask == 0) continue;
+            float4 xl = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            uint32_t kl = 0;
+            if (isf) {
+                xl = pos[pi];
+                kl = rank_of[pi];
+            }
+            if (serial) {
+                // more quadratic candidates than the held chunks: each lane walks its particle
+                if (isf) k3u_list_serial(S, kl, pi, buf);
+                continue;
+            }
+            uint32_t cntl = 0;
+            for (uint32_t c0 = 0; c0 < total; c0 += kListChunks * 64) {
+                // candidate c0 + 64k + lane: bucket position by binary search over the run offsets
+                uint32_t jj[kListChunks];
+#pragma unroll
+                for (uint32_t k = 0; k < kListChunks; k++) {
+                    jj[k] = 0xffffffffu;
+                    if (quad) {
+                        jj[k] = c0 == 0 ? sj[k] : sj[kListChunks + k];
+                    } else if (c0 + k * 64 < total) {
+                        const uint32_t g = c0 + k * 64 + lane;
+                        uint32_t q = 0;
+#pragma unroll
+                        for (uint32_t step = 16; step > 0; step >>= 1) {
+                            const uint32_t probe = q + step;
+                            const uint32_t off = __shfl(roff, (int)min(probe, 31u), 64);
+                            if (probe < 27u && off <= g) q = probe;
+                        }
+                        const uint32_t qs = __shfl(rs, (int)q, 64);
+                        const uint32_t qo = __shfl(roff, (int)q, 64);
+                        if (g < total) jj[k] = vals[qs + (g - qo)];
+                    }
+                }
+                float4 pj[kListChunks];
+                uint32_t ej[kListChunks];
+#pragma unroll
+                for (uint32_t k = 0; k < kListChunks; k++) {
+                    pj[k] = make_float4(__builtin_inff(), __builtin_inff(), __builtin_inff(), 0.0f);
+                    ej[k] = 0;
+                    if (jj[k] != 0xffffffffu) {
+                        const uint32_t j = jj[k];
+                        pj[k] = pos[j];
+                        ej[k] = j | ((idt[j] >> kTypeShift) == 1u ? kBoundaryBit : 0u);
+                    }
+                }
+                const uint32_t left = total - c0;
+                uint64_t fm = fmask;
+                while (fm) {
+                    const uint32_t b = (uint32_t)__ffsll((unsigned long long)fm) - 1u;
+                    fm &= fm - 1;
+                    const float xs = readlane_f(xl.x, b), ys = readlane_f(xl.y, b), zs = readlane_f(xl.z, b);
+                    uint32_t cnt = readlane_u(cntl, b);
+                    uint32_t* const srow = stage + b * (uint32_t)kMaxNeighbors;
+#pragma unroll
+                    for (uint32_t k = 0; k < kListChunks; k++) {
+                        if (k * 64 < left) {
+                            const float dx = xs - pj[k].x, dy = ys - pj[k].y, dz = zs - pj[k].z;
+                            const bool pass = dot3(dx, dy, dz, dx, dy, dz) <= r2;
+                            const uint64_t m = __ballot(pass);
+                            if (pass) {
+                                const uint32_t slot =
+                                    min(__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
+                                                                  __builtin_amdgcn_mbcnt_lo((uint32_t)m, cnt)),
+                                        (uint32_t)kMaxNeighbors - 1u);
+                                srow[slot] = ej[k];
+                            }
+                            cnt += (uint32_t)__popcll(m);
+                        }
+                    }
+                    if (lane == b) cntl = cnt;
+                }
+            }
+            const uint32_t kept = min(cntl, (uint32_t)kMaxNeighbors);
+            if (isf) {
+                if (cntl > (uint32_t)kMaxNeighbors) overflow = true;
+                S.nbr_cnt[kl] = kept;
+            }
+            uint32_t rows = 0;
+            for (uint64_t fm = fmask; fm; fm &= fm - 1)
+                rows = max(rows, readlane_u(kept, (uint32_t)__ffsll((unsigned long long)fm) - 1u));
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            const uint32_t ccnt = (uint32_t)__shfl(kept, (int)col, 64);
+            const uint32_t crank = (uint32_t)__shfl(kl, (int)col, 64);
+            const bool cfluid = (fmask >> col) & 1u;
+            for (uint32_t e0 = 0; e0 < rows; e0 += row_step) {
+                const uint32_t ee = e0 + lane / kStageParticles;
+                if (cfluid && ee < ccnt) nbr[(uint64_t)ee * stride + crank] = stage[col * (uint32_t)kMaxNeighbors + ee];
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        }
+    }
+    if (__any(overflow) && lane == 0) atomicOr(&S.state->status, kStatusCapacity);
 }
 
 // ---- 7. NeighborStorage codec (CompressedNeighbors.cpp:260-330) -----------------
@@ -831,7 +1064,7 @@ void launch3u_search(const LaunchSet& L, uint32_t passes, hipStream_t st) {
     launch_sort_range(L, 0u, 0u, passes, st);
     const uint32_t buf = passes & 1u;
     k3u_cell_table<<<L.nb_all, kBlock, 0, st>>>(L.scenes, L.map_all, buf);
-    if (L.nb_fluid) k3u_lists<<<L.nb_fluid, kBlock, 0, st>>>(L.scenes, L.map_fluid, buf);
+    if (L.nb_fluid) k3u_lists<<<L.nb_all, kBlock, 0, st>>>(L.scenes, L.map_all, buf);
 }
 
 void launch_permute_only(const LaunchSet& L, uint32_t sbuf, hipStream_t st) {

@@ -88,3 +88,37 @@ def test_reupload_keeps_collection_order(gpu_context, oracle_lib, search):
         sc.upload(g)
         orc.upload(g)
     sc.close()
+
+
+@pytest.mark.parametrize("search", ["hashed", "quadratic"])
+@pytest.mark.parametrize("spacing", [0.75, 1.0])
+def test_3d_search_mode_lists_dense_and_sparse(gpu_context, oracle_lib, search, spacing):
+    """Neighbor lists of the hashed and quadratic orders, entry for entry.  At
+    spacing 0.75 a cell holds up to 27 particles, so a quadratic cell's 27
+    neighbor cells hold more candidates than the wave keeps at once and the
+    cell's particles take the serial path; at spacing 1 they do not."""
+    O = oracle_lib
+    from fluidsolver_amd.collection import ParticleCollection
+
+    rng = np.random.default_rng(63)
+    n = 14
+    f = (scenes.lattice((n, n, n)) * np.float32(spacing) + rng.uniform(-0.01, 0.01, (n**3, 3))).astype(np.float32)
+    b = (scenes.box_shell((n, n, n), 2) * np.float32(spacing)).astype(np.float32)
+    coll = _shuffled(ParticleCollection.from_arrays(
+        3, np.concatenate([f, b]) + np.float32(10.0),
+        np.concatenate([np.zeros(len(f), np.uint8), np.ones(len(b), np.uint8)])), 10)
+    d = abi.default_desc(3, abi.SOLVER_IISPH)
+    d.search = abi.SEARCH_HASHED_3D if search == "hashed" else abi.SEARCH_QUADRATIC_3D
+    print("case dense lists", search, spacing, coll.size(), flush=True)
+    sc = Scene(gpu_context, d, coll.copy())
+    sc.find_neighbors()
+    goff, gcols = sc.export_neighbors()
+    orc = O.OracleScene(d, coll.copy(), search=O.SEARCH_HASHED3D if search == "hashed" else O.SEARCH_QUADRATIC3D)
+    orc.find_neighbors()
+    ooff, ocols = orc.export_neighbors()
+    fl = np.nonzero(coll.type == abi.TYPE_NORMAL)[0]
+    for r in fl:
+        np.testing.assert_array_equal(gcols[goff[r]:goff[r + 1]], ocols[ooff[r]:ooff[r + 1]], err_msg=str(r))
+    longest = max(int(goff[r + 1] - goff[r]) for r in fl)
+    assert longest > (70 if spacing < 1.0 else 30), longest
+    sc.close()
