@@ -6,7 +6,7 @@
 //   2. k_radix_*      stable LSD radix sort of (key, index)        (:191-201)
 //   3. k_permute      gather every component into sorted order     (ParticleCollection::swap)
 //   4. k_fluid_scan / k_compact   fluid rank -> combined index
-//   5. k_cell_table   cell -> [start, end) of its run              (:203-225)
+//   5. (in k_permute) cell -> [start, end) of its run              (:203-225)
 //   6. k_lists        27-cell scan in ascending key order, distance
 //                     test dot(x_j - x_i) <= r^2, ascending lists  (:536-595)
 // All kernels take a scene table and a block map, so one launch serves any
@@ -200,6 +200,10 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(const SceneDev* 
 }
 
 // ---- 3. permute ------------------------------------------------------------
+// CellTable: the keys are this search's sorted cell keys, and the cell table
+// (CompressedNeighbors.cpp:203-225: the run of each cell in sorted order) is
+// written from the permuted positions on the way.
+template <bool CellTable>
 __global__ __launch_bounds__(kBlock) void k_permute(const SceneDev* __restrict__ scenes,
                                                     const BlockRef* __restrict__ map, uint32_t sbuf) {
     const BlockRef br = map[blockIdx.x];
@@ -210,7 +214,25 @@ __global__ __launch_bounds__(kBlock) void k_permute(const SceneDev* __restrict__
     bool fluid = false;
     if (i < S.n) {
         const uint32_t s = S.vals[sbuf][i];
-        S.pos[o][i] = S.pos[c][s];
+        const float4 p = S.pos[c][s];
+        S.pos[o][i] = p;
+        const uint32_t* __restrict__ keys = S.keys[sbuf];
+        const uint32_t key = CellTable ? keys[i] : 0u;
+        const bool first = CellTable && (i == 0 || keys[i - 1] != key);
+        const bool last = CellTable && (i + 1 == S.n || keys[i + 1] != key);
+        if (first || last) {
+            uint32_t lin;
+            if (!cell_linear(S, cell_of(p.x, S.search_radius), cell_of(p.y, S.search_radius),
+                             cell_of(p.z, S.search_radius), lin)) {
+                atomicOr(&S.state->status, kStatusGrid);
+            } else {
+                if (first) {
+                    S.cells[lin].x = i;
+                    S.cells[lin].z = S.stamp;
+                }
+                if (last) S.cells[lin].y = i + 1;
+            }
+        }
         S.vel[o][i] = S.vel[c][s];
         S.acc[o][i] = S.acc[c][s];
         if (S.flags & kFlagExternal) S.ext[o][i] = S.ext[c][s];
@@ -262,34 +284,6 @@ __global__ __launch_bounds__(kBlock) void k_compact(const SceneDev* __restrict__
         S.fluid_idx[k] = i;
         S.rank_of[i] = k;
     }
-}
-
-// ---- 5. cell table -----------------------------------------------------------
-
-__global__ __launch_bounds__(kBlock) void k_cell_table(const SceneDev* __restrict__ scenes,
-                                                       const BlockRef* __restrict__ map, uint32_t sbuf) {
-    const BlockRef br = map[blockIdx.x];
-    const SceneDev& S = scenes[br.scene];
-    const uint32_t i = br.block * kBlock + threadIdx.x;
-    if (i >= S.n) return;
-    const uint32_t c = S.state->cur;
-    const uint32_t* __restrict__ keys = S.keys[sbuf];
-    const uint32_t key = keys[i];
-    const bool first = i == 0 || keys[i - 1] != key;
-    const bool last = i + 1 == S.n || keys[i + 1] != key;
-    if (!first && !last) return;
-    const float4 p = S.pos[c][i];
-    uint32_t lin;
-    if (!cell_linear(S, cell_of(p.x, S.search_radius), cell_of(p.y, S.search_radius),
-                     cell_of(p.z, S.search_radius), lin)) {
-        atomicOr(&S.state->status, kStatusGrid);
-        return;
-    }
-    if (first) {
-        S.cells[lin].x = i;
-        S.cells[lin].z = S.stamp;
-    }
-    if (last) S.cells[lin].y = i + 1;
 }
 
 // ---- 6. neighbor lists ---------------------------------------------------------
@@ -1043,10 +1037,9 @@ void launch_sort_range(const LaunchSet& L, uint32_t first_src, uint32_t shift0, 
 }
 
 void launch_reorder(const LaunchSet& L, uint32_t sbuf, hipStream_t st) {
-    k_permute<<<L.nb_all, kBlock, 0, st>>>(L.scenes, L.map_all, sbuf);
+    k_permute<true><<<L.nb_all, kBlock, 0, st>>>(L.scenes, L.map_all, sbuf);
     k_fluid_scan<<<L.nb_scene, kScanThreads, 0, st>>>(L.scenes, L.map_scene);
     k_compact<<<L.nb_all, kBlock, 0, st>>>(L.scenes, L.map_all);
-    k_cell_table<<<L.nb_all, kBlock, 0, st>>>(L.scenes, L.map_all, sbuf);
 }
 
 void launch_lists(const LaunchSet& L, uint32_t sbuf, hipStream_t st) {
@@ -1068,7 +1061,7 @@ void launch3u_search(const LaunchSet& L, uint32_t passes, hipStream_t st) {
 }
 
 void launch_permute_only(const LaunchSet& L, uint32_t sbuf, hipStream_t st) {
-    k_permute<<<L.nb_all, kBlock, 0, st>>>(L.scenes, L.map_all, sbuf);
+    k_permute<false><<<L.nb_all, kBlock, 0, st>>>(L.scenes, L.map_all, sbuf);
     k_fluid_scan<<<L.nb_scene, kScanThreads, 0, st>>>(L.scenes, L.map_scene);
     k_compact<<<L.nb_all, kBlock, 0, st>>>(L.scenes, L.map_all);
 }
