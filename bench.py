@@ -6,7 +6,7 @@ One step = neighbor search + the full IISPH step (every Jacobi iteration, CFL
 correction, integration) for all fluid particles; particle-updates/s =
 fluid particles x steps / seconds.  Inputs are resident in HBM before timing.
 
-Prints one JSON line (the driver contract) with `roofline` for the dominant
+Prints one JSON line with `roofline` for the dominant
 kernel and `cpu_baseline` from the CPU oracle timed on this host.
 """
 import argparse

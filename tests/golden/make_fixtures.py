@@ -1,7 +1,7 @@
 """Writes tests/golden/*.npz from the reference's own scenario fixtures.
 
-Run in the development container only (it reads the reference checkout at
-/root/reference, which the GPU box does not have):
+Run where the reference checkout is present (it reads /root/reference; the
+machines that run the GPU tests only read the committed fixtures):
 
     python tests/golden/make_fixtures.py
 

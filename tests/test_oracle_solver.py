@@ -1,6 +1,6 @@
 """Pins the oracle's IISPH-3D and SESPH-3D steps.
 
-The reference has no solver tests (SURVEY.md §4), so the oracle is pinned by:
+The reference has no solver tests, so the oracle is pinned by:
 * an independent FP64 numpy implementation of one step, written from the
   IISPH equations (Ihmsen et al. 2014, with rest density in the d_ij terms and
   mirrored boundary pressure as IISPHFluidSolver3D.hpp uses) and the SESPH
