@@ -44,6 +44,9 @@ def algorithmic_bytes(nf, nba, L):
         "pressure_a": nf * (32 + L) + 16 * nba,
         "pressure_b": nf * (56 + L) + 16 * nba,
         "integrate": nf * 72,
+        # the first Jacobi iteration (every pressure zero: no neighbor walk):
+        # rank map, diagonal, source, pressure, v* read; acceleration, pressure, error written
+        "pressure_first": nf * (4 + 4 + 4 + 4 + 16 + 16 + 4 + 4),
     }
 
 

@@ -583,6 +583,12 @@ void simulate_set(fsa_context ctx, const LaunchSet& L, std::vector<fsa_scene_s*>
         uint32_t chunk = std::min(max_it, std::max<uint32_t>(guess, 1));
         while (enq < max_it) {
             for (uint32_t k = 0; k < chunk && enq < max_it; k++, enq++) {
+                if (!d2 && enq == 0) {
+                    // every pressure is zero: the first iteration needs no neighbor walk
+                    PhaseTimer t(ctx, FSA_TIMER_PRESSURE_FIRST);
+                    launch_iisph_first_iteration(L, st);
+                    continue;
+                }
                 {
                     PhaseTimer t(ctx, FSA_TIMER_PRESSURE_A);
                     if (d2)
@@ -1118,6 +1124,12 @@ void step_head(fsa_context ctx, const LaunchSet& L, bool d2, bool unsorted, uint
     else
         launch_iisph_begin_loop(L, st);
     for (uint32_t k = 0; k < chunk; k++) {
+        if (!d2 && k == 0) {
+            // every pressure is zero: the first iteration needs no neighbor walk
+            PhaseTimer t(ctx, FSA_TIMER_PRESSURE_FIRST);
+            launch_iisph_first_iteration(L, st);
+            continue;
+        }
         {
             PhaseTimer t(ctx, FSA_TIMER_PRESSURE_A);
             if (d2)
