@@ -28,7 +28,7 @@ constexpr uint32_t kEmptyCell = 0xffffffffu;
 constexpr int kSortThreads = 256;
 constexpr int kSortItems = 8;
 constexpr int kSortTile = kSortThreads * kSortItems;  // 2048 keys per tile
-constexpr int kRadixBits = 8;
+constexpr int kRadixBits = 9;  // 512 digits: a 27-bit cell key (cells in [-256, 255]) in 3 passes
 constexpr int kRadix = 1 << kRadixBits;
 
 // SceneDev.flags

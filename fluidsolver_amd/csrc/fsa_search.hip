@@ -63,6 +63,7 @@ __global__ __launch_bounds__(kBlock) void k_keys(const SceneDev* __restrict__ sc
 }
 
 // ---- 2. radix sort -----------------------------------------------------------
+static_assert(kRadix % kSortThreads == 0, "each sort thread owns a whole number of digits");
 // peers of this lane among `valid` lanes holding the same 8-bit digit
 __device__ __forceinline__ uint64_t match_digit(uint32_t d, uint64_t valid) {
     uint64_t peers = valid;
@@ -81,7 +82,7 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_hist(const SceneDev* __r
     const SceneDev& S = scenes[br.scene];
     const uint32_t tile = br.block;
     __shared__ uint32_t h[kRadix];
-    h[threadIdx.x] = 0;
+    for (uint32_t d = threadIdx.x; d < (uint32_t)kRadix; d += kSortThreads) h[d] = 0;
     __syncthreads();
     const uint32_t* __restrict__ keys = S.keys[src];
     const uint32_t base = tile * kSortTile;
@@ -94,22 +95,22 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_hist(const SceneDev* __r
         if (valid && (peers & lanemask_lt()) == 0) atomicAdd(&h[d], (uint32_t)__popcll(peers));
     }
     __syncthreads();
-    S.hist[threadIdx.x * S.ntiles + tile] = h[threadIdx.x];
+    for (uint32_t d = threadIdx.x; d < (uint32_t)kRadix; d += kSortThreads) S.hist[d * S.ntiles + tile] = h[d];
 }
 
 // wide single-block scans (one block per scene or per (scene, digit))
 constexpr uint32_t kScanThreads = 1024;
 
 // one block per (scene, digit): exclusive scan of the digit's tile counts
-__global__ __launch_bounds__(kScanThreads) void k_radix_scan(const SceneDev* __restrict__ scenes,
-                                                             const BlockRef* __restrict__ map) {
+__global__ __launch_bounds__(kBlock) void k_radix_scan(const SceneDev* __restrict__ scenes,
+                                                       const BlockRef* __restrict__ map) {
     const BlockRef br = map[blockIdx.x];
     const SceneDev& S = scenes[br.scene];
     const uint32_t d = br.block;
     uint32_t* row = S.hist + d * S.ntiles;
-    __shared__ uint32_t lw[kScanThreads / 64];
+    __shared__ uint32_t lw[kBlock / 64];
     uint32_t carry = 0;
-    for (uint32_t base = 0; base < S.ntiles; base += kScanThreads) {
+    for (uint32_t base = 0; base < S.ntiles; base += kBlock) {
         const uint32_t idx = base + threadIdx.x;
         const uint32_t v = idx < S.ntiles ? row[idx] : 0u;
         uint32_t tot;
@@ -135,14 +136,28 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(const SceneDev* 
     __shared__ uint32_t stage_v[kSortTile];
     __shared__ uint32_t lw[kSortThreads / 64];
 
-    // global base of each digit for this tile
+    // global base of each digit for this tile; thread t owns digits t * kDigitsPerThread + q
+    constexpr uint32_t kDigitsPerThread = kRadix / kSortThreads;
     {
+        uint32_t tot_d[kDigitsPerThread], local = 0;
+#pragma unroll
+        for (uint32_t q = 0; q < kDigitsPerThread; q++) {
+            tot_d[q] = S.hist[kRadix * S.ntiles + t * kDigitsPerThread + q];
+            local += tot_d[q];
+        }
         uint32_t tot;
-        const uint32_t ex = block_exclusive_scan(S.hist[kRadix * S.ntiles + t], lw, tot);
-        digit_base[t] = ex + S.hist[t * S.ntiles + tile];
+        uint32_t ex = block_exclusive_scan(local, lw, tot);
+#pragma unroll
+        for (uint32_t q = 0; q < kDigitsPerThread; q++) {
+            const uint32_t d = t * kDigitsPerThread + q;
+            digit_base[d] = ex + S.hist[d * S.ntiles + tile];
+            ex += tot_d[q];
+        }
     }
 #pragma unroll
-    for (int w = 0; w < kSortThreads / 64; w++) wave_cnt[w][t] = 0;
+    for (int w = 0; w < kSortThreads / 64; w++)
+#pragma unroll
+        for (uint32_t q = 0; q < kDigitsPerThread; q++) wave_cnt[w][t * kDigitsPerThread + q] = 0;
     __syncthreads();
 
     const uint32_t* __restrict__ keys = S.keys[src];
@@ -166,15 +181,27 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(const SceneDev* 
     __syncthreads();
     // per digit: exclusive prefix over waves, then exclusive scan over digits
     {
-        uint32_t run = 0;
+        uint32_t runs[kDigitsPerThread], local = 0;
 #pragma unroll
-        for (int w = 0; w < kSortThreads / 64; w++) {
-            const uint32_t c = wave_cnt[w][t];
-            wave_cnt[w][t] = run;
-            run += c;
+        for (uint32_t q = 0; q < kDigitsPerThread; q++) {
+            const uint32_t d = t * kDigitsPerThread + q;
+            uint32_t run = 0;
+#pragma unroll
+            for (int w = 0; w < kSortThreads / 64; w++) {
+                const uint32_t c = wave_cnt[w][d];
+                wave_cnt[w][d] = run;
+                run += c;
+            }
+            runs[q] = run;
+            local += run;
         }
         uint32_t tot;
-        tile_start[t] = block_exclusive_scan(run, lw, tot);
+        uint32_t ex = block_exclusive_scan(local, lw, tot);
+#pragma unroll
+        for (uint32_t q = 0; q < kDigitsPerThread; q++) {
+            tile_start[t * kDigitsPerThread + q] = ex;
+            ex += runs[q];
+        }
     }
     __syncthreads();
 #pragma unroll
@@ -1021,7 +1048,7 @@ void launch_sort(const LaunchSet& L, uint32_t passes, hipStream_t st) {
         const uint32_t src = p & 1u;
         const uint32_t shift = p * kRadixBits;
         k_radix_hist<<<L.nb_tiles, kSortThreads, 0, st>>>(L.scenes, L.map_tiles, shift, src);
-        k_radix_scan<<<L.nb_scan, kScanThreads, 0, st>>>(L.scenes, L.map_scan);
+        k_radix_scan<<<L.nb_scan, kBlock, 0, st>>>(L.scenes, L.map_scan);
         k_radix_scatter<<<L.nb_tiles, kSortThreads, 0, st>>>(L.scenes, L.map_tiles, shift, src);
     }
 }
@@ -1031,7 +1058,7 @@ void launch_sort_range(const LaunchSet& L, uint32_t first_src, uint32_t shift0, 
         const uint32_t src = first_src ^ (p & 1u);
         const uint32_t shift = shift0 + p * kRadixBits;
         k_radix_hist<<<L.nb_tiles, kSortThreads, 0, st>>>(L.scenes, L.map_tiles, shift, src);
-        k_radix_scan<<<L.nb_scan, kScanThreads, 0, st>>>(L.scenes, L.map_scan);
+        k_radix_scan<<<L.nb_scan, kBlock, 0, st>>>(L.scenes, L.map_scan);
         k_radix_scatter<<<L.nb_tiles, kSortThreads, 0, st>>>(L.scenes, L.map_tiles, shift, src);
     }
 }
