@@ -39,8 +39,8 @@ def algorithmic_bytes(nf, nba, L):
     reach of fluid (nba) read once per pass that needs them."""
     return {
         "search": nf * (12 + 2 * 32 + L) + 12 * nba,
-        "density": nf * (64 + L) + 32 * nba,
-        "diagonal": nf * (40 + L) + 16 * nba,
+        "density": nf * (64 + 16 + L) + 32 * nba,  # + d_ii written for phase 2
+        "diagonal": nf * (40 + 16 + L) + 16 * nba,  # + d_ii read
         "pressure_a": nf * (32 + L) + 16 * nba,
         "pressure_b": nf * (56 + L) + 16 * nba,
         "integrate": nf * 72,

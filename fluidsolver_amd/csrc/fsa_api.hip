@@ -257,6 +257,7 @@ uint64_t plan_bytes(const Plan& p) {
     add(16 * n);                               // anp
     add(4 * n);                                // src
     add(4 * n);                                // diag
+    add(16 * std::max<uint64_t>(p.nf, 1));     // dii
     for (int k = 0; k < 4; k++) add(4 * n);    // keys, vals x2
     add(4 * (kRadix * std::max<uint64_t>(p.ntiles, 1) + kRadix));
     add(4 * std::max<uint64_t>(p.nf, 1));      // fluid_idx
@@ -296,6 +297,7 @@ void carve(fsa_scene_s* s, const Plan& p) {
     h.anp = (float4*)take(16 * n);
     h.src = (float*)take(4 * n);
     h.diag = (float*)take(4 * n);
+    h.dii = (float4*)take(16 * std::max<uint64_t>(p.nf, 1));
     for (int k = 0; k < 2; k++) h.keys[k] = (uint32_t*)take(4 * n);
     for (int k = 0; k < 2; k++) h.vals[k] = (uint32_t*)take(4 * n);
     h.hist = (uint32_t*)take(4 * (kRadix * std::max<uint64_t>(p.ntiles, 1) + kRadix));
@@ -979,14 +981,14 @@ void autotune(fsa_context ctx, BatchSet* b, const std::vector<fsa_scene_s*>& v) 
         return;
     }
     hipStream_t st = ctx->stream;
-    // scratch per scene: acc, vstar, anp, ext (16 B) and pres, err, rho_new, src, diag (4 B)
+    // scratch per scene: acc, vstar, anp, ext, dii (16 B) and pres, err, rho_new, src, diag (4 B)
     // per particle, partials per block, one state
     std::vector<uint64_t> offs(v.size());
     uint64_t total = 0;
     for (size_t q = 0; q < v.size(); q++) {
         offs[q] = total;
         const uint64_t n = std::max<uint32_t>(v[q]->n, 1);
-        total += align_up(4 * 16 * n, 256) + align_up(5 * 4 * n, 256) +
+        total += align_up(4 * 16 * n, 256) + align_up(5 * 4 * n, 256) + align_up(16 * n, 256) +
                  align_up(sizeof(Partial) * std::max<uint32_t>(v[q]->hd.nblocks, 1), 256) +
                  align_up(sizeof(SceneState), 256);
     }
@@ -1025,6 +1027,7 @@ void autotune(fsa_context ctx, BatchSet* b, const std::vector<fsa_scene_s*>& v) 
         d.rho_new = f4 + 2 * n;
         d.src = f4 + 3 * n;
         d.diag = f4 + 4 * n;
+        d.dii = (float4*)take(16 * n);
         d.partial = (Partial*)take(sizeof(Partial) * std::max<uint32_t>(h.nblocks, 1));
         d.state = (SceneState*)take(sizeof(SceneState));
         // realistic inputs: copies of the step's arrays, a state that keeps the loop active
@@ -1039,6 +1042,7 @@ void autotune(fsa_context ctx, BatchSet* b, const std::vector<fsa_scene_s*>& v) 
             HIP_CHECK(hipMemcpyAsync(d.rho_new, h.rho_new, 4 * nn, hipMemcpyDeviceToDevice, st));
             HIP_CHECK(hipMemcpyAsync(d.src, h.src, 4 * nn, hipMemcpyDeviceToDevice, st));
             HIP_CHECK(hipMemcpyAsync(d.diag, h.diag, 4 * nn, hipMemcpyDeviceToDevice, st));
+            if (v[q]->nf) HIP_CHECK(hipMemcpyAsync(d.dii, h.dii, 16ull * v[q]->nf, hipMemcpyDeviceToDevice, st));
         }
         SceneState ss = *v[q]->h_state;
         ss.active = 1;

@@ -176,6 +176,7 @@ struct alignas(16) SceneDev {
     float4* anp;           // original non-pressure acceleration
     float* src;            // source term
     float* diag;           // diagonal element a_ii
+    float4* dii;           // IISPH d_ii of fluid rank k, summed in phase 1 for phase 2
     // search
     uint32_t* keys[2];
     uint32_t* vals[2];

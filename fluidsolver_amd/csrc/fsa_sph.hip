@@ -166,6 +166,7 @@ __global__ __launch_bounds__(kBlock) void k_iisph_prepare(const SceneDev* __rest
     }
     const bool sl = (S.flags & kFlagSingleLayer) != 0;
     float tx = 0.0f, ty = 0.0f, tz = 0.0f, rho = 0.0f;
+    float ix = 0.0f, iy = 0.0f, iz = 0.0f;  // d_ii of phase 2, from the same pair gradients
     walk_neighbors<U>(
         gload_ptr(S.nbr + k), S.list_stride, S.nbr_cnt[k],
         [&](uint32_t j) { return PosVel{pos[j], vel[j]}; },
@@ -174,6 +175,15 @@ __global__ __launch_bounds__(kBlock) void k_iisph_prepare(const SceneDev* __rest
             float dx, dy, dz, d2, len;
             Grad g;
             pair_geometry<Spec>(S, xi, xj, dx, dy, dz, d2, len, g);
+            // d_ii (IISPHFluidSolver3D.hpp:231-262): -sum f_j gradW_ij in list order
+            float f;
+            if (!(ent & kBoundaryBit))
+                f = div_rho0sq<Spec>(S, xj.w);
+            else
+                f = sl ? div_rho0sq<Spec>(S, 2.0f * S.gamma * xj.w) : div_rho0sq<Spec>(S, 2.0f * xj.w);
+            ix = ix - f * g.x;
+            iy = iy - f * g.y;
+            iz = iz - f * g.z;
             // ComputeViscosityAcceleration: densities of the start-of-step snapshot
             const float nd = vj.w;
             if (nd != 0.0f) {
@@ -196,6 +206,7 @@ __global__ __launch_bounds__(kBlock) void k_iisph_prepare(const SceneDev* __rest
     S.anp[i] = make_float4(nx, ny, nz, 0.0f);
     S.rho_new[i] = rho;
     S.pres[c][i] = 0.0f;
+    S.dii[k] = make_float4(ix, iy, iz, 0.0f);
     if (S.flags & kFlagExternal) S.ext[c][i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 }
 
@@ -221,6 +232,11 @@ __global__ __launch_bounds__(kBlock) void k_iisph_prepare_boundary(const SceneDe
 }
 
 // ---- IISPH phase 2: source term and diagonal element -----------------------------
+// IISPHFluidSolver3D.hpp:216-339.  The reference walks a particle's neighbors
+// twice: for the source term and d_ii, then for a_ii, which needs the whole
+// d_ii.  Phase 1 already forms every pair gradient, so it sums d_ii there (the
+// same terms in the same order, bit for bit), and phase 2 walks once for the
+// source term and a_ii together.
 template <uint32_t Spec, int U>
 __global__ __launch_bounds__(kBlock) void k_iisph_diagonal(const SceneDev* __restrict__ scenes,
                                                            const BlockRef* __restrict__ map) {
@@ -235,11 +251,13 @@ __global__ __launch_bounds__(kBlock) void k_iisph_diagonal(const SceneDev* __res
     const auto vstar = gload_ptr(S.vstar);
     const float4 xi = pos[i];
     const float4 vs = vstar[i];
-    const bool sl = (S.flags & kFlagSingleLayer) != 0;
     const uint32_t cnt = S.nbr_cnt[k];
     const auto nb = gload_ptr(S.nbr + k);
-    float contrib = 0.0f;
-    float ix = 0.0f, iy = 0.0f, iz = 0.0f;
+    // d_ii was summed in phase 1 over the same pairs, in list order
+    const float4 di = S.dii[k];
+    const float ix = di.x, iy = di.y, iz = di.z;
+    const float cm = div_rho0sq<Spec>(S, xi.w);
+    float contrib = 0.0f, dg = 0.0f;
     walk_neighbors<U>(
         nb, S.list_stride, cnt, [&](uint32_t j) { return PosVel{pos[j], vstar[j]}; },
         [&](uint32_t ent, const PosVel& d) {
@@ -247,35 +265,20 @@ __global__ __launch_bounds__(kBlock) void k_iisph_diagonal(const SceneDev* __res
             float dx, dy, dz, d2, len;
             Grad g;
             pair_geometry<Spec>(S, xi, xj, dx, dy, dz, d2, len, g);
-            float f;
             if (!(ent & kBoundaryBit)) {
                 const float4 vsj = d.v;
                 contrib = contrib + xj.w * dot3(vs.x - vsj.x, vs.y - vsj.y, vs.z - vsj.z, g.x, g.y, g.z);
-                f = div_rho0sq<Spec>(S, xj.w);
             } else {
                 contrib = contrib + xj.w * dot3(vs.x, vs.y, vs.z, g.x, g.y, g.z);
-                f = sl ? div_rho0sq<Spec>(S, 2.0f * S.gamma * xj.w) : div_rho0sq<Spec>(S, 2.0f * xj.w);
             }
-            ix = ix - f * g.x;
-            iy = iy - f * g.y;
-            iz = iz - f * g.z;
-        });
-    const float dt = S.dt;
-    S.src[i] = S.rest_density - S.rho_new[i] - dt * contrib;
-    const float cm = div_rho0sq<Spec>(S, xi.w);
-    float dg = 0.0f;
-    walk_neighbors<U>(
-        nb, S.list_stride, cnt, [&](uint32_t j) { return pos[j]; },
-        [&](uint32_t ent, const float4& xj) {
-            float dx, dy, dz, d2, len;
-            Grad g;
-            pair_geometry<Spec>(S, xi, xj, dx, dy, dz, d2, len, g);
             dg = dg + xj.w * dot3(ix, iy, iz, g.x, g.y, g.z);
             if (!(ent & kBoundaryBit)) {
                 // (m_i / rho0^2) * gradW(x_i, x_j) with gradW(x_i, x_j) = -gradW(x_j, x_i) exactly
                 dg = dg + xj.w * dot3(-(cm * g.x), -(cm * g.y), -(cm * g.z), g.x, g.y, g.z);
             }
         });
+    const float dt = S.dt;
+    S.src[i] = S.rest_density - S.rho_new[i] - dt * contrib;
     S.diag[i] = dg * (dt * dt);
 }
 
