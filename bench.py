@@ -40,13 +40,14 @@ def algorithmic_bytes(nf, nba, L):
     return {
         "search": nf * (12 + 2 * 32 + L) + 12 * nba,
         "density": nf * (64 + 16 + L) + 32 * nba,  # + d_ii written for phase 2
-        "diagonal": nf * (40 + 16 + L) + 16 * nba,  # + d_ii read
+        # + d_ii read, + the first Jacobi iteration's update (pressure read and
+        # written, acceleration and error written)
+        "diagonal": nf * (40 + 16 + 28 + L) + 16 * nba,
         "pressure_a": nf * (32 + L) + 16 * nba,
         "pressure_b": nf * (56 + L) + 16 * nba,
         "integrate": nf * 72,
-        # the first Jacobi iteration (every pressure zero: no neighbor walk):
-        # rank map, diagonal, source, pressure, v* read; acceleration, pressure, error written
-        "pressure_first": nf * (4 + 4 + 4 + 4 + 16 + 16 + 4 + 4),
+        # the first Jacobi iteration's convergence decision: the block partials
+        "pressure_first": 24 * ((nf + 255) // 256),
     }
 
 

@@ -566,6 +566,9 @@ void simulate_set(fsa_context ctx, const LaunchSet& L, std::vector<fsa_scene_s*>
             else
                 launch_iisph_prepare(L, boundary_pass, st);
         }
+        // 3D: the loop state is set before phase 2, which performs the first
+        // Jacobi iteration's pressure update on the way
+        if (!d2) launch_iisph_begin_loop(L, st);
         {
             PhaseTimer t(ctx, FSA_TIMER_DIAGONAL);
             if (d2)
@@ -573,10 +576,7 @@ void simulate_set(fsa_context ctx, const LaunchSet& L, std::vector<fsa_scene_s*>
             else
                 launch_iisph_diagonal(L, st);
         }
-        if (d2)
-            launch2_iisph_begin_loop(L, st);
-        else
-            launch_iisph_begin_loop(L, st);
+        if (d2) launch2_iisph_begin_loop(L, st);
         uint32_t enq = 0, max_it = 0, guess = 0;
         for (auto* s : scenes) {
             max_it = std::max(max_it, s->hd.max_it);
@@ -586,9 +586,9 @@ void simulate_set(fsa_context ctx, const LaunchSet& L, std::vector<fsa_scene_s*>
         while (enq < max_it) {
             for (uint32_t k = 0; k < chunk && enq < max_it; k++, enq++) {
                 if (!d2 && enq == 0) {
-                    // every pressure is zero: the first iteration needs no neighbor walk
+                    // every pressure is zero: phase 2 did the first iteration's update
                     PhaseTimer t(ctx, FSA_TIMER_PRESSURE_FIRST);
-                    launch_iisph_first_iteration(L, st);
+                    launch_iisph_first_converge(L, st);
                     continue;
                 }
                 {
@@ -1116,6 +1116,9 @@ void step_head(fsa_context ctx, const LaunchSet& L, bool d2, bool unsorted, uint
         else
             launch_iisph_prepare(L, bpass, st);
     }
+    // 3D: the loop state is set before phase 2, which performs the first
+    // Jacobi iteration's pressure update on the way
+    if (!d2) launch_iisph_begin_loop(L, st);
     {
         PhaseTimer t(ctx, FSA_TIMER_DIAGONAL);
         if (d2)
@@ -1123,15 +1126,12 @@ void step_head(fsa_context ctx, const LaunchSet& L, bool d2, bool unsorted, uint
         else
             launch_iisph_diagonal(L, st);
     }
-    if (d2)
-        launch2_iisph_begin_loop(L, st);
-    else
-        launch_iisph_begin_loop(L, st);
+    if (d2) launch2_iisph_begin_loop(L, st);
     for (uint32_t k = 0; k < chunk; k++) {
         if (!d2 && k == 0) {
-            // every pressure is zero: the first iteration needs no neighbor walk
+            // every pressure is zero: phase 2 did the first iteration's update
             PhaseTimer t(ctx, FSA_TIMER_PRESSURE_FIRST);
-            launch_iisph_first_iteration(L, st);
+            launch_iisph_first_converge(L, st);
             continue;
         }
         {

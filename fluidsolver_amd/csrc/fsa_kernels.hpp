@@ -74,7 +74,7 @@ void launch_iisph_prepare(const LaunchSet& L, bool boundary_pass, hipStream_t st
 void launch_iisph_diagonal(const LaunchSet& L, hipStream_t st);
 void launch_iisph_begin_loop(const LaunchSet& L, hipStream_t st);
 void launch_iisph_pass_a(const LaunchSet& L, hipStream_t st);
-void launch_iisph_first_iteration(const LaunchSet& L, hipStream_t st);
+void launch_iisph_first_converge(const LaunchSet& L, hipStream_t st);
 void launch_iisph_pass_b(const LaunchSet& L, hipStream_t st);
 void launch_sesph_density(const LaunchSet& L, hipStream_t st);
 void launch_sesph_accel(const LaunchSet& L, hipStream_t st);

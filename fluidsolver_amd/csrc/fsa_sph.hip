@@ -231,6 +231,34 @@ __global__ __launch_bounds__(kBlock) void k_iisph_prepare_boundary(const SceneDe
     if (S.flags & kFlagExternal) S.ext[c][i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 }
 
+// Pass B's end for fluid particle i (IISPHFluidSolver3D.hpp:440-462): ap *= dt^2,
+// the relaxed Jacobi pressure update, the predicted density error, and this
+// particle's share of the convergence and CFL partials.
+__device__ __forceinline__ void pressure_update(const SceneDev& S, uint32_t c, uint32_t i, const float4& ai, float ap,
+                                                double& err_sum, uint32_t& counted, float& a2, float& v2) {
+    const float dt = S.dt;
+    ap = ap * (dt * dt);
+    const float dg = S.diag[i];
+    const float s = S.src[i];
+    float p = S.pres[c][i];
+    if (fabsf(dg) > kEps)
+        p = fmaxf(0.0f, p + S.omega * ((s - ap) / dg));
+    else
+        p = 0.0f;
+    float e = fabsf(s - ap);
+    if (fabsf(dg) > kEps) {
+        if (fabsf(p) <= kEps) e = 0.0f;
+        S.err[c][i] = e;
+        err_sum = (double)e;
+        counted = 1;
+    }
+    S.pres[c][i] = p;
+    a2 = dot3(ai.x, ai.y, ai.z, ai.x, ai.y, ai.z);
+    const float4 vs = S.vstar[i];
+    const float px = vs.x + dt * ai.x, py = vs.y + dt * ai.y, pz = vs.z + dt * ai.z;
+    v2 = dot3(px, py, pz, px, py, pz);
+}
+
 // ---- IISPH phase 2: source term and diagonal element -----------------------------
 // IISPHFluidSolver3D.hpp:216-339.  The reference walks a particle's neighbors
 // twice: for the source term and d_ii, then for a_ii, which needs the whole
@@ -243,43 +271,63 @@ __global__ __launch_bounds__(kBlock) void k_iisph_diagonal(const SceneDev* __res
     const BlockRef br = map[blockIdx.x];
     const SceneDev& S = scenes[br.scene];
     if (scene_failed(S)) return;
+    // the first Jacobi iteration rides along (the loop state is set before this kernel)
+    const bool first = S.state->active != 0;
+    __shared__ RedLds lds;
+    double err_sum = 0.0;
+    uint32_t counted = 0;
+    float a2 = 0.0f, v2 = 0.0f;
     const uint32_t k = br.block * kBlock + threadIdx.x;
-    if (k >= S.nf) return;
-    const uint32_t c = S.state->cur;
-    const uint32_t i = S.fluid_idx[k];
-    const auto pos = gload_ptr(S.pos[c]);
-    const auto vstar = gload_ptr(S.vstar);
-    const float4 xi = pos[i];
-    const float4 vs = vstar[i];
-    const uint32_t cnt = S.nbr_cnt[k];
-    const auto nb = gload_ptr(S.nbr + k);
-    // d_ii was summed in phase 1 over the same pairs, in list order
-    const float4 di = S.dii[k];
-    const float ix = di.x, iy = di.y, iz = di.z;
-    const float cm = div_rho0sq<Spec>(S, xi.w);
-    float contrib = 0.0f, dg = 0.0f;
-    walk_neighbors<U>(
-        nb, S.list_stride, cnt, [&](uint32_t j) { return PosVel{pos[j], vstar[j]}; },
-        [&](uint32_t ent, const PosVel& d) {
-            const float4 xj = d.x;
-            float dx, dy, dz, d2, len;
-            Grad g;
-            pair_geometry<Spec>(S, xi, xj, dx, dy, dz, d2, len, g);
-            if (!(ent & kBoundaryBit)) {
-                const float4 vsj = d.v;
-                contrib = contrib + xj.w * dot3(vs.x - vsj.x, vs.y - vsj.y, vs.z - vsj.z, g.x, g.y, g.z);
-            } else {
-                contrib = contrib + xj.w * dot3(vs.x, vs.y, vs.z, g.x, g.y, g.z);
-            }
-            dg = dg + xj.w * dot3(ix, iy, iz, g.x, g.y, g.z);
-            if (!(ent & kBoundaryBit)) {
-                // (m_i / rho0^2) * gradW(x_i, x_j) with gradW(x_i, x_j) = -gradW(x_j, x_i) exactly
-                dg = dg + xj.w * dot3(-(cm * g.x), -(cm * g.y), -(cm * g.z), g.x, g.y, g.z);
-            }
-        });
-    const float dt = S.dt;
-    S.src[i] = S.rest_density - S.rho_new[i] - dt * contrib;
-    S.diag[i] = dg * (dt * dt);
+    if (k < S.nf) {
+        const uint32_t c = S.state->cur;
+        const uint32_t i = S.fluid_idx[k];
+        const auto pos = gload_ptr(S.pos[c]);
+        const auto vstar = gload_ptr(S.vstar);
+        const float4 xi = pos[i];
+        const float4 vs = vstar[i];
+        const uint32_t cnt = S.nbr_cnt[k];
+        const auto nb = gload_ptr(S.nbr + k);
+        // d_ii was summed in phase 1 over the same pairs, in list order
+        const float4 di = S.dii[k];
+        const float ix = di.x, iy = di.y, iz = di.z;
+        const float cm = div_rho0sq<Spec>(S, xi.w);
+        float contrib = 0.0f, dg = 0.0f;
+        walk_neighbors<U>(
+            nb, S.list_stride, cnt, [&](uint32_t j) { return PosVel{pos[j], vstar[j]}; },
+            [&](uint32_t ent, const PosVel& d) {
+                const float4 xj = d.x;
+                float dx, dy, dz, d2, len;
+                Grad g;
+                pair_geometry<Spec>(S, xi, xj, dx, dy, dz, d2, len, g);
+                if (!(ent & kBoundaryBit)) {
+                    const float4 vsj = d.v;
+                    contrib = contrib + xj.w * dot3(vs.x - vsj.x, vs.y - vsj.y, vs.z - vsj.z, g.x, g.y, g.z);
+                } else {
+                    contrib = contrib + xj.w * dot3(vs.x, vs.y, vs.z, g.x, g.y, g.z);
+                }
+                dg = dg + xj.w * dot3(ix, iy, iz, g.x, g.y, g.z);
+                if (!(ent & kBoundaryBit)) {
+                    // (m_i / rho0^2) * gradW(x_i, x_j) with gradW(x_i, x_j) = -gradW(x_j, x_i) exactly
+                    dg = dg + xj.w * dot3(-(cm * g.x), -(cm * g.y), -(cm * g.z), g.x, g.y, g.z);
+                }
+            });
+        const float dt = S.dt;
+        S.src[i] = S.rest_density - S.rho_new[i] - dt * contrib;
+        S.diag[i] = dg * (dt * dt);
+        if (first) {
+            // Phase 1 set every pressure to zero (IISPHFluidSolver3D.hpp:150-151): the first
+            // iteration's pass A subtracts only signed zeros from +0 (accelerations exactly
+            // (+0, +0, +0)) and its pass B adds only m_j * dot(+0 - +0, gradW) to +0 (ap exactly
+            // +0), so its pressure update follows from this particle's own values.
+            const float4 ai = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            S.acc[c][i] = ai;
+            pressure_update(S, c, i, ai, 0.0f, err_sum, counted, a2, v2);
+        }
+    }
+    if (first) {
+        const Partial part = block_reduce(err_sum, counted, a2, v2, lds);
+        if (threadIdx.x == 0) S.partial[br.block] = part;
+    }
 }
 
 // ---- Jacobi loop control -------------------------------------------------------------
@@ -337,34 +385,6 @@ __global__ __launch_bounds__(kBlock) void k_iisph_pass_a(const SceneDev* __restr
 }
 
 // ---- Jacobi pass B: A*p, relaxed pressure update, error, reductions -------------------
-// Pass B's end for fluid particle i (IISPHFluidSolver3D.hpp:440-462): ap *= dt^2,
-// the relaxed Jacobi pressure update, the predicted density error, and this
-// particle's share of the convergence and CFL partials.
-__device__ __forceinline__ void pressure_update(const SceneDev& S, uint32_t c, uint32_t i, const float4& ai, float ap,
-                                                double& err_sum, uint32_t& counted, float& a2, float& v2) {
-    const float dt = S.dt;
-    ap = ap * (dt * dt);
-    const float dg = S.diag[i];
-    const float s = S.src[i];
-    float p = S.pres[c][i];
-    if (fabsf(dg) > kEps)
-        p = fmaxf(0.0f, p + S.omega * ((s - ap) / dg));
-    else
-        p = 0.0f;
-    float e = fabsf(s - ap);
-    if (fabsf(dg) > kEps) {
-        if (fabsf(p) <= kEps) e = 0.0f;
-        S.err[c][i] = e;
-        err_sum = (double)e;
-        counted = 1;
-    }
-    S.pres[c][i] = p;
-    a2 = dot3(ai.x, ai.y, ai.z, ai.x, ai.y, ai.z);
-    const float4 vs = S.vstar[i];
-    const float px = vs.x + dt * ai.x, py = vs.y + dt * ai.y, pz = vs.z + dt * ai.z;
-    v2 = dot3(px, py, pz, px, py, pz);
-}
-
 template <uint32_t Spec, int U>
 __global__ __launch_bounds__(kBlock) void k_iisph_pass_b(const SceneDev* __restrict__ scenes,
                                                          const BlockRef* __restrict__ map) {
@@ -400,35 +420,6 @@ __global__ __launch_bounds__(kBlock) void k_iisph_pass_b(const SceneDev* __restr
                 }
             });
         pressure_update(S, c, i, ai, ap, err_sum, counted, a2, v2);
-    }
-    const Partial part = block_reduce(err_sum, counted, a2, v2, lds);
-    if (threadIdx.x == 0) S.partial[br.block] = part;
-}
-
-// The first Jacobi iteration without neighbor walks.  Phase 1 sets every
-// pressure to zero (IISPHFluidSolver3D.hpp:150-151), so pass A's sum is
-// -m_j * (0 + 0) * gradW terms: signed zeros subtracted from +0, which leaves
-// the pressure acceleration exactly (+0, +0, +0); pass B's sum adds
-// m_j * dot(+0 - +0, gradW) terms to +0, which leaves ap exactly +0.  The
-// pressure update and the convergence partials then follow from the particle's
-// own values, bit for bit as the walking passes compute them.
-__global__ __launch_bounds__(kBlock) void k_iisph_first_iteration(const SceneDev* __restrict__ scenes,
-                                                                  const BlockRef* __restrict__ map) {
-    const BlockRef br = map[blockIdx.x];
-    const SceneDev& S = scenes[br.scene];
-    if (scene_failed(S)) return;
-    if (!S.state->active) return;
-    __shared__ RedLds lds;
-    const uint32_t k = br.block * kBlock + threadIdx.x;
-    const uint32_t c = S.state->cur;
-    double err_sum = 0.0;
-    uint32_t counted = 0;
-    float a2 = 0.0f, v2 = 0.0f;
-    if (k < S.nf) {
-        const uint32_t i = S.fluid_idx[k];
-        const float4 ai = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        S.acc[c][i] = ai;
-        pressure_update(S, c, i, ai, 0.0f, err_sum, counted, a2, v2);
     }
     const Partial part = block_reduce(err_sum, counted, a2, v2, lds);
     if (threadIdx.x == 0) S.partial[br.block] = part;
@@ -808,8 +799,8 @@ void launch_iisph_begin_loop(const LaunchSet& L, hipStream_t st) {
 void launch_iisph_pass_a(const LaunchSet& L, hipStream_t st) {
     if (L.nb_fluid) FSA_UNROLL_DISPATCH(k_iisph_pass_a, kTunePassA, L.nb_fluid, L.map_fluid);
 }
-void launch_iisph_first_iteration(const LaunchSet& L, hipStream_t st) {
-    if (L.nb_fluid) k_iisph_first_iteration<<<L.nb_fluid, kBlock, 0, st>>>(L.scenes, L.map_fluid);
+// the first Jacobi iteration's convergence decision (its update ran in phase 2)
+void launch_iisph_first_converge(const LaunchSet& L, hipStream_t st) {
     k_iisph_converge<<<L.nb_scene, kBlock, 0, st>>>(L.scenes, L.map_scene);
 }
 void launch_iisph_pass_b(const LaunchSet& L, hipStream_t st) {
