@@ -17,6 +17,7 @@
 #include <cstring>
 #include <exception>
 #include <map>
+#include <future>
 #include <numeric>
 #include <mutex>
 #include <new>
@@ -703,19 +704,19 @@ uint64_t reference_cell_key(int32_t cx, int32_t cy, int32_t cz) {
 
 // Sort::quick_sort (ParticleCollectionAlgorithm.hpp:28-127): Hoare partition
 // around the middle element's key; the permutation (position k takes old
-// position perm[k]) is what swapping every component would produce.
-std::vector<uint32_t> quick_sort_order(std::vector<uint64_t> key) {
-    const size_t n = key.size();
-    std::vector<uint32_t> perm(n);
-    std::iota(perm.begin(), perm.end(), 0u);
+// position perm[k]) is what swapping every component would produce.  Like the
+// reference (which recurses with std::async for parts of 10000 and more,
+// :106-125), disjoint parts are sorted on their own threads; each part's
+// partitioning is the same, so the permutation does not depend on the threads.
+void quick_sort_part(uint64_t* key, uint32_t* perm, int64_t low, int64_t high, int depth) {
     std::vector<std::pair<int64_t, int64_t>> todo;
-    if (n > 1) todo.push_back({0, (int64_t)n - 1});
+    todo.push_back({low, high});
     while (!todo.empty()) {
-        const auto [low, high] = todo.back();
+        const auto [lo, hi] = todo.back();
         todo.pop_back();
-        if (low >= high) continue;
-        const uint64_t pivot = key[low + (high - low) / 2];
-        int64_t i = low - 1, j = high + 1;
+        if (lo >= hi) continue;
+        const uint64_t pivot = key[lo + (hi - lo) / 2];
+        int64_t i = lo - 1, j = hi + 1;
         for (;;) {
             do {
                 i++;
@@ -727,9 +728,22 @@ std::vector<uint32_t> quick_sort_order(std::vector<uint64_t> key) {
             std::swap(key[i], key[j]);
             std::swap(perm[i], perm[j]);
         }
-        todo.push_back({low, j});
-        todo.push_back({j + 1, high});
+        if (depth > 0 && j - lo + 1 >= 10000 && hi - j >= 10000) {
+            auto lower = std::async(std::launch::async, quick_sort_part, key, perm, lo, j, depth - 1);
+            quick_sort_part(key, perm, j + 1, hi, depth - 1);
+            lower.get();
+        } else {
+            todo.push_back({lo, j});
+            todo.push_back({j + 1, hi});
+        }
     }
+}
+
+std::vector<uint32_t> quick_sort_order(std::vector<uint64_t> key) {
+    const size_t n = key.size();
+    std::vector<uint32_t> perm(n);
+    std::iota(perm.begin(), perm.end(), 0u);
+    if (n > 1) quick_sort_part(key.data(), perm.data(), 0, (int64_t)n - 1, 5);  // at most 32 parts at once
     return perm;
 }
 

@@ -122,3 +122,25 @@ def test_3d_search_mode_lists_dense_and_sparse(gpu_context, oracle_lib, search, 
     longest = max(int(goff[r + 1] - goff[r]) for r in fl)
     assert longest > (70 if spacing < 1.0 else 30), longest
     sc.close()
+
+
+def test_hashed_resort_of_a_large_collection(gpu_context, oracle_lib):
+    """The 101st hashed search re-sorts a collection large enough for the
+    quick sort's parts to be partitioned on several threads (parts of 10000
+    and more): the order, and so every later step, equals the oracle's."""
+    O = oracle_lib
+    coll = _shuffled(scenes.dam_scene((24, 24, 24), (40, 30, 30), 2, seed=64), 11)
+    assert coll.size() > 25000
+    d = abi.default_desc(3, abi.SOLVER_SESPH, timestep=0.004, sesph_stiffness_k=1000.0)
+    d.search = abi.SEARCH_HASHED_3D
+    print("case hashed resort large", coll.size(), flush=True)
+    sc = Scene(gpu_context, d, coll.copy())
+    orc = O.OracleScene(d, coll.copy(), search=O.SEARCH_HASHED3D, mode=O.MODE_FP32_R64)
+    for step in range(104):
+        st, ost = sc.step(1), orc.step(1)
+        assert st.actual_time_step == ost.actual_time_step, step
+    assert not np.array_equal(orc.internal_ids(), np.arange(coll.size()))  # the re-sort happened
+    g, r = sc.download(coll.copy()), orc.download()
+    np.testing.assert_array_equal(g.movement, r.movement)
+    np.testing.assert_array_equal(g.particle_data, r.particle_data)
+    sc.close()
