@@ -246,6 +246,8 @@ def main():
     ap.add_argument("--side", type=int, default=100, help="fluid block side (100 -> 1,000,000 fluid)")
     ap.add_argument("--search", choices=sorted(SEARCHES), default="compressed",
                     help="neighbor order of the main workload: the reference search it reproduces")
+    ap.add_argument("--solver", choices=["iisph", "sesph"], default="iisph",
+                    help="solver of the main workload (SESPH: stiffness 1000)")
     ap.add_argument("--cpu-steps", type=int, default=10)
     ap.add_argument("--cpu-side", type=int, default=100)
     ap.add_argument("--no-cpu", action="store_true")
@@ -276,7 +278,10 @@ def main():
     coll = scenes.config2_dam3d(seed=2, side=side, interior=interior)
     counts = coll.counts()
     nf, nb = counts["fluid"], counts["boundary"]
-    desc = abi.default_desc(3, abi.SOLVER_IISPH)
+    if args.solver == "sesph":
+        desc = abi.default_desc(3, abi.SOLVER_SESPH, sesph_stiffness_k=1000.0)
+    else:
+        desc = abi.default_desc(3, abi.SOLVER_IISPH)
     desc.search = SEARCHES[args.search][0]
     log(f"scene: {nf} fluid, {nb} boundary, {args.search} search")
 
@@ -380,7 +385,8 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": {
-            "workload": f"3D IISPH breaking dam, {nf} fluid + {nb} boundary particles, {SEARCHES[args.search][1]} search",
+            "workload": f"3D {args.solver.upper()} breaking dam, {nf} fluid + {nb} boundary particles, "
+                        f"{SEARCHES[args.search][1]} search",
             "fluid_particles": nf,
             "boundary_particles": nb,
             "mean_jacobi_iterations": float(np.mean(iters)),

@@ -534,8 +534,7 @@ struct PhaseTimer {
 // ---- the step, for any set of scenes sharing a launch set --------------------------------
 void search_set(fsa_context ctx, const LaunchSet& L, uint32_t passes) {
     PhaseTimer t(ctx, FSA_TIMER_SEARCH);
-    launch_keys(L, ctx->stream);
-    launch_sort(L, passes, ctx->stream);
+    launch_keys_sort(L, passes, ctx->stream);
     launch_reorder(L, passes & 1u, ctx->stream);
     launch_lists(L, passes & 1u, ctx->stream);
 }

@@ -55,8 +55,7 @@ void read_statistics(const void* host_copy, uint64_t counts[3], float mx[3], flo
 uint64_t statistics_result_bytes();  // one accumulator; the result sits at the end of the scratch
 
 // neighborhood search
-void launch_keys(const LaunchSet& L, hipStream_t st);
-void launch_sort(const LaunchSet& L, uint32_t passes, hipStream_t st);
+void launch_keys_sort(const LaunchSet& L, uint32_t passes, hipStream_t st);
 void launch_reorder(const LaunchSet& L, uint32_t sbuf, hipStream_t st);
 void launch_lists(const LaunchSet& L, uint32_t sbuf, hipStream_t st);
 // hashed / quadratic 3D searches over the kept collection order

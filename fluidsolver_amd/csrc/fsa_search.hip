@@ -2,7 +2,7 @@
 //
 // Replaces CompressedNeighborhoodSearch::find_neighbors
 // (CompressedNeighbors.cpp:168-240):
-//   1. k_keys         cell = floor(x / r), compact Morton key      (:174-189, 37-56)
+//   1. k_keys_hist    cell = floor(x / r), compact Morton key      (:174-189, 37-56)
 //   2. k_radix_*      stable LSD radix sort of (key, index)        (:191-201)
 //   3. k_permute      gather every component into sorted order     (ParticleCollection::swap)
 //   4. k_fluid_scan / k_compact   fluid rank -> combined index
@@ -46,21 +46,7 @@ __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* l
     return off + x - v;
 }
 
-// ---- 1. keys --------------------------------------------------------------
-__global__ __launch_bounds__(kBlock) void k_keys(const SceneDev* __restrict__ scenes,
-                                                 const BlockRef* __restrict__ map) {
-    const BlockRef br = map[blockIdx.x];
-    const SceneDev& S = scenes[br.scene];
-    const uint32_t i = br.block * kBlock + threadIdx.x;
-    if (i >= S.n) return;
-    const uint32_t c = S.state->cur;
-    const float4 p = S.pos[c][i];
-    const int32_t cx = cell_of(p.x, S.search_radius);
-    const int32_t cy = cell_of(p.y, S.search_radius);
-    const int32_t cz = cell_of(p.z, S.search_radius);
-    S.keys[0][i] = compact_key(cx, cy, cz, S.levels);
-    S.vals[0][i] = i;
-}
+// ---- 1. keys (in k_keys_hist, with the first radix histogram) ---------------------
 
 // ---- 2. radix sort -----------------------------------------------------------
 static_assert(kRadix % kSortThreads == 0, "each sort thread owns a whole number of digits");
@@ -100,6 +86,38 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_hist(const SceneDev* __r
 
 // wide single-block scans (one block per scene or per (scene, digit))
 constexpr uint32_t kScanThreads = 1024;
+
+// k_keys and the first pass's k_radix_hist in one pass over the sort tiles:
+// each tile's keys are computed, stored, and counted by their lowest digit.
+__global__ __launch_bounds__(kSortThreads) void k_keys_hist(const SceneDev* __restrict__ scenes,
+                                                            const BlockRef* __restrict__ map) {
+    const BlockRef br = map[blockIdx.x];
+    const SceneDev& S = scenes[br.scene];
+    const uint32_t tile = br.block;
+    __shared__ uint32_t h[kRadix];
+    for (uint32_t d = threadIdx.x; d < (uint32_t)kRadix; d += kSortThreads) h[d] = 0;
+    __syncthreads();
+    const uint32_t c = S.state->cur;
+    const uint32_t base = tile * kSortTile;
+#pragma unroll
+    for (int r = 0; r < kSortItems; r++) {
+        const uint32_t i = base + r * kSortThreads + threadIdx.x;
+        const bool valid = i < S.n;
+        uint32_t key = 0;
+        if (valid) {
+            const float4 p = S.pos[c][i];
+            key = compact_key(cell_of(p.x, S.search_radius), cell_of(p.y, S.search_radius),
+                              cell_of(p.z, S.search_radius), S.levels);
+            S.keys[0][i] = key;
+            S.vals[0][i] = i;
+        }
+        const uint32_t d = key & (kRadix - 1);
+        const uint64_t peers = match_digit(d, __ballot(valid));
+        if (valid && (peers & lanemask_lt()) == 0) atomicAdd(&h[d], (uint32_t)__popcll(peers));
+    }
+    __syncthreads();
+    for (uint32_t d = threadIdx.x; d < (uint32_t)kRadix; d += kSortThreads) S.hist[d * S.ntiles + tile] = h[d];
+}
 
 // one block per (scene, digit): exclusive scan of the digit's tile counts
 __global__ __launch_bounds__(kBlock) void k_radix_scan(const SceneDev* __restrict__ scenes,
@@ -1039,15 +1057,13 @@ __global__ __launch_bounds__(kBlock) void k_codec_finalize(const SceneDev* __res
 }
 
 // ---- launchers -------------------------------------------------------------------
-void launch_keys(const LaunchSet& L, hipStream_t st) {
-    if (L.nb_all) k_keys<<<L.nb_all, kBlock, 0, st>>>(L.scenes, L.map_all);
-}
-
-void launch_sort(const LaunchSet& L, uint32_t passes, hipStream_t st) {
+// keys, then the passes of the radix sort; the first pass's histogram comes with the keys
+void launch_keys_sort(const LaunchSet& L, uint32_t passes, hipStream_t st) {
+    if (L.nb_tiles) k_keys_hist<<<L.nb_tiles, kSortThreads, 0, st>>>(L.scenes, L.map_tiles);
     for (uint32_t p = 0; p < passes; p++) {
         const uint32_t src = p & 1u;
         const uint32_t shift = p * kRadixBits;
-        k_radix_hist<<<L.nb_tiles, kSortThreads, 0, st>>>(L.scenes, L.map_tiles, shift, src);
+        if (p > 0) k_radix_hist<<<L.nb_tiles, kSortThreads, 0, st>>>(L.scenes, L.map_tiles, shift, src);
         k_radix_scan<<<L.nb_scan, kBlock, 0, st>>>(L.scenes, L.map_scan);
         k_radix_scatter<<<L.nb_tiles, kSortThreads, 0, st>>>(L.scenes, L.map_tiles, shift, src);
     }
