@@ -209,6 +209,44 @@ def bench_served(ctx, window_s, rate, max_running, budget_gb, buckets=12):
     }
 
 
+def bench_obstacles(ctx, steps, warmup):
+    """The obstacle configuration: a 200^3 fluid block beside 40 seeded spheres
+    (radius U[5,20]) and 20 tori surface-sampled as boundary particles, inside a
+    3-layer box; compressed search with the NeighborStorage capacity.  SESPH
+    (k = 1000, the reference's linear equation of state) and IISPH run as two
+    scenes from the identical initial state, each timed over `steps` steps
+    after `warmup` steps."""
+    coll = scenes.config3_obstacles(seed=3)
+    counts = coll.counts()
+    nf, nb = counts["fluid"], counts["boundary"]
+    log(f"obstacles: {nf} fluid, {nb} boundary")
+    out = {
+        "workload": f"200^3 fluid block with 40 sphere and 20 torus obstacles, {nf} fluid + {nb} boundary, "
+                    "compressed search with NeighborStorage capacity",
+        "steps": steps,
+        "warmup": warmup,
+    }
+    for name, solver in (("sesph", abi.SOLVER_SESPH), ("iisph", abi.SOLVER_IISPH)):
+        d = abi.default_desc(3, solver, sesph_stiffness_k=1000.0)
+        d.neighbor_lists = abi.LIST_COMPRESSED
+        sc = Scene(ctx, d, coll.copy())
+        for _ in range(warmup):
+            sc.step(1)
+        ctx.synchronize()
+        iters = []
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            iters.append(sc.step(1).iterations)
+        ctx.synchronize()
+        dt = time.perf_counter() - t0
+        sc.close()
+        out[name] = {"ms_per_step": 1000.0 * dt / steps, "particle_updates_per_s": nf * steps / dt}
+        if solver == abi.SOLVER_IISPH:
+            out[name]["mean_jacobi_iterations"] = float(np.mean(iters))
+        log(f"  {name}: {1000.0 * dt / steps:.3f} ms/step, {nf * steps / dt:.3e} particle-updates/s")
+    return out
+
+
 def pmc_workload(args):
     """Run under rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE: one plain device copy of
     COPY_BYTES (read and written once each) calibrates the counters; then the
@@ -261,6 +299,8 @@ def main():
     ap.add_argument("--serve-slots", type=int, default=32)
     ap.add_argument("--serve-budget-gb", type=float, default=200.0)
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--obstacle-steps", type=int, default=10)
+    ap.add_argument("--no-obstacles", action="store_true")
     ap.add_argument("--graph", type=int, default=1, help="replay step heads from hipGraphs (1) or launch eagerly (0)")
     ap.add_argument("--pmc-workload", action="store_true",
                     help="workload for rocprofv3 --pmc: a device copy of 1 GiB (counter calibration), then 3 steps")
@@ -352,6 +392,11 @@ def main():
         served = bench_served(ctx, args.serve_seconds, args.serve_rate, args.serve_slots, args.serve_budget_gb)
         log(json.dumps(served))
 
+    obstacles = None
+    if not args.no_obstacles:
+        obstacles = bench_obstacles(ctx, args.obstacle_steps, 2)
+        log(json.dumps(obstacles))
+
     tuning = ctx.tuning()
     if tuning:
         log("autotuned neighbor-walk unroll:\n" + tuning.rstrip())
@@ -396,6 +441,7 @@ def main():
         "cpu_baseline": cpu,
         "batched": batched,
         "served": served,
+        "obstacles": obstacles,
         "graph": bool(args.graph),
         "autotune": [ln for ln in tuning.splitlines() if ln],
     }
