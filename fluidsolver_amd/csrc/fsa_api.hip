@@ -505,7 +505,9 @@ void read_states(fsa_context ctx, const std::vector<fsa_scene_s*>& v) {
 // elapsed times are collected when the pool fills or when they are read.
 void collect_timing(fsa_context ctx) {
     if (ctx->ev_used == 0) return;
-    HIP_CHECK(hipEventSynchronize(ctx->ev_pool[ctx->ev_used - 1]));
+    // a nested phase's pair follows its enclosing one's, so its stop event is
+    // not the last one recorded: wait for every stop event
+    for (size_t k = 1; k < ctx->ev_used; k += 2) HIP_CHECK(hipEventSynchronize(ctx->ev_pool[k]));
     for (size_t k = 0; k + 1 < ctx->ev_used; k += 2) {
         float ms = 0.0f;
         HIP_CHECK(hipEventElapsedTime(&ms, ctx->ev_pool[k], ctx->ev_pool[k + 1]));
@@ -515,19 +517,25 @@ void collect_timing(fsa_context ctx) {
     ctx->ev_used = 0;
 }
 
+// A phase takes its start/stop pair when it opens, so one phase may enclose
+// another (the list kernel inside the search).  Only an outermost phase may
+// collect the pool, and it leaves room for one nested pair.
 struct PhaseTimer {
     fsa_context ctx;
     int id;
-    PhaseTimer(fsa_context c, int i) : ctx(c), id(i) {
-        if (!ctx->timing) return;
-        if (ctx->ev_used + 2 > ctx->ev_pool.size()) collect_timing(ctx);
-        HIP_CHECK(hipEventRecord(ctx->ev_pool[ctx->ev_used], ctx->stream));
+    size_t slot = 0;
+    bool on;
+    PhaseTimer(fsa_context c, int i, bool nested = false) : ctx(c), id(i), on(c->timing) {
+        if (!on) return;
+        if (!nested && ctx->ev_used + 4 > ctx->ev_pool.size()) collect_timing(ctx);
+        slot = ctx->ev_used;
+        ctx->ev_used += 2;
+        ctx->ev_phase[slot / 2] = id;
+        HIP_CHECK(hipEventRecord(ctx->ev_pool[slot], ctx->stream));
     }
     ~PhaseTimer() noexcept(false) {
-        if (!ctx->timing) return;
-        HIP_CHECK(hipEventRecord(ctx->ev_pool[ctx->ev_used + 1], ctx->stream));
-        ctx->ev_phase[ctx->ev_used / 2] = id;
-        ctx->ev_used += 2;
+        if (!on) return;
+        HIP_CHECK(hipEventRecord(ctx->ev_pool[slot + 1], ctx->stream));
     }
 };
 
@@ -536,7 +544,11 @@ void search_set(fsa_context ctx, const LaunchSet& L, uint32_t passes) {
     PhaseTimer t(ctx, FSA_TIMER_SEARCH);
     launch_keys_sort(L, passes, ctx->stream);
     launch_reorder(L, passes & 1u, ctx->stream);
-    launch_lists(L, passes & 1u, ctx->stream);
+    {
+        PhaseTimer tl(ctx, FSA_TIMER_LISTS, true);
+        launch_lists(L, passes & 1u, ctx->stream);
+    }
+    if (L.codec) launch_codec(L, ctx->stream);
 }
 
 void simulate_set(fsa_context ctx, const LaunchSet& L, std::vector<fsa_scene_s*>& scenes, bool sesph,

@@ -1092,7 +1092,6 @@ void launch_lists(const LaunchSet& L, uint32_t sbuf, hipStream_t st) {
         else
             k_lists<false><<<L.nb_all, kBlock, 0, st>>>(L.scenes, L.map_all, sbuf);
     }
-    if (L.codec) launch_codec(L, st);
 }
 
 void launch3u_search(const LaunchSet& L, uint32_t passes, hipStream_t st) {
