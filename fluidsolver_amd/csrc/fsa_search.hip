@@ -514,6 +514,36 @@ __global__ __launch_bounds__(kBlock) void k_lists(const SceneDev* __restrict__ s
         }
         uint32_t total;
         const uint32_t roff = exclusive_scan32(rl, total);
+        // Up to kListChunks * 64 candidates (the usual case): each run writes its
+        // element indices to slots roff.. of the stage, each lane reads back the
+        // index of candidate k * 64 + lane, and the candidates are loaded once for
+        // every particle group of the cell.  More candidates take the binary
+        // search below, chunk by chunk and group by group.
+        const bool mapped = total <= kListChunks * 64u;
+        float4 pjm[kListChunks];
+        uint32_t ejm[kListChunks];
+        if (mapped) {
+            for (uint32_t t = 0; __any(t < rl); t++)
+                if (t < rl) stage[roff + t] = rs + t;
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+            for (uint32_t k = 0; k < kListChunks; k++) {
+                pjm[k] = make_float4(__builtin_inff(), __builtin_inff(), __builtin_inff(), 0.0f);
+                ejm[k] = 0;
+                const uint32_t g = k * 64 + lane;
+                if (g < total) {
+                    const uint32_t j = stage[g];
+                    pjm[k] = pos[j];
+                    ejm[k] = j | ((idt[j] >> kTypeShift) == 1u ? kBoundaryBit : 0u);
+                }
+            }
+            // the slots are read before the particle groups write their lists there
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        }
         for (uint32_t pbase = cs; pbase < ce; pbase += kStageParticles) {
             // the group's particles in lanes 0..kStageParticles-1 (the first group's were loaded ahead)
             const uint32_t pi = pbase + lane;
@@ -534,10 +564,12 @@ __global__ __launch_bounds__(kBlock) void k_lists(const SceneDev* __restrict__ s
             for (uint32_t c0 = 0; c0 < total; c0 += kListChunks * 64) {
                 // candidates c0 + 64k + lane; a lane past the last candidate sits
                 // at infinity and fails every test
-                float4 pj[kListChunks];
-                uint32_t ej[kListChunks];
+                // the mapped candidates were loaded above; otherwise this chunk's
+                float4* const pj = pjm;
+                uint32_t* const ej = ejm;
 #pragma unroll
                 for (uint32_t k = 0; k < kListChunks; k++) {
+                    if (mapped) break;
                     pj[k] = make_float4(__builtin_inff(), __builtin_inff(), __builtin_inff(), 0.0f);
                     ej[k] = 0;
                     if (c0 + k * 64 < total) {
