@@ -340,14 +340,21 @@ __global__ __launch_bounds__(kBlock) void k_compact(const SceneDev* __restrict__
 // particles, kStageParticles at a time, are taken in turn, each tested against
 // the held chunks with one ballot per chunk, whose prefix (on top of the
 // particle's running count, a scalar) gives each accepted candidate its slot in
-// the particle's row of an LDS stage.  The group's lists then leave the stage
-// row by row: one store covers 64 / kStageParticles list rows of the group's
+// the particle's row of an LDS stage.  Rows of consecutive cells gather in the
+// stage (the fluid ranks of a wave's cells are consecutive) and leave it row by
+// row: one store covers 64 / kStageParticles list rows of up to kStageParticles
 // consecutive fluid ranks, instead of one scattered store per particle and chunk.
+// With at most kListChunks * 64 candidates, a cell's candidates are mapped
+// through the stage rows from kMapRow on and loaded once for all its particles.
 // The next cell's table entries and first particle group are loaded while the
 // current cell is processed.
 
 constexpr uint32_t kListChunks = 4;
 constexpr uint32_t kStageParticles = 16;
+// first stage row of a cell's candidate map (kListChunks * 64 slots)
+constexpr uint32_t kMapRow = 13;
+static_assert(kMapRow * kMaxNeighbors + kListChunks * 64 <= kStageParticles * kMaxNeighbors,
+              "the candidate map fits the stage rows from kMapRow on");
 
 // one compare-exchange stage of a bitonic sort within 32-lane groups; the
 // partner comes from ds_swizzle's bit mode (and 0x1f, xor J), which needs no
@@ -414,9 +421,10 @@ __device__ __forceinline__ uint32_t mul_u24(uint32_t v, uint32_t s) {
 // so slots are addressed in 64 bits; otherwise a slot is one 24-bit multiply and
 // a 32-bit byte offset from the particle's list base.
 template <bool Wide>
-__global__ __launch_bounds__(kBlock) void k_lists(const SceneDev* __restrict__ scenes,
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) void k_lists(const SceneDev* __restrict__ scenes,
                                                   const BlockRef* __restrict__ map, uint32_t sbuf) {
     __shared__ uint32_t stage_all[kBlock / 64][kStageParticles * kMaxNeighbors];
+    __shared__ uint32_t stage_cnt_all[kBlock / 64][kStageParticles];
     const BlockRef br = map[blockIdx.x];
     const SceneDev& S = scenes[br.scene];
     const uint32_t wave = threadIdx.x / 64;
@@ -424,6 +432,7 @@ __global__ __launch_bounds__(kBlock) void k_lists(const SceneDev* __restrict__ s
     const uint32_t base = br.block * kBlock + wave * 64;
     if (base >= S.n) return;
     uint32_t* const stage = stage_all[wave];
+    uint32_t* const stage_cnt = stage_cnt_all[wave];
     const uint32_t c = S.state->cur;
     const auto pos = gload_ptr(S.pos[c]);
     const auto idt = gload_ptr(S.idt[c]);
@@ -446,6 +455,37 @@ __global__ __launch_bounds__(kBlock) void k_lists(const SceneDev* __restrict__ s
     // readout lanes: particle column lane % kStageParticles, row offset lane / kStageParticles
     const uint32_t col = lane % kStageParticles;
     const uint32_t row_step = 64u / kStageParticles;
+    // Stage row r holds the list of fluid rank fbase + r; rows of consecutive
+    // cells gather until the next group's ranks would pass the last row, then
+    // leave together: each store covers 64 / kStageParticles list rows of up to
+    // kStageParticles consecutive ranks.
+    uint32_t fbase = 0, fcount = 0, frows = 0;
+    auto flush = [&]() {
+        if (fcount == 0) return;
+        // the stage rows and counts written by other lanes are complete before they are read
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        const uint32_t ccnt = col < fcount ? stage_cnt[col] : 0u;
+        const uint32_t crank = fbase + col;
+        for (uint32_t e0 = 0; e0 < frows; e0 += row_step) {
+            const uint32_t e = e0 + lane / kStageParticles;
+            if (e < ccnt) {
+                const uint32_t v = stage[col * (uint32_t)kMaxNeighbors + e];
+                if (Wide) {
+                    nbr[(uint64_t)e * stride + crank] = v;
+                } else {
+                    nbr[mul_u24(e, stride) + crank] = v;
+                }
+            }
+        }
+        // every lane's reads of the stage are done before it is written again
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        fcount = 0;
+        frows = 0;
+    };
     // The loads a cell starts from: the table entries of its 27 neighbor cells
     // (lanes 0..26; lane 13 is the cell itself) and the first particle group's
     // type, position and rank.  They depend on the cell's key only, so the next
@@ -523,8 +563,11 @@ __global__ __launch_bounds__(kBlock) void k_lists(const SceneDev* __restrict__ s
         float4 pjm[kListChunks];
         uint32_t ejm[kListChunks];
         if (mapped) {
+            // the map takes the stage from row kMapRow on, which no gathered row may hold
+            if (fcount > kMapRow) flush();
+            uint32_t* const map_slots = stage + kMapRow * (uint32_t)kMaxNeighbors;
             for (uint32_t t = 0; __any(t < rl); t++)
-                if (t < rl) stage[roff + t] = rs + t;
+                if (t < rl) map_slots[roff + t] = rs + t;
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             __builtin_amdgcn_wave_barrier();
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -534,7 +577,7 @@ __global__ __launch_bounds__(kBlock) void k_lists(const SceneDev* __restrict__ s
                 ejm[k] = 0;
                 const uint32_t g = k * 64 + lane;
                 if (g < total) {
-                    const uint32_t j = stage[g];
+                    const uint32_t j = map_slots[g];
                     pjm[k] = pos[j];
                     ejm[k] = j | ((idt[j] >> kTypeShift) == 1u ? kBoundaryBit : 0u);
                 }
@@ -560,6 +603,11 @@ __global__ __launch_bounds__(kBlock) void k_lists(const SceneDev* __restrict__ s
             const bool isf = lane < kStageParticles && pi < ce && (ti >> kTypeShift) == 0u;
             const uint64_t fmask = __ballot(isf);
             if (fmask == 0) continue;
+            // the group's fluid ranks are consecutive: r0 .. r0 + nfl - 1
+            const uint32_t r0 = readlane_u(kl, (uint32_t)__ffsll((unsigned long long)fmask) - 1u);
+            const uint32_t nfl = (uint32_t)__popcll(fmask);
+            if (fcount != 0 && (r0 != fbase + fcount || r0 + nfl - fbase > kStageParticles)) flush();
+            if (fcount == 0) fbase = r0;
             uint32_t cntl = 0;
             for (uint32_t c0 = 0; c0 < total; c0 += kListChunks * 64) {
                 // candidates c0 + 64k + lane; a lane past the last candidate sits
@@ -599,7 +647,7 @@ __global__ __launch_bounds__(kBlock) void k_lists(const SceneDev* __restrict__ s
                     fm &= fm - 1;
                     const float xs = readlane_f(xl.x, b), ys = readlane_f(xl.y, b), zs = readlane_f(xl.z, b);
                     uint32_t cnt = readlane_u(cntl, b);  // the particle's running count (uniform)
-                    uint32_t* const srow = stage + b * (uint32_t)kMaxNeighbors;
+                    uint32_t* const srow = stage + (readlane_u(kl, b) - fbase) * (uint32_t)kMaxNeighbors;
 #pragma unroll
                     for (uint32_t k = 0; k < kListChunks; k++) {
                         if (k * 64 < left) {
@@ -624,37 +672,17 @@ __global__ __launch_bounds__(kBlock) void k_lists(const SceneDev* __restrict__ s
             if (isf) {
                 if (cntl > (uint32_t)kMaxNeighbors) overflow = true;
                 S.nbr_cnt[kl] = kept;
+                stage_cnt[kl - fbase] = kept;
             }
-            // longest list of the group (uniform)
-            uint32_t rows = 0;
+            // longest list of the gathered rows (uniform)
             for (uint64_t fm = fmask; fm; fm &= fm - 1)
-                rows = max(rows, readlane_u(kept, (uint32_t)__ffsll((unsigned long long)fm) - 1u));
-            // the stage rows written by other lanes are complete before they are read
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-            __builtin_amdgcn_wave_barrier();
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-            const uint32_t ccnt = (uint32_t)__shfl(kept, (int)col, 64);
-            const uint32_t crank = (uint32_t)__shfl(kl, (int)col, 64);
-            const bool cfluid = (fmask >> col) & 1u;
-            for (uint32_t e0 = 0; e0 < rows; e0 += row_step) {
-                const uint32_t e = e0 + lane / kStageParticles;
-                if (cfluid && e < ccnt) {
-                    const uint32_t v = stage[col * (uint32_t)kMaxNeighbors + e];
-                    if (Wide) {
-                        nbr[(uint64_t)e * stride + crank] = v;
-                    } else {
-                        nbr[mul_u24(e, stride) + crank] = v;
-                    }
-                }
-            }
-            // every lane's reads of the stage are done before the next group writes it
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-            __builtin_amdgcn_wave_barrier();
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                frows = max(frows, readlane_u(kept, (uint32_t)__ffsll((unsigned long long)fm) - 1u));
+            fcount = r0 + nfl - fbase;
         }
         if (!more) break;
         sl0 = nsl;
     }
+    flush();
     if (__any(overflow) && lane == 0) atomicOr(&S.state->status, kStatusCapacity);
 }
 
