@@ -427,7 +427,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) voi
     __shared__ uint32_t stage_cnt_all[kBlock / 64][kStageParticles];
     const BlockRef br = map[blockIdx.x];
     const SceneDev& S = scenes[br.scene];
-    const uint32_t wave = threadIdx.x / 64;
+    const uint32_t wave = (uint32_t)__builtin_amdgcn_readfirstlane(threadIdx.x / 64);
     const uint32_t lane = __lane_id();
     const uint32_t base = br.block * kBlock + wave * 64;
     if (base >= S.n) return;
