@@ -823,7 +823,7 @@ __global__ __launch_bounds__(kBlock) void k3u_lists(const SceneDev* __restrict__
     __shared__ uint32_t stage_all[kBlock / 64][kStageParticles * kMaxNeighbors];
     const BlockRef br = map[blockIdx.x];
     const SceneDev& S = scenes[br.scene];
-    const uint32_t wave = threadIdx.x / 64;
+    const uint32_t wave = (uint32_t)__builtin_amdgcn_readfirstlane(threadIdx.x / 64);
     const uint32_t lane = __lane_id();
     const uint32_t base = br.block * kBlock + wave * 64;
     if (base >= S.n) return;
