@@ -1176,7 +1176,36 @@ void step_head(fsa_context ctx, const LaunchSet& L, bool d2, bool unsorted, uint
     }
 }
 
-int32_t batch_step_group(fsa_context ctx, const std::vector<fsa_scene_s*>& v, std::vector<fsa_step_stats*>& out) {
+int32_t batch_step_group(fsa_context ctx, const std::vector<fsa_scene_s*>& all, std::vector<fsa_step_stats*>& all_out) {
+    // Per-scene set-up before any launch (the hashed re-sort, the cell grid of the
+    // coming search): a scene that fails here (its cells leave the key range, the
+    // cell table exceeds the budget) fails alone with its status in its stats,
+    // and the other scenes of the group take the step.
+    int32_t first_error = FSA_OK;
+    std::vector<fsa_scene_s*> v;
+    std::vector<fsa_step_stats*> out;
+    for (uint32_t q = 0; q < all.size(); q++) {
+        fsa_scene_s* s = all[q];
+        try {
+            hashed_resort_if_due(s);
+            prepare_grid(s);
+        } catch (const FsaError& e) {
+            s->searched = false;
+            s->lists_valid = false;
+            if (all_out[q]) {
+                fill_stats(s, all_out[q]);
+                all_out[q]->status = (uint32_t)e.code;
+            }
+            if (first_error == FSA_OK) {
+                first_error = e.code;
+                g_error = e.what();
+            }
+            continue;
+        }
+        v.push_back(s);
+        out.push_back(all_out[q]);
+    }
+    if (v.empty()) return first_error;
     BatchSet* b = batch_for(ctx, v);
     hipStream_t st = ctx->stream;
     const bool d2 = v[0]->hd.dim == 2;
@@ -1185,8 +1214,6 @@ int32_t batch_step_group(fsa_context ctx, const std::vector<fsa_scene_s*>& v, st
     uint32_t passes = 1, epasses = 0, cpasses = 1, max_it = 0, guess = 0;
     bool bpass = false;
     for (auto* s : v) {
-        hashed_resort_if_due(s);
-        prepare_grid(s);
         if (d2) {
             epasses = std::max(epasses, entry_passes(s->hd));
             cpasses = std::max(cpasses, cell_passes(s->hd));
@@ -1205,7 +1232,9 @@ int32_t batch_step_group(fsa_context ctx, const std::vector<fsa_scene_s*>& v, st
     upload_batch_table(ctx, b);
     if (ctx->graph && !ctx->timing) {
         const std::array<uint32_t, 5> key = {
-            d2 ? (epasses << 8 | cpasses) : passes, chunk, b->ls.spec, (bpass ? 1u : 0u) | (sesph ? 2u : 0u) | (unsorted ? 4u : 0u) | (b->ls.entities_before ? 8u : 0u),
+            d2 ? (epasses << 8 | cpasses) : passes, chunk, b->ls.spec,
+            (bpass ? 1u : 0u) | (sesph ? 2u : 0u) | (unsorted ? 4u : 0u) | (b->ls.entities_before ? 8u : 0u) |
+                (b->ls.codec ? 16u : 0u) | (b->ls.wide_lists ? 32u : 0u) | (b->ls.remover ? 64u : 0u),
             (uint32_t)b->ls.unroll[0] | (uint32_t)b->ls.unroll[1] << 8 | (uint32_t)b->ls.unroll[2] << 16 |
                 (uint32_t)b->ls.unroll[3] << 24};
         auto it = b->graphs.find(key);
@@ -1267,7 +1296,6 @@ int32_t batch_step_group(fsa_context ctx, const std::vector<fsa_scene_s*>& v, st
     }
     if (b->ls.remover) launch_remover_check(b->ls, st);
     read_states(ctx, v);
-    int32_t first_error = FSA_OK;
     for (uint32_t q = 0; q < v.size(); q++) {
         fsa_scene_s* s = v[q];
         try {
@@ -2287,6 +2315,11 @@ FSA_API int32_t fsa_scene_set_desc(fsa_scene s, const fsa_scene_desc* d) {
         if (!s) throw FsaError(FSA_E_INVALID, "scene is NULL");
         validate_desc(d);
         if (d->dimension != s->desc.dimension) throw FsaError(FSA_E_INVALID, "dimension cannot change");
+        // the search fixes the scene's buffer plan (3D sorted vs unsorted orders carry a
+        // grid-state array; the 2D searches keep their own grid state) and its order
+        // state, as the reference fixes it with the solver's template arguments
+        if (search_of(d) != search_of(&s->desc))
+            throw FsaError(FSA_E_INVALID, "the neighborhood search of a scene cannot change");
         s->desc = *d;
         const uint32_t stamp = s->hd.stamp;
         fill_params(s);
@@ -2825,8 +2858,10 @@ FSA_API int32_t fsa_server_collect(fsa_server srv, uint64_t id, const fsa_partic
         auto it = srv->jobs.find(id);
         if (it == srv->jobs.end()) throw FsaError(FSA_E_INVALID, "unknown job");
         Job* j = it->second;
-        if (j->state != FSA_JOB_DONE) throw FsaError(FSA_E_STATE, "job has not finished");
-        if (v) {
+        if (j->state == FSA_JOB_QUEUED || j->state == FSA_JOB_RUNNING)
+            throw FsaError(FSA_E_STATE, "job has not finished");
+        // a failed or cancelled job has no final state: it is forgotten without a copy
+        if (v && j->state == FSA_JOB_DONE) {
             if (v->count != j->count) throw FsaError(FSA_E_INVALID, "view count mismatch");
             for (uint64_t i = 0; i < j->count; i++) {
                 if (v->movement) std::memcpy((char*)v->movement + i * v->movement_stride, &j->movement[9 * i], 36);

@@ -278,6 +278,17 @@ __device__ __forceinline__ float div_pair_zs(float a, float b, float y) {
     return __builtin_fmaf(r0, y, q0);
 }
 
+// std::abs((int)e) as the reference's x86-64 build evaluates it (IISPHFluidSolver.hpp:400-402):
+// the float-to-int conversion (cvttss2si) truncates in range and gives INT_MIN for NaN
+// or |e| beyond the int range, and abs(INT_MIN) stays INT_MIN (the negation wraps).
+// The C conversion is undefined there and the device's saturates, so both cases
+// are spelled out.
+__device__ __forceinline__ float x86_int_abs(float e) {
+    if (!(e >= -2147483648.0f && e < 2147483648.0f)) return -2147483648.0f;
+    const int v = (int)e;
+    return v == INT_MIN ? -2147483648.0f : (float)(v < 0 ? -v : v);
+}
+
 // Compile-time specialisation of the two parameters that make operations
 // exact no-ops: Spec bit 0 = every scene of the launch has h == 1, bit 1 =
 // every scene has rest_density^2 == 1.  Without the bit the scene flag decides.

@@ -98,7 +98,25 @@ __global__ void k_mc_recip(int exp_lo, int exp_hi, unsigned long long* mism) {
     if (m) atomicAdd(mism, m);
 }
 
+// x86_int_abs of n values
+__global__ void k_mc_int_abs(const float* in, float* out, uint32_t n) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = x86_int_abs(in[i]);
+}
+
 }  // namespace
+
+// device x86_int_abs of host values in[0..n) into out (n <= 4096)
+extern "C" __attribute__((visibility("default"))) int fsa_mc_int_abs(const float* in, float* out, uint32_t n) {
+    if (n == 0 || n > 4096) return 3;
+    float* d = nullptr;
+    if (hipMalloc(&d, 8 * (size_t)n) != hipSuccess) return 1;
+    (void)hipMemcpy(d, in, 4 * (size_t)n, hipMemcpyHostToDevice);
+    k_mc_int_abs<<<(n + 255) / 256, 256>>>(d, d + n, n);
+    const hipError_t e = hipMemcpy(out, d + n, 4 * (size_t)n, hipMemcpyDeviceToHost);
+    (void)hipFree(d);
+    return e == hipSuccess ? 0 : 2;
+}
 
 extern "C" __attribute__((visibility("default"))) int fsa_mc_recip(int exp_lo, int exp_hi, uint64_t* mism) {
     if (exp_lo < -23 || exp_hi > 21 || exp_lo > exp_hi) return 3;

@@ -460,7 +460,7 @@ __global__ __launch_bounds__(kBlock) void k2_iisph_converge(const SceneDev* __re
         const float e = c != 0 ? (float)(tot / (double)c) : 0.0f;
         st->avg_err = (double)e;
         st->err_count = c;
-        const float ea = (S.flags & kFlagIntAbs) ? (float)abs((int)e) : fabsf(e);
+        const float ea = (S.flags & kFlagIntAbs) ? x86_int_abs(e) : fabsf(e);
         st->active = ((it < S.min_it) || (ea > S.max_err)) && (it < S.max_it) ? 1u : 0u;
     }
 }

@@ -50,6 +50,14 @@ class Server:
 
     def cancel(self, job_id):
         check(lib().fsa_server_cancel(self._h, int(job_id)))
+        self.release(job_id)
+
+    def release(self, job_id):
+        """Forgets a failed or cancelled job (it has no final state to collect)."""
+        info = self.job(job_id)
+        if info.state in (abi.JOB_FAILED, abi.JOB_CANCELLED):
+            check(lib().fsa_server_collect(self._h, int(job_id), None))
+            self._collections.pop(job_id, None)
 
     def tick(self):
         """Admit, advance every running scene one step, retire; returns the number of scenes stepped."""

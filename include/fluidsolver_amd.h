@@ -205,7 +205,10 @@ FSA_API int32_t fsa_scene_download(fsa_scene scene, const fsa_particle_view* vie
  * non-inactive particle into records + i * stride. */
 FSA_API int32_t fsa_scene_download_iisph(fsa_scene scene, void* records, uint64_t stride);
 /* Updates parameters and settings (DataChangeStruct semantics); the particle
- * counts and the dimension must not change. */
+ * counts, the dimension and the neighborhood search must not change
+ * (FSA_E_INVALID otherwise: the search is a template argument of the
+ * reference solver, IISPHFluidSolver3D.hpp:22-24, and fixes the scene's
+ * buffers). */
 FSA_API int32_t fsa_scene_set_desc(fsa_scene scene, const fsa_scene_desc* desc);
 
 /* execute_neighborhood_search() — CompressedNeighborhoodSearch::find_neighbors
@@ -316,8 +319,10 @@ FSA_API int32_t fsa_server_cancel(fsa_server srv, uint64_t job_id);
  * of scenes stepped (0 when nothing could run). */
 FSA_API int32_t fsa_server_tick(fsa_server srv, uint32_t* running);
 FSA_API int32_t fsa_server_job(fsa_server srv, uint64_t job_id, fsa_job_info* info);
-/* Writes the final state of a finished job into the caller's view (caller
- * order) and forgets the job. */
+/* Forgets a job in a terminal state.  For a DONE job, first writes its final
+ * state into the caller's view (caller order; view may be NULL); a FAILED or
+ * CANCELLED job has no final state and view is not written.  FSA_E_STATE for a
+ * queued or running job. */
 FSA_API int32_t fsa_server_collect(fsa_server srv, uint64_t job_id, const fsa_particle_view* view);
 FSA_API int32_t fsa_server_get_stats(fsa_server srv, fsa_server_stats* stats);
 

@@ -82,6 +82,11 @@ def lib():
         "orc_codec_get_first": (u64, [vp]),
         "orc_codec_used_bytes": (u64, [vp]),
         "orc_codec_encoded_bytes": (u64, [vp]),
+        "orc_iisph_prepare": (i32, [vp, f32, i32]),
+        "orc_iisph_apply": (i32, [vp, f32, vp, vp, vp]),
+        "orc_iisph_fields": (None, [vp, vp]),
+        "orc_x86_int_abs": (C.c_double, [C.c_double]),
+        "orc_set_mutation": (None, [i32]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -244,6 +249,33 @@ class OracleScene:
         lib().orc_codec_sizes(self._h, cnt.ctypes.data, byt.ctypes.data)
         return cnt, byt
 
+    # ---- IISPH algebra, one phase at a time (solver tests) ----
+    def iisph_prepare(self, dt, solve=False):
+        """IISPH phases 1 and 2 at dt over the last search; with solve, the Jacobi loop
+        too (no CFL correction, no integration).  Returns the Jacobi iteration count."""
+        rc = lib().orc_iisph_prepare(self._h, float(dt), 1 if solve else 0)
+        if rc < 0:
+            _check(-rc)
+        return rc
+
+    def iisph_apply(self, dt, pressures):
+        """(A p, pass-A accelerations) for pressures p (caller order): the operator of the
+        Jacobi passes, at the state of the last iisph_prepare."""
+        p = np.ascontiguousarray(pressures, dtype=np.float64)
+        assert p.shape == (self.n,)
+        ap = np.zeros(self.n, dtype=np.float64)
+        acc = np.zeros((self.n, self.dimension), dtype=np.float64)
+        _check(lib().orc_iisph_apply(self._h, float(dt), p.ctypes.data, ap.ctypes.data, acc.ctypes.data))
+        return ap, acc
+
+    def iisph_fields(self):
+        """Per caller row: dict of density, source, diag, pressure, vstar (n x D), anp (n x D; zero in 2D)."""
+        D = self.dimension
+        out = np.zeros((self.n, 4 + 2 * D), dtype=np.float64)
+        lib().orc_iisph_fields(self._h, out.ctypes.data)
+        return {"density": out[:, 0], "source": out[:, 1], "diag": out[:, 2], "pressure": out[:, 3],
+                "vstar": out[:, 4:4 + D], "anp": out[:, 4 + D:4 + 2 * D]}
+
     def codec_stats(self):
         t = C.c_uint64(0)
         b = C.c_uint64(0)
@@ -299,6 +331,17 @@ def stable_sort(keys):
     perm = np.zeros(keys.shape[0], dtype=np.uint64)
     lib().orc_stable_sort_u64(keys.ctypes.data, keys.shape[0], perm.ctypes.data)
     return perm
+
+
+def set_mutation(m):
+    """Plants mistake m in the oracle's solver (0 = none; see g_mutation in sph_oracle.cpp).
+    Only the tests that show a check detects a planted mistake call this."""
+    lib().orc_set_mutation(int(m))
+
+
+def x86_int_abs(e):
+    """std::abs((int)e) as x86-64 evaluates it (INT_MIN for NaN and out-of-range values)."""
+    return float(lib().orc_x86_int_abs(float(e)))
 
 
 def cfl_timestep(max_a, max_v, particle_size=1.0, timestep=0.001, lambda_a=0.25, lambda_v=0.4):

@@ -47,6 +47,14 @@ namespace orc {
 
 static thread_local std::string g_err;
 static int g_threads = 0;
+// Planted mistakes for the tests that show a solver check detects them (0 = none):
+// 1 boundary factor 2 dropped from d_ii, 2 gradW_ij for gradW_ji in a_ii's second
+// term, 3 boundary factor 2 dropped from pass A, 4 the 3D boundary source term with
+// v*_i - v*_b (2D: with v*_i alone), 5 the Jacobi step's sign flipped, 6 pass A's
+// fluid term with p_i for p_j, 7 SESPH's fluid pressure term with p_i/rho_i^2 for
+// p_j/rho_j^2, 8 SESPH's pressure without the clamp at 0.  Set only by tests
+// (orc_set_mutation).
+static int g_mutation = 0;
 
 static int thread_count() {
     if (g_threads > 0) return g_threads;
@@ -346,6 +354,19 @@ static std::vector<size_t> quick_sort_perm(const std::vector<uint64_t>& keys_in)
 }
 enum { T_NORMAL = 0, T_BOUNDARY = 1, T_INACTIVE = 2 };
 
+// std::abs((int)e) as the reference's x86-64 build evaluates it
+// (IISPHFluidSolver.hpp:400-402, where the float error meets the integer abs):
+// cvttss2si truncates toward zero in range and gives INT_MIN for NaN or a value
+// beyond the int range, and abs(INT_MIN) stays INT_MIN (the negation wraps).
+// C leaves both cases undefined, so they are spelled out.  The float error is
+// exact in double, so one double form serves the FP32 and FP64 modes.
+static inline double x86_int_abs(double e) {
+    if (!(e > -2147483649.0 && e < 2147483648.0)) return -2147483648.0;
+    const long long v = (long long)e;  // truncation toward zero, in int range
+    if (v == -2147483648LL) return -2147483648.0;
+    return (double)(v < 0 ? -v : v);
+}
+
 struct SceneBase {
     virtual ~SceneBase() {}
     virtual void translate(const float* delta) = 0;
@@ -366,6 +387,12 @@ struct SceneBase {
                                 uint64_t* nnz) = 0;
     virtual int sensor_plane(const fsa_sensor_plane* p, float* values) = 0;
     virtual int evaluate_volume(const float* x, uint64_t count, fsa_volume_sample* out) = 0;
+    // IISPH phases one at a time (tests of the solver's algebra): phases 1 and 2 at dt
+    // over the last search; the operator A p of the Jacobi passes for pressures p;
+    // the per-particle fields of phases 1 and 2.  Arrays are in caller order.
+    virtual int iisph_prepare(float dt, int solve) = 0;
+    virtual int iisph_apply(float dt, const double* p, double* ap, double* acc_out) = 0;
+    virtual void iisph_fields(double* out) const = 0;
 };
 
 template <class T, int D>
@@ -915,14 +942,12 @@ struct Scene : SceneBase {
         return (T(2) * nu) * tmp;
     }
 
-    // ---- IISPHFluidSolver3D::execute_simulation_step (IISPHFluidSolver3D.hpp:120-562)
-    int simulate_iisph3d(float desired, fsa_step_stats* st) {
-        T dt = (T)desired;
-        const T rho0 = (T)desc.rest_density;
-        const T rho0sq = rho0 * rho0;
+    // ---- IISPHFluidSolver3D::execute_simulation_step (IISPHFluidSolver3D.hpp:120-562),
+    // in its phases: phase 1 (:141-212), phase 2 (:216-339), the Jacobi passes
+    // (:348-461) and the loop control, CFL and integration (:463-561)
+    void iisph3d_phase1(T dt) {
         const bool sl = desc.iisph_single_layer_boundary != 0;
-        const T gamma = (T)desc.iisph_gamma, g1 = (T)desc.iisph_single_layer_gamma_1;
-        const T eps = Consts<T>::eps();
+        const T g1 = (T)desc.iisph_single_layer_gamma_1;
         density_old = density;
         loop_for(n, [&](size_t i) {
             uint8_t t = type[i];
@@ -947,6 +972,12 @@ struct Scene : SceneBase {
             });
             density[i] = rho;
         });
+    }
+    void iisph3d_phase2(T dt) {
+        const T rho0 = (T)desc.rest_density;
+        const T rho0sq = rho0 * rho0;
+        const bool sl = desc.iisph_single_layer_boundary != 0;
+        const T gamma = (T)desc.iisph_gamma;
         loop_for(n, [&](size_t i) {
             if (type[i] != T_NORMAL) return;
             const V& xi = pos[i];
@@ -956,7 +987,8 @@ struct Scene : SceneBase {
                 if (tj == T_NORMAL)
                     contrib = contrib + mass[j] * dot(vstar[i] - vstar[j], kernel.reversed(pos[j], xi));
                 else if (tj == T_BOUNDARY)
-                    contrib = contrib + mass[j] * dot(vstar[i], kernel.reversed(pos[j], xi));
+                    contrib = contrib + mass[j] * dot(g_mutation == 4 ? vstar[i] - vstar[j] : vstar[i],
+                                                      kernel.reversed(pos[j], xi));
             });
             source[i] = rho0 - density[i] - dt * contrib;
             V inner = vzero<T, D>();
@@ -968,7 +1000,7 @@ struct Scene : SceneBase {
                     if (sl)
                         inner = inner - (T(2) * gamma * mass[j] / rho0sq) * kernel.reversed(pos[j], xi);
                     else
-                        inner = inner - (T(2) * mass[j] / rho0sq) * kernel.reversed(pos[j], xi);
+                        inner = inner - ((g_mutation == 1 ? T(1) : T(2)) * mass[j] / rho0sq) * kernel.reversed(pos[j], xi);
                 }
             });
             T dg = T(0);
@@ -976,7 +1008,8 @@ struct Scene : SceneBase {
                 uint8_t tj = type[j];
                 if (tj == T_NORMAL) {
                     dg = dg + mass[j] * dot(inner, kernel.reversed(pos[j], xi));
-                    dg = dg + mass[j] * dot((mass[i] / rho0sq) * kernel.reversed(xi, pos[j]),
+                    dg = dg + mass[j] * dot((mass[i] / rho0sq) * (g_mutation == 2 ? kernel.reversed(pos[j], xi)
+                                                                                  : kernel.reversed(xi, pos[j])),
                                             kernel.reversed(pos[j], xi));
                 } else if (tj == T_BOUNDARY) {
                     dg = dg + mass[j] * dot(inner, kernel.reversed(pos[j], xi));
@@ -985,42 +1018,59 @@ struct Scene : SceneBase {
             dg = dg * (dt * dt);
             diag[i] = dg;
         });
-
-        T max_v2 = 0, max_a2 = 0;
-        size_t iterations = 0;
-        double avg_out = 0.0;
-        for (size_t it = 0; it < desc.iisph_max_iterations; it++) {
-            loop_for(n, [&](size_t i) {
-                if (type[i] != T_NORMAL) return;
-                const V& xi = pos[i];
-                V pa = vzero<T, D>();
-                for_neighbors(i, [&](size_t j) {
-                    uint8_t tj = type[j];
-                    if (tj == T_NORMAL)
-                        pa = pa - (mass[j] * (pressure[i] / rho0sq + pressure[j] / rho0sq)) * kernel.reversed(pos[j], xi);
-                    else if (tj == T_BOUNDARY) {
-                        if (sl)
-                            pa = pa - (gamma * mass[j] * T(2) * pressure[i] / rho0sq) * kernel.reversed(pos[j], xi);
-                        else
-                            pa = pa - (mass[j] * T(2) * pressure[i] / rho0sq) * kernel.reversed(pos[j], xi);
-                    }
-                });
-                acc[i] = pa;
+    }
+    // Jacobi pass A (:348-397): pressure accelerations from the current pressures
+    void iisph3d_pass_a() {
+        const T rho0 = (T)desc.rest_density;
+        const T rho0sq = rho0 * rho0;
+        const bool sl = desc.iisph_single_layer_boundary != 0;
+        const T gamma = (T)desc.iisph_gamma;
+        loop_for(n, [&](size_t i) {
+            if (type[i] != T_NORMAL) return;
+            const V& xi = pos[i];
+            V pa = vzero<T, D>();
+            for_neighbors(i, [&](size_t j) {
+                uint8_t tj = type[j];
+                if (tj == T_NORMAL)
+                    pa = pa - (mass[j] * (pressure[i] / rho0sq + pressure[g_mutation == 6 ? i : j] / rho0sq)) *
+                                  kernel.reversed(pos[j], xi);
+                else if (tj == T_BOUNDARY) {
+                    if (sl)
+                        pa = pa - (gamma * mass[j] * T(2) * pressure[i] / rho0sq) * kernel.reversed(pos[j], xi);
+                    else
+                        pa = pa - (mass[j] * (g_mutation == 3 ? T(1) : T(2)) * pressure[i] / rho0sq) *
+                                      kernel.reversed(pos[j], xi);
+                }
             });
+            acc[i] = pa;
+        });
+    }
+    // Jacobi pass B's product (:402-440): (A p)_i = dt^2 sum_j m_j (a_i - a_j) . gradW
+    T iisph3d_ap(size_t i, T dt) const {
+        const V& xi = pos[i];
+        T ap = T(0);
+        for_neighbors(i, [&](size_t j) {
+            uint8_t tj = type[j];
+            if (tj == T_NORMAL)
+                ap = ap + mass[j] * dot(acc[i] - acc[j], kernel.reversed(pos[j], xi));
+            else if (tj == T_BOUNDARY)
+                ap = ap + mass[j] * dot(acc[i], kernel.reversed(pos[j], xi));
+        });
+        return ap * (dt * dt);
+    }
+
+    // the Jacobi loop with its convergence test (:346-522); returns the iteration count
+    size_t iisph3d_solve(T dt, T& max_v2, T& max_a2, double& avg_out) {
+        const T eps = Consts<T>::eps();
+        size_t iterations = 0;
+        for (size_t it = 0; it < desc.iisph_max_iterations; it++) {
+            iisph3d_pass_a();
             loop_for(n, [&](size_t i) {
                 if (type[i] != T_NORMAL) return;
-                const V& xi = pos[i];
-                T ap = T(0);
-                for_neighbors(i, [&](size_t j) {
-                    uint8_t tj = type[j];
-                    if (tj == T_NORMAL)
-                        ap = ap + mass[j] * dot(acc[i] - acc[j], kernel.reversed(pos[j], xi));
-                    else if (tj == T_BOUNDARY)
-                        ap = ap + mass[j] * dot(acc[i], kernel.reversed(pos[j], xi));
-                });
-                ap = ap * (dt * dt);
+                const T ap = iisph3d_ap(i, dt);
                 if (std::abs(diag[i]) > eps)
-                    pressure[i] = std::fmax(T(0), pressure[i] + (T)desc.iisph_omega * ((source[i] - ap) / diag[i]));
+                    pressure[i] = std::fmax(T(0), pressure[i] + (g_mutation == 5 ? T(-1) : T(1)) * (T)desc.iisph_omega *
+                                                                   ((source[i] - ap) / diag[i]));
                 else
                     pressure[i] = T(0);
                 T e = std::abs(source[i] - ap);
@@ -1063,6 +1113,16 @@ struct Scene : SceneBase {
                 if (done) break;
             }
         }
+        return iterations;
+    }
+
+    int simulate_iisph3d(float desired, fsa_step_stats* st) {
+        T dt = (T)desired;
+        iisph3d_phase1(dt);
+        iisph3d_phase2(dt);
+        T max_v2 = 0, max_a2 = 0;
+        double avg_out = 0.0;
+        const size_t iterations = iisph3d_solve(dt, max_v2, max_a2, avg_out);
         T old_dt = dt;
         float max_v = (float)std::sqrt((double)max_v2), max_a = (float)std::sqrt((double)max_a2);
         float corrected = std::fmin(cfl_timestep(max_a, max_v), desired);
@@ -1106,7 +1166,7 @@ struct Scene : SceneBase {
             });
             density[i] = rho;
             T p = k * (rho / rho0 - T(1));
-            pressure[i] = std::max(p, T(0));
+            pressure[i] = g_mutation == 8 ? p : std::max(p, T(0));
         });
         loop_for(n, [&](size_t i) {
             if (type[i] != T_NORMAL) return;
@@ -1128,7 +1188,7 @@ struct Scene : SceneBase {
                 } else {
                     T nd = density[j], np = pressure[j];
                     T npdd = nd == T(0) ? T(0) : np / (nd * nd);
-                    pa = pa + (-mass[j] * (pdd + npdd)) * kernel.reversed(pos[j], xi);
+                    pa = pa + (-mass[j] * (pdd + (g_mutation == 7 ? pdd : npdd))) * kernel.reversed(pos[j], xi);
                 }
             });
             acc[i] = pa + npa;
@@ -1165,12 +1225,9 @@ struct Scene : SceneBase {
         return FSA_OK;
     }
 
-    // ---- IISPHFluidSolver (2D) (IISPHFluidSolver.hpp:139-180, 196-485)
-    int simulate_iisph2d(float desired, fsa_step_stats* st) {
-        T dt = (T)desired;
-        const T rho0 = (T)desc.rest_density;
-        const T gamma = (T)desc.iisph_gamma;
-        const T eps = Consts<T>::eps();
+    // ---- IISPHFluidSolver (2D) (IISPHFluidSolver.hpp:139-180, 196-485), in its
+    // phases as the 3D solver above
+    void iisph2d_phase1(T dt) {
         density_old = density;
         loop_for(n, [&](size_t i) {
             uint8_t t = type[i];
@@ -1191,13 +1248,18 @@ struct Scene : SceneBase {
             vstar[i] = vel[i] + dt * npa;
             ext[i] = vzero<T, D>();
         });
+    }
+    void iisph2d_phase2(T dt) {
+        const T rho0 = (T)desc.rest_density;
+        const T gamma = (T)desc.iisph_gamma;
         loop_for(n, [&](size_t i) {
             if (type[i] != T_NORMAL) return;
             const V& xi = pos[i];
             T sum = T(0);  // ComputeSourceTerm
             for_neighbors(i, [&](size_t j) {
                 if (type[j] == T_INACTIVE) return;
-                sum = sum + mass[j] * dot(vstar[i] - vstar[j], kernel.reversed(pos[j], xi));
+                sum = sum + mass[j] * dot(g_mutation == 4 && type[j] == T_BOUNDARY ? vstar[i] : vstar[i] - vstar[j],
+                                          kernel.reversed(pos[j], xi));
             });
             source[i] = rho0 - density[i] - dt * sum;
             // ComputeDiagonalElement: the inner sum does not depend on the outer
@@ -1208,7 +1270,8 @@ struct Scene : SceneBase {
                 if (tk == T_NORMAL)
                     internal = internal + (mass[k] / rho0 / rho0) * kernel.reversed(pos[k], xi);
                 else if (tk == T_BOUNDARY)
-                    internal = internal + (T(2) * gamma * mass[k] / rho0 / rho0) * kernel.reversed(pos[k], xi);
+                    internal = internal + ((g_mutation == 1 ? T(1) : T(2)) * gamma * mass[k] / rho0 / rho0) *
+                                              kernel.reversed(pos[k], xi);
             });
             T s = T(0);
             for_neighbors(i, [&](size_t j) {
@@ -1217,55 +1280,74 @@ struct Scene : SceneBase {
             });
             for_neighbors(i, [&](size_t j) {
                 if (type[j] != T_NORMAL) return;
-                s = s + mass[j] * dot((mass[i] / rho0 / rho0) * kernel.reversed(xi, pos[j]), kernel.reversed(pos[j], xi));
+                s = s + mass[j] * dot((mass[i] / rho0 / rho0) * (g_mutation == 2 ? kernel.reversed(pos[j], xi)
+                                                                                 : kernel.reversed(xi, pos[j])),
+                                      kernel.reversed(pos[j], xi));
             });
             diag[i] = dt * dt * s;
             pressure[i] = T(0);  // InitializePressure
         });
-        // ComputePressure (:396-485)
+    }
+    // ComputePressure's first loop (:408-440): pressure accelerations
+    void iisph2d_pass_a() {
+        const T rho0 = (T)desc.rest_density;
+        const T gamma = (T)desc.iisph_gamma;
+        loop_for(n, [&](size_t i) {
+            if (type[i] != T_NORMAL) return;
+            const V& xi = pos[i];
+            T pi = pressure[i];
+            V sum = vzero<T, D>();
+            for_neighbors(i, [&](size_t j) {
+                uint8_t tj = type[j];
+                if (tj == T_NORMAL)
+                    sum = sum + (mass[j] * (pi / rho0 / rho0 + pressure[g_mutation == 6 ? i : j] / rho0 / rho0)) *
+                                    kernel.reversed(pos[j], xi);
+                else if (tj == T_BOUNDARY)
+                    sum = sum + (gamma * mass[j] * (g_mutation == 3 ? T(1) : T(2)) * (pi / rho0 / rho0)) *
+                                    kernel.reversed(pos[j], xi);
+            });
+            acc[i] = -sum;
+        });
+    }
+    // ComputePressure's second loop, the product (:443-466): dt^2 sum_j m_j (a_i - a_j) . gradW
+    T iisph2d_ap(size_t i, T dt) const {
+        const V& xi = pos[i];
+        const V ai = acc[i];
+        T sum = T(0);
+        for_neighbors(i, [&](size_t j) {
+            uint8_t tj = type[j];
+            if (tj == T_NORMAL)
+                sum = sum + mass[j] * dot(ai - acc[j], kernel.reversed(pos[j], xi));
+            else if (tj == T_BOUNDARY)
+                sum = sum + mass[j] * dot(ai, kernel.reversed(pos[j], xi));
+        });
+        return dt * dt * sum;
+    }
+
+    // ComputePressure (:396-485): returns the iteration count, err the last average error
+    size_t iisph2d_solve(T dt, T& err) {
+        const T eps = Consts<T>::eps();
         size_t iteration = 0;
-        T err = T(0);
+        err = T(0);
         const bool int_abs = (desc.flags & FSA_FLAG_EMULATE_INT_ABS) != 0;
         auto err_abs = [&](T e) -> T {
-            if (int_abs) return (T)std::abs((int)e);
+            if (int_abs) return (T)x86_int_abs((double)e);
             return std::abs(e);
         };
         while ((iteration < desc.iisph_min_iterations || err_abs(err) > (T)desc.iisph_max_density_error_allowed) &&
                iteration < desc.iisph_max_iterations) {
             err = T(0);
             size_t counter = 0;
-            loop_for(n, [&](size_t i) {
-                if (type[i] != T_NORMAL) return;
-                const V& xi = pos[i];
-                T pi = pressure[i];
-                V sum = vzero<T, D>();
-                for_neighbors(i, [&](size_t j) {
-                    uint8_t tj = type[j];
-                    if (tj == T_NORMAL)
-                        sum = sum + (mass[j] * (pi / rho0 / rho0 + pressure[j] / rho0 / rho0)) * kernel.reversed(pos[j], xi);
-                    else if (tj == T_BOUNDARY)
-                        sum = sum + (gamma * mass[j] * T(2) * (pi / rho0 / rho0)) * kernel.reversed(pos[j], xi);
-                });
-                acc[i] = -sum;
-            });
+            iisph2d_pass_a();
             std::vector<T> e_i(n, T(0));
             std::vector<uint8_t> counted(n, 0);
             loop_for(n, [&](size_t i) {
                 if (type[i] != T_NORMAL) return;
-                const V& xi = pos[i];
-                const V ai = acc[i];
-                T sum = T(0);
-                for_neighbors(i, [&](size_t j) {
-                    uint8_t tj = type[j];
-                    if (tj == T_NORMAL)
-                        sum = sum + mass[j] * dot(ai - acc[j], kernel.reversed(pos[j], xi));
-                    else if (tj == T_BOUNDARY)
-                        sum = sum + mass[j] * dot(ai, kernel.reversed(pos[j], xi));
-                });
-                T Ap = dt * dt * sum;
+                T Ap = iisph2d_ap(i, dt);
                 T dgi = diag[i], si = source[i];
                 if (std::abs(dgi) > eps)
-                    pressure[i] = std::fmax(T(0), pressure[i] + (T)desc.iisph_omega * ((si - Ap) / dgi));
+                    pressure[i] = std::fmax(T(0), pressure[i] + (g_mutation == 5 ? T(-1) : T(1)) * (T)desc.iisph_omega *
+                                                                   ((si - Ap) / dgi));
                 else
                     pressure[i] = T(0);
                 T e = std::abs(si - Ap);
@@ -1292,6 +1374,15 @@ struct Scene : SceneBase {
             if (fp64_reduction) err = (T)err64;
             iteration++;
         }
+        return iteration;
+    }
+
+    int simulate_iisph2d(float desired, fsa_step_stats* st) {
+        T dt = (T)desired;
+        iisph2d_phase1(dt);
+        iisph2d_phase2(dt);
+        T err = T(0);
+        const size_t iteration = iisph2d_solve(dt, err);
         loop_for(n, [&](size_t i) {
             if (type[i] == T_INACTIVE) return;
             vel[i] = vstar[i] + dt * acc[i];
@@ -1434,6 +1525,68 @@ struct Scene : SceneBase {
         else rc = simulate_iisph2d(dt, st);
         if (rc == FSA_OK) run_entities(false);
         return rc;
+    }
+
+    bool iisph_ok() const {
+        if (desc.solver != FSA_SOLVER_IISPH) {
+            g_err = "not an IISPH scene";
+            return false;
+        }
+        if (!searched) {
+            g_err = "find_neighbors first";
+            return false;
+        }
+        return true;
+    }
+    // phases 1 and 2 at dt; with solve, the Jacobi loop too (pressures stay with the
+    // particles; no CFL correction, no integration); returns the iteration count
+    int iisph_prepare(float dt, int solve) override {
+        if (!iisph_ok()) return -FSA_E_STATE;
+        size_t it = 0;
+        if (D == 3) {
+            iisph3d_phase1((T)dt);
+            iisph3d_phase2((T)dt);
+            T v2 = 0, a2 = 0;
+            double avg = 0.0;
+            if (solve) it = iisph3d_solve((T)dt, v2, a2, avg);
+        } else {
+            iisph2d_phase1((T)dt);
+            iisph2d_phase2((T)dt);
+            T err = T(0);
+            if (solve) it = iisph2d_solve((T)dt, err);
+        }
+        return (int)it;
+    }
+    // pressure p of caller row id[k] -> A p at that row (0 for non-fluid rows);
+    // acc_out (D per row, may be NULL) receives pass A's accelerations
+    int iisph_apply(float dt, const double* p, double* ap, double* acc_out) override {
+        if (!iisph_ok()) return FSA_E_STATE;
+        for (size_t k = 0; k < n; k++) pressure[k] = (T)p[id[k]];
+        if (D == 3)
+            iisph3d_pass_a();
+        else
+            iisph2d_pass_a();
+        for (size_t k = 0; k < n; k++) {
+            ap[id[k]] = type[k] == T_NORMAL ? (double)(D == 3 ? iisph3d_ap(k, (T)dt) : iisph2d_ap(k, (T)dt)) : 0.0;
+            if (acc_out)
+                for (int c = 0; c < D; c++) acc_out[id[k] * D + c] = type[k] == T_NORMAL ? (double)acc[k].v[c] : 0.0;
+        }
+        return FSA_OK;
+    }
+    // per caller row: density, source term, diagonal element, pressure, v* (D), a_np (D; 0 in 2D)
+    void iisph_fields(double* out) const override {
+        const size_t w = 4 + 2 * D;
+        for (size_t k = 0; k < n; k++) {
+            double* r = out + id[k] * w;
+            r[0] = (double)density[k];
+            r[1] = (double)source[k];
+            r[2] = (double)diag[k];
+            r[3] = (double)pressure[k];
+            for (int c = 0; c < D; c++) {
+                r[4 + c] = (double)vstar[k].v[c];
+                r[4 + D + c] = D == 3 ? (double)anp[k].v[c] : 0.0;
+            }
+        }
     }
 
     int download(const fsa_particle_view* v) override {
@@ -1642,6 +1795,13 @@ void* orc_create(const fsa_scene_desc* d, int search_kind, int mode, const fsa_p
 void orc_destroy(void* s) { delete (SceneBase*)s; }
 int orc_find_neighbors(void* s) { return ((SceneBase*)s)->find_neighbors(); }
 int orc_simulate(void* s, float dt, fsa_step_stats* st) { return ((SceneBase*)s)->simulate(dt, st); }
+int orc_iisph_prepare(void* s, float dt, int solve) { return ((SceneBase*)s)->iisph_prepare(dt, solve); }
+int orc_iisph_apply(void* s, float dt, const double* p, double* ap, double* acc) {
+    return ((SceneBase*)s)->iisph_apply(dt, p, ap, acc);
+}
+void orc_iisph_fields(void* s, double* out) { ((SceneBase*)s)->iisph_fields(out); }
+double orc_x86_int_abs(double e) { return x86_int_abs(e); }
+void orc_set_mutation(int m) { g_mutation = m; }
 float orc_next_timestep(void* s) { return ((SceneBase*)s)->next_timestep(); }
 void orc_set_entities(void* s, const fsa_entity* e, uint32_t count) { ((SceneBase*)s)->set_entities(e, count); }
 void orc_statistics(void* s, float zero_height, fsa_scene_statistics* out) { ((SceneBase*)s)->get_statistics(zero_height, out); }

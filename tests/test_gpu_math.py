@@ -31,6 +31,8 @@ def _lib():
     L.fsa_mc_recip.argtypes = [C.c_int, C.c_int, C.POINTER(C.c_uint64)]
     L.fsa_mc_div.restype = C.c_int
     L.fsa_mc_div.argtypes = [C.c_int, C.c_int, C.c_uint32, C.c_uint64, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]
+    L.fsa_mc_int_abs.restype = C.c_int
+    L.fsa_mc_int_abs.argtypes = [C.c_void_p, C.c_void_p, C.c_uint32]
     return L
 
 
@@ -57,3 +59,19 @@ def test_recip_pair_correctly_rounded():
     print("case recip mantissas", flush=True)
     assert L.fsa_mc_recip(-23, 1, C.byref(m)) == 0
     assert m.value == 0
+
+
+def test_x86_int_abs_on_the_device(oracle_lib):
+    """The 2D loop condition's integer abs() (IISPHFluidSolver.hpp:400-402) on the device
+    equals the oracle's x86 form, NaN and out-of-range values (INT_MIN) included."""
+    import numpy as np
+
+    L = _lib()
+    vals = np.array([0.0, 0.7, -0.7, 1.0, -2.5, 1e-3, 3.99, 2147483520.0, -2147483648.0, 2147483648.0, 3.0e9,
+                     -3.0e9, float("nan"), float("inf"), float("-inf"), 1e30, -1e30, 65536.5, -123456.75],
+                    dtype=np.float32)
+    out = np.zeros_like(vals)
+    print("case x86 int abs", flush=True)
+    assert L.fsa_mc_int_abs(vals.ctypes.data, out.ctypes.data, len(vals)) == 0
+    ref = np.array([oracle_lib.x86_int_abs(float(v)) for v in vals], dtype=np.float32)
+    assert np.array_equal(out, ref), (vals, out, ref)

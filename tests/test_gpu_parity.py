@@ -346,3 +346,32 @@ def test_tiny_coordinates_take_the_ieee_path(gpu_context, oracle_lib):
     sc = Scene(gpu_context, desc, coll.copy())
     orc = O.OracleScene(desc, coll.copy(), mode=O.MODE_FP32_R64)
     _compare_steps(sc, orc, coll, 4, "tiny")
+
+
+def test_set_desc_refuses_a_search_change(gpu_context):
+    """The search fixes a scene's buffers (the hashed and quadratic orders keep grid
+    state a compressed scene has none of): set_desc refuses to change it on the host,
+    and the scene then steps as before."""
+    from fluidsolver_amd.solver import FsaError
+
+    c = scenes.dam_scene((5, 5, 5), (15, 10, 10), 3, seed=31)
+    d = abi.default_desc(3, abi.SOLVER_IISPH)
+    print("case set_desc search change", flush=True)
+    sc = Scene(gpu_context, d, c.copy())
+    for other in (abi.SEARCH_HASHED_3D, abi.SEARCH_QUADRATIC_3D):
+        bad = abi.default_desc(3, abi.SOLVER_IISPH)
+        bad.search = other
+        with pytest.raises(FsaError) as e:
+            sc.set_desc(bad)
+        assert e.value.code == abi.FSA_E_INVALID
+    ok = abi.default_desc(3, abi.SOLVER_IISPH)
+    ok.neighbor_lists = abi.LIST_PLAIN  # a change of settings alone is accepted
+    sc.set_desc(ok)
+    sc.set_desc(d)
+    sc.step(2)
+    got = sc.download(c.copy())
+    sc.close()
+    ref = Scene(gpu_context, d, c.copy())
+    ref.step(2)
+    np.testing.assert_array_equal(got.movement, ref.download(c.copy()).movement)
+    ref.close()

@@ -134,3 +134,31 @@ def test_server_isolates_failed_job(gpu_context):
         np.testing.assert_array_equal(srv.collect(jid).movement, _alone(gpu_context, d, c, steps).movement)
     assert srv.stats()["jobs_failed"] == 1
     srv.close()
+
+
+def test_server_isolates_job_leaving_the_key_range(gpu_context):
+    """A job whose cells leave the 32-bit key range fails alone at the host-side set-up of
+    its tick (before any launch); the jobs of the same batch take that step and finish
+    equal to running alone.  The failed job is forgotten by collect without a copy."""
+    jobs = _jobs()[:3]
+    d, c, _ = jobs[0]
+    far = ParticleCollection.from_arrays(3, np.array([[1500.0, 5.0, 5.0]], np.float32), np.zeros(1, np.uint8))
+    bad = ParticleCollection.concatenate([c.copy(), far])
+    srv = Server(gpu_context)
+    ids = [srv.submit(dd, cc.copy(), s) for dd, cc, s in jobs[:1]]
+    bad_id = srv.submit(d, bad, 4)
+    ids += [srv.submit(dd, cc.copy(), s) for dd, cc, s in jobs[1:]]
+    print("case serving key range", flush=True)
+    assert srv.tick() == 4
+    info = srv.job(bad_id)
+    assert info.state == abi.JOB_FAILED and info.status == abi.FSA_E_INVALID
+    while srv.stats()["jobs_running"] or srv.stats()["jobs_queued"]:
+        srv.tick()
+    for jid, (dd, cc, steps) in zip(ids, jobs):
+        assert srv.job(jid).state == abi.JOB_DONE
+        np.testing.assert_array_equal(srv.collect(jid).movement, _alone(gpu_context, dd, cc, steps).movement)
+    assert srv.stats()["jobs_failed"] == 1
+    srv.release(bad_id)
+    with pytest.raises(Exception):
+        srv.job(bad_id)
+    srv.close()
