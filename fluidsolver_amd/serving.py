@@ -49,15 +49,17 @@ class Server:
         return jid.value
 
     def cancel(self, job_id):
+        """Cancels a queued or running job; its state stays readable through job()
+        until release()."""
         check(lib().fsa_server_cancel(self._h, int(job_id)))
-        self.release(job_id)
+        self._collections.pop(job_id, None)
 
     def release(self, job_id):
         """Forgets a failed or cancelled job (it has no final state to collect)."""
         info = self.job(job_id)
         if info.state in (abi.JOB_FAILED, abi.JOB_CANCELLED):
             check(lib().fsa_server_collect(self._h, int(job_id), None))
-            self._collections.pop(job_id, None)
+        self._collections.pop(job_id, None)
 
     def tick(self):
         """Admit, advance every running scene one step, retire; returns the number of scenes stepped."""

@@ -30,6 +30,7 @@
 
 #include <algorithm>
 #include <array>
+#include <atomic>
 #include <bitset>
 #include <cmath>
 #include <cstdint>
@@ -71,13 +72,18 @@ static void loop_for(size_t n, F&& fn) {
         for (size_t i = 0; i < n; i++) fn(i);
         return;
     }
+    // chunks of 512 indices handed out dynamically (fluid and boundary interleave
+    // unevenly in the sorted order, so static halves would idle most threads)
+    std::atomic<size_t> next{0};
     std::vector<std::thread> pool;
-    size_t chunk = (n + nt - 1) / nt;
     for (int t = 0; t < nt; t++) {
-        size_t b = t * chunk, e = std::min(n, b + chunk);
-        if (b >= e) break;
-        pool.emplace_back([&fn, b, e]() {
-            for (size_t i = b; i < e; i++) fn(i);
+        pool.emplace_back([&fn, &next, n]() {
+            for (;;) {
+                const size_t b = next.fetch_add(512);
+                if (b >= n) break;
+                const size_t e = std::min(n, b + 512);
+                for (size_t i = b; i < e; i++) fn(i);
+            }
         });
     }
     for (auto& th : pool) th.join();
@@ -210,14 +216,19 @@ struct CubicSpline {
 // libmorton morton3D_64_encode as used at CompressedNeighbors.cpp:52-54: x
 // bits land at 3k (22 bits kept), y at 3k+1 and z at 3k+2 (21 bits kept);
 // higher coordinate bits fall off the 64-bit key.
+// The low 21 bits of v spread to every third bit (bit b to bit 3b).
+static inline uint64_t spread3_21(uint64_t v) {
+    v &= 0x1fffffull;
+    v = (v | v << 32) & 0x1f00000000ffffull;
+    v = (v | v << 16) & 0x1f0000ff0000ffull;
+    v = (v | v << 8) & 0x100f00f00f00f00full;
+    v = (v | v << 4) & 0x10c30c30c30c30c3ull;
+    v = (v | v << 2) & 0x1249249249249249ull;
+    return v;
+}
 static inline uint64_t morton3_64(uint32_t x, uint32_t y, uint32_t z) {
-    uint64_t key = 0;
-    for (int b = 0; b < 22; b++) {
-        key |= (uint64_t)((x >> b) & 1u) << (3 * b);
-        if (3 * b + 1 < 64) key |= (uint64_t)((y >> b) & 1u) << (3 * b + 1);
-        if (3 * b + 2 < 64) key |= (uint64_t)((z >> b) & 1u) << (3 * b + 2);
-    }
-    return key;
+    // bits 0..20 of each axis, and x's bit 21 at bit 63
+    return spread3_21(x) | spread3_21(y) << 1 | spread3_21(z) << 2 | (uint64_t)((x >> 21) & 1u) << 63;
 }
 static inline uint32_t to_unsigned(int32_t v) {
     int64_t v2 = (int64_t)v;
@@ -301,6 +312,15 @@ struct NeighborStorage {
         if (!control[k * 2 + 1]) return (uint32_t)deltas[start] + 3u;
         uint32_t v = (uint32_t)deltas[start + 3] << 24 | (uint32_t)deltas[start + 2] << 16 |
                      (uint32_t)deltas[start + 1] << 8 | (uint32_t)deltas[start + 0];
+        return v + 3u + 256u;
+    }
+    // delta k given the payload offset of delta k (advanced past it)
+    uint32_t next_delta(size_t k, size_t& off) const {
+        if (!control[k * 2 + 0]) return control[k * 2 + 1] ? 2u : 1u;
+        if (!control[k * 2 + 1]) return (uint32_t)deltas[off++] + 3u;
+        uint32_t v = (uint32_t)deltas[off + 3] << 24 | (uint32_t)deltas[off + 2] << 16 |
+                     (uint32_t)deltas[off + 1] << 8 | (uint32_t)deltas[off + 0];
+        off += 4;
         return v + 3u + 256u;
     }
     // encoded size model: 4 B first index + 2-bit controls + payload bytes
@@ -441,10 +461,13 @@ struct Scene : SceneBase {
         if (search == SEARCH_COMPRESSED3D) {
             const NeighborStorage& s = storage[i];
             if (s.size() == 0) return;
+            // the iterator of CompressedNeighbors.cpp:381-486: the same deltas as
+            // get_delta, with the payload offset carried along instead of rescanned
             uint64_t cur = s.first_neighbor;
             fn((size_t)cur);
+            size_t off = 0;
             for (size_t c = 1; c < s.size(); c++) {
-                cur += s.get_delta(c - 1);
+                cur += s.next_delta(c - 1, off);
                 fn((size_t)cur);
             }
         } else {
