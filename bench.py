@@ -6,8 +6,10 @@ One step = neighbor search + the full IISPH step (every Jacobi iteration, CFL
 correction, integration) for all fluid particles; particle-updates/s =
 fluid particles x steps / seconds.  Inputs are resident in HBM before timing.
 
-Prints one JSON line with `roofline` for the dominant
-kernel and `cpu_baseline` from the CPU oracle timed on this host.
+Prints one JSON line with `roofline` for the dominant kernel, the search and
+pressure-loop `fractions` of the HBM roofline, a `full_run` of all 1000 steps of
+the configuration from rest, the achievable HBM bandwidth of this GPU, and
+`cpu_baseline` from the CPU oracle timed on this host.
 """
 import argparse
 import ctypes as C
@@ -26,6 +28,7 @@ from fluidsolver_amd._native import lib  # noqa: E402
 from fluidsolver_amd.solver import Context, Scene, check  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
+COPY_BYTES_CAL = 1 << 30
 
 
 def log(msg):
@@ -41,7 +44,7 @@ def algorithmic_bytes(nf, nba, L, ncells=0):
     and each occupied cell's table entry (16 B) once, and writes the lists and
     their counts."""
     return {
-        "search": nf * (12 + 2 * 32 + L) + 12 * nba,
+        "search": nf * (12 + 2 * 32 + L) + 12 * nba + 8 * ncells,
         "lists": (nf + nba) * (4 + 16 + 4 + 4) + 16 * ncells + nf * (L + 4),
         "density": nf * (64 + 16 + L) + 32 * nba,  # + d_ii written for phase 2
         # + d_ii read, + the first Jacobi iteration's update (pressure read and
@@ -81,6 +84,52 @@ def measured_traffic(phase, side):
                 fields = dict(kv.split("=", 1) for kv in line.split()[1:])
                 return int(fields[phase]) if phase in fields else None
     return None
+
+
+def boundary_in_reach(sc, coll):
+    """N_ba: boundary particles inside some fluid particle's search radius, i.e. the
+    distinct boundary entries of the device's fluid neighbor lists."""
+    _, cols = sc.export_neighbors()
+    cols = np.unique(cols.astype(np.int64))
+    return int(np.count_nonzero(coll.type[cols] == abi.TYPE_BOUNDARY))
+
+
+def survey_bytes(nf, nba, L, ncells):
+    """The per-step byte model of the path's measurement definition: the neighbor
+    search B_NS and one Jacobi iteration B_iter."""
+    return {"search": nf * (12 + 2 * 32 + L) + 12 * nba + 8 * ncells, "iteration": nf * (88 + 2 * L) + 32 * nba}
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def hbm_calibration(nbytes=COPY_BYTES_CAL, reps=10):
+    """Achievable HBM bandwidth on this GPU: a device-to-device copy of nbytes (read and
+    written once each), timed with events over reps copies after one untimed copy."""
+    import torch
+
+    a = torch.empty(nbytes // 4, dtype=torch.float32, device="cuda").fill_(1.0)
+    b = torch.empty_like(a)
+    b.copy_(a)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        b.copy_(a)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    del a, b
+    torch.cuda.empty_cache()
+    return 2.0 * nbytes / (ms * 1e-3) / 1e9
 
 
 def cpu_baseline(desc, coll, steps, threads):
@@ -178,27 +227,48 @@ def bench_served(ctx, window_s, rate, max_running, budget_gb, buckets=12):
     desc = abi.default_desc(3, abi.SOLVER_IISPH)
     srv = Server(ctx, hbm_budget_bytes=int(budget_gb * 1e9), max_running=max_running)
     k = 0
+    offered = 0  # particle-updates the arrived jobs ask for
+    open_jobs = {}
+    finished = []  # (submit -> finish seconds, queue wait seconds) of completed jobs
+
+    def harvest():
+        for jid in list(open_jobs):
+            info = srv.job(jid)
+            if info.state == abi.JOB_DONE:
+                finished.append((info.finish_s - info.submit_s, info.admit_s - info.submit_s))
+                srv.release(jid)
+                del open_jobs[jid]
+
     t0 = time.perf_counter()
     next_log = t0 + 5.0
+    ticks = 0
     while True:
         now = time.perf_counter() - t0
         if now >= window_s:
             break
         while k < len(arrivals) and arrivals[k][0] <= now:
             _, nfl, steps = arrivals[k]
-            _, coll = tmpl[int(np.argmin(np.abs(tsizes - np.log(nfl))))]
-            srv.submit(desc, coll, steps)
+            n_t, coll = tmpl[int(np.argmin(np.abs(tsizes - np.log(nfl))))]
+            jid = srv.submit(desc, coll, steps)
+            open_jobs[jid] = True
+            offered += int(np.count_nonzero(coll.type == abi.TYPE_NORMAL)) * steps
             k += 1
         if srv.tick() == 0:
             time.sleep(0.001)
+        ticks += 1
+        if ticks % 50 == 0:
+            harvest()
         if time.perf_counter() > next_log:
             st = srv.stats()
             log(f"  served t={now:.0f}s: done {st['jobs_done']} running {st['jobs_running']} "
                 f"queued {st['jobs_queued']} p99 {st['p99_step_ms']:.2f} ms")
             next_log += 5.0
     elapsed = time.perf_counter() - t0
+    harvest()
     st = srv.stats()
     srv.close()
+    lat = np.array([f[0] for f in finished]) if finished else np.zeros(1)
+    wait = np.array([f[1] for f in finished]) if finished else np.zeros(1)
     return {
         "workload": f"Poisson arrivals {rate}/s of 3D IISPH dam jobs, fluid log-uniform in [10k, 500k] "
         f"({buckets} size templates), 100-2000 steps each",
@@ -214,13 +284,23 @@ def bench_served(ctx, window_s, rate, max_running, budget_gb, buckets=12):
         "p99_step_ms": st["p99_step_ms"],
         "mean_step_ms": st["mean_step_ms"],
         "busy_fraction": st["busy_s"] / elapsed,
+        "offered_particle_updates_per_s": offered / window_s,
+        "sustained_particle_updates_per_s": st["particle_updates"] / elapsed,
+        "jobs_queued_at_end": st["jobs_queued"],
+        "jobs_running_at_end": st["jobs_running"],
+        "job_latency_p50_s": float(np.percentile(lat, 50)),
+        "job_latency_p99_s": float(np.percentile(lat, 99)),
+        "job_queue_wait_p50_s": float(np.percentile(wait, 50)),
+        "job_queue_wait_p99_s": float(np.percentile(wait, 99)),
+        "step_latency_note": "p50/p99_step_ms: wall time of a server tick, counted once per scene it stepped; "
+                             "job_latency: submit to finish of the jobs that finished in the window",
     }
 
 
 def bench_obstacles(ctx, steps, warmup):
-    """The obstacle configuration: a 200^3 fluid block beside 40 seeded spheres
-    (radius U[5,20]) and 20 tori surface-sampled as boundary particles, inside a
-    3-layer box; compressed search with the NeighborStorage capacity.  SESPH
+    """The obstacle configuration: a 200^3 fluid block beside 80 seeded spheres
+    (radius U[5,20]) and 40 tori surface-sampled as boundary particles, inside the
+    3-layer shell of a 600 x 300 x 300 box (2,991,211 boundary particles); compressed search with the NeighborStorage capacity.  SESPH
     (k = 1000, the reference's linear equation of state) and IISPH run as two
     scenes from the identical initial state, each timed over `steps` steps
     after `warmup` steps."""
@@ -229,7 +309,7 @@ def bench_obstacles(ctx, steps, warmup):
     nf, nb = counts["fluid"], counts["boundary"]
     log(f"obstacles: {nf} fluid, {nb} boundary")
     out = {
-        "workload": f"200^3 fluid block with 40 sphere and 20 torus obstacles, {nf} fluid + {nb} boundary, "
+        "workload": f"200^3 fluid block with 80 sphere and 40 torus obstacles in a 600x300x300 box, {nf} fluid + {nb} boundary, "
                     "compressed search with NeighborStorage capacity",
         "steps": steps,
         "warmup": warmup,
@@ -307,7 +387,9 @@ def main():
     ap.add_argument("--serve-slots", type=int, default=32)
     ap.add_argument("--serve-budget-gb", type=float, default=200.0)
     ap.add_argument("--no-serve", action="store_true")
-    ap.add_argument("--obstacle-steps", type=int, default=10)
+    ap.add_argument("--obstacle-steps", type=int, default=200)
+    ap.add_argument("--full-steps", type=int, default=1000,
+                    help="also time this many steps of a fresh 1M scene from rest (the whole configuration; 0: skip)")
     ap.add_argument("--no-obstacles", action="store_true")
     ap.add_argument("--graph", type=int, default=1, help="replay step heads from hipGraphs (1) or launch eagerly (0)")
     ap.add_argument("--pmc-workload", action="store_true",
@@ -321,6 +403,9 @@ def main():
         pmc_workload(args)
         return
 
+    # torch's HIP runtime initialises before the library's (it finds no device after)
+    hbm_gbps = hbm_calibration()
+    log(f"achievable HBM bandwidth (1 GiB device copy): {hbm_gbps:.0f} GB/s")
     side = args.side
     interior = (3 * side, 3 * side // 2, 3 * side // 2)
     coll = scenes.config2_dam3d(seed=2, side=side, interior=interior)
@@ -358,13 +443,14 @@ def main():
     sc.find_neighbors()
     ns = sc.neighbor_stats()
     L = ns["reference_codec_bytes"] / max(nf, 1)
+    nba = boundary_in_reach(sc, coll) if args.search == "compressed" else nb
+    log(f"boundary particles within reach of fluid: {nba} of {nb}")
     sc.simulate()
     ctx.set_timing(True)
     for _ in range(args.profile_steps):
         sc.step(1)
     timing = ctx.timing()
     ctx.set_timing(False)
-    nba = nb  # boundary particles are all within reach of fluid in the worst case
     cells = np.floor(coll.position[:, :3] / np.float32(desc.particle_size * 2)).astype(np.int64)
     ncells = len(np.unique((cells[:, 2] * (1 << 21) + cells[:, 1]) * (1 << 21) + cells[:, 0]))
     algo = algorithmic_bytes(nf, nba, L, ncells)
@@ -406,7 +492,45 @@ def main():
         for k, v in phases.items()
     }
 
+    # the two fractions of the measurement definition: neighbor search B_NS / t_NS and the
+    # pressure loop I * B_iter / t_loop (t_loop: the Jacobi passes and their convergence
+    # decisions; the first iteration's update runs inside the phase-2 kernel)
+    sb = survey_bytes(nf, nba, L, ncells)
+    per_step = {k: v["ms_per_step"] for k, v in phases.items()}
+    t_loop = per_step.get("pressure_a", 0.0) + per_step.get("pressure_b", 0.0) + per_step.get("pressure_first", 0.0)
+    mean_it = float(np.mean(iters))
+    fractions = {
+        "search": {"bytes_per_step": int(sb["search"]), "ms_per_step": per_step.get("search"),
+                   "frac": round(sb["search"] / (per_step["search"] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
+                   if per_step.get("search") else None},
+        "pressure_loop": {"bytes_per_iteration": int(sb["iteration"]), "iterations": mean_it,
+                          "ms_per_step": t_loop,
+                          "frac": round(mean_it * sb["iteration"] / (t_loop * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
+                          if t_loop > 0 else None,
+                          "frac_walking_iterations_only": round(
+                              (mean_it - 1) * sb["iteration"] / (t_loop * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
+                          if t_loop > 0 else None},
+    }
+    log(json.dumps(fractions))
     sc.close()
+
+    full = None
+    if args.full_steps > 0 and args.solver == "iisph":
+        # the whole configuration: a fresh scene from rest, every step timed
+        fs = Scene(ctx, desc, scenes.config2_dam3d(seed=2, side=side, interior=interior))
+        ctx.synchronize()
+        fit = []
+        t0 = time.perf_counter()
+        for _ in range(args.full_steps):
+            fit.append(fs.step(1).iterations)
+        ctx.synchronize()
+        t_full = time.perf_counter() - t0
+        fs.close()
+        full = {"steps": args.full_steps, "seconds": t_full, "ms_per_step": 1000.0 * t_full / args.full_steps,
+                "particle_updates_per_s": nf * args.full_steps / t_full,
+                "mean_jacobi_iterations": float(np.mean(fit)), "max_jacobi_iterations": int(np.max(fit))}
+        log(f"full run: {json.dumps(full)}")
+
     batched = None if args.no_batch else bench_batched(ctx, args.batch_scenes, args.batch_steps, args.alone_steps)
     if batched:
         log(json.dumps(batched))
@@ -436,6 +560,8 @@ def main():
             "kind": "port",
             "sample": f"{args.cpu_steps} IISPH step(s) of the same scene with {cside}^3 fluid after 1 warm-up step, "
             f"{t:.1f}s on {cores} threads",
+            "cpu_model": cpu_model(),
+            "host_cpus": os.cpu_count(),
         }
         log(json.dumps(cpu))
 
@@ -459,7 +585,12 @@ def main():
             "boundary_particles": nb,
             "mean_jacobi_iterations": float(np.mean(iters)),
             "reference_codec_bytes_per_list": round(L, 2),
+            "boundary_within_reach_of_fluid": nba,
+            "occupied_cells": ncells,
         },
+        "fractions": fractions,
+        "full_run": full,
+        "hbm_measured_GBps": round(hbm_gbps, 1),
         "roofline": roofline,
         "roofline_by_phase": roofline_by_phase,
         "cpu_baseline": cpu,

@@ -147,11 +147,14 @@ def config5_arrivals(seed=5, duration_s=600.0, rate=20.0, lo=10000, hi=500000, s
     return [(float(a), int(b), int(c)) for a, b, c in zip(t[keep], sizes[keep], steps[keep])]
 
 
-def config3_obstacles(seed=3, side=200, n_spheres=40, n_tori=20, interior=None):
+def config3_obstacles(seed=3, side=200, n_spheres=80, n_tori=40, interior=None):
     """Fluid block plus seeded sphere and torus obstacles surface-sampled at
-    spacing ~1 (sphere radius U[5,20]); boundary typed 1."""
+    spacing ~1 (sphere radius U[5,20]); boundary typed 1.  The box keeps the
+    proportions of configuration 2 (3a x 1.5a x 1.5a for block side a): at
+    side 200 that is 8,000,000 fluid and 2,991,211 boundary particles (the
+    3-layer shell of a 600 x 300 x 300 box plus the obstacles)."""
     if interior is None:
-        interior = (3 * side // 2, 3 * side // 4 + side // 4, 3 * side // 4 + side // 4)
+        interior = (3 * side, 3 * side // 2, 3 * side // 2)
     base = dam_scene((side, side, side), interior, 3, seed)
     u = uniform24(seed + 77, 8 * (n_spheres + n_tori)).astype(np.float64).reshape(-1, 8)
     pts = []

@@ -79,7 +79,7 @@ def test_obstacles_full_size_lists_and_densities(gpu_context, solver):
     n = coll.size()
     nf = int((coll.type == abi.TYPE_NORMAL).sum())
     print("case obstacles full", solver, nf, n, flush=True)
-    assert nf == 200**3 and n > 9_000_000
+    assert nf == 200**3 and n - nf == 2_991_211
     d = desc_for(solver)
     r = f32(d.particle_size * 2)
     sc = Scene(gpu_context, d, coll.copy())
