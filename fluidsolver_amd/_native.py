@@ -48,6 +48,7 @@ def _declare(L):
         "fsa_scene_next_timestep": (i32, [vp, P(f32)]),
         "fsa_scene_set_entities": (i32, [vp, P(Entity), u32]),
         "fsa_scene_get_statistics": (i32, [vp, f32, P(SceneStatistics)]),
+        "fsa_scene_get_statistics_noalloc": (i32, [vp, f32, P(SceneStatistics)]),
         "fsa_scene_query_neighbors": (i32, [vp, vp, u64, vp, vp, u64, P(u64)]),
         "fsa_sensor_plane_defaults": (None, [P(SensorPlane)]),
         "fsa_scene_sensor_plane": (i32, [vp, P(SensorPlane), vp]),

@@ -88,6 +88,7 @@ struct fsa_context_s {
         uint64_t fluid;
     };
     std::map<uint64_t, Tuned> tuned;
+    void* stat_scratch = nullptr;  // this device's address of the no-allocation statistics scratch
 };
 
 // Launch set over several scenes (one table entry and block-map range each).
@@ -154,6 +155,54 @@ struct fsa_scene_s {
 static_assert(sizeof(EntityDev) == sizeof(fsa_entity), "EntityDev mirrors fsa_entity");
 
 namespace {
+
+// ---- the statistics scratch of callers that cannot allocate -------------------------------
+// fsa_scene_get_statistics_noalloc keeps the sensors' per-block partials and the reduced
+// result in this file-scope device array instead of a per-scene allocation.  Every GPU
+// the library runs on holds its own copy, at its own address, until the process ends;
+// host code reaches it only through the address hipGetSymbolAddress gives on that device.
+constexpr uint32_t kNoAllocStatBlocks = 65536;  // 65,536 blocks of 256: 16,777,216 particles
+static __device__ __align__(256) uint8_t scratch[(kNoAllocStatBlocks + 1) * kStatAccBytes];
+
+constexpr int kScratchDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kScratchDevices] = {};
+
+// address of `scratch` on `device`, which must be current; looked up once per device
+void* scratch_address(int device) {
+    if (device < 0 || device >= kScratchDevices) return nullptr;
+    std::lock_guard<std::mutex> g(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        void* p = nullptr;
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess) return nullptr;
+        g_scratch_ptr[device] = p;
+    }
+    return g_scratch_ptr[device];
+}
+
+// the sensors' values from the reduced accumulator (a host copy of it)
+void fill_statistics(const fsa_scene_s* s, const void* res, fsa_scene_statistics* out) {
+    uint64_t cnt[3];
+    float mx[3], mn[3];
+    double sum[5];
+    read_statistics(res, cnt, mx, mn, sum);
+    std::memset(out, 0, sizeof(*out));
+    out->normal_particles = cnt[0];
+    out->boundary_particles = cnt[1];
+    out->inactive_particles = cnt[2] + s->inactive_rows.size();
+    out->density_max = mx[0];
+    out->density_min = mn[0];
+    out->pressure_max = mx[1];
+    out->pressure_min = mn[1];
+    out->velocity_max = mx[2];
+    out->velocity_min = mn[2];
+    const double c = (double)cnt[0];
+    out->density_avg = cnt[0] ? sum[0] / c : 0.0;
+    out->pressure_avg = cnt[0] ? sum[1] / c : 0.0;
+    out->velocity_avg = cnt[0] ? sum[2] / c : 0.0;
+    out->kinetic_energy = sum[3];
+    out->potential_energy = sum[4];
+}
 
 // the search a description names, with FSA_SEARCH_DEFAULT resolved per dimension
 uint32_t search_of(const fsa_scene_desc* d) {
@@ -2168,6 +2217,7 @@ FSA_API int32_t fsa_context_create(int32_t device, uint64_t budget, fsa_context*
         c->ev_pool.resize(4096);
         c->ev_phase.resize(2048);
         for (auto& e : c->ev_pool) HIP_CHECK(hipEventCreate(&e));
+        c->stat_scratch = scratch_address(device);  // device is current
         *out = c;
     });
 }
@@ -2430,29 +2480,33 @@ FSA_API int32_t fsa_scene_get_statistics(fsa_scene s, float zero_height, fsa_sce
         upload_scene_table(s);
         launch_statistics(s->d_table, s->hd.nblocks, zero_height, s->d_stats, st);
         const uint64_t rb = statistics_result_bytes();
-        std::vector<uint8_t> res(rb);
-        HIP_CHECK(hipMemcpyAsync(res.data(), (char*)s->d_stats + need - rb, rb, hipMemcpyDeviceToHost, st));
+        alignas(16) uint8_t res[kStatAccBytes];
+        HIP_CHECK(hipMemcpyAsync(res, (char*)s->d_stats + need - rb, rb, hipMemcpyDeviceToHost, st));
         HIP_CHECK(hipStreamSynchronize(st));
-        uint64_t cnt[3];
-        float mx[3], mn[3];
-        double sum[5];
-        read_statistics(res.data(), cnt, mx, mn, sum);
-        std::memset(out, 0, sizeof(*out));
-        out->normal_particles = cnt[0];
-        out->boundary_particles = cnt[1];
-        out->inactive_particles = cnt[2] + s->inactive_rows.size();
-        out->density_max = mx[0];
-        out->density_min = mn[0];
-        out->pressure_max = mx[1];
-        out->pressure_min = mn[1];
-        out->velocity_max = mx[2];
-        out->velocity_min = mn[2];
-        const double c = (double)cnt[0];
-        out->density_avg = cnt[0] ? sum[0] / c : 0.0;
-        out->pressure_avg = cnt[0] ? sum[1] / c : 0.0;
-        out->velocity_avg = cnt[0] ? sum[2] / c : 0.0;
-        out->kinetic_energy = sum[3];
-        out->potential_energy = sum[4];
+        fill_statistics(s, res, out);
+    });
+}
+
+FSA_API int32_t fsa_scene_get_statistics_noalloc(fsa_scene s, float zero_height, fsa_scene_statistics* out) {
+    return guarded([&] {
+        if (!s || !out) throw FsaError(FSA_E_INVALID, "NULL argument");
+        const uint32_t nb = std::max<uint32_t>(s->hd.nblocks, 1);
+        if (nb > kNoAllocStatBlocks)
+            throw FsaError(FSA_E_INVALID, "scene exceeds the no-allocation statistics limit of 16,777,216 "
+                                          "resident particles: use fsa_scene_get_statistics");
+        if (!s->ctx->stat_scratch)
+            throw FsaError(FSA_E_DEVICE, "the statistics scratch of this device has no address");
+        HIP_CHECK(hipSetDevice(s->ctx->device));
+        hipStream_t st = s->ctx->stream;
+        upload_scene_table(s);
+        // every block writes its partial and the finalize block writes the result: nothing
+        // in the scratch needs clearing between calls
+        launch_statistics(s->d_table, s->hd.nblocks, zero_height, s->ctx->stat_scratch, st);
+        alignas(16) uint8_t res[kStatAccBytes];
+        HIP_CHECK(hipMemcpyAsync(res, (char*)s->ctx->stat_scratch + (uint64_t)nb * kStatAccBytes, kStatAccBytes,
+                                 hipMemcpyDeviceToHost, st));
+        HIP_CHECK(hipStreamSynchronize(st));
+        fill_statistics(s, res, out);
     });
 }
 

@@ -399,6 +399,7 @@ struct StatAcc {
     uint32_t cnt[3];  // normal, boundary, inactive
     uint32_t pad;
 };
+static_assert(sizeof(StatAcc) == kStatAccBytes, "kStatAccBytes sizes the statistics scratch");
 
 __device__ void stat_block_reduce(StatAcc& a) {
     __shared__ StatAcc part[kBlock / 64];

@@ -49,6 +49,7 @@ struct PlaneParams {
 };
 void launch_sensor_plane(const SceneDev* d_scene, const PlaneParams& P, float* out, hipStream_t st);
 void launch_volume(const SceneDev* d_scene, const float4* q, uint32_t nq, float* out, hipStream_t st);
+constexpr uint64_t kStatAccBytes = 80;  // one block's partial of the global sensors
 uint64_t statistics_scratch_bytes(uint32_t nblocks);
 void launch_statistics(const SceneDev* d_scene, uint32_t nblocks, float zero_height, void* scratch, hipStream_t st);
 void read_statistics(const void* host_copy, uint64_t counts[3], float mx[3], float mn[3], double sums[5]);

@@ -186,10 +186,12 @@ class Scene:
         arr = abi.entity_array(list(entities))
         check(lib().fsa_scene_set_entities(self._h, arr, len(entities)))
 
-    def statistics(self, relative_zero_height=0.0):
-        """The global sensors (density, pressure, |v|, energies, counts) reduced on the device."""
+    def statistics(self, relative_zero_height=0.0, no_alloc=False):
+        """The global sensors (density, pressure, |v|, energies, counts) reduced on the device;
+        no_alloc: through the entry point that allocates no device memory."""
         out = abi.SceneStatistics()
-        check(lib().fsa_scene_get_statistics(self._h, float(relative_zero_height), C.byref(out)))
+        fn = lib().fsa_scene_get_statistics_noalloc if no_alloc else lib().fsa_scene_get_statistics
+        check(fn(self._h, float(relative_zero_height), C.byref(out)))
         return out
 
     def query_neighbors(self, positions):

@@ -391,6 +391,17 @@ typedef struct fsa_scene_statistics {
 } fsa_scene_statistics;
 
 FSA_API int32_t fsa_scene_get_statistics(fsa_scene scene, float relative_zero_height, fsa_scene_statistics* out);
+/* fsa_scene_get_statistics without any device allocation.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (code in a tight
+ * serving loop, harnesses that forbid hipMalloc).  The per-block partials and
+ * the result live in a fixed device array the library holds on each GPU until
+ * the process ends (65,537 x 80 bytes, 5.2 MB); scenes of more than 16,777,216
+ * resident particles are refused with FSA_E_INVALID.  Every call on one GPU
+ * shares that array: calls that overlap on one GPU (several host threads, or
+ * contexts on different streams) race and must be serialized by the caller.
+ * Results equal fsa_scene_get_statistics bit for bit. */
+FSA_API int32_t fsa_scene_get_statistics_noalloc(fsa_scene scene, float relative_zero_height,
+                                                 fsa_scene_statistics* out);
 
 /* ---- position queries (3D) ------------------------------------------------ */
 /* NeighborhoodInterface::get_neighbors(position) (NeighborhoodInterface.cpp; the
