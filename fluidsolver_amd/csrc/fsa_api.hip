@@ -1163,8 +1163,19 @@ void autotune(fsa_context ctx, BatchSet* b, const std::vector<fsa_scene_s*>& v) 
     ctx->tuned[cls] = t;
 }
 
+// 3D: integration gated on the loop state (it acts only on scenes whose Jacobi loop has
+// ended and that have not integrated this step)
+void launch_gated_integrate(fsa_context ctx, const LaunchSet& L) {
+    PhaseTimer t(ctx, FSA_TIMER_INTEGRATE);
+    launch_integrate(L, ctx->stream);
+}
+
+// The launch sequence of a step up to the first host check.  3D scenes take the
+// predicted Jacobi passes, `tail` more passes that act only on scenes still
+// iterating, and the gated integration, so a step whose loops end within them
+// needs a single host synchronisation.
 void step_head(fsa_context ctx, const LaunchSet& L, bool d2, bool unsorted, uint32_t passes, uint32_t ep,
-               uint32_t cp, bool sesph, bool bpass, uint32_t chunk) {
+               uint32_t cp, bool sesph, bool bpass, uint32_t chunk, uint32_t tail) {
     hipStream_t st = ctx->stream;
     if (d2)
         search2d_set(ctx, L, ep, cp);
@@ -1181,6 +1192,7 @@ void step_head(fsa_context ctx, const LaunchSet& L, bool d2, bool unsorted, uint
             launch_sesph_density(L, st);
             launch_sesph_accel(L, st);
         }
+        if (!d2) launch_gated_integrate(ctx, L);
         return;
     }
     {
@@ -1201,7 +1213,7 @@ void step_head(fsa_context ctx, const LaunchSet& L, bool d2, bool unsorted, uint
             launch_iisph_diagonal(L, st);
     }
     if (d2) launch2_iisph_begin_loop(L, st);
-    for (uint32_t k = 0; k < chunk; k++) {
+    for (uint32_t k = 0; k < chunk + (d2 ? 0u : tail); k++) {
         if (!d2 && k == 0) {
             // every pressure is zero: phase 2 did the first iteration's update
             PhaseTimer t(ctx, FSA_TIMER_PRESSURE_FIRST);
@@ -1223,6 +1235,7 @@ void step_head(fsa_context ctx, const LaunchSet& L, bool d2, bool unsorted, uint
                 launch_iisph_pass_b(L, st);
         }
     }
+    if (!d2) launch_gated_integrate(ctx, L);
 }
 
 int32_t batch_step_group(fsa_context ctx, const std::vector<fsa_scene_s*>& all, std::vector<fsa_step_stats*>& all_out) {
@@ -1277,11 +1290,13 @@ int32_t batch_step_group(fsa_context ctx, const std::vector<fsa_scene_s*>& all, 
         guess = std::max(guess, std::max(s->hd.min_it, s->last_iterations));
     }
     const uint32_t chunk = sesph ? 0u : std::min(max_it, std::max<uint32_t>(guess, 1));
+    // one gated pass beyond the prediction (3D IISPH), when the iteration limit allows it
+    const uint32_t tail = (sesph || d2) ? 0u : std::min<uint32_t>(1u, max_it - chunk);
     for (auto* s : v) s->hd.vals_buf = passes & 1u;  // the unsorted searches' final bucket buffer
     upload_batch_table(ctx, b);
     if (ctx->graph && !ctx->timing) {
         const std::array<uint32_t, 5> key = {
-            d2 ? (epasses << 8 | cpasses) : passes, chunk, b->ls.spec,
+            d2 ? (epasses << 8 | cpasses) : passes, chunk | tail << 16, b->ls.spec,
             (bpass ? 1u : 0u) | (sesph ? 2u : 0u) | (unsorted ? 4u : 0u) | (b->ls.entities_before ? 8u : 0u) |
                 (b->ls.codec ? 16u : 0u) | (b->ls.wide_lists ? 32u : 0u) | (b->ls.remover ? 64u : 0u),
             (uint32_t)b->ls.unroll[0] | (uint32_t)b->ls.unroll[1] << 8 | (uint32_t)b->ls.unroll[2] << 16 |
@@ -1291,7 +1306,7 @@ int32_t batch_step_group(fsa_context ctx, const std::vector<fsa_scene_s*>& all, 
             hipGraph_t g = nullptr;
             hipGraphExec_t exec = nullptr;
             HIP_CHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-            step_head(ctx, b->ls, d2, unsorted, passes, epasses, cpasses, sesph, bpass, chunk);
+            step_head(ctx, b->ls, d2, unsorted, passes, epasses, cpasses, sesph, bpass, chunk, tail);
             HIP_CHECK(hipStreamEndCapture(st, &g));
             const hipError_t e = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
             (void)hipGraphDestroy(g);
@@ -1300,7 +1315,7 @@ int32_t batch_step_group(fsa_context ctx, const std::vector<fsa_scene_s*>& all, 
         }
         HIP_CHECK(hipGraphLaunch(it->second, st));
     } else {
-        step_head(ctx, b->ls, d2, unsorted, passes, epasses, cpasses, sesph, bpass, chunk);
+        step_head(ctx, b->ls, d2, unsorted, passes, epasses, cpasses, sesph, bpass, chunk, tail);
     }
     if (d2)
         for (auto* s : v) s->hd.search_count++;
@@ -1310,8 +1325,9 @@ int32_t batch_step_group(fsa_context ctx, const std::vector<fsa_scene_s*>& all, 
         b->tuned = true;
     }
     if (!sesph) {
-        // scenes still converging: further iterations, four between checks
-        uint32_t enq = chunk;
+        // scenes still converging: further iterations, four between checks (3D: then the
+        // gated integration of the scenes whose loops have ended)
+        uint32_t enq = chunk + tail;
         while (enq < max_it) {
             bool any_active = false;
             for (auto* s : v)
@@ -1333,18 +1349,17 @@ int32_t batch_step_group(fsa_context ctx, const std::vector<fsa_scene_s*>& all, 
                         launch_iisph_pass_b(b->ls, st);
                 }
             }
+            if (!d2) launch_gated_integrate(ctx, b->ls);
             read_states(ctx, v);
         }
     }
-    {
+    if (d2) {
         PhaseTimer t(ctx, FSA_TIMER_INTEGRATE);
-        if (d2)
-            launch2_integrate(b->ls, st);
-        else
-            launch_integrate(b->ls, st);
+        launch2_integrate(b->ls, st);
     }
+    // the remover checks the integrated positions; 3D steps otherwise end at the check above
     if (b->ls.remover) launch_remover_check(b->ls, st);
-    read_states(ctx, v);
+    if (d2 || b->ls.remover) read_states(ctx, v);
     for (uint32_t q = 0; q < v.size(); q++) {
         fsa_scene_s* s = v[q];
         try {

@@ -73,6 +73,8 @@ struct alignas(16) SceneState {
     float next_v2, next_a2;  // after integration: max |v|^2, |a|^2 over the particles that move
     uint64_t total_neighbors;
     uint64_t codec_bytes;  // NeighborStorage encoded bytes of all lists of the last search
+    uint32_t pending;      // 3D: this step's integration has not run yet
+    uint32_t pad_pending;
 };
 
 // max of (v2, a2) over a kBlock-thread block, NaN ignored; every thread gets the result

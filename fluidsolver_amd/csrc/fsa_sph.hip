@@ -337,6 +337,7 @@ __global__ void k_iisph_begin_loop(const SceneDev* __restrict__ scenes, const Bl
     if (threadIdx.x == 0) {
         SceneState* st = S.state;
         st->active = S.max_it > 0 ? 1u : 0u;
+        st->pending = 1u;
         st->iterations = 0;
         st->max_a2 = 0.0f;
         st->max_v2 = 0.0f;
@@ -581,15 +582,26 @@ __global__ __launch_bounds__(kBlock) void k_cfl_finalize(const SceneDev* __restr
         S.state->max_v2 = fmaxf(tot.max_v2, S.inactive_v2);
         S.state->iterations = 0;
         S.state->avg_err = 0.0;
+        S.state->active = 0u;
+        S.state->pending = 1u;
     }
 }
 
 // ---- CFL correction + Euler-Cromer integration ------------------------------------------
+// A scene integrates once per step, after its Jacobi loop has ended: a launch that
+// finds the loop still running (it runs on in later passes) or the step already
+// integrated leaves the scene alone, so integration can be enqueued right behind
+// the predicted passes and again after further ones.
+__device__ __forceinline__ bool integration_due(const SceneDev& S) {
+    return S.state->active == 0u && S.state->pending != 0u;
+}
+
 __global__ __launch_bounds__(kBlock) void k_integrate(const SceneDev* __restrict__ scenes,
                                                       const BlockRef* __restrict__ map) {
     const BlockRef br = map[blockIdx.x];
     const SceneDev& S = scenes[br.scene];
     if (scene_failed(S)) return;
+    if (!integration_due(S)) return;
     __shared__ int32_t blo[3][kBlock / 64], bhi[3][kBlock / 64];
     const uint32_t i = br.block * kBlock + threadIdx.x;
     const uint32_t c = S.state->cur;
@@ -689,6 +701,7 @@ __global__ __launch_bounds__(kFinalizeThreads) void k_integrate_finalize(const S
                                                                          const BlockRef* __restrict__ map) {
     const SceneDev& S = scenes[map[blockIdx.x].scene];
     if (scene_failed(S)) return;
+    if (!integration_due(S)) return;
     constexpr uint32_t kWaves = kFinalizeThreads / 64;
     __shared__ int32_t red[8][kWaves];
     int32_t l0 = INT32_MAX, l1 = INT32_MAX, l2 = INT32_MAX, h0 = INT32_MIN, h1 = INT32_MIN, h2 = INT32_MIN;
@@ -751,6 +764,7 @@ __global__ __launch_bounds__(kFinalizeThreads) void k_integrate_finalize(const S
         st->dt_actual = fminf(cfl_timestep(S, max_a, max_v), S.dt);
         st->max_v = max_v;
         st->max_a = max_a;
+        st->pending = 0u;
     }
 }
 
