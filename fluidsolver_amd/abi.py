@@ -54,7 +54,8 @@ JOB_CANCELLED = 4
 JOB_STATE_NAMES = {JOB_QUEUED: "queued", JOB_RUNNING: "running", JOB_DONE: "done", JOB_FAILED: "failed",
                    JOB_CANCELLED: "cancelled"}
 
-TIMER_NAMES = ["search", "density", "diagonal", "pressure_a", "pressure_b", "integrate", "pressure_first", "lists"]
+TIMER_NAMES = ["search", "density", "diagonal", "pressure_a", "pressure_b", "integrate", "pressure_first", "lists",
+               "pressure_tail"]
 
 
 class SceneDesc(C.Structure):

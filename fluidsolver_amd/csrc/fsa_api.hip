@@ -1220,6 +1220,13 @@ void step_head(fsa_context ctx, const LaunchSet& L, bool d2, bool unsorted, uint
             launch_iisph_first_converge(L, st);
             continue;
         }
+        if (k >= chunk) {
+            // the gated pass: timed apart, since it is a no-op when the prediction held
+            PhaseTimer t(ctx, FSA_TIMER_PRESSURE_TAIL);
+            launch_iisph_pass_a(L, st);
+            launch_iisph_pass_b(L, st);
+            continue;
+        }
         {
             PhaseTimer t(ctx, FSA_TIMER_PRESSURE_A);
             if (d2)

@@ -337,7 +337,8 @@ FSA_API int32_t fsa_server_get_stats(fsa_server srv, fsa_server_stats* stats);
 #define FSA_TIMER_INTEGRATE 5
 #define FSA_TIMER_PRESSURE_FIRST 6  /* the first Jacobi iteration (3D IISPH: no neighbor walk) */
 #define FSA_TIMER_LISTS 7  /* 3D compressed search: the list kernel alone (inside FSA_TIMER_SEARCH) */
-#define FSA_TIMER_COUNT 8
+#define FSA_TIMER_PRESSURE_TAIL 8  /* 3D IISPH: the gated Jacobi pass beyond the predicted ones (passes A and B) */
+#define FSA_TIMER_COUNT 9
 FSA_API int32_t fsa_context_set_timing(fsa_context ctx, uint32_t enabled);
 FSA_API int32_t fsa_context_get_timing(fsa_context ctx, double* total_ms, uint64_t* launches, uint32_t n);
 
