@@ -22,7 +22,6 @@ constexpr uint32_t kInactiveBit = 0x40000000u;  // neighbor entry flag (2D searc
 constexpr uint32_t kIndexMask = 0x3fffffffu;    // neighbor entry: particle index
 constexpr uint32_t kTypeShift = 30;    // idt word: caller index (30 bits) | type << 30
 constexpr uint32_t kIdMask = (1u << kTypeShift) - 1u;
-constexpr uint32_t kEmptyCell = 0xffffffffu;
 
 // radix sort geometry
 constexpr int kSortThreads = 256;
@@ -408,24 +407,78 @@ __device__ __host__ __forceinline__ uint32_t compact_key(int32_t cx, int32_t cy,
     return key;
 }
 
-// Cell coordinates of a compact key (the inverse of compact_key): the
-// interleaved bits of each axis gathered with shift-and-mask steps, then the
-// axis sign bit extends the L-bit value.
-__device__ __host__ __forceinline__ uint32_t compact_axis(uint32_t v) {
-    v &= 0x09249249u;
-    v = (v ^ (v >> 2)) & 0x030c30c3u;
-    v = (v ^ (v >> 4)) & 0x0300f00fu;
-    v = (v ^ (v >> 8)) & 0xff0000ffu;
-    v = (v ^ (v >> 16)) & 0x000003ffu;
-    return v;
+
+// Key order of a cell's 27 neighbor cells without sorting.  Along one axis the
+// compact keys of c-1, c, c+1 differ in two steps: one changes only coordinate
+// bit 0 (key bit 0, 1 or 2 for x, y, z), the other changes the coordinate bits
+// up to p (trailing zeros of an even c, trailing ones of an odd c), or, between
+// -1 and 0, the sign bit at the top of the key (x, z, y from the top).  So every
+// neighbor has, per axis, a side bit B (which side of that axis's big step) and
+// a bit S (which value within its side), and the key order of the 27 cells is
+// the order of the 6-bit code: the three B bits, the axis whose big step changes
+// the highest key bit first, above the S bits of z, y, x.  (Negative
+// coordinates carry the sign bit, so in key order 0 < 1 < ... < -2 < -1: for
+// c = 0 the big step lies between c and c - 1, for c = -1 between c + 1 and c.)
+//
+// The order packs into one word: bits 2a..2a+1 the type of axis a (0 even,
+// 1 odd, 2 zero, 3 minus one), bits 6+3a..8+3a the code bit of its B bit
+// (3, 4 or 5), bits 15..20 the code of the cell itself.
+constexpr uint32_t kOrderEven = 0u, kOrderOdd = 1u, kOrderZero = 2u, kOrderMinus1 = 3u;
+// per type, 2 bits per (B, S) at bit 2 * (2B + S): the offset + 1, or 3 where no
+// neighbor has that pair.  even: (0,0) -1, (1,0) 0, (1,1) +1; odd: (0,0) -1,
+// (0,1) 0, (1,0) +1; zero: (0,0) 0, (0,1) +1, (1,0) -1; minus one: (0,0) +1,
+// (1,0) -1, (1,1) 0
+constexpr uint32_t kOrderLuts = 0x9Cu | 0xE4u << 8 | 0xC9u << 16 | 0x4Eu << 24;
+
+__device__ __host__ __forceinline__ uint32_t neighbor_order_word(int32_t cx, int32_t cy, int32_t cz, uint32_t L) {
+    const int32_t c[3] = {cx, cy, cz};
+    const uint32_t sign_pos[3] = {3 * L + 2, 3 * L, 3 * L + 1};  // key bit of the sign of x, y, z
+    uint32_t big[3], type[3];
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        const int32_t v = c[a];
+        const bool zero = v == 0, minus1 = v == -1, odd = (v & 1) != 0;
+        // trailing zeros (even) or ones (odd), at most L - 1: a step past the key
+        // range has no neighbor in the grid, and the clamp keeps the positions distinct
+        const uint32_t w = (odd ? ~(uint32_t)v : (uint32_t)v) | (1u << (L - 1u));
+        big[a] = (zero || minus1) ? sign_pos[a] : 3u * (uint32_t)__builtin_ctz(w) + (uint32_t)a;
+        type[a] = zero ? kOrderZero : minus1 ? kOrderMinus1 : odd ? kOrderOdd : kOrderEven;
+    }
+    uint32_t word = 0;
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        const uint32_t bpos = 3u + (big[a] > big[(a + 1) % 3] ? 1u : 0u) + (big[a] > big[(a + 2) % 3] ? 1u : 0u);
+        // the cell's own (B, S): even (1,0), odd (0,1), zero (0,0), minus one (1,1)
+        const uint32_t selfB = (type[a] == kOrderEven || type[a] == kOrderMinus1) ? 1u : 0u;
+        const uint32_t selfS = (type[a] == kOrderOdd || type[a] == kOrderMinus1) ? 1u : 0u;
+        word |= type[a] << (2 * a) | bpos << (6 + 3 * a) | (selfB << bpos | selfS << a) << 15;
+    }
+    return word;
 }
-__device__ __host__ __forceinline__ void cell_of_key(uint32_t key, uint32_t L, int32_t& cx, int32_t& cy, int32_t& cz) {
-    const uint32_t body = key & ((1u << (3 * L)) - 1u);
-    const uint32_t ext = ~((1u << L) - 1u);
-    const uint32_t x = compact_axis(body), y = compact_axis(body >> 1), z = compact_axis(body >> 2);
-    cy = (int32_t)(((key >> (3 * L)) & 1u) ? (y | ext) : y);
-    cz = (int32_t)(((key >> (3 * L + 1)) & 1u) ? (z | ext) : z);
-    cx = (int32_t)(((key >> (3 * L + 2)) & 1u) ? (x | ext) : x);
+
+struct NeighborOrder {
+    uint32_t lut[3];   // per axis: 2-bit offset + 1 (3: none) at bit 2 * (2B + S)
+    uint32_t bpos[3];  // code bit of the axis's B bit (3, 4 or 5)
+    uint32_t self;     // code of the cell itself
+};
+
+__device__ __host__ __forceinline__ NeighborOrder unpack_order(uint32_t word) {
+    NeighborOrder o;
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        o.lut[a] = (kOrderLuts >> (8u * ((word >> (2 * a)) & 3u))) & 0xffu;
+        o.bpos[a] = (word >> (6 + 3 * a)) & 7u;
+    }
+    o.self = (word >> 15) & 63u;
+    return o;
+}
+
+// offset (-1, 0, +1) of lane `code`'s neighbor along axis a; false where no neighbor has the code
+__device__ __host__ __forceinline__ bool neighbor_offset(const NeighborOrder& o, uint32_t code, int a, int32_t& d) {
+    const uint32_t idx = ((code >> o.bpos[a]) & 1u) << 1 | ((code >> a) & 1u);
+    const uint32_t v = (o.lut[a] >> (2u * idx)) & 3u;
+    d = (int32_t)v - 1;
+    return v != 3u;
 }
 
 __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }

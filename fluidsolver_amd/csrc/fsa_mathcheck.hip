@@ -6,6 +6,8 @@
 //   fsa_mc_div:    every divisor mantissa in [1, 2) at the given exponents, against
 //                  numerators drawn from a hash (zeros, tiny, extreme and random);
 //                  div_pair bit for bit, div_pair_zs by value (zeros of either sign)
+//   fsa_mc_neighbor_order: the code order of a cell's 27 neighbors (neighbor_order_word)
+//   fsa_mc_int_abs: x86_int_abs of given values
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -104,7 +106,37 @@ __global__ void k_mc_int_abs(const float* in, float* out, uint32_t n) {
     if (i < n) out[i] = x86_int_abs(in[i]);
 }
 
+// the neighbor order of n cells: out[64 * i + k] = the neighbor offset index
+// (dx+1)*9 + (dy+1)*3 + (dz+1) of code k, or 255 where no neighbor has code k;
+// self[i] = the cell's own code
+__global__ void k_mc_neighbor_order(const int32_t* cells, uint32_t L, uint32_t n, uint8_t* out, uint8_t* self) {
+    const uint32_t i = blockIdx.x;
+    if (i >= n) return;
+    const uint32_t k = threadIdx.x;
+    const NeighborOrder o = unpack_order(neighbor_order_word(cells[3 * i], cells[3 * i + 1], cells[3 * i + 2], L));
+    int32_t dx, dy, dz;
+    const bool has = neighbor_offset(o, k, 0, dx) & neighbor_offset(o, k, 1, dy) & neighbor_offset(o, k, 2, dz);
+    out[64 * i + k] = has ? (uint8_t)((dx + 1) * 9 + (dy + 1) * 3 + (dz + 1)) : (uint8_t)255;
+    if (k == 0) self[i] = (uint8_t)o.self;
+}
+
 }  // namespace
+
+// device neighbor order of the cells (n x 3 coordinates) at key level L into
+// out (n x 64) and self (n) (n <= 1 << 20)
+extern "C" __attribute__((visibility("default"))) int fsa_mc_neighbor_order(const int32_t* cells, uint32_t L,
+                                                                           uint32_t n, uint8_t* out, uint8_t* self) {
+    if (n == 0 || n > (1u << 20) || L < 1 || L > 9) return 3;
+    uint8_t* d = nullptr;
+    const size_t cb = 12 * (size_t)n, ob = 64 * (size_t)n;
+    if (hipMalloc(&d, cb + ob + n) != hipSuccess) return 1;
+    (void)hipMemcpy(d, cells, cb, hipMemcpyHostToDevice);
+    k_mc_neighbor_order<<<n, 64>>>((const int32_t*)d, L, n, d + cb, d + cb + ob);
+    hipError_t e = hipMemcpy(out, d + cb, ob, hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(self, d + cb + ob, n, hipMemcpyDeviceToHost);
+    (void)hipFree(d);
+    return e == hipSuccess ? 0 : 2;
+}
 
 // device x86_int_abs of host values in[0..n) into out (n <= 4096)
 extern "C" __attribute__((visibility("default"))) int fsa_mc_int_abs(const float* in, float* out, uint32_t n) {

@@ -247,7 +247,8 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(const SceneDev* 
 // ---- 3. permute ------------------------------------------------------------
 // CellTable: the keys are this search's sorted cell keys, and the cell table
 // (CompressedNeighbors.cpp:203-225: the run of each cell in sorted order) is
-// written from the permuted positions on the way.
+// written from the permuted positions on the way, with each run's grid
+// coordinates and neighbor order (neighbor_order_word) for the list kernel.
 template <bool CellTable>
 __global__ __launch_bounds__(kBlock) void k_permute(const SceneDev* __restrict__ scenes,
                                                     const BlockRef* __restrict__ map, uint32_t sbuf) {
@@ -266,14 +267,21 @@ __global__ __launch_bounds__(kBlock) void k_permute(const SceneDev* __restrict__
         const bool first = CellTable && (i == 0 || keys[i - 1] != key);
         const bool last = CellTable && (i + 1 == S.n || keys[i + 1] != key);
         if (first || last) {
+            const int32_t cx = cell_of(p.x, S.search_radius), cy = cell_of(p.y, S.search_radius),
+                          cz = cell_of(p.z, S.search_radius);
             uint32_t lin;
-            if (!cell_linear(S, cell_of(p.x, S.search_radius), cell_of(p.y, S.search_radius),
-                             cell_of(p.z, S.search_radius), lin)) {
+            if (!cell_linear(S, cx, cy, cz, lin)) {
                 atomicOr(&S.state->status, kStatusGrid);
             } else {
                 if (first) {
                     S.cells[lin].x = i;
                     S.cells[lin].z = S.stamp;
+                    // for the list kernel, at the run's first position of the sort's
+                    // idle buffers: the cell's grid coordinates (10 bits each) and the
+                    // key order of its neighbors
+                    S.keys[sbuf ^ 1u][i] = (uint32_t)(cx - S.cmin[0]) | (uint32_t)(cy - S.cmin[1]) << 10 |
+                                           (uint32_t)(cz - S.cmin[2]) << 20;
+                    S.vals[sbuf ^ 1u][i] = neighbor_order_word(cx, cy, cz, S.levels);
                 }
                 if (last) S.cells[lin].y = i + 1;
             }
@@ -333,9 +341,10 @@ __global__ __launch_bounds__(kBlock) void k_compact(const SceneDev* __restrict__
 
 // ---- 6. neighbor lists ---------------------------------------------------------
 // One wave per 64 consecutive sorted particles processes every cell whose run
-// starts among them.  For a cell, the wave sorts the runs of the 27 neighbor
-// cells by start (= by cell key, the reference's std::sort of the 27 keys), so
-// candidate g of the concatenated runs has ascending combined index.  Up to
+// starts among them.  For a cell, lane k loads the run of the neighbor cell of
+// order code k (NeighborOrder), so the runs lie in ascending key order (the
+// reference's std::sort of the 27 keys) without a sort, and candidate g of the
+// concatenated runs has ascending combined index.  Up to
 // kListChunks chunks of 64 candidates are held one per lane; the cell's fluid
 // particles, kStageParticles at a time, are taken in turn, each tested against
 // the held chunks with one ballot per chunk, whose prefix (on top of the
@@ -356,48 +365,21 @@ constexpr uint32_t kMapRow = 13;
 static_assert(kMapRow * kMaxNeighbors + kListChunks * 64 <= kStageParticles * kMaxNeighbors,
               "the candidate map fits the stage rows from kMapRow on");
 
-// one compare-exchange stage of a bitonic sort within 32-lane groups; the
-// partner comes from ds_swizzle's bit mode (and 0x1f, xor J), which needs no
-// address registers
-template <uint32_t K, uint32_t J>
-__device__ __forceinline__ uint32_t bitonic32_stage(uint32_t v, uint32_t lane) {
-    const uint32_t o = (uint32_t)__builtin_amdgcn_ds_swizzle((int)v, (int)(0x1fu | (J << 10)));
-    const bool up = (lane & K) == 0;
-    const bool lower = (lane & J) == 0;
-    return (up == lower) ? min(v, o) : max(v, o);
-}
-
-// lanes 0..31 ascending by lane (lanes 32..63 are sorted descending among themselves)
-__device__ __forceinline__ uint32_t sort32_u32(uint32_t v) {
-    const uint32_t lane = __lane_id();
-    v = bitonic32_stage<2, 1>(v, lane);
-    v = bitonic32_stage<4, 2>(v, lane);
-    v = bitonic32_stage<4, 1>(v, lane);
-    v = bitonic32_stage<8, 4>(v, lane);
-    v = bitonic32_stage<8, 2>(v, lane);
-    v = bitonic32_stage<8, 1>(v, lane);
-    v = bitonic32_stage<16, 8>(v, lane);
-    v = bitonic32_stage<16, 4>(v, lane);
-    v = bitonic32_stage<16, 2>(v, lane);
-    v = bitonic32_stage<16, 1>(v, lane);
-    v = bitonic32_stage<32, 16>(v, lane);
-    v = bitonic32_stage<32, 8>(v, lane);
-    v = bitonic32_stage<32, 4>(v, lane);
-    v = bitonic32_stage<32, 2>(v, lane);
-    v = bitonic32_stage<32, 1>(v, lane);
-    return v;
-}
-
-// exclusive prefix sum over lanes 0..31 (each 32-lane half on its own); total = sum of lanes 0..31
-__device__ __forceinline__ uint32_t exclusive_scan32(uint32_t v, uint32_t& total) {
-    const uint32_t lane = __lane_id() & 31u;
+// exclusive prefix sum over the 64 lanes from DPP row shifts and row broadcasts
+// (no LDS round trip); total = the sum over all lanes, uniform
+__device__ __forceinline__ uint32_t exclusive_scan64(uint32_t v, uint32_t& total) {
     uint32_t x = v;
-#pragma unroll
-    for (int o = 1; o < 32; o <<= 1) {
-        const uint32_t y = __shfl_up(x, o, 32);
-        if (lane >= (uint32_t)o) x += y;
-    }
-    total = (uint32_t)__builtin_amdgcn_readlane((int)x, 31);  // uniform, in a scalar register
+    // inclusive scans within each row of 16 lanes (row_shr:1, 2, 4, 8; lanes
+    // shifted in from outside the row read 0)
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x111, 0xf, 0xf, false);
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x112, 0xf, 0xf, false);
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x114, 0xf, 0xf, false);
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x118, 0xf, 0xf, false);
+    // row_bcast:15 adds lane 15's sum to row 1 and lane 47's to row 3; row_bcast:31
+    // then adds lane 31's (rows 0-1) to rows 2 and 3
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x142, 0xa, 0xf, false);
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x143, 0xc, 0xf, false);
+    total = (uint32_t)__builtin_amdgcn_readlane((int)x, 63);
     return x - v;
 }
 
@@ -441,13 +423,18 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) voi
     FSA_GLOBAL(uint32_t)* __restrict__ nbr = (FSA_GLOBAL(uint32_t)*)S.nbr;
     const float r2 = S.r2;
     const uint32_t stride = S.list_stride;
-    // run starts among this wave's particles
+    // run starts among this wave's particles, with their cells' grid coordinates
+    // and neighbor order (left by k_permute in the sort's idle buffers)
     const uint32_t mi = base + lane;
     bool start = false;
-    uint32_t mykey = 0;
+    uint32_t cgrid = 0, corder = 0;
     if (mi < S.n) {
-        mykey = keys[mi];
+        const uint32_t mykey = keys[mi];
         start = mi == 0 || keys[mi - 1] != mykey;
+        if (start) {
+            cgrid = S.keys[sbuf ^ 1u][mi];
+            corder = S.vals[sbuf ^ 1u][mi];
+        }
     }
     uint64_t starts = __ballot(start);
     if (starts == 0) return;
@@ -486,20 +473,23 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) voi
         fcount = 0;
         frows = 0;
     };
-    // The loads a cell starts from: the table entries of its 27 neighbor cells
-    // (lanes 0..26; lane 13 is the cell itself) and the first particle group's
-    // type, position and rank.  They depend on the cell's key only, so the next
-    // cell's are issued while the current one is processed.
-    auto cell_loads = [&](uint32_t sl, uint4& ent, uint32_t& gidt, float4& gpos, uint32_t& grank) {
-        int32_t cx, cy, cz;
-        cell_of_key(readlane_u(mykey, sl), S.levels, cx, cy, cz);
+    // The loads a cell starts from: the table entries of its 27 neighbor cells,
+    // lane k holding the neighbor of order code k (so the runs lie in ascending
+    // key order by lane; `self` is the cell's own lane), and the first particle
+    // group's type, position and rank.  They depend on the cell's key only, so
+    // the next cell's are issued while the current one is processed.
+    auto cell_loads = [&](uint32_t sl, uint4& ent, uint32_t& self, uint32_t& gidt, float4& gpos, uint32_t& grank) {
+        const uint32_t g = readlane_u(cgrid, sl);
+        const NeighborOrder o = unpack_order(readlane_u(corder, sl));
+        self = o.self;
         ent = make_uint4(0u, 0u, 0u, 0u);  // stamp 0 never matches: an absent cell
-        if (lane < 27) {
-            uint32_t lin;
-            if (cell_linear(S, cx + (int)(lane / 9) - 1, cy + (int)((lane / 3) % 3) - 1, cz + (int)(lane % 3) - 1,
-                            lin))
-                ent = S.cells[lin];
-        }
+        int32_t dx, dy, dz;
+        const bool has = neighbor_offset(o, lane, 0, dx) & neighbor_offset(o, lane, 1, dy) &
+                         neighbor_offset(o, lane, 2, dz);
+        // grid coordinates of the neighbor; -1 wraps past the grid's extent
+        const uint32_t x = (g & 1023u) + (uint32_t)dx, y = ((g >> 10) & 1023u) + (uint32_t)dy,
+                       z = (g >> 20) + (uint32_t)dz;
+        if (has && x < S.cdim[0] && y < S.cdim[1] && z < S.cdim[2]) ent = S.cells[x + S.cdim[0] * (y + S.cdim[1] * z)];
         const uint32_t p = base + sl + lane;
         gidt = 0xffffffffu;  // not a fluid particle
         gpos = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
@@ -513,15 +503,15 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) voi
     uint32_t sl0 = (uint32_t)__ffsll((unsigned long long)starts) - 1u;
     starts &= starts - 1;
     uint4 ent;
-    uint32_t gidt, grank;
+    uint32_t gidt, grank, eself;
     float4 gpos;
-    cell_loads(sl0, ent, gidt, gpos, grank);
+    cell_loads(sl0, ent, eself, gidt, gpos, grank);
     while (true) {
         const uint32_t cs = base + sl0;
         const uint4 e = ent;
         const uint32_t idt0 = gidt, rank0 = grank;
         const float4 pos0 = gpos;
-        const uint32_t ce = readlane_u(e.y, 13);
+        const uint32_t ce = readlane_u(e.y, eself);
         const bool more = starts != 0;
         uint32_t nsl = 0;
         if (more) {
@@ -531,29 +521,21 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) voi
         // a cell without fluid particles needs no lists
         const bool f0 = lane < kStageParticles && cs + lane < ce && (idt0 >> kTypeShift) == 0u;
         const bool work = __ballot(f0) != 0 || ce - cs > kStageParticles;
-        // runs of the 27 neighbor cells in lanes 0..26, sorted by start; each
-        // run's end rides along by lane and is fetched back after the sort
-        uint32_t key = kEmptyCell, cend = 0;
-        if (lane < 27 && e.z == S.stamp) {
-            key = (e.x << 5) | lane;
-            cend = e.y;
+        // the runs of the neighbor cells in key order (the reference's std::sort
+        // of the 27 keys) by lane; absent cells are empty runs
+        uint32_t rs = 0, rl = 0;
+        if (e.z == S.stamp) {
+            rs = e.x;
+            rl = e.y - e.x;
         }
-        if (more) cell_loads(nsl, ent, gidt, gpos, grank);
+        if (more) cell_loads(nsl, ent, eself, gidt, gpos, grank);
         if (!work) {
             if (!more) break;
             sl0 = nsl;
             continue;
         }
-        key = sort32_u32(key);
-        const uint32_t nvalid = (uint32_t)__popcll(__ballot(lane < 32 && key != kEmptyCell));
-        const uint32_t end_of_key = __shfl(cend, (int)(key & 31u), 64);
-        uint32_t rs = 0, rl = 0;
-        if (lane < 32 && key != kEmptyCell) {
-            rs = key >> 5;
-            rl = end_of_key - rs;
-        }
         uint32_t total;
-        const uint32_t roff = exclusive_scan32(rl, total);
+        const uint32_t roff = exclusive_scan64(rl, total);
         // Up to kListChunks * 64 candidates (the usual case): each run writes its
         // element indices to slots roff.. of the stage, each lane reads back the
         // index of candidate k * 64 + lane, and the candidates are loaded once for
@@ -621,15 +603,17 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) voi
                     pj[k] = make_float4(__builtin_inff(), __builtin_inff(), __builtin_inff(), 0.0f);
                     ej[k] = 0;
                     if (c0 + k * 64 < total) {
-                        // run q of candidate g by binary search over the offsets
-                        // (uniform: every lane takes part in each shuffle)
+                        // run q of candidate g: the last lane whose offset is at most g
+                        // (an empty run shares its offset with the next run, so the
+                        // last such lane holds a non-empty run); every lane takes
+                        // part in each shuffle
                         const uint32_t g = c0 + k * 64 + lane;
                         uint32_t q = 0;
 #pragma unroll
-                        for (uint32_t step = 16; step > 0; step >>= 1) {
+                        for (uint32_t step = 32; step > 0; step >>= 1) {
                             const uint32_t probe = q + step;
-                            const uint32_t off = __shfl(roff, (int)min(probe, 31u), 64);
-                            if (probe < nvalid && off <= g) q = probe;
+                            const uint32_t off = __shfl(roff, (int)probe, 64);
+                            if (off <= g) q = probe;
                         }
                         const uint32_t qs = __shfl(rs, (int)q, 64);
                         const uint32_t qo = __shfl(roff, (int)q, 64);
@@ -873,7 +857,7 @@ __global__ __launch_bounds__(kBlock) void k3u_lists(const SceneDev* __restrict__
             rl = e.y - e.x;
         }
         uint32_t total;
-        const uint32_t roff = exclusive_scan32(rl, total);
+        const uint32_t roff = exclusive_scan64(rl, total);
         const bool serial = quad && total > kQuadSorted;
         // quadratic: every candidate index, sorted ascending (two chunk sets at most)
         uint32_t sj[2 * kListChunks];
