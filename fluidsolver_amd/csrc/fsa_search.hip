@@ -353,17 +353,21 @@ __global__ __launch_bounds__(kBlock) void k_compact(const SceneDev* __restrict__
 // stage (the fluid ranks of a wave's cells are consecutive) and leave it row by
 // row: one store covers 64 / kStageParticles list rows of up to kStageParticles
 // consecutive fluid ranks, instead of one scattered store per particle and chunk.
-// With at most kListChunks * 64 candidates, a cell's candidates are mapped
-// through the stage rows from kMapRow on and loaded once for all its particles.
+// With at most kListChunks * 64 candidates, each candidate slot finds its run
+// through marks in the stage rows from kMapRow on, and the candidates are loaded
+// once for all the cell's particles.
 // The next cell's table entries and first particle group are loaded while the
 // current cell is processed.
 
 constexpr uint32_t kListChunks = 4;
 constexpr uint32_t kStageParticles = 16;
-// first stage row of a cell's candidate map (kListChunks * 64 slots)
+// first stage row of a cell's candidate marks (kListChunks * 64 slots from the
+// first 16-byte boundary of the row on)
 constexpr uint32_t kMapRow = 13;
-static_assert(kMapRow * kMaxNeighbors + kListChunks * 64 <= kStageParticles * kMaxNeighbors,
-              "the candidate map fits the stage rows from kMapRow on");
+constexpr uint32_t kMarkWord = (kMapRow * kMaxNeighbors + 3u) & ~3u;
+static_assert(kMarkWord + kListChunks * 64 <= kStageParticles * kMaxNeighbors,
+              "the candidate marks fit the stage rows from kMapRow on");
+static_assert((kStageParticles * kMaxNeighbors) % 4 == 0, "each wave's stage starts on a 16-byte boundary");
 
 // exclusive prefix sum over the 64 lanes from DPP row shifts and row broadcasts
 // (no LDS round trip); total = the sum over all lanes, uniform
@@ -381,6 +385,18 @@ __device__ __forceinline__ uint32_t exclusive_scan64(uint32_t v, uint32_t& total
     x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x143, 0xc, 0xf, false);
     total = (uint32_t)__builtin_amdgcn_readlane((int)x, 63);
     return x - v;
+}
+
+// maximum over lanes 0..lane of values >= 0, from DPP row shifts and row broadcasts
+__device__ __forceinline__ uint32_t inclusive_max64(uint32_t v) {
+    uint32_t x = v;
+    x = max(x, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x111, 0xf, 0xf, false));
+    x = max(x, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x112, 0xf, 0xf, false));
+    x = max(x, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x114, 0xf, 0xf, false));
+    x = max(x, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x118, 0xf, 0xf, false));
+    x = max(x, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x142, 0xa, 0xf, false));
+    x = max(x, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x143, 0xc, 0xf, false));
+    return x;
 }
 
 __device__ __forceinline__ float readlane_f(float v, uint32_t lane) {
@@ -405,7 +421,7 @@ __device__ __forceinline__ uint32_t mul_u24(uint32_t v, uint32_t s) {
 template <bool Wide>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) void k_lists(const SceneDev* __restrict__ scenes,
                                                   const BlockRef* __restrict__ map, uint32_t sbuf) {
-    __shared__ uint32_t stage_all[kBlock / 64][kStageParticles * kMaxNeighbors];
+    __shared__ __align__(16) uint32_t stage_all[kBlock / 64][kStageParticles * kMaxNeighbors];
     __shared__ uint32_t stage_cnt_all[kBlock / 64][kStageParticles];
     const BlockRef br = map[blockIdx.x];
     const SceneDev& S = scenes[br.scene];
@@ -536,35 +552,44 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) voi
         }
         uint32_t total;
         const uint32_t roff = exclusive_scan64(rl, total);
-        // Up to kListChunks * 64 candidates (the usual case): each run writes its
-        // element indices to slots roff.. of the stage, each lane reads back the
-        // index of candidate k * 64 + lane, and the candidates are loaded once for
-        // every particle group of the cell.  More candidates take the binary
-        // search below, chunk by chunk and group by group.
+        // Up to kListChunks * 64 candidates (the usual case): each non-empty run
+        // marks the slot of its first candidate with its lane + 1, so the run of
+        // candidate g = k * 64 + lane is the largest mark at or below slot g (a
+        // running maximum over the lanes, carried from chunk to chunk), and its
+        // element index is rs - roff of that run, plus g.  The candidates are
+        // loaded once for every particle group of the cell.  More candidates take
+        // the binary search below, chunk by chunk and group by group.
         const bool mapped = total <= kListChunks * 64u;
         float4 pjm[kListChunks];
         uint32_t ejm[kListChunks];
         if (mapped) {
-            // the map takes the stage from row kMapRow on, which no gathered row may hold
+            // the marks take the stage from row kMapRow on, which no gathered row may hold
             if (fcount > kMapRow) flush();
-            uint32_t* const map_slots = stage + kMapRow * (uint32_t)kMaxNeighbors;
-            for (uint32_t t = 0; __any(t < rl); t++)
-                if (t < rl) map_slots[roff + t] = rs + t;
+            uint32_t* const marks = stage + kMarkWord;
+            reinterpret_cast<uint4*>(marks)[lane] = make_uint4(0u, 0u, 0u, 0u);  // the last cell's marks
+            if (rl != 0) marks[roff] = lane + 1u;
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             __builtin_amdgcn_wave_barrier();
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            const uint32_t jbase = rs - roff;  // modulo 2^32
+            uint32_t carry = 0;
 #pragma unroll
             for (uint32_t k = 0; k < kListChunks; k++) {
                 pjm[k] = make_float4(__builtin_inff(), __builtin_inff(), __builtin_inff(), 0.0f);
                 ejm[k] = 0;
-                const uint32_t g = k * 64 + lane;
-                if (g < total) {
-                    const uint32_t j = map_slots[g];
-                    pjm[k] = pos[j];
-                    ejm[k] = j | ((idt[j] >> kTypeShift) == 1u ? kBoundaryBit : 0u);
+                if (k * 64 < total) {
+                    const uint32_t g = k * 64 + lane;
+                    // slot 0 holds the first non-empty run's mark, so q1 >= 1
+                    const uint32_t q1 = max(inclusive_max64(marks[g]), carry);
+                    carry = readlane_u(q1, 63);
+                    const uint32_t j = (uint32_t)__shfl((int)jbase, (int)(q1 - 1u), 64) + g;
+                    if (g < total) {
+                        pjm[k] = pos[j];
+                        ejm[k] = j | ((idt[j] >> kTypeShift) == 1u ? kBoundaryBit : 0u);
+                    }
                 }
             }
-            // the slots are read before the particle groups write their lists there
+            // the marks are read before the particle groups write their lists there
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             __builtin_amdgcn_wave_barrier();
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
