@@ -11,6 +11,8 @@
 //                     test dot(x_j - x_i) <= r^2, ascending lists  (:536-595)
 // All kernels take a scene table and a block map, so one launch serves any
 // number of scenes (batching) and a single scene alike.
+#include <type_traits>
+
 #include "fsa_device.hpp"
 #include "fsa_kernels.hpp"
 
@@ -405,6 +407,12 @@ __device__ __forceinline__ float readlane_f(float v, uint32_t lane) {
 __device__ __forceinline__ uint32_t readlane_u(uint32_t v, uint32_t lane) {
     return (uint32_t)__builtin_amdgcn_readlane((int)v, (int)lane);
 }
+// v with lane `lane` (uniform) replaced by the uniform value x (v_writelane_b32,
+// the lane select in M0)
+extern "C" __device__ int fsa_writelane_i32(int x, int lane, int v) __asm("llvm.amdgcn.writelane.i32");
+__device__ __forceinline__ uint32_t writelane_u(uint32_t v, uint32_t x, uint32_t lane) {
+    return (uint32_t)fsa_writelane_i32((int)x, (int)lane, (int)v);
+}
 
 // slot * stride for operands below 2^24: the full-rate 24-bit multiply (the
 // compiler would otherwise fold the byte shift into the multiplier and issue
@@ -649,17 +657,21 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) voi
                         }
                     }
                 }
+                // the chunks holding candidates, 1..kListChunks (the loop below is
+                // compiled for each count, so no chunk test is gated at run time)
                 const uint32_t left = total - c0;
+                const uint32_t nch = min((left + 63u) / 64u, kListChunks);
                 uint64_t fm = fmask;
-                while (fm) {
-                    const uint32_t b = (uint32_t)__ffsll((unsigned long long)fm) - 1u;
-                    fm &= fm - 1;
-                    const float xs = readlane_f(xl.x, b), ys = readlane_f(xl.y, b), zs = readlane_f(xl.z, b);
-                    uint32_t cnt = readlane_u(cntl, b);  // the particle's running count (uniform)
-                    uint32_t* const srow = stage + (readlane_u(kl, b) - fbase) * (uint32_t)kMaxNeighbors;
+                // the fluid lanes' ranks are r0, r0 + 1, ... in lane order: one stage row each
+                uint32_t* srow = stage + (r0 - fbase) * (uint32_t)kMaxNeighbors;
+                auto test_group = [&](auto nchunks) {
+                    while (fm) {
+                        const uint32_t b = (uint32_t)__ffsll((unsigned long long)fm) - 1u;
+                        fm &= fm - 1;
+                        const float xs = readlane_f(xl.x, b), ys = readlane_f(xl.y, b), zs = readlane_f(xl.z, b);
+                        uint32_t cnt = readlane_u(cntl, b);  // the particle's running count (uniform)
 #pragma unroll
-                    for (uint32_t k = 0; k < kListChunks; k++) {
-                        if (k * 64 < left) {
+                        for (uint32_t k = 0; k < decltype(nchunks)::value; k++) {
                             const float dx = pj[k].x - xs, dy = pj[k].y - ys, dz = pj[k].z - zs;
                             const bool pass = dot3(dx, dy, dz, dx, dy, dz) <= r2;
                             const uint64_t m = __ballot(pass);
@@ -673,9 +685,21 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) voi
                             }
                             cnt += (uint32_t)__popcll(m);
                         }
+                        cntl = writelane_u(cntl, cnt, b);
+                        // longest list of the gathered rows (uniform; counts only grow)
+                        frows = max(frows, min(cnt, (uint32_t)kMaxNeighbors));
+                        srow += kMaxNeighbors;
                     }
-                    if (lane == b) cntl = cnt;
-                }
+                };
+                static_assert(kListChunks == 4, "one specialisation per chunk count");
+                if (nch == 4)
+                    test_group(std::integral_constant<uint32_t, 4>{});
+                else if (nch == 3)
+                    test_group(std::integral_constant<uint32_t, 3>{});
+                else if (nch == 2)
+                    test_group(std::integral_constant<uint32_t, 2>{});
+                else
+                    test_group(std::integral_constant<uint32_t, 1>{});
             }
             const uint32_t kept = min(cntl, (uint32_t)kMaxNeighbors);
             if (isf) {
@@ -683,9 +707,6 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) voi
                 S.nbr_cnt[kl] = kept;
                 stage_cnt[kl - fbase] = kept;
             }
-            // longest list of the gathered rows (uniform)
-            for (uint64_t fm = fmask; fm; fm &= fm - 1)
-                frows = max(frows, readlane_u(kept, (uint32_t)__ffsll((unsigned long long)fm) - 1u));
             fcount = r0 + nfl - fbase;
         }
         if (!more) break;
