@@ -450,17 +450,44 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) voi
     // run starts among this wave's particles, with their cells' grid coordinates
     // and neighbor order (left by k_permute in the sort's idle buffers)
     const uint32_t mi = base + lane;
-    bool start = false;
+    bool start = false, fl = false;
     uint32_t cgrid = 0, corder = 0;
     if (mi < S.n) {
         const uint32_t mykey = keys[mi];
         start = mi == 0 || keys[mi - 1] != mykey;
+        fl = (idt[mi] >> kTypeShift) == 0u;
         if (start) {
             cgrid = S.keys[sbuf ^ 1u][mi];
             corder = S.vals[sbuf ^ 1u][mi];
         }
     }
-    uint64_t starts = __ballot(start);
+    // run starts and fluid particles among the next 64 particles, where the
+    // wave's last run ends (past the collection every run has ended)
+    const uint32_t mj = mi + 64u;
+    bool start2 = true, fl2 = false;
+    if (mj < S.n) {
+        start2 = keys[mj - 1u] != keys[mj];
+        fl2 = (idt[mj] >> kTypeShift) == 0u;
+    }
+    const uint64_t fluid = __ballot(fl), starts2 = __ballot(start2), fluid2 = __ballot(fl2);
+    // Only the runs holding a fluid particle need lists (boundary particles get
+    // none): the others are dropped before anything of theirs is loaded.  A run
+    // that does not end within the next 64 particles is kept.
+    uint64_t starts = 0;
+    for (uint64_t m = __ballot(start); m; m &= m - 1) {
+        const uint32_t s = (uint32_t)__ffsll((unsigned long long)m) - 1u;
+        const uint64_t from_s = ~0ull << s;
+        const uint64_t later = m & (from_s << 1);  // starts above s
+        bool any;
+        if (later) {
+            const uint32_t e = (uint32_t)__ffsll((unsigned long long)later) - 1u;
+            any = (fluid & from_s & ((1ull << e) - 1ull)) != 0;
+        } else {
+            any = (fluid & from_s) != 0 || starts2 == 0 ||
+                  (fluid2 & ((1ull << ((uint32_t)__ffsll((unsigned long long)starts2) - 1u)) - 1ull)) != 0;
+        }
+        if (any) starts |= 1ull << s;
+    }
     if (starts == 0) return;
     bool overflow = false;
     // readout lanes: particle column lane % kStageParticles, row offset lane / kStageParticles
