@@ -15,7 +15,7 @@ LIBDIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIBDIR, "libfluidsolver_amd.so")
 OBJDIR = os.path.join(LIBDIR, "obj")
 ARCH = os.environ.get("FSA_OFFLOAD_ARCH", "gfx950")
-SOURCES = ["fsa_search.hip", "fsa_sph.hip", "fsa_sph2d.hip", "fsa_entities.hip", "fsa_api.hip"]
+SOURCES = ["fsa_search.hip", "fsa_resort.hip", "fsa_sph.hip", "fsa_sph2d.hip", "fsa_entities.hip", "fsa_api.hip"]
 HOST_SOURCES = ["fsa_io.cpp"]  # host-only C++ (scenario files), built with g++
 HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-fvisibility=hidden", "-Wall", "-Wextra", "-Wno-unused-parameter"]
 

@@ -17,7 +17,6 @@
 #include <cstring>
 #include <exception>
 #include <map>
-#include <future>
 #include <numeric>
 #include <mutex>
 #include <new>
@@ -130,6 +129,18 @@ struct fsa_scene_s {
     // order the reference would hold (caller rows, inactive included)
     uint32_t hashed_calls = 0;
     std::vector<uint32_t> ref_order;
+    // hashed-3D re-sort on the device: scratch of the quick sort, and the
+    // collection order (rows) it leaves on the device; the host copy above is
+    // refreshed from it when an upload needs it
+    void* qs_mem = nullptr;
+    uint64_t qs_bytes = 0, qs_rows = 0;
+    QsBuffers qs{};
+    uint32_t* qs_ref[2] = {nullptr, nullptr};
+    uint32_t* qs_inv = nullptr;
+    uint32_t* qs_block_cnt = nullptr;
+    uint32_t qs_cur = 0;
+    bool qs_ref_valid = false;  // qs_ref[qs_cur] holds the current order
+    bool qs_ref_newer = false;  // ... and the host's ref_order is out of date
     bool lists_cached = false;
     bool lists_valid = false;  // lists of the last search (valid until the next search or upload)
     std::vector<uint64_t> csr_off, csr_cols;
@@ -750,68 +761,116 @@ void search3u_set(fsa_context ctx, const LaunchSet& L, uint32_t passes) {
 // libmorton morton3D_64_encode of the cell key (CompressedNeighbors.cpp:43-56,
 // HashedNeighborhoodSearch3D.cpp:236-244): coordinate + 2^31, x bits at 3k
 // (22 kept), y at 3k + 1 and z at 3k + 2 (21 kept)
-uint64_t reference_cell_key(int32_t cx, int32_t cy, int32_t cz) {
-    const uint32_t u[3] = {(uint32_t)((int64_t)cx - INT32_MIN), (uint32_t)((int64_t)cy - INT32_MIN),
-                           (uint32_t)((int64_t)cz - INT32_MIN)};
-    uint64_t key = 0;
-    for (int b = 0; b < 22; b++)
-        for (int a = 0; a < 3; a++) {
-            const int bit = 3 * b + a;
-            if (bit < 64) key |= (uint64_t)((u[a] >> b) & 1u) << bit;
-        }
-    return key;
-}
 
-// Sort::quick_sort (ParticleCollectionAlgorithm.hpp:28-127): Hoare partition
-// around the middle element's key; the permutation (position k takes old
-// position perm[k]) is what swapping every component would produce.  Like the
-// reference (which recurses with std::async for parts of 10000 and more,
-// :106-125), disjoint parts are sorted on their own threads; each part's
-// partitioning is the same, so the permutation does not depend on the threads.
-void quick_sort_part(uint64_t* key, uint32_t* perm, int64_t low, int64_t high, int depth) {
-    std::vector<std::pair<int64_t, int64_t>> todo;
-    todo.push_back({low, high});
-    while (!todo.empty()) {
-        const auto [lo, hi] = todo.back();
-        todo.pop_back();
-        if (lo >= hi) continue;
-        const uint64_t pivot = key[lo + (hi - lo) / 2];
-        int64_t i = lo - 1, j = hi + 1;
-        for (;;) {
-            do {
-                i++;
-            } while (key[i] < pivot);
-            do {
-                j--;
-            } while (key[j] > pivot);
-            if (i >= j) break;
-            std::swap(key[i], key[j]);
-            std::swap(perm[i], perm[j]);
+// Scratch of the device quick sort for `rows` collection rows (kept for the scene's life).
+void ensure_quick_sort(fsa_scene_s* s, uint64_t rows) {
+    if (s->qs_mem && s->qs_rows >= rows) return;
+    const uint64_t r = std::max<uint64_t>(rows, 1);
+    const uint64_t nseg = r / (kQsSmall + 1) + 2, nchunk = r / kQsChunk + nseg + 1, nsmall = r / 2 + 1;
+    const uint64_t nblk = (r + 255) / 256 + 1;
+    uint64_t bytes = 0;
+    auto add = [&](uint64_t sz) { bytes += align_up(sz, 256); };
+    add(8 * r);                        // key
+    for (int k = 0; k < 6; k++) add(4 * r);  // pay, apos, bpos, ref x2, inv
+    add(4 * nblk);                     // block counts
+    add(sizeof(QsSeg) * nseg);
+    add(sizeof(QsChunk) * nchunk);
+    add(sizeof(uint2) * nsmall);
+    if (s->ctx->budget && s->ctx->in_use - s->qs_bytes + bytes > s->ctx->budget)
+        throw FsaError(FSA_E_BUDGET, "the hashed search's re-sort does not fit the HBM budget");
+    void* m = nullptr;
+    if (hipMalloc(&m, bytes) != hipSuccess) throw FsaError(FSA_E_NOMEM, "device allocation failed");
+    if (s->qs_mem) {
+        // the order on the device moves to the new scratch
+        if (s->qs_ref_newer) {
+            s->ref_order.resize(s->qs_rows);
+            HIP_CHECK(hipMemcpyAsync(s->ref_order.data(), s->qs_ref[s->qs_cur], 4 * s->qs_rows, hipMemcpyDeviceToHost,
+                                     s->ctx->stream));
+            HIP_CHECK(hipStreamSynchronize(s->ctx->stream));
+            s->qs_ref_newer = false;
         }
-        if (depth > 0 && j - lo + 1 >= 10000 && hi - j >= 10000) {
-            auto lower = std::async(std::launch::async, quick_sort_part, key, perm, lo, j, depth - 1);
-            quick_sort_part(key, perm, j + 1, hi, depth - 1);
-            lower.get();
-        } else {
-            todo.push_back({lo, j});
-            todo.push_back({j + 1, hi});
-        }
+        (void)hipFree(s->qs_mem);
+        s->ctx->in_use -= s->qs_bytes;
     }
+    s->qs_mem = m;
+    s->qs_bytes = bytes;
+    s->qs_rows = r;
+    s->ctx->in_use += bytes;
+    char* base = (char*)m;
+    uint64_t off = 0;
+    auto take = [&](uint64_t sz) {
+        void* p = base + off;
+        off += align_up(sz, 256);
+        return p;
+    };
+    s->qs.key = (uint64_t*)take(8 * r);
+    s->qs.pay = (uint32_t*)take(4 * r);
+    s->qs.apos = (uint32_t*)take(4 * r);
+    s->qs.bpos = (uint32_t*)take(4 * r);
+    s->qs_ref[0] = (uint32_t*)take(4 * r);
+    s->qs_ref[1] = (uint32_t*)take(4 * r);
+    s->qs_inv = (uint32_t*)take(4 * r);
+    s->qs_block_cnt = (uint32_t*)take(4 * nblk);
+    s->qs.seg = (QsSeg*)take(sizeof(QsSeg) * nseg);
+    s->qs.chunk = (QsChunk*)take(sizeof(QsChunk) * nchunk);
+    s->qs.small = (uint2*)take(sizeof(uint2) * nsmall);
+    s->qs_cur = 0;
+    s->qs_ref_valid = false;
 }
 
-std::vector<uint32_t> quick_sort_order(std::vector<uint64_t> key) {
-    const size_t n = key.size();
-    std::vector<uint32_t> perm(n);
-    std::iota(perm.begin(), perm.end(), 0u);
-    if (n > 1) quick_sort_part(key.data(), perm.data(), 0, (int64_t)n - 1, 5);  // at most 32 parts at once
-    return perm;
+// Sort::quick_sort (ParticleCollectionAlgorithm.hpp:28-127) of qs.key / qs.pay
+// [0, rows): parts above kQsSmall elements are partitioned level by level on
+// the device (the host reads back each part's split, J), smaller parts sorted in
+// one block each at the end.
+void device_quick_sort(fsa_scene_s* s, uint64_t rows) {
+    hipStream_t st = s->ctx->stream;
+    std::vector<QsSeg> big, next;
+    std::vector<uint2> small;
+    std::vector<QsChunk> chunks;
+    auto add = [&](std::vector<QsSeg>& to, uint32_t lo, uint32_t hi) {
+        if (lo >= hi) return;
+        if (hi - lo + 1 <= kQsSmall)
+            small.push_back(make_uint2(lo, hi));
+        else
+            to.push_back(QsSeg{lo, hi, 0, 0, 0, 0, 0, 0});
+    };
+    add(big, 0, (uint32_t)(rows - 1));
+    while (!big.empty()) {
+        chunks.clear();
+        for (uint32_t i = 0; i < big.size(); i++) {
+            big[i].c0 = (uint32_t)chunks.size();
+            for (uint64_t lo = big[i].lo; lo <= big[i].hi; lo += kQsChunk)
+                chunks.push_back(QsChunk{i, (uint32_t)lo, (uint32_t)std::min<uint64_t>(big[i].hi, lo + kQsChunk - 1), 0,
+                                         0, 0, 0, 0});
+            big[i].c1 = (uint32_t)chunks.size();
+        }
+        HIP_CHECK(hipMemcpyAsync(s->qs.seg, big.data(), sizeof(QsSeg) * big.size(), hipMemcpyHostToDevice, st));
+        HIP_CHECK(hipMemcpyAsync(s->qs.chunk, chunks.data(), sizeof(QsChunk) * chunks.size(), hipMemcpyHostToDevice, st));
+        launch_qs_level(s->qs, (uint32_t)big.size(), (uint32_t)chunks.size(), st);
+        HIP_CHECK(hipMemcpyAsync(big.data(), s->qs.seg, sizeof(QsSeg) * big.size(), hipMemcpyDeviceToHost, st));
+        HIP_CHECK(hipStreamSynchronize(st));
+        next.clear();
+        for (const QsSeg& g : big) {
+            // a Hoare partition of [lo, hi] returns J in [lo, hi - 1]
+            if (g.J < g.lo || g.J >= g.hi) throw FsaError(FSA_E_DEVICE, "quick sort partition out of range (internal error)");
+            add(next, g.lo, g.J);
+            add(next, g.J + 1, g.hi);
+        }
+        big.swap(next);
+    }
+    if (!small.empty()) {
+        HIP_CHECK(hipMemcpyAsync(s->qs.small, small.data(), sizeof(uint2) * small.size(), hipMemcpyHostToDevice, st));
+        launch_qs_small(s->qs, (uint32_t)small.size(), st);
+        HIP_CHECK(hipStreamSynchronize(st));  // the host vector stays alive until the copy is done
+    }
 }
 
 // HashedNeighborhoodSearch3D::improve_cache_efficiency (HashedNeighborhoodSearch3D.cpp:219-247):
 // on every 101st call the collection, inactive particles included, is re-sorted
 // by the Morton key of each particle's cell from the previous grid update
 // (GridCellState.current; INT_MIN, key 0, for particles never updated).  The
-// device applies the active particles' share of that permutation.
+// device computes the reference quick sort's permutation over every row, keeps
+// the new collection order, and applies the resident rows' share of it.
 void hashed_resort_if_due(fsa_scene_s* s) {
     if (search_of(&s->desc) != FSA_SEARCH_HASHED_3D) return;
     s->hashed_calls++;
@@ -819,37 +878,25 @@ void hashed_resort_if_due(fsa_scene_s* s) {
     s->hashed_calls = 0;
     const uint32_t n = s->n;
     const uint64_t N = s->total;
+    if (N < 2) return;
     hipStream_t st = s->ctx->stream;
+    ensure_quick_sort(s, N);
     read_state(s);
-    std::vector<uint32_t> idt(n);
-    std::vector<int4> cells(n);
+    if (!s->qs_ref_valid) {
+        HIP_CHECK(hipMemcpyAsync(s->qs_ref[s->qs_cur], s->ref_order.data(), 4 * N, hipMemcpyHostToDevice, st));
+        s->qs_ref_valid = true;
+    }
+    launch_qs_keys(s->hd.idt[s->h_state->cur], n, s->hd.cell_last, s->qs_ref[s->qs_cur], (uint32_t)N, s->qs_inv,
+                   s->qs.key, s->qs.pay, st);
+    device_quick_sort(s, N);
+    launch_qs_apply(s->qs, s->qs_ref[s->qs_cur], (uint32_t)N, s->qs_inv, s->qs_ref[s->qs_cur ^ 1u], s->qs_block_cnt,
+                    s->hd.vals[0], st);
+    s->qs_cur ^= 1u;
+    s->qs_ref_newer = true;
     if (n) {
-        HIP_CHECK(hipMemcpyAsync(idt.data(), s->hd.idt[s->h_state->cur], 4ull * n, hipMemcpyDeviceToHost, st));
-        HIP_CHECK(hipMemcpyAsync(cells.data(), s->hd.cell_last, 16ull * n, hipMemcpyDeviceToHost, st));
-        HIP_CHECK(hipStreamSynchronize(st));
+        upload_scene_table(s);
+        launch_permute_only(s->single, 0u, st);
     }
-    const uint64_t never = reference_cell_key(INT32_MIN, INT32_MIN, INT32_MIN);
-    std::vector<uint64_t> key_of_row(N, never);
-    std::vector<uint32_t> pos_of_row(N, UINT32_MAX);
-    for (uint32_t k = 0; k < n; k++) {
-        const uint32_t row = idt[k] & kIdMask;
-        key_of_row[row] = reference_cell_key(cells[k].x, cells[k].y, cells[k].z);
-        pos_of_row[row] = k;
-    }
-    std::vector<uint64_t> keys(N);
-    for (uint64_t p = 0; p < N; p++) keys[p] = key_of_row[s->ref_order[p]];
-    const std::vector<uint32_t> perm = quick_sort_order(keys);
-    std::vector<uint32_t> next(N), src;
-    src.reserve(n);
-    for (uint64_t p = 0; p < N; p++) {
-        next[p] = s->ref_order[perm[p]];
-        if (pos_of_row[next[p]] != UINT32_MAX) src.push_back(pos_of_row[next[p]]);
-    }
-    s->ref_order.swap(next);
-    if (!n) return;
-    HIP_CHECK(hipMemcpyAsync(s->hd.vals[0], src.data(), 4ull * n, hipMemcpyHostToDevice, st));
-    upload_scene_table(s);
-    launch_permute_only(s->single, 0u, st);
     HIP_CHECK(hipStreamSynchronize(st));
 }
 
@@ -1631,6 +1678,16 @@ void do_upload(fsa_scene_s* s, const fsa_particle_view* v, bool first,
         s->ref_order.resize(N);
         std::iota(s->ref_order.begin(), s->ref_order.end(), 0u);
         s->hashed_calls = 0;
+        s->qs_ref_valid = false;
+        s->qs_ref_newer = false;
+    }
+    if (!first && hashed && s->qs_ref_newer) {
+        // the last re-sort left the collection order on the device
+        s->ref_order.resize(s->total);
+        HIP_CHECK(hipMemcpyAsync(s->ref_order.data(), s->qs_ref[s->qs_cur], 4ull * s->total, hipMemcpyDeviceToHost,
+                                 s->ctx->stream));
+        HIP_CHECK(hipStreamSynchronize(s->ctx->stream));
+        s->qs_ref_newer = false;
     }
     if (!first) {
         std::vector<uint32_t> order;
@@ -2028,6 +2085,10 @@ void destroy_scene(fsa_scene_s* s) {
     if (s->cells) {
         (void)hipFree(s->cells);
         s->ctx->in_use -= s->cells_cap * sizeof(uint4);
+    }
+    if (s->qs_mem) {
+        (void)hipFree(s->qs_mem);
+        s->ctx->in_use -= s->qs_bytes;
     }
     if (s->d_table) (void)hipFree(s->d_table);
     for (auto* m : s->map_allocs) (void)hipFree(m);

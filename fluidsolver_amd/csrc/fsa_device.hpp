@@ -408,6 +408,21 @@ __device__ __host__ __forceinline__ uint32_t compact_key(int32_t cx, int32_t cy,
 }
 
 
+// The reference's 64-bit cell key (CompressedNeighbors.cpp:43-56, also the
+// hashed search's re-sort key): libmorton's morton3D_64 of c + 2^31, bit b of
+// x, y, z at key bits 3b, 3b + 1, 3b + 2 (x keeps 22 bits, y and z 21).
+__device__ __host__ __forceinline__ uint64_t reference_cell_key(int32_t cx, int32_t cy, int32_t cz) {
+    const uint32_t u[3] = {(uint32_t)((int64_t)cx - INT32_MIN), (uint32_t)((int64_t)cy - INT32_MIN),
+                           (uint32_t)((int64_t)cz - INT32_MIN)};
+    uint64_t key = 0;
+    for (int b = 0; b < 22; b++)
+        for (int a = 0; a < 3; a++) {
+            const int bit = 3 * b + a;
+            if (bit < 64) key |= (uint64_t)((u[a] >> b) & 1u) << bit;
+        }
+    return key;
+}
+
 // Key order of a cell's 27 neighbor cells without sorting.  Along one axis the
 // compact keys of c-1, c, c+1 differ in two steps: one changes only coordinate
 // bit 0 (key bit 0, 1 or 2 for x, y, z), the other changes the coordinate bits

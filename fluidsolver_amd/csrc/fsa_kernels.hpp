@@ -69,6 +69,42 @@ void launch_codec(const LaunchSet& L, hipStream_t st);
 // stable LSD radix passes over (keys, vals)[first_src], digits from bit shift0 upward
 void launch_sort_range(const LaunchSet& L, uint32_t first_src, uint32_t shift0, uint32_t passes, hipStream_t st);
 
+// the hashed-3D search's re-sort (Sort::quick_sort) on the device: parts of more
+// than kQsSmall elements are partitioned level by level in chunks of kQsChunk,
+// smaller parts sorted in one block each
+constexpr uint32_t kQsChunk = 2048;
+constexpr uint32_t kQsSmall = 1024;
+struct QsSeg {
+    uint32_t lo, hi;  // the part [lo, hi]
+    uint32_t c0, c1;  // its chunks in the chunk table
+    uint32_t nA, nB;  // keys >= pivot, keys <= pivot
+    uint32_t K, J;    // swaps, returned index
+};
+struct QsChunk {
+    uint32_t seg, lo, hi;  // part and [lo, hi] of the chunk
+    uint32_t cA, cB;       // counts of keys >= pivot and <= pivot
+    uint32_t offA, offB;   // A items in the part's chunks before it, B items in those after it
+    uint32_t pad;
+};
+struct QsBuffers {
+    uint64_t* key;
+    uint32_t* pay;   // old collection position of each element
+    uint32_t* apos;  // [lo + t]: position of the part's t-th key >= pivot from the left
+    uint32_t* bpos;  // [lo + t]: position of the part's t-th key <= pivot from the right
+    QsSeg* seg;
+    QsChunk* chunk;
+    uint2* small;    // parts of at most kQsSmall elements: {lo, hi}
+};
+// keys of every collection position (rows in collection order ref[0..rows), the
+// resident rows' cells from the last grid update), old positions as payload
+void launch_qs_keys(const uint32_t* idt, uint32_t n, const int4* cell_last, const uint32_t* ref, uint32_t rows,
+                    uint32_t* inv, uint64_t* key, uint32_t* pay, hipStream_t st);
+void launch_qs_level(const QsBuffers& B, uint32_t nseg, uint32_t nchunk, hipStream_t st);
+void launch_qs_small(const QsBuffers& B, uint32_t nsmall, hipStream_t st);
+// the new collection order (ref_out) and the permute's source list of the resident rows
+void launch_qs_apply(const QsBuffers& B, const uint32_t* ref, uint32_t rows, const uint32_t* inv, uint32_t* ref_out,
+                     uint32_t* block_cnt, uint32_t* src, hipStream_t st);
+
 // IISPH / SESPH
 void launch_iisph_prepare(const LaunchSet& L, bool boundary_pass, hipStream_t st);
 void launch_iisph_diagonal(const LaunchSet& L, hipStream_t st);

@@ -144,3 +144,41 @@ def test_hashed_resort_of_a_large_collection(gpu_context, oracle_lib):
     np.testing.assert_array_equal(g.movement, r.movement)
     np.testing.assert_array_equal(g.particle_data, r.particle_data)
     sc.close()
+
+
+def _fluid_lists(off, cols, fl):
+    """The entries of rows fl of a CSR list array, concatenated, and their lengths."""
+    off = np.asarray(off, np.int64)
+    starts, lens = off[fl], off[fl + 1] - off[fl]
+    idx = np.repeat(starts - np.cumsum(lens) + lens, lens) + np.arange(int(lens.sum()))
+    return np.asarray(cols)[idx], lens
+
+
+@pytest.mark.parametrize("inactive", [False, True])
+def test_hashed_resort_on_the_device_large(gpu_context, oracle_lib, inactive):
+    """The 101st hashed search re-sorts a collection of about 250k rows on the
+    device (parts split level by level in chunks, the small parts sorted per
+    block): every fluid particle's list afterwards, entry for entry, equals the
+    oracle's, whose quick sort replays Sort::quick_sort.  Inactive rows take
+    part in the sort (their key is the never-updated cell's)."""
+    O = oracle_lib
+    coll = _shuffled(scenes.dam_scene((56, 56, 56), (80, 60, 60), 2, seed=65), 12)
+    if inactive:
+        t = coll.info["type"]
+        t[np.nonzero(t == abi.TYPE_NORMAL)[0][::13]] = abi.TYPE_INACTIVE
+    assert coll.size() > 200000
+    d = abi.default_desc(3, abi.SOLVER_IISPH)
+    d.search = abi.SEARCH_HASHED_3D
+    print("case hashed device resort", coll.size(), "inactive" if inactive else "all active", flush=True)
+    sc = Scene(gpu_context, d, coll.copy())
+    orc = O.OracleScene(d, coll.copy(), search=O.SEARCH_HASHED3D)
+    for _ in range(101):
+        sc.find_neighbors()
+        orc.find_neighbors()
+    assert not np.array_equal(orc.internal_ids(), np.arange(coll.size()))  # the re-sort happened
+    fl = np.nonzero(coll.type == abi.TYPE_NORMAL)[0]
+    gcols, glen = _fluid_lists(*sc.export_neighbors(), fl)
+    ocols, olen = _fluid_lists(*orc.export_neighbors(), fl)
+    np.testing.assert_array_equal(glen, olen)
+    np.testing.assert_array_equal(gcols, ocols)
+    sc.close()
