@@ -521,6 +521,23 @@ def main():
     log(json.dumps(fractions))
     sc.close()
 
+    resort = None
+    if args.search == "hashed":
+        # the hashed search re-sorts the collection on every 101st call: the
+        # wall time of each of 202 searches of a fresh scene (two re-sorts)
+        hs = Scene(ctx, desc, scenes.config2_dam3d(seed=2, side=side, interior=interior))
+        ctx.synchronize()
+        call_ms = []
+        for _ in range(202):
+            t0 = time.perf_counter()
+            hs.find_neighbors()
+            ctx.synchronize()
+            call_ms.append(1000.0 * (time.perf_counter() - t0))
+        hs.close()
+        resort = {"resort_call_ms": [round(call_ms[100], 3), round(call_ms[201], 3)],
+                  "other_calls_median_ms": round(float(np.median(call_ms[:100] + call_ms[101:201])), 3)}
+        log(f"hashed re-sort: {json.dumps(resort)}")
+
     full = None
     if args.full_steps > 0 and args.solver == "iisph":
         # the whole configuration: a fresh scene from rest, every step timed
@@ -597,6 +614,7 @@ def main():
         },
         "fractions": fractions,
         "full_run": full,
+        "hashed_resort": resort,
         "hbm_measured_GBps": round(hbm_gbps, 1),
         "roofline": roofline,
         "roofline_by_phase": roofline_by_phase,

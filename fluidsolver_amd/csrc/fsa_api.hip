@@ -762,20 +762,37 @@ void search3u_set(fsa_context ctx, const LaunchSet& L, uint32_t passes) {
 // HashedNeighborhoodSearch3D.cpp:236-244): coordinate + 2^31, x bits at 3k
 // (22 kept), y at 3k + 1 and z at 3k + 2 (21 kept)
 
+// table sizes of the device quick sort for r rows
+struct QsSizes {
+    uint64_t nseg, nchunk, nsmall, nblk;
+    explicit QsSizes(uint64_t r)
+        : nseg(r / (kQsSmall + 1) + 2), nchunk(r / kQsChunk + r / (kQsSmall + 1) + 3), nsmall(r / 2 + 1),
+          nblk((r + 255) / 256 + 1) {}
+};
+
+// device bytes of the hashed search's re-sort scratch for `rows` collection rows
+uint64_t quick_sort_bytes(uint64_t rows) {
+    const uint64_t r = std::max<uint64_t>(rows, 1);
+    const QsSizes z(r);
+    uint64_t bytes = 0;
+    auto add = [&](uint64_t sz) { bytes += align_up(sz, 256); };
+    add(8 * r);                              // key
+    for (int k = 0; k < 6; k++) add(4 * r);  // pay, apos, bpos, ref x2, inv
+    add(4 * z.nblk);                         // block counts
+    for (int k = 0; k < 2; k++) add(sizeof(QsSeg) * z.nseg);
+    for (int k = 0; k < 2; k++) add(sizeof(QsChunk) * z.nchunk);
+    add(sizeof(uint2) * z.nsmall);
+    add(sizeof(QsCtr) * 3);
+    return bytes;
+}
+
 // Scratch of the device quick sort for `rows` collection rows (kept for the scene's life).
 void ensure_quick_sort(fsa_scene_s* s, uint64_t rows) {
     if (s->qs_mem && s->qs_rows >= rows) return;
     const uint64_t r = std::max<uint64_t>(rows, 1);
-    const uint64_t nseg = r / (kQsSmall + 1) + 2, nchunk = r / kQsChunk + nseg + 1, nsmall = r / 2 + 1;
-    const uint64_t nblk = (r + 255) / 256 + 1;
-    uint64_t bytes = 0;
-    auto add = [&](uint64_t sz) { bytes += align_up(sz, 256); };
-    add(8 * r);                        // key
-    for (int k = 0; k < 6; k++) add(4 * r);  // pay, apos, bpos, ref x2, inv
-    add(4 * nblk);                     // block counts
-    add(sizeof(QsSeg) * nseg);
-    add(sizeof(QsChunk) * nchunk);
-    add(sizeof(uint2) * nsmall);
+    const QsSizes z(r);
+    const uint64_t nseg = z.nseg, nchunk = z.nchunk, nsmall = z.nsmall, nblk = z.nblk;
+    const uint64_t bytes = quick_sort_bytes(r);
     if (s->ctx->budget && s->ctx->in_use - s->qs_bytes + bytes > s->ctx->budget)
         throw FsaError(FSA_E_BUDGET, "the hashed search's re-sort does not fit the HBM budget");
     void* m = nullptr;
@@ -811,58 +828,57 @@ void ensure_quick_sort(fsa_scene_s* s, uint64_t rows) {
     s->qs_ref[1] = (uint32_t*)take(4 * r);
     s->qs_inv = (uint32_t*)take(4 * r);
     s->qs_block_cnt = (uint32_t*)take(4 * nblk);
-    s->qs.seg = (QsSeg*)take(sizeof(QsSeg) * nseg);
-    s->qs.chunk = (QsChunk*)take(sizeof(QsChunk) * nchunk);
+    for (int k = 0; k < 2; k++) s->qs.seg[k] = (QsSeg*)take(sizeof(QsSeg) * nseg);
+    for (int k = 0; k < 2; k++) s->qs.chunk[k] = (QsChunk*)take(sizeof(QsChunk) * nchunk);
     s->qs.small = (uint2*)take(sizeof(uint2) * nsmall);
+    s->qs.ctr = (QsCtr*)take(sizeof(QsCtr) * 3);
+    s->qs.seg_cap = (uint32_t)nseg;
+    s->qs.chunk_cap = (uint32_t)nchunk;
+    s->qs.small_cap = (uint32_t)nsmall;
     s->qs_cur = 0;
     s->qs_ref_valid = false;
 }
 
 // Sort::quick_sort (ParticleCollectionAlgorithm.hpp:28-127) of qs.key / qs.pay
 // [0, rows): parts above kQsSmall elements are partitioned level by level on
-// the device (the host reads back each part's split, J), smaller parts sorted in
-// one block each at the end.
+// the device, each level's split writing the next level's tables; the host
+// checks every kQsLevelsPerCheck levels whether parts are left, then the small
+// parts are sorted one per lane.
+constexpr uint32_t kQsLevelsPerCheck = 4;
+
 void device_quick_sort(fsa_scene_s* s, uint64_t rows) {
     hipStream_t st = s->ctx->stream;
-    std::vector<QsSeg> big, next;
-    std::vector<uint2> small;
+    const uint32_t hi = (uint32_t)(rows - 1);
+    QsCtr ctr[3] = {{0u, 0u}, {0u, 0u}, {0u, 0u}};
     std::vector<QsChunk> chunks;
-    auto add = [&](std::vector<QsSeg>& to, uint32_t lo, uint32_t hi) {
-        if (lo >= hi) return;
-        if (hi - lo + 1 <= kQsSmall)
-            small.push_back(make_uint2(lo, hi));
-        else
-            to.push_back(QsSeg{lo, hi, 0, 0, 0, 0, 0, 0});
-    };
-    add(big, 0, (uint32_t)(rows - 1));
-    while (!big.empty()) {
-        chunks.clear();
-        for (uint32_t i = 0; i < big.size(); i++) {
-            big[i].c0 = (uint32_t)chunks.size();
-            for (uint64_t lo = big[i].lo; lo <= big[i].hi; lo += kQsChunk)
-                chunks.push_back(QsChunk{i, (uint32_t)lo, (uint32_t)std::min<uint64_t>(big[i].hi, lo + kQsChunk - 1), 0,
-                                         0, 0, 0, 0});
-            big[i].c1 = (uint32_t)chunks.size();
-        }
-        HIP_CHECK(hipMemcpyAsync(s->qs.seg, big.data(), sizeof(QsSeg) * big.size(), hipMemcpyHostToDevice, st));
-        HIP_CHECK(hipMemcpyAsync(s->qs.chunk, chunks.data(), sizeof(QsChunk) * chunks.size(), hipMemcpyHostToDevice, st));
-        launch_qs_level(s->qs, (uint32_t)big.size(), (uint32_t)chunks.size(), st);
-        HIP_CHECK(hipMemcpyAsync(big.data(), s->qs.seg, sizeof(QsSeg) * big.size(), hipMemcpyDeviceToHost, st));
+    QsSeg top{0u, hi, 0u, 0u, 0u, 0u, 0u, 0u};
+    uint2 part = make_uint2(0u, hi);
+    if (rows > kQsSmall) {
+        for (uint64_t lo = 0; lo <= hi; lo += kQsChunk)
+            chunks.push_back(QsChunk{0u, (uint32_t)lo, (uint32_t)std::min<uint64_t>(hi, lo + kQsChunk - 1), 0u, 0u, 0u,
+                                     0u, 0u});
+        top.c1 = (uint32_t)chunks.size();
+        ctr[0] = QsCtr{1u, (uint32_t)chunks.size()};
+        HIP_CHECK(hipMemcpyAsync(s->qs.seg[0], &top, sizeof(QsSeg), hipMemcpyHostToDevice, st));
+        HIP_CHECK(hipMemcpyAsync(s->qs.chunk[0], chunks.data(), sizeof(QsChunk) * chunks.size(), hipMemcpyHostToDevice,
+                                 st));
+    } else {
+        ctr[2] = QsCtr{1u, 0u};
+        HIP_CHECK(hipMemcpyAsync(s->qs.small, &part, sizeof(uint2), hipMemcpyHostToDevice, st));
+    }
+    HIP_CHECK(hipMemcpyAsync(s->qs.ctr, ctr, sizeof(ctr), hipMemcpyHostToDevice, st));
+    uint32_t cur = 0;
+    uint64_t levels = 0;
+    while (ctr[cur].nseg != 0) {
+        for (uint32_t k = 0; k < kQsLevelsPerCheck; k++, cur ^= 1u) launch_qs_level(s->qs, cur, st);
+        levels += kQsLevelsPerCheck;
+        HIP_CHECK(hipMemcpyAsync(ctr, s->qs.ctr, sizeof(ctr), hipMemcpyDeviceToHost, st));
         HIP_CHECK(hipStreamSynchronize(st));
-        next.clear();
-        for (const QsSeg& g : big) {
-            // a Hoare partition of [lo, hi] returns J in [lo, hi - 1]
-            if (g.J < g.lo || g.J >= g.hi) throw FsaError(FSA_E_DEVICE, "quick sort partition out of range (internal error)");
-            add(next, g.lo, g.J);
-            add(next, g.J + 1, g.hi);
-        }
-        big.swap(next);
+        // every level splits each part, so rows levels always suffice
+        if (ctr[2].nchunk != 0 || levels > rows + kQsLevelsPerCheck)
+            throw FsaError(FSA_E_DEVICE, "the hashed search's re-sort failed (internal error)");
     }
-    if (!small.empty()) {
-        HIP_CHECK(hipMemcpyAsync(s->qs.small, small.data(), sizeof(uint2) * small.size(), hipMemcpyHostToDevice, st));
-        launch_qs_small(s->qs, (uint32_t)small.size(), st);
-        HIP_CHECK(hipStreamSynchronize(st));  // the host vector stays alive until the copy is done
-    }
+    launch_qs_small(s->qs, ctr[2].nseg, st);
 }
 
 // HashedNeighborhoodSearch3D::improve_cache_efficiency (HashedNeighborhoodSearch3D.cpp:219-247):
@@ -1680,6 +1696,7 @@ void do_upload(fsa_scene_s* s, const fsa_particle_view* v, bool first,
         s->hashed_calls = 0;
         s->qs_ref_valid = false;
         s->qs_ref_newer = false;
+        ensure_quick_sort(s, N);  // the re-sort's scratch, allocated up front
     }
     if (!first && hashed && s->qs_ref_newer) {
         // the last re-sort left the collection order on the device
@@ -2350,6 +2367,8 @@ FSA_API int32_t fsa_scene_memory_estimate(const fsa_scene_desc* desc, uint64_t n
         *bytes = plan_bytes(make_plan(n_fluid + n_boundary, n_fluid, desc->dimension,
                                       search == FSA_SEARCH_HASHED_3D || search == FSA_SEARCH_QUADRATIC_3D)) +
                  sizeof(SceneDev);
+        // the hashed search's re-sort scratch (for the active rows; inactive rows add to it)
+        if (search == FSA_SEARCH_HASHED_3D) *bytes += quick_sort_bytes(n_fluid + n_boundary);
     });
 }
 

@@ -70,10 +70,11 @@ void launch_codec(const LaunchSet& L, hipStream_t st);
 void launch_sort_range(const LaunchSet& L, uint32_t first_src, uint32_t shift0, uint32_t passes, hipStream_t st);
 
 // the hashed-3D search's re-sort (Sort::quick_sort) on the device: parts of more
-// than kQsSmall elements are partitioned level by level in chunks of kQsChunk,
-// smaller parts sorted in one block each
+// than kQsSmall elements are partitioned level by level in chunks of kQsChunk
+// (each level's split writing the next level's tables), smaller parts sorted one
+// per lane at the end
 constexpr uint32_t kQsChunk = 2048;
-constexpr uint32_t kQsSmall = 1024;
+constexpr uint32_t kQsSmall = 64;
 struct QsSeg {
     uint32_t lo, hi;  // the part [lo, hi]
     uint32_t c0, c1;  // its chunks in the chunk table
@@ -86,20 +87,28 @@ struct QsChunk {
     uint32_t offA, offB;   // A items in the part's chunks before it, B items in those after it
     uint32_t pad;
 };
+// counters: [0], [1] the parts and chunks of the level tables 0 and 1; [2] the
+// small parts and an error flag (a table would overflow: internal error)
+struct QsCtr {
+    uint32_t nseg, nchunk;
+};
 struct QsBuffers {
     uint64_t* key;
     uint32_t* pay;   // old collection position of each element
     uint32_t* apos;  // [lo + t]: position of the part's t-th key >= pivot from the left
     uint32_t* bpos;  // [lo + t]: position of the part's t-th key <= pivot from the right
-    QsSeg* seg;
-    QsChunk* chunk;
+    QsSeg* seg[2];   // level tables, alternating
+    QsChunk* chunk[2];
     uint2* small;    // parts of at most kQsSmall elements: {lo, hi}
+    QsCtr* ctr;      // 3 entries
+    uint32_t seg_cap, chunk_cap, small_cap;
 };
 // keys of every collection position (rows in collection order ref[0..rows), the
 // resident rows' cells from the last grid update), old positions as payload
 void launch_qs_keys(const uint32_t* idt, uint32_t n, const int4* cell_last, const uint32_t* ref, uint32_t rows,
                     uint32_t* inv, uint64_t* key, uint32_t* pay, hipStream_t st);
-void launch_qs_level(const QsBuffers& B, uint32_t nseg, uint32_t nchunk, hipStream_t st);
+// one level: the parts of table cur are partitioned, their children go to table cur ^ 1 or the small list
+void launch_qs_level(const QsBuffers& B, uint32_t cur, hipStream_t st);
 void launch_qs_small(const QsBuffers& B, uint32_t nsmall, hipStream_t st);
 // the new collection order (ref_out) and the permute's source list of the resident rows
 void launch_qs_apply(const QsBuffers& B, const uint32_t* ref, uint32_t rows, const uint32_t* inv, uint32_t* ref_out,

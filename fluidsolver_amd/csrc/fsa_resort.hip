@@ -21,10 +21,15 @@
 //   k_qs_ranks   per chunk: A[rank] and B[rank] for every element
 //   k_qs_split   per part: K (a binary search, the predicate is monotone) and J
 //   k_qs_swap    per chunk of ranks: the K swaps, in place (disjoint pairs)
-// Parts split level by level until they hold at most kQsSmall elements; those
-// are sorted by the reference's own sequential partition in LDS, one block each
-// (disjoint parts do not interact, so the order in which parts are done does not
-// change the permutation).
+// Parts split level by level until they hold at most kQsSmall elements.  Each
+// level's split writes the next level's part and chunk tables itself (atomic
+// appends: the parts are disjoint, so their order in a table does not matter,
+// and a part's chunks take one contiguous range), so the host only checks every
+// few levels whether any part is left.  The small parts are then sorted by the
+// reference's own sequential partition, one part per lane (disjoint parts do not
+// interact, so the order in which parts are done does not change the permutation).
+#include <algorithm>
+
 #include "fsa_device.hpp"
 #include "fsa_kernels.hpp"
 
@@ -91,205 +96,249 @@ __device__ __forceinline__ uint64_t qs_pivot(const uint64_t* key, const QsSeg& s
     return key[s.lo + (s.hi - s.lo) / 2];
 }
 
-__global__ __launch_bounds__(kQsThreads) void k_qs_count(QsBuffers B) {
-    QsChunk& c = B.chunk[blockIdx.x];
-    const QsSeg s = B.seg[c.seg];
-    const uint64_t P = qs_pivot(B.key, s);
-    uint32_t a = 0, b = 0;
-    for (uint32_t x = c.lo + threadIdx.x; x <= c.hi; x += kQsThreads) {
-        const uint64_t k = B.key[x];
-        a += k >= P ? 1u : 0u;
-        b += k <= P ? 1u : 0u;
-    }
+__global__ __launch_bounds__(kQsThreads) void k_qs_count(QsBuffers B, uint32_t cur) {
+    // the next level's table starts empty (the level before used it last)
+    if (blockIdx.x == 0 && threadIdx.x == 0) B.ctr[cur ^ 1u] = QsCtr{0u, 0u};
+    const uint32_t nchunk = B.ctr[cur].nchunk;
     __shared__ uint32_t lw[kQsThreads / 64];
-    uint32_t ta, tb;
-    (void)qs_block_scan(a, lw, ta);
-    (void)qs_block_scan(b, lw, tb);
-    if (threadIdx.x == 0) {
-        c.cA = ta;
-        c.cB = tb;
+    for (uint32_t ci = blockIdx.x; ci < nchunk; ci += gridDim.x) {
+        QsChunk& c = B.chunk[cur][ci];
+        const QsSeg s = B.seg[cur][c.seg];
+        const uint64_t P = qs_pivot(B.key, s);
+        uint32_t a = 0, b = 0;
+        for (uint32_t x = c.lo + threadIdx.x; x <= c.hi; x += kQsThreads) {
+            const uint64_t k = B.key[x];
+            a += k >= P ? 1u : 0u;
+            b += k <= P ? 1u : 0u;
+        }
+        uint32_t ta, tb;
+        (void)qs_block_scan(a, lw, ta);
+        (void)qs_block_scan(b, lw, tb);
+        if (threadIdx.x == 0) {
+            c.cA = ta;
+            c.cB = tb;
+        }
     }
 }
 
 // per part: A offsets of its chunks from the left, B offsets from the right
-__global__ __launch_bounds__(kQsThreads) void k_qs_offsets(QsBuffers B) {
-    QsSeg& s = B.seg[blockIdx.x];
+__global__ __launch_bounds__(kQsThreads) void k_qs_offsets(QsBuffers B, uint32_t cur) {
+    const uint32_t nseg = B.ctr[cur].nseg;
     __shared__ uint32_t lw[kQsThreads / 64];
-    uint32_t carry_a = 0;
-    for (uint32_t c0 = s.c0; c0 < s.c1; c0 += kQsThreads) {
-        const uint32_t c = c0 + threadIdx.x;
-        const uint32_t v = c < s.c1 ? B.chunk[c].cA : 0u;
-        uint32_t tot;
-        const uint32_t ex = qs_block_scan(v, lw, tot);
-        if (c < s.c1) B.chunk[c].offA = carry_a + ex;
-        carry_a += tot;
-    }
-    uint32_t carry_b = 0;  // B elements in the chunks after the current window
-    const uint32_t nc = s.c1 - s.c0;
-    for (uint32_t r0 = 0; r0 < nc; r0 += kQsThreads) {
-        // window of chunks taken from the right: thread t holds chunk c1 - 1 - (r0 + t)
-        const uint32_t r = r0 + threadIdx.x;
-        const uint32_t c = s.c1 - 1u - r;
-        const uint32_t v = r < nc ? B.chunk[c].cB : 0u;
-        uint32_t tot;
-        const uint32_t ex = qs_block_scan(v, lw, tot);
-        if (r < nc) B.chunk[c].offB = carry_b + ex;
-        carry_b += tot;
-    }
-    if (threadIdx.x == 0) {
-        s.nA = carry_a;
-        s.nB = carry_b;
-    }
-}
-
-// A[rank] and B[rank] of every element of the chunk
-__global__ __launch_bounds__(kQsThreads) void k_qs_ranks(QsBuffers B) {
-    const QsChunk c = B.chunk[blockIdx.x];
-    const QsSeg s = B.seg[c.seg];
-    const uint64_t P = qs_pivot(B.key, s);
-    const uint32_t x0 = c.lo + threadIdx.x * kQsPerThread;
-    uint32_t fa = 0, fb = 0;  // flags of the thread's items
-    uint32_t na = 0, nb = 0;
-#pragma unroll
-    for (uint32_t q = 0; q < kQsPerThread; q++) {
-        const uint32_t x = x0 + q;
-        if (x <= c.hi) {
-            const uint64_t k = B.key[x];
-            if (k >= P) {
-                fa |= 1u << q;
-                na++;
-            }
-            if (k <= P) {
-                fb |= 1u << q;
-                nb++;
-            }
+    for (uint32_t si = blockIdx.x; si < nseg; si += gridDim.x) {
+        QsSeg& s = B.seg[cur][si];
+        QsChunk* const ch = B.chunk[cur];
+        uint32_t carry_a = 0;
+        for (uint32_t c0 = s.c0; c0 < s.c1; c0 += kQsThreads) {
+            const uint32_t c = c0 + threadIdx.x;
+            const uint32_t v = c < s.c1 ? ch[c].cA : 0u;
+            uint32_t tot;
+            const uint32_t ex = qs_block_scan(v, lw, tot);
+            if (c < s.c1) ch[c].offA = carry_a + ex;
+            carry_a += tot;
+        }
+        uint32_t carry_b = 0;  // B elements in the chunks right of the current window
+        const uint32_t nc = s.c1 - s.c0;
+        for (uint32_t r0 = 0; r0 < nc; r0 += kQsThreads) {
+            // a window of chunks taken from the right: thread t holds chunk c1 - 1 - (r0 + t)
+            const uint32_t r = r0 + threadIdx.x;
+            const uint32_t c = s.c1 - 1u - r;
+            const uint32_t v = r < nc ? ch[c].cB : 0u;
+            uint32_t tot;
+            const uint32_t ex = qs_block_scan(v, lw, tot);
+            if (r < nc) ch[c].offB = carry_b + ex;
+            carry_b += tot;
+        }
+        if (threadIdx.x == 0) {
+            s.nA = carry_a;
+            s.nB = carry_b;
         }
     }
+}
+
+// A[rank] and B[rank] of every element of each chunk
+__global__ __launch_bounds__(kQsThreads) void k_qs_ranks(QsBuffers B, uint32_t cur) {
+    const uint32_t nchunk = B.ctr[cur].nchunk;
     __shared__ uint32_t lw[kQsThreads / 64];
-    uint32_t ta, tb;
-    uint32_t ra = c.offA + qs_block_scan(na, lw, ta);
-    // B ranks count from the right: the B items after this thread's in the chunk,
-    // plus the chunk's offset
-    const uint32_t before_b = qs_block_scan(nb, lw, tb);
-    uint32_t rb = c.offB + (tb - before_b - nb);
+    for (uint32_t ci = blockIdx.x; ci < nchunk; ci += gridDim.x) {
+        const QsChunk c = B.chunk[cur][ci];
+        const QsSeg s = B.seg[cur][c.seg];
+        const uint64_t P = qs_pivot(B.key, s);
+        const uint32_t x0 = c.lo + threadIdx.x * kQsPerThread;
+        uint32_t fa = 0, fb = 0;  // flags of the thread's items
+        uint32_t na = 0, nb = 0;
 #pragma unroll
-    for (uint32_t q = 0; q < kQsPerThread; q++) {
-        const uint32_t x = x0 + q;
-        if (fa & (1u << q)) B.apos[s.lo + ra++] = x;
-    }
-    // this thread's B items from its last to its first
+        for (uint32_t q = 0; q < kQsPerThread; q++) {
+            const uint32_t x = x0 + q;
+            if (x <= c.hi) {
+                const uint64_t k = B.key[x];
+                if (k >= P) {
+                    fa |= 1u << q;
+                    na++;
+                }
+                if (k <= P) {
+                    fb |= 1u << q;
+                    nb++;
+                }
+            }
+        }
+        uint32_t ta, tb;
+        uint32_t ra = c.offA + qs_block_scan(na, lw, ta);
+        // B ranks count from the right: the B items after this thread's in the
+        // chunk, plus the chunk's offset
+        const uint32_t before_b = qs_block_scan(nb, lw, tb);
+        uint32_t rb = c.offB + (tb - before_b - nb);
 #pragma unroll
-    for (int q = (int)kQsPerThread - 1; q >= 0; q--) {
-        const uint32_t x = x0 + (uint32_t)q;
-        if (fb & (1u << q)) B.bpos[s.lo + rb++] = x;
+        for (uint32_t q = 0; q < kQsPerThread; q++)
+            if (fa & (1u << q)) B.apos[s.lo + ra++] = x0 + q;
+        // this thread's B items from its last to its first
+#pragma unroll
+        for (int q = (int)kQsPerThread - 1; q >= 0; q--)
+            if (fb & (1u << q)) B.bpos[s.lo + rb++] = x0 + (uint32_t)q;
     }
 }
 
-// per part (one thread each): the number of swaps K and the returned index J
-__global__ __launch_bounds__(64) void k_qs_split(QsBuffers B, uint32_t nseg) {
-    const uint32_t i = blockIdx.x * 64 + threadIdx.x;
-    if (i >= nseg) return;
-    QsSeg& s = B.seg[i];
-    const uint32_t* A = B.apos + s.lo;
-    const uint32_t* Bp = B.bpos + s.lo;
-    // first t in [0, m) with A[t] >= B[t], or m
-    uint32_t lo = 0, hi = min(s.nA, s.nB);
-    while (lo < hi) {
-        const uint32_t mid = lo + (hi - lo) / 2;
-        if (A[mid] < Bp[mid])
-            lo = mid + 1;
+// a child part of the split: into the next level's tables, or the small list
+__device__ __forceinline__ void qs_child(const QsBuffers& B, uint32_t nxt, uint32_t lo, uint32_t hi) {
+    if (lo >= hi) return;
+    if (hi - lo + 1u <= kQsSmall) {
+        const uint32_t q = atomicAdd(&B.ctr[2].nseg, 1u);
+        if (q < B.small_cap)
+            B.small[q] = make_uint2(lo, hi);
         else
-            hi = mid;
+            atomicOr(&B.ctr[2].nchunk, 1u);
+        return;
     }
-    const uint32_t K = lo;
-    uint32_t J;
-    if (K == 0)
-        J = Bp[0];  // the pivot's position is in B
-    else if (K < s.nB)
-        J = max(Bp[K], A[K - 1]);
-    else
-        J = A[K - 1];
-    s.K = K;
-    s.J = J;
+    const uint32_t nch = (hi - lo) / kQsChunk + 1u;
+    const uint32_t si = atomicAdd(&B.ctr[nxt].nseg, 1u);
+    const uint32_t c0 = atomicAdd(&B.ctr[nxt].nchunk, nch);
+    if (si >= B.seg_cap || c0 + nch > B.chunk_cap) {
+        atomicOr(&B.ctr[2].nchunk, 1u);
+        return;
+    }
+    B.seg[nxt][si] = QsSeg{lo, hi, c0, c0 + nch, 0u, 0u, 0u, 0u};
+    for (uint32_t q = 0; q < nch; q++) {
+        const uint32_t clo = lo + q * kQsChunk;
+        B.chunk[nxt][c0 + q] = QsChunk{si, clo, min(hi, clo + kQsChunk - 1u), 0u, 0u, 0u, 0u, 0u};
+    }
+}
+
+// per part (one thread each): the number of swaps K and the returned index J;
+// the two halves become parts of the next level (or small parts)
+__global__ __launch_bounds__(64) void k_qs_split(QsBuffers B, uint32_t cur) {
+    const uint32_t nseg = B.ctr[cur].nseg;
+    for (uint32_t i = blockIdx.x * 64 + threadIdx.x; i < nseg; i += gridDim.x * 64) {
+        QsSeg& s = B.seg[cur][i];
+        const uint32_t* A = B.apos + s.lo;
+        const uint32_t* Bp = B.bpos + s.lo;
+        // first t in [0, m) with A[t] >= B[t], or m
+        uint32_t lo = 0, hi = min(s.nA, s.nB);
+        while (lo < hi) {
+            const uint32_t mid = lo + (hi - lo) / 2;
+            if (A[mid] < Bp[mid])
+                lo = mid + 1;
+            else
+                hi = mid;
+        }
+        const uint32_t K = lo;
+        uint32_t J;
+        if (K == 0)
+            J = Bp[0];  // the pivot's position is in B
+        else if (K < s.nB)
+            J = max(Bp[K], A[K - 1]);
+        else
+            J = A[K - 1];
+        s.K = K;
+        s.J = J;
+        if (J < s.lo || J >= s.hi) {
+            // a Hoare partition of [lo, hi] returns J in [lo, hi - 1] (internal error)
+            atomicOr(&B.ctr[2].nchunk, 1u);
+            s.K = 0;
+            continue;
+        }
+        qs_child(B, cur ^ 1u, s.lo, J);
+        qs_child(B, cur ^ 1u, J + 1u, s.hi);
+    }
 }
 
 // the first K rank pairs of each part swap their elements
-__global__ __launch_bounds__(kQsThreads) void k_qs_swap(QsBuffers B) {
-    const QsChunk c = B.chunk[blockIdx.x];
-    const QsSeg s = B.seg[c.seg];
-    for (uint32_t x = c.lo + threadIdx.x; x <= c.hi; x += kQsThreads) {
-        const uint32_t t = x - s.lo;
-        if (t >= s.K) break;
-        const uint32_t pa = B.apos[s.lo + t], pb = B.bpos[s.lo + t];
-        const uint64_t ka = B.key[pa], kb = B.key[pb];
-        const uint32_t va = B.pay[pa], vb = B.pay[pb];
-        B.key[pa] = kb;
-        B.key[pb] = ka;
-        B.pay[pa] = vb;
-        B.pay[pb] = va;
+__global__ __launch_bounds__(kQsThreads) void k_qs_swap(QsBuffers B, uint32_t cur) {
+    const uint32_t nchunk = B.ctr[cur].nchunk;
+    for (uint32_t ci = blockIdx.x; ci < nchunk; ci += gridDim.x) {
+        const QsChunk c = B.chunk[cur][ci];
+        const QsSeg s = B.seg[cur][c.seg];
+        for (uint32_t x = c.lo + threadIdx.x; x <= c.hi; x += kQsThreads) {
+            const uint32_t t = x - s.lo;
+            if (t >= s.K) break;
+            const uint32_t pa = B.apos[s.lo + t], pb = B.bpos[s.lo + t];
+            const uint64_t ka = B.key[pa], kb = B.key[pb];
+            const uint32_t va = B.pay[pa], vb = B.pay[pb];
+            B.key[pa] = kb;
+            B.key[pb] = ka;
+            B.pay[pa] = vb;
+            B.pay[pb] = va;
+        }
     }
 }
 
-// parts of at most kQsSmall elements: the reference's sequential quick sort
-// (partition around the middle key, both halves in turn) in LDS, by one thread;
-// the other threads load and store
-__global__ __launch_bounds__(64) void k_qs_small(QsBuffers B) {
-    const uint2 part = B.small[blockIdx.x];
-    const uint32_t base = part.x, n = part.y - part.x + 1;
-    __shared__ uint64_t key[kQsSmall];
-    __shared__ uint32_t pay[kQsSmall];
-    __shared__ uint32_t stack[2 * 64];
-    for (uint32_t q = threadIdx.x; q < n; q += 64) {
-        key[q] = B.key[base + q];
-        pay[q] = B.pay[base + q];
+// parts of at most kQsSmall elements, one per lane: the reference's sequential
+// quick sort (partition around the middle key, the smaller half first, the
+// larger one waiting on the lane's stack) in LDS, element i of lane l at i * 64 + l
+__global__ __launch_bounds__(64) void k_qs_small(QsBuffers B, uint32_t nsmall) {
+    __shared__ uint64_t key[kQsSmall * 64];
+    __shared__ uint32_t pay[kQsSmall * 64];
+    __shared__ uint32_t stack[8 * 64];  // lo | hi << 16; at most log2(kQsSmall) + 1 parts wait
+    static_assert(kQsSmall <= 128, "a small part's stack fits 8 entries");
+    const uint32_t l = threadIdx.x;
+    const uint32_t p = blockIdx.x * 64 + l;
+    if (p >= nsmall) return;
+    const uint2 part = B.small[p];
+    const uint32_t base = part.x;
+    const int32_t n = (int32_t)(part.y - part.x + 1u);
+    for (int32_t q = 0; q < n; q++) {
+        key[q * 64 + l] = B.key[base + q];
+        pay[q * 64 + l] = B.pay[base + q];
     }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        // the smaller half is taken next and the larger one waits on the stack,
-        // so at most log2(kQsSmall) + 1 parts wait
-        uint32_t sp = 0;
-        int32_t lo = 0, hi = (int32_t)n - 1;
-        for (;;) {
-            if (lo >= hi) {
-                if (sp == 0) break;
-                sp--;
-                lo = (int32_t)stack[2 * sp];
-                hi = (int32_t)stack[2 * sp + 1];
-                continue;
-            }
-            const uint64_t P = key[lo + (hi - lo) / 2];
-            int32_t i = lo - 1, j = hi + 1;
-            for (;;) {
-                do {
-                    i++;
-                } while (key[i] < P);
-                do {
-                    j--;
-                } while (key[j] > P);
-                if (i >= j) break;
-                const uint64_t tk = key[i];
-                key[i] = key[j];
-                key[j] = tk;
-                const uint32_t tv = pay[i];
-                pay[i] = pay[j];
-                pay[j] = tv;
-            }
-            if (j - lo < hi - j) {
-                stack[2 * sp] = (uint32_t)(j + 1);
-                stack[2 * sp + 1] = (uint32_t)hi;
-                hi = j;
-            } else {
-                stack[2 * sp] = (uint32_t)lo;
-                stack[2 * sp + 1] = (uint32_t)j;
-                lo = j + 1;
-            }
-            sp++;
+    uint32_t sp = 0;
+    int32_t lo = 0, hi = n - 1;
+    for (;;) {
+        if (lo >= hi) {
+            if (sp == 0) break;
+            sp--;
+            const uint32_t e = stack[sp * 64 + l];
+            lo = (int32_t)(e & 0xffffu);
+            hi = (int32_t)(e >> 16);
+            continue;
         }
+        const uint64_t P = key[(lo + (hi - lo) / 2) * 64 + l];
+        int32_t i = lo - 1, j = hi + 1;
+        for (;;) {
+            do {
+                i++;
+            } while (key[i * 64 + l] < P);
+            do {
+                j--;
+            } while (key[j * 64 + l] > P);
+            if (i >= j) break;
+            const uint64_t tk = key[i * 64 + l];
+            key[i * 64 + l] = key[j * 64 + l];
+            key[j * 64 + l] = tk;
+            const uint32_t tv = pay[i * 64 + l];
+            pay[i * 64 + l] = pay[j * 64 + l];
+            pay[j * 64 + l] = tv;
+        }
+        if (j - lo < hi - j) {
+            stack[sp * 64 + l] = (uint32_t)(j + 1) | (uint32_t)hi << 16;
+            hi = j;
+        } else {
+            stack[sp * 64 + l] = (uint32_t)lo | (uint32_t)j << 16;
+            lo = j + 1;
+        }
+        sp++;
     }
-    __syncthreads();
-    for (uint32_t q = threadIdx.x; q < n; q += 64) {
-        B.key[base + q] = key[q];
-        B.pay[base + q] = pay[q];
+    for (int32_t q = 0; q < n; q++) {
+        B.key[base + q] = key[q * 64 + l];
+        B.pay[base + q] = pay[q * 64 + l];
     }
 }
 
@@ -349,16 +398,20 @@ void launch_qs_keys(const uint32_t* idt, uint32_t n, const int4* cell_last, cons
     if (gr) k_qs_keys<<<gr, kQsThreads, 0, st>>>(ref, inv, cell_last, rows, key, pay);
 }
 
-void launch_qs_level(const QsBuffers& B, uint32_t nseg, uint32_t nchunk, hipStream_t st) {
-    k_qs_count<<<nchunk, kQsThreads, 0, st>>>(B);
-    k_qs_offsets<<<nseg, kQsThreads, 0, st>>>(B);
-    k_qs_ranks<<<nchunk, kQsThreads, 0, st>>>(B);
-    k_qs_split<<<(nseg + 63) / 64, 64, 0, st>>>(B, nseg);
-    k_qs_swap<<<nchunk, kQsThreads, 0, st>>>(B);
+// Grids of a level are fixed (the counts live on the device); blocks loop over
+// the level's chunks and parts.
+void launch_qs_level(const QsBuffers& B, uint32_t cur, hipStream_t st) {
+    const uint32_t gc = std::max(1u, std::min(B.chunk_cap, 2048u));
+    const uint32_t gs = std::max(1u, std::min(B.seg_cap, 1024u));
+    k_qs_count<<<gc, kQsThreads, 0, st>>>(B, cur);
+    k_qs_offsets<<<gs, kQsThreads, 0, st>>>(B, cur);
+    k_qs_ranks<<<gc, kQsThreads, 0, st>>>(B, cur);
+    k_qs_split<<<std::max(1u, std::min((B.seg_cap + 63) / 64, 512u)), 64, 0, st>>>(B, cur);
+    k_qs_swap<<<gc, kQsThreads, 0, st>>>(B, cur);
 }
 
 void launch_qs_small(const QsBuffers& B, uint32_t nsmall, hipStream_t st) {
-    if (nsmall) k_qs_small<<<nsmall, 64, 0, st>>>(B);
+    if (nsmall) k_qs_small<<<(nsmall + 63) / 64, 64, 0, st>>>(B, nsmall);
 }
 
 void launch_qs_apply(const QsBuffers& B, const uint32_t* ref, uint32_t rows, const uint32_t* inv, uint32_t* ref_out,
