@@ -284,6 +284,8 @@ def bench_served(ctx, window_s, rate, max_running, budget_gb, buckets=12):
         "p99_step_ms": st["p99_step_ms"],
         "mean_step_ms": st["mean_step_ms"],
         "busy_fraction": st["busy_s"] / elapsed,
+        "tick_seconds": {"admit": round(st["admit_s"], 3), "step": round(st["step_s"], 3),
+                         "retire": round(st["retire_s"], 3), "window": round(elapsed, 3)},
         "offered_particle_updates_per_s": offered / window_s,
         "sustained_particle_updates_per_s": st["particle_updates"] / elapsed,
         "jobs_queued_at_end": st["jobs_queued"],

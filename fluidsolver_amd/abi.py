@@ -199,6 +199,9 @@ class ServerStats(C.Structure):
         ("p99_step_ms", C.c_double),
         ("mean_step_ms", C.c_double),
         ("max_step_ms", C.c_double),
+        ("admit_s", C.c_double),
+        ("step_s", C.c_double),
+        ("retire_s", C.c_double),
     ]
 
     def as_dict(self):

@@ -2164,6 +2164,7 @@ struct fsa_server_s {
     std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
     uint64_t ticks = 0, scene_steps = 0, particle_updates = 0, submitted = 0, done = 0, failed = 0;
     double busy_s = 0.0;
+    double admit_s = 0.0, step_s = 0.0, retire_s = 0.0;  // parts of busy_s and of retirement
     std::vector<std::pair<double, uint32_t>> step_ms;  // tick latency, scenes stepped
     double now() const {
         return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
@@ -2230,6 +2231,7 @@ void destroy_server(fsa_server srv) {
 }
 
 void retire(fsa_server srv, Job* j, uint32_t state, uint32_t status) {
+    const double t0 = srv->now();
     if (j->scene) {
         if (state == FSA_JOB_DONE) {
             fsa_particle_view v = j->view();
@@ -2250,6 +2252,7 @@ void retire(fsa_server srv, Job* j, uint32_t state, uint32_t status) {
         j->external.clear();
         j->info.clear();
     }
+    srv->retire_s += srv->now() - t0;
 }
 
 }  // namespace
@@ -2951,6 +2954,8 @@ FSA_API int32_t fsa_server_tick(fsa_server srv, uint32_t* running) {
         if (!srv) throw FsaError(FSA_E_INVALID, "server is NULL");
         const double t0 = srv->now();
         admit(srv);
+        const double ta = srv->now();
+        srv->admit_s += ta - t0;
         // jobs with no steps left (steps == 0) finish without a device step
         for (uint64_t id : std::vector<uint64_t>(srv->running)) {
             Job* j = srv->jobs[id];
@@ -2967,8 +2972,10 @@ FSA_API int32_t fsa_server_tick(fsa_server srv, uint32_t* running) {
         std::vector<fsa_step_stats> st(scenes.size());
         constexpr uint32_t kNotStepped = 0xffffffffu;
         for (auto& e : st) e.status = kNotStepped;
+        const double ts = srv->now();
         const int32_t rc = fsa_batch_step(srv->ctx, scenes.data(), (uint32_t)scenes.size(), 1, st.data());
         const double t1 = srv->now();
+        srv->step_s += t1 - ts;
         srv->ticks++;
         srv->busy_s += t1 - t0;
         srv->step_ms.push_back({1000.0 * (t1 - t0), (uint32_t)scenes.size()});
@@ -3047,6 +3054,9 @@ FSA_API int32_t fsa_server_get_stats(fsa_server srv, fsa_server_stats* out) {
         out->jobs_running = srv->running.size();
         out->hbm_in_use = srv->ctx->in_use.load();
         out->busy_s = srv->busy_s;
+        out->admit_s = srv->admit_s;
+        out->step_s = srv->step_s;
+        out->retire_s = srv->retire_s;
         // percentiles over scene-steps: a tick's latency counts once per scene it stepped
         std::vector<std::pair<double, uint32_t>> v = srv->step_ms;
         std::sort(v.begin(), v.end());

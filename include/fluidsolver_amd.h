@@ -304,6 +304,7 @@ typedef struct fsa_server_stats {
     uint64_t hbm_in_use;
     double busy_s;               /* time spent inside ticks */
     double p50_step_ms, p99_step_ms, mean_step_ms, max_step_ms;  /* per scene-step latency */
+    double admit_s, step_s, retire_s;  /* time inside ticks spent admitting, stepping and retiring jobs */
 } fsa_server_stats;
 
 typedef struct fsa_server_s* fsa_server;
