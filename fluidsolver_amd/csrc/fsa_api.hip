@@ -22,6 +22,7 @@
 #include <new>
 #include <stdexcept>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/fluidsolver_amd.h"
@@ -88,6 +89,10 @@ struct fsa_context_s {
     };
     std::map<uint64_t, Tuned> tuned;
     void* stat_scratch = nullptr;  // this device's address of the no-allocation statistics scratch
+    // pinned host staging of uploads and downloads, grown as needed (one user at a time)
+    void* pin = nullptr;
+    uint64_t pin_bytes = 0;
+    std::mutex pin_mu;
 };
 
 // Launch set over several scenes (one table entry and block-map range each).
@@ -1461,6 +1466,40 @@ int32_t batch_step_group(fsa_context ctx, const std::vector<fsa_scene_s*>& all, 
 // ---- upload / download ------------------------------------------------------------------
 // 2D: every particle on the device in the caller's order, z = 0
 // (MovementData {vec2 position, velocity, acceleration}, ExternalForces {vec2})
+// The context's pinned staging buffer of at least `bytes` (the caller holds pin_mu).
+char* pinned_staging(fsa_context ctx, uint64_t bytes) {
+    if (ctx->pin_bytes < bytes) {
+        const uint64_t want = std::max<uint64_t>(bytes, ctx->pin_bytes + ctx->pin_bytes / 2);
+        if (ctx->pin) HIP_CHECK(hipHostFree(ctx->pin));
+        ctx->pin = nullptr;
+        ctx->pin_bytes = 0;
+        if (hipHostMalloc(&ctx->pin, want, hipHostMallocDefault) != hipSuccess) {
+            ctx->pin = nullptr;
+            throw FsaError(FSA_E_NOMEM, "pinned host allocation failed");
+        }
+        ctx->pin_bytes = want;
+    }
+    return (char*)ctx->pin;
+}
+
+// f(lo, hi, part) over [0, n) in up to kHostParts contiguous parts on host
+// threads (one part below kHostSerial items); parts write disjoint outputs.
+constexpr unsigned kHostParts = 8;
+constexpr uint64_t kHostSerial = 1u << 16;
+template <class F>
+void host_parallel(uint64_t n, F&& f) {
+    unsigned parts = std::min(kHostParts, std::max(1u, std::thread::hardware_concurrency()));
+    parts = (unsigned)std::min<uint64_t>(parts, std::max<uint64_t>(1, n / (kHostSerial / 2)));
+    if (n < kHostSerial || parts <= 1) {
+        f((uint64_t)0, n, 0u);
+        return;
+    }
+    std::vector<std::thread> th;
+    for (unsigned t = 1; t < parts; t++) th.emplace_back([&, t] { f(n * t / parts, n * (t + 1) / parts, t); });
+    f((uint64_t)0, n / parts, 0u);
+    for (auto& x : th) x.join();
+}
+
 void do_upload2d(fsa_scene_s* s, const fsa_particle_view* v, bool first) {
     const uint64_t N = v->count;
     if (N >= (1ull << kTypeShift)) throw FsaError(FSA_E_INVALID, "too many particles (max 2^30)");
@@ -1734,40 +1773,70 @@ void do_upload(fsa_scene_s* s, const fsa_particle_view* v, bool first,
     s->nf = nf;
     s->nb = nb;
     const uint32_t n = s->n;
-    std::vector<float4> pos(n), vel(n), acc(n), ext(n);
-    std::vector<float> pres(n), zero(n, 0.0f);
-    std::vector<uint32_t> idt(n);
+    // The device arrays are packed in the context's pinned staging (pos, vel, acc,
+    // ext, pres, err, idt), the rows split over host threads, each with its own
+    // flags and cell bounds.
+    std::unique_lock<std::mutex> pin_lock(s->ctx->pin_mu);
+    // 4 x 16 B + 3 x 4 B per particle
+    char* stg = pinned_staging(s->ctx, 76ull * std::max<uint32_t>(n, 1));
+    float4* const pos = (float4*)stg;
+    float4* const vel = pos + n;
+    float4* const acc = vel + n;
+    float4* const ext = acc + n;
+    float* const pres = (float*)(ext + n);
+    float* const errs = pres + n;
+    uint32_t* const idt = (uint32_t*)(errs + n);
+    struct Part {
+        bool any_ext = false, dyn_boundary = false, any_tiny = false;
+        int32_t lo[3] = {INT32_MAX, INT32_MAX, INT32_MAX}, hi[3] = {INT32_MIN, INT32_MIN, INT32_MIN};
+    };
+    Part parts[kHostParts];
+    const float r = s->desc.particle_size * 2.0f;
+    const bool carry = s->carry_err.size() == N;
+    host_parallel(n, [&](uint64_t k0, uint64_t k1, unsigned part) {
+        Part P;
+        for (uint64_t k = k0; k < k1; k++) {
+            const uint64_t i = active[k];
+            const float* m = (const float*)((const char*)v->movement + i * v->movement_stride);
+            const float* pd = (const float*)((const char*)v->particle_data + i * v->particle_data_stride);
+            const uint8_t t = ((const uint8_t*)v->info + i * v->info_stride)[4];
+            float e[3] = {0.0f, 0.0f, 0.0f};
+            if (v->external) {
+                const float* ep = (const float*)((const char*)v->external + i * v->external_stride);
+                e[0] = ep[0];
+                e[1] = ep[1];
+                e[2] = ep[2];
+            }
+            pos[k] = make_float4(m[0], m[1], m[2], pd[0]);
+            vel[k] = make_float4(m[3], m[4], m[5], pd[2]);
+            acc[k] = make_float4(m[6], m[7], m[8], 0.0f);
+            ext[k] = make_float4(e[0], e[1], e[2], 0.0f);
+            pres[k] = pd[1];
+            errs[k] = carry ? s->carry_err[i] : 0.0f;
+            idt[k] = (uint32_t)i | ((uint32_t)t << kTypeShift);
+            if (e[0] != 0.0f || e[1] != 0.0f || e[2] != 0.0f) P.any_ext = true;
+            if (tiny_coordinate(m[0]) || tiny_coordinate(m[1]) || tiny_coordinate(m[2])) P.any_tiny = true;
+            if (t == FSA_TYPE_BOUNDARY &&
+                (m[3] != 0.0f || m[4] != 0.0f || m[5] != 0.0f || m[6] != 0.0f || m[7] != 0.0f || m[8] != 0.0f ||
+                 e[0] != 0.0f || e[1] != 0.0f || e[2] != 0.0f))
+                P.dyn_boundary = true;
+            const int32_t c[3] = {host_cell(m[0], r), host_cell(m[1], r), host_cell(m[2], r)};
+            for (int a = 0; a < 3; a++) {
+                P.lo[a] = std::min(P.lo[a], c[a]);
+                P.hi[a] = std::max(P.hi[a], c[a]);
+            }
+        }
+        parts[part] = P;
+    });
     bool any_ext = false, dyn_boundary = false, any_tiny = false;
     int32_t lo[3] = {INT32_MAX, INT32_MAX, INT32_MAX}, hi[3] = {INT32_MIN, INT32_MIN, INT32_MIN};
-    const float r = s->desc.particle_size * 2.0f;
-    for (uint32_t k = 0; k < n; k++) {
-        const uint64_t i = active[k];
-        const float* m = (const float*)((const char*)v->movement + i * v->movement_stride);
-        const float* pd = (const float*)((const char*)v->particle_data + i * v->particle_data_stride);
-        const uint8_t t = ((const uint8_t*)v->info + i * v->info_stride)[4];
-        float e[3] = {0.0f, 0.0f, 0.0f};
-        if (v->external) {
-            const float* ep = (const float*)((const char*)v->external + i * v->external_stride);
-            e[0] = ep[0];
-            e[1] = ep[1];
-            e[2] = ep[2];
-        }
-        pos[k] = make_float4(m[0], m[1], m[2], pd[0]);
-        vel[k] = make_float4(m[3], m[4], m[5], pd[2]);
-        acc[k] = make_float4(m[6], m[7], m[8], 0.0f);
-        ext[k] = make_float4(e[0], e[1], e[2], 0.0f);
-        pres[k] = pd[1];
-        idt[k] = (uint32_t)i | ((uint32_t)t << kTypeShift);
-        if (e[0] != 0.0f || e[1] != 0.0f || e[2] != 0.0f) any_ext = true;
-        if (tiny_coordinate(m[0]) || tiny_coordinate(m[1]) || tiny_coordinate(m[2])) any_tiny = true;
-        if (t == FSA_TYPE_BOUNDARY &&
-            (m[3] != 0.0f || m[4] != 0.0f || m[5] != 0.0f || m[6] != 0.0f || m[7] != 0.0f || m[8] != 0.0f ||
-             e[0] != 0.0f || e[1] != 0.0f || e[2] != 0.0f))
-            dyn_boundary = true;
-        const int32_t c[3] = {host_cell(m[0], r), host_cell(m[1], r), host_cell(m[2], r)};
+    for (const Part& P : parts) {
+        any_ext = any_ext || P.any_ext;
+        dyn_boundary = dyn_boundary || P.dyn_boundary;
+        any_tiny = any_tiny || P.any_tiny;
         for (int a = 0; a < 3; a++) {
-            lo[a] = std::min(lo[a], c[a]);
-            hi[a] = std::max(hi[a], c[a]);
+            lo[a] = std::min(lo[a], P.lo[a]);
+            hi[a] = std::max(hi[a], P.hi[a]);
         }
     }
     // inactive rows are carried on the host unchanged (every solver loop skips them)
@@ -1811,20 +1880,18 @@ void do_upload(fsa_scene_s* s, const fsa_particle_view* v, bool first,
         init.bbox_max[a] = n ? hi[a] : 0;
     }
     if (n) {
-        HIP_CHECK(hipMemcpyAsync(h.pos[0], pos.data(), 16ull * n, hipMemcpyHostToDevice, st));
-        HIP_CHECK(hipMemcpyAsync(h.vel[0], vel.data(), 16ull * n, hipMemcpyHostToDevice, st));
-        HIP_CHECK(hipMemcpyAsync(h.acc[0], acc.data(), 16ull * n, hipMemcpyHostToDevice, st));
-        HIP_CHECK(hipMemcpyAsync(h.ext[0], ext.data(), 16ull * n, hipMemcpyHostToDevice, st));
-        HIP_CHECK(hipMemcpyAsync(h.pres[0], pres.data(), 4ull * n, hipMemcpyHostToDevice, st));
-        std::vector<float> err(n, 0.0f);
-        if (s->carry_err.size() == N)
-            for (uint32_t k = 0; k < n; k++) err[k] = s->carry_err[idt[k] & kIdMask];
-        HIP_CHECK(hipMemcpyAsync(h.err[0], err.data(), 4ull * n, hipMemcpyHostToDevice, st));
-        HIP_CHECK(hipMemcpyAsync(h.idt[0], idt.data(), 4ull * n, hipMemcpyHostToDevice, st));
+        HIP_CHECK(hipMemcpyAsync(h.pos[0], pos, 16ull * n, hipMemcpyHostToDevice, st));
+        HIP_CHECK(hipMemcpyAsync(h.vel[0], vel, 16ull * n, hipMemcpyHostToDevice, st));
+        HIP_CHECK(hipMemcpyAsync(h.acc[0], acc, 16ull * n, hipMemcpyHostToDevice, st));
+        HIP_CHECK(hipMemcpyAsync(h.ext[0], ext, 16ull * n, hipMemcpyHostToDevice, st));
+        HIP_CHECK(hipMemcpyAsync(h.pres[0], pres, 4ull * n, hipMemcpyHostToDevice, st));
+        HIP_CHECK(hipMemcpyAsync(h.err[0], errs, 4ull * n, hipMemcpyHostToDevice, st));
+        HIP_CHECK(hipMemcpyAsync(h.idt[0], idt, 4ull * n, hipMemcpyHostToDevice, st));
     }
     s->carry_err.clear();
     HIP_CHECK(hipMemcpyAsync(h.state, &init, sizeof(SceneState), hipMemcpyHostToDevice, st));
     HIP_CHECK(hipStreamSynchronize(st));
+    pin_lock.unlock();  // the staging copies are done
     *s->h_state = init;
     s->searched = false;
     s->boundary_dirty = true;
@@ -1866,8 +1933,82 @@ void fetch(fsa_scene_s* s, HostCopy& hc, bool with_err) {
     HIP_CHECK(hipStreamSynchronize(st));
 }
 
+// 3D download: the live arrays into the context's pinned staging, then the
+// rows written in caller order on host threads (each device particle owns its row)
+void do_download3d(fsa_scene_s* s, const fsa_particle_view* v) {
+    read_state(s);
+    const uint32_t c = s->h_state->cur;
+    const uint32_t n = s->n;
+    const bool ext_live = (s->hd.flags & kFlagExternal) != 0;
+    const SceneDev& h = s->hd;
+    hipStream_t st = s->ctx->stream;
+    std::unique_lock<std::mutex> pin_lock(s->ctx->pin_mu);
+    // 4 x 16 B + 2 x 4 B per particle
+    char* stg = pinned_staging(s->ctx, 72ull * std::max<uint32_t>(n, 1));
+    float4* const pos = (float4*)stg;
+    float4* const vel = pos + n;
+    float4* const acc = vel + n;
+    float4* const ext = acc + n;
+    float* const pres = (float*)(ext + n);
+    uint32_t* const idt = (uint32_t*)(pres + n);
+    if (n) {
+        HIP_CHECK(hipMemcpyAsync(pos, h.pos[c], 16ull * n, hipMemcpyDeviceToHost, st));
+        HIP_CHECK(hipMemcpyAsync(vel, h.vel[c], 16ull * n, hipMemcpyDeviceToHost, st));
+        HIP_CHECK(hipMemcpyAsync(acc, h.acc[c], 16ull * n, hipMemcpyDeviceToHost, st));
+        if (ext_live && v->external) HIP_CHECK(hipMemcpyAsync(ext, h.ext[c], 16ull * n, hipMemcpyDeviceToHost, st));
+        HIP_CHECK(hipMemcpyAsync(pres, h.pres[c], 4ull * n, hipMemcpyDeviceToHost, st));
+        HIP_CHECK(hipMemcpyAsync(idt, h.idt[c], 4ull * n, hipMemcpyDeviceToHost, st));
+    }
+    HIP_CHECK(hipStreamSynchronize(st));
+    host_parallel(n, [&](uint64_t k0, uint64_t k1, unsigned) {
+        for (uint64_t k = k0; k < k1; k++) {
+            const uint64_t i = idt[k] & kIdMask;
+            const uint8_t t = (uint8_t)(idt[k] >> kTypeShift);
+            if (v->movement) {
+                float* m = (float*)((char*)v->movement + i * v->movement_stride);
+                m[0] = pos[k].x;
+                m[1] = pos[k].y;
+                m[2] = pos[k].z;
+                m[3] = vel[k].x;
+                m[4] = vel[k].y;
+                m[5] = vel[k].z;
+                m[6] = acc[k].x;
+                m[7] = acc[k].y;
+                m[8] = acc[k].z;
+            }
+            if (v->particle_data) {
+                float* pd = (float*)((char*)v->particle_data + i * v->particle_data_stride);
+                pd[0] = pos[k].w;
+                pd[1] = pres[k];
+                pd[2] = vel[k].w;
+            }
+            if (v->info) ((uint8_t*)v->info + i * v->info_stride)[4] = t;
+            if (v->external) {
+                float* e = (float*)((char*)v->external + i * v->external_stride);
+                e[0] = ext_live ? ext[k].x : 0.0f;
+                e[1] = ext_live ? ext[k].y : 0.0f;
+                e[2] = ext_live ? ext[k].z : 0.0f;
+            }
+        }
+    });
+    pin_lock.unlock();
+    for (size_t k = 0; k < s->inactive_rows.size(); k++) {
+        const uint64_t i = s->inactive_rows[k];
+        if (v->movement) std::memcpy((char*)v->movement + i * v->movement_stride, &s->inactive_movement[9 * k], 36);
+        if (v->particle_data)
+            std::memcpy((char*)v->particle_data + i * v->particle_data_stride, &s->inactive_pdata[3 * k], 12);
+        if (v->info) std::memcpy((char*)v->info + i * v->info_stride, &s->inactive_info[8 * k], 8);
+        if (v->external)
+            std::memcpy((char*)v->external + i * v->external_stride, &s->inactive_ext[3 * k], 12);
+    }
+}
+
 void do_download(fsa_scene_s* s, const fsa_particle_view* v) {
     if (!v || v->count != s->total) throw FsaError(FSA_E_INVALID, "download view count mismatch");
+    if (s->desc.dimension == 3) {
+        do_download3d(s, v);
+        return;
+    }
     HostCopy hc;
     fetch(s, hc, false);
     if (s->desc.dimension == 2) {
@@ -2341,6 +2482,7 @@ FSA_API int32_t fsa_context_destroy(fsa_context c) {
         c->batches.clear();
         if (c->d_states) (void)hipFree(c->d_states);
         if (c->h_states) (void)hipHostFree(c->h_states);
+        if (c->pin) (void)hipHostFree(c->pin);
         for (auto& e : c->ev_pool) (void)hipEventDestroy(e);
         (void)hipStreamDestroy(c->stream);
         delete c;
