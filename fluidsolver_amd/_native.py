@@ -57,6 +57,7 @@ def _declare(L):
         "fsa_batch_set_graph": (i32, [vp, u32]),
         "fsa_context_set_block_size": (i32, [vp, u32]),
         "fsa_context_set_autotune": (i32, [vp, u32]),
+        "fsa_context_set_walk_variant": (i32, [vp, u32]),
         "fsa_context_get_tuning": (i32, [vp, C.c_char_p, u64, P(u64)]),
         "fsa_context_set_timing": (i32, [vp, u32]),
         "fsa_context_get_timing": (i32, [vp, P(C.c_double), P(u64), u32]),

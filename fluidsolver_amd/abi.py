@@ -28,6 +28,14 @@ STATUS_NAMES = {
     FSA_E_IO: "FSA_E_IO",
 }
 
+# neighbor-walk variants of the 3D IISPH kernels (autotuner candidates)
+WALK_U2 = 0x02
+WALK_U4 = 0x04
+WALK_U8 = 0x08
+WALK_U4_W8 = 0x84
+WALK_U8_W6 = 0x68
+WALK_VARIANTS = (WALK_U2, WALK_U4, WALK_U8, WALK_U4_W8, WALK_U8_W6)
+
 SOLVER_IISPH = 0
 SOLVER_SESPH = 1
 LIST_COMPRESSED = 0

@@ -80,14 +80,15 @@ struct fsa_context_s {
     std::map<std::vector<fsa_scene_s*>, struct BatchSet*> batches;
     uint64_t batch_clock = 0;
     std::vector<struct fsa_server_s*> servers;  // destroyed with the context
-    // autotuned neighbor-walk unroll per size class, with the timings behind each choice
+    // autotuned neighbor-walk variant per size class, with the timings behind each choice
     bool autotune = true;
     struct Tuned {
-        std::array<uint8_t, 4> unroll;
-        std::array<std::array<float, 3>, 4> us;  // per family, per candidate (2, 4, 8)
+        std::array<uint8_t, 4> unroll;  // kVariant* per family
+        std::array<std::array<float, kTuneVariants>, 4> us;  // per family, per kTuneVariant entry
         uint64_t fluid;
     };
     std::map<uint64_t, Tuned> tuned;
+    uint8_t walk_variant = 0;  // a kVariant* forced on every family, or 0: the autotuner's choice
     void* stat_scratch = nullptr;  // this device's address of the no-allocation statistics scratch
     // pinned host staging of uploads and downloads, grown as needed (one user at a time)
     void* pin = nullptr;
@@ -1090,6 +1091,8 @@ void upload_batch_table(fsa_context ctx, BatchSet* b) {
     b->ls.spec = spec;
     b->ls.codec = codec;
     b->ls.wide_lists = wide;
+    if (ctx->walk_variant)
+        for (int f = 0; f < kTuneFamilies; f++) b->ls.unroll[f] = ctx->walk_variant;
     HIP_CHECK(hipMemcpyAsync(b->d_table, b->h_table.data(), sizeof(SceneDev) * b->h_table.size(),
                              hipMemcpyHostToDevice, ctx->stream));
 }
@@ -1201,12 +1204,11 @@ void autotune(fsa_context ctx, BatchSet* b, const std::vector<fsa_scene_s*>& v) 
     fsa_context_s::Tuned t{};
     t.fluid = 0;
     for (auto* s : v) t.fluid += s->nf;
-    const uint8_t cand[3] = {2, 4, 8};
     for (int f = 0; f < kTuneFamilies; f++) {
         float best = 0.0f;
-        uint8_t pick = 4;
-        for (int c = 0; c < 3; c++) {
-            L.unroll[f] = cand[c];
+        uint8_t pick = kVariantU4;
+        for (int c = 0; c < kTuneVariants; c++) {
+            L.unroll[f] = kTuneVariant[c];
             launch_tuned_kernel(L, f, st);  // warm-up
             HIP_CHECK(hipEventRecord(e0, st));
             for (int r = 0; r < 3; r++) launch_tuned_kernel(L, f, st);
@@ -1217,7 +1219,7 @@ void autotune(fsa_context ctx, BatchSet* b, const std::vector<fsa_scene_s*>& v) 
             t.us[f][c] = 1000.0f * ms / 3.0f;
             if (c == 0 || t.us[f][c] < best) {
                 best = t.us[f][c];
-                pick = cand[c];
+                pick = kTuneVariant[c];
             }
         }
         L.unroll[f] = pick;
@@ -1396,7 +1398,7 @@ int32_t batch_step_group(fsa_context ctx, const std::vector<fsa_scene_s*>& all, 
         for (auto* s : v) s->hd.search_count++;
     read_states(ctx, v);
     if (!b->tuned) {
-        if (!d2 && !sesph && ctx->autotune) autotune(ctx, b, v);
+        if (!d2 && !sesph && ctx->autotune && !ctx->walk_variant) autotune(ctx, b, v);
         b->tuned = true;
     }
     if (!sesph) {
@@ -2969,6 +2971,21 @@ FSA_API int32_t fsa_context_set_autotune(fsa_context c, uint32_t enabled) {
     });
 }
 
+FSA_API int32_t fsa_context_set_walk_variant(fsa_context c, uint32_t variant) {
+    return guarded([&] {
+        if (!c) throw FsaError(FSA_E_INVALID, "context is NULL");
+        if (variant && std::find(kTuneVariant, kTuneVariant + kTuneVariants, variant) == kTuneVariant + kTuneVariants)
+            throw FsaError(FSA_E_INVALID, "walk variant is none of the autotuner's candidates");
+        c->walk_variant = (uint8_t)variant;
+        // cached batches follow: a forced variant at their next step, else a fresh choice
+        for (auto& kv : c->batches) {
+            if (variant) continue;
+            for (int f = 0; f < kTuneFamilies; f++) kv.second->ls.unroll[f] = kVariantU4;
+            kv.second->tuned = false;
+        }
+    });
+}
+
 FSA_API int32_t fsa_context_get_tuning(fsa_context c, char* buf, uint64_t cap, uint64_t* len) {
     return guarded([&] {
         if (!c || !len) throw FsaError(FSA_E_INVALID, "NULL argument");
@@ -2981,9 +2998,19 @@ FSA_API int32_t fsa_context_get_tuning(fsa_context c, char* buf, uint64_t cap, u
                      (unsigned long long)kv.second.fluid);
             out += line;
             for (int f = 0; f < kTuneFamilies; f++) {
-                snprintf(line, sizeof(line), " %s=U%u [U2 %.1f us, U4 %.1f us, U8 %.1f us]", names[f],
-                         (unsigned)kv.second.unroll[f], kv.second.us[f][0], kv.second.us[f][1], kv.second.us[f][2]);
-                out += line;
+                // a variant reads U<unroll> or U<unroll>W<waves per SIMD hint>
+                auto name = [](uint8_t v) {
+                    std::string r = "U" + std::to_string(v & 15u);
+                    if (v >> 4) r += "W" + std::to_string(v >> 4);
+                    return r;
+                };
+                out += " " + std::string(names[f]) + "=" + name(kv.second.unroll[f]) + " [";
+                for (int c = 0; c < kTuneVariants; c++) {
+                    snprintf(line, sizeof(line), "%s%s %.1f us", c ? ", " : "", name(kTuneVariant[c]).c_str(),
+                             kv.second.us[f][c]);
+                    out += line;
+                }
+                out += "]";
             }
             out += "\n";
         }

@@ -8,6 +8,13 @@
 
 namespace fsa {
 
+// neighbor-walk variants the autotuner times per kernel family: the unroll in
+// the low nibble, an occupancy hint (at least that many waves per SIMD) in the high one
+constexpr uint8_t kVariantU2 = 0x02, kVariantU4 = 0x04, kVariantU8 = 0x08;
+constexpr uint8_t kVariantU4W8 = 0x84, kVariantU8W6 = 0x68;
+constexpr int kTuneVariants = 5;
+constexpr uint8_t kTuneVariant[kTuneVariants] = {kVariantU2, kVariantU4, kVariantU8, kVariantU4W8, kVariantU8W6};
+
 // Device-resident launch description of a set of scenes: the scene table and
 // one block map per launch domain.
 struct LaunchSet {
@@ -19,8 +26,8 @@ struct LaunchSet {
     BlockRef* map_scene = nullptr;   // one block per scene
     uint32_t nb_all = 0, nb_fluid = 0, nb_tiles = 0, nb_scan = 0, nb_scene = 0;
     uint32_t spec = 0;  // kSpec* bits shared by every scene of the set
-    // neighbor-walk unroll per IISPH kernel family (kTune*), chosen by the autotuner
-    uint8_t unroll[4] = {4, 4, 4, 4};
+    // neighbor-walk variant (kVariant*) per IISPH kernel family (kTune*), chosen by the autotuner
+    uint8_t unroll[4] = {kVariantU4, kVariantU4, kVariantU4, kVariantU4};
     bool codec = false;  // some scene of the set checks the NeighborStorage codec each search
     bool wide_lists = false;  // some scene's neighbor-list array spans 4 GiB or more
     bool entities_before = false;  // some scene has a boundary preprocessor or a velocity alteration

@@ -144,7 +144,7 @@ struct PosP {
 };
 
 template <uint32_t Spec, int U>
-__global__ __launch_bounds__(kBlock) void k_iisph_prepare(const SceneDev* __restrict__ scenes,
+__device__ __forceinline__ void iisph_prepare_body(const SceneDev* __restrict__ scenes,
                                                           const BlockRef* __restrict__ map) {
     const BlockRef br = map[blockIdx.x];
     const SceneDev& S = scenes[br.scene];
@@ -210,6 +210,19 @@ __global__ __launch_bounds__(kBlock) void k_iisph_prepare(const SceneDev* __rest
     if (S.flags & kFlagExternal) S.ext[c][i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 }
 
+// the walk above as a kernel, and with an occupancy hint (at least W waves per
+// SIMD): variants the autotuner times
+template <uint32_t Spec, int U>
+__global__ __launch_bounds__(kBlock) void k_iisph_prepare(const SceneDev* __restrict__ scenes,
+                                                  const BlockRef* __restrict__ map) {
+    iisph_prepare_body<Spec, U>(scenes, map);
+}
+template <uint32_t Spec, int U, int W>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(W))) void k_iisph_prepare_w(
+    const SceneDev* __restrict__ scenes, const BlockRef* __restrict__ map) {
+    iisph_prepare_body<Spec, U>(scenes, map);
+}
+
 // phase 1 for boundary particles: no forces of their own, density untouched
 // (IISPHFluidSolver3D.hpp:152-176, 180-182)
 __global__ __launch_bounds__(kBlock) void k_iisph_prepare_boundary(const SceneDev* __restrict__ scenes,
@@ -266,7 +279,7 @@ __device__ __forceinline__ void pressure_update(const SceneDev& S, uint32_t c, u
 // same terms in the same order, bit for bit), and phase 2 walks once for the
 // source term and a_ii together.
 template <uint32_t Spec, int U>
-__global__ __launch_bounds__(kBlock) void k_iisph_diagonal(const SceneDev* __restrict__ scenes,
+__device__ __forceinline__ void iisph_diagonal_body(const SceneDev* __restrict__ scenes,
                                                            const BlockRef* __restrict__ map) {
     const BlockRef br = map[blockIdx.x];
     const SceneDev& S = scenes[br.scene];
@@ -330,6 +343,19 @@ __global__ __launch_bounds__(kBlock) void k_iisph_diagonal(const SceneDev* __res
     }
 }
 
+// the walk above as a kernel, and with an occupancy hint (at least W waves per
+// SIMD): variants the autotuner times
+template <uint32_t Spec, int U>
+__global__ __launch_bounds__(kBlock) void k_iisph_diagonal(const SceneDev* __restrict__ scenes,
+                                                  const BlockRef* __restrict__ map) {
+    iisph_diagonal_body<Spec, U>(scenes, map);
+}
+template <uint32_t Spec, int U, int W>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(W))) void k_iisph_diagonal_w(
+    const SceneDev* __restrict__ scenes, const BlockRef* __restrict__ map) {
+    iisph_diagonal_body<Spec, U>(scenes, map);
+}
+
 // ---- Jacobi loop control -------------------------------------------------------------
 __global__ void k_iisph_begin_loop(const SceneDev* __restrict__ scenes, const BlockRef* __restrict__ map) {
     const SceneDev& S = scenes[map[blockIdx.x].scene];
@@ -348,7 +374,7 @@ __global__ void k_iisph_begin_loop(const SceneDev* __restrict__ scenes, const Bl
 
 // ---- Jacobi pass A: pressure acceleration ------------------------------------------
 template <uint32_t Spec, int U>
-__global__ __launch_bounds__(kBlock) void k_iisph_pass_a(const SceneDev* __restrict__ scenes,
+__device__ __forceinline__ void iisph_pass_a_body(const SceneDev* __restrict__ scenes,
                                                          const BlockRef* __restrict__ map) {
     const BlockRef br = map[blockIdx.x];
     const SceneDev& S = scenes[br.scene];
@@ -385,9 +411,22 @@ __global__ __launch_bounds__(kBlock) void k_iisph_pass_a(const SceneDev* __restr
     S.acc[c][i] = make_float4(ax, ay, az, 0.0f);
 }
 
+// the walk above as a kernel, and with an occupancy hint (at least W waves per
+// SIMD): variants the autotuner times
+template <uint32_t Spec, int U>
+__global__ __launch_bounds__(kBlock) void k_iisph_pass_a(const SceneDev* __restrict__ scenes,
+                                                  const BlockRef* __restrict__ map) {
+    iisph_pass_a_body<Spec, U>(scenes, map);
+}
+template <uint32_t Spec, int U, int W>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(W))) void k_iisph_pass_a_w(
+    const SceneDev* __restrict__ scenes, const BlockRef* __restrict__ map) {
+    iisph_pass_a_body<Spec, U>(scenes, map);
+}
+
 // ---- Jacobi pass B: A*p, relaxed pressure update, error, reductions -------------------
 template <uint32_t Spec, int U>
-__global__ __launch_bounds__(kBlock) void k_iisph_pass_b(const SceneDev* __restrict__ scenes,
+__device__ __forceinline__ void iisph_pass_b_body(const SceneDev* __restrict__ scenes,
                                                          const BlockRef* __restrict__ map) {
     const BlockRef br = map[blockIdx.x];
     const SceneDev& S = scenes[br.scene];
@@ -424,6 +463,19 @@ __global__ __launch_bounds__(kBlock) void k_iisph_pass_b(const SceneDev* __restr
     }
     const Partial part = block_reduce(err_sum, counted, a2, v2, lds);
     if (threadIdx.x == 0) S.partial[br.block] = part;
+}
+
+// the walk above as a kernel, and with an occupancy hint (at least W waves per
+// SIMD): variants the autotuner times
+template <uint32_t Spec, int U>
+__global__ __launch_bounds__(kBlock) void k_iisph_pass_b(const SceneDev* __restrict__ scenes,
+                                                  const BlockRef* __restrict__ map) {
+    iisph_pass_b_body<Spec, U>(scenes, map);
+}
+template <uint32_t Spec, int U, int W>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(W))) void k_iisph_pass_b_w(
+    const SceneDev* __restrict__ scenes, const BlockRef* __restrict__ map) {
+    iisph_pass_b_body<Spec, U>(scenes, map);
 }
 
 // convergence decision of one Jacobi iteration (IISPHFluidSolver3D.hpp:463-522),
@@ -791,11 +843,25 @@ __global__ __launch_bounds__(kFinalizeThreads) void k_integrate_finalize(const S
             default: kernel<0, U><<<grid, kBlock, 0, st>>>(L.scenes, map); break;                   \
         }                                                                                           \
     } while (0)
+// spec x unroll x occupancy hint (the variant chosen by the autotuner per kernel family:
+// the unroll in the low nibble, the waves-per-SIMD hint in the high nibble)
+#define FSA_SPEC_UW(kernel, U, W, grid, map)                                                        \
+    do {                                                                                            \
+        switch (L.spec) {                                                                           \
+            case 7: kernel##_w<7, U, W><<<grid, kBlock, 0, st>>>(L.scenes, map); break;             \
+            case 5: kernel##_w<5, U, W><<<grid, kBlock, 0, st>>>(L.scenes, map); break;             \
+            case 3: kernel##_w<3, U, W><<<grid, kBlock, 0, st>>>(L.scenes, map); break;             \
+            case 1: kernel##_w<1, U, W><<<grid, kBlock, 0, st>>>(L.scenes, map); break;             \
+            default: kernel##_w<0, U, W><<<grid, kBlock, 0, st>>>(L.scenes, map); break;            \
+        }                                                                                           \
+    } while (0)
 #define FSA_UNROLL_DISPATCH(kernel, family, grid, map)                                              \
     do {                                                                                            \
         switch (L.unroll[family]) {                                                                 \
-            case 2: FSA_SPEC_U(kernel, 2, grid, map); break;                                        \
-            case 8: FSA_SPEC_U(kernel, 8, grid, map); break;                                        \
+            case kVariantU2: FSA_SPEC_U(kernel, 2, grid, map); break;                               \
+            case kVariantU8: FSA_SPEC_U(kernel, 8, grid, map); break;                               \
+            case kVariantU4W8: FSA_SPEC_UW(kernel, 4, 8, grid, map); break;                         \
+            case kVariantU8W6: FSA_SPEC_UW(kernel, 8, 6, grid, map); break;                         \
             default: FSA_SPEC_U(kernel, 4, grid, map); break;                                       \
         }                                                                                           \
     } while (0)

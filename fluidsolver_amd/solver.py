@@ -89,8 +89,13 @@ class Context:
     def set_autotune(self, enabled):
         check(lib().fsa_context_set_autotune(self._h, 1 if enabled else 0))
 
+    def set_walk_variant(self, variant):
+        """Force one abi.WALK_* neighbor-walk variant on every 3D IISPH kernel
+        family (0: the autotuner chooses)."""
+        check(lib().fsa_context_set_walk_variant(self._h, int(variant)))
+
     def tuning(self):
-        """Text report of the autotuned unroll choices and their timings."""
+        """Text report of the autotuned walk variants and their timings."""
         n = C.c_uint64(0)
         check(lib().fsa_context_get_tuning(self._h, None, 0, C.byref(n)))
         buf = C.create_string_buffer(n.value + 1)

@@ -256,11 +256,24 @@ FSA_API int32_t fsa_batch_step(fsa_context ctx, const fsa_scene* scenes, uint32_
 FSA_API int32_t fsa_batch_set_graph(fsa_context ctx, uint32_t enabled);
 /* Autotuning (no counterpart in the reference): on the first batched step of
  * a size class (kernel spec, total fluid rounded to a power of two) the
- * neighbor-walk unroll of each IISPH kernel family is chosen by timing the
- * candidates on the live neighbor lists, writing only to scratch copies.
- * Enabled by default; the choices and their timings are reported as text. */
+ * neighbor-walk variant of each 3D IISPH kernel family is chosen by timing the
+ * candidates on the live neighbor lists, writing only to scratch copies.  The
+ * candidates are FSA_WALK_* below: the unroll of the neighbor loop, with or
+ * without an occupancy hint (at least that many waves per SIMD).  Every
+ * variant gives the same results bit for bit.  Enabled by default; the choices
+ * and their timings are reported as text. */
+#define FSA_WALK_U2 0x02u
+#define FSA_WALK_U4 0x04u
+#define FSA_WALK_U8 0x08u
+#define FSA_WALK_U4_W8 0x84u
+#define FSA_WALK_U8_W6 0x68u
 FSA_API int32_t fsa_context_set_autotune(fsa_context ctx, uint32_t enabled);
 FSA_API int32_t fsa_context_get_tuning(fsa_context ctx, char* buf, uint64_t cap, uint64_t* len);
+/* Forces one FSA_WALK_* variant on every kernel family and skips the
+ * autotuner (0 hands the choice back to it).  Other values: FSA_E_INVALID.
+ * No counterpart in the reference: for callers that want a fixed launch
+ * configuration (reproducible timings, a graph captured once). */
+FSA_API int32_t fsa_context_set_walk_variant(fsa_context ctx, uint32_t variant);
 /* Threads per block of the per-particle kernels: 0 (the default) or 256, the
  * one shape this build's block maps are laid out for. */
 FSA_API int32_t fsa_context_set_block_size(fsa_context ctx, uint32_t threads);

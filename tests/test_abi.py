@@ -53,6 +53,7 @@ def test_null_arguments_return_status():
     assert L.fsa_scene_find_neighbors(None) == abi.FSA_E_INVALID
     assert L.fsa_scene_simulate(None, 0.001, None) == abi.FSA_E_INVALID
     assert L.fsa_context_create(0, 0, None) == abi.FSA_E_INVALID
+    assert L.fsa_context_set_walk_variant(None, abi.WALK_U4) == abi.FSA_E_INVALID
     assert L.fsa_server_create(None, None, None) == abi.FSA_E_INVALID
     assert L.fsa_server_tick(None, None) == abi.FSA_E_INVALID
     assert L.fsa_server_submit(None, None, None, 1, None) == abi.FSA_E_INVALID

@@ -60,6 +60,34 @@ def test_graph_replays_equal_eager():
             np.testing.assert_array_equal(a, b, err_msg=f"step {k}")
 
 
+def test_every_walk_variant_gives_the_same_steps():
+    """Each neighbor-walk variant the autotuner may pick (unroll 2/4/8, with and
+    without an occupancy hint), forced on every kernel family, steps the scenes
+    bit for bit alike (graph replays included)."""
+    setups = _setups()[:2]
+    ref = None
+    for v in abi.WALK_VARIANTS:
+        print(f"case walk variant 0x{v:02x}", flush=True)
+        ctx = Context(0)
+        ctx.set_walk_variant(v)
+        sc = [Scene(ctx, d, c.copy()) for d, c in setups]
+        st = [[x.as_dict() for x in ctx.batch_step(sc, 1)] for _ in range(4)]
+        mv = [s.download(c.copy()) for s, (_, c) in zip(sc, setups)]
+        assert ctx.tuning() == ""  # a forced variant skips the autotuner
+        with pytest.raises(Exception):
+            ctx.set_walk_variant(0x44)
+        for s in sc:
+            s.close()
+        ctx.close()
+        if ref is None:
+            ref = (st, mv)
+            continue
+        assert st == ref[0], hex(v)
+        for a, b in zip(mv, ref[1]):
+            for f in ("position", "velocity", "acceleration", "pressure", "density"):
+                np.testing.assert_array_equal(getattr(a, f), getattr(b, f), err_msg=f"{hex(v)} {f}")
+
+
 def test_autotune_keeps_results_and_reports():
     """Steps with the autotuner on equal steps with it off (the timed launches
     write only to scratch copies); the report names a choice per family."""
