@@ -1147,10 +1147,26 @@ __global__ __launch_bounds__(kBlock) void k_codec_finalize(const SceneDev* __res
     __shared__ uint64_t wsum[2][kBlock / 64];
     const uint32_t nparts = (S.nf + kBlock - 1) / kBlock;
     uint64_t c = 0, b = 0;
-    for (uint32_t q = threadIdx.x; q < nparts; q += kBlock) {
-        const Partial p = S.partial[q];
-        c += p.count;
-        b += (uint64_t)p.err;
+    // integer sums: 16 partials per thread are loaded ahead of their adds
+    constexpr uint32_t kAhead = 16;
+    for (uint32_t q0 = threadIdx.x; q0 < nparts; q0 += kAhead * kBlock) {
+        uint32_t qc[kAhead];
+        double qe[kAhead];
+#pragma unroll
+        for (uint32_t u = 0; u < kAhead; u++) {
+            const uint32_t q = q0 + u * kBlock;
+            qc[u] = 0;
+            qe[u] = 0.0;
+            if (q < nparts) {
+                qc[u] = S.partial[q].count;
+                qe[u] = S.partial[q].err;
+            }
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < kAhead; u++) {
+            c += qc[u];
+            b += (uint64_t)qe[u];
+        }
     }
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) {
