@@ -5,7 +5,9 @@
 //   1. k_keys_hist    cell = floor(x / r), compact Morton key      (:174-189, 37-56)
 //   2. k_radix_*      stable LSD radix sort of (key, index)        (:191-201)
 //   3. k_permute      gather every component into sorted order     (ParticleCollection::swap)
-//   4. k_fluid_scan / k_compact   fluid rank -> combined index
+//   4. k_fluid_scan   fluid counts per block to offsets: a fluid particle's
+//                     rank is its block's offset plus its in-block rank
+//                     (k_permute); k_lists writes fluid rank -> combined index
 //   5. (in k_permute) cell -> [start, end) of its run              (:203-225)
 //   6. k_lists        27-cell scan in ascending key order, distance
 //                     test dot(x_j - x_i) <= r^2, ascending lists  (:536-595)
@@ -297,8 +299,18 @@ __global__ __launch_bounds__(kBlock) void k_permute(const SceneDev* __restrict__
         S.idt[o][i] = idt;
         fluid = (idt >> kTypeShift) == 0u;
     }
-    const int cnt = __syncthreads_count(fluid);
-    if (threadIdx.x == 0) S.block_count[br.block] = (uint32_t)cnt;
+    if (CellTable) {
+        // each fluid particle's rank within its block: k_lists adds the block's
+        // scanned count for the fluid rank and writes fluid_idx itself
+        __shared__ uint32_t lw[kBlock / 64];
+        uint32_t tot;
+        const uint32_t ex = block_exclusive_scan(fluid ? 1u : 0u, lw, tot);
+        if (fluid) S.rank_of[i] = ex;
+        if (threadIdx.x == 0) S.block_count[br.block] = tot;
+    } else {
+        const int cnt = __syncthreads_count(fluid);
+        if (threadIdx.x == 0) S.block_count[br.block] = (uint32_t)cnt;
+    }
 }
 
 // one wide block per scene: exclusive scan of the per-block fluid counts, each
@@ -443,7 +455,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) voi
     const auto pos = gload_ptr(S.pos[c]);
     const auto idt = gload_ptr(S.idt[c]);
     const auto keys = gload_ptr(S.keys[sbuf]);
-    const auto rank_of = gload_ptr(S.rank_of);
+    const auto rank_of = gload_ptr(S.rank_of);  // in-block fluid rank (k_permute)
+    const auto block_count = gload_ptr(S.block_count);  // fluid before the block (k_fluid_scan)
     FSA_GLOBAL(uint32_t)* __restrict__ nbr = (FSA_GLOBAL(uint32_t)*)S.nbr;
     const float r2 = S.r2;
     const uint32_t stride = S.list_stride;
@@ -548,7 +561,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) voi
         if (lane < kStageParticles && p < S.n) {
             gidt = idt[p];
             gpos = pos[p];
-            grank = rank_of[p];
+            grank = block_count[p / kBlock] + rank_of[p];
         }
     };
     uint32_t sl0 = (uint32_t)__ffsll((unsigned long long)starts) - 1u;
@@ -639,7 +652,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) voi
                 if (lane < kStageParticles && pi < ce) {
                     ti = idt[pi];
                     xl = pos[pi];
-                    kl = rank_of[pi];
+                    kl = block_count[pi / kBlock] + rank_of[pi];
                 }
             }
             const bool isf = lane < kStageParticles && pi < ce && (ti >> kTypeShift) == 0u;
@@ -732,6 +745,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) voi
             if (isf) {
                 if (cntl > (uint32_t)kMaxNeighbors) overflow = true;
                 S.nbr_cnt[kl] = kept;
+                S.fluid_idx[kl] = pi;
                 stage_cnt[kl - fbase] = kept;
             }
             fcount = r0 + nfl - fbase;
@@ -1212,10 +1226,11 @@ void launch_sort_range(const LaunchSet& L, uint32_t first_src, uint32_t shift0, 
     }
 }
 
+// the fluid ranks come from k_permute's in-block ranks and k_fluid_scan's block
+// offsets; k_lists writes fluid_idx
 void launch_reorder(const LaunchSet& L, uint32_t sbuf, hipStream_t st) {
     k_permute<true><<<L.nb_all, kBlock, 0, st>>>(L.scenes, L.map_all, sbuf);
     k_fluid_scan<<<L.nb_scene, kScanThreads, 0, st>>>(L.scenes, L.map_scene);
-    k_compact<<<L.nb_all, kBlock, 0, st>>>(L.scenes, L.map_all);
 }
 
 void launch_lists(const LaunchSet& L, uint32_t sbuf, hipStream_t st) {
