@@ -131,6 +131,11 @@ struct fsa_scene_s {
     bool searched = false;
     bool boundary_dirty = true;
     uint32_t last_iterations = 2;
+    // 3D IISPH: the Jacobi passes predicted for the last step, and how many steps in a
+    // row the loop has ended within the prediction (the gated extra pass is left out
+    // of steady scenes' steps)
+    uint32_t step_chunk = 0;
+    uint32_t steady_steps = 0;
     // HashedNeighborhoodSearch3D: calls since the last re-sort and the collection
     // order the reference would hold (caller rows, inactive included)
     uint32_t hashed_calls = 0;
@@ -995,6 +1000,10 @@ void set_step_dt(fsa_scene_s* s, float dt) {
     }
 }
 
+// steps in a row whose Jacobi loop ended within the predicted passes before the
+// gated extra pass is left out
+constexpr uint32_t kSteadySteps = 4;
+
 // after the step's final state read
 void post_simulate(fsa_scene_s* s, fsa_step_stats* stats) {
     s->boundary_dirty = false;
@@ -1008,7 +1017,10 @@ void post_simulate(fsa_scene_s* s, fsa_step_stats* stats) {
     s->cur_v2 = s->h_state->next_v2;
     s->cur_a2 = s->h_state->next_a2;
     if (s->h_state->status & kStatusTinyCoord) s->hd.flags |= kFlagTinyCoords;
-    if (s->desc.solver == FSA_SOLVER_IISPH) s->last_iterations = std::max<uint32_t>(s->h_state->iterations, 1);
+    if (s->desc.solver == FSA_SOLVER_IISPH) {
+        s->last_iterations = std::max<uint32_t>(s->h_state->iterations, 1);
+        s->steady_steps = s->h_state->iterations <= s->step_chunk ? s->steady_steps + 1 : 0;
+    }
     fill_stats(s, stats);
     s->searched = false;
     check_status(s);
@@ -1367,8 +1379,15 @@ int32_t batch_step_group(fsa_context ctx, const std::vector<fsa_scene_s*>& all, 
         guess = std::max(guess, std::max(s->hd.min_it, s->last_iterations));
     }
     const uint32_t chunk = sesph ? 0u : std::min(max_it, std::max<uint32_t>(guess, 1));
-    // one gated pass beyond the prediction (3D IISPH), when the iteration limit allows it
-    const uint32_t tail = (sesph || d2) ? 0u : std::min<uint32_t>(1u, max_it - chunk);
+    // one gated pass beyond the prediction (3D IISPH), when the iteration limit allows
+    // it, unless every scene's loop has ended within the prediction for kSteadySteps
+    // steps in a row (a loop that runs longer then takes the host's further passes)
+    bool steady = true;
+    for (auto* s : v) {
+        s->step_chunk = chunk;
+        if (s->steady_steps < kSteadySteps) steady = false;
+    }
+    const uint32_t tail = (sesph || d2 || steady) ? 0u : std::min<uint32_t>(1u, max_it - chunk);
     for (auto* s : v) s->hd.vals_buf = passes & 1u;  // the unsorted searches' final bucket buffer
     upload_batch_table(ctx, b);
     if (ctx->graph && !ctx->timing) {
