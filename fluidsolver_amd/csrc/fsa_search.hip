@@ -73,19 +73,25 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_hist(const SceneDev* __r
     const uint32_t tile = br.block;
     __shared__ uint32_t h[kRadix];
     for (uint32_t d = threadIdx.x; d < (uint32_t)kRadix; d += kSortThreads) h[d] = 0;
-    __syncthreads();
-    const uint32_t* __restrict__ keys = S.keys[src];
+    const uint32_t n = S.n, ntiles = S.ntiles;
+    const auto keys = gload_ptr(S.keys[src]);
+    FSA_GLOBAL(uint32_t)* __restrict__ hist = (FSA_GLOBAL(uint32_t)*)S.hist;
     const uint32_t base = tile * kSortTile;
+    // every item's key is loaded before the first count (past the end: the last key, not counted)
+    uint32_t kk[kSortItems];
+#pragma unroll
+    for (int r = 0; r < kSortItems; r++) kk[r] = keys[min(base + r * kSortThreads + threadIdx.x, n - 1u)];
+    __syncthreads();
 #pragma unroll
     for (int r = 0; r < kSortItems; r++) {
         const uint32_t i = base + r * kSortThreads + threadIdx.x;
-        const bool valid = i < S.n;
-        const uint32_t d = valid ? (keys[i] >> shift) & (kRadix - 1) : 0u;
+        const bool valid = i < n;
+        const uint32_t d = (kk[r] >> shift) & (kRadix - 1);
         const uint64_t peers = match_digit(d, __ballot(valid));
         if (valid && (peers & lanemask_lt()) == 0) atomicAdd(&h[d], (uint32_t)__popcll(peers));
     }
     __syncthreads();
-    for (uint32_t d = threadIdx.x; d < (uint32_t)kRadix; d += kSortThreads) S.hist[d * S.ntiles + tile] = h[d];
+    for (uint32_t d = threadIdx.x; d < (uint32_t)kRadix; d += kSortThreads) hist[d * ntiles + tile] = h[d];
 }
 
 // wide single-block scans (one block per scene or per (scene, digit))
@@ -100,27 +106,36 @@ __global__ __launch_bounds__(kSortThreads) void k_keys_hist(const SceneDev* __re
     const uint32_t tile = br.block;
     __shared__ uint32_t h[kRadix];
     for (uint32_t d = threadIdx.x; d < (uint32_t)kRadix; d += kSortThreads) h[d] = 0;
-    __syncthreads();
-    const uint32_t c = S.state->cur;
+    // the scene's fields are read once (the stores below could otherwise alias
+    // the scene table), and every item's position is loaded before the first store
+    const uint32_t n = S.n, ntiles = S.ntiles, levels = S.levels;
+    const float radius = S.search_radius;
+    const auto pos = gload_ptr(S.pos[S.state->cur]);
+    FSA_GLOBAL(uint32_t)* __restrict__ okeys = (FSA_GLOBAL(uint32_t)*)S.keys[0];
+    FSA_GLOBAL(uint32_t)* __restrict__ ovals = (FSA_GLOBAL(uint32_t)*)S.vals[0];
+    FSA_GLOBAL(uint32_t)* __restrict__ hist = (FSA_GLOBAL(uint32_t)*)S.hist;
     const uint32_t base = tile * kSortTile;
+    // past the end: the last particle's position, neither stored nor counted
+    float4 p[kSortItems];
+#pragma unroll
+    for (int r = 0; r < kSortItems; r++) p[r] = pos[min(base + r * kSortThreads + threadIdx.x, n - 1u)];
+    __syncthreads();
 #pragma unroll
     for (int r = 0; r < kSortItems; r++) {
         const uint32_t i = base + r * kSortThreads + threadIdx.x;
-        const bool valid = i < S.n;
+        const bool valid = i < n;
         uint32_t key = 0;
         if (valid) {
-            const float4 p = S.pos[c][i];
-            key = compact_key(cell_of(p.x, S.search_radius), cell_of(p.y, S.search_radius),
-                              cell_of(p.z, S.search_radius), S.levels);
-            S.keys[0][i] = key;
-            S.vals[0][i] = i;
+            key = compact_key(cell_of(p[r].x, radius), cell_of(p[r].y, radius), cell_of(p[r].z, radius), levels);
+            okeys[i] = key;
+            ovals[i] = i;
         }
         const uint32_t d = key & (kRadix - 1);
         const uint64_t peers = match_digit(d, __ballot(valid));
         if (valid && (peers & lanemask_lt()) == 0) atomicAdd(&h[d], (uint32_t)__popcll(peers));
     }
     __syncthreads();
-    for (uint32_t d = threadIdx.x; d < (uint32_t)kRadix; d += kSortThreads) S.hist[d * S.ntiles + tile] = h[d];
+    for (uint32_t d = threadIdx.x; d < (uint32_t)kRadix; d += kSortThreads) hist[d * ntiles + tile] = h[d];
 }
 
 // one block per (scene, digit): exclusive scan of the digit's tile counts
@@ -158,21 +173,43 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(const SceneDev* 
     __shared__ uint32_t stage_v[kSortTile];
     __shared__ uint32_t lw[kSortThreads / 64];
 
+    // the scene's fields once, and the tile's items and this tile's digit counts
+    // loaded before anything waits on them
+    const uint32_t ntiles = S.ntiles;
+    const auto hist = gload_ptr(S.hist);
+    const auto keys = gload_ptr(S.keys[src]);
+    const auto vals = gload_ptr(S.vals[src]);
+    FSA_GLOBAL(uint32_t)* __restrict__ okeys = (FSA_GLOBAL(uint32_t)*)S.keys[src ^ 1u];
+    FSA_GLOBAL(uint32_t)* __restrict__ ovals = (FSA_GLOBAL(uint32_t)*)S.vals[src ^ 1u];
+    const uint32_t base = tile * kSortTile;
+    const uint32_t tile_n = min((uint32_t)kSortTile, S.n - base);
+    uint32_t k[kSortItems], v[kSortItems], rank[kSortItems];
+    // wave w owns items [w*512, (w+1)*512) of the tile, in rounds of 64 consecutive
+    // items (past the tile's end: its last item, which takes no part)
+#pragma unroll
+    for (int r = 0; r < kSortItems; r++) {
+        const uint32_t li = min(wave * (kSortTile / (kSortThreads / 64)) + r * 64 + __lane_id(), tile_n - 1u);
+        k[r] = keys[base + li];
+        v[r] = vals[base + li];
+    }
+
     // global base of each digit for this tile; thread t owns digits t * kDigitsPerThread + q
     constexpr uint32_t kDigitsPerThread = kRadix / kSortThreads;
     {
-        uint32_t tot_d[kDigitsPerThread], local = 0;
+        uint32_t tot_d[kDigitsPerThread], at_d[kDigitsPerThread], local = 0;
 #pragma unroll
         for (uint32_t q = 0; q < kDigitsPerThread; q++) {
-            tot_d[q] = S.hist[kRadix * S.ntiles + t * kDigitsPerThread + q];
-            local += tot_d[q];
+            tot_d[q] = hist[kRadix * ntiles + t * kDigitsPerThread + q];
+            at_d[q] = hist[(t * kDigitsPerThread + q) * ntiles + tile];
         }
+#pragma unroll
+        for (uint32_t q = 0; q < kDigitsPerThread; q++) local += tot_d[q];
         uint32_t tot;
         uint32_t ex = block_exclusive_scan(local, lw, tot);
 #pragma unroll
         for (uint32_t q = 0; q < kDigitsPerThread; q++) {
             const uint32_t d = t * kDigitsPerThread + q;
-            digit_base[d] = ex + S.hist[d * S.ntiles + tile];
+            digit_base[d] = ex + at_d[q];
             ex += tot_d[q];
         }
     }
@@ -182,18 +219,10 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(const SceneDev* 
         for (uint32_t q = 0; q < kDigitsPerThread; q++) wave_cnt[w][t * kDigitsPerThread + q] = 0;
     __syncthreads();
 
-    const uint32_t* __restrict__ keys = S.keys[src];
-    const uint32_t* __restrict__ vals = S.vals[src];
-    const uint32_t base = tile * kSortTile;
-    const uint32_t tile_n = min((uint32_t)kSortTile, S.n - base);
-    uint32_t k[kSortItems], v[kSortItems], rank[kSortItems];
-    // wave w owns items [w*512, (w+1)*512) of the tile, in rounds of 64 consecutive items
 #pragma unroll
     for (int r = 0; r < kSortItems; r++) {
         const uint32_t li = wave * (kSortTile / (kSortThreads / 64)) + r * 64 + __lane_id();
         const bool valid = li < tile_n;
-        k[r] = valid ? keys[base + li] : 0u;
-        v[r] = valid ? vals[base + li] : 0u;
         const uint32_t d = (k[r] >> shift) & (kRadix - 1);
         const uint64_t peers = match_digit(d, __ballot(valid));
         uint32_t before = valid ? wave_cnt[wave][d] : 0u;
@@ -237,8 +266,6 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(const SceneDev* 
         }
     }
     __syncthreads();
-    uint32_t* __restrict__ okeys = S.keys[src ^ 1u];
-    uint32_t* __restrict__ ovals = S.vals[src ^ 1u];
     for (uint32_t idx = t; idx < tile_n; idx += kSortThreads) {
         const uint32_t key = stage_k[idx];
         const uint32_t d = (key >> shift) & (kRadix - 1);
@@ -261,43 +288,73 @@ __global__ __launch_bounds__(kBlock) void k_permute(const SceneDev* __restrict__
     const uint32_t i = br.block * kBlock + threadIdx.x;
     const uint32_t c = S.state->cur;
     const uint32_t o = c ^ 1u;
+    const uint32_t n = S.n;
+    const bool ext = (S.flags & kFlagExternal) != 0;
     bool fluid = false;
-    if (i < S.n) {
-        const uint32_t s = S.vals[sbuf][i];
-        const float4 p = S.pos[c][s];
-        S.pos[o][i] = p;
-        const uint32_t* __restrict__ keys = S.keys[sbuf];
-        const uint32_t key = CellTable ? keys[i] : 0u;
-        const bool first = CellTable && (i == 0 || keys[i - 1] != key);
-        const bool last = CellTable && (i + 1 == S.n || keys[i + 1] != key);
+    if (i < n) {
+        // every gather is issued before the first store (the stores could
+        // otherwise alias the scene table and the source arrays)
+        const uint32_t s = gload_ptr(S.vals[sbuf])[i];
+        const auto keys = gload_ptr(S.keys[sbuf]);
+        const float4 p = gload_ptr(S.pos[c])[s];
+        const float4 vel = gload_ptr(S.vel[c])[s];
+        const float4 acc = gload_ptr(S.acc[c])[s];
+        const float4 ex = ext ? gload_ptr(S.ext[c])[s] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        const float pres = gload_ptr(S.pres[c])[s];
+        const float err = gload_ptr(S.err[c])[s];
+        const uint32_t idt = gload_ptr(S.idt[c])[s];
+        uint32_t key = 0, key_prev = 0, key_next = 0;
+        if (CellTable) {
+            key = keys[i];
+            key_prev = keys[i == 0 ? 0u : i - 1u];
+            key_next = keys[i + 1 == n ? i : i + 1u];
+        }
+        const bool first = CellTable && (i == 0 || key_prev != key);
+        const bool last = CellTable && (i + 1 == n || key_next != key);
+        // the cell table's fields, read before the stores too
+        const float radius = S.search_radius;
+        const int32_t cmin0 = S.cmin[0], cmin1 = S.cmin[1], cmin2 = S.cmin[2];
+        const uint32_t cdim0 = S.cdim[0], cdim1 = S.cdim[1], cdim2 = S.cdim[2];
+        const uint32_t stamp = S.stamp, levels = S.levels;
+        uint4* __restrict__ cells = S.cells;
+        uint32_t* __restrict__ run_grid = S.keys[sbuf ^ 1u];
+        uint32_t* __restrict__ run_order = S.vals[sbuf ^ 1u];
+        SceneState* const state = S.state;
+        FSA_GLOBAL(fsa_f4)* __restrict__ opos = (FSA_GLOBAL(fsa_f4)*)S.pos[o];
+        FSA_GLOBAL(fsa_f4)* __restrict__ ovel = (FSA_GLOBAL(fsa_f4)*)S.vel[o];
+        FSA_GLOBAL(fsa_f4)* __restrict__ oacc = (FSA_GLOBAL(fsa_f4)*)S.acc[o];
+        FSA_GLOBAL(fsa_f4)* __restrict__ oext = (FSA_GLOBAL(fsa_f4)*)S.ext[o];
+        FSA_GLOBAL(float)* __restrict__ opres = (FSA_GLOBAL(float)*)S.pres[o];
+        FSA_GLOBAL(float)* __restrict__ oerr = (FSA_GLOBAL(float)*)S.err[o];
+        FSA_GLOBAL(uint32_t)* __restrict__ oidt = (FSA_GLOBAL(uint32_t)*)S.idt[o];
+        opos[i] = fsa_f4{p.x, p.y, p.z, p.w};
+        ovel[i] = fsa_f4{vel.x, vel.y, vel.z, vel.w};
+        oacc[i] = fsa_f4{acc.x, acc.y, acc.z, acc.w};
+        if (ext) oext[i] = fsa_f4{ex.x, ex.y, ex.z, ex.w};
+        opres[i] = pres;
+        oerr[i] = err;
+        oidt[i] = idt;
+        fluid = (idt >> kTypeShift) == 0u;
         if (first || last) {
-            const int32_t cx = cell_of(p.x, S.search_radius), cy = cell_of(p.y, S.search_radius),
-                          cz = cell_of(p.z, S.search_radius);
-            uint32_t lin;
-            if (!cell_linear(S, cx, cy, cz, lin)) {
-                atomicOr(&S.state->status, kStatusGrid);
+            const int32_t cx = cell_of(p.x, radius), cy = cell_of(p.y, radius), cz = cell_of(p.z, radius);
+            // dense index in the step's cell grid (cell_linear)
+            const int32_t x = cx - cmin0, y = cy - cmin1, z = cz - cmin2;
+            if (x < 0 || y < 0 || z < 0 || x >= (int32_t)cdim0 || y >= (int32_t)cdim1 || z >= (int32_t)cdim2) {
+                atomicOr(&state->status, kStatusGrid);
             } else {
+                const uint32_t lin = (uint32_t)x + cdim0 * ((uint32_t)y + cdim1 * (uint32_t)z);
                 if (first) {
-                    S.cells[lin].x = i;
-                    S.cells[lin].z = S.stamp;
+                    cells[lin].x = i;
+                    cells[lin].z = stamp;
                     // for the list kernel, at the run's first position of the sort's
                     // idle buffers: the cell's grid coordinates (10 bits each) and the
                     // key order of its neighbors
-                    S.keys[sbuf ^ 1u][i] = (uint32_t)(cx - S.cmin[0]) | (uint32_t)(cy - S.cmin[1]) << 10 |
-                                           (uint32_t)(cz - S.cmin[2]) << 20;
-                    S.vals[sbuf ^ 1u][i] = neighbor_order_word(cx, cy, cz, S.levels);
+                    run_grid[i] = (uint32_t)x | (uint32_t)y << 10 | (uint32_t)z << 20;
+                    run_order[i] = neighbor_order_word(cx, cy, cz, levels);
                 }
-                if (last) S.cells[lin].y = i + 1;
+                if (last) cells[lin].y = i + 1;
             }
         }
-        S.vel[o][i] = S.vel[c][s];
-        S.acc[o][i] = S.acc[c][s];
-        if (S.flags & kFlagExternal) S.ext[o][i] = S.ext[c][s];
-        S.pres[o][i] = S.pres[c][s];
-        S.err[o][i] = S.err[c][s];
-        const uint32_t idt = S.idt[c][s];
-        S.idt[o][i] = idt;
-        fluid = (idt >> kTypeShift) == 0u;
     }
     if (CellTable) {
         // each fluid particle's rank within its block: k_lists adds the block's
