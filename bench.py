@@ -146,10 +146,16 @@ def cpu_baseline(desc, coll, steps, threads):
     return nf * steps / dt, O.get_threads(), dt
 
 
+# neighbor-list storage of every 3D scene the bench creates (--lists)
+LISTS = {"compressed": abi.LIST_COMPRESSED, "delta": abi.LIST_DELTA, "plain": abi.LIST_PLAIN}
+LIST_MODE = abi.LIST_COMPRESSED
+
+
 def sweep_descs(sweep):
     descs = []
     for sc in sweep:
-        descs.append(abi.default_desc(3, abi.SOLVER_IISPH, iisph_gamma=sc["gamma"], iisph_viscosity=sc["viscosity"]))
+        descs.append(abi.default_desc(3, abi.SOLVER_IISPH, iisph_gamma=sc["gamma"], iisph_viscosity=sc["viscosity"],
+                                      neighbor_lists=LIST_MODE))
     return descs
 
 
@@ -224,7 +230,7 @@ def bench_served(ctx, window_s, rate, max_running, budget_gb, buckets=12):
     arrivals = scenes.config5_arrivals(seed=5, duration_s=window_s, rate=rate)
     tmpl = _templates(10000, 500000, buckets, 5)
     tsizes = np.array([np.log(n) for n, _ in tmpl])
-    desc = abi.default_desc(3, abi.SOLVER_IISPH)
+    desc = abi.default_desc(3, abi.SOLVER_IISPH, neighbor_lists=LIST_MODE)
     srv = Server(ctx, hbm_budget_bytes=int(budget_gb * 1e9), max_running=max_running)
     k = 0
     offered = 0  # particle-updates the arrived jobs ask for
@@ -318,7 +324,7 @@ def bench_obstacles(ctx, steps, warmup):
     }
     for name, solver in (("sesph", abi.SOLVER_SESPH), ("iisph", abi.SOLVER_IISPH)):
         d = abi.default_desc(3, solver, sesph_stiffness_k=1000.0)
-        d.neighbor_lists = abi.LIST_COMPRESSED
+        d.neighbor_lists = LIST_MODE if LIST_MODE != abi.LIST_PLAIN else abi.LIST_COMPRESSED
         sc = Scene(ctx, d, coll.copy())
         for _ in range(warmup):
             sc.step(1)
@@ -346,7 +352,7 @@ def pmc_workload(args):
     side = args.side
     coll = scenes.config2_dam3d(seed=2, side=side, interior=(3 * side, 3 * side // 2, 3 * side // 2))
     ctx = Context(0)
-    sc = Scene(ctx, abi.default_desc(3, abi.SOLVER_IISPH), coll)
+    sc = Scene(ctx, abi.default_desc(3, abi.SOLVER_IISPH, neighbor_lists=LIST_MODE), coll)
     sc.step(2)
     ctx.synchronize()
     log("pmc workload: calibration copy")
@@ -395,9 +401,15 @@ def main():
                     help="also time this many steps of a fresh 1M scene from rest (the whole configuration; 0: skip)")
     ap.add_argument("--no-obstacles", action="store_true")
     ap.add_argument("--graph", type=int, default=1, help="replay step heads from hipGraphs (1) or launch eagerly (0)")
+    ap.add_argument("--lists", choices=sorted(LISTS), default="compressed",
+                    help="neighbor-list storage the solver walks read: 32-bit entries with the NeighborStorage "
+                         "capacity (compressed), 16-bit delta codes with the same capacity (delta), or 32-bit "
+                         "entries with the 97-neighbor capacity only (plain)")
     ap.add_argument("--pmc-workload", action="store_true",
                     help="workload for rocprofv3 --pmc: a device copy of 1 GiB (counter calibration), then 3 steps")
     args = ap.parse_args()
+    global LIST_MODE
+    LIST_MODE = LISTS[args.lists]
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus above 1 is not supported"}))
         sys.exit(2)
@@ -419,6 +431,7 @@ def main():
     else:
         desc = abi.default_desc(3, abi.SOLVER_IISPH)
     desc.search = SEARCHES[args.search][0]
+    desc.neighbor_lists = LIST_MODE
     log(f"scene: {nf} fluid, {nb} boundary, {args.search} search")
 
     ctx = Context(0)
@@ -615,6 +628,7 @@ def main():
             "reference_codec_bytes_per_list": round(L, 2),
             "boundary_within_reach_of_fluid": nba,
             "occupied_cells": ncells,
+            "neighbor_lists": args.lists,
         },
         "fractions": fractions,
         "full_run": full,

@@ -40,6 +40,7 @@ SOLVER_IISPH = 0
 SOLVER_SESPH = 1
 LIST_COMPRESSED = 0
 LIST_PLAIN = 1
+LIST_DELTA = 2
 SEARCH_DEFAULT = 0
 SEARCH_COMPRESSED_3D = 1
 SEARCH_HASHED_3D = 2

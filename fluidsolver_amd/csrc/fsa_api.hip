@@ -51,6 +51,7 @@ uint64_t align_up(uint64_t v, uint64_t a) { return (v + a - 1) / a * a; }
 struct Plan {
     uint64_t n, nf, stride, nblocks, nfblocks, ntiles, dim;
     bool unsorted;  // 3D hashed / quadratic: per-particle cell of the last grid update
+    bool delta;     // 3D compressed search with delta-coded lists
     uint64_t bytes;
     std::vector<std::pair<void**, uint64_t>> parts;
 };
@@ -232,6 +233,11 @@ uint32_t search_of(const fsa_scene_desc* d) {
     return d->dimension == 2 ? FSA_SEARCH_HASHED_2D : FSA_SEARCH_COMPRESSED_3D;
 }
 
+// the scene keeps delta-coded neighbor lists (3D compressed search only)
+bool delta_lists(const fsa_scene_desc& d) {
+    return d.dimension == 3 && search_of(&d) == FSA_SEARCH_COMPRESSED_3D && d.neighbor_lists == FSA_LIST_DELTA;
+}
+
 void fill_params(fsa_scene_s* s) {
     const fsa_scene_desc& d = s->desc;
     SceneDev& h = s->hd;
@@ -275,7 +281,8 @@ void fill_params(fsa_scene_s* s) {
     const uint32_t search = search_of(&d);
     if (search == FSA_SEARCH_QUADRATIC_2D || search == FSA_SEARCH_QUADRATIC_3D) f |= kFlagQuadratic;
     if (search == FSA_SEARCH_HASHED_3D || search == FSA_SEARCH_QUADRATIC_3D) f |= kFlagUnsorted;
-    if (search == FSA_SEARCH_COMPRESSED_3D && d.neighbor_lists == FSA_LIST_COMPRESSED) f |= kFlagCodec;
+    if (search == FSA_SEARCH_COMPRESSED_3D && d.neighbor_lists != FSA_LIST_PLAIN) f |= kFlagCodec;
+    if (delta_lists(d)) f |= kFlagDeltaLists;
     h.flags = f;
 }
 
@@ -290,6 +297,7 @@ void validate_desc(const fsa_scene_desc* d) {
         throw FsaError(FSA_E_INVALID, "2D scenes use the hashed or quadratic-dynamic-allocated neighbor order");
     if (d->solver != FSA_SOLVER_IISPH && d->solver != FSA_SOLVER_SESPH)
         throw FsaError(FSA_E_INVALID, "unknown solver");
+    if (d->neighbor_lists > FSA_LIST_DELTA) throw FsaError(FSA_E_INVALID, "unknown neighbor-list mode");
     if (!(d->particle_size > 0.0f)) throw FsaError(FSA_E_INVALID, "particle_size must be > 0");
     if (!(d->rest_density > 0.0f)) throw FsaError(FSA_E_INVALID, "rest_density must be > 0");
     if (!(d->timestep > 0.0f)) throw FsaError(FSA_E_INVALID, "timestep must be > 0");
@@ -305,10 +313,11 @@ void validate_desc(const fsa_scene_desc* d) {
     }
 }
 
-Plan make_plan(uint64_t n, uint64_t nf, uint32_t dim = 3, bool unsorted = false) {
+Plan make_plan(uint64_t n, uint64_t nf, uint32_t dim = 3, bool unsorted = false, bool delta = false) {
     Plan p{};
     p.dim = dim;
     p.unsorted = unsorted;
+    p.delta = delta;
     p.n = n;
     p.nf = nf;
     p.stride = align_up(std::max<uint64_t>(nf, 1), 64);
@@ -337,6 +346,7 @@ uint64_t plan_bytes(const Plan& p) {
     add(4 * std::max<uint64_t>(p.nblocks, 1));
     add(4ull * kMaxNeighbors * p.stride);      // nbr
     add(4 * std::max<uint64_t>(p.nf, 1));      // nbr_cnt
+    if (p.delta) add(4ull * kDeltaRows * p.stride);  // dlist
     add(sizeof(Partial) * std::max<uint64_t>(p.nblocks, 1));
     add(sizeof(BoxPartial) * std::max<uint64_t>(p.nblocks, 1));
     if (p.dim == 2) {
@@ -378,6 +388,7 @@ void carve(fsa_scene_s* s, const Plan& p) {
     h.block_count = (uint32_t*)take(4 * std::max<uint64_t>(p.nblocks, 1));
     h.nbr = (uint32_t*)take(4ull * kMaxNeighbors * p.stride);
     h.nbr_cnt = (uint32_t*)take(4 * std::max<uint64_t>(p.nf, 1));
+    h.dlist = p.delta ? (uint32_t*)take(4ull * kDeltaRows * p.stride) : nullptr;
     h.partial = (Partial*)take(sizeof(Partial) * std::max<uint64_t>(p.nblocks, 1));
     h.box = (BoxPartial*)take(sizeof(BoxPartial) * std::max<uint64_t>(p.nblocks, 1));
     if (p.dim == 2) {
@@ -543,6 +554,7 @@ uint32_t spec_of(const SceneDev& h) {
     // the shared-reciprocal division and the short sqrt correction are exact for
     // h == 1 as long as no coordinate is tiny (2D runs its own kernels)
     if (h.dim == 3 && (h.flags & kFlagHOne) && !(h.flags & kFlagTinyCoords)) sp |= kSpecFastMath;
+    if (h.dim == 3 && (h.flags & kFlagDeltaLists)) sp |= kSpecDelta;
     return sp;
 }
 
@@ -1088,7 +1100,7 @@ void drop_batches_of(fsa_context ctx, fsa_scene_s* s) {
 }
 
 void upload_batch_table(fsa_context ctx, BatchSet* b) {
-    uint32_t spec = kSpecHOne | kSpecRho0SqOne | kSpecFastMath;  // the specialisations every scene allows
+    uint32_t spec = kSpecHOne | kSpecRho0SqOne | kSpecFastMath | kSpecDelta;  // the specialisations every scene allows
     bool codec = false, wide = false, ent = false, rem = false;
     for (uint32_t q = 0; q < b->scenes.size(); q++) {
         b->h_table[q] = b->scenes[q]->hd;
@@ -2109,7 +2121,7 @@ void relayout(fsa_scene_s* s, const fsa_particle_view* v, const std::vector<uint
         nf += t == FSA_TYPE_NORMAL;
         nb += t == FSA_TYPE_BOUNDARY;
     }
-    const Plan p = make_plan(nf + nb, nf, 3, (s->hd.flags & kFlagUnsorted) != 0);
+    const Plan p = make_plan(nf + nb, nf, 3, (s->hd.flags & kFlagUnsorted) != 0, (s->hd.flags & kFlagDeltaLists) != 0);
     const uint64_t bytes = plan_bytes(p);
     HIP_CHECK(hipStreamSynchronize(s->ctx->stream));
     if (bytes > s->bytes) {
@@ -2348,7 +2360,8 @@ void admit(fsa_server srv) {
         for (uint64_t i = 0; i < j->count; i++)
             if (j->info[8 * i + 4] == FSA_TYPE_BOUNDARY) nb++;
         // scene plan + cell table allowance (the grid of a dam scene is about the particle count)
-        const uint64_t need = plan_bytes(make_plan(j->fluid + nb, j->fluid)) + sizeof(uint4) * (j->fluid + nb);
+        const uint64_t need = plan_bytes(make_plan(j->fluid + nb, j->fluid, 3, false, delta_lists(j->desc))) +
+                              sizeof(uint4) * (j->fluid + nb);
         const uint64_t budget = server_budget(srv);
         if (budget && srv->ctx->in_use.load() + need > budget) {
             if (!srv->running.empty()) return;  // FIFO: wait until running jobs free memory
@@ -2531,7 +2544,8 @@ FSA_API int32_t fsa_scene_memory_estimate(const fsa_scene_desc* desc, uint64_t n
         if (!bytes) throw FsaError(FSA_E_INVALID, "bytes is NULL");
         const uint32_t search = search_of(desc);
         *bytes = plan_bytes(make_plan(n_fluid + n_boundary, n_fluid, desc->dimension,
-                                      search == FSA_SEARCH_HASHED_3D || search == FSA_SEARCH_QUADRATIC_3D)) +
+                                      search == FSA_SEARCH_HASHED_3D || search == FSA_SEARCH_QUADRATIC_3D,
+                                      delta_lists(*desc))) +
                  sizeof(SceneDev);
         // the hashed search's re-sort scratch (for the active rows; inactive rows add to it)
         if (search == FSA_SEARCH_HASHED_3D) *bytes += quick_sort_bytes(n_fluid + n_boundary);
@@ -2557,7 +2571,8 @@ FSA_API int32_t fsa_scene_create(fsa_context c, const fsa_scene_desc* desc, cons
         const uint64_t ndev = desc->dimension == 2 ? view->count : nf + nb;
         const uint32_t search = search_of(desc);
         const Plan p = make_plan(ndev, nf, desc->dimension,
-                                 search == FSA_SEARCH_HASHED_3D || search == FSA_SEARCH_QUADRATIC_3D);
+                                 search == FSA_SEARCH_HASHED_3D || search == FSA_SEARCH_QUADRATIC_3D,
+                                 delta_lists(*desc));
         if (p.n >= (1ull << 27)) throw FsaError(FSA_E_INVALID, "scene too large for 27-bit run starts");
         const uint64_t bytes = plan_bytes(p);
         if (c->budget && c->in_use.load() + bytes > c->budget)
@@ -2638,6 +2653,9 @@ FSA_API int32_t fsa_scene_set_desc(fsa_scene s, const fsa_scene_desc* d) {
         // state, as the reference fixes it with the solver's template arguments
         if (search_of(d) != search_of(&s->desc))
             throw FsaError(FSA_E_INVALID, "the neighborhood search of a scene cannot change");
+        // delta-coded lists take their own buffer
+        if (delta_lists(*d) != delta_lists(s->desc))
+            throw FsaError(FSA_E_INVALID, "a scene cannot switch to or from delta-coded neighbor lists");
         s->desc = *d;
         const uint32_t stamp = s->hd.stamp;
         fill_params(s);
@@ -2689,7 +2707,9 @@ FSA_API int32_t fsa_scene_neighbor_stats(fsa_scene s, uint64_t* total, uint64_t*
         launch_codec(L, s->ctx->stream);
         read_state(s);
         if (total) *total = s->h_state->total_neighbors;
-        if (dev_bytes) *dev_bytes = 4 * s->h_state->total_neighbors;
+        if (dev_bytes)
+            *dev_bytes = (s->hd.flags & kFlagDeltaLists) && !s->h_state->delta_plain ? s->h_state->delta_bytes
+                                                                                     : 4 * s->h_state->total_neighbors;
         if (ref_bytes) *ref_bytes = s->h_state->codec_bytes;
     });
 }

@@ -43,6 +43,26 @@ constexpr uint32_t kFlagQuadratic = 256u;   // 2D: QuadraticNeighborhoodSearchDy
 constexpr uint32_t kFlagCodec = 512u;       // enforce the NeighborStorage codec capacity each search
 constexpr uint32_t kFlagUnsorted = 1024u;   // 3D hashed / quadratic searches: the collection keeps its order
 constexpr uint32_t kFlagTinyCoords = 2048u; // some coordinate lies in (0, 2^-36): pair math takes the IEEE path
+constexpr uint32_t kFlagDeltaLists = 4096u; // 3D compressed search: the walks read the lists as 16-bit delta codes
+
+// Delta-coded neighbor lists (3D compressed search, kFlagDeltaLists).  Each
+// list is re-encoded after the search from its 32-bit entries:
+//   code rows: entry e of fluid rank k has a 16-bit code in dword row e / 2
+//     (low half for even e) at row * list_stride + k, so a wave reads the
+//     codes of two entries of 64 consecutive ranks in one 256-byte load;
+//     code = delta to the previous entry's index (1..32767) | boundary << 15,
+//     or 0: the entry is the next one of the escape rows (entry 0 always is);
+//   escape rows: the x-th escaped entry of rank k (its full 32-bit entry) at
+//     (kDeltaCodeRows + x) * list_stride + k.  Entry 0 carries kDeltaManyBit
+//     when the list has more than two escapes (the walks keep two in
+//     registers and load the others as they are reached).
+// A search whose lists need more than kDeltaEscRows escapes somewhere leaves
+// SceneState.delta_plain set, and that step's walks read the 32-bit lists.
+constexpr uint32_t kDeltaCodeRows = (kMaxNeighbors + 1) / 2;  // 49 rows: entries 0..96
+constexpr uint32_t kDeltaEscRows = 15;
+constexpr uint32_t kDeltaRows = kDeltaCodeRows + kDeltaEscRows;  // 64 dwords per fluid rank
+constexpr uint32_t kDeltaMax = 0x7fffu;
+constexpr uint32_t kDeltaManyBit = 0x40000000u;
 
 // 2D scenes (IISPHFluidSolver / SESPHFluidSolver with HashedNeighborhoodSearch or
 // QuadraticNeighborhoodSearchDynamicAllocated) keep every particle, inactive
@@ -73,7 +93,9 @@ struct alignas(16) SceneState {
     uint64_t total_neighbors;
     uint64_t codec_bytes;  // NeighborStorage encoded bytes of all lists of the last search
     uint32_t pending;      // 3D: this step's integration has not run yet
-    uint32_t pad_pending;
+    uint32_t delta_spill;  // delta lists: some list of the search in progress exceeds the escape rows
+    uint32_t delta_plain;  // delta lists: the last search's walks read the 32-bit lists
+    uint64_t delta_bytes;  // delta lists: code and escape bytes of all lists of the last search
 };
 
 // max of (v2, a2) over a kBlock-thread block, NaN ignored; every thread gets the result
@@ -101,7 +123,7 @@ struct alignas(16) Partial {
     uint32_t count;
     float max_a2;
     float max_v2;
-    uint32_t pad;
+    uint32_t aux;  // k_codec: the block's delta-coded list bytes
 };
 
 struct alignas(16) BoxPartial {
@@ -187,6 +209,7 @@ struct alignas(16) SceneDev {
     uint32_t* rank_of;     // fluid rank of combined index i (fluid particles only)
     uint32_t* block_count; // fluid count per kBlock-block, scanned in place
     uint32_t* nbr;         // neighbor entries, entry e of fluid rank k at e * list_stride + k
+    uint32_t* dlist;       // delta-coded lists (kFlagDeltaLists): kDeltaRows rows of list_stride dwords
     uint32_t* nbr_cnt;
     Partial* partial;      // per fluid block
     BoxPartial* box;       // per combined block
@@ -300,6 +323,8 @@ constexpr uint32_t kSpecRho0SqOne = 2u;
 // and lengths with a nonzero gradient lie in (2^-23, 2], the ranges on which
 // sqrt_rn_pair and div_pair equal the IEEE operations.
 constexpr uint32_t kSpecFastMath = 4u;
+// Bit 3: every scene of the launch keeps delta-coded neighbor lists (kFlagDeltaLists).
+constexpr uint32_t kSpecDelta = 8u;
 // smallest magnitude a nonzero coordinate may have for the fast pair math
 constexpr float kTinyCoord = 1.4551915228366852e-11f;  // 2^-36
 
@@ -544,6 +569,79 @@ __device__ __forceinline__ void walk_neighbors(GViewU nb, uint32_t stride, uint3
         const uint32_t en = nb[e * stride];
         const T dd = load(en & kIndexMask);
         body(en, dd);
+    }
+}
+
+// One entry of a delta-coded list from its 16-bit code c: the previous index
+// plus the delta (boundary flag from bit 15), or the next escaped entry.  The
+// escape queue holds two entries (qa, then qb); a list with more escapes
+// (`many`) loads each further one when the queue moves up.
+struct DeltaCursor {
+    uint32_t prev;  // index of the previous entry
+    uint32_t qa, qb;
+    uint32_t next;  // escape row to load into qb when the queue moves up
+    bool many;
+};
+__device__ __forceinline__ uint32_t delta_entry(uint32_t c, DeltaCursor& q, GViewU esc, uint32_t stride) {
+    const uint32_t delta = c & kDeltaMax;
+    const bool is_esc = delta == 0u;
+    const uint32_t en = is_esc ? q.qa : ((q.prev + delta) | ((c & 0x8000u) << 16));
+    q.prev = en & kIndexMask;
+    q.qa = is_esc ? q.qb : q.qa;
+    // a walked list has at most kDeltaEscRows escapes, so rows from there on are never needed
+    if (is_esc && q.many && q.next < kDeltaEscRows) {
+        q.qb = esc[q.next * stride];
+        q.next++;
+    }
+    return en;
+}
+
+// walk_neighbors over a delta-coded list: codes of 64 consecutive ranks are
+// read two entries per 256-byte load, the entries decoded in order, and their
+// gathers issued before any of them is used.  U is even.
+template <int U, class Load, class Body>
+__device__ __forceinline__ void walk_neighbors_delta(GViewU codes, GViewU esc, uint32_t stride, uint32_t cnt,
+                                                     Load load, Body body) {
+    static_assert(U % 2 == 0, "codes come in pairs");
+    using T = decltype(load(0u));
+    DeltaCursor q;
+    q.prev = 0;
+    q.qa = esc[0];
+    q.qb = esc[stride];
+    q.many = (q.qa & kDeltaManyBit) != 0u;
+    q.next = 2;
+    uint32_t e = 0;
+    for (; e + U <= cnt; e += U) {
+        uint32_t w[U / 2];
+#pragma unroll
+        for (int r = 0; r < U / 2; r++) w[r] = codes[(e / 2 + r) * stride];
+        uint32_t ent[U];
+#pragma unroll
+        for (int t = 0; t < U; t++) ent[t] = delta_entry((t & 1) ? (w[t / 2] >> 16) : w[t / 2], q, esc, stride);
+        T d[U];
+#pragma unroll
+        for (int t = 0; t < U; t++) d[t] = load(ent[t] & kIndexMask);
+#pragma unroll
+        for (int t = 0; t < U; t++) body(ent[t], d[t]);
+    }
+    for (; e < cnt; e++) {
+        const uint32_t w = codes[(e / 2) * stride];
+        const uint32_t en = delta_entry((e & 1u) ? (w >> 16) : w, q, esc, stride);
+        const T dd = load(en & kIndexMask);
+        body(en, dd);
+    }
+}
+
+// The neighbor list of fluid rank k: delta-coded when the launch is
+// specialised for it (Spec & kSpecDelta: every scene of the launch keeps delta
+// lists) and the last search could code every list, otherwise the 32-bit entries.
+template <uint32_t Spec, int U, class Load, class Body>
+__device__ __forceinline__ void walk_list(const SceneDev& S, uint32_t k, uint32_t cnt, Load load, Body body) {
+    if ((Spec & kSpecDelta) && !S.state->delta_plain) {
+        walk_neighbors_delta<(U < 2 ? 2 : U)>(gload_ptr(S.dlist + k), gload_ptr(S.dlist + kDeltaCodeRows * S.list_stride + k),
+                                           S.list_stride, cnt, load, body);
+    } else {
+        walk_neighbors<U>(gload_ptr(S.nbr + k), S.list_stride, cnt, load, body);
     }
 }
 

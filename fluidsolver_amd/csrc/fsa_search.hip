@@ -1148,65 +1148,96 @@ __global__ __launch_bounds__(kBlock) void k3u_lists(const SceneDev* __restrict__
 // where the reference's fixed NeighborStorage overflows; the scene then
 // reports the capacity error.  Block totals of neighbors and encoded bytes go
 // to the scene state.
+// With Encode, each list is also written delta-coded (kFlagDeltaLists): one
+// 16-bit code per entry, the first entry and deltas beyond kDeltaMax in the
+// escape rows (layout at kDeltaCodeRows in the device header).
+template <bool Encode>
 __global__ __launch_bounds__(kBlock) void k_codec(const SceneDev* __restrict__ scenes,
                                                   const BlockRef* __restrict__ map) {
     const BlockRef br = map[blockIdx.x];
     const SceneDev& S = scenes[br.scene];
-    __shared__ uint64_t wsum[2][kBlock / 64];
+    __shared__ uint64_t wsum[3][kBlock / 64];
     const uint32_t k = br.block * kBlock + threadIdx.x;
-    uint64_t cnt = 0, bytes = 0;
-    bool over = false;
+    const bool encode = Encode && (S.flags & kFlagDeltaLists) != 0;
+    uint64_t cnt = 0, bytes = 0, dbytes = 0;
+    bool over = false, spill = false;
     if (k < S.nf) {
         const uint32_t n = S.nbr_cnt[k];
         cnt = n;
         if (n) {
             const auto nb = gload_ptr(S.nbr + k);
             const uint32_t stride = S.list_stride;
-            uint32_t prev = nb[0] & kIndexMask, payload = 0;
+            FSA_GLOBAL(uint32_t)* __restrict__ codes = (FSA_GLOBAL(uint32_t)*)S.dlist + k;
+            FSA_GLOBAL(uint32_t)* __restrict__ esc = codes + kDeltaCodeRows * stride;
+            const uint32_t first = nb[0];
+            uint32_t prev = first & kIndexMask, payload = 0;
+            uint32_t nesc = 1;  // the first entry is escape 0
+            uint32_t low = 0;   // code of the last even entry, waiting for its pair (entry 0: an escape)
+            auto code = [&](uint32_t e, uint32_t ent) {
+                const uint32_t j = ent & kIndexMask;
+                const uint32_t d = j - prev;
+                prev = j;
+                payload += d <= 2u ? 0u : (d <= 258u ? 1u : 4u);
+                if (encode) {
+                    uint32_t c = 0;
+                    if (d <= kDeltaMax) {
+                        c = d | ((ent >> 16) & 0x8000u);
+                    } else {
+                        if (nesc < kDeltaEscRows) esc[nesc * stride] = ent;
+                        nesc++;
+                    }
+                    if (e & 1u)
+                        codes[(e / 2u) * stride] = low | c << 16;
+                    else
+                        low = c;
+                }
+            };
             uint32_t e = 1;
             // eight independent loads in flight, then the deltas in list order
             for (; e + 8 <= n; e += 8) {
                 uint32_t j[8];
 #pragma unroll
-                for (int q = 0; q < 8; q++) j[q] = nb[(e + q) * stride] & kIndexMask;
+                for (int q = 0; q < 8; q++) j[q] = nb[(e + q) * stride];
 #pragma unroll
-                for (int q = 0; q < 8; q++) {
-                    const uint32_t d = j[q] - prev;
-                    prev = j[q];
-                    payload += d <= 2u ? 0u : (d <= 258u ? 1u : 4u);
-                }
+                for (int q = 0; q < 8; q++) code(e + q, j[q]);
             }
-            for (; e < n; e++) {
-                const uint32_t j = nb[e * stride] & kIndexMask;
-                const uint32_t d = j - prev;
-                prev = j;
-                payload += d <= 2u ? 0u : (d <= 258u ? 1u : 4u);
+            for (; e < n; e++) code(e, nb[e * stride]);
+            if (encode) {
+                if (n & 1u) codes[((n - 1u) / 2u) * stride] = low;  // the last entry is even
+                esc[0] = first | (nesc > 2u ? kDeltaManyBit : 0u);
+                spill = nesc > kDeltaEscRows;
+                dbytes = 4u * ((n + 1u) / 2u) + 4u * min(nesc, kDeltaEscRows);
             }
             over = payload > 96u || n - 1u > 96u;
             bytes = 4u + (2u * (n - 1u) + 7u) / 8u + payload;
         }
     }
+    if (encode && __any(spill) && __lane_id() == 0) atomicOr(&S.state->delta_spill, 1u);
     if ((S.flags & kFlagCodec) && __any(over) && __lane_id() == 0) atomicOr(&S.state->status, kStatusCapacity);
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) {
         cnt += __shfl_xor(cnt, o, 64);
         bytes += __shfl_xor(bytes, o, 64);
+        dbytes += __shfl_xor(dbytes, o, 64);
     }
     if (__lane_id() == 0) {
         wsum[0][threadIdx.x / 64] = cnt;
         wsum[1][threadIdx.x / 64] = bytes;
+        wsum[2][threadIdx.x / 64] = dbytes;
     }
     __syncthreads();
     if (threadIdx.x == 0) {
-        uint64_t c = 0, b = 0;
+        uint64_t c = 0, b = 0, db = 0;
         for (int w = 0; w < kBlock / 64; w++) {
             c += wsum[0][w];
             b += wsum[1][w];
+            db += wsum[2][w];
         }
-        // block totals as a partial: neighbors in count, encoded bytes in err
+        // block totals as a partial: neighbors in count, encoded bytes in err, delta-coded bytes in aux
         Partial p{};
         p.count = (uint32_t)c;
         p.err = (double)b;
+        p.aux = (uint32_t)db;
         S.partial[br.block] = p;
     }
 }
@@ -1215,27 +1246,30 @@ __global__ __launch_bounds__(kBlock) void k_codec(const SceneDev* __restrict__ s
 __global__ __launch_bounds__(kBlock) void k_codec_finalize(const SceneDev* __restrict__ scenes,
                                                            const BlockRef* __restrict__ map) {
     const SceneDev& S = scenes[map[blockIdx.x].scene];
-    __shared__ uint64_t wsum[2][kBlock / 64];
+    __shared__ uint64_t wsum[3][kBlock / 64];
     const uint32_t nparts = (S.nf + kBlock - 1) / kBlock;
-    uint64_t c = 0, b = 0;
+    uint64_t c = 0, b = 0, db = 0;
     // integer sums: 16 partials per thread are loaded ahead of their adds
     constexpr uint32_t kAhead = 16;
     for (uint32_t q0 = threadIdx.x; q0 < nparts; q0 += kAhead * kBlock) {
-        uint32_t qc[kAhead];
+        uint32_t qc[kAhead], qd[kAhead];
         double qe[kAhead];
 #pragma unroll
         for (uint32_t u = 0; u < kAhead; u++) {
             const uint32_t q = q0 + u * kBlock;
             qc[u] = 0;
+            qd[u] = 0;
             qe[u] = 0.0;
             if (q < nparts) {
                 qc[u] = S.partial[q].count;
+                qd[u] = S.partial[q].aux;
                 qe[u] = S.partial[q].err;
             }
         }
 #pragma unroll
         for (uint32_t u = 0; u < kAhead; u++) {
             c += qc[u];
+            db += qd[u];
             b += (uint64_t)qe[u];
         }
     }
@@ -1243,20 +1277,27 @@ __global__ __launch_bounds__(kBlock) void k_codec_finalize(const SceneDev* __res
     for (int o = 32; o > 0; o >>= 1) {
         c += __shfl_xor(c, o, 64);
         b += __shfl_xor(b, o, 64);
+        db += __shfl_xor(db, o, 64);
     }
     if (__lane_id() == 0) {
         wsum[0][threadIdx.x / 64] = c;
         wsum[1][threadIdx.x / 64] = b;
+        wsum[2][threadIdx.x / 64] = db;
     }
     __syncthreads();
     if (threadIdx.x == 0) {
-        uint64_t tc = 0, tb = 0;
+        uint64_t tc = 0, tb = 0, tdb = 0;
         for (int w = 0; w < kBlock / 64; w++) {
             tc += wsum[0][w];
             tb += wsum[1][w];
+            tdb += wsum[2][w];
         }
         S.state->total_neighbors = tc;
         S.state->codec_bytes = tb;
+        // this search's delta lists are walked unless one of them did not fit
+        S.state->delta_bytes = tdb;
+        S.state->delta_plain = S.state->delta_spill;
+        S.state->delta_spill = 0u;
     }
 }
 
@@ -1314,7 +1355,12 @@ void launch_permute_only(const LaunchSet& L, uint32_t sbuf, hipStream_t st) {
 }
 
 void launch_codec(const LaunchSet& L, hipStream_t st) {
-    if (L.nb_fluid) k_codec<<<L.nb_fluid, kBlock, 0, st>>>(L.scenes, L.map_fluid);
+    if (L.nb_fluid) {
+        if (L.spec & kSpecDelta)
+            k_codec<true><<<L.nb_fluid, kBlock, 0, st>>>(L.scenes, L.map_fluid);
+        else
+            k_codec<false><<<L.nb_fluid, kBlock, 0, st>>>(L.scenes, L.map_fluid);
+    }
     k_codec_finalize<<<L.nb_scene, kBlock, 0, st>>>(L.scenes, L.map_scene);
 }
 

@@ -93,7 +93,7 @@ __device__ __forceinline__ Partial block_reduce(double err, uint32_t cnt, float 
     p.count = 0;
     p.max_a2 = 0.0f;
     p.max_v2 = 0.0f;
-    p.pad = 0;
+    p.aux = 0;
     for (int k = 0; k < kBlock / 64; k++) {
         p.err += lds.err[k];
         p.count += lds.cnt[k];
@@ -169,8 +169,8 @@ __device__ __forceinline__ void iisph_prepare_body(const SceneDev* __restrict__ 
     const bool sl = (S.flags & kFlagSingleLayer) != 0;
     float tx = 0.0f, ty = 0.0f, tz = 0.0f, rho = 0.0f;
     float ix = 0.0f, iy = 0.0f, iz = 0.0f;  // d_ii of phase 2, from the same pair gradients
-    walk_neighbors<U>(
-        gload_ptr(S.nbr + k), S.list_stride, S.nbr_cnt[k],
+    walk_list<Spec, U>(
+        S, k, S.nbr_cnt[k],
         [&](uint32_t j) { return PosVel{pos[j], vel[j]}; },
         [&](uint32_t ent, const PosVel& d) {
             const float4 xj = d.x, vj = d.v;
@@ -301,14 +301,13 @@ __device__ __forceinline__ void iisph_diagonal_body(const SceneDev* __restrict__
         const float4 xi = pos[i];
         const float4 vs = vstar[i];
         const uint32_t cnt = S.nbr_cnt[k];
-        const auto nb = gload_ptr(S.nbr + k);
         // d_ii was summed in phase 1 over the same pairs, in list order
         const float4 di = S.dii[k];
         const float ix = di.x, iy = di.y, iz = di.z;
         const float cm = div_rho0sq<Spec>(S, xi.w);
         float contrib = 0.0f, dg = 0.0f;
-        walk_neighbors<U>(
-            nb, S.list_stride, cnt, [&](uint32_t j) { return PosVel{pos[j], vstar[j]}; },
+        walk_list<Spec, U>(
+            S, k, cnt, [&](uint32_t j) { return PosVel{pos[j], vstar[j]}; },
             [&](uint32_t ent, const PosVel& d) {
                 const float4 xj = d.x;
                 float dx, dy, dz, d2, len;
@@ -393,8 +392,8 @@ __device__ __forceinline__ void iisph_pass_a_body(const SceneDev* __restrict__ s
     const float pir = div_rho0sq<Spec>(S, pi);
     const bool sl = (S.flags & kFlagSingleLayer) != 0;
     float ax = 0.0f, ay = 0.0f, az = 0.0f;
-    walk_neighbors<U>(
-        gload_ptr(S.nbr + k), S.list_stride, S.nbr_cnt[k], [&](uint32_t j) { return PosP{pos[j], pres[j]}; },
+    walk_list<Spec, U>(
+        S, k, S.nbr_cnt[k], [&](uint32_t j) { return PosP{pos[j], pres[j]}; },
         [&](uint32_t ent, const PosP& d) {
             const float4 xj = d.x;
             float dx, dy, dz, d2, len;
@@ -447,8 +446,8 @@ __device__ __forceinline__ void iisph_pass_b_body(const SceneDev* __restrict__ s
         const float4 xi = pos[i];
         const float4 ai = acc[i];
         float ap = 0.0f;
-        walk_neighbors<U>(
-            gload_ptr(S.nbr + k), S.list_stride, S.nbr_cnt[k], [&](uint32_t j) { return PosAcc{pos[j], acc[j]}; },
+        walk_list<Spec, U>(
+            S, k, S.nbr_cnt[k], [&](uint32_t j) { return PosAcc{pos[j], acc[j]}; },
             [&](uint32_t ent, const PosAcc& d) {
                 const float4 xj = d.x;
                 float dx, dy, dz, d2, len;
@@ -518,8 +517,8 @@ __global__ __launch_bounds__(kBlock) void k_sesph_density(const SceneDev* __rest
     const float4 xi = pos[i];
     const bool sl = (S.flags & kFlagSingleLayer) != 0;
     float rho = 0.0f;
-    walk_neighbors<kUnroll>(
-        gload_ptr(S.nbr + k), S.list_stride, S.nbr_cnt[k], [&](uint32_t j) { return pos[j]; },
+    walk_list<Spec, kUnroll>(
+        S, k, S.nbr_cnt[k], [&](uint32_t j) { return pos[j]; },
         [&](uint32_t ent, const float4& xj) {
             const float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
             const float len = pair_length<Spec>(dot3(dx, dy, dz, dx, dy, dz));
@@ -561,8 +560,8 @@ __global__ __launch_bounds__(kBlock) void k_sesph_accel(const SceneDev* __restri
     const bool sl = (S.flags & kFlagSingleLayer) != 0;
     float tx = 0.0f, ty = 0.0f, tz = 0.0f;
     float ax = 0.0f, ay = 0.0f, az = 0.0f;
-    walk_neighbors<kUnroll>(
-        gload_ptr(S.nbr + k), S.list_stride, S.nbr_cnt[k],
+    walk_list<Spec, kUnroll>(
+        S, k, S.nbr_cnt[k],
         [&](uint32_t j) { return PosVelP{pos[j], vel[j], pres[j]}; },
         [&](uint32_t ent, const PosVelP& d) {
             const float4 xj = d.x, vj = d.v;
@@ -835,6 +834,11 @@ __global__ __launch_bounds__(kFinalizeThreads) void k_integrate_finalize(const S
 #define FSA_SPEC_DISPATCH(kernel, grid, map)                                                        \
     do {                                                                                            \
         switch (L.spec) {                                                                           \
+            case 15: kernel<15><<<grid, kBlock, 0, st>>>(L.scenes, map); break;                     \
+            case 13: kernel<13><<<grid, kBlock, 0, st>>>(L.scenes, map); break;                     \
+            case 11: kernel<11><<<grid, kBlock, 0, st>>>(L.scenes, map); break;                     \
+            case 9: kernel<9><<<grid, kBlock, 0, st>>>(L.scenes, map); break;                       \
+            case 8: kernel<8><<<grid, kBlock, 0, st>>>(L.scenes, map); break;                       \
             case 7: kernel<7><<<grid, kBlock, 0, st>>>(L.scenes, map); break;                       \
             case 5: kernel<5><<<grid, kBlock, 0, st>>>(L.scenes, map); break;                       \
             case 3: kernel<3><<<grid, kBlock, 0, st>>>(L.scenes, map); break;                       \
@@ -847,6 +851,11 @@ __global__ __launch_bounds__(kFinalizeThreads) void k_integrate_finalize(const S
 #define FSA_SPEC_U(kernel, U, grid, map)                                                            \
     do {                                                                                            \
         switch (L.spec) {                                                                           \
+            case 15: kernel<15, U><<<grid, kBlock, 0, st>>>(L.scenes, map); break;                  \
+            case 13: kernel<13, U><<<grid, kBlock, 0, st>>>(L.scenes, map); break;                  \
+            case 11: kernel<11, U><<<grid, kBlock, 0, st>>>(L.scenes, map); break;                  \
+            case 9: kernel<9, U><<<grid, kBlock, 0, st>>>(L.scenes, map); break;                    \
+            case 8: kernel<8, U><<<grid, kBlock, 0, st>>>(L.scenes, map); break;                    \
             case 7: kernel<7, U><<<grid, kBlock, 0, st>>>(L.scenes, map); break;                    \
             case 5: kernel<5, U><<<grid, kBlock, 0, st>>>(L.scenes, map); break;                    \
             case 3: kernel<3, U><<<grid, kBlock, 0, st>>>(L.scenes, map); break;                    \
@@ -859,6 +868,11 @@ __global__ __launch_bounds__(kFinalizeThreads) void k_integrate_finalize(const S
 #define FSA_SPEC_UW(kernel, U, W, grid, map)                                                        \
     do {                                                                                            \
         switch (L.spec) {                                                                           \
+            case 15: kernel##_w<15, U, W><<<grid, kBlock, 0, st>>>(L.scenes, map); break;           \
+            case 13: kernel##_w<13, U, W><<<grid, kBlock, 0, st>>>(L.scenes, map); break;           \
+            case 11: kernel##_w<11, U, W><<<grid, kBlock, 0, st>>>(L.scenes, map); break;           \
+            case 9: kernel##_w<9, U, W><<<grid, kBlock, 0, st>>>(L.scenes, map); break;             \
+            case 8: kernel##_w<8, U, W><<<grid, kBlock, 0, st>>>(L.scenes, map); break;             \
             case 7: kernel##_w<7, U, W><<<grid, kBlock, 0, st>>>(L.scenes, map); break;             \
             case 5: kernel##_w<5, U, W><<<grid, kBlock, 0, st>>>(L.scenes, map); break;             \
             case 3: kernel##_w<3, U, W><<<grid, kBlock, 0, st>>>(L.scenes, map); break;             \

@@ -62,13 +62,20 @@ extern "C" {
 #define FSA_SOLVER_IISPH 0   /* IISPHFluidSolver3D / IISPHFluidSolver            */
 #define FSA_SOLVER_SESPH 1   /* SESPHFluidSolver3D / SESPHFluidSolver            */
 
-/* fsa_scene_desc.neighbor_lists (3D).  The device walks 32-bit neighbor lists
- * in either case; COMPRESSED also sizes every list in the compressed search's
- * NeighborStorage delta code each search (CompressedNeighbors.cpp:260-330) and
- * fails the step with FSA_E_CAPACITY where that fixed-size storage would
- * overflow (more than 96 deltas or 96 payload bytes), as the reference does. */
+/* fsa_scene_desc.neighbor_lists (3D).  The search writes 32-bit neighbor
+ * lists in every case; COMPRESSED also sizes every list in the compressed
+ * search's NeighborStorage delta code each search (CompressedNeighbors.cpp:260-330)
+ * and fails the step with FSA_E_CAPACITY where that fixed-size storage would
+ * overflow (more than 96 deltas or 96 payload bytes), as the reference does.
+ * DELTA (compressed-3d search only; other searches treat it as COMPRESSED) does
+ * the same and also re-encodes each list as 16-bit deltas to the previous
+ * neighbor (the compressed search's NeighborStorage idea, CompressedNeighbors.hpp:26-60,
+ * in a layout the GPU reads two entries at a time), which the solver walks
+ * read instead of the 32-bit entries; the lists and results are the same.
+ * A scene cannot switch to or from DELTA after creation. */
 #define FSA_LIST_COMPRESSED 0 /* NeighborStorage capacity and sizes (default)    */
 #define FSA_LIST_PLAIN 1      /* 97-neighbor capacity only                       */
+#define FSA_LIST_DELTA 2      /* COMPRESSED + walks over 16-bit delta-coded lists */
 
 /* fsa_scene_desc.search: the reference search whose neighbor order the device
  * reproduces (registration strings of SolverSerializer.cpp:137-219). */
