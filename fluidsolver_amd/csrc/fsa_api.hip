@@ -631,7 +631,7 @@ void search_set(fsa_context ctx, const LaunchSet& L, uint32_t passes) {
         PhaseTimer tl(ctx, FSA_TIMER_LISTS, true);
         launch_lists(L, passes & 1u, ctx->stream);
     }
-    if (L.codec) launch_codec(L, ctx->stream);
+    if (L.codec) launch_codec(L, ctx->stream, false);
 }
 
 void simulate_set(fsa_context ctx, const LaunchSet& L, std::vector<fsa_scene_s*>& scenes, bool sesph,
@@ -2704,7 +2704,7 @@ FSA_API int32_t fsa_scene_neighbor_stats(fsa_scene s, uint64_t* total, uint64_t*
         // the NeighborStorage sizes of the current lists, counted on the device
         upload_scene_table(s);
         LaunchSet L = s->single;
-        launch_codec(L, s->ctx->stream);
+        launch_codec(L, s->ctx->stream, true);
         read_state(s);
         if (total) *total = s->h_state->total_neighbors;
         if (dev_bytes)

@@ -96,6 +96,10 @@ struct alignas(16) SceneState {
     uint32_t delta_spill;  // delta lists: some list of the search in progress exceeds the escape rows
     uint32_t delta_plain;  // delta lists: the last search's walks read the 32-bit lists
     uint64_t delta_bytes;  // delta lists: code and escape bytes of all lists of the last search
+    // compressed search: some list of the search in progress may need more than the
+    // NeighborStorage payload (k_lists' bound), so k_codec sizes every list exactly
+    uint32_t codec_check;
+    uint32_t pad_codec[3];
 };
 
 // max of (v2, a2) over a kBlock-thread block, NaN ignored; every thread gets the result

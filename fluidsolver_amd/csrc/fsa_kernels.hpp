@@ -71,8 +71,10 @@ void launch3u_search(const LaunchSet& L, uint32_t passes, hipStream_t st);
 // gather every particle array into the order vals[sbuf] gives, flip the buffers,
 // rebuild the fluid rank maps
 void launch_permute_only(const LaunchSet& L, uint32_t sbuf, hipStream_t st);
-// NeighborStorage sizes and capacity of the lists just built
-void launch_codec(const LaunchSet& L, hipStream_t st);
+// NeighborStorage sizes and capacity of the lists just built: every scene's with
+// `force`, otherwise those whose lists k_lists could not bound within the payload
+// (and delta-coded ones, which are encoded here)
+void launch_codec(const LaunchSet& L, hipStream_t st, bool force);
 // stable LSD radix passes over (keys, vals)[first_src], digits from bit shift0 upward
 void launch_sort_range(const LaunchSet& L, uint32_t first_src, uint32_t shift0, uint32_t passes, hipStream_t st);
 

@@ -560,6 +560,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) voi
     }
     if (starts == 0) return;
     bool overflow = false;
+    bool codec_check = false;  // some list may exceed the NeighborStorage payload
     // readout lanes: particle column lane % kStageParticles, row offset lane / kStageParticles
     const uint32_t col = lane % kStageParticles;
     const uint32_t row_step = 64u / kStageParticles;
@@ -657,6 +658,30 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) voi
         }
         uint32_t total;
         const uint32_t roff = exclusive_scan64(rl, total);
+        // NeighborStorage bound: the cell's non-empty runs (in key order, so in
+        // ascending rows) fall into `segs` groups of consecutive rows (run i
+        // continues the previous non-empty run exactly when rs - roff is the same
+        // for both).  While no group spans more than 259 rows, a delta inside a
+        // group is at most 258 (at most 1 payload byte) and only a move to the
+        // next group can take 4, so a list of c entries needs at most
+        // c - 1 + 3 (segs - 1) bytes.  With at most 259 candidates no group can
+        // be longer; otherwise the longest group is measured.
+        uint32_t segs = 0;
+        bool long_group = false;
+        {
+            const bool ne = rl != 0;
+            const uint32_t jb = ne ? rs - roff + 1u : 0u;
+            const uint32_t upto = inclusive_max64(jb);
+            const uint32_t before = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)upto, 0x138, 0xf, 0xf, false);
+            const bool newseg = ne && jb > before;
+            segs = (uint32_t)__popcll(__ballot(newseg));
+            if (total > 259u) {
+                // each non-empty run's group starts at the last group start at or before it
+                const uint32_t gs = inclusive_max64(newseg ? roff + 1u : 0u);
+                const uint32_t glen = ne ? roff + rl - (gs - 1u) : 0u;
+                long_group = readlane_u(inclusive_max64(glen), 63) > 259u;
+            }
+        }
         // Up to kListChunks * 64 candidates (the usual case): each non-empty run
         // marks the slot of its first candidate with its lane + 1, so the run of
         // candidate g = k * 64 + lane is the largest mark at or below slot g (a
@@ -801,6 +826,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) voi
             const uint32_t kept = min(cntl, (uint32_t)kMaxNeighbors);
             if (isf) {
                 if (cntl > (uint32_t)kMaxNeighbors) overflow = true;
+                if (long_group || cntl + 3u * segs > 100u) codec_check = true;
                 S.nbr_cnt[kl] = kept;
                 S.fluid_idx[kl] = pi;
                 stage_cnt[kl - fbase] = kept;
@@ -812,6 +838,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) voi
     }
     flush();
     if (__any(overflow) && lane == 0) atomicOr(&S.state->status, kStatusCapacity);
+    if ((S.flags & kFlagCodec) && __any(codec_check) && lane == 0) atomicOr(&S.state->codec_check, 1u);
 }
 
 // ---- 8. hashed / quadratic 3D searches (collection order kept) -------------------
@@ -1151,11 +1178,18 @@ __global__ __launch_bounds__(kBlock) void k3u_lists(const SceneDev* __restrict__
 // With Encode, each list is also written delta-coded (kFlagDeltaLists): one
 // 16-bit code per entry, the first entry and deltas beyond kDeltaMax in the
 // escape rows (layout at kDeltaCodeRows in the device header).
+// Without `force`, a scene's lists are sized only when k_lists could not bound
+// them within the NeighborStorage payload, or when they are delta-coded.
+__device__ __forceinline__ bool codec_due(const SceneDev& S, uint32_t force) {
+    return force || (S.flags & kFlagDeltaLists) || S.state->codec_check;
+}
+
 template <bool Encode>
 __global__ __launch_bounds__(kBlock) void k_codec(const SceneDev* __restrict__ scenes,
-                                                  const BlockRef* __restrict__ map) {
+                                                  const BlockRef* __restrict__ map, uint32_t force) {
     const BlockRef br = map[blockIdx.x];
     const SceneDev& S = scenes[br.scene];
+    if (!codec_due(S, force)) return;
     __shared__ uint64_t wsum[3][kBlock / 64];
     const uint32_t k = br.block * kBlock + threadIdx.x;
     const bool encode = Encode && (S.flags & kFlagDeltaLists) != 0;
@@ -1244,8 +1278,9 @@ __global__ __launch_bounds__(kBlock) void k_codec(const SceneDev* __restrict__ s
 
 // one block per scene: the block partials of k_codec, in block order
 __global__ __launch_bounds__(kBlock) void k_codec_finalize(const SceneDev* __restrict__ scenes,
-                                                           const BlockRef* __restrict__ map) {
+                                                           const BlockRef* __restrict__ map, uint32_t force) {
     const SceneDev& S = scenes[map[blockIdx.x].scene];
+    if (!codec_due(S, force)) return;
     __shared__ uint64_t wsum[3][kBlock / 64];
     const uint32_t nparts = (S.nf + kBlock - 1) / kBlock;
     uint64_t c = 0, b = 0, db = 0;
@@ -1298,6 +1333,7 @@ __global__ __launch_bounds__(kBlock) void k_codec_finalize(const SceneDev* __res
         S.state->delta_bytes = tdb;
         S.state->delta_plain = S.state->delta_spill;
         S.state->delta_spill = 0u;
+        S.state->codec_check = 0u;
     }
 }
 
@@ -1354,14 +1390,15 @@ void launch_permute_only(const LaunchSet& L, uint32_t sbuf, hipStream_t st) {
     k_compact<<<L.nb_all, kBlock, 0, st>>>(L.scenes, L.map_all);
 }
 
-void launch_codec(const LaunchSet& L, hipStream_t st) {
+void launch_codec(const LaunchSet& L, hipStream_t st, bool force) {
+    const uint32_t f = force ? 1u : 0u;
     if (L.nb_fluid) {
         if (L.spec & kSpecDelta)
-            k_codec<true><<<L.nb_fluid, kBlock, 0, st>>>(L.scenes, L.map_fluid);
+            k_codec<true><<<L.nb_fluid, kBlock, 0, st>>>(L.scenes, L.map_fluid, f);
         else
-            k_codec<false><<<L.nb_fluid, kBlock, 0, st>>>(L.scenes, L.map_fluid);
+            k_codec<false><<<L.nb_fluid, kBlock, 0, st>>>(L.scenes, L.map_fluid, f);
     }
-    k_codec_finalize<<<L.nb_scene, kBlock, 0, st>>>(L.scenes, L.map_scene);
+    k_codec_finalize<<<L.nb_scene, kBlock, 0, st>>>(L.scenes, L.map_scene, f);
 }
 
 }  // namespace fsa
