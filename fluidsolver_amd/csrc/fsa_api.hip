@@ -123,8 +123,12 @@ struct fsa_scene_s {
     SceneDev* d_table = nullptr;   // single-scene launch table
     void* dmem = nullptr;
     uint64_t bytes = 0;
-    uint4* cells = nullptr;
+    uint4* cells = nullptr;     // cell table, then the rank table (kCellBytes per grid cell)
     uint64_t cells_cap = 0;
+    uint32_t* ranks = nullptr;
+    bool ranks_valid = false;   // the rank table holds the grid below
+    int32_t rank_cmin[3] = {0, 0, 0};
+    uint32_t rank_cdim[3] = {0, 0, 0}, rank_levels = 0;
     SceneState* h_state = nullptr;  // pinned mirror of this scene's slot in the context pool
     uint32_t slot = 0;
     LaunchSet single{};
@@ -340,7 +344,7 @@ uint64_t plan_bytes(const Plan& p) {
     add(4 * n);                                // diag
     add(16 * std::max<uint64_t>(p.nf, 1));     // dii
     for (int k = 0; k < 4; k++) add(4 * n);    // keys, vals x2
-    add(4 * (kRadix * std::max<uint64_t>(p.ntiles, 1) + kRadix));
+    add(4 * (kRadixMaxDigits * std::max<uint64_t>(p.ntiles, 1) + kRadixMaxDigits));
     add(4 * std::max<uint64_t>(p.nf, 1));      // fluid_idx
     add(4 * n);                                // rank_of
     add(4 * std::max<uint64_t>(p.nblocks, 1));
@@ -382,7 +386,7 @@ void carve(fsa_scene_s* s, const Plan& p) {
     h.dii = (float4*)take(16 * std::max<uint64_t>(p.nf, 1));
     for (int k = 0; k < 2; k++) h.keys[k] = (uint32_t*)take(4 * n);
     for (int k = 0; k < 2; k++) h.vals[k] = (uint32_t*)take(4 * n);
-    h.hist = (uint32_t*)take(4 * (kRadix * std::max<uint64_t>(p.ntiles, 1) + kRadix));
+    h.hist = (uint32_t*)take(4 * (kRadixMaxDigits * std::max<uint64_t>(p.ntiles, 1) + kRadixMaxDigits));
     h.fluid_idx = (uint32_t*)take(4 * std::max<uint64_t>(p.nf, 1));
     h.rank_of = (uint32_t*)take(4 * n);
     h.block_count = (uint32_t*)take(4 * std::max<uint64_t>(p.nblocks, 1));
@@ -478,6 +482,11 @@ void build_launch_set(const MapBuilder& mb, SceneDev* table, LaunchSet& L, std::
 // ---- grid ---------------------------------------------------------------------------
 inline int32_t host_cell(float x, float r) { return (int32_t)std::floor(x / r); }
 
+constexpr uint64_t kCellBytes = sizeof(uint4) + sizeof(uint32_t);  // a cell table and a rank table entry
+uint32_t bits_of(uint64_t v);
+// passes of the compressed search's sort over compact keys of L levels (3L + 3 bits)
+uint32_t key_passes(uint32_t L) { return (3 * L + 3 + kSearchSortBits - 1) / kSearchSortBits; }
+
 uint32_t levels_for(const int32_t lo[3], const int32_t hi[3]) {
     uint32_t L = 1;
     for (;;) {
@@ -529,17 +538,42 @@ void prepare_grid(fsa_scene_s* s) {
     if (cells > s->cells_cap) {
         if (s->cells) {
             HIP_CHECK(hipFree(s->cells));
-            s->ctx->in_use -= s->cells_cap * sizeof(uint4);
+            s->ctx->in_use -= s->cells_cap * kCellBytes;
         }
         const uint64_t cap = cells + cells / 4 + 64;
-        if (s->ctx->budget && s->ctx->in_use + cap * sizeof(uint4) > s->ctx->budget)
+        if (s->ctx->budget && s->ctx->in_use + cap * kCellBytes > s->ctx->budget)
             throw FsaError(FSA_E_BUDGET, "cell table exceeds the HBM budget");
-        HIP_CHECK(hipMalloc(&s->cells, cap * sizeof(uint4)));
+        HIP_CHECK(hipMalloc(&s->cells, cap * kCellBytes));
         HIP_CHECK(hipMemsetAsync(s->cells, 0, cap * sizeof(uint4), s->ctx->stream));
         s->cells_cap = cap;
-        s->ctx->in_use += cap * sizeof(uint4);
+        s->ranks = (uint32_t*)(s->cells + cap);
+        s->ranks_valid = false;
+        s->ctx->in_use += cap * kCellBytes;
     }
     h.cells = s->cells;
+    // The compressed search sorts by each cell's rank among the grid's cells
+    // (the compact-key order in fewer bits) when that saves a sort pass; the
+    // rank table is rebuilt when the grid changes.
+    h.rank_keys = 0;
+    h.ranks = s->ranks;
+    if (h.dim == 3 && !(h.flags & kFlagUnsorted)) {
+        const uint32_t rank_bits = bits_of(cells - 1);
+        const uint32_t rank_passes = std::max<uint32_t>(1, (rank_bits + kSearchSortBits - 1) / kSearchSortBits);
+        if (rank_passes < key_passes(L)) {
+            h.rank_keys = 1;
+            bool same = s->ranks_valid && s->rank_levels == L;
+            for (int a = 0; a < 3; a++) same = same && s->rank_cmin[a] == h.cmin[a] && s->rank_cdim[a] == h.cdim[a];
+            if (!same) {
+                launch_rank_table(s->ranks, h.cmin, h.cdim, L, s->ctx->stream);
+                for (int a = 0; a < 3; a++) {
+                    s->rank_cmin[a] = h.cmin[a];
+                    s->rank_cdim[a] = h.cdim[a];
+                }
+                s->rank_levels = L;
+                s->ranks_valid = true;
+            }
+        }
+    }
     h.stamp++;
     if (h.stamp == 0) {
         HIP_CHECK(hipMemsetAsync(s->cells, 0, s->cells_cap * sizeof(uint4), s->ctx->stream));
@@ -563,7 +597,11 @@ bool tiny_coordinate(float v) {
     return a > 0.0f && a < kTinyCoord;
 }
 
-uint32_t sort_passes(const SceneDev& h) { return (3 * h.levels + 3 + kRadixBits - 1) / kRadixBits; }
+uint32_t sort_passes(const SceneDev& h) {
+    if (h.rank_keys)
+        return std::max<uint32_t>(1, (bits_of((uint64_t)h.cdim[3] - 1) + kSearchSortBits - 1) / kSearchSortBits);
+    return key_passes(h.levels);
+}
 
 void read_state(fsa_scene_s* s) {
     HIP_CHECK(hipMemcpyAsync(s->h_state, s->hd.state, sizeof(SceneState), hipMemcpyDeviceToHost, s->ctx->stream));
@@ -2275,7 +2313,7 @@ void destroy_scene(fsa_scene_s* s) {
     }
     if (s->cells) {
         (void)hipFree(s->cells);
-        s->ctx->in_use -= s->cells_cap * sizeof(uint4);
+        s->ctx->in_use -= s->cells_cap * kCellBytes;
     }
     if (s->qs_mem) {
         (void)hipFree(s->qs_mem);
@@ -2361,7 +2399,7 @@ void admit(fsa_server srv) {
             if (j->info[8 * i + 4] == FSA_TYPE_BOUNDARY) nb++;
         // scene plan + cell table allowance (the grid of a dam scene is about the particle count)
         const uint64_t need = plan_bytes(make_plan(j->fluid + nb, j->fluid, 3, false, delta_lists(j->desc))) +
-                              sizeof(uint4) * (j->fluid + nb);
+                              kCellBytes * (j->fluid + nb);
         const uint64_t budget = server_budget(srv);
         if (budget && srv->ctx->in_use.load() + need > budget) {
             if (!srv->running.empty()) return;  // FIFO: wait until running jobs free memory

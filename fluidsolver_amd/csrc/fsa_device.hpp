@@ -27,8 +27,12 @@ constexpr uint32_t kIdMask = (1u << kTypeShift) - 1u;
 constexpr int kSortThreads = 256;
 constexpr int kSortItems = 8;
 constexpr int kSortTile = kSortThreads * kSortItems;  // 2048 keys per tile
-constexpr int kRadixBits = 9;  // 512 digits: a 27-bit cell key (cells in [-256, 255]) in 3 passes
+constexpr int kRadixBits = 9;  // 512 digits: the 2D and hashed-3D searches' sorts
 constexpr int kRadix = 1 << kRadixBits;
+// The compressed search sorts 10-bit digits: the cell rank keys of grids up to
+// 2^20 cells in 2 passes, compact keys up to 30 bits in 3.
+constexpr int kSearchSortBits = 10;
+constexpr int kRadixMaxDigits = 1 << kSearchSortBits;  // tile counts the histogram buffer holds per tile
 
 // SceneDev.flags
 constexpr uint32_t kFlagHOne = 1u;          // kernel h == 1: q = |x|, pre = x / |x| exactly
@@ -185,6 +189,8 @@ struct alignas(16) SceneDev {
     int32_t cmin[4];
     uint32_t cdim[4];      // x, y, z, cell count
     uint32_t levels;       // Morton levels of the compact key
+    uint32_t rank_keys;    // compressed search: sort by the rank table instead of the compact key
+    const uint32_t* ranks; // rank of each grid cell in compact-key order among the grid's cells
     uint32_t stamp;        // cell-table generation
     uint32_t ntiles;       // radix tiles
     uint32_t nblocks;      // kBlock-blocks over n
@@ -436,6 +442,54 @@ __device__ __host__ __forceinline__ uint32_t compact_key(int32_t cx, int32_t cy,
     return key;
 }
 
+
+// Rank of cell c in compact_key order among the cells of the box [lo, hi]
+// (inclusive, every coordinate in [-2^L, 2^L)): the number of box cells with
+// a smaller key.  The key orders first by the sign bits (sx, sz, sy), then by
+// the interleaved low L bits (z, y, x within each level, highest level first),
+// and the low bits of either sign half are monotone in the coordinate, so the
+// count is the box cells of the earlier sign octants plus, inside c's octant,
+// those of the earlier children at every level of the octree walk down to c.
+__device__ __host__ __forceinline__ uint32_t box_rank(int32_t cx, int32_t cy, int32_t cz, const int32_t lo[3],
+                                                      const int32_t hi[3], uint32_t L) {
+    const int32_t c[3] = {cx, cy, cz};
+    const int32_t half = 1 << L;
+    int32_t alo[3], ahi[3];  // the box inside c's sign half, in low-bit coordinates
+    uint32_t n0[3], n1[3];   // box cells per axis with sign 0 and with sign 1
+    uint32_t s[3];
+    for (int a = 0; a < 3; a++) {
+        const int32_t p0 = lo[a] > 0 ? lo[a] : 0, q1 = hi[a] < -1 ? hi[a] : -1;
+        n0[a] = hi[a] >= p0 ? (uint32_t)(hi[a] - p0 + 1) : 0u;
+        n1[a] = q1 >= lo[a] ? (uint32_t)(q1 - lo[a] + 1) : 0u;
+        s[a] = c[a] < 0 ? 1u : 0u;
+        alo[a] = s[a] ? lo[a] + half : p0;
+        ahi[a] = s[a] ? q1 + half : hi[a];
+    }
+    // earlier octants in (sx, sz, sy) order
+    const uint32_t tz = n0[2] + n1[2], ty = n0[1] + n1[1];
+    const uint32_t nx = s[0] ? n1[0] : n0[0], nz = s[2] ? n1[2] : n0[2];
+    uint32_t rank = (s[0] ? n0[0] * tz * ty : 0u) + nx * ((s[2] ? n0[2] * ty : 0u) + nz * (s[1] ? n0[1] : 0u));
+    // inside the octant: Morton rank of the low bits in the box [alo, ahi]
+    int32_t base[3] = {0, 0, 0};
+    const int32_t u[3] = {c[0] + (int32_t)s[0] * half, c[1] + (int32_t)s[1] * half, c[2] + (int32_t)s[2] * half};
+    for (int32_t b = (int32_t)L - 1; b >= 0; b--) {
+        const int32_t w = 1 << b;
+        uint32_t ov[3][2], bit[3];
+        for (int a = 0; a < 3; a++) {
+            for (int h = 0; h < 2; h++) {
+                const int32_t l0 = base[a] + h * w, l1 = l0 + w - 1;
+                const int32_t x0 = alo[a] > l0 ? alo[a] : l0, x1 = ahi[a] < l1 ? ahi[a] : l1;
+                ov[a][h] = x1 >= x0 ? (uint32_t)(x1 - x0 + 1) : 0u;
+            }
+            bit[a] = (uint32_t)(u[a] >> b) & 1u;
+        }
+        const uint32_t vx = ov[0][0] + ov[0][1], vy = ov[1][0] + ov[1][1];
+        rank += (bit[2] ? ov[2][0] * vy * vx : 0u) +
+                ov[2][bit[2]] * ((bit[1] ? ov[1][0] * vx : 0u) + ov[1][bit[1]] * (bit[0] ? ov[0][0] : 0u));
+        for (int a = 0; a < 3; a++) base[a] += (int32_t)bit[a] * w;
+    }
+    return rank;
+}
 
 // The reference's 64-bit cell key (CompressedNeighbors.cpp:43-56, also the
 // hashed search's re-sort key): libmorton's morton3D_64 of c + 2^31, bit b of

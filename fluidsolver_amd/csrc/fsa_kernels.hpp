@@ -64,6 +64,9 @@ uint64_t statistics_result_bytes();  // one accumulator; the result sits at the 
 
 // neighborhood search
 void launch_keys_sort(const LaunchSet& L, uint32_t passes, hipStream_t st);
+// the rank table of a grid of cdim cells from cmin (box_rank of every cell)
+void launch_rank_table(uint32_t* ranks, const int32_t cmin[3], const uint32_t cdim[3], uint32_t levels,
+                       hipStream_t st);
 void launch_reorder(const LaunchSet& L, uint32_t sbuf, hipStream_t st);
 void launch_lists(const LaunchSet& L, uint32_t sbuf, hipStream_t st);
 // hashed / quadratic 3D searches over the kept collection order

@@ -54,25 +54,28 @@ __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* l
 
 // ---- 2. radix sort -----------------------------------------------------------
 static_assert(kRadix % kSortThreads == 0, "each sort thread owns a whole number of digits");
-// peers of this lane among `valid` lanes holding the same 8-bit digit
+// peers of this lane among `valid` lanes holding the same Bits-bit digit
+template <int Bits>
 __device__ __forceinline__ uint64_t match_digit(uint32_t d, uint64_t valid) {
     uint64_t peers = valid;
 #pragma unroll
-    for (int b = 0; b < kRadixBits; b++) {
+    for (int b = 0; b < Bits; b++) {
         uint64_t bal = __ballot((d >> b) & 1u);
         peers &= ((d >> b) & 1u) ? bal : ~bal;
     }
     return peers;
 }
 
+template <int Bits>
 __global__ __launch_bounds__(kSortThreads) void k_radix_hist(const SceneDev* __restrict__ scenes,
                                                              const BlockRef* __restrict__ map, uint32_t shift,
                                                              uint32_t src) {
+    constexpr uint32_t kDigits = 1u << Bits;
     const BlockRef br = map[blockIdx.x];
     const SceneDev& S = scenes[br.scene];
     const uint32_t tile = br.block;
-    __shared__ uint32_t h[kRadix];
-    for (uint32_t d = threadIdx.x; d < (uint32_t)kRadix; d += kSortThreads) h[d] = 0;
+    __shared__ uint32_t h[kDigits];
+    for (uint32_t d = threadIdx.x; d < kDigits; d += kSortThreads) h[d] = 0;
     const uint32_t n = S.n, ntiles = S.ntiles;
     const auto keys = gload_ptr(S.keys[src]);
     FSA_GLOBAL(uint32_t)* __restrict__ hist = (FSA_GLOBAL(uint32_t)*)S.hist;
@@ -86,12 +89,12 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_hist(const SceneDev* __r
     for (int r = 0; r < kSortItems; r++) {
         const uint32_t i = base + r * kSortThreads + threadIdx.x;
         const bool valid = i < n;
-        const uint32_t d = (kk[r] >> shift) & (kRadix - 1);
-        const uint64_t peers = match_digit(d, __ballot(valid));
+        const uint32_t d = (kk[r] >> shift) & (kDigits - 1);
+        const uint64_t peers = match_digit<Bits>(d, __ballot(valid));
         if (valid && (peers & lanemask_lt()) == 0) atomicAdd(&h[d], (uint32_t)__popcll(peers));
     }
     __syncthreads();
-    for (uint32_t d = threadIdx.x; d < (uint32_t)kRadix; d += kSortThreads) hist[d * ntiles + tile] = h[d];
+    for (uint32_t d = threadIdx.x; d < kDigits; d += kSortThreads) hist[d * ntiles + tile] = h[d];
 }
 
 // wide single-block scans (one block per scene or per (scene, digit))
@@ -99,16 +102,26 @@ constexpr uint32_t kScanThreads = 1024;
 
 // k_keys and the first pass's k_radix_hist in one pass over the sort tiles:
 // each tile's keys are computed, stored, and counted by their lowest digit.
+// A particle's key is its cell's compact key, or, where the scene keeps a rank
+// table (rank_keys), the cell's rank in compact-key order among the cells of
+// the step's grid: the same order in fewer bits.
+template <int Bits>
 __global__ __launch_bounds__(kSortThreads) void k_keys_hist(const SceneDev* __restrict__ scenes,
                                                             const BlockRef* __restrict__ map) {
+    constexpr uint32_t kDigits = 1u << Bits;
     const BlockRef br = map[blockIdx.x];
     const SceneDev& S = scenes[br.scene];
     const uint32_t tile = br.block;
-    __shared__ uint32_t h[kRadix];
-    for (uint32_t d = threadIdx.x; d < (uint32_t)kRadix; d += kSortThreads) h[d] = 0;
+    __shared__ uint32_t h[kDigits];
+    for (uint32_t d = threadIdx.x; d < kDigits; d += kSortThreads) h[d] = 0;
     // the scene's fields are read once (the stores below could otherwise alias
     // the scene table), and every item's position is loaded before the first store
     const uint32_t n = S.n, ntiles = S.ntiles, levels = S.levels;
+    const bool rank_keys = S.rank_keys != 0u;
+    const auto ranks = gload_ptr(S.ranks);
+    const int32_t cmin0 = S.cmin[0], cmin1 = S.cmin[1], cmin2 = S.cmin[2];
+    const uint32_t cdim0 = S.cdim[0], cdim1 = S.cdim[1], cdim2 = S.cdim[2];
+    SceneState* const state = S.state;
     const float radius = S.search_radius;
     const auto pos = gload_ptr(S.pos[S.state->cur]);
     FSA_GLOBAL(uint32_t)* __restrict__ okeys = (FSA_GLOBAL(uint32_t)*)S.keys[0];
@@ -126,49 +139,92 @@ __global__ __launch_bounds__(kSortThreads) void k_keys_hist(const SceneDev* __re
         const bool valid = i < n;
         uint32_t key = 0;
         if (valid) {
-            key = compact_key(cell_of(p[r].x, radius), cell_of(p[r].y, radius), cell_of(p[r].z, radius), levels);
+            const int32_t cx = cell_of(p[r].x, radius), cy = cell_of(p[r].y, radius), cz = cell_of(p[r].z, radius);
+            if (rank_keys) {
+                const int32_t x = cx - cmin0, y = cy - cmin1, z = cz - cmin2;
+                if (x < 0 || y < 0 || z < 0 || x >= (int32_t)cdim0 || y >= (int32_t)cdim1 || z >= (int32_t)cdim2)
+                    atomicOr(&state->status, kStatusGrid);
+                else
+                    key = ranks[(uint32_t)x + cdim0 * ((uint32_t)y + cdim1 * (uint32_t)z)];
+            } else {
+                key = compact_key(cx, cy, cz, levels);
+            }
             okeys[i] = key;
             ovals[i] = i;
         }
-        const uint32_t d = key & (kRadix - 1);
-        const uint64_t peers = match_digit(d, __ballot(valid));
+        const uint32_t d = key & (kDigits - 1);
+        const uint64_t peers = match_digit<Bits>(d, __ballot(valid));
         if (valid && (peers & lanemask_lt()) == 0) atomicAdd(&h[d], (uint32_t)__popcll(peers));
     }
     __syncthreads();
-    for (uint32_t d = threadIdx.x; d < (uint32_t)kRadix; d += kSortThreads) hist[d * ntiles + tile] = h[d];
+    for (uint32_t d = threadIdx.x; d < kDigits; d += kSortThreads) hist[d * ntiles + tile] = h[d];
 }
 
-// one block per (scene, digit): exclusive scan of the digit's tile counts
+// The rank table: each cell of the step's grid gets its rank in compact-key
+// order among the grid's cells (box_rank).
+__global__ __launch_bounds__(kBlock) void k_rank_table(uint32_t* __restrict__ ranks, int32_t cmin0, int32_t cmin1,
+                                                       int32_t cmin2, uint32_t cdim0, uint32_t cdim1, uint32_t cdim2,
+                                                       uint32_t levels) {
+    const uint32_t ncells = cdim0 * cdim1 * cdim2;
+    const uint32_t lin = blockIdx.x * kBlock + threadIdx.x;
+    if (lin >= ncells) return;
+    const uint32_t x = lin % cdim0, y = (lin / cdim0) % cdim1, z = lin / (cdim0 * cdim1);
+    const int32_t lo[3] = {cmin0, cmin1, cmin2};
+    const int32_t hi[3] = {cmin0 + (int32_t)cdim0 - 1, cmin1 + (int32_t)cdim1 - 1, cmin2 + (int32_t)cdim2 - 1};
+    ranks[lin] = box_rank(cmin0 + (int32_t)x, cmin1 + (int32_t)y, cmin2 + (int32_t)z, lo, hi, levels);
+}
+
+// one block per (scene, digit d < kRadix): exclusive scan of the tile counts
+// of digits d, d + kRadix, ... below 2^Bits, side by side (their loads in flight together)
+template <int Bits>
 __global__ __launch_bounds__(kBlock) void k_radix_scan(const SceneDev* __restrict__ scenes,
                                                        const BlockRef* __restrict__ map) {
+    constexpr uint32_t kDigits = 1u << Bits;
+    constexpr uint32_t kRows = kDigits / (uint32_t)kRadix;
+    static_assert(kDigits % (uint32_t)kRadix == 0, "whole digit rows per block");
     const BlockRef br = map[blockIdx.x];
     const SceneDev& S = scenes[br.scene];
-    const uint32_t d = br.block;
-    uint32_t* row = S.hist + d * S.ntiles;
-    __shared__ uint32_t lw[kBlock / 64];
-    uint32_t carry = 0;
-    for (uint32_t base = 0; base < S.ntiles; base += kBlock) {
-        const uint32_t idx = base + threadIdx.x;
-        const uint32_t v = idx < S.ntiles ? row[idx] : 0u;
-        uint32_t tot;
-        const uint32_t ex = block_exclusive_scan(v, lw, tot);
-        if (idx < S.ntiles) row[idx] = carry + ex;
-        carry += tot;
+    __shared__ uint32_t lw[kRows][kBlock / 64];
+    const uint32_t ntiles = S.ntiles;
+    uint32_t* row[kRows];
+    uint32_t carry[kRows];
+#pragma unroll
+    for (uint32_t q = 0; q < kRows; q++) {
+        row[q] = S.hist + (br.block + q * (uint32_t)kRadix) * ntiles;
+        carry[q] = 0;
     }
-    if (threadIdx.x == 0) S.hist[kRadix * S.ntiles + d] = carry;
+    for (uint32_t base = 0; base < ntiles; base += kBlock) {
+        const uint32_t idx = base + threadIdx.x;
+        uint32_t v[kRows];
+#pragma unroll
+        for (uint32_t q = 0; q < kRows; q++) v[q] = idx < ntiles ? row[q][idx] : 0u;
+#pragma unroll
+        for (uint32_t q = 0; q < kRows; q++) {
+            uint32_t tot;
+            const uint32_t ex = block_exclusive_scan(v[q], lw[q], tot);
+            if (idx < ntiles) row[q][idx] = carry[q] + ex;
+            carry[q] += tot;
+        }
+    }
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (uint32_t q = 0; q < kRows; q++) S.hist[kDigits * ntiles + br.block + q * (uint32_t)kRadix] = carry[q];
+    }
 }
 
+template <int Bits>
 __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(const SceneDev* __restrict__ scenes,
                                                                 const BlockRef* __restrict__ map, uint32_t shift,
                                                                 uint32_t src) {
+    constexpr uint32_t kDigits = 1u << Bits;
     const BlockRef br = map[blockIdx.x];
     const SceneDev& S = scenes[br.scene];
     const uint32_t tile = br.block;
     const uint32_t t = threadIdx.x;
     const uint32_t wave = t / 64;
-    __shared__ uint32_t digit_base[kRadix];
-    __shared__ uint32_t tile_start[kRadix];
-    __shared__ uint32_t wave_cnt[kSortThreads / 64][kRadix];
+    __shared__ uint32_t digit_base[kDigits];
+    __shared__ uint32_t tile_start[kDigits];
+    __shared__ uint32_t wave_cnt[kSortThreads / 64][kDigits];
     __shared__ uint32_t stage_k[kSortTile];
     __shared__ uint32_t stage_v[kSortTile];
     __shared__ uint32_t lw[kSortThreads / 64];
@@ -194,12 +250,12 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(const SceneDev* 
     }
 
     // global base of each digit for this tile; thread t owns digits t * kDigitsPerThread + q
-    constexpr uint32_t kDigitsPerThread = kRadix / kSortThreads;
+    constexpr uint32_t kDigitsPerThread = kDigits / kSortThreads;
     {
         uint32_t tot_d[kDigitsPerThread], at_d[kDigitsPerThread], local = 0;
 #pragma unroll
         for (uint32_t q = 0; q < kDigitsPerThread; q++) {
-            tot_d[q] = hist[kRadix * ntiles + t * kDigitsPerThread + q];
+            tot_d[q] = hist[kDigits * ntiles + t * kDigitsPerThread + q];
             at_d[q] = hist[(t * kDigitsPerThread + q) * ntiles + tile];
         }
 #pragma unroll
@@ -223,8 +279,8 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(const SceneDev* 
     for (int r = 0; r < kSortItems; r++) {
         const uint32_t li = wave * (kSortTile / (kSortThreads / 64)) + r * 64 + __lane_id();
         const bool valid = li < tile_n;
-        const uint32_t d = (k[r] >> shift) & (kRadix - 1);
-        const uint64_t peers = match_digit(d, __ballot(valid));
+        const uint32_t d = (k[r] >> shift) & (kDigits - 1);
+        const uint64_t peers = match_digit<Bits>(d, __ballot(valid));
         uint32_t before = valid ? wave_cnt[wave][d] : 0u;
         rank[r] = before + (uint32_t)__popcll(peers & lanemask_lt());
         if (valid && (peers & lanemask_lt()) == 0) wave_cnt[wave][d] = before + (uint32_t)__popcll(peers);
@@ -259,7 +315,7 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(const SceneDev* 
     for (int r = 0; r < kSortItems; r++) {
         const uint32_t li = wave * (kSortTile / (kSortThreads / 64)) + r * 64 + __lane_id();
         if (li < tile_n) {
-            const uint32_t d = (k[r] >> shift) & (kRadix - 1);
+            const uint32_t d = (k[r] >> shift) & (kDigits - 1);
             const uint32_t pos = tile_start[d] + wave_cnt[wave][d] + rank[r];
             stage_k[pos] = k[r];
             stage_v[pos] = v[r];
@@ -268,7 +324,7 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(const SceneDev* 
     __syncthreads();
     for (uint32_t idx = t; idx < tile_n; idx += kSortThreads) {
         const uint32_t key = stage_k[idx];
-        const uint32_t d = (key >> shift) & (kRadix - 1);
+        const uint32_t d = (key >> shift) & (kDigits - 1);
         const uint32_t g = digit_base[d] + (idx - tile_start[d]);
         okeys[g] = key;
         ovals[g] = stage_v[idx];
@@ -1339,24 +1395,34 @@ __global__ __launch_bounds__(kBlock) void k_codec_finalize(const SceneDev* __res
 
 // ---- launchers -------------------------------------------------------------------
 // keys, then the passes of the radix sort; the first pass's histogram comes with the keys
+// the compressed search's sort: kSearchSortBits-bit digits
 void launch_keys_sort(const LaunchSet& L, uint32_t passes, hipStream_t st) {
-    if (L.nb_tiles) k_keys_hist<<<L.nb_tiles, kSortThreads, 0, st>>>(L.scenes, L.map_tiles);
+    constexpr int B = kSearchSortBits;
+    if (L.nb_tiles) k_keys_hist<B><<<L.nb_tiles, kSortThreads, 0, st>>>(L.scenes, L.map_tiles);
     for (uint32_t p = 0; p < passes; p++) {
         const uint32_t src = p & 1u;
-        const uint32_t shift = p * kRadixBits;
-        if (p > 0) k_radix_hist<<<L.nb_tiles, kSortThreads, 0, st>>>(L.scenes, L.map_tiles, shift, src);
-        k_radix_scan<<<L.nb_scan, kBlock, 0, st>>>(L.scenes, L.map_scan);
-        k_radix_scatter<<<L.nb_tiles, kSortThreads, 0, st>>>(L.scenes, L.map_tiles, shift, src);
+        const uint32_t shift = p * (uint32_t)B;
+        if (p > 0) k_radix_hist<B><<<L.nb_tiles, kSortThreads, 0, st>>>(L.scenes, L.map_tiles, shift, src);
+        k_radix_scan<B><<<L.nb_scan, kBlock, 0, st>>>(L.scenes, L.map_scan);
+        k_radix_scatter<B><<<L.nb_tiles, kSortThreads, 0, st>>>(L.scenes, L.map_tiles, shift, src);
     }
+}
+
+void launch_rank_table(uint32_t* ranks, const int32_t cmin[3], const uint32_t cdim[3], uint32_t levels,
+                       hipStream_t st) {
+    const uint64_t cells = (uint64_t)cdim[0] * cdim[1] * cdim[2];
+    if (cells)
+        k_rank_table<<<(uint32_t)((cells + kBlock - 1) / kBlock), kBlock, 0, st>>>(ranks, cmin[0], cmin[1], cmin[2],
+                                                                                  cdim[0], cdim[1], cdim[2], levels);
 }
 
 void launch_sort_range(const LaunchSet& L, uint32_t first_src, uint32_t shift0, uint32_t passes, hipStream_t st) {
     for (uint32_t p = 0; p < passes; p++) {
         const uint32_t src = first_src ^ (p & 1u);
         const uint32_t shift = shift0 + p * kRadixBits;
-        k_radix_hist<<<L.nb_tiles, kSortThreads, 0, st>>>(L.scenes, L.map_tiles, shift, src);
-        k_radix_scan<<<L.nb_scan, kBlock, 0, st>>>(L.scenes, L.map_scan);
-        k_radix_scatter<<<L.nb_tiles, kSortThreads, 0, st>>>(L.scenes, L.map_tiles, shift, src);
+        k_radix_hist<kRadixBits><<<L.nb_tiles, kSortThreads, 0, st>>>(L.scenes, L.map_tiles, shift, src);
+        k_radix_scan<kRadixBits><<<L.nb_scan, kBlock, 0, st>>>(L.scenes, L.map_scan);
+        k_radix_scatter<kRadixBits><<<L.nb_tiles, kSortThreads, 0, st>>>(L.scenes, L.map_tiles, shift, src);
     }
 }
 
