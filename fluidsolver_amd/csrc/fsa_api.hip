@@ -126,6 +126,10 @@ struct fsa_scene_s {
     uint4* cells = nullptr;     // cell table, then the rank table (kCellBytes per grid cell)
     uint64_t cells_cap = 0;
     uint32_t* ranks = nullptr;
+    uint32_t* bcnt = nullptr;   // counting sort: bucket counts (zero between searches), offsets, part sums
+    uint32_t* boff = nullptr;
+    uint32_t* bsum = nullptr;
+    uint32_t radix_steps = 0;   // steps left on the radix sort after a counting-sort retry
     bool ranks_valid = false;   // the rank table holds the grid below
     int32_t rank_cmin[3] = {0, 0, 0};
     uint32_t rank_cdim[3] = {0, 0, 0}, rank_levels = 0;
@@ -409,8 +413,9 @@ void carve(fsa_scene_s* s, const Plan& p) {
 
 // ---- block maps --------------------------------------------------------------------
 struct MapBuilder {
-    std::vector<BlockRef> all, fluid, tiles, scan, scene;
+    std::vector<BlockRef> all, fluid, tiles, scan, scene, cnt;
     void add(uint32_t sidx, const SceneDev& h) {
+        for (uint32_t b = 0; b < kCountBlocks; b++) cnt.push_back({sidx, b});
         for (uint32_t b = 0; b < h.nblocks; b++) all.push_back({sidx, b});
         const uint32_t nfb = (h.nf + kBlock - 1) / kBlock;
         for (uint32_t b = 0; b < nfb; b++) fluid.push_back({sidx, b});
@@ -472,6 +477,8 @@ void build_launch_set(const MapBuilder& mb, SceneDev* table, LaunchSet& L, std::
     L.map_tiles = upload_map(mb.tiles, owned);
     L.map_scan = upload_map(mb.scan, owned);
     L.map_scene = upload_map(mb.scene, owned);
+    L.map_cnt = upload_map(mb.cnt, owned);
+    L.nb_cnt = (uint32_t)mb.cnt.size();
     L.nb_all = (uint32_t)mb.all.size();
     L.nb_fluid = (uint32_t)mb.fluid.size();
     L.nb_tiles = (uint32_t)mb.tiles.size();
@@ -482,7 +489,10 @@ void build_launch_set(const MapBuilder& mb, SceneDev* table, LaunchSet& L, std::
 // ---- grid ---------------------------------------------------------------------------
 inline int32_t host_cell(float x, float r) { return (int32_t)std::floor(x / r); }
 
-constexpr uint64_t kCellBytes = sizeof(uint4) + sizeof(uint32_t);  // a cell table and a rank table entry
+// per grid cell: the cell table entry, its rank, the counting sort's bucket count and offset
+constexpr uint64_t kCellBytes = sizeof(uint4) + 3 * sizeof(uint32_t);
+constexpr uint64_t kCellExtra = sizeof(uint32_t) * (1 + kCountBlocks);  // the last offset, the part sums
+uint64_t cell_alloc_bytes(uint64_t cap) { return cap * kCellBytes + kCellExtra; }
 uint32_t bits_of(uint64_t v);
 // passes of the compressed search's sort over compact keys of L levels (3L + 3 bits)
 uint32_t key_passes(uint32_t L) { return (3 * L + 3 + kSearchSortBits - 1) / kSearchSortBits; }
@@ -538,24 +548,32 @@ void prepare_grid(fsa_scene_s* s) {
     if (cells > s->cells_cap) {
         if (s->cells) {
             HIP_CHECK(hipFree(s->cells));
-            s->ctx->in_use -= s->cells_cap * kCellBytes;
+            s->ctx->in_use -= cell_alloc_bytes(s->cells_cap);
         }
         const uint64_t cap = cells + cells / 4 + 64;
-        if (s->ctx->budget && s->ctx->in_use + cap * kCellBytes > s->ctx->budget)
+        if (s->ctx->budget && s->ctx->in_use + cell_alloc_bytes(cap) > s->ctx->budget)
             throw FsaError(FSA_E_BUDGET, "cell table exceeds the HBM budget");
-        HIP_CHECK(hipMalloc(&s->cells, cap * kCellBytes));
+        HIP_CHECK(hipMalloc(&s->cells, cell_alloc_bytes(cap)));
         HIP_CHECK(hipMemsetAsync(s->cells, 0, cap * sizeof(uint4), s->ctx->stream));
         s->cells_cap = cap;
         s->ranks = (uint32_t*)(s->cells + cap);
+        s->bcnt = s->ranks + cap;
+        s->boff = s->bcnt + cap;      // cap + 1 entries
+        s->bsum = s->boff + cap + 1;  // kCountBlocks entries
+        HIP_CHECK(hipMemsetAsync(s->bcnt, 0, cap * sizeof(uint32_t), s->ctx->stream));
         s->ranks_valid = false;
-        s->ctx->in_use += cap * kCellBytes;
+        s->ctx->in_use += cell_alloc_bytes(cap);
     }
     h.cells = s->cells;
     // The compressed search sorts by each cell's rank among the grid's cells
     // (the compact-key order in fewer bits) when that saves a sort pass; the
     // rank table is rebuilt when the grid changes.
+    if (s->radix_steps) s->radix_steps--;
     h.rank_keys = 0;
     h.ranks = s->ranks;
+    h.bcnt = s->bcnt;
+    h.boff = s->boff;
+    h.bsum = s->bsum;
     if (h.dim == 3 && !(h.flags & kFlagUnsorted)) {
         const uint32_t rank_bits = bits_of(cells - 1);
         const uint32_t rank_passes = std::max<uint32_t>(1, (rank_bits + kSearchSortBits - 1) / kSearchSortBits);
@@ -663,11 +681,17 @@ struct PhaseTimer {
 // ---- the step, for any set of scenes sharing a launch set --------------------------------
 void search_set(fsa_context ctx, const LaunchSet& L, uint32_t passes) {
     PhaseTimer t(ctx, FSA_TIMER_SEARCH);
-    launch_keys_sort(L, passes, ctx->stream);
-    launch_reorder(L, passes & 1u, ctx->stream);
+    uint32_t sbuf = passes & 1u;
+    if (L.counting) {
+        launch_count_sort(L, ctx->stream);
+        sbuf = 1u;
+    } else {
+        launch_keys_sort(L, passes, ctx->stream);
+    }
+    launch_reorder(L, sbuf, ctx->stream);
     {
         PhaseTimer tl(ctx, FSA_TIMER_LISTS, true);
-        launch_lists(L, passes & 1u, ctx->stream);
+        launch_lists(L, sbuf, ctx->stream);
     }
     if (L.codec) launch_codec(L, ctx->stream, false);
 }
@@ -782,8 +806,13 @@ void check_status(fsa_scene_s* s) {
 // neighbor-list byte offsets need more than 32 bits
 bool wide_lists(const SceneDev& h) { return 4ull * kMaxNeighbors * h.list_stride > 0xffffffffull; }
 
+// the counting sort serves a launch set whose scenes all sort rank keys and
+// none is back on the radix sort after a retry
+bool counting_sort_ok(const fsa_scene_s* s) { return s->hd.rank_keys && s->radix_steps == 0; }
+
 void upload_scene_table(fsa_scene_s* s) {
     s->single.spec = spec_of(s->hd);
+    s->single.counting = counting_sort_ok(s);
     s->single.codec = (s->hd.flags & kFlagCodec) != 0;
     s->single.wide_lists = wide_lists(s->hd);
     s->single.entities_before = (s->hd.entity_bits & (kEntityBitBoundary | kEntityBitVelocity)) != 0;
@@ -982,6 +1011,20 @@ void search2d_set(fsa_context ctx, const LaunchSet& L, uint32_t epasses, uint32_
     launch2_search(L, epasses, cpasses, ctx->stream);
 }
 
+// A counting sort that met a bucket it does not order left the scene as it was
+// before the search (kStatusSortRetry): the flag is cleared and the scene takes
+// the radix sort for kRadixRetrySteps steps.  True when the search must be redone.
+constexpr uint32_t kRadixRetrySteps = 64;
+bool take_sort_retry(fsa_scene_s* s) {
+    if (!(s->h_state->status & kStatusSortRetry)) return false;
+    s->h_state->status &= ~kStatusSortRetry;
+    HIP_CHECK(hipMemcpyAsync(&s->hd.state->status, &s->h_state->status, sizeof(uint32_t), hipMemcpyHostToDevice,
+                             s->ctx->stream));
+    HIP_CHECK(hipStreamSynchronize(s->ctx->stream));
+    s->radix_steps = kRadixRetrySteps;
+    return true;
+}
+
 void do_find_neighbors(fsa_scene_s* s) {
     hashed_resort_if_due(s);
     prepare_grid(s);
@@ -996,6 +1039,10 @@ void do_find_neighbors(fsa_scene_s* s) {
         search_set(s->ctx, s->single, sort_passes(s->hd));
     }
     read_state(s);
+    if (take_sort_retry(s)) {
+        do_find_neighbors(s);  // the radix sort now
+        return;
+    }
     check_status(s);
     s->searched = true;
     s->lists_valid = true;
@@ -1139,8 +1186,9 @@ void drop_batches_of(fsa_context ctx, fsa_scene_s* s) {
 
 void upload_batch_table(fsa_context ctx, BatchSet* b) {
     uint32_t spec = kSpecHOne | kSpecRho0SqOne | kSpecFastMath | kSpecDelta;  // the specialisations every scene allows
-    bool codec = false, wide = false, ent = false, rem = false;
+    bool codec = false, wide = false, ent = false, rem = false, counting = true;
     for (uint32_t q = 0; q < b->scenes.size(); q++) {
+        counting = counting && counting_sort_ok(b->scenes[q]);
         b->h_table[q] = b->scenes[q]->hd;
         spec &= spec_of(b->scenes[q]->hd);
         codec = codec || (b->scenes[q]->hd.flags & kFlagCodec);
@@ -1153,6 +1201,7 @@ void upload_batch_table(fsa_context ctx, BatchSet* b) {
     b->ls.spec = spec;
     b->ls.codec = codec;
     b->ls.wide_lists = wide;
+    b->ls.counting = counting;
     if (ctx->walk_variant)
         for (int f = 0; f < kTuneFamilies; f++) b->ls.unroll[f] = ctx->walk_variant;
     HIP_CHECK(hipMemcpyAsync(b->d_table, b->h_table.data(), sizeof(SceneDev) * b->h_table.size(),
@@ -1444,7 +1493,8 @@ int32_t batch_step_group(fsa_context ctx, const std::vector<fsa_scene_s*>& all, 
         const std::array<uint32_t, 5> key = {
             d2 ? (epasses << 8 | cpasses) : passes, chunk | tail << 16, b->ls.spec,
             (bpass ? 1u : 0u) | (sesph ? 2u : 0u) | (unsorted ? 4u : 0u) | (b->ls.entities_before ? 8u : 0u) |
-                (b->ls.codec ? 16u : 0u) | (b->ls.wide_lists ? 32u : 0u) | (b->ls.remover ? 64u : 0u),
+                (b->ls.codec ? 16u : 0u) | (b->ls.wide_lists ? 32u : 0u) | (b->ls.remover ? 64u : 0u) |
+                (b->ls.counting ? 128u : 0u),
             (uint32_t)b->ls.unroll[0] | (uint32_t)b->ls.unroll[1] << 8 | (uint32_t)b->ls.unroll[2] << 16 |
                 (uint32_t)b->ls.unroll[3] << 24};
         auto it = b->graphs.find(key);
@@ -1466,7 +1516,13 @@ int32_t batch_step_group(fsa_context ctx, const std::vector<fsa_scene_s*>& all, 
     if (d2)
         for (auto* s : v) s->hd.search_count++;
     read_states(ctx, v);
-    if (!b->tuned) {
+    // scenes whose counting sort asked for a retry took no part in the step (every
+    // kernel after the sort skipped them): they step again below with the radix sort
+    std::vector<bool> retry(v.size(), false);
+    bool any_retry = false;
+    for (uint32_t q = 0; q < v.size(); q++)
+        if (v[q]->h_state->status & kStatusSortRetry) retry[q] = any_retry = true;
+    if (!b->tuned && !any_retry) {
         if (!d2 && !sesph && ctx->autotune && !ctx->walk_variant) autotune(ctx, b, v);
         b->tuned = true;
     }
@@ -1508,6 +1564,7 @@ int32_t batch_step_group(fsa_context ctx, const std::vector<fsa_scene_s*>& all, 
     if (d2 || b->ls.remover) read_states(ctx, v);
     for (uint32_t q = 0; q < v.size(); q++) {
         fsa_scene_s* s = v[q];
+        if (retry[q]) continue;
         try {
             check_status(s);
         } catch (const FsaError& e) {
@@ -1529,8 +1586,20 @@ int32_t batch_step_group(fsa_context ctx, const std::vector<fsa_scene_s*>& all, 
         post_simulate(s, out[q]);
     }
     // particle removers act after the step; the launch tables of this group are not used again
-    for (auto* s : v)
-        if (s->removal_pending) apply_removal(s);
+    for (uint32_t q = 0; q < v.size(); q++)
+        if (!retry[q] && v[q]->removal_pending) apply_removal(v[q]);
+    if (any_retry) {
+        std::vector<fsa_scene_s*> again;
+        std::vector<fsa_step_stats*> again_out;
+        for (uint32_t q = 0; q < v.size(); q++) {
+            if (!retry[q]) continue;
+            (void)take_sort_retry(v[q]);
+            again.push_back(v[q]);
+            again_out.push_back(out[q]);
+        }
+        const int32_t rc = batch_step_group(ctx, again, again_out);
+        if (first_error == FSA_OK) first_error = rc;
+    }
     return first_error;
 }
 
@@ -2313,7 +2382,7 @@ void destroy_scene(fsa_scene_s* s) {
     }
     if (s->cells) {
         (void)hipFree(s->cells);
-        s->ctx->in_use -= s->cells_cap * kCellBytes;
+        s->ctx->in_use -= cell_alloc_bytes(s->cells_cap);
     }
     if (s->qs_mem) {
         (void)hipFree(s->qs_mem);

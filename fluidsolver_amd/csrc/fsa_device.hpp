@@ -33,6 +33,7 @@ constexpr int kRadix = 1 << kRadixBits;
 // 2^20 cells in 2 passes, compact keys up to 30 bits in 3.
 constexpr int kSearchSortBits = 10;
 constexpr int kRadixMaxDigits = 1 << kSearchSortBits;  // tile counts the histogram buffer holds per tile
+constexpr uint32_t kCountBlocks = 1024;  // blocks per scene over its rank range (counting sort scans)
 
 // SceneDev.flags
 constexpr uint32_t kFlagHOne = 1u;          // kernel h == 1: q = |x|, pre = x / |x| exactly
@@ -80,6 +81,7 @@ constexpr uint32_t kStatusGrid = 2u;        // a particle left the cell grid of 
 constexpr uint32_t kStatusNonFinite = 4u;
 constexpr uint32_t kStatusTinyCoord = 8u;   // a coordinate lies in (0, 2^-36) in magnitude (not an error)
 constexpr uint32_t kStatusRemoval = 16u;    // a particle remover entity found members among the active particles
+constexpr uint32_t kStatusSortRetry = 32u;  // the counting sort met a cell it does not order: redo the step with the radix sort
 
 // Per-scene state written by the device and read back once per step.
 struct alignas(16) SceneState {
@@ -191,6 +193,12 @@ struct alignas(16) SceneDev {
     uint32_t levels;       // Morton levels of the compact key
     uint32_t rank_keys;    // compressed search: sort by the rank table instead of the compact key
     const uint32_t* ranks; // rank of each grid cell in compact-key order among the grid's cells
+    // counting sort over the ranks: particles per rank (zero between searches),
+    // first sorted position of each rank (cdim[3] + 1 entries), sums of the
+    // kCountBlocks parts of the rank range
+    uint32_t* bcnt;
+    uint32_t* boff;
+    uint32_t* bsum;
     uint32_t stamp;        // cell-table generation
     uint32_t ntiles;       // radix tiles
     uint32_t nblocks;      // kBlock-blocks over n
@@ -249,10 +257,13 @@ struct BlockRef {
 
 // A scene whose search reported an error (neighbor capacity, cell grid) is
 // left untouched by the solver kernels of that step: its particle state stays
-// as the search left it, and the host reports the error.
+// as the search left it, and the host reports the error.  A scene whose
+// counting sort asked for a retry is left as it was before the search (the
+// search kernels after the sort skip it too) and steps again with the radix sort.
 __device__ __forceinline__ bool scene_failed(const SceneDev& S) {
-    return (S.state->status & (kStatusCapacity | kStatusGrid)) != 0u;
+    return (S.state->status & (kStatusCapacity | kStatusGrid | kStatusSortRetry)) != 0u;
 }
+__device__ __forceinline__ bool sort_retry(const SceneDev& S) { return (S.state->status & kStatusSortRetry) != 0u; }
 
 // ---- reference-order arithmetic ------------------------------------------
 // glm::dot for vec3: ((x*x') + (y*y')) + (z*z')

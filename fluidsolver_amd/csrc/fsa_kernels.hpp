@@ -24,7 +24,8 @@ struct LaunchSet {
     BlockRef* map_tiles = nullptr;   // radix tiles
     BlockRef* map_scan = nullptr;    // (scene, digit)
     BlockRef* map_scene = nullptr;   // one block per scene
-    uint32_t nb_all = 0, nb_fluid = 0, nb_tiles = 0, nb_scan = 0, nb_scene = 0;
+    BlockRef* map_cnt = nullptr;     // kCountBlocks blocks per scene over its rank range
+    uint32_t nb_all = 0, nb_fluid = 0, nb_tiles = 0, nb_scan = 0, nb_scene = 0, nb_cnt = 0;
     uint32_t spec = 0;  // kSpec* bits shared by every scene of the set
     // neighbor-walk variant (kVariant*) per IISPH kernel family (kTune*), chosen by the autotuner
     uint8_t unroll[4] = {kVariantU4, kVariantU4, kVariantU4, kVariantU4};
@@ -32,6 +33,7 @@ struct LaunchSet {
     bool wide_lists = false;  // some scene's neighbor-list array spans 4 GiB or more
     bool entities_before = false;  // some scene has a boundary preprocessor or a velocity alteration
     bool remover = false;          // some scene has a particle remover
+    bool counting = false;         // compressed search: the counting sort over rank keys (every scene has them)
 };
 
 // kernel families the autotuner times
@@ -64,6 +66,8 @@ uint64_t statistics_result_bytes();  // one accumulator; the result sits at the 
 
 // neighborhood search
 void launch_keys_sort(const LaunchSet& L, uint32_t passes, hipStream_t st);
+// the counting sort over rank keys: sorted (rank, row) pairs in keys[1] / vals[1]
+void launch_count_sort(const LaunchSet& L, hipStream_t st);
 // the rank table of a grid of cdim cells from cmin (box_rank of every cell)
 void launch_rank_table(uint32_t* ranks, const int32_t cmin[3], const uint32_t cdim[3], uint32_t levels,
                        hipStream_t st);

@@ -52,6 +52,195 @@ __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* l
 
 // ---- 1. keys (in k_keys_hist, with the first radix histogram) ---------------------
 
+// ---- 2a. counting sort over cell ranks ---------------------------------------
+// With rank keys every cell of the grid is a bucket, and the collection comes
+// in the order of the last search, so a cell's particles mostly arrive as runs
+// of consecutive rows.  k_rank_count gives each particle a slot in its bucket
+// (one atomic per run of equal ranks within a wave, slots in row order within
+// the run), k_bucket_sums / k_bucket_offsets scan the bucket counts, k_place
+// writes each particle at its bucket's start plus its slot, and k_fixup puts
+// every bucket's rows in ascending order, which makes the result the stable
+// sort.  Buckets above 64 rows are not ordered here: the scene's step is
+// retried with the radix sort (kStatusSortRetry).
+constexpr uint32_t kFixupFast = 12;  // bucket rows ordered in a lane's registers (up to 64 by the wave)
+
+__global__ __launch_bounds__(kBlock) void k_rank_count(const SceneDev* __restrict__ scenes,
+                                                       const BlockRef* __restrict__ map) {
+    const BlockRef br = map[blockIdx.x];
+    const SceneDev& S = scenes[br.scene];
+    const uint32_t i = br.block * kBlock + threadIdx.x;
+    const uint32_t lane = __lane_id();
+    const uint32_t n = S.n;
+    const bool valid = i < n;
+    uint32_t r = 0xffffffffu;
+    if (valid) {
+        const float4 p = gload_ptr(S.pos[S.state->cur])[i];
+        const float radius = S.search_radius;
+        const int32_t x = cell_of(p.x, radius) - S.cmin[0], y = cell_of(p.y, radius) - S.cmin[1],
+                      z = cell_of(p.z, radius) - S.cmin[2];
+        if (x < 0 || y < 0 || z < 0 || x >= (int32_t)S.cdim[0] || y >= (int32_t)S.cdim[1] || z >= (int32_t)S.cdim[2]) {
+            atomicOr(&S.state->status, kStatusGrid);  // the scene fails; any bucket keeps the sort a permutation
+            r = 0;
+        } else {
+            r = S.ranks[(uint32_t)x + S.cdim[0] * ((uint32_t)y + S.cdim[1] * (uint32_t)z)];
+        }
+    }
+    // runs of equal ranks in consecutive lanes (the valid lanes are a prefix of the wave)
+    const uint32_t rp = (uint32_t)__builtin_amdgcn_update_dpp((int)0xffffffffu, (int)r, 0x138, 0xf, 0xf, false);
+    const bool head = valid && (lane == 0 || rp != r);
+    const uint64_t heads = __ballot(head);
+    const uint64_t nvalid = __ballot(valid);
+    uint32_t base = 0;
+    if (head) {
+        const uint64_t later = heads & ~((2ull << lane) - 1ull);  // heads above this lane (none above 63)
+        const uint32_t next = later ? (uint32_t)__ffsll((unsigned long long)later) - 1u
+                                    : (uint32_t)__popcll(nvalid);
+        base = atomicAdd(&S.bcnt[r], next - lane);
+    }
+    const uint64_t upto = heads & (lane == 63 ? ~0ull : ((2ull << lane) - 1ull));
+    const uint32_t h = upto ? 63u - (uint32_t)__clzll(upto) : 0u;
+    const uint32_t slot = (uint32_t)__shfl((int)base, (int)h, 64) + (lane - h);
+    if (valid) {
+        S.keys[0][i] = r;
+        S.vals[0][i] = slot;
+    }
+}
+
+// A scene's rank range [0, cdim[3]) in parts of about kCountPart ranks, at most
+// kCountBlocks of them (the blocks of the map above the part count exit)
+constexpr uint32_t kCountPart = 1024;
+__device__ __forceinline__ uint32_t count_parts(const SceneDev& S) {
+    return max(1u, min(kCountBlocks, (S.cdim[3] + kCountPart - 1) / kCountPart));
+}
+__device__ __forceinline__ void count_part(const SceneDev& S, uint32_t parts, uint32_t b, uint32_t& lo, uint32_t& hi) {
+    const uint32_t c = S.cdim[3];
+    lo = (uint32_t)((uint64_t)c * b / parts);
+    hi = (uint32_t)((uint64_t)c * (b + 1) / parts);
+}
+
+__global__ __launch_bounds__(kBlock) void k_bucket_sums(const SceneDev* __restrict__ scenes,
+                                                        const BlockRef* __restrict__ map) {
+    const BlockRef br = map[blockIdx.x];
+    const SceneDev& S = scenes[br.scene];
+    __shared__ uint32_t lw[kBlock / 64];
+    const uint32_t parts = count_parts(S);
+    if (br.block >= parts) return;
+    uint32_t lo, hi;
+    count_part(S, parts, br.block, lo, hi);
+    const auto bcnt = gload_ptr(S.bcnt);
+    uint32_t sum = 0;
+    for (uint32_t c = lo + threadIdx.x; c < hi; c += kBlock) sum += bcnt[c];
+    uint32_t tot;
+    (void)block_exclusive_scan(sum, lw, tot);
+    if (threadIdx.x == 0) S.bsum[br.block] = tot;
+}
+
+__global__ __launch_bounds__(kBlock) void k_bucket_offsets(const SceneDev* __restrict__ scenes,
+                                                           const BlockRef* __restrict__ map) {
+    const BlockRef br = map[blockIdx.x];
+    const SceneDev& S = scenes[br.scene];
+    __shared__ uint32_t lw[kBlock / 64];
+    const uint32_t parts = count_parts(S);
+    if (br.block >= parts) return;
+    uint32_t lo, hi;
+    count_part(S, parts, br.block, lo, hi);
+    // rows of the parts before this one
+    uint32_t before = 0;
+    for (uint32_t q = threadIdx.x; q < br.block; q += kBlock) before += S.bsum[q];
+    uint32_t carry;
+    (void)block_exclusive_scan(before, lw, carry);
+    uint32_t* __restrict__ bcnt = S.bcnt;
+    uint32_t* __restrict__ boff = S.boff;
+    for (uint32_t c0 = lo; c0 < hi; c0 += kBlock) {
+        const uint32_t c = c0 + threadIdx.x;
+        const uint32_t v = c < hi ? bcnt[c] : 0u;
+        uint32_t tot;
+        const uint32_t ex = block_exclusive_scan(v, lw, tot);
+        if (c < hi) {
+            boff[c] = carry + ex;
+            bcnt[c] = 0u;  // zero for the next search
+        }
+        carry += tot;
+    }
+    if (br.block == parts - 1 && threadIdx.x == 0) boff[S.cdim[3]] = carry;
+}
+
+__global__ __launch_bounds__(kBlock) void k_place(const SceneDev* __restrict__ scenes,
+                                                  const BlockRef* __restrict__ map) {
+    const BlockRef br = map[blockIdx.x];
+    const SceneDev& S = scenes[br.scene];
+    const uint32_t i = br.block * kBlock + threadIdx.x;
+    if (i >= S.n) return;
+    const uint32_t r = S.keys[0][i], slot = S.vals[0][i];
+    const uint32_t p = S.boff[r] + slot;
+    S.keys[1][p] = r;
+    S.vals[1][p] = i;
+}
+
+// The first row of each bucket orders the bucket's rows (source indices)
+// ascending: up to kFixupFast rows in its lane's registers (a sorting network
+// of compare-exchanges), up to 64 rows by the whole wave (a bitonic sort across
+// the lanes, one bucket at a time), more rows not at all (retry).
+__global__ __launch_bounds__(kBlock) void k_fixup(const SceneDev* __restrict__ scenes,
+                                                  const BlockRef* __restrict__ map) {
+    const BlockRef br = map[blockIdx.x];
+    const SceneDev& S = scenes[br.scene];
+    const uint32_t p = br.block * kBlock + threadIdx.x;
+    const uint32_t lane = __lane_id();
+    uint32_t k = 0;
+    if (p < S.n) {
+        const uint32_t* __restrict__ keys = S.keys[1];
+        const uint32_t r = keys[p];
+        if (p == 0 || keys[p - 1] != r) k = S.boff[r + 1] - S.boff[r];
+    }
+    uint32_t* __restrict__ vals = S.vals[1];
+    const bool fast = k >= 2 && k <= kFixupFast;
+    uint32_t v[kFixupFast];
+    bool sorted = true;
+    if (fast) {
+#pragma unroll
+        for (uint32_t q = 0; q < kFixupFast; q++) v[q] = q < k ? vals[p + q] : 0xffffffffu;
+#pragma unroll
+        for (uint32_t q = 1; q < kFixupFast; q++) sorted = sorted && v[q - 1] <= v[q];
+    }
+    // most buckets arrive in order (one run of rows, or runs whose atomics came in
+    // row order): the network runs only where some bucket of the wave did not
+    if (__any(fast && !sorted) && fast && !sorted) {
+        // insertion network; the padding (all ones) stays at the end
+#pragma unroll
+        for (uint32_t q = 1; q < kFixupFast; q++) {
+#pragma unroll
+            for (uint32_t t = q; t > 0; t--) {
+                const uint32_t a = v[t - 1], b = v[t];
+                v[t - 1] = min(a, b);
+                v[t] = max(a, b);
+            }
+        }
+#pragma unroll
+        for (uint32_t q = 0; q < kFixupFast; q++)
+            if (q < k) vals[p + q] = v[q];
+    }
+    if (k > 64u) atomicOr(&S.state->status, kStatusSortRetry);
+    // buckets of kFixupFast + 1 .. 64 rows, one at a time across the wave
+    for (uint64_t m = __ballot(k > kFixupFast && k <= 64u); m; m &= m - 1) {
+        const uint32_t src = (uint32_t)__ffsll((unsigned long long)m) - 1u;
+        const uint32_t q0 = (uint32_t)__builtin_amdgcn_readlane((int)p, (int)src);
+        const uint32_t kk = (uint32_t)__builtin_amdgcn_readlane((int)k, (int)src);
+        uint32_t x = lane < kk ? vals[q0 + lane] : 0xffffffffu;
+#pragma unroll
+        for (uint32_t size = 2; size <= 64; size <<= 1) {
+#pragma unroll
+            for (uint32_t stride = size / 2; stride > 0; stride >>= 1) {
+                const uint32_t y = (uint32_t)__shfl_xor((int)x, (int)stride, 64);
+                const bool up = (lane & size) == 0;     // this part sorts ascending
+                const bool low = (lane & stride) == 0;  // this lane keeps the smaller value when ascending
+                x = (up == low) ? min(x, y) : max(x, y);
+            }
+        }
+        if (lane < kk) vals[q0 + lane] = x;
+    }
+}
+
 // ---- 2. radix sort -----------------------------------------------------------
 static_assert(kRadix % kSortThreads == 0, "each sort thread owns a whole number of digits");
 // peers of this lane among `valid` lanes holding the same Bits-bit digit
@@ -341,6 +530,7 @@ __global__ __launch_bounds__(kBlock) void k_permute(const SceneDev* __restrict__
                                                     const BlockRef* __restrict__ map, uint32_t sbuf) {
     const BlockRef br = map[blockIdx.x];
     const SceneDev& S = scenes[br.scene];
+    if (sort_retry(S)) return;
     const uint32_t i = br.block * kBlock + threadIdx.x;
     const uint32_t c = S.state->cur;
     const uint32_t o = c ^ 1u;
@@ -432,6 +622,7 @@ __global__ __launch_bounds__(kBlock) void k_permute(const SceneDev* __restrict__
 __global__ __launch_bounds__(kScanThreads) void k_fluid_scan(const SceneDev* __restrict__ scenes,
                                                              const BlockRef* __restrict__ map) {
     const SceneDev& S = scenes[map[blockIdx.x].scene];
+    if (sort_retry(S)) return;  // the permute skipped this scene: the buffers stay as they are
     __shared__ uint32_t lw[kScanThreads / 64];
     const uint32_t n = S.nblocks;
     const uint32_t per = (n + kScanThreads - 1) / kScanThreads;
@@ -558,6 +749,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) voi
     __shared__ uint32_t stage_cnt_all[kBlock / 64][kStageParticles];
     const BlockRef br = map[blockIdx.x];
     const SceneDev& S = scenes[br.scene];
+    if (sort_retry(S)) return;
     const uint32_t wave = (uint32_t)__builtin_amdgcn_readfirstlane(threadIdx.x / 64);
     const uint32_t lane = __lane_id();
     const uint32_t base = br.block * kBlock + wave * 64;
@@ -1237,7 +1429,7 @@ __global__ __launch_bounds__(kBlock) void k3u_lists(const SceneDev* __restrict__
 // Without `force`, a scene's lists are sized only when k_lists could not bound
 // them within the NeighborStorage payload, or when they are delta-coded.
 __device__ __forceinline__ bool codec_due(const SceneDev& S, uint32_t force) {
-    return force || (S.flags & kFlagDeltaLists) || S.state->codec_check;
+    return !sort_retry(S) && (force || (S.flags & kFlagDeltaLists) || S.state->codec_check);
 }
 
 template <bool Encode>
@@ -1406,6 +1598,16 @@ void launch_keys_sort(const LaunchSet& L, uint32_t passes, hipStream_t st) {
         k_radix_scan<B><<<L.nb_scan, kBlock, 0, st>>>(L.scenes, L.map_scan);
         k_radix_scatter<B><<<L.nb_tiles, kSortThreads, 0, st>>>(L.scenes, L.map_tiles, shift, src);
     }
+}
+
+// the compressed search's counting sort over rank keys (sorted pairs in buffer 1)
+void launch_count_sort(const LaunchSet& L, hipStream_t st) {
+    if (!L.nb_all) return;
+    k_rank_count<<<L.nb_all, kBlock, 0, st>>>(L.scenes, L.map_all);
+    k_bucket_sums<<<L.nb_cnt, kBlock, 0, st>>>(L.scenes, L.map_cnt);
+    k_bucket_offsets<<<L.nb_cnt, kBlock, 0, st>>>(L.scenes, L.map_cnt);
+    k_place<<<L.nb_all, kBlock, 0, st>>>(L.scenes, L.map_all);
+    k_fixup<<<L.nb_all, kBlock, 0, st>>>(L.scenes, L.map_all);
 }
 
 void launch_rank_table(uint32_t* ranks, const int32_t cmin[3], const uint32_t cdim[3], uint32_t levels,
