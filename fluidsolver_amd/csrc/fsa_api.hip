@@ -810,9 +810,13 @@ bool wide_lists(const SceneDev& h) { return 4ull * kMaxNeighbors * h.list_stride
 // none is back on the radix sort after a retry
 bool counting_sort_ok(const fsa_scene_s* s) { return s->hd.rank_keys && s->radix_steps == 0; }
 
+// the first Jacobi iteration cannot end the loop: its check can be left out
+bool first_check_needless(const SceneDev& h) { return h.min_it >= 2 && h.max_it >= 2; }
+
 void upload_scene_table(fsa_scene_s* s) {
     s->single.spec = spec_of(s->hd);
     s->single.counting = counting_sort_ok(s);
+    s->single.skip_first_check = first_check_needless(s->hd);
     s->single.codec = (s->hd.flags & kFlagCodec) != 0;
     s->single.wide_lists = wide_lists(s->hd);
     s->single.entities_before = (s->hd.entity_bits & (kEntityBitBoundary | kEntityBitVelocity)) != 0;
@@ -1186,9 +1190,10 @@ void drop_batches_of(fsa_context ctx, fsa_scene_s* s) {
 
 void upload_batch_table(fsa_context ctx, BatchSet* b) {
     uint32_t spec = kSpecHOne | kSpecRho0SqOne | kSpecFastMath | kSpecDelta;  // the specialisations every scene allows
-    bool codec = false, wide = false, ent = false, rem = false, counting = true;
+    bool codec = false, wide = false, ent = false, rem = false, counting = true, skip_first = true;
     for (uint32_t q = 0; q < b->scenes.size(); q++) {
         counting = counting && counting_sort_ok(b->scenes[q]);
+        skip_first = skip_first && first_check_needless(b->scenes[q]->hd);
         b->h_table[q] = b->scenes[q]->hd;
         spec &= spec_of(b->scenes[q]->hd);
         codec = codec || (b->scenes[q]->hd.flags & kFlagCodec);
@@ -1202,6 +1207,7 @@ void upload_batch_table(fsa_context ctx, BatchSet* b) {
     b->ls.codec = codec;
     b->ls.wide_lists = wide;
     b->ls.counting = counting;
+    b->ls.skip_first_check = skip_first;
     if (ctx->walk_variant)
         for (int f = 0; f < kTuneFamilies; f++) b->ls.unroll[f] = ctx->walk_variant;
     HIP_CHECK(hipMemcpyAsync(b->d_table, b->h_table.data(), sizeof(SceneDev) * b->h_table.size(),
@@ -1494,7 +1500,7 @@ int32_t batch_step_group(fsa_context ctx, const std::vector<fsa_scene_s*>& all, 
             d2 ? (epasses << 8 | cpasses) : passes, chunk | tail << 16, b->ls.spec,
             (bpass ? 1u : 0u) | (sesph ? 2u : 0u) | (unsorted ? 4u : 0u) | (b->ls.entities_before ? 8u : 0u) |
                 (b->ls.codec ? 16u : 0u) | (b->ls.wide_lists ? 32u : 0u) | (b->ls.remover ? 64u : 0u) |
-                (b->ls.counting ? 128u : 0u),
+                (b->ls.counting ? 128u : 0u) | (b->ls.skip_first_check ? 256u : 0u),
             (uint32_t)b->ls.unroll[0] | (uint32_t)b->ls.unroll[1] << 8 | (uint32_t)b->ls.unroll[2] << 16 |
                 (uint32_t)b->ls.unroll[3] << 24};
         auto it = b->graphs.find(key);

@@ -34,6 +34,7 @@ struct LaunchSet {
     bool entities_before = false;  // some scene has a boundary preprocessor or a velocity alteration
     bool remover = false;          // some scene has a particle remover
     bool counting = false;         // compressed search: the counting sort over rank keys (every scene has them)
+    bool skip_first_check = false; // 3D IISPH: every scene iterates at least twice (no check after the first iteration)
 };
 
 // kernel families the autotuner times

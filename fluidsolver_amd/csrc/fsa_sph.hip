@@ -358,14 +358,19 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(W))) voi
 }
 
 // ---- Jacobi loop control -------------------------------------------------------------
-__global__ void k_iisph_begin_loop(const SceneDev* __restrict__ scenes, const BlockRef* __restrict__ map) {
+// first_done: the first iteration's convergence check is not launched (every
+// scene of the launch has min and max iterations of at least 2, so it could
+// only continue the loop, and pass B's check overwrites what it records): the
+// loop starts with that iteration counted.
+__global__ void k_iisph_begin_loop(const SceneDev* __restrict__ scenes, const BlockRef* __restrict__ map,
+                                   uint32_t first_done) {
     const SceneDev& S = scenes[map[blockIdx.x].scene];
     if (scene_failed(S)) return;
     if (threadIdx.x == 0) {
         SceneState* st = S.state;
         st->active = S.max_it > 0 ? 1u : 0u;
         st->pending = 1u;
-        st->iterations = 0;
+        st->iterations = first_done;
         st->max_a2 = 0.0f;
         st->max_v2 = 0.0f;
         st->avg_err = 0.0;
@@ -899,14 +904,14 @@ void launch_iisph_diagonal(const LaunchSet& L, hipStream_t st) {
     if (L.nb_fluid) FSA_UNROLL_DISPATCH(k_iisph_diagonal, kTuneDiagonal, L.nb_fluid, L.map_fluid);
 }
 void launch_iisph_begin_loop(const LaunchSet& L, hipStream_t st) {
-    k_iisph_begin_loop<<<L.nb_scene, 64, 0, st>>>(L.scenes, L.map_scene);
+    k_iisph_begin_loop<<<L.nb_scene, 64, 0, st>>>(L.scenes, L.map_scene, L.skip_first_check ? 1u : 0u);
 }
 void launch_iisph_pass_a(const LaunchSet& L, hipStream_t st) {
     if (L.nb_fluid) FSA_UNROLL_DISPATCH(k_iisph_pass_a, kTunePassA, L.nb_fluid, L.map_fluid);
 }
 // the first Jacobi iteration's convergence decision (its update ran in phase 2)
 void launch_iisph_first_converge(const LaunchSet& L, hipStream_t st) {
-    k_iisph_converge<<<L.nb_scene, kBlock, 0, st>>>(L.scenes, L.map_scene);
+    if (!L.skip_first_check) k_iisph_converge<<<L.nb_scene, kBlock, 0, st>>>(L.scenes, L.map_scene);
 }
 void launch_iisph_pass_b(const LaunchSet& L, hipStream_t st) {
     if (L.nb_fluid) FSA_UNROLL_DISPATCH(k_iisph_pass_b, kTunePassB, L.nb_fluid, L.map_fluid);
