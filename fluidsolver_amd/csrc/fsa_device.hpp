@@ -603,10 +603,19 @@ struct GView4 {
         const fsa_f4 v = p[i];
         return make_float4(v.x, v.y, v.z, v.w);
     }
+    // element i of a particle array (i < 2^27) at a 32-bit byte offset from the
+    // array's scalar base: the load takes the base from SGPRs and one VGPR offset
+    __device__ __forceinline__ float4 at(uint32_t i) const {
+        const fsa_f4 v = *(const FSA_GLOBAL(fsa_f4)*)((const FSA_GLOBAL(char)*)p + (i << 4));
+        return make_float4(v.x, v.y, v.z, v.w);
+    }
 };
 struct GViewF {
     const FSA_GLOBAL(float) * p;
     __device__ __forceinline__ float operator[](uint32_t i) const { return p[i]; }
+    __device__ __forceinline__ float at(uint32_t i) const {
+        return *(const FSA_GLOBAL(float)*)((const FSA_GLOBAL(char)*)p + (i << 2));
+    }
 };
 struct GViewU {
     const FSA_GLOBAL(uint32_t) * p;
@@ -665,6 +674,35 @@ __device__ __forceinline__ uint32_t delta_entry(uint32_t c, DeltaCursor& q, GVie
     return en;
 }
 
+// walk_neighbors with the list entries addressed by 32-bit byte offsets from the
+// list array's scalar base (entry e of rank k at (e * stride + k) * 4, below 4 GiB)
+template <int U, class Load, class Body>
+__device__ __forceinline__ void walk_neighbors32(const uint32_t* nbr, uint32_t k, uint32_t stride, uint32_t cnt,
+                                                 Load load, Body body) {
+    using T = decltype(load(0u));
+    const FSA_GLOBAL(char)* base = (const FSA_GLOBAL(char)*)nbr;
+    const uint32_t step = stride << 2;
+    uint32_t off = k << 2;
+    uint32_t e = 0;
+    for (; e + U <= cnt; e += U) {
+        uint32_t ent[U];
+#pragma unroll
+        for (int q = 0; q < U; q++) ent[q] = *(const FSA_GLOBAL(uint32_t)*)(base + (off + (uint32_t)q * step));
+        off += (uint32_t)U * step;
+        T d[U];
+#pragma unroll
+        for (int q = 0; q < U; q++) d[q] = load(ent[q] & kIndexMask);
+#pragma unroll
+        for (int q = 0; q < U; q++) body(ent[q], d[q]);
+    }
+    for (; e < cnt; e++) {
+        const uint32_t en = *(const FSA_GLOBAL(uint32_t)*)(base + off);
+        off += step;
+        const T dd = load(en & kIndexMask);
+        body(en, dd);
+    }
+}
+
 // walk_neighbors over a delta-coded list: codes of 64 consecutive ranks are
 // read two entries per 256-byte load, the entries decoded in order, and their
 // gathers issued before any of them is used.  U is even.
@@ -709,6 +747,8 @@ __device__ __forceinline__ void walk_list(const SceneDev& S, uint32_t k, uint32_
     if ((Spec & kSpecDelta) && !S.state->delta_plain) {
         walk_neighbors_delta<(U < 2 ? 2 : U)>(gload_ptr(S.dlist + k), gload_ptr(S.dlist + kDeltaCodeRows * S.list_stride + k),
                                            S.list_stride, cnt, load, body);
+    } else if ((uint64_t)S.list_stride * (uint64_t)(4 * kMaxNeighbors) <= 0xffffffffull) {
+        walk_neighbors32<U>(S.nbr, k, S.list_stride, cnt, load, body);
     } else {
         walk_neighbors<U>(gload_ptr(S.nbr + k), S.list_stride, cnt, load, body);
     }

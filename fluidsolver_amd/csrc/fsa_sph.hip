@@ -153,7 +153,7 @@ __device__ __forceinline__ void iisph_prepare_body(const SceneDev* __restrict__ 
     if (scene_failed(S)) return;
     const uint32_t k = br.block * kBlock + threadIdx.x;
     if (k >= S.nf) return;
-    const uint32_t c = S.state->cur;
+    const uint32_t c = (uint32_t)__builtin_amdgcn_readfirstlane((int)S.state->cur);  // uniform: the arrays' bases stay scalar
     const uint32_t i = S.fluid_idx[k];
     const auto pos = gload_ptr(S.pos[c]);
     const auto vel = gload_ptr(S.vel[c]);
@@ -171,7 +171,7 @@ __device__ __forceinline__ void iisph_prepare_body(const SceneDev* __restrict__ 
     float ix = 0.0f, iy = 0.0f, iz = 0.0f;  // d_ii of phase 2, from the same pair gradients
     walk_list<Spec, U>(
         S, k, S.nbr_cnt[k],
-        [&](uint32_t j) { return PosVel{pos[j], vel[j]}; },
+        [&](uint32_t j) { return PosVel{pos.at(j), vel.at(j)}; },
         [&](uint32_t ent, const PosVel& d) {
             const float4 xj = d.x, vj = d.v;
             float dx, dy, dz, d2, len;
@@ -234,7 +234,7 @@ __global__ __launch_bounds__(kBlock) void k_iisph_prepare_boundary(const SceneDe
     if (scene_failed(S)) return;
     const uint32_t i = br.block * kBlock + threadIdx.x;
     if (i >= S.n) return;
-    const uint32_t c = S.state->cur;
+    const uint32_t c = (uint32_t)__builtin_amdgcn_readfirstlane((int)S.state->cur);  // uniform: the arrays' bases stay scalar
     if ((S.idt[c][i] >> kTypeShift) != 1u) return;
     float4 e = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     if (S.flags & kFlagExternal) e = S.ext[c][i];
@@ -294,7 +294,7 @@ __device__ __forceinline__ void iisph_diagonal_body(const SceneDev* __restrict__
     float a2 = 0.0f, v2 = 0.0f;
     const uint32_t k = br.block * kBlock + threadIdx.x;
     if (k < S.nf) {
-        const uint32_t c = S.state->cur;
+        const uint32_t c = (uint32_t)__builtin_amdgcn_readfirstlane((int)S.state->cur);  // uniform: the arrays' bases stay scalar
         const uint32_t i = S.fluid_idx[k];
         const auto pos = gload_ptr(S.pos[c]);
         const auto vstar = gload_ptr(S.vstar);
@@ -307,7 +307,7 @@ __device__ __forceinline__ void iisph_diagonal_body(const SceneDev* __restrict__
         const float cm = div_rho0sq<Spec>(S, xi.w);
         float contrib = 0.0f, dg = 0.0f;
         walk_list<Spec, U>(
-            S, k, cnt, [&](uint32_t j) { return PosVel{pos[j], vstar[j]}; },
+            S, k, cnt, [&](uint32_t j) { return PosVel{pos.at(j), vstar.at(j)}; },
             [&](uint32_t ent, const PosVel& d) {
                 const float4 xj = d.x;
                 float dx, dy, dz, d2, len;
@@ -388,7 +388,7 @@ __device__ __forceinline__ void iisph_pass_a_body(const SceneDev* __restrict__ s
     if (!S.state->active) return;
     const uint32_t k = br.block * kBlock + threadIdx.x;
     if (k >= S.nf) return;
-    const uint32_t c = S.state->cur;
+    const uint32_t c = (uint32_t)__builtin_amdgcn_readfirstlane((int)S.state->cur);  // uniform: the arrays' bases stay scalar
     const uint32_t i = S.fluid_idx[k];
     const auto pos = gload_ptr(S.pos[c]);
     const auto pres = gload_ptr(S.pres[c]);
@@ -398,7 +398,7 @@ __device__ __forceinline__ void iisph_pass_a_body(const SceneDev* __restrict__ s
     const bool sl = (S.flags & kFlagSingleLayer) != 0;
     float ax = 0.0f, ay = 0.0f, az = 0.0f;
     walk_list<Spec, U>(
-        S, k, S.nbr_cnt[k], [&](uint32_t j) { return PosP{pos[j], pres[j]}; },
+        S, k, S.nbr_cnt[k], [&](uint32_t j) { return PosP{pos.at(j), pres.at(j)}; },
         [&](uint32_t ent, const PosP& d) {
             const float4 xj = d.x;
             float dx, dy, dz, d2, len;
@@ -440,7 +440,7 @@ __device__ __forceinline__ void iisph_pass_b_body(const SceneDev* __restrict__ s
     if (!S.state->active) return;
     __shared__ RedLds lds;
     const uint32_t k = br.block * kBlock + threadIdx.x;
-    const uint32_t c = S.state->cur;
+    const uint32_t c = (uint32_t)__builtin_amdgcn_readfirstlane((int)S.state->cur);  // uniform: the arrays' bases stay scalar
     double err_sum = 0.0;
     uint32_t counted = 0;
     float a2 = 0.0f, v2 = 0.0f;
@@ -452,7 +452,7 @@ __device__ __forceinline__ void iisph_pass_b_body(const SceneDev* __restrict__ s
         const float4 ai = acc[i];
         float ap = 0.0f;
         walk_list<Spec, U>(
-            S, k, S.nbr_cnt[k], [&](uint32_t j) { return PosAcc{pos[j], acc[j]}; },
+            S, k, S.nbr_cnt[k], [&](uint32_t j) { return PosAcc{pos.at(j), acc.at(j)}; },
             [&](uint32_t ent, const PosAcc& d) {
                 const float4 xj = d.x;
                 float dx, dy, dz, d2, len;
@@ -516,14 +516,14 @@ __global__ __launch_bounds__(kBlock) void k_sesph_density(const SceneDev* __rest
     if (scene_failed(S)) return;
     const uint32_t k = br.block * kBlock + threadIdx.x;
     if (k >= S.nf) return;
-    const uint32_t c = S.state->cur;
+    const uint32_t c = (uint32_t)__builtin_amdgcn_readfirstlane((int)S.state->cur);  // uniform: the arrays' bases stay scalar
     const uint32_t i = S.fluid_idx[k];
     const auto pos = gload_ptr(S.pos[c]);
     const float4 xi = pos[i];
     const bool sl = (S.flags & kFlagSingleLayer) != 0;
     float rho = 0.0f;
     walk_list<Spec, kUnroll>(
-        S, k, S.nbr_cnt[k], [&](uint32_t j) { return pos[j]; },
+        S, k, S.nbr_cnt[k], [&](uint32_t j) { return pos.at(j); },
         [&](uint32_t ent, const float4& xj) {
             const float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
             const float len = pair_length<Spec>(dot3(dx, dy, dz, dx, dy, dz));
@@ -553,7 +553,7 @@ __global__ __launch_bounds__(kBlock) void k_sesph_accel(const SceneDev* __restri
     if (scene_failed(S)) return;
     const uint32_t k = br.block * kBlock + threadIdx.x;
     if (k >= S.nf) return;
-    const uint32_t c = S.state->cur;
+    const uint32_t c = (uint32_t)__builtin_amdgcn_readfirstlane((int)S.state->cur);  // uniform: the arrays' bases stay scalar
     const uint32_t i = S.fluid_idx[k];
     const auto pos = gload_ptr(S.pos[c]);
     const auto vel = gload_ptr(S.vel[c]);
@@ -567,7 +567,7 @@ __global__ __launch_bounds__(kBlock) void k_sesph_accel(const SceneDev* __restri
     float ax = 0.0f, ay = 0.0f, az = 0.0f;
     walk_list<Spec, kUnroll>(
         S, k, S.nbr_cnt[k],
-        [&](uint32_t j) { return PosVelP{pos[j], vel[j], pres[j]}; },
+        [&](uint32_t j) { return PosVelP{pos.at(j), vel.at(j), pres.at(j)}; },
         [&](uint32_t ent, const PosVelP& d) {
             const float4 xj = d.x, vj = d.v;
             float dx, dy, dz, d2, len;
@@ -615,7 +615,7 @@ __global__ __launch_bounds__(kBlock) void k_cfl_all(const SceneDev* __restrict__
     if (scene_failed(S)) return;
     __shared__ RedLds lds;
     const uint32_t i = br.block * kBlock + threadIdx.x;
-    const uint32_t c = S.state->cur;
+    const uint32_t c = (uint32_t)__builtin_amdgcn_readfirstlane((int)S.state->cur);  // uniform: the arrays' bases stay scalar
     float a2 = 0.0f, v2 = 0.0f;
     if (i < S.n) {
         const float4 a = S.acc[c][i];
@@ -662,7 +662,7 @@ __global__ __launch_bounds__(kBlock) void k_integrate(const SceneDev* __restrict
     if (!integration_due(S)) return;
     __shared__ int32_t blo[3][kBlock / 64], bhi[3][kBlock / 64];
     const uint32_t i = br.block * kBlock + threadIdx.x;
-    const uint32_t c = S.state->cur;
+    const uint32_t c = (uint32_t)__builtin_amdgcn_readfirstlane((int)S.state->cur);  // uniform: the arrays' bases stay scalar
     const float max_v = (float)sqrt((double)S.state->max_v2);
     const float max_a = (float)sqrt((double)S.state->max_a2);
     const float corrected = fminf(cfl_timestep(S, max_a, max_v), S.dt);
