@@ -1428,17 +1428,14 @@ __global__ __launch_bounds__(kBlock) void k3u_lists(const SceneDev* __restrict__
 // escape rows (layout at kDeltaCodeRows in the device header).
 // Without `force`, a scene's lists are sized only when k_lists could not bound
 // them within the NeighborStorage payload, or when they are delta-coded.
+__device__ __forceinline__ uint32_t scene_fluid_parts(const SceneDev& S) { return (S.nf + kBlock - 1) / kBlock; }
 __device__ __forceinline__ bool codec_due(const SceneDev& S, uint32_t force) {
     return !sort_retry(S) && (force || (S.flags & kFlagDeltaLists) || S.state->codec_check);
 }
 
+// one kBlock part of a scene's fluid lists: sizes, capacity, delta codes; the part's totals in partial[br.block]
 template <bool Encode>
-__global__ __launch_bounds__(kBlock) void k_codec(const SceneDev* __restrict__ scenes,
-                                                  const BlockRef* __restrict__ map, uint32_t force) {
-    const BlockRef br = map[blockIdx.x];
-    const SceneDev& S = scenes[br.scene];
-    if (!codec_due(S, force)) return;
-    __shared__ uint64_t wsum[3][kBlock / 64];
+__device__ __forceinline__ void codec_part(const SceneDev& S, const BlockRef br, uint64_t (*wsum)[kBlock / 64]) {
     const uint32_t k = br.block * kBlock + threadIdx.x;
     const bool encode = Encode && (S.flags & kFlagDeltaLists) != 0;
     uint64_t cnt = 0, bytes = 0, dbytes = 0;
@@ -1524,12 +1521,8 @@ __global__ __launch_bounds__(kBlock) void k_codec(const SceneDev* __restrict__ s
     }
 }
 
-// one block per scene: the block partials of k_codec, in block order
-__global__ __launch_bounds__(kBlock) void k_codec_finalize(const SceneDev* __restrict__ scenes,
-                                                           const BlockRef* __restrict__ map, uint32_t force) {
-    const SceneDev& S = scenes[map[blockIdx.x].scene];
-    if (!codec_due(S, force)) return;
-    __shared__ uint64_t wsum[3][kBlock / 64];
+// the scene's totals from the part partials, in part order (one block)
+__device__ __forceinline__ void codec_totals(const SceneDev& S, uint64_t (*wsum)[kBlock / 64]) {
     const uint32_t nparts = (S.nf + kBlock - 1) / kBlock;
     uint64_t c = 0, b = 0, db = 0;
     // integer sums: 16 partials per thread are loaded ahead of their adds
@@ -1582,6 +1575,36 @@ __global__ __launch_bounds__(kBlock) void k_codec_finalize(const SceneDev* __res
         S.state->delta_plain = S.state->delta_spill;
         S.state->delta_spill = 0u;
         S.state->codec_check = 0u;
+    }
+}
+
+
+// The lists of the scenes due (codec_due) in parts of kBlock fluid ranks: the
+// blocks of the launch take the map's parts in turn, and the block that
+// completes a scene's last part (counted in state->done[0] after a device-wide
+// fence) sums the scene's part totals.  Scenes not due leave at once.
+template <bool Encode>
+__global__ __launch_bounds__(kBlock) void k_codec(const SceneDev* __restrict__ scenes,
+                                                  const BlockRef* __restrict__ map, uint32_t nmap, uint32_t force) {
+    __shared__ uint64_t wsum[3][kBlock / 64];
+    __shared__ uint32_t last;
+    for (uint32_t m = blockIdx.x; m < nmap; m += gridDim.x) {
+        const BlockRef br = map[m];
+        const SceneDev& S = scenes[br.scene];
+        if (!codec_due(S, force)) continue;
+        codec_part<Encode>(S, br, wsum);
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            __threadfence();  // this part's totals are visible before it is counted
+            last = atomicAdd(&S.state->done[0], 1u) == scene_fluid_parts(S) - 1u ? 1u : 0u;
+        }
+        __syncthreads();
+        if (last) {
+            __threadfence();  // every part's totals are read after the count
+            codec_totals(S, wsum);
+            if (threadIdx.x == 0) S.state->done[0] = 0u;
+        }
+        __syncthreads();
     }
 }
 
@@ -1658,15 +1681,15 @@ void launch_permute_only(const LaunchSet& L, uint32_t sbuf, hipStream_t st) {
     k_compact<<<L.nb_all, kBlock, 0, st>>>(L.scenes, L.map_all);
 }
 
+constexpr uint32_t kCodecGrid = 1024;  // blocks of a k_codec launch (each takes parts in turn)
 void launch_codec(const LaunchSet& L, hipStream_t st, bool force) {
     const uint32_t f = force ? 1u : 0u;
-    if (L.nb_fluid) {
-        if (L.spec & kSpecDelta)
-            k_codec<true><<<L.nb_fluid, kBlock, 0, st>>>(L.scenes, L.map_fluid, f);
-        else
-            k_codec<false><<<L.nb_fluid, kBlock, 0, st>>>(L.scenes, L.map_fluid, f);
-    }
-    k_codec_finalize<<<L.nb_scene, kBlock, 0, st>>>(L.scenes, L.map_scene, f);
+    if (!L.nb_fluid) return;
+    const uint32_t grid = std::min(L.nb_fluid, kCodecGrid);
+    if (L.spec & kSpecDelta)
+        k_codec<true><<<grid, kBlock, 0, st>>>(L.scenes, L.map_fluid, L.nb_fluid, f);
+    else
+        k_codec<false><<<grid, kBlock, 0, st>>>(L.scenes, L.map_fluid, L.nb_fluid, f);
 }
 
 }  // namespace fsa
