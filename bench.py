@@ -218,7 +218,7 @@ def _templates(lo, hi, buckets, seed):
     return [(int(n), scenes.scaled_dam3d(int(n), seed * 100 + k)) for k, n in enumerate(sizes)]
 
 
-def bench_served(ctx, window_s, rate, max_running, budget_gb, buckets=12):
+def bench_served(ctx, window_s, rate, max_running, budget_gb, buckets=12, serial_ticks=False):
     """Served scene stream: Poisson arrivals (rate/s) of 3D IISPH dam jobs with
     fluid counts log-uniform in [10k, 500k] and 100-2000 steps.  Jobs are
     submitted when their arrival time has passed, the server admits them
@@ -231,7 +231,7 @@ def bench_served(ctx, window_s, rate, max_running, budget_gb, buckets=12):
     tmpl = _templates(10000, 500000, buckets, 5)
     tsizes = np.array([np.log(n) for n, _ in tmpl])
     desc = abi.default_desc(3, abi.SOLVER_IISPH, neighbor_lists=LIST_MODE)
-    srv = Server(ctx, hbm_budget_bytes=int(budget_gb * 1e9), max_running=max_running)
+    srv = Server(ctx, hbm_budget_bytes=int(budget_gb * 1e9), max_running=max_running, serial_ticks=serial_ticks)
     k = 0
     offered = 0  # particle-updates the arrived jobs ask for
     open_jobs = {}
@@ -279,10 +279,12 @@ def bench_served(ctx, window_s, rate, max_running, budget_gb, buckets=12):
         "workload": f"Poisson arrivals {rate}/s of 3D IISPH dam jobs, fluid log-uniform in [10k, 500k] "
         f"({buckets} size templates), 100-2000 steps each",
         "window_s": window_s,
+        "ticks": "serial" if serial_ticks else "overlapped",
         "max_running": max_running,
         "hbm_budget_gb": budget_gb,
         "jobs_arrived": k,
         "jobs_done": st["jobs_done"],
+        "jobs_failed": st["jobs_failed"],
         "scenes_per_s": st["jobs_done"] / elapsed,
         "scene_steps_per_s": st["scene_steps"] / elapsed,
         "particle_updates_per_s": st["particle_updates"] / elapsed,
@@ -396,6 +398,8 @@ def main():
     ap.add_argument("--serve-slots", type=int, default=32)
     ap.add_argument("--serve-budget-gb", type=float, default=200.0)
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serial-ticks", action="store_true",
+                    help="served leg: admit, step and retire one after another instead of overlapping the transfers")
     ap.add_argument("--obstacle-steps", type=int, default=200)
     ap.add_argument("--full-steps", type=int, default=1000,
                     help="also time this many steps of a fresh 1M scene from rest (the whole configuration; 0: skip)")
@@ -577,7 +581,8 @@ def main():
         log(json.dumps(batched))
     served = None
     if not args.no_serve:
-        served = bench_served(ctx, args.serve_seconds, args.serve_rate, args.serve_slots, args.serve_budget_gb)
+        served = bench_served(ctx, args.serve_seconds, args.serve_rate, args.serve_slots, args.serve_budget_gb,
+                              serial_ticks=args.serial_ticks)
         log(json.dumps(served))
 
     obstacles = None

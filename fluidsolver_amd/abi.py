@@ -173,7 +173,8 @@ class ServerConfig(C.Structure):
     _fields_ = [
         ("hbm_budget_bytes", C.c_uint64),
         ("max_running", C.c_uint32),
-        ("reserved", C.c_uint32 * 5),
+        ("serial_ticks", C.c_uint32),
+        ("reserved", C.c_uint32 * 4),
     ]
 
 

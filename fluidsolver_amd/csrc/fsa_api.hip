@@ -97,6 +97,14 @@ struct fsa_context_s {
     std::mutex pin_mu;
 };
 
+namespace {
+// The stream of a thread's scene work: the context stream, or a stream of the
+// thread's own while another thread steps scenes on the context stream (a
+// server tick uploads admitted jobs and downloads finished ones beside the step).
+thread_local hipStream_t t_stream = nullptr;
+inline hipStream_t stream_of(const fsa_context_s* c) { return t_stream ? t_stream : c->stream; }
+}  // namespace
+
 // Launch set over several scenes (one table entry and block-map range each).
 struct BatchSet {
     std::vector<fsa_scene_s*> scenes;
@@ -461,23 +469,27 @@ std::vector<BlockRef> xcd_order(const std::vector<BlockRef>& v, size_t chunk) {
     return out;
 }
 
-BlockRef* upload_map(const std::vector<BlockRef>& v, std::vector<BlockRef*>& owned) {
+// Block maps are copied on the caller's stream (a synchronous copy on the null
+// stream would break a graph capture that another thread runs meanwhile).
+BlockRef* upload_map(const std::vector<BlockRef>& v, std::vector<BlockRef*>& owned, hipStream_t st) {
     if (v.empty()) return nullptr;
     BlockRef* d = nullptr;
     HIP_CHECK(hipMalloc(&d, v.size() * sizeof(BlockRef)));
     owned.push_back(d);
-    HIP_CHECK(hipMemcpy(d, v.data(), v.size() * sizeof(BlockRef), hipMemcpyHostToDevice));
+    HIP_CHECK(hipMemcpyAsync(d, v.data(), v.size() * sizeof(BlockRef), hipMemcpyHostToDevice, st));
+    HIP_CHECK(hipStreamSynchronize(st));
     return d;
 }
 
-void build_launch_set(const MapBuilder& mb, SceneDev* table, LaunchSet& L, std::vector<BlockRef*>& owned) {
+void build_launch_set(const MapBuilder& mb, SceneDev* table, LaunchSet& L, std::vector<BlockRef*>& owned,
+                      hipStream_t st) {
     L.scenes = table;
-    L.map_all = upload_map(xcd_order(mb.all, 16), owned);
-    L.map_fluid = upload_map(xcd_order(mb.fluid, 0), owned);
-    L.map_tiles = upload_map(mb.tiles, owned);
-    L.map_scan = upload_map(mb.scan, owned);
-    L.map_scene = upload_map(mb.scene, owned);
-    L.map_cnt = upload_map(mb.cnt, owned);
+    L.map_all = upload_map(xcd_order(mb.all, 16), owned, st);
+    L.map_fluid = upload_map(xcd_order(mb.fluid, 0), owned, st);
+    L.map_tiles = upload_map(mb.tiles, owned, st);
+    L.map_scan = upload_map(mb.scan, owned, st);
+    L.map_scene = upload_map(mb.scene, owned, st);
+    L.map_cnt = upload_map(mb.cnt, owned, st);
     L.nb_cnt = (uint32_t)mb.cnt.size();
     L.nb_all = (uint32_t)mb.all.size();
     L.nb_fluid = (uint32_t)mb.fluid.size();
@@ -554,13 +566,13 @@ void prepare_grid(fsa_scene_s* s) {
         if (s->ctx->budget && s->ctx->in_use + cell_alloc_bytes(cap) > s->ctx->budget)
             throw FsaError(FSA_E_BUDGET, "cell table exceeds the HBM budget");
         HIP_CHECK(hipMalloc(&s->cells, cell_alloc_bytes(cap)));
-        HIP_CHECK(hipMemsetAsync(s->cells, 0, cap * sizeof(uint4), s->ctx->stream));
+        HIP_CHECK(hipMemsetAsync(s->cells, 0, cap * sizeof(uint4), stream_of(s->ctx)));
         s->cells_cap = cap;
         s->ranks = (uint32_t*)(s->cells + cap);
         s->bcnt = s->ranks + cap;
         s->boff = s->bcnt + cap;      // cap + 1 entries
         s->bsum = s->boff + cap + 1;  // kCountBlocks entries
-        HIP_CHECK(hipMemsetAsync(s->bcnt, 0, cap * sizeof(uint32_t), s->ctx->stream));
+        HIP_CHECK(hipMemsetAsync(s->bcnt, 0, cap * sizeof(uint32_t), stream_of(s->ctx)));
         s->ranks_valid = false;
         s->ctx->in_use += cell_alloc_bytes(cap);
     }
@@ -582,7 +594,7 @@ void prepare_grid(fsa_scene_s* s) {
             bool same = s->ranks_valid && s->rank_levels == L;
             for (int a = 0; a < 3; a++) same = same && s->rank_cmin[a] == h.cmin[a] && s->rank_cdim[a] == h.cdim[a];
             if (!same) {
-                launch_rank_table(s->ranks, h.cmin, h.cdim, L, s->ctx->stream);
+                launch_rank_table(s->ranks, h.cmin, h.cdim, L, stream_of(s->ctx));
                 for (int a = 0; a < 3; a++) {
                     s->rank_cmin[a] = h.cmin[a];
                     s->rank_cdim[a] = h.cdim[a];
@@ -594,7 +606,7 @@ void prepare_grid(fsa_scene_s* s) {
     }
     h.stamp++;
     if (h.stamp == 0) {
-        HIP_CHECK(hipMemsetAsync(s->cells, 0, s->cells_cap * sizeof(uint4), s->ctx->stream));
+        HIP_CHECK(hipMemsetAsync(s->cells, 0, s->cells_cap * sizeof(uint4), stream_of(s->ctx)));
         h.stamp = 1;
     }
 }
@@ -622,8 +634,8 @@ uint32_t sort_passes(const SceneDev& h) {
 }
 
 void read_state(fsa_scene_s* s) {
-    HIP_CHECK(hipMemcpyAsync(s->h_state, s->hd.state, sizeof(SceneState), hipMemcpyDeviceToHost, s->ctx->stream));
-    HIP_CHECK(hipStreamSynchronize(s->ctx->stream));
+    HIP_CHECK(hipMemcpyAsync(s->h_state, s->hd.state, sizeof(SceneState), hipMemcpyDeviceToHost, stream_of(s->ctx)));
+    HIP_CHECK(hipStreamSynchronize(stream_of(s->ctx)));
 }
 
 // one copy of the pool range covering every scene of the list
@@ -821,7 +833,7 @@ void upload_scene_table(fsa_scene_s* s) {
     s->single.wide_lists = wide_lists(s->hd);
     s->single.entities_before = (s->hd.entity_bits & (kEntityBitBoundary | kEntityBitVelocity)) != 0;
     s->single.remover = (s->hd.entity_bits & kEntityBitRemover) != 0;
-    HIP_CHECK(hipMemcpyAsync(s->d_table, &s->hd, sizeof(SceneDev), hipMemcpyHostToDevice, s->ctx->stream));
+    HIP_CHECK(hipMemcpyAsync(s->d_table, &s->hd, sizeof(SceneDev), hipMemcpyHostToDevice, stream_of(s->ctx)));
 }
 
 uint32_t bits_of(uint64_t v) {
@@ -896,8 +908,8 @@ void ensure_quick_sort(fsa_scene_s* s, uint64_t rows) {
         if (s->qs_ref_newer) {
             s->ref_order.resize(s->qs_rows);
             HIP_CHECK(hipMemcpyAsync(s->ref_order.data(), s->qs_ref[s->qs_cur], 4 * s->qs_rows, hipMemcpyDeviceToHost,
-                                     s->ctx->stream));
-            HIP_CHECK(hipStreamSynchronize(s->ctx->stream));
+                                     stream_of(s->ctx)));
+            HIP_CHECK(hipStreamSynchronize(stream_of(s->ctx)));
             s->qs_ref_newer = false;
         }
         (void)hipFree(s->qs_mem);
@@ -941,7 +953,7 @@ void ensure_quick_sort(fsa_scene_s* s, uint64_t rows) {
 constexpr uint32_t kQsLevelsPerCheck = 4;
 
 void device_quick_sort(fsa_scene_s* s, uint64_t rows) {
-    hipStream_t st = s->ctx->stream;
+    hipStream_t st = stream_of(s->ctx);
     const uint32_t hi = (uint32_t)(rows - 1);
     QsCtr ctr[3] = {{0u, 0u}, {0u, 0u}, {0u, 0u}};
     std::vector<QsChunk> chunks;
@@ -989,7 +1001,7 @@ void hashed_resort_if_due(fsa_scene_s* s) {
     const uint32_t n = s->n;
     const uint64_t N = s->total;
     if (N < 2) return;
-    hipStream_t st = s->ctx->stream;
+    hipStream_t st = stream_of(s->ctx);
     ensure_quick_sort(s, N);
     read_state(s);
     if (!s->qs_ref_valid) {
@@ -1023,8 +1035,8 @@ bool take_sort_retry(fsa_scene_s* s) {
     if (!(s->h_state->status & kStatusSortRetry)) return false;
     s->h_state->status &= ~kStatusSortRetry;
     HIP_CHECK(hipMemcpyAsync(&s->hd.state->status, &s->h_state->status, sizeof(uint32_t), hipMemcpyHostToDevice,
-                             s->ctx->stream));
-    HIP_CHECK(hipStreamSynchronize(s->ctx->stream));
+                             stream_of(s->ctx)));
+    HIP_CHECK(hipStreamSynchronize(stream_of(s->ctx)));
     s->radix_steps = kRadixRetrySteps;
     return true;
 }
@@ -1146,6 +1158,9 @@ void free_batch(BatchSet* b);
 constexpr size_t kBatchCacheSize = 16;
 
 BatchSet* batch_for(fsa_context ctx, const std::vector<fsa_scene_s*>& v) {
+    // the cache is also pruned by scene destruction, which a server tick may run
+    // on another thread while this one steps
+    std::lock_guard<std::mutex> g(ctx->mu);
     auto it = ctx->batches.find(v);
     if (it != ctx->batches.end()) {
         it->second->last_use = ++ctx->batch_clock;
@@ -1165,7 +1180,7 @@ BatchSet* batch_for(fsa_context ctx, const std::vector<fsa_scene_s*>& v) {
     HIP_CHECK(hipMalloc(&b->d_table, sizeof(SceneDev) * v.size()));
     MapBuilder mb;
     for (uint32_t q = 0; q < v.size(); q++) mb.add(q, v[q]->hd);
-    build_launch_set(mb, b->d_table, b->ls, b->owned);
+    build_launch_set(mb, b->d_table, b->ls, b->owned, ctx->stream);
     ctx->batches[v] = b;
     return b;
 }
@@ -1701,7 +1716,7 @@ void do_upload2d(fsa_scene_s* s, const fsa_particle_view* v, bool first) {
     h.flags &= ~(kFlagExternal | kFlagDynamicBoundary);
     if (any_ext) h.flags |= kFlagExternal;
     h.grid_count = (h.flags & kFlagQuadratic) ? n : (uint32_t)(uint16_t)n;  // particleAmount_t (uint16) count
-    hipStream_t st = s->ctx->stream;
+    hipStream_t st = stream_of(s->ctx);
     SceneState init{};
     init.dt_actual = s->desc.timestep;
     for (int a = 0; a < 2; a++) {
@@ -1809,7 +1824,7 @@ void refresh_entities(fsa_scene_s* s) {
         }
         if (!s->row_tags.empty())
             HIP_CHECK(hipMemcpyAsync(s->d_tags, s->row_tags.data(), 4 * s->row_tags.size(), hipMemcpyHostToDevice,
-                                     s->ctx->stream));
+                                     stream_of(s->ctx)));
     }
     h.tag_of_row = s->d_tags;
     if (bits & kEntityBitBoundary) {
@@ -1887,8 +1902,8 @@ void do_upload(fsa_scene_s* s, const fsa_particle_view* v, bool first,
         // the last re-sort left the collection order on the device
         s->ref_order.resize(s->total);
         HIP_CHECK(hipMemcpyAsync(s->ref_order.data(), s->qs_ref[s->qs_cur], 4ull * s->total, hipMemcpyDeviceToHost,
-                                 s->ctx->stream));
-        HIP_CHECK(hipStreamSynchronize(s->ctx->stream));
+                                 stream_of(s->ctx)));
+        HIP_CHECK(hipStreamSynchronize(stream_of(s->ctx)));
         s->qs_ref_newer = false;
     }
     if (!first) {
@@ -1904,8 +1919,8 @@ void do_upload(fsa_scene_s* s, const fsa_particle_view* v, bool first,
             read_state(s);
             std::vector<uint32_t> idt(s->n);
             HIP_CHECK(hipMemcpyAsync(idt.data(), s->hd.idt[s->h_state->cur], 4ull * s->n, hipMemcpyDeviceToHost,
-                                     s->ctx->stream));
-            HIP_CHECK(hipStreamSynchronize(s->ctx->stream));
+                                     stream_of(s->ctx)));
+            HIP_CHECK(hipStreamSynchronize(stream_of(s->ctx)));
             for (uint32_t w : idt) order.push_back(w & kIdMask);
         }
         if (order.size() == active.size()) {
@@ -2004,7 +2019,7 @@ void do_upload(fsa_scene_s* s, const fsa_particle_view* v, bool first,
     if (any_ext) h.flags |= kFlagExternal;
     if (dyn_boundary) h.flags |= kFlagDynamicBoundary;
     if (any_tiny) h.flags |= kFlagTinyCoords;
-    hipStream_t st = s->ctx->stream;
+    hipStream_t st = stream_of(s->ctx);
     if ((h.flags & kFlagUnsorted) && n) {
         // the collection keeps its order between searches: fluid ranks are fixed here
         std::vector<uint32_t> fidx, rank(n, 0);
@@ -2062,7 +2077,7 @@ void fetch(fsa_scene_s* s, HostCopy& hc, bool with_err) {
     hc.ext.resize(n);
     hc.pres.resize(n);
     hc.idt.resize(n);
-    hipStream_t st = s->ctx->stream;
+    hipStream_t st = stream_of(s->ctx);
     const SceneDev& h = s->hd;
     if (n) {
         HIP_CHECK(hipMemcpyAsync(hc.pos.data(), h.pos[c], 16ull * n, hipMemcpyDeviceToHost, st));
@@ -2087,7 +2102,7 @@ void do_download3d(fsa_scene_s* s, const fsa_particle_view* v) {
     const uint32_t n = s->n;
     const bool ext_live = (s->hd.flags & kFlagExternal) != 0;
     const SceneDev& h = s->hd;
-    hipStream_t st = s->ctx->stream;
+    hipStream_t st = stream_of(s->ctx);
     std::unique_lock<std::mutex> pin_lock(s->ctx->pin_mu);
     // 4 x 16 B + 2 x 4 B per particle
     char* stg = pinned_staging(s->ctx, 72ull * std::max<uint32_t>(n, 1));
@@ -2236,7 +2251,7 @@ void relayout(fsa_scene_s* s, const fsa_particle_view* v, const std::vector<uint
     }
     const Plan p = make_plan(nf + nb, nf, 3, (s->hd.flags & kFlagUnsorted) != 0, (s->hd.flags & kFlagDeltaLists) != 0);
     const uint64_t bytes = plan_bytes(p);
-    HIP_CHECK(hipStreamSynchronize(s->ctx->stream));
+    HIP_CHECK(hipStreamSynchronize(stream_of(s->ctx)));
     if (bytes > s->bytes) {
         void* m = nullptr;
         if (hipMalloc(&m, bytes) != hipSuccess) throw FsaError(FSA_E_NOMEM, "device allocation failed");
@@ -2251,7 +2266,7 @@ void relayout(fsa_scene_s* s, const fsa_particle_view* v, const std::vector<uint
     s->map_allocs.clear();
     MapBuilder mb;
     mb.add(0, s->hd);
-    build_launch_set(mb, s->d_table, s->single, s->map_allocs);
+    build_launch_set(mb, s->d_table, s->single, s->map_allocs, stream_of(s->ctx));
     {
         std::lock_guard<std::mutex> g(s->ctx->mu);
         drop_batches_of(s->ctx, s);
@@ -2305,7 +2320,7 @@ void do_download_iisph(fsa_scene_s* s, void* rec, uint64_t stride) {
     const uint32_t n = s->n;
     std::vector<float4> vs(n), an(n);
     std::vector<float> src(n), dg(n);
-    hipStream_t st = s->ctx->stream;
+    hipStream_t st = stream_of(s->ctx);
     if (n) {
         HIP_CHECK(hipMemcpyAsync(vs.data(), s->hd.vstar, 16ull * n, hipMemcpyDeviceToHost, st));
         HIP_CHECK(hipMemcpyAsync(an.data(), s->hd.anp, 16ull * n, hipMemcpyDeviceToHost, st));
@@ -2346,7 +2361,7 @@ void build_csr(fsa_scene_s* s) {
     const uint32_t c = s->h_state->cur;
     const uint32_t n = s->n, nf = s->nf;
     std::vector<uint32_t> idt(n), fidx(nf), cnt(nf), nbr((size_t)kMaxNeighbors * s->hd.list_stride);
-    hipStream_t st = s->ctx->stream;
+    hipStream_t st = stream_of(s->ctx);
     if (n) HIP_CHECK(hipMemcpyAsync(idt.data(), s->hd.idt[c], 4ull * n, hipMemcpyDeviceToHost, st));
     if (nf) {
         HIP_CHECK(hipMemcpyAsync(fidx.data(), s->hd.fluid_idx, 4ull * nf, hipMemcpyDeviceToHost, st));
@@ -2453,6 +2468,7 @@ struct fsa_server_s {
     double busy_s = 0.0;
     double admit_s = 0.0, step_s = 0.0, retire_s = 0.0;  // parts of busy_s and of retirement
     std::vector<std::pair<double, uint32_t>> step_ms;  // tick latency, scenes stepped
+    hipStream_t xfer = nullptr;  // uploads and downloads beside a tick's step (default priority)
     double now() const {
         return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     }
@@ -2513,15 +2529,21 @@ void destroy_server(fsa_server srv) {
         if (kv.second->scene) destroy_scene(kv.second->scene);
         delete kv.second;
     }
+    if (srv->xfer) {
+        (void)hipStreamSynchronize(srv->xfer);
+        (void)hipStreamDestroy(srv->xfer);
+    }
     auto& v = srv->ctx->servers;
     v.erase(std::remove(v.begin(), v.end(), srv), v.end());
     delete srv;
 }
 
-void retire(fsa_server srv, Job* j, uint32_t state, uint32_t status) {
+// Ends a running job: a DONE job's final state is copied into the job's host
+// arrays first (unless `fetched`: copied already), then the scene is destroyed.
+void retire(fsa_server srv, Job* j, uint32_t state, uint32_t status, bool fetched = false) {
     const double t0 = srv->now();
     if (j->scene) {
-        if (state == FSA_JOB_DONE) {
+        if (state == FSA_JOB_DONE && !fetched) {
             fsa_particle_view v = j->view();
             do_download(j->scene, &v);
         }
@@ -2541,6 +2563,113 @@ void retire(fsa_server srv, Job* j, uint32_t state, uint32_t status) {
         j->info.clear();
     }
     srv->retire_s += srv->now() - t0;
+}
+
+constexpr uint32_t kNotStepped = 0xffffffffu;
+
+// One step of the jobs' scenes in shared launches; st receives each scene's
+// stats (status kNotStepped for a scene the batch did not reach).
+int32_t step_jobs(fsa_server srv, const std::vector<Job*>& js, std::vector<fsa_step_stats>& st) {
+    std::vector<fsa_scene> scenes;
+    for (Job* j : js) scenes.push_back(j->scene);
+    st.assign(js.size(), fsa_step_stats{});
+    for (auto& e : st) e.status = kNotStepped;
+    return fsa_batch_step(srv->ctx, scenes.data(), (uint32_t)scenes.size(), 1, st.data());
+}
+
+// Books a step of the jobs: a scene that failed fails its job alone (a failed
+// batch reports per scene; scenes with status OK completed the step).  Jobs
+// whose last step this was are retired now with `retire_done`, otherwise in the
+// next tick.
+void book_step(fsa_server srv, const std::vector<Job*>& js, const std::vector<fsa_step_stats>& st, int32_t rc,
+               bool retire_done) {
+    for (size_t q = 0; q < js.size(); q++) {
+        Job* j = js[q];
+        const uint32_t status = rc == FSA_OK ? FSA_OK : (st[q].status == kNotStepped ? (uint32_t)rc : st[q].status);
+        if (status != FSA_OK) {
+            retire(srv, j, FSA_JOB_FAILED, status);
+            continue;
+        }
+        j->last = st[q];
+        j->steps_done++;
+        srv->particle_updates += j->fluid;
+        if (retire_done && j->steps_done >= j->steps_total) retire(srv, j, FSA_JOB_DONE, FSA_OK);
+    }
+}
+
+// While alive, the calling thread's scene work goes to `st` and its capture
+// mode is relaxed, so that its allocations and copies are not refused while
+// another thread captures the step's launches into a graph.
+struct StreamScope {
+    hipStreamCaptureMode mode = hipStreamCaptureModeRelaxed;
+    explicit StreamScope(hipStream_t st) {
+        HIP_CHECK(hipThreadExchangeStreamCaptureMode(&mode));
+        t_stream = st;
+    }
+    ~StreamScope() {
+        t_stream = nullptr;
+        (void)hipThreadExchangeStreamCaptureMode(&mode);
+    }
+};
+
+// An overlapped tick: the stepping jobs take their step on a worker thread
+// (context stream) while this thread copies the finished jobs' final states to
+// host memory and admits queued jobs (uploads on the server's stream).  Device
+// memory is freed after the step; admitted jobs take their first step in the
+// next tick.
+void overlapped_tick(fsa_server srv, const std::vector<Job*>& stepping, const std::vector<Job*>& finished) {
+    fsa_context c = srv->ctx;
+    // the pinned staging is grown here, not while the step runs
+    uint64_t stage = 0;
+    for (Job* j : finished) stage = std::max<uint64_t>(stage, 72ull * j->scene->n);
+    for (uint64_t id : srv->queue) stage = std::max<uint64_t>(stage, 76ull * srv->jobs[id]->count);
+    if (stage) {
+        std::lock_guard<std::mutex> g(c->pin_mu);
+        pinned_staging(c, stage);
+    }
+    std::vector<fsa_step_stats> st;
+    int32_t rc = FSA_OK;
+    std::string err;
+    double step_s = 0.0;
+    std::thread worker([&] {
+        const double ts = srv->now();
+        rc = step_jobs(srv, stepping, st);
+        if (rc != FSA_OK) err = g_error;
+        step_s = srv->now() - ts;
+    });
+    std::vector<uint32_t> fetched(finished.size(), FSA_OK);
+    std::vector<uint64_t> admitted;
+    std::exception_ptr failure;
+    try {
+        StreamScope scope(srv->xfer);
+        const double tr = srv->now();
+        for (size_t q = 0; q < finished.size(); q++) {
+            try {
+                fsa_particle_view v = finished[q]->view();
+                do_download(finished[q]->scene, &v);
+            } catch (const FsaError& e) {
+                fetched[q] = (uint32_t)e.code;
+            }
+        }
+        srv->retire_s += srv->now() - tr;
+        const size_t before = srv->running.size();
+        const double ta = srv->now();
+        admit(srv);
+        srv->admit_s += srv->now() - ta;
+        admitted.assign(srv->running.begin() + (std::ptrdiff_t)before, srv->running.end());
+    } catch (...) {
+        failure = std::current_exception();
+    }
+    worker.join();
+    srv->step_s += step_s;
+    // the step's state reads copy a range of slots and may have covered an
+    // admitted scene's slot while it was written: read those states again
+    for (uint64_t id : admitted) read_state(srv->jobs[id]->scene);
+    book_step(srv, stepping, st, rc, false);
+    for (size_t q = 0; q < finished.size(); q++)
+        retire(srv, finished[q], fetched[q] == FSA_OK ? FSA_JOB_DONE : FSA_JOB_FAILED, fetched[q], true);
+    if (failure) std::rethrow_exception(failure);
+    if (rc != FSA_OK) g_error = err;
 }
 
 }  // namespace
@@ -2717,10 +2846,10 @@ FSA_API int32_t fsa_scene_create(fsa_context c, const fsa_scene_desc* desc, cons
         HIP_CHECK(hipMalloc(&s->d_table, sizeof(SceneDev)));
         MapBuilder mb;
         mb.add(0, s->hd);
-        build_launch_set(mb, s->d_table, s->single, s->map_allocs);
+        build_launch_set(mb, s->d_table, s->single, s->map_allocs, stream_of(s->ctx));
         do_upload(s, view, true);
         upload_scene_table(s);
-        HIP_CHECK(hipStreamSynchronize(c->stream));
+        HIP_CHECK(hipStreamSynchronize(stream_of(c)));
         *out = s;
     });
     if (rc != FSA_OK && s) destroy_scene(s);
@@ -2730,7 +2859,7 @@ FSA_API int32_t fsa_scene_create(fsa_context c, const fsa_scene_desc* desc, cons
 FSA_API int32_t fsa_scene_destroy(fsa_scene s) {
     return guarded([&] {
         if (!s) return;
-        (void)hipStreamSynchronize(s->ctx->stream);
+        (void)hipStreamSynchronize(stream_of(s->ctx));
         destroy_scene(s);
     });
 }
@@ -2817,7 +2946,7 @@ FSA_API int32_t fsa_scene_neighbor_stats(fsa_scene s, uint64_t* total, uint64_t*
         // the NeighborStorage sizes of the current lists, counted on the device
         upload_scene_table(s);
         LaunchSet L = s->single;
-        launch_codec(L, s->ctx->stream, true);
+        launch_codec(L, stream_of(s->ctx), true);
         read_state(s);
         if (total) *total = s->h_state->total_neighbors;
         if (dev_bytes)
@@ -2845,7 +2974,7 @@ FSA_API int32_t fsa_scene_set_entities(fsa_scene s, const fsa_entity* e, uint32_
         if (count && s->desc.dimension != 3)
             throw FsaError(FSA_E_INVALID, "the entities act on 3D collections (MovementData3D)");
         HIP_CHECK(hipSetDevice(s->ctx->device));
-        HIP_CHECK(hipStreamSynchronize(s->ctx->stream));
+        HIP_CHECK(hipStreamSynchronize(stream_of(s->ctx)));
         s->entities.assign(e, e + count);
         if (s->d_entities) (void)hipFree(s->d_entities);
         s->d_entities = nullptr;
@@ -2854,12 +2983,13 @@ FSA_API int32_t fsa_scene_set_entities(fsa_scene s, const fsa_entity* e, uint32_
                 s->d_entities = nullptr;
                 throw FsaError(FSA_E_NOMEM, "device allocation failed");
             }
-            HIP_CHECK(hipMemcpy(s->d_entities, e, sizeof(EntityDev) * count, hipMemcpyHostToDevice));
+            HIP_CHECK(hipMemcpyAsync(s->d_entities, e, sizeof(EntityDev) * count, hipMemcpyHostToDevice,
+                                     stream_of(s->ctx)));
         }
         refresh_entities(s);
         s->boundary_dirty = true;
         upload_scene_table(s);
-        HIP_CHECK(hipStreamSynchronize(s->ctx->stream));
+        HIP_CHECK(hipStreamSynchronize(stream_of(s->ctx)));
     });
 }
 
@@ -2867,7 +2997,7 @@ FSA_API int32_t fsa_scene_get_statistics(fsa_scene s, float zero_height, fsa_sce
     return guarded([&] {
         if (!s || !out) throw FsaError(FSA_E_INVALID, "NULL argument");
         HIP_CHECK(hipSetDevice(s->ctx->device));
-        hipStream_t st = s->ctx->stream;
+        hipStream_t st = stream_of(s->ctx);
         const uint64_t need = statistics_scratch_bytes(std::max<uint32_t>(s->hd.nblocks, 1));
         if (s->d_stats_bytes < need) {
             if (s->d_stats) (void)hipFree(s->d_stats);
@@ -2898,7 +3028,7 @@ FSA_API int32_t fsa_scene_get_statistics_noalloc(fsa_scene s, float zero_height,
         if (!s->ctx->stat_scratch)
             throw FsaError(FSA_E_DEVICE, "the statistics scratch of this device has no address");
         HIP_CHECK(hipSetDevice(s->ctx->device));
-        hipStream_t st = s->ctx->stream;
+        hipStream_t st = stream_of(s->ctx);
         upload_scene_table(s);
         // every block writes its partial and the finalize block writes the result: nothing
         // in the scratch needs clearing between calls
@@ -2944,7 +3074,7 @@ float4* upload_queries(fsa_scene_s* s, const float* positions, uint64_t count, u
     }
     grow(s->d_query, s->d_query_bytes, std::max<uint64_t>(16 * count + extra, 256));
     if (count)
-        HIP_CHECK(hipMemcpyAsync(s->d_query, q.data(), 16 * count, hipMemcpyHostToDevice, s->ctx->stream));
+        HIP_CHECK(hipMemcpyAsync(s->d_query, q.data(), 16 * count, hipMemcpyHostToDevice, stream_of(s->ctx)));
     return (float4*)s->d_query;
 }
 
@@ -2954,7 +3084,7 @@ FSA_API int32_t fsa_scene_query_neighbors(fsa_scene s, const float* positions, u
         if (!s || !nnz) throw FsaError(FSA_E_INVALID, "NULL argument");
         HIP_CHECK(hipSetDevice(s->ctx->device));
         check_queryable(s);
-        hipStream_t st = s->ctx->stream;
+        hipStream_t st = stream_of(s->ctx);
         const uint64_t cbytes = (4 * count + 15) & ~15ull;
         float4* q = upload_queries(s, positions, count, cbytes + 8 * count);
         uint32_t* d_counts = (uint32_t*)((char*)q + 16 * count);
@@ -3032,7 +3162,7 @@ FSA_API int32_t fsa_scene_sensor_plane(fsa_scene s, const fsa_sensor_plane* p, f
         P.sub_den = (float)p->sub_sample_grid_size + 1.0f;
         const float g = (float)(p->sub_sample_grid_size + 1);
         P.div = g * g;
-        hipStream_t st = s->ctx->stream;
+        hipStream_t st = stream_of(s->ctx);
         grow(s->d_query_out, s->d_query_out_bytes, 12 * samples);
         upload_scene_table(s);
         launch_sensor_plane(s->d_table, P, (float*)s->d_query_out, st);
@@ -3047,7 +3177,7 @@ FSA_API int32_t fsa_scene_evaluate_volume(fsa_scene s, const float* positions, u
         if (!s || (count && !out)) throw FsaError(FSA_E_INVALID, "NULL argument");
         HIP_CHECK(hipSetDevice(s->ctx->device));
         check_queryable(s);
-        hipStream_t st = s->ctx->stream;
+        hipStream_t st = stream_of(s->ctx);
         float4* q = upload_queries(s, positions, count, 0);
         if (!count) return;
         grow(s->d_query_out, s->d_query_out_bytes, 36 * count);
@@ -3209,8 +3339,13 @@ FSA_API int32_t fsa_context_get_timing(fsa_context c, double* total_ms, uint64_t
 FSA_API int32_t fsa_server_create(fsa_context c, const fsa_server_config* cfg, fsa_server* out) {
     return guarded([&] {
         if (!c || !out) throw FsaError(FSA_E_INVALID, "NULL argument");
+        HIP_CHECK(hipSetDevice(c->device));
+        hipStream_t xfer = nullptr;
+        if (hipStreamCreateWithFlags(&xfer, hipStreamNonBlocking) != hipSuccess)
+            throw FsaError(FSA_E_DEVICE, "hipStreamCreate failed");
         auto* s = new fsa_server_s();
         s->ctx = c;
+        s->xfer = xfer;
         if (cfg) s->cfg = *cfg;
         c->servers.push_back(s);
         *out = s;
@@ -3265,7 +3400,11 @@ FSA_API int32_t fsa_server_cancel(fsa_server srv, uint64_t id) {
             j->state = FSA_JOB_CANCELLED;
             j->finish_s = srv->now();
         } else if (j->state == FSA_JOB_RUNNING) {
-            retire(srv, j, FSA_JOB_CANCELLED, FSA_OK);
+            // a job whose steps have all run (retired by the next tick) is done, not cancelled
+            if (j->steps_done >= j->steps_total)
+                retire(srv, j, FSA_JOB_DONE, FSA_OK);
+            else
+                retire(srv, j, FSA_JOB_CANCELLED, FSA_OK);
         }
     });
 }
@@ -3273,47 +3412,49 @@ FSA_API int32_t fsa_server_cancel(fsa_server srv, uint64_t id) {
 FSA_API int32_t fsa_server_tick(fsa_server srv, uint32_t* running) {
     return guarded([&] {
         if (!srv) throw FsaError(FSA_E_INVALID, "server is NULL");
+        if (running) *running = 0;
         const double t0 = srv->now();
-        admit(srv);
+        // running jobs with steps left, and jobs whose steps ended in an earlier tick
+        std::vector<Job*> stepping, finished;
+        for (uint64_t id : srv->running) {
+            Job* j = srv->jobs[id];
+            (j->steps_done >= j->steps_total ? finished : stepping).push_back(j);
+        }
+        if (!srv->cfg.serial_ticks && !stepping.empty() && !srv->ctx->timing) {
+            if (running) *running = (uint32_t)stepping.size();
+            overlapped_tick(srv, stepping, finished);
+            const double t1 = srv->now();
+            srv->ticks++;
+            srv->busy_s += t1 - t0;
+            srv->step_ms.push_back({1000.0 * (t1 - t0), (uint32_t)stepping.size()});
+            srv->scene_steps += stepping.size();
+            return;
+        }
+        // one after another: retire, admit, step every running job
+        for (Job* j : finished) retire(srv, j, FSA_JOB_DONE, FSA_OK);
         const double ta = srv->now();
-        srv->admit_s += ta - t0;
-        // jobs with no steps left (steps == 0) finish without a device step
+        admit(srv);
+        srv->admit_s += srv->now() - ta;
+        stepping.clear();
         for (uint64_t id : std::vector<uint64_t>(srv->running)) {
             Job* j = srv->jobs[id];
-            if (j->steps_done >= j->steps_total) retire(srv, j, FSA_JOB_DONE, FSA_OK);
+            if (j->steps_done >= j->steps_total)
+                retire(srv, j, FSA_JOB_DONE, FSA_OK);  // admitted with no steps
+            else
+                stepping.push_back(j);
         }
-        std::vector<fsa_scene> scenes;
-        std::vector<Job*> js;
-        for (uint64_t id : srv->running) {
-            js.push_back(srv->jobs[id]);
-            scenes.push_back(srv->jobs[id]->scene);
-        }
-        if (running) *running = (uint32_t)scenes.size();
-        if (scenes.empty()) return;
-        std::vector<fsa_step_stats> st(scenes.size());
-        constexpr uint32_t kNotStepped = 0xffffffffu;
-        for (auto& e : st) e.status = kNotStepped;
+        if (running) *running = (uint32_t)stepping.size();
+        if (stepping.empty()) return;
+        std::vector<fsa_step_stats> st;
         const double ts = srv->now();
-        const int32_t rc = fsa_batch_step(srv->ctx, scenes.data(), (uint32_t)scenes.size(), 1, st.data());
+        const int32_t rc = step_jobs(srv, stepping, st);
         const double t1 = srv->now();
         srv->step_s += t1 - ts;
         srv->ticks++;
         srv->busy_s += t1 - t0;
-        srv->step_ms.push_back({1000.0 * (t1 - t0), (uint32_t)scenes.size()});
-        srv->scene_steps += scenes.size();
-        for (size_t q = 0; q < js.size(); q++) {
-            Job* j = js[q];
-            // a failed batch reports per scene: scenes whose status is OK completed the step
-            const uint32_t status = rc == FSA_OK ? FSA_OK : (st[q].status == kNotStepped ? (uint32_t)rc : st[q].status);
-            if (status != FSA_OK) {
-                retire(srv, j, FSA_JOB_FAILED, status);
-                continue;
-            }
-            j->last = st[q];
-            j->steps_done++;
-            srv->particle_updates += j->fluid;
-            if (j->steps_done >= j->steps_total) retire(srv, j, FSA_JOB_DONE, FSA_OK);
-        }
+        srv->step_ms.push_back({1000.0 * (t1 - t0), (uint32_t)stepping.size()});
+        srv->scene_steps += stepping.size();
+        book_step(srv, stepping, st, rc, srv->cfg.serial_ticks != 0);
     });
 }
 

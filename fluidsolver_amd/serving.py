@@ -17,10 +17,11 @@ from .solver import check
 class Server:
     """Continuous batching of scene jobs on one context."""
 
-    def __init__(self, context, hbm_budget_bytes=0, max_running=0):
+    def __init__(self, context, hbm_budget_bytes=0, max_running=0, serial_ticks=False):
         cfg = abi.ServerConfig()
         cfg.hbm_budget_bytes = int(hbm_budget_bytes)
         cfg.max_running = int(max_running)
+        cfg.serial_ticks = int(bool(serial_ticks))
         h = C.c_void_p()
         check(lib().fsa_server_create(context.handle, C.byref(cfg), C.byref(h)))
         self.context = context

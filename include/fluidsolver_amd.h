@@ -302,7 +302,12 @@ FSA_API int32_t fsa_context_set_block_size(fsa_context ctx, uint32_t threads);
 typedef struct fsa_server_config {
     uint64_t hbm_budget_bytes;   /* admission budget; 0 = the context's budget */
     uint32_t max_running;        /* slot limit; 0 = no limit */
-    uint32_t reserved[5];
+    /* 0: while a tick steps the running scenes (on a worker thread, context
+     * stream), the tick uploads admitted jobs and downloads the jobs whose steps
+     * ended in the previous tick on a second stream; admitted jobs take their
+     * first step in the next tick.  1: admit, step and retire one after another. */
+    uint32_t serial_ticks;
+    uint32_t reserved[4];
 } fsa_server_config;
 
 typedef struct fsa_job_info {
@@ -335,9 +340,12 @@ FSA_API int32_t fsa_server_destroy(fsa_server srv);
 /* Copies the particles of the job; the caller's view may be released after the call. */
 FSA_API int32_t fsa_server_submit(fsa_server srv, const fsa_scene_desc* desc, const fsa_particle_view* view,
                                   uint32_t steps, uint64_t* job_id);
+/* A queued job, or a running one with steps left, is CANCELLED; a running job
+ * whose steps have all run is retired DONE. */
 FSA_API int32_t fsa_server_cancel(fsa_server srv, uint64_t job_id);
 /* Admit, step every running scene once, retire; *running receives the number
- * of scenes stepped (0 when nothing could run). */
+ * of scenes stepped (0 when nothing could run).  With overlapped ticks a job
+ * whose last step ran in this tick is retired (DONE) in the next one. */
 FSA_API int32_t fsa_server_tick(fsa_server srv, uint32_t* running);
 FSA_API int32_t fsa_server_job(fsa_server srv, uint64_t job_id, fsa_job_info* info);
 /* Forgets a job in a terminal state.  For a DONE job, first writes its final

@@ -36,67 +36,47 @@ def _alone(ctx, d, c, steps):
     return res
 
 
-def test_server_mix_equals_alone(gpu_context, oracle_lib):
+@pytest.mark.parametrize("serial,graph", [(True, False), (False, False), (False, True)])
+def test_server_mix_equals_alone(gpu_context, oracle_lib, serial, graph):
+    """Jobs arriving while others run.  Serial ticks admit, step and retire one
+    after another.  Overlapped ticks (the default) step the running jobs on a
+    worker thread while arrivals are uploaded and finished jobs downloaded on a
+    second stream, so arrivals take their first step in the tick after their
+    admission.  With graphs, they are switched on as the arrivals come, so the
+    step's launches are captured (new graph keys) while the arrivals are
+    allocated and uploaded, and again while the first finished job is
+    downloaded.  Every job equals running it alone."""
     jobs = _jobs()
-    print("case serving mix", len(jobs), "jobs", flush=True)
-    srv = Server(gpu_context)
-    ids = [srv.submit(d, c.copy(), s) for d, c, s in jobs[:3]]
-    assert srv.tick() == 3
-    assert srv.tick() == 3
-    ids += [srv.submit(d, c.copy(), s) for d, c, s in jobs[3:]]  # arrivals while others run
-    running = []
-    while True:
-        st = srv.stats()
-        if st["jobs_queued"] == 0 and st["jobs_running"] == 0:
-            break
-        running.append(srv.tick())
-    assert max(running) == 6
-    for jid, (d, c, steps) in zip(ids, jobs):
-        info = srv.job(jid)
-        assert info.state == abi.JOB_DONE and info.steps_done == steps
-        got = srv.collect(jid)
-        ref = _alone(gpu_context, d, c, steps)
-        np.testing.assert_array_equal(got.movement, ref.movement)
-        np.testing.assert_array_equal(got.particle_data, ref.particle_data)
-    st = srv.stats()
-    assert st["jobs_done"] == len(jobs) and st["jobs_failed"] == 0
-    assert st["scene_steps"] == sum(s for _, _, s in jobs)
-    assert st["particle_updates"] == sum(int((c.type == 0).sum()) * s for _, c, s in jobs)
-    assert 0 < st["p50_step_ms"] <= st["p99_step_ms"] <= st["max_step_ms"]
-    srv.close()
-    # one job against the oracle directly (device semantics: FP64 convergence sums)
-    O = oracle_lib
-    d, c, steps = jobs[1]
-    orc = O.OracleScene(d, c.copy(), mode=O.MODE_FP32_R64)
-    for _ in range(steps):
-        orc.find_neighbors()
-        orc.simulate()
-    want = orc.download(c.copy())
-    ref = _alone(gpu_context, d, c, steps)
-    np.testing.assert_array_equal(ref.movement, want.movement)
-
-
-def test_server_budget_admission(gpu_context):
-    jobs = _jobs()
-    need = [memory_estimate(d, int((c.type == 0).sum()), int((c.type == 1).sum())) for d, c, _ in jobs]
-    base = gpu_context.memory_in_use()
-    budget = base + 2 * max(need) + (1 << 20)
-    print("case serving budget", budget - base, flush=True)
-    srv = Server(gpu_context, hbm_budget_bytes=budget)
-    ids = [srv.submit(d, c.copy(), s) for d, c, s in jobs]
-    peak_running = 0
-    while True:
-        st = srv.stats()
-        if st["jobs_queued"] == 0 and st["jobs_running"] == 0:
-            break
-        n = srv.tick()
-        peak_running = max(peak_running, n)
-        assert gpu_context.memory_in_use() <= budget
-    assert 1 <= peak_running < len(jobs)
-    for jid, (d, c, steps) in zip(ids, jobs):
-        got = srv.collect(jid)
-        np.testing.assert_array_equal(got.movement, _alone(gpu_context, d, c, steps).movement)
-    srv.close()
+    print("case serving mix", len(jobs), "jobs", "serial" if serial else "overlapped", "graph" if graph else "",
+          flush=True)
+    try:
+        srv = Server(gpu_context, serial_ticks=serial)
+        ids = [srv.submit(d, c.copy(), s) for d, c, s in jobs[:3]]
+        assert srv.tick() == 3
+        assert srv.tick() == 3
+        ids += [srv.submit(d, c.copy(), s) for d, c, s in jobs[3:]]  # arrivals while others run
+        gpu_context.set_graph(graph)
+        running = []
+        while True:
+            st = srv.stats()
+            if st["jobs_queued"] == 0 and st["jobs_running"] == 0:
+                break
+            running.append(srv.tick())
+        # the first jobs have 4, 6 and 3 steps: serial ticks step the arrivals
+        # beside all three in tick 3; overlapped ticks from tick 4, beside the two
+        # with steps left
+        assert max(running) == (6 if serial else 5)
+        assert srv.stats()["jobs_failed"] == 0
+        for jid, (d, c, steps) in zip(ids, jobs):
+            info = srv.job(jid)
+            assert info.state == abi.JOB_DONE and info.steps_done == steps
+            got = srv.collect(jid)
+            ref = _alone(gpu_context, d, c, steps)
+            np.testing.assert_array_equal(got.movement, ref.movement)
+            np.testing.assert_array_equal(got.particle_data, ref.particle_data)
+        srv.close()
+    finally:
+        gpu_context.set_graph(False)
 
 
 def test_server_slot_limit_and_cancel(gpu_context):
