@@ -404,7 +404,8 @@ def main():
     ap.add_argument("--full-steps", type=int, default=1000,
                     help="also time this many steps of a fresh 1M scene from rest (the whole configuration; 0: skip)")
     ap.add_argument("--no-obstacles", action="store_true")
-    ap.add_argument("--graph", type=int, default=1, help="replay step heads from hipGraphs (1) or launch eagerly (0)")
+    ap.add_argument("--graph", type=int, default=0,
+                    help="replay step heads from hipGraphs (1) or launch eagerly (0, the faster on MI355X)")
     ap.add_argument("--lists", choices=sorted(LISTS), default="compressed",
                     help="neighbor-list storage the solver walks read: 32-bit entries with the NeighborStorage "
                          "capacity (compressed), 16-bit delta codes with the same capacity (delta), or 32-bit "
