@@ -94,7 +94,7 @@ def boundary_in_reach(sc, coll):
     return int(np.count_nonzero(coll.type[cols] == abi.TYPE_BOUNDARY))
 
 
-def survey_bytes(nf, nba, L, ncells):
+def phase_byte_model(nf, nba, L, ncells):
     """The per-step byte model of the path's measurement definition: the neighbor
     search B_NS and one Jacobi iteration B_iter."""
     return {"search": nf * (12 + 2 * 32 + L) + 12 * nba + 8 * ncells, "iteration": nf * (88 + 2 * L) + 32 * nba}
@@ -524,7 +524,7 @@ def main():
     # pressure loop I * B_iter / t_loop (t_loop: the Jacobi passes and their convergence
     # decisions, the gated pass beyond the predicted ones included; the first iteration's
     # update runs inside the phase-2 kernel)
-    sb = survey_bytes(nf, nba, L, ncells)
+    sb = phase_byte_model(nf, nba, L, ncells)
     per_step = {k: v["ms_per_step"] for k, v in phases.items()}
     t_loop = sum(per_step.get(k, 0.0) for k in ("pressure_a", "pressure_b", "pressure_first", "pressure_tail"))
     mean_it = float(np.mean(iters))
