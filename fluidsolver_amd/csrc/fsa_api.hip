@@ -2633,8 +2633,13 @@ void overlapped_tick(fsa_server srv, const std::vector<Job*>& stepping, const st
     double step_s = 0.0;
     std::thread worker([&] {
         const double ts = srv->now();
-        rc = step_jobs(srv, stepping, st);
-        if (rc != FSA_OK) err = g_error;
+        try {
+            rc = step_jobs(srv, stepping, st);
+            if (rc != FSA_OK) err = g_error;
+        } catch (const std::exception& e) {  // host allocation failure before the batch ran
+            rc = FSA_E_NOMEM;
+            err = e.what();
+        }
         step_s = srv->now() - ts;
     });
     std::vector<uint32_t> fetched(finished.size(), FSA_OK);
@@ -2662,12 +2667,23 @@ void overlapped_tick(fsa_server srv, const std::vector<Job*>& stepping, const st
     }
     worker.join();
     srv->step_s += step_s;
-    // the step's state reads copy a range of slots and may have covered an
-    // admitted scene's slot while it was written: read those states again
-    for (uint64_t id : admitted) read_state(srv->jobs[id]->scene);
+    if (st.size() != stepping.size()) {  // the batch never started
+        st.assign(stepping.size(), fsa_step_stats{});
+        for (auto& e : st) e.status = kNotStepped;
+    }
     book_step(srv, stepping, st, rc, false);
     for (size_t q = 0; q < finished.size(); q++)
         retire(srv, finished[q], fetched[q] == FSA_OK ? FSA_JOB_DONE : FSA_JOB_FAILED, fetched[q], true);
+    // the step's state reads copy a range of slots and may have covered an
+    // admitted scene's slot while it was written: read those states again
+    for (uint64_t id : admitted) {
+        Job* j = srv->jobs[id];
+        try {
+            read_state(j->scene);
+        } catch (const FsaError& e) {
+            retire(srv, j, FSA_JOB_FAILED, (uint32_t)e.code);
+        }
+    }
     if (failure) std::rethrow_exception(failure);
     if (rc != FSA_OK) g_error = err;
 }

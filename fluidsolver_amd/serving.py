@@ -51,9 +51,11 @@ class Server:
 
     def cancel(self, job_id):
         """Cancels a queued or running job; its state stays readable through job()
-        until release()."""
+        until release().  A running job whose steps have all run is retired DONE
+        instead, and its result stays collectable."""
         check(lib().fsa_server_cancel(self._h, int(job_id)))
-        self._collections.pop(job_id, None)
+        if self.job(job_id).state != abi.JOB_DONE:
+            self._collections.pop(job_id, None)
 
     def release(self, job_id):
         """Forgets a job in a terminal state without copying its final state (a failed

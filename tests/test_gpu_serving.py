@@ -79,6 +79,31 @@ def test_server_mix_equals_alone(gpu_context, oracle_lib, serial, graph):
         gpu_context.set_graph(False)
 
 
+def test_cancel_between_last_step_and_retirement(gpu_context):
+    """With overlapped ticks a job is retired in the tick after its last step.
+    Cancelling it in between retires it DONE, with the result of running it
+    alone; the job beside it is unaffected."""
+    jobs = _jobs()
+    d, c, steps = jobs[5]  # 2 steps
+    d1, c1, s1 = jobs[1]  # 6 steps: the second tick steps a running job, so it overlaps
+    srv = Server(gpu_context)
+    a = srv.submit(d1, c1.copy(), s1)
+    b = srv.submit(d, c.copy(), steps)
+    print("case cancel after the last step", flush=True)
+    for _ in range(steps):
+        assert srv.tick() == 2
+    info = srv.job(b)
+    assert info.state == abi.JOB_RUNNING and info.steps_done == steps
+    srv.cancel(b)
+    assert srv.job(b).state == abi.JOB_DONE
+    np.testing.assert_array_equal(srv.collect(b).movement, _alone(gpu_context, d, c, steps).movement)
+    while srv.stats()["jobs_running"] or srv.stats()["jobs_queued"]:
+        srv.tick()
+    np.testing.assert_array_equal(srv.collect(a).movement, _alone(gpu_context, d1, c1, s1).movement)
+    assert srv.stats()["jobs_done"] == 2
+    srv.close()
+
+
 def test_server_slot_limit_and_cancel(gpu_context):
     jobs = _jobs()
     srv = Server(gpu_context, max_running=2)
