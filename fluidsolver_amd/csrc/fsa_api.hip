@@ -422,11 +422,13 @@ void carve(fsa_scene_s* s, const Plan& p) {
 // ---- block maps --------------------------------------------------------------------
 struct MapBuilder {
     std::vector<BlockRef> all, fluid, tiles, scan, scene, cnt;
+    bool fluidless = false;  // some scene has no fluid block
     void add(uint32_t sidx, const SceneDev& h) {
         for (uint32_t b = 0; b < kCountBlocks; b++) cnt.push_back({sidx, b});
         for (uint32_t b = 0; b < h.nblocks; b++) all.push_back({sidx, b});
         const uint32_t nfb = (h.nf + kBlock - 1) / kBlock;
         for (uint32_t b = 0; b < nfb; b++) fluid.push_back({sidx, b});
+        fluidless = fluidless || nfb == 0;
         for (uint32_t b = 0; b < h.ntiles; b++) tiles.push_back({sidx, b});
         for (uint32_t d = 0; d < (uint32_t)kRadix; d++) scan.push_back({sidx, d});
         scene.push_back({sidx, 0});
@@ -491,6 +493,7 @@ void build_launch_set(const MapBuilder& mb, SceneDev* table, LaunchSet& L, std::
     L.map_scene = upload_map(mb.scene, owned, st);
     L.map_cnt = upload_map(mb.cnt, owned, st);
     L.nb_cnt = (uint32_t)mb.cnt.size();
+    L.fluidless_scene = mb.fluidless;
     L.nb_all = (uint32_t)mb.all.size();
     L.nb_fluid = (uint32_t)mb.fluid.size();
     L.nb_tiles = (uint32_t)mb.tiles.size();
@@ -735,8 +738,9 @@ void simulate_set(fsa_context ctx, const LaunchSet& L, std::vector<fsa_scene_s*>
             else
                 launch_iisph_prepare(L, boundary_pass, st);
         }
-        // 3D: the loop state is set before phase 2, which performs the first
-        // Jacobi iteration's pressure update on the way
+        // 3D: the loop state is set before phase 2 (by phase 1, and for scenes
+        // without fluid here), which performs the first Jacobi iteration's
+        // pressure update on the way
         if (!d2) launch_iisph_begin_loop(L, st);
         {
             PhaseTimer t(ctx, FSA_TIMER_DIAGONAL);
@@ -1404,8 +1408,9 @@ void step_head(fsa_context ctx, const LaunchSet& L, bool d2, bool unsorted, uint
         else
             launch_iisph_prepare(L, bpass, st);
     }
-    // 3D: the loop state is set before phase 2, which performs the first
-    // Jacobi iteration's pressure update on the way
+    // 3D: the loop state is set before phase 2 (by phase 1, and for scenes
+    // without fluid here), which performs the first Jacobi iteration's pressure
+    // update on the way
     if (!d2) launch_iisph_begin_loop(L, st);
     {
         PhaseTimer t(ctx, FSA_TIMER_DIAGONAL);

@@ -35,6 +35,7 @@ struct LaunchSet {
     bool remover = false;          // some scene has a particle remover
     bool counting = false;         // compressed search: the counting sort over rank keys (every scene has them)
     bool skip_first_check = false; // 3D IISPH: every scene iterates at least twice (no check after the first iteration)
+    bool fluidless_scene = false;  // some scene has no fluid particle (no phase-1 block sets its loop state)
 };
 
 // kernel families the autotuner times
@@ -134,6 +135,7 @@ void launch_qs_apply(const QsBuffers& B, const uint32_t* ref, uint32_t rows, con
 // IISPH / SESPH
 void launch_iisph_prepare(const LaunchSet& L, bool boundary_pass, hipStream_t st);
 void launch_iisph_diagonal(const LaunchSet& L, hipStream_t st);
+// the Jacobi loop state of scenes without fluid (phase 1 sets the others')
 void launch_iisph_begin_loop(const LaunchSet& L, hipStream_t st);
 void launch_iisph_pass_a(const LaunchSet& L, hipStream_t st);
 void launch_iisph_first_converge(const LaunchSet& L, hipStream_t st);

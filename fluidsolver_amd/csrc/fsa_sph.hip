@@ -145,12 +145,32 @@ struct PosP {
     float p;
 };
 
+// The Jacobi loop's state at the start of a step's loop (3D IISPH).  first_done:
+// the first iteration's convergence check is not launched (every scene of the
+// launch has min and max iterations of at least 2, so it could only continue
+// the loop, and pass B's check overwrites what it records): the loop starts
+// with that iteration counted.
+__device__ __forceinline__ void begin_loop_state(const SceneDev& S, uint32_t first_done) {
+    SceneState* st = S.state;
+    st->active = S.max_it > 0 ? 1u : 0u;
+    st->pending = 1u;
+    st->iterations = first_done;
+    st->max_a2 = 0.0f;
+    st->max_v2 = 0.0f;
+    st->avg_err = 0.0;
+    st->err_count = 0;
+}
+
+// loop_start: 0, or 1 + first_done, and the first thread of the scene's block 0
+// sets the loop state (no kernel between phase 1 and phase 2 reads it; a scene
+// without fluid has no block here and gets it from k_iisph_begin_loop)
 template <uint32_t Spec, int U>
 __device__ __forceinline__ void iisph_prepare_body(const SceneDev* __restrict__ scenes,
-                                                          const BlockRef* __restrict__ map) {
+                                                          const BlockRef* __restrict__ map, uint32_t loop_start) {
     const BlockRef br = map[blockIdx.x];
     const SceneDev& S = scenes[br.scene];
     if (scene_failed(S)) return;
+    if (loop_start && br.block == 0 && threadIdx.x == 0) begin_loop_state(S, loop_start - 1);
     const uint32_t k = br.block * kBlock + threadIdx.x;
     if (k >= S.nf) return;
     const uint32_t c = (uint32_t)__builtin_amdgcn_readfirstlane((int)S.state->cur);  // uniform: the arrays' bases stay scalar
@@ -216,13 +236,13 @@ __device__ __forceinline__ void iisph_prepare_body(const SceneDev* __restrict__ 
 // SIMD): variants the autotuner times
 template <uint32_t Spec, int U>
 __global__ __launch_bounds__(kBlock) void k_iisph_prepare(const SceneDev* __restrict__ scenes,
-                                                  const BlockRef* __restrict__ map) {
-    iisph_prepare_body<Spec, U>(scenes, map);
+                                                  const BlockRef* __restrict__ map, uint32_t loop_start) {
+    iisph_prepare_body<Spec, U>(scenes, map, loop_start);
 }
 template <uint32_t Spec, int U, int W>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(W))) void k_iisph_prepare_w(
-    const SceneDev* __restrict__ scenes, const BlockRef* __restrict__ map) {
-    iisph_prepare_body<Spec, U>(scenes, map);
+    const SceneDev* __restrict__ scenes, const BlockRef* __restrict__ map, uint32_t loop_start) {
+    iisph_prepare_body<Spec, U>(scenes, map, loop_start);
 }
 
 // phase 1 for boundary particles: no forces of their own, density untouched
@@ -358,24 +378,13 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(W))) voi
 }
 
 // ---- Jacobi loop control -------------------------------------------------------------
-// first_done: the first iteration's convergence check is not launched (every
-// scene of the launch has min and max iterations of at least 2, so it could
-// only continue the loop, and pass B's check overwrites what it records): the
-// loop starts with that iteration counted.
+// the loop state of every scene (launched when some scene of the set has no
+// fluid: phase 1 sets it for the others)
 __global__ void k_iisph_begin_loop(const SceneDev* __restrict__ scenes, const BlockRef* __restrict__ map,
                                    uint32_t first_done) {
     const SceneDev& S = scenes[map[blockIdx.x].scene];
     if (scene_failed(S)) return;
-    if (threadIdx.x == 0) {
-        SceneState* st = S.state;
-        st->active = S.max_it > 0 ? 1u : 0u;
-        st->pending = 1u;
-        st->iterations = first_done;
-        st->max_a2 = 0.0f;
-        st->max_v2 = 0.0f;
-        st->avg_err = 0.0;
-        st->err_count = 0;
-    }
+    if (threadIdx.x == 0) begin_loop_state(S, first_done);
 }
 
 // ---- Jacobi pass A: pressure acceleration ------------------------------------------
@@ -853,58 +862,61 @@ __global__ __launch_bounds__(kFinalizeThreads) void k_integrate_finalize(const S
     } while (0)
 
 // spec x neighbor-walk unroll (the unroll chosen by the autotuner per kernel family)
-#define FSA_SPEC_U(kernel, U, grid, map)                                                            \
+#define FSA_SPEC_U(kernel, U, grid, ...)                                                            \
     do {                                                                                            \
         switch (L.spec) {                                                                           \
-            case 15: kernel<15, U><<<grid, kBlock, 0, st>>>(L.scenes, map); break;                  \
-            case 13: kernel<13, U><<<grid, kBlock, 0, st>>>(L.scenes, map); break;                  \
-            case 11: kernel<11, U><<<grid, kBlock, 0, st>>>(L.scenes, map); break;                  \
-            case 9: kernel<9, U><<<grid, kBlock, 0, st>>>(L.scenes, map); break;                    \
-            case 8: kernel<8, U><<<grid, kBlock, 0, st>>>(L.scenes, map); break;                    \
-            case 7: kernel<7, U><<<grid, kBlock, 0, st>>>(L.scenes, map); break;                    \
-            case 5: kernel<5, U><<<grid, kBlock, 0, st>>>(L.scenes, map); break;                    \
-            case 3: kernel<3, U><<<grid, kBlock, 0, st>>>(L.scenes, map); break;                    \
-            case 1: kernel<1, U><<<grid, kBlock, 0, st>>>(L.scenes, map); break;                    \
-            default: kernel<0, U><<<grid, kBlock, 0, st>>>(L.scenes, map); break;                   \
+            case 15: kernel<15, U><<<grid, kBlock, 0, st>>>(L.scenes, __VA_ARGS__); break;                  \
+            case 13: kernel<13, U><<<grid, kBlock, 0, st>>>(L.scenes, __VA_ARGS__); break;                  \
+            case 11: kernel<11, U><<<grid, kBlock, 0, st>>>(L.scenes, __VA_ARGS__); break;                  \
+            case 9: kernel<9, U><<<grid, kBlock, 0, st>>>(L.scenes, __VA_ARGS__); break;                    \
+            case 8: kernel<8, U><<<grid, kBlock, 0, st>>>(L.scenes, __VA_ARGS__); break;                    \
+            case 7: kernel<7, U><<<grid, kBlock, 0, st>>>(L.scenes, __VA_ARGS__); break;                    \
+            case 5: kernel<5, U><<<grid, kBlock, 0, st>>>(L.scenes, __VA_ARGS__); break;                    \
+            case 3: kernel<3, U><<<grid, kBlock, 0, st>>>(L.scenes, __VA_ARGS__); break;                    \
+            case 1: kernel<1, U><<<grid, kBlock, 0, st>>>(L.scenes, __VA_ARGS__); break;                    \
+            default: kernel<0, U><<<grid, kBlock, 0, st>>>(L.scenes, __VA_ARGS__); break;                   \
         }                                                                                           \
     } while (0)
 // spec x unroll x occupancy hint (the variant chosen by the autotuner per kernel family:
 // the unroll in the low nibble, the waves-per-SIMD hint in the high nibble)
-#define FSA_SPEC_UW(kernel, U, W, grid, map)                                                        \
+#define FSA_SPEC_UW(kernel, U, W, grid, ...)                                                        \
     do {                                                                                            \
         switch (L.spec) {                                                                           \
-            case 15: kernel##_w<15, U, W><<<grid, kBlock, 0, st>>>(L.scenes, map); break;           \
-            case 13: kernel##_w<13, U, W><<<grid, kBlock, 0, st>>>(L.scenes, map); break;           \
-            case 11: kernel##_w<11, U, W><<<grid, kBlock, 0, st>>>(L.scenes, map); break;           \
-            case 9: kernel##_w<9, U, W><<<grid, kBlock, 0, st>>>(L.scenes, map); break;             \
-            case 8: kernel##_w<8, U, W><<<grid, kBlock, 0, st>>>(L.scenes, map); break;             \
-            case 7: kernel##_w<7, U, W><<<grid, kBlock, 0, st>>>(L.scenes, map); break;             \
-            case 5: kernel##_w<5, U, W><<<grid, kBlock, 0, st>>>(L.scenes, map); break;             \
-            case 3: kernel##_w<3, U, W><<<grid, kBlock, 0, st>>>(L.scenes, map); break;             \
-            case 1: kernel##_w<1, U, W><<<grid, kBlock, 0, st>>>(L.scenes, map); break;             \
-            default: kernel##_w<0, U, W><<<grid, kBlock, 0, st>>>(L.scenes, map); break;            \
+            case 15: kernel##_w<15, U, W><<<grid, kBlock, 0, st>>>(L.scenes, __VA_ARGS__); break;           \
+            case 13: kernel##_w<13, U, W><<<grid, kBlock, 0, st>>>(L.scenes, __VA_ARGS__); break;           \
+            case 11: kernel##_w<11, U, W><<<grid, kBlock, 0, st>>>(L.scenes, __VA_ARGS__); break;           \
+            case 9: kernel##_w<9, U, W><<<grid, kBlock, 0, st>>>(L.scenes, __VA_ARGS__); break;             \
+            case 8: kernel##_w<8, U, W><<<grid, kBlock, 0, st>>>(L.scenes, __VA_ARGS__); break;             \
+            case 7: kernel##_w<7, U, W><<<grid, kBlock, 0, st>>>(L.scenes, __VA_ARGS__); break;             \
+            case 5: kernel##_w<5, U, W><<<grid, kBlock, 0, st>>>(L.scenes, __VA_ARGS__); break;             \
+            case 3: kernel##_w<3, U, W><<<grid, kBlock, 0, st>>>(L.scenes, __VA_ARGS__); break;             \
+            case 1: kernel##_w<1, U, W><<<grid, kBlock, 0, st>>>(L.scenes, __VA_ARGS__); break;             \
+            default: kernel##_w<0, U, W><<<grid, kBlock, 0, st>>>(L.scenes, __VA_ARGS__); break;            \
         }                                                                                           \
     } while (0)
-#define FSA_UNROLL_DISPATCH(kernel, family, grid, map)                                              \
+#define FSA_UNROLL_DISPATCH(kernel, family, grid, ...)                                              \
     do {                                                                                            \
         switch (L.unroll[family]) {                                                                 \
-            case kVariantU2: FSA_SPEC_U(kernel, 2, grid, map); break;                               \
-            case kVariantU8: FSA_SPEC_U(kernel, 8, grid, map); break;                               \
-            case kVariantU4W8: FSA_SPEC_UW(kernel, 4, 8, grid, map); break;                         \
-            case kVariantU8W6: FSA_SPEC_UW(kernel, 8, 6, grid, map); break;                         \
-            default: FSA_SPEC_U(kernel, 4, grid, map); break;                                       \
+            case kVariantU2: FSA_SPEC_U(kernel, 2, grid, __VA_ARGS__); break;                               \
+            case kVariantU8: FSA_SPEC_U(kernel, 8, grid, __VA_ARGS__); break;                               \
+            case kVariantU4W8: FSA_SPEC_UW(kernel, 4, 8, grid, __VA_ARGS__); break;                         \
+            case kVariantU8W6: FSA_SPEC_UW(kernel, 8, 6, grid, __VA_ARGS__); break;                         \
+            default: FSA_SPEC_U(kernel, 4, grid, __VA_ARGS__); break;                                       \
         }                                                                                           \
     } while (0)
 
+// phase 1 also sets the Jacobi loop state of every scene with fluid
+uint32_t loop_start(const LaunchSet& L) { return L.skip_first_check ? 2u : 1u; }
 void launch_iisph_prepare(const LaunchSet& L, bool boundary_pass, hipStream_t st) {
-    if (L.nb_fluid) FSA_UNROLL_DISPATCH(k_iisph_prepare, kTunePrepare, L.nb_fluid, L.map_fluid);
+    if (L.nb_fluid) FSA_UNROLL_DISPATCH(k_iisph_prepare, kTunePrepare, L.nb_fluid, L.map_fluid, loop_start(L));
     if (boundary_pass && L.nb_all) k_iisph_prepare_boundary<<<L.nb_all, kBlock, 0, st>>>(L.scenes, L.map_all);
 }
 void launch_iisph_diagonal(const LaunchSet& L, hipStream_t st) {
     if (L.nb_fluid) FSA_UNROLL_DISPATCH(k_iisph_diagonal, kTuneDiagonal, L.nb_fluid, L.map_fluid);
 }
 void launch_iisph_begin_loop(const LaunchSet& L, hipStream_t st) {
-    k_iisph_begin_loop<<<L.nb_scene, 64, 0, st>>>(L.scenes, L.map_scene, L.skip_first_check ? 1u : 0u);
+    if (L.fluidless_scene)
+        k_iisph_begin_loop<<<L.nb_scene, 64, 0, st>>>(L.scenes, L.map_scene, L.skip_first_check ? 1u : 0u);
 }
 void launch_iisph_pass_a(const LaunchSet& L, hipStream_t st) {
     if (L.nb_fluid) FSA_UNROLL_DISPATCH(k_iisph_pass_a, kTunePassA, L.nb_fluid, L.map_fluid);
@@ -920,7 +932,9 @@ void launch_iisph_pass_b(const LaunchSet& L, hipStream_t st) {
 void launch_tuned_kernel(const LaunchSet& L, int family, hipStream_t st) {
     if (!L.nb_fluid) return;
     switch (family) {
-        case kTunePrepare: FSA_UNROLL_DISPATCH(k_iisph_prepare, kTunePrepare, L.nb_fluid, L.map_fluid); break;
+        case kTunePrepare:
+            FSA_UNROLL_DISPATCH(k_iisph_prepare, kTunePrepare, L.nb_fluid, L.map_fluid, loop_start(L));
+            break;
         case kTuneDiagonal: FSA_UNROLL_DISPATCH(k_iisph_diagonal, kTuneDiagonal, L.nb_fluid, L.map_fluid); break;
         case kTunePassA: FSA_UNROLL_DISPATCH(k_iisph_pass_a, kTunePassA, L.nb_fluid, L.map_fluid); break;
         default: FSA_UNROLL_DISPATCH(k_iisph_pass_b, kTunePassB, L.nb_fluid, L.map_fluid); break;

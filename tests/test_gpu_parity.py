@@ -9,6 +9,7 @@ import numpy as np
 import pytest
 
 from fluidsolver_amd import abi, scenes
+from fluidsolver_amd.collection import ParticleCollection
 from fluidsolver_amd.solver import Scene
 
 pytestmark = pytest.mark.gpu
@@ -241,6 +242,51 @@ def test_batch_equals_alone(gpu_context):
         g = sc.download(colls[q].copy())
         np.testing.assert_array_equal(g.movement, alone[q][1].movement)
         np.testing.assert_array_equal(g.particle_data, alone[q][1].particle_data)
+
+
+def test_scene_without_fluid_alone_and_in_batches(gpu_context, oracle_lib):
+    """A 3D IISPH scene with no fluid particle has no phase-1 block, so its
+    Jacobi loop state comes from the separate loop-start launch; the scenes with
+    fluid get theirs from phase 1.  Stepped alone it equals the oracle; in a
+    batch with two dam scenes every scene equals running it alone, in sets that
+    skip the first iteration's check (min iterations 2) and sets that keep it."""
+    O = oracle_lib
+    dam = scenes.dam_scene((6, 7, 6), (18, 12, 12), 3, seed=41)
+    walls = ParticleCollection.from_arrays(3, dam.position[dam.type == abi.TYPE_BOUNDARY].copy(),
+                                           np.full(int(np.count_nonzero(dam.type == abi.TYPE_BOUNDARY)),
+                                                   abi.TYPE_BOUNDARY, np.uint8))
+    other = scenes.dam_scene((5, 5, 5), (15, 10, 10), 3, seed=42)
+    for min_it in (1, 2):
+        d = abi.default_desc(3, abi.SOLVER_IISPH)
+        d.iisph_min_iterations = min_it
+        print(f"case fluid-free scene, min iterations {min_it}", flush=True)
+        sc = Scene(gpu_context, d, walls.copy())
+        orc = O.OracleScene(d, walls.copy(), mode=O.MODE_FP32_R64)
+        for step in range(3):
+            st = sc.step(1)
+            ost = orc.step(1)
+            assert st.status == abi.FSA_OK
+            assert st.iterations == ost.iterations, (min_it, step)
+            assert st.average_density_error == ost.average_density_error or (
+                np.isnan(st.average_density_error) and np.isnan(ost.average_density_error)), (min_it, step)
+        sc.close()
+        colls = [dam, walls, other]
+        alone = []
+        for c in colls:
+            a = Scene(gpu_context, d, c.copy())
+            alone.append(([a.step(1).as_dict() for _ in range(3)], a.download(c.copy())))
+            a.close()
+        batch = [Scene(gpu_context, d, c.copy()) for c in colls]
+        bstats = [[] for _ in batch]
+        for _ in range(3):
+            for q, st in enumerate(gpu_context.batch_step(batch, 1)):
+                bstats[q].append(st.as_dict())
+        for q, (b, c) in enumerate(zip(batch, colls)):
+            assert str(bstats[q]) == str(alone[q][0]), (min_it, q)
+            g = b.download(c.copy())
+            np.testing.assert_array_equal(g.movement, alone[q][1].movement)
+            np.testing.assert_array_equal(g.particle_data, alone[q][1].particle_data)
+            b.close()
 
 
 def test_cpp_dropin_program(oracle_lib, tmp_path):
